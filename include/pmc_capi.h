@@ -1,0 +1,207 @@
+/* pmc_capi.h -- C ABI of the MI355X deviational Monte Carlo step.
+ *
+ * One context owns every device buffer of one simulation instance on the
+ * current HIP device: structure-of-arrays carriers, mode/rate tables, geometry,
+ * boundary tables, cell tallies and ledgers.  The host hands over immutable
+ * tables once; pmc_step() then advances whole time steps on the device.
+ *
+ * Each entry point replaces a reference (lyushisyan/PhonoMC) interface, cited
+ * as file:line.  All functions return PMC_OK (0) or an error code; after a
+ * failed step the context is latched and every later call returns
+ * PMC_ERR_LATCHED, like the reference's "Cannot resume a failed timestep"
+ * guard (src/MonteCarloSolver.cpp:349-354).  Not reentrant: one host thread
+ * per context. */
+#ifndef PMC_CAPI_H
+#define PMC_CAPI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pmc_status {
+    PMC_OK = 0,
+    PMC_ERR_ARGUMENT = 1,       /* invalid argument or table shape */
+    PMC_ERR_DEVICE = 2,         /* HIP runtime failure */
+    PMC_ERR_EXCESS_EVENTS = 3,  /* a carrier needed more than 64 events in one step */
+    PMC_ERR_BOUNDARY = 4,       /* boundary intersection / outgoing mode failed */
+    PMC_ERR_ESCAPED = 5,        /* carrier found outside the domain */
+    PMC_ERR_INVALID_STATE = 6,  /* non-finite occupation or energy */
+    PMC_ERR_CAPACITY = 7,       /* carrier slots exhausted */
+    PMC_ERR_LATCHED = 8,        /* an earlier step failed; create a new context */
+    PMC_ERR_UNSUPPORTED = 9     /* feature outside this build */
+};
+
+typedef struct pmc_ctx pmc_ctx;
+
+/* Step controls; replaces the constructor arguments MonteCarloSolver reads from
+ * SimulationConfig (src/MonteCarloSolver.cpp:50-207, include/SimulationConfig.h:42-146). */
+typedef struct pmc_setup {
+    int64_t capacity;              /* carrier slots to allocate (>= initial count) */
+    double time_step_ps;
+    double push_distance_a;        /* 1e-10*max(dt,1) in the reference (src/MonteCarloSolver.cpp:74) */
+    uint64_t seed;                 /* Philox key (with instance) */
+    uint32_t instance;
+    double background_temperature; /* deviational reference T_bg */
+    int32_t lifetime_local;        /* 1: rates at cell temperature; 0: at lifetime_temperature */
+    double lifetime_temperature;
+    int32_t refill_policy;         /* 0 one-to-one (previous absorbed count), 1 fixed incoming flux */
+    int32_t mode_sampling;         /* 0 alias table, 1 inverse CDF (reference index order) */
+    int32_t temperature_inversion; /* 0 lookup table (reference), 1 Newton on the Bose-Einstein sum */
+    int32_t reserved[8];
+} pmc_setup;
+
+/* Material and background tables by active-mode index; replaces PhononMaterial
+ * queries (include/PhononMaterial.h:14-125) and BackgroundCache
+ * (include/solver/BackgroundCache.h:11-55). */
+typedef struct pmc_mode_table {
+    int32_t n_active;
+    int32_t n_flat;                /* qpoints*branches */
+    const int32_t* active_flat;    /* [n_active] flat mode index */
+    const double* omega;           /* [n_active] rad/ps */
+    const double* velocity;        /* [n_active*3] Angstrom/ps */
+    const double* neq_background;  /* [n_active] Bose occupation at T_bg */
+    int32_t n_temps;
+    const double* temps;           /* [n_temps] K */
+    const double* gamma_total;     /* [n_temps*n_flat] THz linewidth (RTA total) */
+    double energy_weight;          /* M_active*V_particle/max(1,Nq*Vuc) */
+    double background_energy_density;
+    double zero_point_density;
+    int32_t lut_n;
+    const double* lut_t;           /* [lut_n] K */
+    const double* lut_e;           /* [lut_n] eV/A^3, strictly increasing */
+} pmc_mode_table;
+
+/* One surface facet; replaces SimulationDomain facet queries
+ * (include/SimulationDomain.h:58-66). */
+typedef struct pmc_facet {
+    double normal[3];              /* outward unit normal */
+    double shift[3];               /* periodic translation */
+    int32_t condition;             /* 'T', 'P' or 'R' */
+    int32_t rough_table;           /* index or -1 */
+    int32_t reservoir;             /* index or -1 */
+    int32_t periodic_pair;         /* facet or -1 */
+} pmc_facet;
+
+/* Geometry and cell grid; replaces SimulationDomain::trace_boundary_intersection
+ * (src/SimulationDomain.cpp:342-450) and fast_grid_index (:92-126). */
+typedef struct pmc_geometry {
+    int32_t kind;                  /* 0: analytic axis-aligned box */
+    double bmin[3], bmax[3];
+    int32_t n_facets;
+    const pmc_facet* facets;
+    int32_t box_plane_facet[6];    /* xmin,xmax,ymin,ymax,zmin,zmax -> facet */
+    int32_t grid[3];
+    double inv_cell[3];
+    const int32_t* voxel_to_cell;  /* [grid product] */
+    int32_t n_cells;
+    const double* cell_volume;     /* [n_cells] A^3 */
+} pmc_geometry;
+
+/* Rough-facet reflection table; replaces RoughBoundarySampler
+ * (src/solver/RoughBoundarySampler.cpp:77-533). */
+typedef struct pmc_rough_table {
+    const double* specularity;     /* [n_active] */
+    const int32_t* spec_match;     /* [n_active] */
+    const int32_t* diffuse_begin;  /* [n_active] */
+    const int32_t* diffuse_end;    /* [n_active] */
+    int32_t n_outgoing;
+    const int32_t* outgoing_active;/* [n_outgoing] active order */
+    const int32_t* outgoing_sorted;/* [n_outgoing] frequency order */
+    const double* residual_prefix; /* [n_outgoing+1] */
+    int32_t n_roulette;
+    const int32_t* roulette_active;
+    const double* roulette_cdf;
+    int32_t strict_elastic;
+} pmc_rough_table;
+
+/* Thermal reservoir on one facet; replaces ReservoirSampler/ReservoirInjector
+ * (src/solver/ReservoirSampler.cpp:17-146, src/solver/ReservoirInjector.cpp:62-132). */
+typedef struct pmc_reservoir {
+    int32_t facet;
+    double temperature;
+    double normal[3];
+    double expected_count;
+    int32_t initial_count;
+    const double* cdf;             /* [n_active] inward-flux CDF */
+    const double* alias_prob;      /* [n_active] */
+    const int32_t* alias_index;    /* [n_active] */
+    const double* occupation;      /* [n_active] Bose occupation at the reservoir T */
+    int32_t n_faces;
+    const double* face_cdf;        /* [n_faces] */
+    const double* triangles;       /* [n_faces*9] */
+} pmc_reservoir;
+
+/* Per-step and cumulative ledgers; replaces BoundaryLedger/EnergyLedger
+ * (src/solver/BoundaryLedger.cpp:30-141, src/solver/EnergyLedger.cpp:58-90). */
+typedef struct pmc_ledger {
+    int64_t step;
+    int64_t live_carriers;
+    int64_t slots;
+    int64_t step_absorbed, step_injected;
+    double step_absorbed_energy_ev, step_injected_energy_ev;
+    double step_lifetime_residual_ev, step_balance_residual_ev;
+    int64_t total_absorbed, total_injected;
+    double total_absorbed_energy_ev, total_injected_energy_ev;
+    double total_lifetime_residual_ev, total_balance_residual_ev;
+    double thermal_energy_ev;
+    int64_t carrier_steps;         /* sum of live carriers at step start */
+    int64_t rough_events, rough_specular, rough_diffuse;
+    int64_t newton_passes;         /* RTA pseudo-temperature passes run */
+} pmc_ledger;
+
+typedef struct pmc_error {
+    int32_t code;
+    int64_t carrier;               /* first offending slot or -1 */
+    float position[3];
+    int32_t facet;
+    char message[256];
+} pmc_error;
+
+/* Initial population (src/solver/Initialization.cpp:32-275). */
+typedef struct pmc_init {
+    int64_t count;
+    int32_t temperature_mode;      /* 0 uniform, 1 linear cold->hot, 2 per-cell field */
+    double uniform_temperature;
+    double cold_point[3], hot_point[3];
+    double cold_temperature, hot_temperature;
+    const double* cell_temperature;/* [n_cells] for mode 2 (warm start) */
+} pmc_init;
+
+int pmc_create(const pmc_setup* setup, pmc_ctx** out);
+int pmc_upload_material(pmc_ctx* ctx, const pmc_mode_table* table);
+int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* geometry);
+int pmc_upload_boundary_tables(pmc_ctx* ctx, int32_t n_rough, const pmc_rough_table* rough,
+                               int32_t n_reservoirs, const pmc_reservoir* reservoirs);
+/* Samples the initial population on the device (Initialization.cpp:48-113) and
+ * performs the initial temperature refresh (Observables.cpp:12-21). */
+int pmc_initialize_carriers(pmc_ctx* ctx, const pmc_init* init);
+/* Uploads an explicit population instead (positions FP32, active-mode index, occupation). */
+int pmc_seed_carriers(pmc_ctx* ctx, int64_t n, const float* position, const int32_t* mode,
+                      const double* occupation);
+/* Runs nsteps calls of the time step (MonteCarloSolver::run_timestep,
+ * src/MonteCarloSolver.cpp:348-398) without host synchronisation between steps. */
+int pmc_step(pmc_ctx* ctx, int32_t nsteps);
+/* Cell temperatures and energy densities of the latest refresh. */
+int pmc_read_cells(pmc_ctx* ctx, double* temperature, double* energy_density);
+/* Heat-flux tally (src/solver/GridStatistics.cpp:269-347), W/m^2, [n_cells*3]. */
+int pmc_read_heat_flux(pmc_ctx* ctx, double* flux);
+int pmc_read_ledger(pmc_ctx* ctx, pmc_ledger* ledger);
+/* Carrier state for tests and checkpoints: slots in [0, *n) (dead slots have mode -1). */
+int pmc_read_carriers(pmc_ctx* ctx, int64_t* n, float* position, int32_t* mode, double* occupation,
+                      int32_t* cell);
+/* Philox stream ids of slots [0, n) (initial carriers 0..N-1, injected ones numbered on). */
+int pmc_read_uids(pmc_ctx* ctx, int64_t n, uint32_t* uid);
+int pmc_last_error(pmc_ctx* ctx, pmc_error* error);
+/* Kernel durations of the last pmc_step call, measured with HIP events on the
+ * context's stream: [0] transport, [1] injection, [2] RTA passes, [3] relax, [4] whole steps (ms). */
+int pmc_kernel_times(pmc_ctx* ctx, double* ms, int32_t n);
+/* Enables per-kernel event timing for later pmc_step calls (off by default). */
+int pmc_set_timing(pmc_ctx* ctx, int32_t enabled);
+void pmc_destroy(pmc_ctx* ctx);
+const char* pmc_status_string(int32_t code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,859 @@
+// CPU restatement of the reference's deviational Monte Carlo step (see oracle.h).
+// Checker only: FP64 throughout, positions kept in FP64, the collision cache and
+// push/nudge placement of TransportOperator kept as the reference has them.
+#include "oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+constexpr double kHbar = 6.582119569e-4;
+constexpr double kBoltz = 8.617333262145e-5;
+constexpr double kPi = 3.14159265358979323846;
+using V3 = std::array<double, 3>;
+
+V3 add(const V3& a, const V3& b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+V3 sub(const V3& a, const V3& b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+V3 mul(const V3& a, double s) { return {a[0] * s, a[1] * s, a[2] * s}; }
+double dot(const V3& a, const V3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+double norm(const V3& a) { return std::sqrt(dot(a, a)); }
+
+// src/material/Thermodynamics.cpp:71-82.
+double bose(double t, double w) {
+    if (t <= 0.0 || w <= 0.0) return 0.0;
+    const double x = kHbar * w / (kBoltz * t);
+    if (x > 700.0) return 0.0;
+    return 1.0 / std::max(std::exp(x) - 1.0, 1e-12);
+}
+
+double lerp(double x0, double x1, double y0, double y1, double x) {
+    if (std::abs(x1 - x0) <= 1e-18) return y0;
+    const double t = (x - x0) / (x1 - x0);
+    return y0 * (1.0 - t) + y1 * t;
+}
+
+// src/material/Thermodynamics.cpp:41-69 (clamped, upper_bound).
+double interp(const double* xs, const double* ys, int n, double x) {
+    if (n <= 0) return 0.0;
+    if (n == 1) return ys[0];
+    if (x <= xs[0]) return ys[0];
+    if (x >= xs[n - 1]) return ys[n - 1];
+    const int hi = static_cast<int>(std::upper_bound(xs, xs + n, x) - xs);
+    return lerp(xs[hi - 1], xs[hi], ys[hi - 1], ys[hi], x);
+}
+
+// src/material/Thermodynamics.cpp:95-133.
+double relaxation_weight(const pmc_mode_table* m, double t, int active, double dt) {
+    const int flat = m->active_flat[active];
+    const size_t nm = static_cast<size_t>(m->n_flat);
+    const int nt = m->n_temps;
+    double g;
+    if (t <= m->temps[0]) g = m->gamma_total[flat];
+    else if (t >= m->temps[nt - 1]) g = m->gamma_total[(nt - 1) * nm + flat];
+    else {
+        const int hi = static_cast<int>(std::upper_bound(m->temps, m->temps + nt, t) - m->temps);
+        g = lerp(m->temps[hi - 1], m->temps[hi], m->gamma_total[(hi - 1) * nm + flat], m->gamma_total[hi * nm + flat], t);
+    }
+    if (g == 0.0 || dt == 0.0) return 0.0;
+    return -std::expm1(-(g * dt) * (4.0 * kPi));
+}
+
+struct Uniforms {
+    uint32_t uid, step, ev, k0, k1;
+    int used = 0;
+    double u[2] = {0, 0};
+    double next() {
+        if ((used & 1) == 0) {
+            uint32_t w[4];
+            orc_philox(uid, step, 2 * ev + (used >> 1), 1u, k0, k1, w);
+            u[0] = orc_u53(w[0], w[1]);
+            u[1] = orc_u53(w[2], w[3]);
+        }
+        return u[used++ & 1];
+    }
+};
+} // namespace
+
+extern "C" {
+
+void orc_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t out[4]) {
+    // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): round multipliers and
+    // Weyl key increments of the published algorithm.
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c0;
+        const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c2;
+        const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+        const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+double orc_u53(uint32_t lo, uint32_t hi) {
+    const uint64_t x = ((static_cast<uint64_t>(hi) << 32) | lo) >> 11;
+    return static_cast<double>(x) * 0x1.0p-53;
+}
+
+int orc_trace_box(const pmc_geometry* g, const double pos[3], const double vel[3], double point[3], double* t) {
+    // SimulationDomain::trace_boundary_intersection, analytic box branch.
+    const double inf = std::numeric_limits<double>::infinity();
+    const double velocity_eps = 1e-18, time_eps = 1e-14;
+    double ext[3];
+    for (int a = 0; a < 3; ++a) ext[a] = g->bmax[a] - g->bmin[a];
+    const double scale = std::max({1.0, std::abs(ext[0]), std::abs(ext[1]), std::abs(ext[2])});
+    const double tol = std::max(1e-9, 64.0 * std::numeric_limits<double>::epsilon() * scale);
+    for (int a = 0; a < 3; ++a) {
+        point[a] = pos[a];
+        if (!std::isfinite(pos[a]) || pos[a] < g->bmin[a] - tol || pos[a] > g->bmax[a] + tol) {
+            *t = inf;
+            return -1;
+        }
+    }
+    double best = inf;
+    int plane = -1;
+    for (int a = 0; a < 3; ++a) {
+        const double c = vel[a];
+        if (!std::isfinite(c) || std::abs(c) <= velocity_eps) continue;
+        const int side = c > 0.0 ? 1 : 0;
+        const double target = side ? g->bmax[a] : g->bmin[a];
+        const double ht = (target - pos[a]) / c;
+        if (std::isfinite(ht) && ht > time_eps && ht < best) {
+            best = ht;
+            plane = 2 * a + side;
+        }
+    }
+    if (plane < 0) {
+        *t = inf;
+        return -1;
+    }
+    double col[3];
+    for (int a = 0; a < 3; ++a) col[a] = pos[a] + vel[a] * best;
+    const int ha = plane / 2;
+    col[ha] = (plane % 2 == 0) ? g->bmin[ha] : g->bmax[ha];
+    for (int a = 0; a < 3; ++a) {
+        if (!std::isfinite(col[a]) || col[a] < g->bmin[a] - tol || col[a] > g->bmax[a] + tol) {
+            *t = inf;
+            return -1;
+        }
+        col[a] = std::clamp(col[a], g->bmin[a], g->bmax[a]);
+    }
+    for (int a = 0; a < 3; ++a) point[a] = col[a];
+    *t = best;
+    return g->box_plane_facet[plane];
+}
+
+int orc_grid_index(const pmc_geometry* g, const double p[3]) {
+    int idx[3];
+    for (int a = 0; a < 3; ++a) {
+        const int n = g->grid[a];
+        if (n <= 1 || (g->bmax[a] - g->bmin[a]) <= 0.0) {
+            idx[a] = 0;
+            continue;
+        }
+        int k = static_cast<int>(std::floor((p[a] - g->bmin[a]) * g->inv_cell[a]));
+        idx[a] = std::clamp(k, 0, n - 1);
+    }
+    return g->voxel_to_cell[(idx[0] * g->grid[1] + idx[1]) * g->grid[2] + idx[2]];
+}
+
+int orc_rough_sample(const pmc_mode_table* m, const pmc_rough_table* r, int in_ai, double in_occ, const double* u,
+                     int* out_ai, double* out_occ, int* kind) {
+    int used = 0;
+    *out_ai = in_ai;
+    *out_occ = in_occ;
+    *kind = 0;
+    const int na = m->n_active;
+    const double p_spec = std::clamp(r->specularity[in_ai], 0.0, 1.0);
+    const double w_in = m->omega[in_ai], neq_in = m->neq_background[in_ai];
+    auto mapped_to = [&](int o) {
+        const double mapped = m->neq_background[o] + w_in * (in_occ - neq_in) / std::max(1e-30, m->omega[o]);
+        if (std::isfinite(mapped) && mapped >= 0.0) {
+            *out_ai = o;
+            *out_occ = mapped;
+        }
+    };
+    if (u[used++] <= p_spec) {
+        const int o = r->spec_match[in_ai];
+        if (o >= 0 && o < na) {
+            *kind = 1;
+            mapped_to(o);
+            return used;
+        }
+    }
+    // sample_diffuse (src/solver/RoughBoundarySampler.cpp:373-456).
+    int o = -1;
+    const int begin = r->diffuse_begin[in_ai], end = r->diffuse_end[in_ai];
+    bool chosen = false;
+    if (begin >= 0 && end > begin && end <= r->n_outgoing) {
+        const double uw = u[used++];
+        // flux_weighted_window_index (include/solver/BoundaryFluxSampling.h:64-88).
+        const double* prefix = r->residual_prefix;
+        const double fb = prefix[begin], fe = prefix[end], wf = fe - fb;
+        if (wf > 0.0 && std::isfinite(wf)) {
+            const double uc = std::clamp(uw, 0.0, std::nextafter(1.0, 0.0));
+            const double target = fb + uc * wf;
+            const double* it = std::upper_bound(prefix + begin + 1, prefix + end + 1, target);
+            const int pos = std::clamp(static_cast<int>(it - prefix) - 1, begin, end - 1);
+            o = r->outgoing_sorted[pos];
+            chosen = true;
+        }
+    }
+    if (!chosen) {
+        if (r->strict_elastic) return used;
+        if (r->n_roulette > 0 && r->roulette_cdf[r->n_roulette - 1] > 0.0) {
+            const double x = u[used++] * r->roulette_cdf[r->n_roulette - 1];
+            size_t pos = static_cast<size_t>(std::lower_bound(r->roulette_cdf, r->roulette_cdf + r->n_roulette, x) -
+                                             r->roulette_cdf);
+            if (pos >= static_cast<size_t>(r->n_roulette)) pos = r->n_roulette - 1;
+            o = r->roulette_active[pos];
+        } else if (r->n_outgoing > 0) {
+            const int k = std::min(static_cast<int>(u[used++] * r->n_outgoing), r->n_outgoing - 1);
+            o = r->outgoing_active[k];
+        } else {
+            o = std::min(static_cast<int>(u[used++] * na), na - 1);
+        }
+    }
+    *kind = 2;
+    mapped_to(o);
+    return used;
+}
+
+int orc_reservoir_mode(const pmc_reservoir* r, int na, double u, int sampling) {
+    if (sampling == 1) {
+        const double* it = std::lower_bound(r->cdf, r->cdf + na, u);
+        return it == r->cdf + na ? na - 1 : static_cast<int>(it - r->cdf);
+    }
+    const double x = u * na;
+    int j = std::min(static_cast<int>(x), na - 1);
+    return (x - j) < r->alias_prob[j] ? j : r->alias_index[j];
+}
+
+void orc_surface_point(const pmc_reservoir* r, double u_face, double r1, double r2, double point[3]) {
+    int f = static_cast<int>(std::lower_bound(r->face_cdf, r->face_cdf + r->n_faces, u_face) - r->face_cdf);
+    f = std::min(f, r->n_faces - 1);
+    const double* tri = r->triangles + 9 * f;
+    const double sr1 = std::sqrt(r1);
+    const double w0 = 1.0 - sr1, w1 = sr1 * (1.0 - r2), w2 = sr1 * r2;
+    for (int a = 0; a < 3; ++a) point[a] = w0 * tri[a] + w1 * tri[3 + a] + w2 * tri[6 + a];
+}
+
+void orc_refresh(const pmc_mode_table* m, int ncells, const double* volume, int64_t n, const int32_t* active,
+                 const int32_t* cell, const double* occ, double* e, double* t) {
+    for (int c = 0; c < ncells; ++c) e[c] = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int a = active[i];
+        const double energy = kHbar * m->omega[a] * (occ[i] - m->neq_background[a]);
+        e[cell[i]] += energy * m->energy_weight;
+    }
+    for (int c = 0; c < ncells; ++c) {
+        e[c] = e[c] / volume[c] + m->background_energy_density;
+        t[c] = interp(m->lut_e, m->lut_t, m->lut_n, e[c]);
+    }
+}
+
+double orc_rta(const pmc_mode_table* m, int ncells, const double* cell_T, int64_t n, const int32_t* active,
+               const int32_t* cell, double* occ, double dt, double weight_e, int local, double lifetime_T,
+               double* pseudo) {
+    // Cell-ordered index, as CellParticleIndex::ensure builds it (src/solver/CellParticleIndex.cpp:14-44).
+    std::vector<int64_t> offsets(ncells + 1, 0), order(n);
+    for (int64_t i = 0; i < n; ++i) offsets[cell[i] + 1]++;
+    for (int c = 0; c < ncells; ++c) offsets[c + 1] += offsets[c];
+    std::vector<int64_t> cursor(offsets.begin(), offsets.end() - 1);
+    for (int64_t i = 0; i < n; ++i) order[cursor[cell[i]]++] = i;
+    std::vector<double> a(n);
+    for (int64_t i = 0; i < n; ++i) a[i] = relaxation_weight(m, local ? cell_T[cell[i]] : lifetime_T, active[i], dt);
+    double residual = 0.0;
+    for (int c = 0; c < ncells; ++c) {
+        const int64_t b = offsets[c], e = offsets[c + 1];
+        if (pseudo) pseudo[c] = cell_T[c];
+        if (e <= b) continue;
+        auto evaluate = [&](double temperature, double* derivative) {
+            double value = 0.0, slope = 0.0;
+            const double T = std::max(0.0, temperature);
+            for (int64_t k = b; k < e; ++k) {
+                const int64_t i = order[k];
+                const double w = std::max(0.0, m->omega[active[i]]);
+                const double hw = kHbar * w;
+                if (!(hw > 0.0) || !(a[i] > 0.0)) continue;
+                const double neq = bose(T, w);
+                value += hw * a[i] * (occ[i] - neq);
+                if (T > 0.0) slope -= hw * a[i] * (hw * neq * (neq + 1.0) / (kBoltz * T * T));
+            }
+            if (derivative) *derivative = slope;
+            return value;
+        };
+        double low = 0.0, high = std::max(1000.0, 2.0 * std::max(1.0, cell_T[c]));
+        const double f_low = evaluate(low, nullptr);
+        double f_high = evaluate(high, nullptr);
+        for (int expand = 0; f_high > 0.0 && expand < 20; ++expand) {
+            high *= 2.0;
+            f_high = evaluate(high, nullptr);
+        }
+        double T = std::clamp(cell_T[c], low, high);
+        if (f_low <= 0.0) T = low;
+        else if (f_high >= 0.0) T = high;
+        else {
+            double escale = 0.0;
+            for (int64_t k = b; k < e; ++k) {
+                const int64_t i = order[k];
+                const double hw = kHbar * std::max(0.0, m->omega[active[i]]);
+                escale += hw * a[i] * std::max(1.0, occ[i]);
+            }
+            for (int iter = 0; iter < 40; ++iter) {
+                double d = 0.0;
+                const double v = evaluate(T, &d);
+                if (std::abs(v) <= 1e-13 * std::max(1.0, escale)) break;
+                if (v > 0.0) low = T;
+                else high = T;
+                double cand = 0.5 * (low + high);
+                if (d < 0.0 && std::isfinite(d)) {
+                    const double newton = T - v / d;
+                    if (newton > low && newton < high && std::isfinite(newton)) cand = newton;
+                }
+                T = cand;
+            }
+        }
+        if (pseudo) pseudo[c] = T;
+        double before = 0.0, after = 0.0;
+        for (int64_t k = b; k < e; ++k) {
+            const int64_t i = order[k];
+            const double w = std::max(0.0, m->omega[active[i]]);
+            const double hw = kHbar * w;
+            const double old = occ[i];
+            const double nn = old + a[i] * (bose(T, w) - old);
+            before += hw * old;
+            after += hw * nn;
+            occ[i] = std::max(0.0, nn);
+        }
+        residual += (after - before) * weight_e;
+    }
+    return residual;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- whole-step simulator
+struct orc_sim {
+    pmc_setup setup;
+    pmc_mode_table m;
+    pmc_geometry g;
+    std::vector<pmc_facet> facets;
+    std::vector<pmc_rough_table> rough;
+    std::vector<pmc_reservoir> res;
+    uint32_t k0 = 0, k1 = 0;
+    int step = 0;
+    // carriers
+    std::vector<V3> pos, vel, cpoint;
+    std::vector<int> mode, cell, cfacet;
+    std::vector<double> occ, ctime;
+    std::vector<uint32_t> uid;
+    std::vector<char> alive;
+    uint32_t next_uid = 0;
+    // cells
+    std::vector<double> cell_e, cell_T, cell_flux;
+    // ledgers
+    std::vector<int> previous_absorbed;
+    pmc_ledger ledger{};
+    std::string error;
+};
+
+namespace {
+V3 table_velocity(const orc_sim& s, int a) {
+    return {s.m.velocity[3 * a], s.m.velocity[3 * a + 1], s.m.velocity[3 * a + 2]};
+}
+
+// TransportOperator::update_collision_cache_single (src/solver/TransportOperator.cpp:186-250)
+// for the analytic box: nudge candidates along the nearest facet normal and the velocity.
+bool update_cache(orc_sim& s, int64_t i) {
+    const double speed = norm(s.vel[i]);
+    if (std::isfinite(speed) && speed <= 1e-18) {
+        s.cpoint[i] = s.pos[i];
+        s.cfacet[i] = -1;
+        s.ctime[i] = std::numeric_limits<double>::infinity();
+        return true;
+    }
+    double cp[3], t = 0;
+    int f = orc_trace_box(&s.g, s.pos[i].data(), s.vel[i].data(), cp, &t);
+    if (f < 0 || !std::isfinite(t)) {
+        const V3 original = s.pos[i];
+        double ext = 1.0;
+        for (int a = 0; a < 3; ++a) ext = std::max(ext, std::abs(s.g.bmax[a] - s.g.bmin[a]));
+        const double eps = std::max(s.setup.push_distance_a, 1e-10 * ext);
+        // nearest box plane
+        int best = 0;
+        double bd = std::numeric_limits<double>::infinity();
+        for (int p = 0; p < 6; ++p) {
+            const int a = p / 2;
+            const double d = std::abs(original[a] - (p % 2 ? s.g.bmax[a] : s.g.bmin[a]));
+            if (d < bd) {
+                bd = d;
+                best = p;
+            }
+        }
+        const pmc_facet& F = s.facets[s.g.box_plane_facet[best]];
+        const V3 n{F.normal[0], F.normal[1], F.normal[2]};
+        const V3 dir = mul(s.vel[i], 1.0 / speed);
+        const V3 cands[4] = {add(original, mul(n, eps)), add(original, mul(n, -eps)), add(original, mul(dir, eps)),
+                             add(original, mul(dir, -eps))};
+        for (const V3& c : cands) {
+            bool inside = true;
+            for (int a = 0; a < 3; ++a) inside = inside && c[a] >= s.g.bmin[a] && c[a] <= s.g.bmax[a];
+            if (!inside) continue;
+            f = orc_trace_box(&s.g, c.data(), s.vel[i].data(), cp, &t);
+            if (f >= 0 && std::isfinite(t)) {
+                s.pos[i] = c;
+                break;
+            }
+        }
+    }
+    if (f < 0 || !std::isfinite(t)) {
+        s.cfacet[i] = -1;
+        s.ctime[i] = std::numeric_limits<double>::infinity();
+        return false;
+    }
+    s.cpoint[i] = {cp[0], cp[1], cp[2]};
+    s.cfacet[i] = f;
+    s.ctime[i] = t / s.setup.time_step_ps;
+    return true;
+}
+
+struct StepTally {
+    std::vector<int> absorbed_by_res;
+    double absorbed_e = 0;
+    long long absorbed = 0;
+    long long rough_events = 0, rough_spec = 0, rough_diff = 0;
+};
+
+// TransportOperator::advance_particle (src/solver/TransportOperator.cpp:405-466) with
+// process_boundary_collision (:287-362) and interact_boundary
+// (src/solver/BoundaryInteraction.cpp:53-162), non-conservative policy.
+int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
+    const double dt = s.setup.time_step_ps;
+    const double push = s.setup.push_distance_a;
+    Uniforms U;
+    U.uid = s.uid[i];
+    U.step = static_cast<uint32_t>(s.step);
+    U.k0 = s.k0;
+    U.k1 = s.k1;
+    for (int ev = 0; remaining > 1e-14 && ev < 64; ++ev) {
+        const double t_wall = s.ctime[i] * dt;
+        if (!std::isfinite(t_wall) || t_wall > remaining) {
+            s.pos[i] = add(s.pos[i], mul(s.vel[i], remaining));
+            if (std::isfinite(s.ctime[i])) s.ctime[i] -= remaining / dt;
+            return 0;
+        }
+        remaining -= std::max(0.0, t_wall);
+        const int facet = s.cfacet[i];
+        s.pos[i] = s.cpoint[i];
+        const pmc_facet& F = s.facets[facet];
+        const V3 n{F.normal[0], F.normal[1], F.normal[2]};
+        const int a = s.mode[i];
+        if (F.condition == 'T') {
+            const double dE = kHbar * s.m.omega[a] * (s.occ[i] - s.m.neq_background[a]);
+            tally.absorbed_e += dE * s.m.energy_weight;
+            tally.absorbed++;
+            if (F.reservoir >= 0) tally.absorbed_by_res[F.reservoir]++;
+            s.alive[i] = 0;
+            return 1;
+        }
+        if (F.condition == 'P' && F.periodic_pair >= 0) {
+            s.pos[i] = add(s.pos[i], V3{F.shift[0], F.shift[1], F.shift[2]});
+        } else if (F.condition == 'R' && F.rough_table >= 0) {
+            U.ev = static_cast<uint32_t>(ev);
+            U.used = 0;
+            double u[4];
+            for (int k = 0; k < 4; ++k) u[k] = U.next();
+            int out, kind;
+            double o;
+            orc_rough_sample(&s.m, &s.rough[F.rough_table], a, s.occ[i], u, &out, &o, &kind);
+            tally.rough_events++;
+            if (kind == 1) tally.rough_spec++;
+            if (kind == 2) tally.rough_diff++;
+            V3 v = table_velocity(s, out);
+            if (dot(v, n) > 0.0) v = sub(v, mul(n, 2.0 * dot(v, n)));
+            s.mode[i] = out;
+            s.vel[i] = v;
+            s.occ[i] = o;
+        } else if (F.condition == 'R') {
+            return -PMC_ERR_UNSUPPORTED;
+        } else {
+            s.vel[i] = sub(s.vel[i], mul(n, 2.0 * dot(s.vel[i], n)));
+        }
+        s.pos[i] = add(s.pos[i], mul(s.vel[i], push / std::max(norm(s.vel[i]), 1e-12)));
+        if (!update_cache(s, i)) return -PMC_ERR_BOUNDARY;
+        if (s.ctime[i] * dt < 1e-12) {
+            s.pos[i] = add(s.pos[i], mul(s.vel[i], 1e-12));
+            if (!update_cache(s, i)) return -PMC_ERR_BOUNDARY;
+        }
+    }
+    return remaining > 1e-14 ? -PMC_ERR_EXCESS_EVENTS : 0;
+}
+
+void compact(orc_sim& s) {
+    size_t w = 0;
+    for (size_t r = 0; r < s.alive.size(); ++r) {
+        if (!s.alive[r]) continue;
+        if (w != r) {
+            s.pos[w] = s.pos[r];
+            s.vel[w] = s.vel[r];
+            s.cpoint[w] = s.cpoint[r];
+            s.mode[w] = s.mode[r];
+            s.cell[w] = s.cell[r];
+            s.cfacet[w] = s.cfacet[r];
+            s.occ[w] = s.occ[r];
+            s.ctime[w] = s.ctime[r];
+            s.uid[w] = s.uid[r];
+        }
+        ++w;
+    }
+    for (auto* v : {&s.pos, &s.vel, &s.cpoint}) v->resize(w);
+    for (auto* v : {&s.mode, &s.cell, &s.cfacet}) v->resize(w);
+    for (auto* v : {&s.occ, &s.ctime}) v->resize(w);
+    s.uid.resize(w);
+    s.alive.assign(w, 1);
+}
+
+void refresh(orc_sim& s) {
+    const int nc = s.g.n_cells;
+    std::vector<int32_t> act(s.mode.begin(), s.mode.end());
+    orc_refresh(&s.m, nc, s.g.cell_volume, static_cast<int64_t>(s.mode.size()), act.data(), s.cell.data(),
+                s.occ.data(), s.cell_e.data(), s.cell_T.data());
+}
+
+double thermal_energy(const orc_sim& s) {
+    double total = 0.0;
+    for (int c = 0; c < s.g.n_cells; ++c) total += (s.cell_e[c] - s.m.zero_point_density) * s.g.cell_volume[c];
+    return total;
+}
+
+template <class T>
+void set_error(orc_sim* s, const T& msg) {
+    s->error = msg;
+}
+} // namespace
+
+extern "C" {
+
+void orc_set_threads(int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+}
+
+int orc_create(const pmc_setup* setup, const pmc_mode_table* modes, const pmc_geometry* geometry, int32_t n_rough,
+               const pmc_rough_table* rough, int32_t n_res, const pmc_reservoir* res, orc_sim** out) {
+    if (!setup || !modes || !geometry || !out || geometry->kind != 0) return PMC_ERR_ARGUMENT;
+    auto* s = new orc_sim();
+    s->setup = *setup;
+    s->m = *modes;
+    s->g = *geometry;
+    s->facets.assign(geometry->facets, geometry->facets + geometry->n_facets);
+    s->g.facets = s->facets.data();
+    s->rough.assign(rough, rough + n_rough);
+    s->res.assign(res, res + n_res);
+    s->k0 = static_cast<uint32_t>(setup->seed & 0xffffffffu);
+    s->k1 = static_cast<uint32_t>(setup->seed >> 32) ^ (setup->instance * 0x9E3779B9u);
+    s->cell_e.assign(geometry->n_cells, 0.0);
+    s->cell_T.assign(geometry->n_cells, 0.0);
+    s->cell_flux.assign(3 * static_cast<size_t>(geometry->n_cells), 0.0);
+    s->previous_absorbed.resize(n_res);
+    for (int r = 0; r < n_res; ++r) s->previous_absorbed[r] = res[r].initial_count;
+    *out = s;
+    return PMC_OK;
+}
+
+static int finish_population(orc_sim* s) {
+    const size_t n = s->mode.size();
+    s->vel.resize(n);
+    s->cpoint.resize(n);
+    s->cfacet.assign(n, -1);
+    s->ctime.assign(n, std::numeric_limits<double>::infinity());
+    s->alive.assign(n, 1);
+    s->cell.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        s->vel[i] = table_velocity(*s, s->mode[i]);
+        s->cell[i] = orc_grid_index(&s->g, s->pos[i].data());
+    }
+    bool ok = true;
+#pragma omp parallel for reduction(&& : ok)
+    for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) ok = update_cache(*s, i) && ok;
+    if (!ok) {
+        set_error(s, "Boundary intersection failed while updating collision cache.");
+        return PMC_ERR_BOUNDARY;
+    }
+    s->next_uid = static_cast<uint32_t>(n);
+    refresh(*s);
+    s->ledger = pmc_ledger{};
+    s->ledger.thermal_energy_ev = thermal_energy(*s);
+    s->ledger.live_carriers = static_cast<int64_t>(n);
+    s->ledger.slots = static_cast<int64_t>(n);
+    return PMC_OK;
+}
+
+int orc_initialize(orc_sim* s, const pmc_init* init) {
+    // Same draws as the device's initialization kernel.
+    const int64_t n = init->count;
+    s->pos.resize(n);
+    s->mode.resize(n);
+    s->occ.resize(n);
+    s->uid.resize(n);
+    const int na = s->m.n_active;
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t a[4], b[4];
+        orc_philox(static_cast<uint32_t>(i), 0, 0, 3u, s->k0, s->k1, a);
+        orc_philox(static_cast<uint32_t>(i), 0, 1, 3u, s->k0, s->k1, b);
+        const double u[4] = {orc_u53(a[0], a[1]), orc_u53(a[2], a[3]), orc_u53(b[0], b[1]), orc_u53(b[2], b[3])};
+        V3 p;
+        for (int k = 0; k < 3; ++k) p[k] = s->g.bmin[k] + (s->g.bmax[k] - s->g.bmin[k]) * u[k];
+        const int m = std::min(static_cast<int>(u[3] * na), na - 1);
+        double T = init->uniform_temperature;
+        if (init->temperature_mode == 1) {
+            const V3 ax{init->hot_point[0] - init->cold_point[0], init->hot_point[1] - init->cold_point[1],
+                        init->hot_point[2] - init->cold_point[2]};
+            const V3 rel{p[0] - init->cold_point[0], p[1] - init->cold_point[1], p[2] - init->cold_point[2]};
+            const double sc = std::clamp(dot(rel, ax) / dot(ax, ax), 0.0, 1.0);
+            T = init->cold_temperature + sc * (init->hot_temperature - init->cold_temperature);
+        }
+        V3 pf;
+        for (int k = 0; k < 3; ++k)
+            pf[k] = std::clamp(static_cast<double>(static_cast<float>(p[k])), static_cast<double>(static_cast<float>(s->g.bmin[k])),
+                               static_cast<double>(static_cast<float>(s->g.bmax[k])));
+        if (init->temperature_mode == 2) T = init->cell_temperature[orc_grid_index(&s->g, pf.data())];
+        s->pos[i] = pf;
+        s->mode[i] = m;
+        s->occ[i] = bose(T, s->m.omega[m]);
+        s->uid[i] = static_cast<uint32_t>(i);
+    }
+    return finish_population(s);
+}
+
+int orc_seed(orc_sim* s, int64_t n, const float* position, const int32_t* mode, const double* occupation) {
+    s->pos.resize(n);
+    s->mode.assign(mode, mode + n);
+    s->occ.assign(occupation, occupation + n);
+    s->uid.resize(n);
+    for (int64_t i = 0; i < n; ++i) {
+        s->pos[i] = {position[3 * i], position[3 * i + 1], position[3 * i + 2]};
+        s->uid[i] = static_cast<uint32_t>(i);
+    }
+    return finish_population(s);
+}
+
+int orc_step(orc_sim* s, int32_t nsteps) {
+    const double dt = s->setup.time_step_ps;
+    const int nres = static_cast<int>(s->res.size());
+    for (int it = 0; it < nsteps; ++it) {
+        pmc_ledger& L = s->ledger;
+        L.carrier_steps += static_cast<int64_t>(s->mode.size());
+        StepTally total;
+        total.absorbed_by_res.assign(nres, 0);
+        int err = 0;
+        // Transport of existing carriers (src/MonteCarloSolver.cpp:301-346).
+        const int64_t n0 = static_cast<int64_t>(s->mode.size());
+#pragma omp parallel
+        {
+            StepTally local;
+            local.absorbed_by_res.assign(nres, 0);
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t i = 0; i < n0; ++i) {
+                const int rc = advance(*s, i, dt, local);
+                if (rc < 0) {
+#pragma omp critical
+                    err = err ? err : -rc;
+                }
+                if (s->alive[i]) s->cell[i] = orc_grid_index(&s->g, s->pos[i].data());
+            }
+#pragma omp critical
+            {
+                total.absorbed_e += local.absorbed_e;
+                total.absorbed += local.absorbed;
+                total.rough_events += local.rough_events;
+                total.rough_spec += local.rough_spec;
+                total.rough_diff += local.rough_diff;
+                for (int r = 0; r < nres; ++r) total.absorbed_by_res[r] += local.absorbed_by_res[r];
+            }
+        }
+        if (err) {
+            set_error(s, "transport failure");
+            return err;
+        }
+        compact(*s);
+        // Injection (src/solver/ReservoirInjector.cpp:62-132), one-to-one refill.
+        std::vector<int> counts(s->previous_absorbed);
+        int emissions = 0;
+        for (int c : counts) emissions += c;
+        const int64_t base = static_cast<int64_t>(s->mode.size());
+        s->pos.resize(base + emissions);
+        s->vel.resize(base + emissions);
+        s->cpoint.resize(base + emissions);
+        s->mode.resize(base + emissions);
+        s->cell.resize(base + emissions);
+        s->cfacet.resize(base + emissions);
+        s->occ.resize(base + emissions);
+        s->ctime.resize(base + emissions);
+        s->uid.resize(base + emissions);
+        s->alive.resize(base + emissions, 1);
+        std::vector<double> t_emit(emissions);
+        double injected_e = 0.0;
+        int k = 0;
+        for (int r = 0; r < nres; ++r) {
+            const pmc_reservoir& R = s->res[r];
+            for (int j = 0; j < counts[r]; ++j, ++k) {
+                uint32_t w0[4], w1[4], w2[4];
+                orc_philox(static_cast<uint32_t>(k), static_cast<uint32_t>(s->step), 0, 2u, s->k0, s->k1, w0);
+                orc_philox(static_cast<uint32_t>(k), static_cast<uint32_t>(s->step), 1, 2u, s->k0, s->k1, w1);
+                orc_philox(static_cast<uint32_t>(k), static_cast<uint32_t>(s->step), 2, 2u, s->k0, s->k1, w2);
+                const int m = orc_reservoir_mode(&R, s->m.n_active, orc_u53(w0[0], w0[1]), s->setup.mode_sampling);
+                t_emit[k] = dt * orc_u53(w0[2], w0[3]);
+                double pt[3];
+                orc_surface_point(&R, orc_u53(w1[0], w1[1]), orc_u53(w1[2], w1[3]), orc_u53(w2[0], w2[1]), pt);
+                V3 v = table_velocity(*s, m);
+                const V3 n{R.normal[0], R.normal[1], R.normal[2]};
+                if (dot(v, n) > 0.0) v = mul(v, -1.0);
+                const double speed = std::max(1e-12, norm(v));
+                const int64_t i = base + k;
+                s->pos[i] = add(V3{pt[0], pt[1], pt[2]}, mul(v, s->setup.push_distance_a / speed));
+                s->vel[i] = v;
+                s->mode[i] = m;
+                s->occ[i] = R.occupation[m];
+                s->uid[i] = s->next_uid + static_cast<uint32_t>(k);
+                s->alive[i] = 1;
+                injected_e += kHbar * s->m.omega[m] * (s->occ[i] - s->m.neq_background[m]) * s->m.energy_weight;
+            }
+        }
+        for (int64_t kk = 0; kk < emissions; ++kk)
+            if (!update_cache(*s, base + kk)) {
+                set_error(s, "Boundary intersection failed while updating collision cache.");
+                return PMC_ERR_BOUNDARY;
+            }
+        {
+            StepTally local;
+            local.absorbed_by_res.assign(nres, 0);
+            for (int64_t kk = 0; kk < emissions; ++kk) {
+                const int64_t i = base + kk;
+                const double remain = dt - t_emit[kk];
+                if (remain > 1e-14) {
+                    const int rc = advance(*s, i, remain, local);
+                    if (rc < 0) {
+                        set_error(s, "transport failure");
+                        return -rc;
+                    }
+                }
+                if (s->alive[i]) s->cell[i] = orc_grid_index(&s->g, s->pos[i].data());
+            }
+            total.absorbed_e += local.absorbed_e;
+            total.absorbed += local.absorbed;
+            total.rough_events += local.rough_events;
+            total.rough_spec += local.rough_spec;
+            total.rough_diff += local.rough_diff;
+            for (int r = 0; r < nres; ++r) total.absorbed_by_res[r] += local.absorbed_by_res[r];
+        }
+        compact(*s);
+        s->next_uid += static_cast<uint32_t>(emissions);
+        // Refresh, RTA, refresh (src/MonteCarloSolver.cpp:376-381).
+        refresh(*s);
+        std::vector<int32_t> act(s->mode.begin(), s->mode.end());
+        const double residual = orc_rta(&s->m, s->g.n_cells, s->cell_T.data(), static_cast<int64_t>(act.size()),
+                                        act.data(), s->cell.data(), s->occ.data(), dt, s->m.energy_weight,
+                                        s->setup.lifetime_local, s->setup.lifetime_temperature, nullptr);
+        refresh(*s);
+        // Ledgers (src/solver/EnergyLedger.cpp:58-90, src/solver/BoundaryLedger.cpp:86-141).
+        const double energy = thermal_energy(*s);
+        const double balance = energy - L.thermal_energy_ev - 0.0 - 0.0 - injected_e + total.absorbed_e - residual;
+        L.step_absorbed = total.absorbed;
+        L.step_injected = emissions;
+        L.step_absorbed_energy_ev = total.absorbed_e;
+        L.step_injected_energy_ev = injected_e;
+        L.step_lifetime_residual_ev = residual;
+        L.step_balance_residual_ev = balance;
+        L.total_absorbed += total.absorbed;
+        L.total_injected += emissions;
+        L.total_absorbed_energy_ev += total.absorbed_e;
+        L.total_injected_energy_ev += injected_e;
+        L.total_lifetime_residual_ev += residual;
+        L.total_balance_residual_ev += balance;
+        L.thermal_energy_ev = energy;
+        L.rough_events += total.rough_events;
+        L.rough_specular += total.rough_spec;
+        L.rough_diffuse += total.rough_diff;
+        L.live_carriers = static_cast<int64_t>(s->mode.size());
+        L.slots = L.live_carriers;
+        s->previous_absorbed = total.absorbed_by_res;
+        s->step++;
+        L.step = s->step;
+    }
+    return PMC_OK;
+}
+
+int orc_cells(orc_sim* s, double* temperature, double* energy) {
+    for (int c = 0; c < s->g.n_cells; ++c) {
+        if (temperature) temperature[c] = s->cell_T[c];
+        if (energy) energy[c] = s->cell_e[c];
+    }
+    return PMC_OK;
+}
+
+int orc_heat_flux(orc_sim* s, double* flux) {
+    const int nc = s->g.n_cells;
+    std::vector<double> q(3 * static_cast<size_t>(nc), 0.0);
+    for (size_t i = 0; i < s->mode.size(); ++i) {
+        const int a = s->mode[i];
+        const double e = kHbar * s->m.omega[a] * (s->occ[i] - s->m.neq_background[a]) * s->m.energy_weight;
+        for (int k = 0; k < 3; ++k) q[3 * s->cell[i] + k] += s->vel[i][k] * e;
+    }
+    for (int c = 0; c < nc; ++c)
+        for (int k = 0; k < 3; ++k) flux[3 * c + k] = q[3 * c + k] * (1.0 / s->g.cell_volume[c]) * 1.602176634e13;
+    return PMC_OK;
+}
+
+int orc_ledger(orc_sim* s, pmc_ledger* out) {
+    *out = s->ledger;
+    return PMC_OK;
+}
+
+int64_t orc_carrier_count(orc_sim* s) { return static_cast<int64_t>(s->mode.size()); }
+
+int orc_carriers(orc_sim* s, double* position, int32_t* mode, double* occupation, int32_t* cell, uint32_t* uid) {
+    const size_t n = s->mode.size();
+    std::vector<size_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return s->uid[a] < s->uid[b]; });
+    for (size_t k = 0; k < n; ++k) {
+        const size_t i = order[k];
+        for (int a = 0; a < 3; ++a)
+            if (position) position[3 * k + a] = s->pos[i][a];
+        if (mode) mode[k] = s->mode[i];
+        if (occupation) occupation[k] = s->occ[i];
+        if (cell) cell[k] = s->cell[i];
+        if (uid) uid[k] = s->uid[i];
+    }
+    return PMC_OK;
+}
+
+const char* orc_last_error(orc_sim* s) { return s->error.c_str(); }
+
+void orc_destroy(orc_sim* s) { delete s; }
+
+} // extern "C"

@@ -1,0 +1,74 @@
+/* oracle.h -- CPU restatement (FP64, OpenMP) of the reference's deviational
+ * Monte Carlo step, used as the checker of the HIP path.
+ *
+ * Test infrastructure only: tests/, __graft_entry__.smoke() and the CPU
+ * baseline leg of bench.py load oracle/_build/liboracle.so.  The product never
+ * links or calls it.
+ *
+ * Functions restate the reference (lyushisyan/PhonoMC) file:line cited at each
+ * definition in oracle.cpp.  Random draws come from Philox4x32-10 with the same
+ * counter layout as the device path (id, step, draw, stream), so a GPU run and
+ * an oracle run started from the same population see the same uniforms. */
+#ifndef PMC_ORACLE_H
+#define PMC_ORACLE_H
+#include <stdint.h>
+
+#include "pmc_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Philox4x32-10 block function. */
+void orc_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t out[4]);
+double orc_u53(uint32_t lo, uint32_t hi);
+
+/* Analytic box trace (src/SimulationDomain.cpp:342-450): returns facet or -1. */
+int orc_trace_box(const pmc_geometry* g, const double pos[3], const double vel[3], double point[3], double* t);
+/* fast_grid_index (src/SimulationDomain.cpp:92-126). */
+int orc_grid_index(const pmc_geometry* g, const double pos[3]);
+
+/* RoughBoundarySampler::sample (src/solver/RoughBoundarySampler.cpp:459-533) fed
+ * explicit uniforms u[0..2] in draw order; returns the number consumed. */
+int orc_rough_sample(const pmc_mode_table* m, const pmc_rough_table* r, int in_active, double in_occupation,
+                     const double* u, int* out_active, double* out_occupation, int* kind);
+
+/* Reservoir mode/time draw (src/solver/ReservoirSampler.cpp:127-146 for
+ * sampling = 1; Walker alias for sampling = 0) and surface point
+ * (src/geometry/SurfaceSampling.cpp:54-72). */
+int orc_reservoir_mode(const pmc_reservoir* r, int n_active, double u, int sampling);
+void orc_surface_point(const pmc_reservoir* r, double u_face, double r1, double r2, double point[3]);
+
+/* GridStatistics::refresh_temperatures (src/solver/GridStatistics.cpp:95-235). */
+void orc_refresh(const pmc_mode_table* m, int ncells, const double* volume, int64_t n, const int32_t* active,
+                 const int32_t* cell, const double* occupation, double* energy_density, double* temperature);
+
+/* RtaCollisionOperator::apply (src/solver/RtaCollisionOperator.cpp:13-189);
+ * occupation updated in place, pseudo-temperatures written per cell, residual returned. */
+double orc_rta(const pmc_mode_table* m, int ncells, const double* cell_temperature, int64_t n,
+               const int32_t* active, const int32_t* cell, double* occupation, double dt, double energy_weight,
+               int local_lifetime, double lifetime_temperature, double* pseudo_temperature);
+
+/* Whole-step simulator. */
+typedef struct orc_sim orc_sim;
+int orc_create(const pmc_setup* setup, const pmc_mode_table* modes, const pmc_geometry* geometry, int32_t n_rough,
+               const pmc_rough_table* rough, int32_t n_res, const pmc_reservoir* res, orc_sim** out);
+/* Same population as pmc_initialize_carriers (positions held in FP64). */
+int orc_initialize(orc_sim* sim, const pmc_init* init);
+/* Explicit population: FP32 positions as uploaded to the device, uid = index. */
+int orc_seed(orc_sim* sim, int64_t n, const float* position, const int32_t* mode, const double* occupation);
+int orc_step(orc_sim* sim, int32_t nsteps);
+int orc_cells(orc_sim* sim, double* temperature, double* energy_density);
+int orc_heat_flux(orc_sim* sim, double* flux);
+int orc_ledger(orc_sim* sim, pmc_ledger* ledger);
+/* Live carriers sorted by uid. */
+int64_t orc_carrier_count(orc_sim* sim);
+int orc_carriers(orc_sim* sim, double* position, int32_t* mode, double* occupation, int32_t* cell, uint32_t* uid);
+void orc_set_threads(int threads);
+const char* orc_last_error(orc_sim* sim);
+void orc_destroy(orc_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

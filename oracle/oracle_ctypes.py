@@ -1,0 +1,211 @@
+"""ctypes binding of oracle/_build/liboracle.so (see oracle/oracle.h).
+
+Checker infrastructure for tests/, __graft_entry__.smoke() and the CPU-baseline
+leg of bench.py.  The product never imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
+        from phonomc_amd import runtime as rt
+        L = C.CDLL(LIB)
+        P, d = C.c_void_p, C.c_double
+        L.orc_u53.restype = d
+        L.orc_u53.argtypes = [C.c_uint32, C.c_uint32]
+        L.orc_philox.argtypes = [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)]
+        L.orc_trace_box.argtypes = [C.POINTER(rt.Geometry), C.POINTER(d), C.POINTER(d), C.POINTER(d), C.POINTER(d)]
+        L.orc_grid_index.argtypes = [C.POINTER(rt.Geometry), C.POINTER(d)]
+        L.orc_rough_sample.argtypes = [C.POINTER(rt.ModeTable), C.POINTER(rt.RoughTable), C.c_int, d, C.POINTER(d),
+                                       C.POINTER(C.c_int), C.POINTER(d), C.POINTER(C.c_int)]
+        L.orc_reservoir_mode.argtypes = [C.POINTER(rt.Reservoir), C.c_int, d, C.c_int]
+        L.orc_surface_point.argtypes = [C.POINTER(rt.Reservoir), d, d, d, C.POINTER(d)]
+        L.orc_refresh.argtypes = [C.POINTER(rt.ModeTable), C.c_int, C.POINTER(d), C.c_int64, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.POINTER(d), C.POINTER(d), C.POINTER(d)]
+        L.orc_rta.restype = d
+        L.orc_rta.argtypes = [C.POINTER(rt.ModeTable), C.c_int, C.POINTER(d), C.c_int64, C.POINTER(C.c_int32),
+                              C.POINTER(C.c_int32), C.POINTER(d), d, d, C.c_int, d, C.POINTER(d)]
+        L.orc_create.argtypes = [C.POINTER(rt.Setup), C.POINTER(rt.ModeTable), C.POINTER(rt.Geometry), C.c_int32,
+                                 C.POINTER(rt.RoughTable), C.c_int32, C.POINTER(rt.Reservoir), C.POINTER(P)]
+        L.orc_initialize.argtypes = [P, C.POINTER(rt.Init)]
+        L.orc_seed.argtypes = [P, C.c_int64, C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(d)]
+        L.orc_step.argtypes = [P, C.c_int32]
+        L.orc_cells.argtypes = [P, C.POINTER(d), C.POINTER(d)]
+        L.orc_heat_flux.argtypes = [P, C.POINTER(d)]
+        L.orc_ledger.argtypes = [P, C.POINTER(rt.Ledger)]
+        L.orc_carrier_count.restype = C.c_int64
+        L.orc_carrier_count.argtypes = [P]
+        L.orc_carriers.argtypes = [P, C.POINTER(d), C.POINTER(C.c_int32), C.POINTER(d), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_uint32)]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_last_error.argtypes = [P]
+        L.orc_destroy.argtypes = [P]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def philox(c, k):
+    out = (C.c_uint32 * 4)()
+    lib().orc_philox(*[int(x) & 0xffffffff for x in c], *[int(x) & 0xffffffff for x in k], out)
+    return [int(x) for x in out]
+
+
+class Sim:
+    """Whole-step oracle built from a phonomc_amd.runtime.Problem's table views."""
+
+    def __init__(self, problem, seed=None, mode_sampling=None, threads=0):
+        s, m, g, i, rough, res = problem.views()
+        if seed is not None:
+            s.seed = seed
+        if mode_sampling is not None:
+            s.mode_sampling = mode_sampling
+        self.problem, self.setup, self.modes, self.geometry, self.init = problem, s, m, g, i
+        self._rough = (type(rough[0]) * len(rough))(*rough) if rough else None
+        self._res = (type(res[0]) * len(res))(*res) if res else None
+        if threads:
+            lib().orc_set_threads(threads)
+        h = C.c_void_p()
+        rc = lib().orc_create(C.byref(s), C.byref(m), C.byref(g), len(rough), self._rough, len(res), self._res,
+                              C.byref(h))
+        if rc:
+            raise RuntimeError(f"orc_create failed: {rc}")
+        self.h = h
+        self.n_cells = g.n_cells
+
+    def _check(self, rc):
+        if rc:
+            raise RuntimeError(f"oracle error {rc}: {lib().orc_last_error(self.h).decode()}")
+
+    def initialize(self, init=None):
+        self._check(lib().orc_initialize(self.h, C.byref(init if init is not None else self.init)))
+
+    def seed(self, positions, modes, occupations):
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        md = np.ascontiguousarray(modes, dtype=np.int32)
+        oc = np.ascontiguousarray(occupations, dtype=np.float64)
+        self._check(lib().orc_seed(self.h, len(md), pos.ctypes.data_as(C.POINTER(C.c_float)), _ip(md), _dp(oc)))
+
+    def step(self, n=1):
+        self._check(lib().orc_step(self.h, n))
+
+    def cells(self):
+        t, e = np.zeros(self.n_cells), np.zeros(self.n_cells)
+        lib().orc_cells(self.h, _dp(t), _dp(e))
+        return t, e
+
+    def heat_flux(self):
+        q = np.zeros(3 * self.n_cells)
+        lib().orc_heat_flux(self.h, _dp(q))
+        return q.reshape(-1, 3)
+
+    def ledger(self):
+        from phonomc_amd import runtime as rt
+        led = rt.Ledger()
+        lib().orc_ledger(self.h, C.byref(led))
+        return {k: getattr(led, k) for k, _ in rt.Ledger._fields_}
+
+    def carriers(self):
+        n = lib().orc_carrier_count(self.h)
+        pos, occ = np.zeros(3 * n), np.zeros(n)
+        mode, cell = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.int32)
+        uid = np.zeros(n, dtype=np.uint32)
+        lib().orc_carriers(self.h, _dp(pos), _ip(mode), _dp(occ), _ip(cell),
+                           uid.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return dict(position=pos.reshape(n, 3), mode=mode, occupation=occ, cell=cell, uid=uid)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rough_sample(problem, table_index, in_active, in_occ, uniforms):
+    _, m, _, _, rough, _ = problem.views()
+    r = rough[table_index]
+    out_a, out_o, kind = C.c_int(), C.c_double(), C.c_int()
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    used = lib().orc_rough_sample(C.byref(m), C.byref(r), int(in_active), float(in_occ), _dp(u), C.byref(out_a),
+                                  C.byref(out_o), C.byref(kind))
+    return out_a.value, out_o.value, kind.value, used
+
+
+def trace_box(problem, pos, vel):
+    _, _, g, _, _, _ = problem.views()
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    v = np.ascontiguousarray(vel, dtype=np.float64)
+    pt, t = np.zeros(3), C.c_double()
+    f = lib().orc_trace_box(C.byref(g), _dp(p), _dp(v), _dp(pt), C.byref(t))
+    return f, t.value, pt
+
+
+def grid_index(problem, pos):
+    _, _, g, _, _, _ = problem.views()
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    return lib().orc_grid_index(C.byref(g), _dp(p))
+
+
+def refresh(problem, active, cell, occupation):
+    _, m, g, _, _, _ = problem.views()
+    nc = g.n_cells
+    vol = np.ctypeslib.as_array(g.cell_volume, shape=(nc,)).copy()
+    a = np.ascontiguousarray(active, dtype=np.int32)
+    c = np.ascontiguousarray(cell, dtype=np.int32)
+    o = np.ascontiguousarray(occupation, dtype=np.float64)
+    e, t = np.zeros(nc), np.zeros(nc)
+    lib().orc_refresh(C.byref(m), nc, _dp(vol), len(a), _ip(a), _ip(c), _dp(o), _dp(e), _dp(t))
+    return e, t
+
+
+def rta(problem, cell_temperature, active, cell, occupation, dt, local, lifetime_temperature):
+    _, m, g, _, _, _ = problem.views()
+    ct = np.ascontiguousarray(cell_temperature, dtype=np.float64)
+    a = np.ascontiguousarray(active, dtype=np.int32)
+    c = np.ascontiguousarray(cell, dtype=np.int32)
+    o = np.array(occupation, dtype=np.float64)
+    pseudo = np.zeros(len(ct))
+    res = lib().orc_rta(C.byref(m), len(ct), _dp(ct), len(a), _ip(a), _ip(c), _dp(o), dt, m.energy_weight, int(local),
+                        lifetime_temperature, _dp(pseudo))
+    return o, res, pseudo
+
+
+def reservoir_mode(problem, reservoir_index, u, sampling):
+    _, m, _, _, _, res = problem.views()
+    return lib().orc_reservoir_mode(C.byref(res[reservoir_index]), m.n_active, float(u), int(sampling))
+
+
+def surface_point(problem, reservoir_index, u_face, r1, r2):
+    _, _, _, _, _, res = problem.views()
+    pt = np.zeros(3)
+    lib().orc_surface_point(C.byref(res[reservoir_index]), u_face, r1, r2, _dp(pt))
+    return pt

@@ -1,0 +1,399 @@
+// C entry points into the reference sources (see ref_harness.h).  Test
+// infrastructure: builds reference objects from in-memory inputs and calls
+// their public methods unchanged.
+#include "ref_harness.h"
+
+#include "MonteCarloSolver.h"
+#include "PhononMaterial.h"
+#include "SimulationConfig.h"
+#include "SimulationDomain.h"
+#include "material/MaterialData.h"
+#include "solver/BackgroundCache.h"
+#include "solver/BoundaryInteraction.h"
+#include "solver/GridState.h"
+#include "solver/GridStatistics.h"
+#include "solver/ParticleStorage.h"
+#include "solver/ReservoirSampler.h"
+#include "solver/RoughBoundarySampler.h"
+#include "solver/RtaCollisionOperator.h"
+
+#include <chrono>
+#include <cstring>
+#include <exception>
+#include <functional>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#ifdef PHONOMC_USE_OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+void copy_error(const std::exception& e, char* err, int errlen) {
+    if (err && errlen > 0) {
+        std::strncpy(err, e.what(), static_cast<size_t>(errlen - 1));
+        err[errlen - 1] = '\0';
+    }
+}
+
+template <class F>
+int guarded(char* err, int errlen, F&& body) {
+    try {
+        body();
+        return 0;
+    } catch (const std::exception& e) {
+        copy_error(e, err, errlen);
+        return 1;
+    }
+}
+
+phonomc::MaterialData to_material_data(const ref_material_arrays& a) {
+    phonomc::MaterialData d;
+    d.lattice.volume_a3 = a.volume_a3;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) d.lattice.reciprocal[r][c] = a.reciprocal[3 * r + c];
+    d.qpoints = a.qpoints;
+    d.branches = a.branches;
+    d.mesh = {a.mesh[0], a.mesh[1], a.mesh[2]};
+    const size_t nm = static_cast<size_t>(a.qpoints) * a.branches;
+    d.frequency_thz.assign(a.frequency_thz, a.frequency_thz + nm);
+    d.qpoint_fractions.resize(a.qpoints);
+    for (int q = 0; q < a.qpoints; ++q)
+        d.qpoint_fractions[q] = {a.qpoint_fractions[3 * q], a.qpoint_fractions[3 * q + 1],
+                                 a.qpoint_fractions[3 * q + 2]};
+    d.group_velocity.resize(nm);
+    for (size_t m = 0; m < nm; ++m)
+        d.group_velocity[m] = {a.group_velocity[3 * m], a.group_velocity[3 * m + 1],
+                               a.group_velocity[3 * m + 2]};
+    d.temperatures.assign(a.temperatures, a.temperatures + a.ntemps);
+    d.gamma.assign(a.gamma, a.gamma + nm * a.ntemps);
+    if (a.gamma_normal && a.gamma_umklapp) {
+        d.gamma_normal.assign(a.gamma_normal, a.gamma_normal + nm * a.ntemps);
+        d.gamma_umklapp.assign(a.gamma_umklapp, a.gamma_umklapp + nm * a.ntemps);
+    }
+    if (a.gamma_isotope) d.gamma_resistive["gamma_isotope"].assign(a.gamma_isotope, a.gamma_isotope + nm);
+    return d;
+}
+
+struct Domain {
+    SimulationConfig config;
+    std::unique_ptr<SimulationDomain> domain;
+};
+
+PhononMaterial& mat(void* p) { return *static_cast<PhononMaterial*>(p); }
+Domain& dom(void* p) { return *static_cast<Domain*>(p); }
+
+// Rough sampler configured the way MonteCarloSolver::initialize_rough_boundary_scattering
+// does for non-conservative transport.
+phonomc::RoughBoundarySampler make_rough(const SimulationDomain& g, const PhononMaterial& m) {
+    std::vector<phonomc::RoughFacetSpec> facets;
+    for (int f = 0; f < g.mesh().facet_count(); ++f) {
+        facets.push_back({g.is_rough_facet(f), g.mesh().facet_normals()[f],
+                          std::max(0.0, g.roughness_for_facet(f, 0.0))});
+    }
+    phonomc::RoughBoundarySampler s;
+    s.configure(facets, {&m}, nullptr, false, 0.0);
+    return s;
+}
+
+double canonical(std::mt19937_64& rng) {
+    return std::uniform_real_distribution<double>(0.0, 1.0)(rng);
+}
+} // namespace
+
+extern "C" {
+
+int ref_material_create(const ref_material_arrays* arrays, void** out, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        *out = new PhononMaterial(to_material_data(*arrays), arrays->temperature_lookup_dt);
+    });
+}
+
+void ref_material_destroy(void* material) { delete static_cast<PhononMaterial*>(material); }
+
+int ref_material_active_count(void* material) { return mat(material).active_mode_count(); }
+
+void ref_material_active_modes(void* material, int* q, int* b) {
+    const auto& list = mat(material).active_mode_list();
+    for (size_t i = 0; i < list.size(); ++i) {
+        q[i] = list[i][0];
+        b[i] = list[i][1];
+    }
+}
+
+double ref_bose_occupation(void* material, double t, int q, int b) {
+    return mat(material).bose_occupation(t, {q, b});
+}
+
+double ref_relaxation_weight(void* material, double t, int q, int b, double dt) {
+    return mat(material).mode_relaxation_weight(t, {q, b}, dt);
+}
+
+double ref_temperature_from_energy_density(void* material, double e) {
+    return mat(material).temperature_from_energy_density(e);
+}
+
+double ref_energy_density_from_temperature(void* material, double t) {
+    return mat(material).energy_density_from_temperature(t);
+}
+
+double ref_crystal_energy_density(void* material, double t) {
+    return mat(material).crystal_energy_density(t);
+}
+
+double ref_zero_point_energy_density(void* material) {
+    return mat(material).zero_point_energy_density();
+}
+
+double ref_energy_density_normalization(void* material) {
+    return mat(material).energy_density_normalization();
+}
+
+int ref_domain_create(const char* toml_path, void** out, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        auto d = std::make_unique<Domain>();
+        d->config = load_simulation_config(toml_path);
+        d->config.output_folder.clear();
+        d->domain = std::make_unique<SimulationDomain>(d->config);
+        *out = d.release();
+    });
+}
+
+void ref_domain_destroy(void* domain) { delete static_cast<Domain*>(domain); }
+
+int ref_domain_facet_count(void* domain) { return dom(domain).domain->mesh().facet_count(); }
+
+int ref_domain_grid_count(void* domain) { return dom(domain).domain->grid_count(); }
+
+void ref_domain_facets(void* domain, double* normals, double* centroids, double* areas,
+                       char* conditions, double* shifts, double* roughness, int* rough,
+                       int* face_counts) {
+    const auto& g = *dom(domain).domain;
+    const auto& m = g.mesh();
+    for (int f = 0; f < m.facet_count(); ++f) {
+        for (int a = 0; a < 3; ++a) {
+            normals[3 * f + a] = m.facet_normals()[f][a];
+            centroids[3 * f + a] = m.facet_centroids()[f][a];
+            shifts[3 * f + a] = g.periodic_shift_for_facet(f)[a];
+        }
+        areas[f] = m.facet_areas()[f];
+        conditions[f] = g.facet_boundary_condition(f);
+        roughness[f] = g.roughness_for_facet(f, 0.0);
+        rough[f] = g.is_rough_facet(f) ? 1 : 0;
+        face_counts[f] = static_cast<int>(m.facets()[f].faces.size());
+    }
+}
+
+void ref_domain_grid(void* domain, double* centers, double* volumes) {
+    const auto& g = *dom(domain).domain;
+    for (int c = 0; c < g.grid_count(); ++c) {
+        for (int a = 0; a < 3; ++a) centers[3 * c + a] = g.grid_centers()[c][a];
+        volumes[c] = g.grid_volumes()[c];
+    }
+}
+
+void ref_domain_bounds(void* domain, double* bmin, double* bmax, double* volume) {
+    const auto& g = *dom(domain).domain;
+    for (int a = 0; a < 3; ++a) {
+        bmin[a] = g.bounds_min()[a];
+        bmax[a] = g.bounds_max()[a];
+    }
+    *volume = g.volume();
+}
+
+void ref_domain_trace(void* domain, int n, const double* pos, const double* vel, double* point,
+                      double* t, int* facet) {
+    const auto& g = *dom(domain).domain;
+    for (int i = 0; i < n; ++i) {
+        auto [cp, th, f] = g.trace_boundary_intersection({pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]},
+                                                         {vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]});
+        for (int a = 0; a < 3; ++a) point[3 * i + a] = cp[a];
+        t[i] = th;
+        facet[i] = f;
+    }
+}
+
+void ref_domain_grid_index(void* domain, int n, const double* pos, int* cell) {
+    const auto& g = *dom(domain).domain;
+    for (int i = 0; i < n; ++i) cell[i] = g.fast_grid_index({pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
+}
+
+int ref_domain_sample_surface(void* domain, int facet, int n, uint64_t seed, double* points,
+                              double* uniforms, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& g = *dom(domain).domain;
+        std::mt19937_64 rng(seed);
+        std::mt19937_64 probe = rng;
+        const auto pts = g.mesh().sample_surface_points(n, {facet}, rng);
+        for (int i = 0; i < n; ++i) {
+            for (int a = 0; a < 3; ++a) points[3 * i + a] = pts[i][a];
+            for (int k = 0; k < 3; ++k) uniforms[3 * i + k] = canonical(probe);
+        }
+    });
+}
+
+int ref_rough_sample(void* domain, void* material, int facet, double reference_temperature,
+                     int n, const int* in_active, const double* in_occupation,
+                     const uint64_t* seeds, int* out_active, double* out_occupation,
+                     double* uniforms, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& g = *dom(domain).domain;
+        const auto& m = mat(material);
+        const auto sampler = make_rough(g, m);
+        const int table = sampler.table_index(0, facet);
+        for (int i = 0; i < n; ++i) {
+            std::mt19937_64 rng(seeds[i]);
+            std::mt19937_64 probe = rng;
+            uniforms[2 * i] = canonical(probe);
+            uniforms[2 * i + 1] = canonical(probe);
+            const auto r = sampler.sample(m, table, m.active_mode_at(in_active[i]), in_occupation[i],
+                                          reference_temperature, rng, false);
+            out_active[i] = m.active_index_for_mode(r.mode);
+            out_occupation[i] = r.occupation;
+        }
+    });
+}
+
+int ref_interact_boundary(void* domain, void* material, int facet, double reference_temperature,
+                          double energy_weight, int n, const int* in_active,
+                          const double* in_velocity, const double* in_occupation,
+                          const uint64_t* seeds, int* action, int* out_active,
+                          double* out_velocity, double* out_occupation, double* absorbed_ev,
+                          double* shift, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& g = *dom(domain).domain;
+        const auto& m = mat(material);
+        const auto sampler = make_rough(g, m);
+        // Hit record assembled as TransportOperator::process_boundary_collision does.
+        phonomc::BoundaryHit hit;
+        hit.valid = facet >= 0 && facet < g.mesh().facet_count();
+        hit.condition = g.facet_boundary_condition(facet);
+        hit.outward_normal = g.mesh().facet_normals()[facet];
+        hit.has_periodic_pair = g.has_periodic_pair(facet);
+        if (hit.has_periodic_pair) hit.periodic_shift = g.periodic_shift_for_facet(facet);
+        hit.rough_table = sampler.table_index(0, facet);
+        if (hit.condition == 'R' && hit.rough_table < 0) hit.roughness = g.roughness_for_facet(facet, 0.0);
+        for (int i = 0; i < n; ++i) {
+            const auto mode = m.active_mode_at(in_active[i]);
+            const phonomc::BoundaryState incoming{
+                mode, 0, {in_velocity[3 * i], in_velocity[3 * i + 1], in_velocity[3 * i + 2]},
+                m.mode_angular_frequency(mode), in_occupation[i]};
+            std::mt19937_64 rng(seeds[i]);
+            const auto r = phonomc::interact_boundary(incoming, hit, {false, reference_temperature}, m,
+                                                      energy_weight, sampler, rng);
+            action[i] = static_cast<int>(r.action);
+            out_active[i] = m.active_index_for_mode(r.state.mode);
+            for (int a = 0; a < 3; ++a) {
+                out_velocity[3 * i + a] = r.state.velocity[a];
+                shift[3 * i + a] = r.position_shift[a];
+            }
+            out_occupation[i] = r.state.occupation;
+            absorbed_ev[i] = r.absorbed_energy_ev;
+        }
+    });
+}
+
+int ref_reservoir(void* domain, void* material, int facet, double particle_density,
+                  double time_step, uint64_t seed, int n, int* initial_count,
+                  int* out_active, double* out_time, double* uniforms, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& g = *dom(domain).domain;
+        const auto& m = mat(material);
+        phonomc::ReservoirSampler sampler;
+        sampler.configure({{facet, 0, g.reservoir_value_for_facet(facet, 300.0),
+                            g.mesh().facet_areas()[facet], g.mesh().facet_normals()[facet]}},
+                          {&m}, g.mesh().facet_count(), particle_density, time_step);
+        *initial_count = sampler.initial_emission_count(0);
+        std::mt19937_64 rng(seed);
+        std::mt19937_64 probe = rng;
+        const auto draws = sampler.sample(0, n, rng);
+        for (int i = 0; i < n; ++i) {
+            out_active[i] = static_cast<int>(draws[i].active_index);
+            out_time[i] = draws[i].time_ps;
+            uniforms[2 * i] = canonical(probe);
+            uniforms[2 * i + 1] = canonical(probe);
+        }
+    });
+}
+
+int ref_refresh_temperatures(void* material, int ncells, const double* volumes, double t_bg,
+                             double particle_volume, int n, const int* active,
+                             const int* cell, const double* occupation, double* cell_energy,
+                             double* cell_temperature, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& m = mat(material);
+        std::vector<phonomc::BackgroundCache> background;
+        background.emplace_back(m, t_bg, particle_volume, false);
+        phonomc::ParticleStorage p;
+        p.reset(n);
+        for (int i = 0; i < n; ++i) {
+            p.modes[i] = m.active_mode_at(active[i]);
+            p.grid_ids[i] = cell[i];
+            p.occupation[i] = occupation[i];
+            p.material_ids[i] = 0;
+        }
+        phonomc::GridState grid;
+        grid.material_ids.assign(ncells, 0);
+        std::vector<double> vol(volumes, volumes + ncells);
+        phonomc::GridStatistics stats;
+        stats.refresh_temperatures(p, grid, vol, {&m}, background, false);
+        for (int c = 0; c < ncells; ++c) {
+            cell_energy[c] = grid.energy_density[c];
+            cell_temperature[c] = grid.temperatures[c];
+        }
+    });
+}
+
+int ref_rta_apply(void* material, int ncells, const double* cell_temperature, int n,
+                  const int* active, const int* cell, const double* carrier_temperature,
+                  double* occupation, double dt, double particle_volume, int local_lifetime,
+                  double lifetime_temperature, double* residual_ev, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& m = mat(material);
+        phonomc::ParticleStorage p;
+        p.reset(n);
+        for (int i = 0; i < n; ++i) {
+            p.modes[i] = m.active_mode_at(active[i]);
+            p.grid_ids[i] = cell[i];
+            p.occupation[i] = occupation[i];
+            p.temperatures[i] = carrier_temperature[i];
+            p.material_ids[i] = 0;
+        }
+        phonomc::CellParticleIndex index;
+        index.ensure(p, ncells);
+        std::vector<int> materials(ncells, 0);
+        std::vector<double> temps(cell_temperature, cell_temperature + ncells);
+        phonomc::RtaCollisionOperator rta;
+        *residual_ev = rta.apply(p, {&m}, {index.counts(), materials, temps, &index},
+                                 {dt, particle_volume, local_lifetime != 0, lifetime_temperature});
+        for (int i = 0; i < n; ++i) occupation[i] = p.occupation[i];
+    });
+}
+
+int ref_run(const char* toml_path, const ref_material_arrays* arrays, int steps, int threads,
+            double* seconds, long long* particle_steps, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+#ifdef PHONOMC_USE_OPENMP
+        if (threads > 0) omp_set_num_threads(threads);
+#else
+        (void)threads;
+#endif
+        SimulationConfig config = load_simulation_config(toml_path);
+        SimulationDomain domain(config);
+        PhononMaterial material(to_material_data(*arrays), config.temperature_lookup_dt);
+        std::vector<std::reference_wrapper<const PhononMaterial>> materials{std::cref(material)};
+        MonteCarloSolver solver(config, domain, materials);
+        long long carried = 0;
+        const auto begin = std::chrono::steady_clock::now();
+        for (int s = 0; s < steps; ++s) {
+            carried += solver.particle_count();
+            solver.run_timestep();
+        }
+        const auto end = std::chrono::steady_clock::now();
+        *seconds = std::chrono::duration<double>(end - begin).count();
+        *particle_steps = carried;
+    });
+}
+} // extern "C"

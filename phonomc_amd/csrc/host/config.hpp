@@ -1,0 +1,72 @@
+// Sectioned-TOML input for the Monte Carlo step, with the reference's key set,
+// units (nm in, Angstrom inside) and rejection messages
+// (src/config/ConfigParsing.cpp:330-971, include/SimulationConfig.h:42-146).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <optional>
+#include <string>
+#include <vector>
+
+namespace pmc {
+
+enum class CollisionModel { Rta, Callaway };
+enum class HeatProfile { Uniform, Gaussian };
+
+struct SimConfig {
+    std::string input_directory;
+    std::string model = "box";
+    std::vector<double> sizes{2000.0, 2000.0, 2000.0}; // Angstrom
+    bool merge_coplanar_facets = true;
+    double particle_count = 1e4;
+    double time_step = 1.0; // ps
+    int iterations = 10000;
+    int convergence_write_interval = 10;
+    std::uint64_t random_seed = 12345;
+    bool compute_kappa = false;
+    bool write_cell_heat_flux = false;
+    int transport_axis = -1;
+    bool profile_timers = false;
+    bool progress_temperature_summary_only = false;
+    double temperature_lookup_dt = 0.1;
+    double background_temperature = 300.0;
+    bool lifetime_local = true;
+    double lifetime_temperature = 300.0;
+    bool initial_linear = false;
+    double initial_uniform_temperature = 300.0;
+    std::array<int, 3> grid{1, 1, 1};
+    CollisionModel collision = CollisionModel::Rta;
+    bool callaway_nonlinear = false;
+    bool conservative_boundaries = false;
+
+    // Boundary selectors: relative boxes [xmin,ymin,zmin,xmax,ymax,zmax].
+    std::vector<std::array<double, 6>> boundary_boxes;
+    std::vector<char> boundary_conditions; // 'T', 'P', 'R'
+    std::vector<double> boundary_values;   // raw TOML values (K or nm)
+    std::vector<std::array<double, 6>> periodic_pairs;
+
+    std::string material_folder;
+    std::string output_folder;
+
+    // Prescribed heat source (Angstrom inside).
+    bool heat_source_enabled = false;
+    HeatProfile heat_profile = HeatProfile::Uniform;
+    std::vector<double> heat_min, heat_max, heat_center, heat_sigma;
+    double heat_power_density = 0.0; // W/m^3
+    std::optional<double> heat_total_power;
+
+    // Keys parsed for compatibility whose features this build rejects at solver set-up.
+    bool has_gradient_drive = false;
+    bool has_layered_materials = false;
+    bool has_convergence_stop = false;
+    bool has_mode_excitation = false;
+    bool has_non_thermal_spectrum = false;
+    bool has_time_profile = false;
+};
+
+SimConfig parse_toml_file(const std::string& path);
+SimConfig parse_toml_text(const std::string& text, const std::string& input_directory);
+void validate_config(const SimConfig& c);
+std::string create_indexed_output_folder(const std::string& base);
+
+} // namespace pmc

@@ -1,0 +1,379 @@
+#include "solver.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <filesystem>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <stdexcept>
+#include <string>
+
+namespace pmc {
+
+std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions& opt) {
+    if (c.collision != CollisionModel::Rta)
+        throw std::runtime_error("scattering.model = 'callaway' is not supported by this build (RTA only).");
+    if (c.heat_source_enabled) throw std::runtime_error("heat_source is not supported by this build.");
+    if (c.has_gradient_drive) throw std::runtime_error("driving.temperature_gradient is not supported by this build.");
+    if (c.has_layered_materials) throw std::runtime_error("materials.folders (layered materials) is not supported by this build.");
+    if (c.has_convergence_stop) throw std::runtime_error("convergence.enabled requires gradient driving.");
+    if (c.has_mode_excitation) throw std::runtime_error("mode_excitation is not supported by this build.");
+    if (c.conservative_boundaries) throw std::runtime_error("simulation.conservative_boundaries is not supported by this build.");
+    if (c.material_folder.empty()) throw std::runtime_error("io.material_folder is required.");
+
+    auto p = std::make_unique<Problem>();
+    p->config = c;
+    p->raw = load_material(c.material_folder, c.input_directory);
+    p->modes = build_mode_tables(p->raw, c.temperature_lookup_dt);
+    p->domain = build_domain(c);
+    const auto& m = p->modes;
+    const auto& d = p->domain;
+    const long long n = std::llround(c.particle_count);
+    // src/MonteCarloSolver.cpp:72-77.
+    p->particle_density = static_cast<double>(n) / d.volume;
+    p->particle_volume = d.volume / static_cast<double>(n);
+    // include/solver/BackgroundCache.h:18-21.
+    p->energy_weight = static_cast<double>(m.n_active) * p->particle_volume / m.energy_normalization;
+    p->background_energy_density = m.energy_density_from_temperature(c.background_temperature);
+    p->neq_background.resize(m.n_active);
+    p->omega_active.resize(m.n_active);
+    p->velocity_active.resize(3 * static_cast<size_t>(m.n_active));
+    for (int a = 0; a < m.n_active; ++a) {
+        const int f = m.active_flat[a];
+        p->neq_background[a] = m.bose(c.background_temperature, f);
+        p->omega_active[a] = m.omega_flat[f];
+        for (int k = 0; k < 3; ++k) p->velocity_active[3 * a + k] = m.velocity_flat[f][k];
+    }
+    // Rough tables on every 'R' facet (src/solver/RoughBoundaryInitialization.cpp:9-25).
+    std::vector<bool> enabled(d.facets.size());
+    for (size_t f = 0; f < d.facets.size(); ++f) enabled[f] = d.facets[f].condition == 'R';
+    p->rough = build_rough_tables(d, m, enabled, 0.0, p->facet_to_rough);
+    p->reservoirs = build_reservoir_tables(d, m, p->particle_density, c.time_step, p->facet_to_reservoir);
+
+    // C-ABI views.
+    for (size_t f = 0; f < d.facets.size(); ++f) {
+        pmc_facet v{};
+        for (int k = 0; k < 3; ++k) {
+            v.normal[k] = d.facets[f].normal[k];
+            v.shift[k] = d.facets[f].shift[k];
+        }
+        v.condition = d.facets[f].condition;
+        v.rough_table = p->facet_to_rough[f];
+        v.reservoir = p->facet_to_reservoir[f];
+        v.periodic_pair = d.facets[f].periodic_pair;
+        p->facets.push_back(v);
+    }
+    auto& mv = p->mode_view;
+    mv.n_active = m.n_active;
+    mv.n_flat = m.n_flat;
+    mv.active_flat = m.active_flat.data();
+    mv.omega = p->omega_active.data();
+    mv.velocity = p->velocity_active.data();
+    mv.neq_background = p->neq_background.data();
+    mv.n_temps = static_cast<int>(m.temps.size());
+    mv.temps = m.temps.data();
+    mv.gamma_total = m.gamma_total.data();
+    mv.energy_weight = p->energy_weight;
+    mv.background_energy_density = p->background_energy_density;
+    mv.zero_point_density = m.zero_point_density;
+    mv.lut_n = static_cast<int>(m.lut_t.size());
+    mv.lut_t = m.lut_t.data();
+    mv.lut_e = m.lut_e.data();
+    auto& g = p->geometry_view;
+    g.kind = 0;
+    for (int k = 0; k < 3; ++k) {
+        g.bmin[k] = d.bmin[k];
+        g.bmax[k] = d.bmax[k];
+        g.grid[k] = d.grid[k];
+        g.inv_cell[k] = d.inv_cell[k];
+    }
+    g.n_facets = static_cast<int>(p->facets.size());
+    g.facets = p->facets.data();
+    for (int k = 0; k < 6; ++k) g.box_plane_facet[k] = d.box_plane_facet[k];
+    g.voxel_to_cell = d.voxel_to_cell.data();
+    g.n_cells = d.cell_count();
+    g.cell_volume = d.cell_volume.data();
+    for (const auto& r : p->rough) {
+        pmc_rough_table v{};
+        v.specularity = r.specularity.data();
+        v.spec_match = r.spec_match.data();
+        v.diffuse_begin = r.diffuse_begin.data();
+        v.diffuse_end = r.diffuse_end.data();
+        v.n_outgoing = static_cast<int>(r.outgoing_sorted.size());
+        v.outgoing_active = r.outgoing_active.data();
+        v.outgoing_sorted = r.outgoing_sorted.data();
+        v.residual_prefix = r.residual_prefix.data();
+        v.n_roulette = static_cast<int>(r.roulette_active.size());
+        v.roulette_active = r.roulette_active.data();
+        v.roulette_cdf = r.roulette_cdf.data();
+        v.strict_elastic = r.strict_elastic ? 1 : 0;
+        p->rough_views.push_back(v);
+    }
+    double expected = 0;
+    for (const auto& r : p->reservoirs) {
+        pmc_reservoir v{};
+        v.facet = r.facet;
+        v.temperature = r.temperature;
+        for (int k = 0; k < 3; ++k) v.normal[k] = r.normal[k];
+        v.expected_count = r.expected;
+        v.initial_count = r.initial_count;
+        v.cdf = r.cdf.data();
+        v.alias_prob = r.alias_prob.data();
+        v.alias_index = r.alias_index.data();
+        v.occupation = r.occupation.data();
+        v.n_faces = static_cast<int>(r.face_cdf.size());
+        v.face_cdf = r.face_cdf.data();
+        v.triangles = r.triangles.data();
+        p->reservoir_views.push_back(v);
+        expected += r.expected;
+    }
+    auto& s = p->setup;
+    s.capacity = static_cast<int64_t>(std::ceil(static_cast<double>(n) * opt.capacity_factor + 64.0 * expected + 4096.0));
+    s.time_step_ps = c.time_step;
+    s.push_distance_a = 1e-10 * std::max(c.time_step, 1.0);
+    s.seed = c.random_seed;
+    s.instance = 0;
+    s.background_temperature = c.background_temperature;
+    s.lifetime_local = c.lifetime_local ? 1 : 0;
+    s.lifetime_temperature = c.lifetime_temperature;
+    s.refill_policy = 0;
+    s.mode_sampling = opt.mode_sampling;
+    s.temperature_inversion = 0;
+
+    // Initial temperatures (src/solver/Initialization.cpp:181-265).
+    auto& in = p->init;
+    in.count = n;
+    double tmin = 299.0, tmax = 301.0;
+    bool any = false;
+    for (const auto& r : p->reservoirs) {
+        if (!std::isfinite(r.temperature)) continue;
+        tmin = any ? std::min(tmin, r.temperature) : r.temperature;
+        tmax = any ? std::max(tmax, r.temperature) : r.temperature;
+        any = true;
+    }
+    if (tmin > tmax) std::swap(tmin, tmax);
+    if (!c.initial_linear) {
+        in.temperature_mode = 0;
+        in.uniform_temperature = c.initial_uniform_temperature;
+    } else {
+        int cold = -1, hot = -1;
+        double ct = std::numeric_limits<double>::infinity(), ht = -std::numeric_limits<double>::infinity();
+        for (int f : d.reservoir_facets()) {
+            const double t = d.facets[f].value;
+            if (!std::isfinite(t)) continue;
+            if (t < ct) { ct = t; cold = f; }
+            if (t > ht) { ht = t; hot = f; }
+        }
+        in.temperature_mode = 0;
+        in.uniform_temperature = 0.5 * (tmin + tmax);
+        if (cold >= 0 && hot >= 0 && ht > ct + 1e-12) {
+            const auto& pc = d.facets[cold].centroid;
+            const auto& ph = d.facets[hot].centroid;
+            const double ax[3] = {ph[0] - pc[0], ph[1] - pc[1], ph[2] - pc[2]};
+            if (ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2] > 1e-20) {
+                in.temperature_mode = 1;
+                for (int k = 0; k < 3; ++k) {
+                    in.cold_point[k] = pc[k];
+                    in.hot_point[k] = ph[k];
+                }
+                in.cold_temperature = ct;
+                in.hot_temperature = ht;
+            }
+        }
+    }
+    return p;
+}
+
+MonteCarloSolver::MonteCarloSolver(std::unique_ptr<Problem> problem, std::string folder, std::ostream* log)
+    : problem_(std::move(problem)), folder_(std::move(folder)), log_(log) {
+    auto& p = *problem_;
+    throw_if_failed(pmc_create(&p.setup, &ctx_));
+    throw_if_failed(pmc_upload_material(ctx_, &p.mode_view));
+    throw_if_failed(pmc_upload_geometry(ctx_, &p.geometry_view));
+    throw_if_failed(pmc_upload_boundary_tables(ctx_, static_cast<int>(p.rough_views.size()), p.rough_views.data(),
+                                               static_cast<int>(p.reservoir_views.size()), p.reservoir_views.data()));
+    throw_if_failed(pmc_initialize_carriers(ctx_, &p.init));
+    if (!folder_.empty()) {
+        write_domain_summary(p.domain, p.config, folder_);
+        write_convergence_header();
+    }
+    if (log_) {
+        *log_ << "MonteCarloSolver initialized: particle_count=" << p.init.count << ", time_step=" << p.config.time_step
+              << ", convergence_write_interval=" << p.config.convergence_write_interval
+              << ", density=" << p.particle_density << '\n';
+    }
+    record_observables();
+}
+
+MonteCarloSolver::~MonteCarloSolver() { pmc_destroy(ctx_); }
+
+void MonteCarloSolver::throw_if_failed(int rc) const {
+    if (rc == PMC_OK) return;
+    pmc_error e{};
+    if (ctx_) pmc_last_error(ctx_, &e);
+    std::string msg = e.message[0] ? e.message : pmc_status_string(rc);
+    throw std::runtime_error(msg);
+}
+
+long long MonteCarloSolver::particle_count() const { return ledger().live_carriers; }
+
+pmc_ledger MonteCarloSolver::ledger() const {
+    pmc_ledger l{};
+    pmc_read_ledger(ctx_, &l);
+    return l;
+}
+
+std::vector<double> MonteCarloSolver::cell_temperatures() const {
+    std::vector<double> t(problem_->domain.cell_count());
+    pmc_read_cells(ctx_, t.data(), nullptr);
+    return t;
+}
+
+void MonteCarloSolver::run_timestep() { run_steps(1); }
+
+void MonteCarloSolver::run_steps(int n) {
+    const int interval = problem_->config.convergence_write_interval;
+    while (n > 0) {
+        int next = step_ == 0 ? 1 : ((step_ / interval) + 1) * interval;
+        const int k = std::min(n, next - step_);
+        const auto t0 = std::chrono::steady_clock::now();
+        throw_if_failed(pmc_step(ctx_, k));
+        step_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        step_ += k;
+        n -= k;
+        if (step_ == 1 || step_ % interval == 0) record_observables();
+    }
+}
+
+void MonteCarloSolver::record_observables() {
+    // MonteCarloSolver::update_heat_flux_and_conductivity (src/solver/Observables.cpp:38-142).
+    const auto& p = *problem_;
+    const auto& c = p.config;
+    const auto& d = p.domain;
+    const int nc = d.cell_count();
+    cell_T_ = cell_temperatures();
+    std::vector<double> q(3 * static_cast<size_t>(nc));
+    throw_if_failed(pmc_read_heat_flux(ctx_, q.data()));
+    flux_.assign(nc, {0, 0, 0});
+    for (int i = 0; i < nc; ++i) flux_[i] = {q[3 * i], q[3 * i + 1], q[3 * i + 2]};
+    int axis = 0;
+    if (c.transport_axis >= 0) axis = c.transport_axis;
+    else if (c.grid[1] > c.grid[0] && c.grid[1] >= c.grid[2]) axis = 1;
+    else if (c.grid[2] > c.grid[0] && c.grid[2] > c.grid[1]) axis = 2;
+    double phi = 0, vol = 0;
+    for (int i = 0; i < nc; ++i) {
+        phi += flux_[i][axis] * d.cell_volume[i];
+        vol += d.cell_volume[i];
+    }
+    mean_flux_ = phi / vol;
+    kappa_eff_ = kappa_int_ = 0;
+    if (c.compute_kappa) {
+        double tl = cell_T_.front(), tr = cell_T_.back();
+        const auto rf = d.reservoir_facets();
+        if (rf.size() >= 2) {
+            int fl = rf.front(), fr = rf.front();
+            for (int f : rf) {
+                if (d.facets[f].centroid[axis] < d.facets[fl].centroid[axis]) fl = f;
+                if (d.facets[f].centroid[axis] > d.facets[fr].centroid[axis]) fr = f;
+            }
+            tl = d.facets[fl].value;
+            tr = d.facets[fr].value;
+        }
+        const double L = std::abs(d.bmax[axis] - d.bmin[axis]) * 1e-10;
+        const double grad_end = std::abs(L) > 1e-24 ? (tr - tl) / L : 0.0;
+        kappa_eff_ = std::abs(grad_end) > 1e-18 ? -mean_flux_ / grad_end : 0.0;
+        double sx = 0, st = 0, sxx = 0, sxt = 0;
+        int nfit = 0;
+        for (int i = 0; i < nc; ++i) {
+            const double x = d.cell_center[i][axis] * 1e-10, t = cell_T_[i];
+            sx += x;
+            st += t;
+            sxx += x * x;
+            sxt += x * t;
+            ++nfit;
+        }
+        double grad_fit = 0;
+        const double window = std::abs(cell_T_.back() - cell_T_.front());
+        if (nfit >= 2) {
+            const double den = static_cast<double>(nfit) * sxx - sx * sx;
+            if (std::abs(den) > 1e-30 && window > 1e-6) grad_fit = (static_cast<double>(nfit) * sxt - sx * st) / den;
+        }
+        kappa_int_ = std::abs(grad_fit) > 1e-18 ? -mean_flux_ / grad_fit : 0.0;
+    }
+    if (!folder_.empty()) append_convergence_row();
+}
+
+void MonteCarloSolver::write_convergence_header() {
+    // ResultWriter::format_convergence_header (src/solver/ResultWriter.cpp:82-98).
+    std::ofstream out(std::filesystem::path(folder_) / "convergence.txt", std::ios::trunc);
+    if (!out) throw std::runtime_error("Cannot open output file: " + folder_ + "/convergence.txt");
+    out << "# timestep time_ps";
+    for (int i = 0; i < problem_->domain.cell_count(); ++i) out << " T_" << (i + 1);
+    out << " heatflux kappa_int kappa_eff particle_count absorbed injected recovered net"
+           " reservoir_absorbed_energy_ev reservoir_injected_energy_ev"
+           " hs_occupation_updates hs_injected_energy_ev lifetime_energy_residual_ev"
+           " energy_balance_residual_ev total_thermal_energy_ev"
+           " hs_prescribed_total_ev hs_pending_total_ev hs_ledger_residual_ev\n";
+    if (problem_->config.write_cell_heat_flux) {
+        std::ofstream f(std::filesystem::path(folder_) / "grid_heat_flux.csv", std::ios::trunc);
+        f << "timestep,time_ps,cell_index,temperature_K,qx_W_m2,qy_W_m2,qz_W_m2\n";
+    }
+}
+
+void MonteCarloSolver::append_convergence_row() {
+    // ResultWriter::format_convergence (src/solver/ResultWriter.cpp:100-119).
+    const pmc_ledger l = ledger();
+    const double time = static_cast<double>(step_) * problem_->config.time_step;
+    std::ofstream out(std::filesystem::path(folder_) / "convergence.txt", std::ios::app);
+    if (!out) throw std::runtime_error("Cannot open output file: " + folder_ + "/convergence.txt");
+    out << step_ << " " << time;
+    for (double t : cell_T_) out << " " << t;
+    out << std::setprecision(12);
+    out << " " << mean_flux_ << " " << kappa_int_ << " " << kappa_eff_ << " " << l.live_carriers << " "
+        << l.step_absorbed << " " << l.step_injected << " " << 0 << " " << (l.step_injected - l.step_absorbed) << " "
+        << l.step_absorbed_energy_ev << " " << l.step_injected_energy_ev << " " << 0 << " " << 0.0 << " "
+        << l.step_lifetime_residual_ev << " " << l.step_balance_residual_ev << " " << l.thermal_energy_ev << " "
+        << 0.0 << " " << 0.0 << " " << 0.0 << '\n';
+    if (problem_->config.write_cell_heat_flux) {
+        std::ofstream f(std::filesystem::path(folder_) / "grid_heat_flux.csv", std::ios::app);
+        f << std::setprecision(17);
+        for (size_t i = 0; i < flux_.size(); ++i)
+            f << step_ << ',' << time << ',' << i << ',' << cell_T_[i] << ',' << flux_[i][0] << ',' << flux_[i][1]
+              << ',' << flux_[i][2] << '\n';
+    }
+}
+
+void MonteCarloSolver::append_profile_summary(std::ostream& out) const {
+    // [performance] / [rough_boundary_diagnostics] keys of
+    // ResultWriter::format_run_summary (src/solver/ResultWriter.cpp:172-341) plus a [gpu] section.
+    const pmc_ledger l = ledger();
+    out << std::setprecision(17);
+    out << "\n[performance]\n";
+    out << "openmp_enabled = false\n";
+    out << "openmp_thread_count = 1\n";
+    out << "random_seed = " << problem_->config.random_seed << '\n';
+    out << "profile_timers_enabled = " << (problem_->config.profile_timers ? "true" : "false") << '\n';
+    out << "collision_cache_corrections_total = 0\n";
+    out << "collision_cache_failures_total = 0\n";
+    out << "transport_failure_policy = strict\n";
+    out << "excessive_collision_recoveries_total = 0\n";
+    const double den = std::max(1.0, static_cast<double>(l.rough_events));
+    out << "\n[rough_boundary_diagnostics]\n";
+    out << "events_total = " << l.rough_events << '\n';
+    out << "specular_selected = " << l.rough_specular << '\n';
+    out << "diffuse_selected = " << l.rough_diffuse << '\n';
+    out << "specular_ratio = " << static_cast<double>(l.rough_specular) / den << '\n';
+    out << "diffuse_ratio = " << static_cast<double>(l.rough_diffuse) / den << '\n';
+    out << "\n[gpu]\n";
+    out << "device_steps = " << l.step << '\n';
+    out << "carrier_steps = " << l.carrier_steps << '\n';
+    out << "step_loop_seconds = " << step_seconds_ << '\n';
+    out << "particle_steps_per_second = " << (step_seconds_ > 0 ? static_cast<double>(l.carrier_steps) / step_seconds_ : 0.0)
+        << '\n';
+    out << "rta_pseudo_temperature_passes = " << l.newton_passes << '\n';
+    out << "total_energy_balance_residual_ev = " << l.total_balance_residual_ev << '\n';
+}
+
+} // namespace pmc

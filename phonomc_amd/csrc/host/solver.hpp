@@ -1,0 +1,95 @@
+// Host facade with the surface of the reference's MonteCarloSolver
+// (include/MonteCarloSolver.h:42-103): builds every immutable table once on the
+// host, hands them to the device context through the C ABI (pmc_capi.h) and
+// advances whole time steps there.  Observables and result files follow the
+// reference cadence and formats (src/solver/Diagnostics.cpp:200-293,
+// src/solver/ResultWriter.cpp:25-119).
+#pragma once
+#include "boundary_tables.hpp"
+#include "config.hpp"
+#include "domain.hpp"
+#include "material.hpp"
+#include "pmc_capi.h"
+
+#include <array>
+#include <memory>
+#include <ostream>
+#include <string>
+#include <vector>
+
+namespace pmc {
+
+// Everything the device needs, built on the host from one configuration.
+struct Problem {
+    SimConfig config;
+    Domain domain;
+    RawMaterial raw;
+    ModeTables modes;
+    std::vector<RoughTable> rough;
+    std::vector<ReservoirTable> reservoirs;
+    std::vector<int> facet_to_rough, facet_to_reservoir;
+    double particle_volume = 0;   // V / N
+    double particle_density = 0;  // N / V
+    double energy_weight = 0;
+    double background_energy_density = 0;
+    std::vector<double> neq_background; // per active mode
+    std::vector<double> omega_active, velocity_active; // per active mode (3 per mode for velocity)
+    // C-ABI views (pointers into the vectors above)
+    std::vector<pmc_facet> facets;
+    std::vector<pmc_rough_table> rough_views;
+    std::vector<pmc_reservoir> reservoir_views;
+    pmc_mode_table mode_view{};
+    pmc_geometry geometry_view{};
+    pmc_setup setup{};
+    pmc_init init{};
+};
+
+struct ProblemOptions {
+    int mode_sampling = 0;       // 0 alias, 1 inverse CDF
+    double capacity_factor = 1.25;
+};
+
+std::unique_ptr<Problem> build_problem(const SimConfig& config, const ProblemOptions& options = {});
+
+class MonteCarloSolver {
+public:
+    MonteCarloSolver(std::unique_ptr<Problem> problem, std::string output_folder, std::ostream* log = nullptr);
+    ~MonteCarloSolver();
+    MonteCarloSolver(const MonteCarloSolver&) = delete;
+    MonteCarloSolver& operator=(const MonteCarloSolver&) = delete;
+
+    void run_timestep();
+    // Advances n steps, recording observables at the reference cadence.
+    void run_steps(int n);
+    int current_timestep() const { return step_; }
+    bool converged() const { return false; }
+    long long particle_count() const;
+    void append_profile_summary(std::ostream& out) const;
+
+    const Problem& problem() const { return *problem_; }
+    pmc_ctx* context() const { return ctx_; }
+    std::vector<double> cell_temperatures() const;
+    double kappa_eff() const { return kappa_eff_; }
+    double kappa_int() const { return kappa_int_; }
+    double average_heat_flux() const { return mean_flux_; }
+    pmc_ledger ledger() const;
+    double step_seconds() const { return step_seconds_; }
+
+private:
+    void throw_if_failed(int rc) const;
+    void record_observables();
+    void write_convergence_header();
+    void append_convergence_row();
+
+    std::unique_ptr<Problem> problem_;
+    std::string folder_;
+    std::ostream* log_;
+    pmc_ctx* ctx_ = nullptr;
+    int step_ = 0;
+    double kappa_eff_ = 0, kappa_int_ = 0, mean_flux_ = 0;
+    std::vector<double> cell_T_;
+    std::vector<std::array<double, 3>> flux_;
+    double step_seconds_ = 0;
+};
+
+} // namespace pmc

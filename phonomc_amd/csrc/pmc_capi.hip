@@ -1,0 +1,623 @@
+// C ABI of the device step (include/pmc_capi.h): owns device buffers and
+// sequences the step kernels on one HIP stream.
+#include "pmc_capi.h"
+#include "pmc_device.hpp"
+#include "pmc_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using pmc_dev::DevState;
+using pmc_dev::FacetDev;
+using pmc_dev::NewtonCell;
+using pmc_dev::ResDev;
+using pmc_dev::RoughDev;
+using pmc_dev::View;
+
+namespace {
+struct Carriers {
+    float *px = nullptr, *py = nullptr, *pz = nullptr;
+    int* mode = nullptr;
+    double* occ = nullptr;
+    int* cell = nullptr;
+    unsigned* uid = nullptr;
+};
+} // namespace
+
+struct pmc_ctx {
+    pmc_setup setup{};
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocations;
+    Carriers buf[2];
+    int cur = 0;
+    View view{};
+    bool has_material = false, has_geometry = false, has_tables = false, initialized = false;
+    bool latched = false;
+    pmc_error error{};
+    int64_t host_step = 0;
+    int sort_interval = 8;
+    int steps_since_sort = 0;
+    int n_reservoirs = 0;
+    double expected_injection = 0;
+    bool timing = false;
+    hipError_t hip_error = hipSuccess; // first failure of an unchecked-in-place HIP call
+    hipEvent_t ev[12] = {};
+    void note(hipError_t e) {
+        if (e != hipSuccess && hip_error == hipSuccess) hip_error = e;
+    }
+    double kernel_ms[5] = {0, 0, 0, 0, 0};
+
+    template <class T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        if (n == 0) n = 1;
+        if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) {
+            note(hipErrorOutOfMemory);
+            return nullptr;
+        }
+        allocations.push_back(p);
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    T* upload(const T* host, size_t n) {
+        T* d = alloc<T>(n);
+        if (d && host && n) note(hipMemcpy(d, host, n * sizeof(T), hipMemcpyHostToDevice));
+        return d;
+    }
+};
+
+namespace {
+int fail(pmc_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) {
+        ctx->error.code = code;
+        std::snprintf(ctx->error.message, sizeof(ctx->error.message), "%s", msg.c_str());
+    }
+    return code;
+}
+
+int check_hip(pmc_ctx* ctx, hipError_t e, const char* what) {
+    if (e == hipSuccess && ctx) e = ctx->hip_error;
+    if (e == hipSuccess) return PMC_OK;
+    return fail(ctx, PMC_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+View& view(pmc_ctx* ctx) {
+    View& v = ctx->view;
+    const Carriers& c = ctx->buf[ctx->cur];
+    v.px = c.px;
+    v.py = c.py;
+    v.pz = c.pz;
+    v.mode = c.mode;
+    v.occ = c.occ;
+    v.cell = c.cell;
+    v.uid = c.uid;
+    return v;
+}
+
+// Reads the device error word and latches the context with the reference's message.
+int collect_error(pmc_ctx* ctx) {
+    DevState st{};
+    hipError_t e = hipMemcpyAsync(&st, ctx->view.st, sizeof(DevState), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        ctx->latched = true;
+        return check_hip(ctx, e, "step");
+    }
+    if (st.error_code == 0) return PMC_OK;
+    ctx->latched = true;
+    ctx->error.code = st.error_code;
+    ctx->error.carrier = st.error_slot;
+    std::memcpy(ctx->error.position, st.error_pos, sizeof(st.error_pos));
+    ctx->error.facet = st.error_facet;
+    char msg[256];
+    const long long i = st.error_slot;
+    const double x = st.error_pos[0], y = st.error_pos[1], z = st.error_pos[2];
+    switch (st.error_code) {
+    case PMC_ERR_EXCESS_EVENTS:
+        std::snprintf(msg, sizeof msg,
+                      "Particle %lld exceeded 64 transport events in one timestep at (%f, %f, %f). Reduce "
+                      "simulation.time_step or inspect narrow STL features; no recovery was applied.",
+                      i, x, y, z);
+        break;
+    case PMC_ERR_ESCAPED:
+        std::snprintf(msg, sizeof msg,
+                      "Particle %lld escaped the domain at (%f, %f, %f). Refusing to relocate or rethermalize it; "
+                      "inspect the geometry and time step.",
+                      i, x, y, z);
+        break;
+    case PMC_ERR_INVALID_STATE:
+        if (st.error_facet == -2)
+            std::snprintf(msg, sizeof msg, "RTA pseudo-temperature of cell %lld did not converge within the pass budget.", i);
+        else
+            std::snprintf(msg, sizeof msg, "Invalid particle in grid energy statistics.");
+        break;
+    case PMC_ERR_CAPACITY:
+        std::snprintf(msg, sizeof msg, "Carrier slot capacity exhausted while injecting into slot %lld.", i);
+        break;
+    case PMC_ERR_UNSUPPORTED:
+        std::snprintf(msg, sizeof msg, "Rough facet without a reflection table reached by particle %lld.", i);
+        break;
+    default:
+        std::snprintf(msg, sizeof msg, "Boundary intersection failed for particle %lld at (%f, %f, %f).", i, x, y, z);
+    }
+    std::snprintf(ctx->error.message, sizeof(ctx->error.message), "%s", msg);
+    return st.error_code;
+}
+
+int sort_carriers(pmc_ctx* ctx) {
+    View& v = view(ctx);
+    Carriers& d = ctx->buf[1 - ctx->cur];
+    pmc_launch::sort_by_cell(v, ctx->stream, d.px, d.py, d.pz, d.mode, d.occ, d.cell, d.uid);
+    ctx->cur = 1 - ctx->cur;
+    ctx->steps_since_sort = 0;
+    return check_hip(ctx, hipGetLastError(), "sort");
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+    return ms;
+}
+} // namespace
+
+extern "C" {
+
+const char* pmc_status_string(int32_t code) {
+    switch (code) {
+    case PMC_OK: return "ok";
+    case PMC_ERR_ARGUMENT: return "invalid argument";
+    case PMC_ERR_DEVICE: return "device error";
+    case PMC_ERR_EXCESS_EVENTS: return "excess transport events";
+    case PMC_ERR_BOUNDARY: return "boundary failure";
+    case PMC_ERR_ESCAPED: return "escaped carrier";
+    case PMC_ERR_INVALID_STATE: return "invalid state";
+    case PMC_ERR_CAPACITY: return "capacity exhausted";
+    case PMC_ERR_LATCHED: return "context latched after failure";
+    case PMC_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown";
+    }
+}
+
+int pmc_create(const pmc_setup* setup, pmc_ctx** out) {
+    if (!setup || !out) return PMC_ERR_ARGUMENT;
+    *out = nullptr;
+    if (!(setup->time_step_ps > 0) || !std::isfinite(setup->time_step_ps) || setup->capacity <= 0 ||
+        setup->capacity > (1ll << 30) || !(setup->push_distance_a >= 0))
+        return PMC_ERR_ARGUMENT;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return PMC_ERR_DEVICE;
+    auto* ctx = new pmc_ctx();
+    ctx->setup = *setup;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return PMC_ERR_DEVICE;
+    }
+    for (auto& e : ctx->ev) ctx->note(hipEventCreate(&e));
+    View& v = ctx->view;
+    v.capacity = static_cast<int>(setup->capacity);
+    v.dt = setup->time_step_ps;
+    v.dtf = static_cast<float>(setup->time_step_ps);
+    v.push = setup->push_distance_a;
+    v.key0 = static_cast<uint32_t>(setup->seed & 0xffffffffu);
+    v.key1 = static_cast<uint32_t>(setup->seed >> 32) ^ (setup->instance * 0x9E3779B9u);
+    v.lifetime_local = setup->lifetime_local;
+    v.lifetime_T = setup->lifetime_temperature;
+    v.sampling = setup->mode_sampling;
+    v.inversion = setup->temperature_inversion;
+    for (auto& b : ctx->buf) {
+        const size_t n = static_cast<size_t>(setup->capacity);
+        b.px = ctx->alloc<float>(n);
+        b.py = ctx->alloc<float>(n);
+        b.pz = ctx->alloc<float>(n);
+        b.mode = ctx->alloc<int>(n);
+        b.occ = ctx->alloc<double>(n);
+        b.cell = ctx->alloc<int>(n);
+        b.uid = ctx->alloc<unsigned>(n);
+        if (!b.px || !b.py || !b.pz || !b.mode || !b.occ || !b.cell || !b.uid) {
+            pmc_destroy(ctx);
+            return PMC_ERR_DEVICE;
+        }
+    }
+    v.st = ctx->alloc<DevState>(1);
+    if (!v.st) {
+        pmc_destroy(ctx);
+        return PMC_ERR_DEVICE;
+    }
+    ctx->note(hipMemset(v.st, 0, sizeof(DevState)));
+    *out = ctx;
+    return PMC_OK;
+}
+
+int pmc_upload_material(pmc_ctx* ctx, const pmc_mode_table* t) {
+    if (!ctx || !t || t->n_active <= 0 || t->n_flat <= 0 || t->n_temps <= 0 || t->lut_n < 2)
+        return fail(ctx, PMC_ERR_ARGUMENT, "invalid mode table");
+    View& v = ctx->view;
+    v.na = t->n_active;
+    v.nflat = t->n_flat;
+    v.active_flat = ctx->upload(t->active_flat, t->n_active);
+    v.omega = ctx->upload(t->omega, t->n_active);
+    std::vector<float4> vel(t->n_active);
+    for (int a = 0; a < t->n_active; ++a) {
+        const double* p = t->velocity + 3 * a;
+        vel[a] = make_float4(static_cast<float>(p[0]), static_cast<float>(p[1]), static_cast<float>(p[2]),
+                             static_cast<float>(std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2])));
+    }
+    v.vel = ctx->upload(vel.data(), vel.size());
+    v.veld = ctx->upload(t->velocity, 3 * static_cast<size_t>(t->n_active));
+    v.neq = ctx->upload(t->neq_background, t->n_active);
+    v.ntemps = t->n_temps;
+    v.temps = ctx->upload(t->temps, t->n_temps);
+    v.gamma = ctx->upload(t->gamma_total, static_cast<size_t>(t->n_temps) * t->n_flat);
+    v.w = t->energy_weight;
+    v.e_bg = t->background_energy_density;
+    v.zp = t->zero_point_density;
+    v.lut_n = t->lut_n;
+    v.lut_t = ctx->upload(t->lut_t, t->lut_n);
+    v.lut_e = ctx->upload(t->lut_e, t->lut_n);
+    if (!v.active_flat || !v.omega || !v.vel || !v.veld || !v.neq || !v.temps || !v.gamma || !v.lut_t || !v.lut_e)
+        return fail(ctx, PMC_ERR_DEVICE, "material upload failed");
+    ctx->has_material = true;
+    return PMC_OK;
+}
+
+int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
+    if (!ctx || !g || g->n_facets <= 0 || g->n_facets > pmc_dev::kMaxFacetsDev || g->n_cells <= 0)
+        return fail(ctx, PMC_ERR_ARGUMENT, "invalid geometry");
+    if (g->kind != 0) return fail(ctx, PMC_ERR_UNSUPPORTED, "only the analytic box geometry is supported");
+    View& v = ctx->view;
+    for (int a = 0; a < 3; ++a) {
+        v.bmin[a] = g->bmin[a];
+        v.bmax[a] = g->bmax[a];
+        v.fbmin[a] = static_cast<float>(g->bmin[a]);
+        v.fbmax[a] = static_cast<float>(g->bmax[a]);
+        v.grid[a] = g->grid[a];
+        v.inv_cell[a] = g->inv_cell[a];
+    }
+    for (int k = 0; k < 6; ++k) {
+        if (g->box_plane_facet[k] < 0 || g->box_plane_facet[k] >= g->n_facets)
+            return fail(ctx, PMC_ERR_ARGUMENT, "box plane without facet");
+        v.plane_facet[k] = g->box_plane_facet[k];
+    }
+    std::vector<FacetDev> f(g->n_facets);
+    for (int i = 0; i < g->n_facets; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            f[i].n[a] = g->facets[i].normal[a];
+            f[i].shift[a] = g->facets[i].shift[a];
+        }
+        f[i].cond = g->facets[i].condition;
+        f[i].rough = g->facets[i].rough_table;
+        f[i].res = g->facets[i].reservoir;
+        f[i].pair = g->facets[i].periodic_pair;
+    }
+    v.nfacets = g->n_facets;
+    v.facets = ctx->upload(f.data(), f.size());
+    const size_t nvox = static_cast<size_t>(g->grid[0]) * g->grid[1] * g->grid[2];
+    v.voxel = ctx->upload(g->voxel_to_cell, nvox);
+    v.ncells = g->n_cells;
+    v.cell_vol = ctx->upload(g->cell_volume, g->n_cells);
+    const size_t nc = static_cast<size_t>(g->n_cells);
+    v.cell_e = ctx->alloc<double>(nc);
+    v.cell_tally = ctx->alloc<double>(pmc_dev::kTallyStride * nc);
+    v.partial_rows = pmc_launch::partial_rows(v.capacity);
+    v.partials = ctx->alloc<double>(static_cast<size_t>(v.partial_rows) * pmc_dev::kPartialCols);
+    v.cell_T = ctx->alloc<double>(nc);
+    v.cell_acc = ctx->alloc<double>(8 * nc);
+    v.cell_flux = ctx->alloc<double>(3 * nc);
+    v.cell_count = ctx->alloc<int>(nc);
+    v.cell_cursor = ctx->alloc<int>(nc);
+    v.cell_glo = ctx->alloc<int>(nc);
+    v.cell_ghi = ctx->alloc<int>(nc);
+    v.cell_gt = ctx->alloc<double>(nc);
+    v.newton = ctx->alloc<NewtonCell>(nc);
+    if (!v.facets || !v.voxel || !v.cell_vol || !v.cell_e || !v.cell_T || !v.cell_acc || !v.cell_tally || !v.partials ||
+        !v.cell_flux || !v.cell_count || !v.cell_cursor || !v.cell_glo || !v.cell_ghi || !v.cell_gt || !v.newton)
+        return fail(ctx, PMC_ERR_DEVICE, "geometry upload failed");
+    ctx->note(hipMemset(v.cell_acc, 0, 8 * nc * sizeof(double)));
+    ctx->note(hipMemset(v.cell_tally, 0, pmc_dev::kTallyStride * nc * sizeof(double)));
+    ctx->note(hipMemset(v.partials, 0, static_cast<size_t>(v.partial_rows) * pmc_dev::kPartialCols * sizeof(double)));
+    ctx->note(hipMemset(v.newton, 0, nc * sizeof(NewtonCell)));
+    ctx->has_geometry = true;
+    return PMC_OK;
+}
+
+int pmc_upload_boundary_tables(pmc_ctx* ctx, int32_t n_rough, const pmc_rough_table* rough, int32_t n_res,
+                               const pmc_reservoir* res) {
+    if (!ctx || !ctx->has_material || n_rough < 0 || n_res < 0 || n_res > pmc_dev::kMaxReservoirs)
+        return fail(ctx, PMC_ERR_ARGUMENT, "invalid boundary tables");
+    View& v = ctx->view;
+    const int na = v.na;
+    std::vector<RoughDev> rd(std::max(1, n_rough));
+    for (int r = 0; r < n_rough; ++r) {
+        const auto& t = rough[r];
+        rd[r].spec = ctx->upload(t.specularity, na);
+        rd[r].match = ctx->upload(t.spec_match, na);
+        rd[r].dbeg = ctx->upload(t.diffuse_begin, na);
+        rd[r].dend = ctx->upload(t.diffuse_end, na);
+        rd[r].n_out = t.n_outgoing;
+        rd[r].out_active = ctx->upload(t.outgoing_active, t.n_outgoing);
+        rd[r].out_sorted = ctx->upload(t.outgoing_sorted, t.n_outgoing);
+        rd[r].prefix = ctx->upload(t.residual_prefix, t.n_outgoing + 1);
+        rd[r].n_roul = t.n_roulette;
+        rd[r].roul_active = ctx->upload(t.roulette_active, t.n_roulette);
+        rd[r].roul_cdf = ctx->upload(t.roulette_cdf, t.n_roulette);
+        rd[r].strict = t.strict_elastic;
+    }
+    v.rough = ctx->upload(rd.data(), rd.size());
+    std::vector<ResDev> dd(std::max(1, n_res));
+    double expected = 0;
+    std::vector<int> initial(pmc_dev::kMaxReservoirs, 0);
+    for (int r = 0; r < n_res; ++r) {
+        const auto& t = res[r];
+        dd[r].facet = t.facet;
+        dd[r].temperature = t.temperature;
+        for (int a = 0; a < 3; ++a) dd[r].n[a] = t.normal[a];
+        dd[r].initial = t.initial_count;
+        dd[r].expected = t.expected_count;
+        dd[r].cdf = ctx->upload(t.cdf, na);
+        dd[r].aprob = ctx->upload(t.alias_prob, na);
+        dd[r].aidx = ctx->upload(t.alias_index, na);
+        dd[r].occ = ctx->upload(t.occupation, na);
+        dd[r].nfaces = t.n_faces;
+        dd[r].face_cdf = ctx->upload(t.face_cdf, t.n_faces);
+        dd[r].tri = ctx->upload(t.triangles, 9 * static_cast<size_t>(t.n_faces));
+        expected += t.expected_count;
+        initial[r] = t.initial_count;
+    }
+    v.res = ctx->upload(dd.data(), dd.size());
+    v.nres = n_res;
+    ctx->n_reservoirs = n_res;
+    ctx->expected_injection = expected;
+    // BoundaryLedger::configure: the first step refills the rounded expected counts.
+    ctx->note(hipMemcpy(reinterpret_cast<char*>(v.st) + offsetof(DevState, absorbed_now), initial.data(),
+              sizeof(int) * pmc_dev::kMaxReservoirs, hipMemcpyHostToDevice));
+    ctx->has_tables = true;
+    return PMC_OK;
+}
+
+static int finish_population(pmc_ctx* ctx, int64_t count) {
+    View& v = view(ctx);
+    DevState st{};
+    ctx->note(hipMemcpy(&st, v.st, sizeof(DevState), hipMemcpyDeviceToHost));
+    st.n_slots = static_cast<int>(count);
+    st.n_live = static_cast<int>(count);
+    st.n_slots_start = static_cast<int>(count);
+    st.next_uid = static_cast<unsigned>(count);
+    ctx->note(hipMemcpy(v.st, &st, sizeof(DevState), hipMemcpyHostToDevice));
+    int rc = sort_carriers(ctx);
+    if (rc) return rc;
+    View& w = view(ctx);
+    pmc_launch::clear_cells(w, ctx->stream);
+    pmc_launch::tally_all(w, ctx->stream);
+    pmc_launch::refresh(w, ctx->stream, 0);
+    rc = check_hip(ctx, hipGetLastError(), "initial refresh");
+    if (rc) return rc;
+    ctx->initialized = true;
+    return collect_error(ctx);
+}
+
+int pmc_initialize_carriers(pmc_ctx* ctx, const pmc_init* init) {
+    if (!ctx || !init || !ctx->has_material || !ctx->has_geometry || !ctx->has_tables)
+        return fail(ctx, PMC_ERR_ARGUMENT, "tables must be uploaded before initialization");
+    if (init->count <= 0 || init->count > ctx->setup.capacity) return fail(ctx, PMC_ERR_ARGUMENT, "invalid carrier count");
+    if (init->temperature_mode == 2 && !init->cell_temperature)
+        return fail(ctx, PMC_ERR_ARGUMENT, "per-cell initial temperature field missing");
+    View& v = view(ctx);
+    double* field = nullptr;
+    if (init->temperature_mode == 2) field = ctx->upload(init->cell_temperature, v.ncells);
+    pmc_launch::init(v, ctx->stream, static_cast<int>(init->count), init->temperature_mode, init->uniform_temperature,
+                     init->cold_point, init->hot_point, init->cold_temperature, init->hot_temperature, field);
+    int rc = check_hip(ctx, hipGetLastError(), "init");
+    if (rc) return rc;
+    return finish_population(ctx, init->count);
+}
+
+int pmc_seed_carriers(pmc_ctx* ctx, int64_t n, const float* position, const int32_t* mode, const double* occupation) {
+    if (!ctx || n <= 0 || n > ctx->setup.capacity || !position || !mode || !occupation || !ctx->has_geometry ||
+        !ctx->has_material || !ctx->has_tables)
+        return fail(ctx, PMC_ERR_ARGUMENT, "invalid carrier seed");
+    View& v = view(ctx);
+    std::vector<float> x(n), y(n), z(n);
+    std::vector<int> cell(n);
+    std::vector<unsigned> uid(n);
+    for (int64_t i = 0; i < n; ++i) {
+        x[i] = position[3 * i];
+        y[i] = position[3 * i + 1];
+        z[i] = position[3 * i + 2];
+        if ((mode[i] & pmc_dev::kModeMask) >= v.na || mode[i] < 0) return fail(ctx, PMC_ERR_ARGUMENT, "mode out of range");
+        uid[i] = static_cast<unsigned>(i);
+        // Cell index as the device computes it (FP64 on the FP32 position).
+        int idx[3];
+        const double p[3] = {x[i], y[i], z[i]};
+        for (int a = 0; a < 3; ++a) {
+            const int nn = v.grid[a];
+            if (nn <= 1 || !(v.bmax[a] - v.bmin[a] > 0.0)) idx[a] = 0;
+            else {
+                int k = static_cast<int>(std::floor((p[a] - v.bmin[a]) * v.inv_cell[a]));
+                idx[a] = std::min(std::max(k, 0), nn - 1);
+            }
+        }
+        cell[i] = (idx[0] * v.grid[1] + idx[1]) * v.grid[2] + idx[2];
+    }
+    std::vector<int> vox(static_cast<size_t>(v.grid[0]) * v.grid[1] * v.grid[2]);
+    ctx->note(hipMemcpy(vox.data(), v.voxel, vox.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (auto& c : cell) c = vox[c];
+    ctx->note(hipMemcpy(v.px, x.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    ctx->note(hipMemcpy(v.py, y.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    ctx->note(hipMemcpy(v.pz, z.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    ctx->note(hipMemcpy(v.mode, mode, n * sizeof(int), hipMemcpyHostToDevice));
+    ctx->note(hipMemcpy(v.occ, occupation, n * sizeof(double), hipMemcpyHostToDevice));
+    ctx->note(hipMemcpy(v.cell, cell.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    ctx->note(hipMemcpy(v.uid, uid.data(), n * sizeof(unsigned), hipMemcpyHostToDevice));
+    return finish_population(ctx, n);
+}
+
+int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
+    if (!ctx) return PMC_ERR_ARGUMENT;
+    if (ctx->latched) return PMC_ERR_LATCHED; // the first failure's message stays in ctx->error
+    if (!ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "carriers not initialized");
+    if (nsteps < 0) return fail(ctx, PMC_ERR_ARGUMENT, "negative step count");
+    for (int k = 0; k < 5; ++k) ctx->kernel_ms[k] = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        View& v = view(ctx);
+        const bool t = ctx->timing;
+        hipStream_t st = ctx->stream;
+        if (t) ctx->note(hipEventRecord(ctx->ev[0], st));
+        pmc_launch::begin(v, st);
+        if (ctx->host_step % 100 == 0) pmc_launch::containment(v, st);
+        if (t) ctx->note(hipEventRecord(ctx->ev[1], st));
+        pmc_launch::transport(v, st);
+        if (t) ctx->note(hipEventRecord(ctx->ev[2], st));
+        pmc_launch::inject(v, st);
+        if (t) ctx->note(hipEventRecord(ctx->ev[3], st));
+        pmc_launch::refresh(v, st, 1);
+        for (int j = 0; j < pmc_dev::kNewtonSlots; ++j) {
+            pmc_launch::rta_pass(v, st, j == 0 ? 1 : 0);
+            pmc_launch::rta_update(v, st);
+        }
+        if (t) ctx->note(hipEventRecord(ctx->ev[4], st));
+        pmc_launch::relax(v, st);
+        if (t) ctx->note(hipEventRecord(ctx->ev[5], st));
+        pmc_launch::refresh(v, st, 2);
+        if (t) ctx->note(hipEventRecord(ctx->ev[6], st));
+        int rc = check_hip(ctx, hipGetLastError(), "step launch");
+        if (rc) {
+            ctx->latched = true;
+            return rc;
+        }
+        ctx->host_step++;
+        ctx->steps_since_sort++;
+        if (t) {
+            ctx->note(hipEventSynchronize(ctx->ev[6]));
+            ctx->kernel_ms[0] += elapsed(ctx->ev[1], ctx->ev[2]);
+            ctx->kernel_ms[1] += elapsed(ctx->ev[2], ctx->ev[3]);
+            ctx->kernel_ms[2] += elapsed(ctx->ev[3], ctx->ev[4]);
+            ctx->kernel_ms[3] += elapsed(ctx->ev[4], ctx->ev[5]);
+            ctx->kernel_ms[4] += elapsed(ctx->ev[0], ctx->ev[6]);
+        }
+        if (ctx->steps_since_sort >= ctx->sort_interval || s == nsteps - 1) {
+            rc = collect_error(ctx);
+            if (rc) return rc;
+            if (ctx->steps_since_sort >= ctx->sort_interval) {
+                rc = sort_carriers(ctx);
+                if (rc) return rc;
+            }
+        }
+    }
+    return collect_error(ctx);
+}
+
+int pmc_read_cells(pmc_ctx* ctx, double* temperature, double* energy_density) {
+    if (!ctx || !ctx->has_geometry) return PMC_ERR_ARGUMENT;
+    const View& v = ctx->view;
+    ctx->note(hipStreamSynchronize(ctx->stream));
+    if (temperature) ctx->note(hipMemcpy(temperature, v.cell_T, v.ncells * sizeof(double), hipMemcpyDeviceToHost));
+    if (energy_density) ctx->note(hipMemcpy(energy_density, v.cell_e, v.ncells * sizeof(double), hipMemcpyDeviceToHost));
+    return check_hip(ctx, hipGetLastError(), "read cells");
+}
+
+int pmc_read_heat_flux(pmc_ctx* ctx, double* flux) {
+    if (!ctx || !ctx->initialized || !flux) return PMC_ERR_ARGUMENT;
+    View& v = view(ctx);
+    pmc_launch::flux(v, ctx->stream);
+    ctx->note(hipStreamSynchronize(ctx->stream));
+    ctx->note(hipMemcpy(flux, v.cell_flux, 3 * v.ncells * sizeof(double), hipMemcpyDeviceToHost));
+    return check_hip(ctx, hipGetLastError(), "heat flux");
+}
+
+int pmc_read_ledger(pmc_ctx* ctx, pmc_ledger* out) {
+    if (!ctx || !out) return PMC_ERR_ARGUMENT;
+    DevState st{};
+    ctx->note(hipStreamSynchronize(ctx->stream));
+    ctx->note(hipMemcpy(&st, ctx->view.st, sizeof(DevState), hipMemcpyDeviceToHost));
+    out->step = st.step;
+    out->live_carriers = st.n_live;
+    out->slots = st.n_slots;
+    out->step_absorbed = st.step_absorbed;
+    out->step_injected = st.step_injected;
+    out->step_absorbed_energy_ev = st.step_absorbed_e;
+    out->step_injected_energy_ev = st.step_injected_e;
+    out->step_lifetime_residual_ev = st.step_residual_e;
+    out->step_balance_residual_ev = st.step_balance_e;
+    out->total_absorbed = st.total_absorbed;
+    out->total_injected = st.total_injected;
+    out->total_absorbed_energy_ev = st.total_absorbed_e;
+    out->total_injected_energy_ev = st.total_injected_e;
+    out->total_lifetime_residual_ev = st.total_residual_e;
+    out->total_balance_residual_ev = st.total_balance_e;
+    out->thermal_energy_ev = st.thermal_e;
+    out->carrier_steps = st.carrier_steps;
+    out->rough_events = st.rough_events;
+    out->rough_specular = st.rough_specular;
+    out->rough_diffuse = st.rough_diffuse;
+    out->newton_passes = st.newton_passes;
+    return check_hip(ctx, hipGetLastError(), "read ledger");
+}
+
+int pmc_read_carriers(pmc_ctx* ctx, int64_t* n, float* position, int32_t* mode, double* occupation, int32_t* cell) {
+    if (!ctx || !n) return PMC_ERR_ARGUMENT;
+    View& v = view(ctx);
+    DevState st{};
+    ctx->note(hipStreamSynchronize(ctx->stream));
+    ctx->note(hipMemcpy(&st, v.st, sizeof(DevState), hipMemcpyDeviceToHost));
+    const int64_t count = st.n_slots;
+    if (position) {
+        std::vector<float> x(count), y(count), z(count);
+        ctx->note(hipMemcpy(x.data(), v.px, count * sizeof(float), hipMemcpyDeviceToHost));
+        ctx->note(hipMemcpy(y.data(), v.py, count * sizeof(float), hipMemcpyDeviceToHost));
+        ctx->note(hipMemcpy(z.data(), v.pz, count * sizeof(float), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < count; ++i) {
+            position[3 * i] = x[i];
+            position[3 * i + 1] = y[i];
+            position[3 * i + 2] = z[i];
+        }
+    }
+    if (mode) ctx->note(hipMemcpy(mode, v.mode, count * sizeof(int), hipMemcpyDeviceToHost));
+    if (occupation) ctx->note(hipMemcpy(occupation, v.occ, count * sizeof(double), hipMemcpyDeviceToHost));
+    if (cell) ctx->note(hipMemcpy(cell, v.cell, count * sizeof(int), hipMemcpyDeviceToHost));
+    *n = count;
+    return check_hip(ctx, hipGetLastError(), "read carriers");
+}
+
+int pmc_read_uids(pmc_ctx* ctx, int64_t n, uint32_t* uid) {
+    if (!ctx || !uid) return PMC_ERR_ARGUMENT;
+    View& v = view(ctx);
+    ctx->note(hipStreamSynchronize(ctx->stream));
+    ctx->note(hipMemcpy(uid, v.uid, n * sizeof(unsigned), hipMemcpyDeviceToHost));
+    return check_hip(ctx, hipGetLastError(), "read uids");
+}
+
+int pmc_last_error(pmc_ctx* ctx, pmc_error* error) {
+    if (!ctx || !error) return PMC_ERR_ARGUMENT;
+    *error = ctx->error;
+    return PMC_OK;
+}
+
+int pmc_kernel_times(pmc_ctx* ctx, double* ms, int32_t n) {
+    if (!ctx || !ms) return PMC_ERR_ARGUMENT;
+    for (int i = 0; i < n && i < 5; ++i) ms[i] = ctx->kernel_ms[i];
+    return PMC_OK;
+}
+
+int pmc_set_timing(pmc_ctx* ctx, int32_t enabled) {
+    if (!ctx) return PMC_ERR_ARGUMENT;
+    ctx->timing = enabled != 0;
+    return PMC_OK;
+}
+
+void pmc_destroy(pmc_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : ctx->allocations) (void)hipFree(p);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+} // extern "C"

@@ -1,0 +1,179 @@
+// Device-side data layout of one Monte Carlo context (internal header shared by
+// the HIP translation units).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pmc_dev {
+
+constexpr double kHbar = 6.582119569e-4;      // eV*ps
+constexpr double kBoltz = 8.617333262145e-5;  // eV/K
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kFluxUnit = 1.602176634e13;  // eV/(ps*A^2) -> W/m^2
+
+// Carrier mode word: bits 0..27 active-mode index, bits 28..30 velocity sign
+// flips (x,y,z) from mirror reflections, -1 marks an absorbed (dead) slot.
+constexpr int kModeMask = (1 << 28) - 1;
+constexpr int kFlipShift = 28;
+
+// Philox counter word 3 tags one random stream per use.
+enum Stream : uint32_t { kStreamTransport = 1, kStreamInject = 2, kStreamInit = 3, kStreamFlux = 4 };
+
+constexpr int kMaxReservoirs = 16;
+constexpr int kMaxFacetsDev = 64;
+constexpr int kNewtonSlots = 8;   // RTA pseudo-temperature passes per step (early exit when done)
+
+// Carrier passes: each wave walks kRowsPerWave rows of 64 consecutive slots.
+constexpr int kRowsPerWave = 16;
+constexpr int kWavesPerBlock = 4;
+constexpr int kSlotsPerBlock = 64 * kRowsPerWave * kWavesPerBlock;
+constexpr int kInjectBlocks = 64;
+
+// Per-block ledger partials (one row per block, summed by the refresh kernel).
+enum Partial : int {
+    kPAbsorbedE = 0,
+    kPInjectedE = 1,
+    kPAbsorbedN = 2,
+    kPInjectedN = 3,
+    kPRoughEvents = 4,
+    kPRoughSpecular = 5,
+    kPRoughDiffuse = 6,
+    kPAbsorbedRes = 8, // + reservoir index
+    kPartialCols = 8 + kMaxReservoirs
+};
+constexpr int kTallyStride = 4; // cell tally: energy, residual, count
+
+struct FacetDev {
+    double n[3];
+    double shift[3];
+    int cond, rough, res, pair;
+};
+
+struct RoughDev {
+    const double* spec;
+    const int* match;
+    const int* dbeg;
+    const int* dend;
+    int n_out;
+    const int* out_active;
+    const int* out_sorted;
+    const double* prefix;
+    int n_roul;
+    const int* roul_active;
+    const double* roul_cdf;
+    int strict;
+};
+
+struct ResDev {
+    int facet;
+    double temperature;
+    double n[3];
+    int initial;
+    double expected;
+    const double* cdf;
+    const double* aprob;
+    const int* aidx;
+    const double* occ;
+    int nfaces;
+    const double* face_cdf;
+    const double* tri;
+};
+
+// Scalars that live on the device across kernels of one step.
+struct DevState {
+    int n_slots;          // slots in use (live + dead)
+    int n_live;
+    int n_slots_start;    // slots before injection this step
+    int step;             // completed steps
+    unsigned next_uid;
+    int emit_total;
+    int emit_count[kMaxReservoirs];     // emissions this step per reservoir
+    int absorbed_now[kMaxReservoirs];   // absorptions this step per reservoir
+    long long step_absorbed, step_injected;
+    double step_absorbed_e, step_injected_e;
+    double step_residual_e, step_balance_e;
+    long long total_absorbed, total_injected;
+    double total_absorbed_e, total_injected_e, total_residual_e, total_balance_e;
+    double thermal_e;
+    long long carrier_steps;
+    long long rough_events, rough_specular, rough_diffuse;
+    long long newton_passes;
+    int newton_active;    // cells still iterating
+    int error_code;
+    long long error_slot;
+    float error_pos[3];
+    int error_facet;
+};
+
+// Per-cell RTA pseudo-temperature iteration state (src/solver/RtaCollisionOperator.cpp:118-165).
+struct NewtonCell {
+    double low, high, t, value, slope, s0, escale, fhigh;
+    int iter, expand, phase, done;
+};
+
+struct View {
+    // carriers (structure of arrays)
+    float* px;
+    float* py;
+    float* pz;
+    int* mode;
+    double* occ;
+    int* cell;
+    unsigned* uid;
+    int capacity;
+    // mode tables
+    int na, nflat;
+    const int* active_flat;
+    const double* omega;
+    const float4* vel;      // x,y,z,|v| FP32
+    const double* veld;     // FP64 velocity, 3 per mode
+    const double* neq;
+    int ntemps;
+    const double* temps;
+    const double* gamma;
+    double w, e_bg, zp;
+    int lut_n;
+    const double* lut_t;
+    const double* lut_e;
+    // geometry
+    double bmin[3], bmax[3];
+    float fbmin[3], fbmax[3];
+    int plane_facet[6];
+    int nfacets;
+    const FacetDev* facets;
+    int grid[3];
+    double inv_cell[3];
+    const int* voxel;
+    int ncells;
+    const double* cell_vol;
+    // boundary tables
+    const RoughDev* rough;
+    const ResDev* res;
+    int nres;
+    // cells
+    double* cell_tally;   // kTallyStride per cell: energy, residual, count
+    double* partials;     // kPartialCols per block row
+    int partial_rows;     // transport blocks + injection blocks
+    double* cell_e;       // energy density of the latest refresh
+    double* cell_T;
+    double* cell_acc;     // RTA accumulators, 8 per cell
+    double* cell_flux;    // 3 per cell
+    int* cell_count;
+    int* cell_cursor;
+    int* cell_glo;        // gamma interpolation row below
+    int* cell_ghi;
+    double* cell_gt;
+    NewtonCell* newton;
+    DevState* st;
+    // controls
+    double dt;
+    float dtf;
+    double push;
+    uint32_t key0, key1;
+    int lifetime_local;
+    double lifetime_T;
+    int sampling;
+    int inversion;
+};
+
+} // namespace pmc_dev

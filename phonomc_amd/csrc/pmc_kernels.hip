@@ -1,0 +1,1190 @@
+// CDNA4 (gfx950) kernels of the deviational Monte Carlo time step.
+//
+// One lane per carrier slot.  Carriers are a structure of arrays in HBM
+// (FP32 position, packed mode word, FP64 occupation, cell id, Philox id) and are
+// kept roughly cell-sorted so that wave-segmented reductions issue about one
+// FP64 atomic per wave per tallied quantity.
+#include "pmc_device.hpp"
+#include "pmc_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace pmc_dev {
+
+// ---------------------------------------------------------------- random numbers
+// Philox4x32-10 (Salmon et al., SC'11), counter = (id, step, draw, stream).
+__device__ __forceinline__ uint4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                        uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(M0, c0), lo0 = M0 * c0;
+        const uint32_t hi1 = __umulhi(M1, c2), lo1 = M1 * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return make_uint4(c0, c1, c2, c3);
+}
+
+// 53-bit uniform in [0, 1) from two words (lo, hi).
+__device__ __forceinline__ double u53(uint32_t lo, uint32_t hi) {
+    const uint64_t x = ((static_cast<uint64_t>(hi) << 32) | lo) >> 11;
+    return static_cast<double>(x) * 0x1.0p-53;
+}
+
+// Uniforms of one transport event, drawn on demand in the order the reference
+// consumes them (specular test, diffuse window, roulette).
+struct EventUniforms {
+    uint32_t uid, step, ev, k0, k1;
+    int used;
+    double u[4];
+    __device__ double next() {
+        if ((used & 1) == 0) {
+            const uint4 w = philox(uid, step, 2 * ev + (used >> 1), kStreamTransport, k0, k1);
+            u[0] = u53(w.x, w.y);
+            u[1] = u53(w.z, w.w);
+        }
+        const double r = u[used & 1];
+        ++used;
+        return r;
+    }
+};
+
+// ---------------------------------------------------------------- physics helpers
+__device__ __forceinline__ double bose(double t, double w) {
+    // src/material/Thermodynamics.cpp:71-82.
+    if (t <= 0.0 || w <= 0.0) return 0.0;
+    const double x = kHbar * w / (kBoltz * t);
+    if (x > 700.0) return 0.0;
+    const double ex = exp(x);
+    return 1.0 / fmax(ex - 1.0, 1e-12);
+}
+
+__device__ __forceinline__ int grid_index(const View& V, double x, double y, double z) {
+    // src/SimulationDomain.cpp:92-126 in FP64 on the FP32 position.
+    const double p[3] = {x, y, z};
+    int idx[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int n = V.grid[a];
+        if (n <= 1 || !(V.bmax[a] - V.bmin[a] > 0.0)) {
+            idx[a] = 0;
+        } else {
+            int k = static_cast<int>(floor((p[a] - V.bmin[a]) * V.inv_cell[a]));
+            idx[a] = k < 0 ? 0 : (k >= n ? n - 1 : k);
+        }
+    }
+    return V.voxel[(idx[0] * V.grid[1] + idx[1]) * V.grid[2] + idx[2]];
+}
+
+__device__ __forceinline__ float4 carrier_velocity(const View& V, int mw) {
+    float4 v = V.vel[mw & kModeMask];
+    if (mw & (1 << kFlipShift)) v.x = -v.x;
+    if (mw & (2 << kFlipShift)) v.y = -v.y;
+    if (mw & (4 << kFlipShift)) v.z = -v.z;
+    return v;
+}
+
+__device__ __forceinline__ double lerp_gamma(const View& V, int cell, int flat) {
+    // src/material/Thermodynamics.cpp:108-133 with the interpolation row and
+    // weight precomputed per cell.
+    const int lo = V.cell_glo[cell], hi = V.cell_ghi[cell];
+    const double t = V.cell_gt[cell];
+    const double y0 = V.gamma[static_cast<size_t>(lo) * V.nflat + flat];
+    if (lo == hi) return y0;
+    const double y1 = V.gamma[static_cast<size_t>(hi) * V.nflat + flat];
+    return y0 * (1.0 - t) + y1 * t;
+}
+
+__device__ __forceinline__ double relax_weight(const View& V, int cell, int m) {
+    const double g = lerp_gamma(V, cell, V.active_flat[m]);
+    if (g == 0.0 || V.dt == 0.0) return 0.0;
+    return -expm1(-(g * V.dt) * (4.0 * kPi));
+}
+
+// ---------------------------------------------------------------- wave reductions
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Adds v[k] of every active lane to base[key*K + k], one atomic per distinct key
+// present in the wave.  All lanes of the wave must call it.
+template <int K, int STRIDE = K>
+__device__ __forceinline__ void seg_atomic_add(bool active, int key, const double (&v)[K], double* base) {
+    const int lane = __lane_id();
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const int lkey = __shfl(key, leader, 64);
+        const bool mine = ((pending >> lane) & 1ull) && key == lkey;
+        const uint64_t mask = __ballot(mine);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double s = wave_sum(mine ? v[k] : 0.0);
+            if (lane == leader) unsafeAtomicAdd(&base[static_cast<size_t>(lkey) * STRIDE + k], s);
+        }
+        pending &= ~mask;
+    }
+}
+
+__device__ __forceinline__ void seg_count(bool active, int key, int* base) {
+    const int lane = __lane_id();
+    uint64_t pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const int lkey = __shfl(key, leader, 64);
+        const bool mine = ((pending >> lane) & 1ull) && key == lkey;
+        const uint64_t mask = __ballot(mine);
+        if (lane == leader) atomicAdd(&base[lkey], __popcll(mask));
+        pending &= ~mask;
+    }
+}
+
+__device__ void raise_error(const View& V, int code, long long slot, const float* p, int facet) {
+    if (atomicCAS(&V.st->error_code, 0, code) == 0) {
+        V.st->error_slot = slot;
+        if (p) {
+            V.st->error_pos[0] = p[0];
+            V.st->error_pos[1] = p[1];
+            V.st->error_pos[2] = p[2];
+        }
+        V.st->error_facet = facet;
+    }
+}
+
+// Logical block index grouping consecutive carrier chunks on one XCD (blocks are
+// dispatched round-robin over the 8 XCDs of an MI355X).
+__device__ __forceinline__ int xcd_block() {
+    const int g = gridDim.x, b = blockIdx.x;
+    if ((g & 7) != 0) return b;
+    return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+__device__ __forceinline__ int carrier_slot() { return xcd_block() * blockDim.x + threadIdx.x; }
+
+// ---------------------------------------------------------------- boundary events
+// RoughBoundarySampler::sample (src/solver/RoughBoundarySampler.cpp:459-533)
+// with sample_diffuse (:373-456) and flux_weighted_window_index
+// (include/solver/BoundaryFluxSampling.h:64-88).  Returns the outgoing active mode
+// and occupation; kind = 1 specular, 2 diffuse, 0 unchanged.
+__device__ void rough_sample(const View& V, const RoughDev& R, int in_ai, double in_occ, EventUniforms& U,
+                             int& out_ai, double& out_occ, int& kind) {
+    out_ai = in_ai;
+    out_occ = in_occ;
+    kind = 0;
+    double p_spec = R.spec[in_ai];
+    p_spec = p_spec < 0.0 ? 0.0 : (p_spec > 1.0 ? 1.0 : p_spec);
+    const double w_in = V.omega[in_ai];
+    const double neq_in = V.neq[in_ai];
+    if (U.next() <= p_spec) {
+        const int o = R.match[in_ai];
+        if (o >= 0 && o < V.na) {
+            const double w_out = V.omega[o];
+            const double mapped = V.neq[o] + w_in * (in_occ - neq_in) / fmax(1e-30, w_out);
+            if (isfinite(mapped) && mapped >= 0.0) {
+                out_ai = o;
+                out_occ = mapped;
+            }
+            kind = 1;
+            return;
+        }
+    }
+    int o = -1;
+    const int begin = R.dbeg[in_ai], end = R.dend[in_ai];
+    bool done = false;
+    if (begin >= 0 && end > begin && end <= R.n_out) {
+        const double u = U.next();
+        const double fb = R.prefix[begin], fe = R.prefix[end];
+        const double wf = fe - fb;
+        if (wf > 0.0 && isfinite(wf)) {
+            const double uc = u < 0.0 ? 0.0 : (u > 0x1.fffffffffffffp-1 ? 0x1.fffffffffffffp-1 : u);
+            const double target = fb + uc * wf;
+            // upper_bound over prefix[begin+1 .. end]
+            int lo = begin + 1, hi = end + 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (R.prefix[mid] <= target) lo = mid + 1;
+                else hi = mid;
+            }
+            int pos = lo - 1;
+            pos = pos < begin ? begin : (pos > end - 1 ? end - 1 : pos);
+            o = R.out_sorted[pos];
+            done = true;
+        }
+    }
+    if (!done) {
+        if (R.strict) {
+            return; // no cross-frequency fallback: caller mirrors the incoming mode
+        }
+        if (R.n_roul > 0 && R.roul_cdf[R.n_roul - 1] > 0.0) {
+            const double r = U.next() * R.roul_cdf[R.n_roul - 1];
+            int lo = 0, hi = R.n_roul;
+            while (lo < hi) { // lower_bound
+                const int mid = (lo + hi) >> 1;
+                if (R.roul_cdf[mid] < r) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo >= R.n_roul) lo = R.n_roul - 1;
+            o = R.roul_active[lo];
+        } else if (R.n_out > 0) {
+            int k = static_cast<int>(U.next() * R.n_out);
+            o = R.out_active[k < R.n_out ? k : R.n_out - 1];
+        } else {
+            int k = static_cast<int>(U.next() * V.na);
+            o = k < V.na ? k : V.na - 1;
+        }
+    }
+    kind = 2;
+    const double w_out = V.omega[o];
+    const double mapped = V.neq[o] + w_in * (in_occ - neq_in) / fmax(1e-30, w_out);
+    if (isfinite(mapped) && mapped >= 0.0) {
+        out_ai = o;
+        out_occ = mapped;
+    }
+}
+
+struct Tally {
+    double absorbed_e;
+    int absorbed_res;  // reservoir index or -1
+    bool absorbed;
+    int rough_kind;    // last rough event kind seen (for counters)
+    int rough_events, rough_spec, rough_diff;
+};
+
+// Free flight for `remaining` ps with up to 64 wall events
+// (TransportOperator::advance_particle, src/solver/TransportOperator.cpp:405-466,
+// against the analytic box of SimulationDomain::trace_boundary_intersection,
+// src/SimulationDomain.cpp:342-450, and interact_boundary,
+// src/solver/BoundaryInteraction.cpp:53-162).  FP32 positions: a carrier on a
+// wall always moves away from it (planes are chosen by velocity sign), so no
+// push distance is needed; positions are clamped into the closed box.
+// Returns 0 alive, 1 absorbed, or a negative error.
+__device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned uid, float remaining, Tally& T,
+                       bool& mode_changed, bool& occ_changed) {
+    const uint32_t step = static_cast<uint32_t>(V.st->step);
+    EventUniforms U;
+    for (int ev = 0; ev < 64; ++ev) {
+        if (!(remaining > 1e-14f)) return 0;
+        const float4 v = carrier_velocity(V, mw);
+        const float vv[3] = {v.x, v.y, v.z};
+        float tbest = INFINITY;
+        int plane = -1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (vv[a] != 0.0f) {
+                const int side = vv[a] > 0.0f ? 1 : 0;
+                const float target = side ? V.fbmax[a] : V.fbmin[a];
+                float t = (target - p[a]) / vv[a];
+                t = t > 0.0f ? t : 0.0f;
+                if (t < tbest) {
+                    tbest = t;
+                    plane = 2 * a + side;
+                }
+            }
+        }
+        if (plane < 0 || tbest > remaining) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float x = fmaf(vv[a], remaining, p[a]);
+                p[a] = fminf(fmaxf(x, V.fbmin[a]), V.fbmax[a]);
+            }
+            return 0;
+        }
+        remaining -= tbest;
+        const int ha = plane >> 1;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float x = fmaf(vv[a], tbest, p[a]);
+            p[a] = fminf(fmaxf(x, V.fbmin[a]), V.fbmax[a]);
+        }
+        p[ha] = (plane & 1) ? V.fbmax[ha] : V.fbmin[ha];
+        const int facet = V.plane_facet[plane];
+        const FacetDev& F = V.facets[facet];
+        const int m = mw & kModeMask;
+        if (F.cond == 'T') {
+            T.absorbed = true;
+            T.absorbed_res = F.res;
+            T.absorbed_e = (kHbar * V.omega[m] * (occ - V.neq[m])) * V.w;
+            return 1;
+        }
+        if (F.cond == 'P' && F.pair >= 0) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float x = p[a] + static_cast<float>(F.shift[a]);
+                p[a] = fminf(fmaxf(x, V.fbmin[a]), V.fbmax[a]);
+            }
+            continue;
+        }
+        if (F.cond == 'R' && F.rough >= 0) {
+            if (ev == 0 || U.uid != uid) {
+                U.uid = uid;
+                U.step = step;
+                U.k0 = V.key0;
+                U.k1 = V.key1;
+            }
+            U.ev = static_cast<uint32_t>(ev);
+            U.used = 0;
+            int out_ai, kind;
+            double out_occ;
+            rough_sample(V, V.rough[F.rough], m, occ, U, out_ai, out_occ, kind);
+            T.rough_events++;
+            if (kind == 1) T.rough_spec++;
+            else if (kind == 2) T.rough_diff++;
+            // New velocity is the table velocity of the outgoing mode, mirrored if it
+            // points out of the facet.
+            int nw = out_ai;
+            const double* vo = V.veld + 3 * static_cast<size_t>(out_ai);
+            const double vn = vo[0] * F.n[0] + vo[1] * F.n[1] + vo[2] * F.n[2];
+            if (vn > 0.0) nw |= (1 << ha) << kFlipShift;
+            if (nw != mw) mode_changed = true;
+            if (out_occ != occ) occ_changed = true;
+            mw = nw;
+            occ = out_occ;
+            continue;
+        }
+        if (F.cond == 'R') return -PMC_ERR_UNSUPPORTED; // table-less rough facet
+        // Specular mirror (periodic facet without a partner).
+        mw ^= (1 << ha) << kFlipShift;
+        mode_changed = true;
+    }
+    return remaining > 1e-14f ? -PMC_ERR_EXCESS_EVENTS : 0;
+}
+
+// ---------------------------------------------------------------- carrier passes
+// Every carrier pass is wave-persistent: a wave walks kRowsPerWave rows of 64
+// consecutive (cell-sorted) slots, keeps lane-local running sums per cell and
+// flushes them with one segmented reduction + atomic per distinct cell only
+// when some lane's cell changes.  Rare scalar events (absorption, injection,
+// rough reflections) go to LDS and leave as one partial row per block.
+
+__device__ __forceinline__ int wave_global() {
+    return xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
+}
+
+template <int K, int STRIDE>
+struct SegAcc {
+    int key;
+    bool has;
+    double v[K];
+    __device__ void reset() {
+        has = false;
+        key = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = 0.0;
+    }
+    __device__ void flush(double* base) {
+        seg_atomic_add<K, STRIDE>(has, key, v, base);
+        reset();
+    }
+    // All lanes of the wave must call add() together.
+    __device__ void add(bool active, int k, const double (&x)[K], double* base) {
+        if (__ballot(active && has && k != key)) flush(base);
+        if (active) {
+            key = k;
+            has = true;
+#pragma unroll
+            for (int j = 0; j < K; ++j) v[j] += x[j];
+        }
+    }
+};
+
+struct BlockPartials {
+    double* s; // shared, kPartialCols
+    __device__ void zero() {
+        for (int c = threadIdx.x; c < kPartialCols; c += blockDim.x) s[c] = 0.0;
+        __syncthreads();
+    }
+    __device__ void add(int col, double x) { atomicAdd(&s[col], x); }
+    __device__ void write(const View& V, int row) {
+        __syncthreads();
+        for (int c = threadIdx.x; c < kPartialCols; c += blockDim.x) V.partials[row * kPartialCols + c] = s[c];
+    }
+};
+
+__device__ __forceinline__ void record_events(BlockPartials& P, const Tally& T) {
+    if (T.absorbed) {
+        P.add(kPAbsorbedE, T.absorbed_e);
+        P.add(kPAbsorbedN, 1.0);
+        if (T.absorbed_res >= 0) P.add(kPAbsorbedRes + T.absorbed_res, 1.0);
+    }
+    if (T.rough_events) {
+        P.add(kPRoughEvents, T.rough_events);
+        if (T.rough_spec) P.add(kPRoughSpecular, T.rough_spec);
+        if (T.rough_diff) P.add(kPRoughDiffuse, T.rough_diff);
+    }
+}
+
+__global__ void k_begin(View V) {
+    DevState* st = V.st;
+    const int t = threadIdx.x;
+    if (t < kMaxReservoirs) st->emit_count[t] = t < V.nres ? st->absorbed_now[t] : 0;
+    if (t == 0) {
+        st->n_slots_start = st->n_slots;
+        st->carrier_steps += st->n_live;
+        st->newton_active = 0;
+    }
+    __syncthreads();
+    if (t == 0) {
+        int total = 0;
+        for (int r = 0; r < V.nres; ++r) total += st->emit_count[r];
+        st->emit_total = total;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_transport(View V) {
+    __shared__ double sp[kPartialCols];
+    BlockPartials P{sp};
+    P.zero();
+    const int n = V.st->n_slots_start;
+    const int base = wave_global() * 64 * kRowsPerWave;
+    const int lane = __lane_id();
+    SegAcc<1, kTallyStride> acc;
+    acc.reset();
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const int row = base + r * 64;
+        if (row >= n) break;
+        const int i = row + lane;
+        int mw = i < n ? V.mode[i] : -1;
+        const bool alive0 = mw >= 0;
+        Tally T{};
+        T.absorbed_res = -1;
+        bool alive = false;
+        int cell = 0;
+        double e[1] = {0.0};
+        if (alive0) {
+            float p[3] = {V.px[i], V.py[i], V.pz[i]};
+            double occ = V.occ[i];
+            bool mc = false, oc = false;
+            const int rc = advance(V, p, mw, occ, V.uid[i], V.dtf, T, mc, oc);
+            if (rc < 0) {
+                raise_error(V, -rc, i, p, -1);
+            } else if (rc == 1) {
+                V.mode[i] = -1;
+            } else {
+                alive = true;
+                V.px[i] = p[0];
+                V.py[i] = p[1];
+                V.pz[i] = p[2];
+                if (mc) V.mode[i] = mw;
+                if (oc) V.occ[i] = occ;
+                cell = grid_index(V, p[0], p[1], p[2]);
+                V.cell[i] = cell;
+                const int m = mw & kModeMask;
+                e[0] = (kHbar * V.omega[m] * (occ - V.neq[m])) * V.w;
+                if (!isfinite(occ)) raise_error(V, PMC_ERR_INVALID_STATE, i, p, -1);
+            }
+            record_events(P, T);
+        }
+        acc.add(alive, cell, e, V.cell_tally);
+    }
+    acc.flush(V.cell_tally);
+    P.write(V, blockIdx.x);
+}
+
+// Reservoir injection (ReservoirInjector::inject, src/solver/ReservoirInjector.cpp:62-132)
+// followed by the remaining flight of each injected carrier
+// (TransportOperator::advance_injected, src/solver/TransportOperator.cpp:134-164).
+__global__ void __launch_bounds__(256) k_inject(View V, int row0) {
+    __shared__ double sp[kPartialCols];
+    BlockPartials P{sp};
+    P.zero();
+    DevState* st = V.st;
+    const int total = st->emit_total;
+    const int base_slot = st->n_slots_start;
+    const uint32_t step = static_cast<uint32_t>(st->step);
+    SegAcc<1, kTallyStride> acc;
+    acc.reset();
+    for (int k0 = blockIdx.x * blockDim.x; k0 < total; k0 += gridDim.x * blockDim.x) {
+        const int k = k0 + threadIdx.x;
+        const bool active = k < total;
+        Tally T{};
+        T.absorbed_res = -1;
+        bool alive = false;
+        int cell = 0;
+        double e[1] = {0.0};
+        const int slot = base_slot + k;
+        if (active && slot >= V.capacity) {
+            raise_error(V, PMC_ERR_CAPACITY, slot, nullptr, -1);
+        } else if (active) {
+            int r = 0, accn = 0;
+            while (r < V.nres - 1 && k >= accn + st->emit_count[r]) accn += st->emit_count[r++];
+            const ResDev& R = V.res[r];
+            const uint4 w0 = philox(static_cast<uint32_t>(k), step, 0, kStreamInject, V.key0, V.key1);
+            const uint4 w1 = philox(static_cast<uint32_t>(k), step, 1, kStreamInject, V.key0, V.key1);
+            const uint4 w2 = philox(static_cast<uint32_t>(k), step, 2, kStreamInject, V.key0, V.key1);
+            const double u_mode = u53(w0.x, w0.y), u_time = u53(w0.z, w0.w);
+            const double u_face = u53(w1.x, w1.y), r1 = u53(w1.z, w1.w), r2 = u53(w2.x, w2.y);
+            int m;
+            if (V.sampling == 1) { // inverse CDF, std::lower_bound
+                int lo = 0, hi = V.na;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (R.cdf[mid] < u_mode) lo = mid + 1;
+                    else hi = mid;
+                }
+                m = lo < V.na ? lo : V.na - 1;
+            } else { // Walker alias
+                const double x = u_mode * V.na;
+                int j = static_cast<int>(x);
+                j = j < V.na ? j : V.na - 1;
+                m = (x - j) < R.aprob[j] ? j : R.aidx[j];
+            }
+            const double t_emit = V.dt * u_time;
+            int f = 0; // std::discrete_distribution: lower_bound on the area CDF
+            while (f < R.nfaces - 1 && R.face_cdf[f] < u_face) ++f;
+            const double* tri = R.tri + 9 * f;
+            const double sr1 = sqrt(r1);
+            const double b0 = 1.0 - sr1, b1 = sr1 * (1.0 - r2), b2 = sr1 * r2;
+            double pt[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) pt[a] = b0 * tri[a] + b1 * tri[3 + a] + b2 * tri[6 + a];
+            const double* vd = V.veld + 3 * static_cast<size_t>(m);
+            double vx = vd[0], vy = vd[1], vz = vd[2];
+            int mw = m;
+            if (vx * R.n[0] + vy * R.n[1] + vz * R.n[2] > 0.0) {
+                vx = -vx;
+                vy = -vy;
+                vz = -vz;
+                mw |= 7 << kFlipShift;
+            }
+            const double speed = fmax(1e-12, sqrt(vx * vx + vy * vy + vz * vz));
+            const double s = V.push / speed;
+            float p[3] = {static_cast<float>(pt[0] + vx * s), static_cast<float>(pt[1] + vy * s),
+                          static_cast<float>(pt[2] + vz * s)};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p[a] = fminf(fmaxf(p[a], V.fbmin[a]), V.fbmax[a]);
+            double occ = R.occ[m];
+            P.add(kPInjectedE, (kHbar * V.omega[m] * (occ - V.neq[m])) * V.w);
+            P.add(kPInjectedN, 1.0);
+            const unsigned uid = st->next_uid + static_cast<unsigned>(k);
+            const double remain = V.dt - t_emit;
+            int rc = 0;
+            bool mc = false, oc = false;
+            if (remain > 1e-14) rc = advance(V, p, mw, occ, uid, static_cast<float>(remain), T, mc, oc);
+            V.uid[slot] = uid;
+            if (rc < 0) {
+                raise_error(V, -rc, slot, p, -1);
+                V.mode[slot] = -1;
+            } else if (rc == 1) {
+                V.mode[slot] = -1;
+            } else {
+                alive = true;
+                cell = grid_index(V, p[0], p[1], p[2]);
+                V.px[slot] = p[0];
+                V.py[slot] = p[1];
+                V.pz[slot] = p[2];
+                V.mode[slot] = mw;
+                V.occ[slot] = occ;
+                V.cell[slot] = cell;
+                const int mm = mw & kModeMask;
+                e[0] = (kHbar * V.omega[mm] * (occ - V.neq[mm])) * V.w;
+            }
+            record_events(P, T);
+        }
+        acc.add(alive, cell, e, V.cell_tally);
+    }
+    acc.flush(V.cell_tally);
+    P.write(V, row0 + blockIdx.x);
+}
+
+__device__ double lut_inverse(const View& V, double e) {
+    // PhononMaterial::temperature_from_energy_density: clamped linear interpolation.
+    const int n = V.lut_n;
+    if (e <= V.lut_e[0]) return V.lut_t[0];
+    if (e >= V.lut_e[n - 1]) return V.lut_t[n - 1];
+    int lo = 0, hi = n; // upper_bound
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (V.lut_e[mid] <= e) lo = mid + 1;
+        else hi = mid;
+    }
+    const int h = lo, l = lo - 1;
+    const double x0 = V.lut_e[l], x1 = V.lut_e[h], y0 = V.lut_t[l], y1 = V.lut_t[h];
+    if (fabs(x1 - x0) <= 1e-18) return y0;
+    const double t = (e - x0) / (x1 - x0);
+    return y0 * (1.0 - t) + y1 * t;
+}
+
+__device__ void gamma_row(const View& V, double temp, int& lo, int& hi, double& t) {
+    const int n = V.ntemps;
+    if (temp <= V.temps[0]) { lo = hi = 0; t = 0; return; }
+    if (temp >= V.temps[n - 1]) { lo = hi = n - 1; t = 0; return; }
+    int a = 0, b = n;
+    while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (V.temps[mid] <= temp) a = mid + 1;
+        else b = mid;
+    }
+    hi = a;
+    lo = a - 1;
+    const double x0 = V.temps[lo], x1 = V.temps[hi];
+    if (fabs(x1 - x0) <= 1e-18) { hi = lo; t = 0; return; }
+    t = (temp - x0) / (x1 - x0);
+}
+
+__device__ double block_sum(double x, double* red) {
+    red[threadIdx.x] = x;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// Cell energy density -> temperature (GridStatistics::update_energy_density and
+// refresh_temperatures, src/solver/GridStatistics.cpp:95-235).  Phase 1 also
+// reduces the transport/injection ledger rows (BoundaryLedger::finish_transport,
+// src/solver/BoundaryLedger.cpp:86-115) and sets up the RTA bracket of
+// src/solver/RtaCollisionOperator.cpp:118-133; phase 2 closes the energy ledger.
+__global__ void k_refresh(View V, int phase) {
+    DevState* st = V.st;
+    __shared__ double red[1024];
+    if (phase == 1) {
+        for (int c = 0; c < kPartialCols; ++c) {
+            double x = 0.0;
+            for (int r = threadIdx.x; r < V.partial_rows; r += blockDim.x) x += V.partials[r * kPartialCols + c];
+            const double s = block_sum(x, red);
+            if (threadIdx.x == 0) {
+                switch (c) {
+                case kPAbsorbedE: st->step_absorbed_e = s; break;
+                case kPInjectedE: st->step_injected_e = s; break;
+                case kPAbsorbedN: st->step_absorbed = static_cast<long long>(s); break;
+                case kPInjectedN: st->step_injected = static_cast<long long>(s); break;
+                case kPRoughEvents: st->rough_events += static_cast<long long>(s); break;
+                case kPRoughSpecular: st->rough_specular += static_cast<long long>(s); break;
+                case kPRoughDiffuse: st->rough_diffuse += static_cast<long long>(s); break;
+                default:
+                    if (c >= kPAbsorbedRes && c < kPAbsorbedRes + kMaxReservoirs)
+                        st->absorbed_now[c - kPAbsorbedRes] = static_cast<int>(s);
+                }
+            }
+        }
+    }
+    double local = 0.0, residual = 0.0;
+    for (int c = threadIdx.x; c < V.ncells; c += blockDim.x) {
+        double* tally = V.cell_tally + static_cast<size_t>(kTallyStride) * c;
+        const double e = tally[0] / V.cell_vol[c] + V.e_bg;
+        residual += tally[1];
+        tally[0] = 0.0;
+        tally[1] = 0.0;
+        tally[2] = 0.0;
+        V.cell_e[c] = e;
+        const double T = lut_inverse(V, e);
+        V.cell_T[c] = T;
+        local += (e - V.zp) * V.cell_vol[c];
+        if (phase == 1) {
+            const double tl = V.lifetime_local ? T : V.lifetime_T;
+            int lo, hi;
+            double t;
+            gamma_row(V, tl, lo, hi, t);
+            V.cell_glo[c] = lo;
+            V.cell_ghi[c] = hi;
+            V.cell_gt[c] = t;
+            NewtonCell nc{};
+            nc.low = 0.0;
+            nc.high = fmax(1000.0, 2.0 * fmax(1.0, T));
+            nc.t = T < nc.low ? nc.low : (T > nc.high ? nc.high : T);
+            V.newton[c] = nc;
+            for (int k = 0; k < 8; ++k) V.cell_acc[8 * c + k] = 0.0;
+        }
+    }
+    const double energy = block_sum(local, red);
+    const double res = block_sum(residual, red);
+    if (threadIdx.x == 0) {
+        if (phase == 0) st->thermal_e = energy;
+        if (phase == 1) {
+            st->newton_active = V.ncells;
+            st->n_live = st->n_live - static_cast<int>(st->step_absorbed) + static_cast<int>(st->step_injected);
+            const int grown = st->n_slots_start + st->emit_total;
+            st->n_slots = grown < V.capacity ? grown : V.capacity;
+            st->next_uid += static_cast<unsigned>(st->emit_total);
+        }
+        if (phase == 2) {
+            // EnergyLedger::finish_step (src/solver/EnergyLedger.cpp:58-90) and
+            // BoundaryLedger::commit_step (src/solver/BoundaryLedger.cpp:117-141).
+            st->step_residual_e = res;
+            const double balance = energy - st->thermal_e - 0.0 - 0.0 - st->step_injected_e + st->step_absorbed_e - res;
+            st->step_balance_e = balance;
+            st->total_absorbed += st->step_absorbed;
+            st->total_injected += st->step_injected;
+            st->total_absorbed_e += st->step_absorbed_e;
+            st->total_injected_e += st->step_injected_e;
+            st->total_residual_e += res;
+            st->total_balance_e += balance;
+            st->thermal_e = energy;
+            st->step += 1;
+        }
+    }
+}
+
+// One pass of the per-cell pseudo-temperature iteration: evaluates
+// f(T) = sum hw*a*(n - n_BE(T)) and f'(T) at each cell's current candidate
+// (the evaluate lambda of src/solver/RtaCollisionOperator.cpp:91-116).  The first
+// pass also accumulates f(0), f(high) and the energy scale.
+__global__ void __launch_bounds__(256) k_rta_pass(View V, int first) {
+    DevState* st = V.st;
+    if (st->newton_active == 0) return;
+    const int n = st->n_slots;
+    const int base = wave_global() * 64 * kRowsPerWave;
+    const int lane = __lane_id();
+    SegAcc<5, 8> acc5;
+    SegAcc<2, 8> acc2;
+    acc5.reset();
+    acc2.reset();
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const int row = base + r * 64;
+        if (row >= n) break;
+        const int i = row + lane;
+        const int mw = i < n ? V.mode[i] : -1;
+        bool active = mw >= 0;
+        int cell = 0;
+        double vals[5] = {0, 0, 0, 0, 0};
+        if (active) {
+            cell = V.cell[i];
+            const NewtonCell& nc = V.newton[cell];
+            active = !nc.done;
+            if (active) {
+                const int m = mw & kModeMask;
+                const double w = fmax(0.0, V.omega[m]);
+                const double hw = kHbar * w;
+                const double a = relax_weight(V, cell, m);
+                const double occ = V.occ[i];
+                if (hw > 0.0 && a > 0.0) {
+                    const double T = fmax(0.0, nc.t);
+                    const double neq = bose(T, w);
+                    vals[0] = hw * a * (occ - neq);
+                    if (T > 0.0) vals[1] = -(hw * a * (hw * neq * (neq + 1.0) / (kBoltz * T * T)));
+                    if (first) {
+                        vals[2] = hw * a * occ;                      // f(0)
+                        vals[3] = hw * a * (occ - bose(nc.high, w)); // f(high)
+                        vals[4] = hw * a * fmax(1.0, occ);           // energy scale
+                    }
+                } else {
+                    active = false;
+                }
+            }
+        }
+        if (first) {
+            acc5.add(active, cell, vals, V.cell_acc);
+        } else {
+            const double v2[2] = {vals[0], vals[1]};
+            acc2.add(active, cell, v2, V.cell_acc);
+        }
+    }
+    if (first) acc5.flush(V.cell_acc);
+    else acc2.flush(V.cell_acc);
+}
+
+// Per-cell bracket/Newton update: one step of the loop at
+// src/solver/RtaCollisionOperator.cpp:118-165 per pass.
+__global__ void k_rta_update(View V) {
+    DevState* st = V.st;
+    if (st->newton_active == 0) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= V.ncells) return;
+    NewtonCell nc = V.newton[c];
+    if (nc.done) return;
+    double* acc = V.cell_acc + 8 * static_cast<size_t>(c);
+    const double value = acc[0], slope = acc[1];
+    bool run_iter = false;
+    if (nc.phase == 0) {
+        nc.s0 = acc[2];
+        nc.fhigh = acc[3];
+        nc.escale = acc[4];
+        if (nc.fhigh > 0.0 && nc.expand < 20) {
+            nc.expand++;
+            nc.high *= 2.0;
+            nc.t = nc.high;
+            nc.phase = 2;
+        } else if (nc.s0 <= 0.0) {
+            nc.t = nc.low;
+            nc.done = 1;
+        } else if (nc.fhigh >= 0.0) {
+            nc.t = nc.high;
+            nc.done = 1;
+        } else {
+            nc.phase = 1;
+            run_iter = true; // value and slope at the clamped cell temperature
+        }
+    } else if (nc.phase == 2) {
+        nc.fhigh = value;
+        if (nc.fhigh > 0.0 && nc.expand < 20) {
+            nc.expand++;
+            nc.high *= 2.0;
+            nc.t = nc.high;
+        } else if (nc.s0 <= 0.0) {
+            nc.t = nc.low;
+            nc.done = 1;
+        } else if (nc.fhigh >= 0.0) {
+            nc.t = nc.high;
+            nc.done = 1;
+        } else {
+            const double T = V.cell_T[c];
+            nc.t = T < nc.low ? nc.low : (T > nc.high ? nc.high : T);
+            nc.phase = 1; // evaluated by the next pass
+        }
+    } else {
+        run_iter = true;
+    }
+    if (run_iter) {
+        if (fabs(value) <= 1e-13 * fmax(1.0, nc.escale)) {
+            nc.done = 1;
+        } else {
+            if (value > 0.0) nc.low = nc.t;
+            else nc.high = nc.t;
+            double cand = 0.5 * (nc.low + nc.high);
+            if (slope < 0.0 && isfinite(slope)) {
+                const double newton = nc.t - value / slope;
+                if (newton > nc.low && newton < nc.high && isfinite(newton)) cand = newton;
+            }
+            nc.t = cand;
+            nc.iter++;
+            if (nc.iter >= 40) nc.done = 1;
+        }
+    }
+    acc[0] = 0.0;
+    acc[1] = 0.0;
+    V.newton[c] = nc;
+}
+
+// Counts unfinished cells after an update.
+__global__ void k_rta_count(View V) {
+    DevState* st = V.st;
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    if (st->newton_active == 0) return;
+    for (int c = threadIdx.x; c < V.ncells; c += blockDim.x)
+        if (!V.newton[c].done) atomicAdd(&cnt, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->newton_passes += 1;
+        st->newton_active = cnt;
+    }
+}
+
+// Relaxation n <- max(0, n + a (n_BE(T_R) - n)) and residual
+// (src/solver/RtaCollisionOperator.cpp:167-183), fused with the energy tally of
+// the following temperature refresh.
+__global__ void __launch_bounds__(256) k_relax(View V) {
+    DevState* st = V.st;
+    const int n = st->n_slots;
+    const int base = wave_global() * 64 * kRowsPerWave;
+    const int lane = __lane_id();
+    SegAcc<3, kTallyStride> acc;
+    acc.reset();
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const int row = base + r * 64;
+        if (row >= n) break;
+        const int i = row + lane;
+        const int mw = i < n ? V.mode[i] : -1;
+        const bool alive = mw >= 0;
+        int cell = 0;
+        double vals[3] = {0.0, 0.0, 0.0};
+        if (alive) {
+            cell = V.cell[i];
+            const NewtonCell& nc = V.newton[cell];
+            if (!nc.done) raise_error(V, PMC_ERR_INVALID_STATE, cell, nullptr, -2);
+            const int m = mw & kModeMask;
+            const double w = fmax(0.0, V.omega[m]);
+            const double hw = kHbar * w;
+            const double a = relax_weight(V, cell, m);
+            const double old = V.occ[i];
+            const double neq = bose(nc.t, w);
+            const double nn = old + a * (neq - old);
+            vals[1] = (hw * nn - hw * old) * V.w;
+            const double occ = fmax(0.0, nn);
+            if (occ != old) V.occ[i] = occ;
+            vals[0] = (kHbar * V.omega[m] * (occ - V.neq[m])) * V.w;
+            vals[2] = 1.0;
+            if (!isfinite(occ)) raise_error(V, PMC_ERR_INVALID_STATE, i, nullptr, -1);
+        }
+        acc.add(alive, cell, vals, V.cell_tally);
+    }
+    acc.flush(V.cell_tally);
+}
+
+__global__ void k_zero_tally(View V) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < kTallyStride * V.ncells; c += gridDim.x * blockDim.x)
+        V.cell_tally[c] = 0.0;
+}
+
+__global__ void k_zero_counts(View V) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < V.ncells; c += gridDim.x * blockDim.x) V.cell_count[c] = 0;
+}
+
+__global__ void k_zero_flux(View V) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < 3 * V.ncells; c += gridDim.x * blockDim.x) V.cell_flux[c] = 0.0;
+}
+
+// Heat-flux tally (GridStatistics::update_heat_flux, src/solver/GridStatistics.cpp:269-347).
+__global__ void __launch_bounds__(256) k_flux(View V) {
+    const int n = V.st->n_slots;
+    const int base = wave_global() * 64 * kRowsPerWave;
+    const int lane = __lane_id();
+    SegAcc<3, 3> acc;
+    acc.reset();
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const int row = base + r * 64;
+        if (row >= n) break;
+        const int i = row + lane;
+        const int mw = i < n ? V.mode[i] : -1;
+        const bool alive = mw >= 0;
+        int cell = 0;
+        double q[3] = {0, 0, 0};
+        if (alive) {
+            cell = V.cell[i];
+            const int m = mw & kModeMask;
+            const double e = (kHbar * V.omega[m] * (V.occ[i] - V.neq[m])) * V.w;
+            const double* vd = V.veld + 3 * static_cast<size_t>(m);
+            q[0] = ((mw >> kFlipShift) & 1 ? -vd[0] : vd[0]) * e;
+            q[1] = ((mw >> kFlipShift) & 2 ? -vd[1] : vd[1]) * e;
+            q[2] = ((mw >> kFlipShift) & 4 ? -vd[2] : vd[2]) * e;
+        }
+        acc.add(alive, cell, q, V.cell_flux);
+    }
+    acc.flush(V.cell_flux);
+}
+
+__global__ void k_flux_finish(View V) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < V.ncells; c += gridDim.x * blockDim.x)
+        for (int a = 0; a < 3; ++a) V.cell_flux[3 * c + a] = V.cell_flux[3 * c + a] * (1.0 / V.cell_vol[c]) * kFluxUnit;
+}
+
+// Domain membership check (TransportOperator::validate_containment,
+// src/solver/TransportOperator.cpp:364-402).
+__global__ void __launch_bounds__(256) k_containment(View V) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = V.st->n_slots;
+    if (i >= n || V.mode[i] < 0) return;
+    const float p[3] = {V.px[i], V.py[i], V.pz[i]};
+    double scale = 1.0;
+    for (int a = 0; a < 3; ++a) scale = fmax(scale, fabs(V.bmax[a] - V.bmin[a]));
+    const double tol = 1e-10 * scale;
+    for (int a = 0; a < 3; ++a)
+        if (!isfinite(p[a]) || p[a] < V.bmin[a] - tol || p[a] > V.bmax[a] + tol) {
+            raise_error(V, PMC_ERR_ESCAPED, i, p, -1);
+            return;
+        }
+}
+
+// ---------------------------------------------------------------- initialization
+// Initial population (src/solver/Initialization.cpp:48-113 and :181-265):
+// uniform positions in the box, uniform active modes, Bose occupations at the
+// initial temperature of the carrier's position.
+__global__ void __launch_bounds__(256) k_init(View V, int count, int tmode, double t0, double c0, double c1, double c2,
+                                              double h0, double h1, double h2, double tcold, double thot,
+                                              const double* field) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint4 a = philox(static_cast<uint32_t>(i), 0, 0, kStreamInit, V.key0, V.key1);
+    const uint4 b = philox(static_cast<uint32_t>(i), 0, 1, kStreamInit, V.key0, V.key1);
+    const double u[4] = {u53(a.x, a.y), u53(a.z, a.w), u53(b.x, b.y), u53(b.z, b.w)};
+    double p[3];
+    for (int k = 0; k < 3; ++k) p[k] = V.bmin[k] + (V.bmax[k] - V.bmin[k]) * u[k];
+    float fp[3];
+    for (int k = 0; k < 3; ++k) {
+        fp[k] = static_cast<float>(p[k]);
+        fp[k] = fminf(fmaxf(fp[k], V.fbmin[k]), V.fbmax[k]);
+    }
+    int m = static_cast<int>(u[3] * V.na);
+    m = m < V.na ? m : V.na - 1;
+    double T = t0;
+    const int cell = grid_index(V, fp[0], fp[1], fp[2]);
+    if (tmode == 1) {
+        const double ax[3] = {h0 - c0, h1 - c1, h2 - c2};
+        const double rel[3] = {p[0] - c0, p[1] - c1, p[2] - c2};
+        const double len2 = ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2];
+        double s = (rel[0] * ax[0] + rel[1] * ax[1] + rel[2] * ax[2]) / len2;
+        s = s < 0.0 ? 0.0 : (s > 1.0 ? 1.0 : s);
+        T = tcold + s * (thot - tcold);
+    } else if (tmode == 2) {
+        T = field[cell];
+    }
+    V.px[i] = fp[0];
+    V.py[i] = fp[1];
+    V.pz[i] = fp[2];
+    V.mode[i] = m;
+    V.occ[i] = bose(T, V.omega[m]);
+    V.cell[i] = cell;
+    V.uid[i] = static_cast<unsigned>(i);
+}
+
+__global__ void __launch_bounds__(256) k_tally_all(View V) {
+    const int n = V.st->n_slots;
+    const int base = wave_global() * 64 * kRowsPerWave;
+    const int lane = __lane_id();
+    SegAcc<1, kTallyStride> acc;
+    acc.reset();
+    for (int r = 0; r < kRowsPerWave; ++r) {
+        const int row = base + r * 64;
+        if (row >= n) break;
+        const int i = row + lane;
+        const int mw = i < n ? V.mode[i] : -1;
+        const bool alive = mw >= 0;
+        int cell = 0;
+        double e[1] = {0.0};
+        if (alive) {
+            cell = V.cell[i];
+            const int m = mw & kModeMask;
+            e[0] = (kHbar * V.omega[m] * (V.occ[i] - V.neq[m])) * V.w;
+        }
+        acc.add(alive, cell, e, V.cell_tally);
+    }
+    acc.flush(V.cell_tally);
+}
+
+// ---------------------------------------------------------------- compaction
+__global__ void __launch_bounds__(256) k_histogram(View V) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = V.st->n_slots;
+    const int mw = i < n ? V.mode[i] : -1;
+    const bool alive = mw >= 0;
+    seg_count(alive, alive ? V.cell[i] : 0, V.cell_count);
+}
+
+__global__ void k_scan_cells(View V) {
+    // Exclusive scan of cell counts into cursors (single block).
+    __shared__ int partial[1024];
+    const int n = V.ncells;
+    const int per = (n + blockDim.x - 1) / blockDim.x;
+    const int begin = threadIdx.x * per;
+    int s = 0;
+    for (int c = begin; c < begin + per && c < n; ++c) s += V.cell_count[c];
+    partial[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < static_cast<int>(blockDim.x); ++t) {
+            const int x = partial[t];
+            partial[t] = acc;
+            acc += x;
+        }
+    }
+    __syncthreads();
+    int acc = partial[threadIdx.x];
+    for (int c = begin; c < begin + per && c < n; ++c) {
+        V.cell_cursor[c] = acc;
+        acc += V.cell_count[c];
+    }
+}
+
+struct Scatter {
+    float *px, *py, *pz;
+    int* mode;
+    double* occ;
+    int* cell;
+    unsigned* uid;
+};
+
+// Counting-sort scatter of live carriers into cell order (dst buffers).
+__global__ void __launch_bounds__(256) k_scatter(View V, Scatter D) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = V.st->n_slots;
+    const int mw = i < n ? V.mode[i] : -1;
+    const bool alive = mw >= 0;
+    const int cell = alive ? V.cell[i] : 0;
+    const int lane = __lane_id();
+    uint64_t pending = __ballot(alive);
+    int dst = -1;
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const int lkey = __shfl(cell, leader, 64);
+        const bool mine = ((pending >> lane) & 1ull) && cell == lkey;
+        const uint64_t mask = __ballot(mine);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&V.cell_cursor[lkey], __popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (mine) dst = base + __popcll(mask & ((1ull << lane) - 1ull));
+        pending &= ~mask;
+    }
+    if (alive) {
+        D.px[dst] = V.px[i];
+        D.py[dst] = V.py[i];
+        D.pz[dst] = V.pz[i];
+        D.mode[dst] = mw;
+        D.occ[dst] = V.occ[i];
+        D.cell[dst] = cell;
+        D.uid[dst] = V.uid[i];
+    }
+}
+
+__global__ void k_after_sort(View V) {
+    if (threadIdx.x == 0) V.st->n_slots = V.st->n_live;
+}
+
+} // namespace pmc_dev
+
+// ---------------------------------------------------------------- launchers
+namespace pmc_launch {
+using namespace pmc_dev;
+
+int wave_blocks(int capacity) {
+    const int b = (capacity + kSlotsPerBlock - 1) / kSlotsPerBlock;
+    return (b + 7) & ~7;
+}
+
+int partial_rows(int capacity) { return wave_blocks(capacity) + kInjectBlocks; }
+
+static int slot_blocks(int capacity) { return (capacity + 255) / 256; }
+static int cell_blocks(int ncells) { return (ncells + 255) / 256; }
+static const dim3 kWaveBlock(64 * kWavesPerBlock);
+
+void begin(const View& v, hipStream_t s) { hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, s, v); }
+void transport(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_transport, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+}
+void inject(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_inject, dim3(kInjectBlocks), dim3(256), 0, s, v, wave_blocks(v.capacity));
+}
+void refresh(const View& v, hipStream_t s, int phase) { hipLaunchKernelGGL(k_refresh, dim3(1), dim3(1024), 0, s, v, phase); }
+void rta_pass(const View& v, hipStream_t s, int first) {
+    hipLaunchKernelGGL(k_rta_pass, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v, first);
+}
+void rta_update(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_rta_update, dim3(cell_blocks(v.ncells)), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_rta_count, dim3(1), dim3(1024), 0, s, v);
+}
+void relax(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_relax, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+}
+void clear_cells(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_tally, dim3(cell_blocks(kTallyStride * v.ncells)), dim3(256), 0, s, v);
+}
+void flux(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_flux, dim3(cell_blocks(3 * v.ncells)), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_flux, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+    hipLaunchKernelGGL(k_flux_finish, dim3(cell_blocks(v.ncells)), dim3(256), 0, s, v);
+}
+void containment(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_containment, dim3(slot_blocks(v.capacity)), dim3(256), 0, s, v);
+}
+void init(const View& v, hipStream_t s, int count, int tmode, double t0, const double* cold, const double* hot,
+          double tcold, double thot, const double* field) {
+    hipLaunchKernelGGL(k_init, dim3((count + 255) / 256), dim3(256), 0, s, v, count, tmode, t0, cold[0], cold[1],
+                       cold[2], hot[0], hot[1], hot[2], tcold, thot, field);
+}
+void tally_all(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_tally_all, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+}
+void sort_by_cell(const View& v, hipStream_t s, float* px, float* py, float* pz, int* mode, double* occ, int* cell,
+                  unsigned* uid) {
+    hipLaunchKernelGGL(k_zero_counts, dim3(cell_blocks(v.ncells)), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_histogram, dim3(slot_blocks(v.capacity)), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, v);
+    Scatter d{px, py, pz, mode, occ, cell, uid};
+    hipLaunchKernelGGL(k_scatter, dim3(slot_blocks(v.capacity)), dim3(256), 0, s, v, d);
+    hipLaunchKernelGGL(k_after_sort, dim3(1), dim3(64), 0, s, v);
+}
+} // namespace pmc_launch

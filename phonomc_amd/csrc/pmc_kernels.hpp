@@ -1,0 +1,24 @@
+// Launchers for the step kernels (defined in pmc_kernels.hip).
+#pragma once
+#include "pmc_capi.h"
+#include "pmc_device.hpp"
+
+namespace pmc_launch {
+int wave_blocks(int capacity);
+int partial_rows(int capacity);
+void begin(const pmc_dev::View& v, hipStream_t s);
+void transport(const pmc_dev::View& v, hipStream_t s);
+void inject(const pmc_dev::View& v, hipStream_t s);
+void refresh(const pmc_dev::View& v, hipStream_t s, int phase);
+void rta_pass(const pmc_dev::View& v, hipStream_t s, int first);
+void rta_update(const pmc_dev::View& v, hipStream_t s);
+void relax(const pmc_dev::View& v, hipStream_t s);
+void clear_cells(const pmc_dev::View& v, hipStream_t s);
+void flux(const pmc_dev::View& v, hipStream_t s);
+void containment(const pmc_dev::View& v, hipStream_t s);
+void init(const pmc_dev::View& v, hipStream_t s, int count, int tmode, double t0, const double* cold,
+          const double* hot, double tcold, double thot, const double* field);
+void tally_all(const pmc_dev::View& v, hipStream_t s);
+void sort_by_cell(const pmc_dev::View& v, hipStream_t s, float* px, float* py, float* pz, int* mode, double* occ,
+                  int* cell, unsigned* uid);
+} // namespace pmc_launch
