@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Throughput of the deviational Monte Carlo time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Workload: finite in-plane film, 1000 nm (x) by 100 nm (z), periodic in y,
+50x10 cells, 1 nm rough top/bottom walls, reservoirs 302/298 K, local
+lifetimes, 1e7 carriers, dt = 0.5 ps, synthetic Si-like 16^3 x 6 table.
+A step is one call of the time step (transport, injection, temperature
+refresh, RTA collisions, refresh, ledgers) over all carriers; the metric is
+particle-steps/s = sum over timed steps of live carriers at step start / wall
+time of the timed step loop, with all state resident in HBM.
+
+Prints one JSON line.  The roofline object describes the kernel with the
+largest share of step time (HIP-event durations on the step's stream); the
+cpu_baseline object times the reference PhonoMC solver compiled from its own
+sources (oracle/_ref) on a bounded sample of the same workload on the host
+cores of this machine.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOAD = "c2_inplane_film.toml"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per carrier of each kernel class (reads + writes of the
+# carrier arrays the kernel must touch; tables and cell tallies excluded).
+BYTES_PER_CARRIER = {
+    "transport": 24 + 16,  # read pos(12) mode(4) occ(8); write pos(12) cell(4)
+    "rta_first": 16,       # read mode(4) cell(4) occ(8)
+    "rta_pass": 16,
+    "relax": 16 + 8,       # read mode, cell, occ; write occ
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(problem_text, steps):
+    """Times the reference solver (oracle/_ref) on the full workload for `steps` steps."""
+    from oracle import ref_ctypes
+    from phonomc_amd import runtime as rt
+
+    cores = host_cores()
+    tmp = tempfile.mkdtemp(prefix="pmc_ref_")
+    text = problem_text.replace('output_folder = "results/c2_inplane_film"', f'output_folder = "{tmp}/out"')
+    path = os.path.join(tmp, "c2.toml")
+    with open(path, "w") as f:
+        f.write(text)
+    problem = rt.Problem(text)
+    if ref_ctypes.available():
+        secs, ps = ref_ctypes.run(path, problem.material(), steps, threads=cores)
+        kind = "reference"
+    else:  # restatement when the reference build is absent
+        from oracle import oracle_ctypes as O
+        sim = O.Sim(problem, threads=cores)
+        sim.initialize()
+        t0 = time.perf_counter()
+        before = sim.ledger()["carrier_steps"]
+        sim.step(steps)
+        secs = time.perf_counter() - t0
+        ps = sim.ledger()["carrier_steps"] - before
+        kind = "port"
+    return {"value": ps / secs, "unit": "particle-steps/s", "cores": cores, "kind": kind,
+            "sample": f"{WORKLOAD} at 1e7 carriers, {steps} steps of run_timestep after initialization, "
+                      f"OpenMP threads={cores}", "seconds": secs}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--timing-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (multi-GPU sharding is "
+              f"a design note, not part of this build).", file=sys.stderr)
+        return 2
+
+    from phonomc_amd import runtime as rt
+
+    text = rt.config_text(WORKLOAD)
+    problem = rt.Problem(text)
+    ctx = rt.Context(problem)
+    n0 = ctx.ledger()["live_carriers"]
+    log(f"[bench] {WORKLOAD}: {n0} carriers initialized; warmup {args.warmup} steps")
+    ctx.step(args.warmup)
+    led0 = ctx.ledger()  # synchronizes the device
+    t0 = time.perf_counter()
+    ctx.step(args.steps)  # returns after the device finished the last step
+    t1 = time.perf_counter()
+    led1 = ctx.ledger()
+    elapsed = t1 - t0
+    carrier_steps = led1["carrier_steps"] - led0["carrier_steps"]
+    value = carrier_steps / elapsed
+    log(f"[bench] {args.steps} steps in {elapsed:.4f} s -> {value:.4e} particle-steps/s")
+
+    # Per-kernel durations with HIP events on the context stream.
+    ctx.set_timing(True)
+    ctx.reset_kernel_times()
+    ctx.step(args.timing_steps)
+    times = ctx.kernel_times()
+    ctx.set_timing(False)
+    live = ctx.ledger()["live_carriers"]
+    candidates = {k: v for k, v in times.items() if k in BYTES_PER_CARRIER and v[1] > 0}
+    dominant = max(candidates, key=lambda k: candidates[k][0])
+    total_ms, launches = times[dominant]
+    avg_s = total_ms / launches / 1e3
+    traffic_bytes = BYTES_PER_CARRIER[dominant] * live
+    achieved = traffic_bytes / avg_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dominant}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dominant,
+                "avg_launch_ms": total_ms / launches,
+                "step_breakdown_ms": {k: v[0] / max(1, times["step"][1]) for k, v in times.items() if v[1]}}
+    log(f"[bench] kernel times (ms/step): {json.dumps(roofline['step_breakdown_ms'])}")
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        log(f"[bench] CPU baseline: reference solver, {args.cpu_steps} steps")
+        baseline = cpu_baseline(text, args.cpu_steps)
+        log(f"[bench] CPU baseline {baseline['value']:.4e} particle-steps/s on {baseline['cores']} threads")
+
+    out = {
+        "metric": "particle-steps/s @1 GPU",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 flight / f64 tallies",
+        "data": "synthetic (seeded Si-like 16^3x6 phonon table, Philox-sampled carriers)",
+        "config": {"workload": "in-plane film 1000x100 nm, 50x10 cells, rough 1 nm walls, 302/298 K reservoirs, "
+                               "local lifetimes, 1e7 carriers, dt 0.5 ps",
+                   "carriers": n0, "cells": int(problem.scalars()["cells"]), "time_step_ps": 0.5},
+        "roofline": roofline,
+        "cpu_baseline": baseline,
+    }
+    if baseline:
+        out["speedup_vs_cpu"] = value / baseline["value"]
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -193,9 +193,24 @@ int pmc_read_carriers(pmc_ctx* ctx, int64_t* n, float* position, int32_t* mode, 
 /* Philox stream ids of slots [0, n) (initial carriers 0..N-1, injected ones numbered on). */
 int pmc_read_uids(pmc_ctx* ctx, int64_t n, uint32_t* uid);
 int pmc_last_error(pmc_ctx* ctx, pmc_error* error);
-/* Kernel durations of the last pmc_step call, measured with HIP events on the
- * context's stream: [0] transport, [1] injection, [2] RTA passes, [3] relax, [4] whole steps (ms). */
-int pmc_kernel_times(pmc_ctx* ctx, double* ms, int32_t n);
+/* Kernel classes of the per-kernel timers. */
+enum pmc_kernel_class {
+    PMC_KERNEL_BEGIN = 0,      /* step set-up (+ containment check every 100 steps) */
+    PMC_KERNEL_TRANSPORT = 1,  /* free flight, boundary events, energy tally */
+    PMC_KERNEL_INJECT = 2,     /* reservoir injection and remaining flight */
+    PMC_KERNEL_REFRESH = 3,    /* cell temperatures and ledgers */
+    PMC_KERNEL_RTA_FIRST = 4,  /* first pseudo-temperature pass (bracket + Newton) */
+    PMC_KERNEL_RTA_PASS = 5,   /* later pseudo-temperature passes */
+    PMC_KERNEL_RTA_UPDATE = 6, /* per-cell Newton updates */
+    PMC_KERNEL_RELAX = 7,      /* relaxation + energy tally */
+    PMC_KERNEL_SORT = 8,       /* cell-order compaction */
+    PMC_KERNEL_STEP = 9,       /* whole steps */
+    PMC_KERNEL_CLASSES = 10
+};
+/* Accumulated kernel time (ms) and launch counts per class since the last reset,
+ * measured with HIP events on the context's stream while timing is enabled. */
+int pmc_kernel_times(pmc_ctx* ctx, double* total_ms, int64_t* launches, int32_t n);
+int pmc_reset_kernel_times(pmc_ctx* ctx);
 /* Enables per-kernel event timing for later pmc_step calls (off by default). */
 int pmc_set_timing(pmc_ctx* ctx, int32_t enabled);
 void pmc_destroy(pmc_ctx* ctx);

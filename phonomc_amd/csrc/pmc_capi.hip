@@ -47,11 +47,13 @@ struct pmc_ctx {
     double expected_injection = 0;
     bool timing = false;
     hipError_t hip_error = hipSuccess; // first failure of an unchecked-in-place HIP call
-    hipEvent_t ev[12] = {};
+    static constexpr int kEvents = 64;
+    hipEvent_t ev[kEvents] = {};
     void note(hipError_t e) {
         if (e != hipSuccess && hip_error == hipSuccess) hip_error = e;
     }
-    double kernel_ms[5] = {0, 0, 0, 0, 0};
+    double kernel_ms[PMC_KERNEL_CLASSES] = {};
+    int64_t kernel_launches[PMC_KERNEL_CLASSES] = {};
 
     template <class T>
     T* alloc(size_t n) {
@@ -457,55 +459,79 @@ int pmc_seed_carriers(pmc_ctx* ctx, int64_t n, const float* position, const int3
     return finish_population(ctx, n);
 }
 
+} // extern "C"
+
+namespace {
+// Launches one kernel (or kernel group) of class c, bracketed by events when timing.
+struct StepTimer {
+    pmc_ctx* ctx;
+    int used = 0;
+    int cls[pmc_ctx::kEvents / 2];
+    template <class F>
+    void run(int c, F&& launch) {
+        if (!ctx->timing || used >= pmc_ctx::kEvents / 2) {
+            launch();
+            return;
+        }
+        ctx->note(hipEventRecord(ctx->ev[2 * used], ctx->stream));
+        launch();
+        ctx->note(hipEventRecord(ctx->ev[2 * used + 1], ctx->stream));
+        cls[used++] = c;
+    }
+    void collect(bool whole_step = true) {
+        if (!used) return;
+        ctx->note(hipEventSynchronize(ctx->ev[2 * used - 1]));
+        for (int k = 0; k < used; ++k) {
+            ctx->kernel_ms[cls[k]] += elapsed(ctx->ev[2 * k], ctx->ev[2 * k + 1]);
+            ctx->kernel_launches[cls[k]] += 1;
+        }
+        if (!whole_step) return;
+        ctx->kernel_ms[PMC_KERNEL_STEP] += elapsed(ctx->ev[0], ctx->ev[2 * used - 1]);
+        ctx->kernel_launches[PMC_KERNEL_STEP] += 1;
+    }
+};
+} // namespace
+
+extern "C" {
+
 int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
     if (!ctx) return PMC_ERR_ARGUMENT;
     if (ctx->latched) return PMC_ERR_LATCHED; // the first failure's message stays in ctx->error
     if (!ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "carriers not initialized");
     if (nsteps < 0) return fail(ctx, PMC_ERR_ARGUMENT, "negative step count");
-    for (int k = 0; k < 5; ++k) ctx->kernel_ms[k] = 0;
     for (int s = 0; s < nsteps; ++s) {
         View& v = view(ctx);
-        const bool t = ctx->timing;
         hipStream_t st = ctx->stream;
-        if (t) ctx->note(hipEventRecord(ctx->ev[0], st));
-        pmc_launch::begin(v, st);
-        if (ctx->host_step % 100 == 0) pmc_launch::containment(v, st);
-        if (t) ctx->note(hipEventRecord(ctx->ev[1], st));
-        pmc_launch::transport(v, st);
-        if (t) ctx->note(hipEventRecord(ctx->ev[2], st));
-        pmc_launch::inject(v, st);
-        if (t) ctx->note(hipEventRecord(ctx->ev[3], st));
-        pmc_launch::refresh(v, st, 1);
+        StepTimer T{ctx};
+        T.run(PMC_KERNEL_BEGIN, [&] {
+            pmc_launch::begin(v, st);
+            if (ctx->host_step % 100 == 0) pmc_launch::containment(v, st);
+        });
+        T.run(PMC_KERNEL_TRANSPORT, [&] { pmc_launch::transport(v, st); });
+        T.run(PMC_KERNEL_INJECT, [&] { pmc_launch::inject(v, st); });
+        T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::refresh(v, st, 1); });
         for (int j = 0; j < pmc_dev::kNewtonSlots; ++j) {
-            pmc_launch::rta_pass(v, st, j == 0 ? 1 : 0);
-            pmc_launch::rta_update(v, st);
+            T.run(j == 0 ? PMC_KERNEL_RTA_FIRST : PMC_KERNEL_RTA_PASS, [&] { pmc_launch::rta_pass(v, st, j == 0 ? 1 : 0); });
+            T.run(PMC_KERNEL_RTA_UPDATE, [&] { pmc_launch::rta_update(v, st); });
         }
-        if (t) ctx->note(hipEventRecord(ctx->ev[4], st));
-        pmc_launch::relax(v, st);
-        if (t) ctx->note(hipEventRecord(ctx->ev[5], st));
-        pmc_launch::refresh(v, st, 2);
-        if (t) ctx->note(hipEventRecord(ctx->ev[6], st));
+        T.run(PMC_KERNEL_RELAX, [&] { pmc_launch::relax(v, st); });
+        T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::refresh(v, st, 2); });
         int rc = check_hip(ctx, hipGetLastError(), "step launch");
         if (rc) {
             ctx->latched = true;
             return rc;
         }
+        T.collect();
         ctx->host_step++;
         ctx->steps_since_sort++;
-        if (t) {
-            ctx->note(hipEventSynchronize(ctx->ev[6]));
-            ctx->kernel_ms[0] += elapsed(ctx->ev[1], ctx->ev[2]);
-            ctx->kernel_ms[1] += elapsed(ctx->ev[2], ctx->ev[3]);
-            ctx->kernel_ms[2] += elapsed(ctx->ev[3], ctx->ev[4]);
-            ctx->kernel_ms[3] += elapsed(ctx->ev[4], ctx->ev[5]);
-            ctx->kernel_ms[4] += elapsed(ctx->ev[0], ctx->ev[6]);
-        }
         if (ctx->steps_since_sort >= ctx->sort_interval || s == nsteps - 1) {
             rc = collect_error(ctx);
             if (rc) return rc;
             if (ctx->steps_since_sort >= ctx->sort_interval) {
-                rc = sort_carriers(ctx);
+                StepTimer S{ctx};
+                S.run(PMC_KERNEL_SORT, [&] { rc = sort_carriers(ctx); });
                 if (rc) return rc;
+                S.collect(false);
             }
         }
     }
@@ -598,9 +624,21 @@ int pmc_last_error(pmc_ctx* ctx, pmc_error* error) {
     return PMC_OK;
 }
 
-int pmc_kernel_times(pmc_ctx* ctx, double* ms, int32_t n) {
-    if (!ctx || !ms) return PMC_ERR_ARGUMENT;
-    for (int i = 0; i < n && i < 5; ++i) ms[i] = ctx->kernel_ms[i];
+int pmc_kernel_times(pmc_ctx* ctx, double* total_ms, int64_t* launches, int32_t n) {
+    if (!ctx) return PMC_ERR_ARGUMENT;
+    for (int i = 0; i < n && i < PMC_KERNEL_CLASSES; ++i) {
+        if (total_ms) total_ms[i] = ctx->kernel_ms[i];
+        if (launches) launches[i] = ctx->kernel_launches[i];
+    }
+    return PMC_OK;
+}
+
+int pmc_reset_kernel_times(pmc_ctx* ctx) {
+    if (!ctx) return PMC_ERR_ARGUMENT;
+    for (int i = 0; i < PMC_KERNEL_CLASSES; ++i) {
+        ctx->kernel_ms[i] = 0;
+        ctx->kernel_launches[i] = 0;
+    }
     return PMC_OK;
 }
 

@@ -13,6 +13,8 @@ BUILD = os.path.join(PKG, "_build")
 CONFIGS = os.path.join(PKG, "configs")
 
 PMC_OK = 0
+KERNEL_CLASSES = ["begin", "transport", "inject", "refresh", "rta_first", "rta_pass", "rta_update", "relax", "sort",
+                  "step"]
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "excess transport events", 4: "boundary failure",
           5: "escaped carrier", 6: "invalid state", 7: "capacity exhausted", 8: "latched", 9: "unsupported"}
 
@@ -154,7 +156,8 @@ def lib():
                                         C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
         "pmc_read_uids": (C.c_int, [P, C.c_int64, C.POINTER(C.c_uint32)]),
         "pmc_last_error": (C.c_int, [P, C.POINTER(Error)]),
-        "pmc_kernel_times": (C.c_int, [P, C.POINTER(C.c_double), C.c_int32]),
+        "pmc_kernel_times": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
+        "pmc_reset_kernel_times": (C.c_int, [P]),
         "pmc_set_timing": (C.c_int, [P, C.c_int32]),
         "pmc_destroy": (None, [P]),
         "pmc_status_string": (C.c_char_p, [C.c_int32]),
@@ -327,9 +330,13 @@ class Context:
         self._check(lib().pmc_set_timing(self.h, 1 if on else 0))
 
     def kernel_times(self):
-        out = (C.c_double * 5)()
-        self._check(lib().pmc_kernel_times(self.h, out, 5))
-        return list(out)
+        """Per-class (total ms, launches) since the last reset."""
+        ms, n = (C.c_double * len(KERNEL_CLASSES))(), (C.c_int64 * len(KERNEL_CLASSES))()
+        self._check(lib().pmc_kernel_times(self.h, ms, n, len(KERNEL_CLASSES)))
+        return {name: (ms[k], n[k]) for k, name in enumerate(KERNEL_CLASSES)}
+
+    def reset_kernel_times(self):
+        self._check(lib().pmc_reset_kernel_times(self.h))
 
     def last_error(self):
         e = Error()

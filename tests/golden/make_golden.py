@@ -1,0 +1,194 @@
+"""Writes the known-answer fixtures of tests/golden/ from the reference PhonoMC
+sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
+
+    make -f oracle/ref.mk REF=/root/reference && python tests/golden/make_golden.py
+
+The reference ships no tests or golden vectors, so these vectors are produced by
+calling its own functions on the synthetic Si-like table and the two film
+workloads.  Each fixture holds inputs, the uniforms the reference consumed (read
+from a copy of its mt19937_64 engine) and the reference outputs.  Only data is
+written; nothing of the reference's code is copied.
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import ref_ctypes as R  # noqa: E402
+from phonomc_amd import runtime as rt  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml"}
+
+
+def problem_and_reference(key, **overrides):
+    text = rt.config_text(WORKLOADS[key], **overrides)
+    problem = rt.Problem(text, mode_sampling=1)
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, f"{key}.toml")
+    with open(path, "w") as f:
+        f.write(text)
+    return problem, R.RefMaterial(problem.material()), R.RefDomain(path), path
+
+
+def material_vectors(problem, rm, rng):
+    s, m, g, i, rough, res = problem.views()
+    na = m.n_active
+    flat = rt.arr(m.active_flat, na, np.int32)
+    q, b = rm.active_modes()
+    e = np.concatenate([np.linspace(0.0035, 0.0060, 3001), rng.uniform(0.0, 0.02, 200)])
+    t_of_e = np.array([rm.temperature_from_energy(x) for x in e])
+    t = np.concatenate([[0.0, 1e-3, 5.0, 150.0, 299.95, 300.0, 300.05, 450.0, 999.9, 1000.0, 1500.0],
+                        rng.uniform(0, 1100, 200)])
+    e_of_t = np.array([rm.energy_from_temperature(x) for x in t])
+    crystal = np.array([rm.crystal_energy(x) for x in t[:40]])
+    modes = rng.integers(0, na, 400)
+    temps = rng.uniform(0, 1100, 400)
+    temps[:20] = [0, 0.5, 1.0, 10.0, 300, 300, 1000, 1000, 2000, 5, 290, 295, 305, 310, 150, 450, 999, 1001, 12, 7]
+    bose = np.array([rm.bose(temps[k], q[modes[k]], b[modes[k]]) for k in range(400)])
+    weight = np.array([rm.relaxation_weight(temps[k], q[modes[k]], b[modes[k]], 0.5) for k in range(400)])
+    return dict(active_flat=q * 6 + b, lut_e=e, lut_t=t_of_e, e_t=t, e_of_t=e_of_t, crystal_t=t[:40],
+                crystal_e=crystal, q_modes=modes, q_temps=temps, bose=bose, weight_dt05=weight,
+                zero_point=rm.zero_point(), normalization=rm.normalization(), active_check=np.array_equal(flat, q * 6 + b))
+
+
+def domain_vectors(problem, rd, rng):
+    f = rd.facets()
+    centers, volumes = rd.grid()
+    bmin, bmax, volume = rd.bounds()
+    pos = bmin + rng.random((3000, 3)) * (bmax - bmin)
+    vel = rng.normal(size=(3000, 3)) * 40.0
+    vel[:100, 0] = 0.0
+    vel[100:200, 1:] = 0.0
+    pt, t, facet = rd.trace(pos, vel)
+    gpos = bmin + rng.random((3000, 3)) * (bmax - bmin)
+    gpos[:50] = bmin
+    gpos[50:100] = bmax
+    cells = rd.grid_index(gpos)
+    return dict(normals=f["normals"], centroids=f["centroids"], areas=f["areas"],
+                conditions=np.frombuffer(f["conditions"].encode(), dtype=np.uint8), shifts=f["shifts"],
+                roughness=f["roughness"], rough=f["rough"], faces=f["faces"], centers=centers, volumes=volumes,
+                bmin=bmin, bmax=bmax, volume=volume, trace_pos=pos, trace_vel=vel, trace_point=pt, trace_t=t,
+                trace_facet=facet, grid_pos=gpos, grid_cell=cells)
+
+
+def rough_vectors(problem, rm, rd, rng, facets):
+    s, m, g, i, rough, res = problem.views()
+    na = m.n_active
+    out = {}
+    for facet in facets:
+        n = 4000
+        ins = rng.integers(0, na, n).astype(np.int32)
+        occ = rng.uniform(0.0, 3.0, n)
+        seeds = rng.integers(1, 2 ** 62, n, dtype=np.uint64)
+        oa, oo, u = rd.rough_sample(rm, facet, 300.0, ins, occ, seeds)
+        out[f"f{facet}_in"] = ins
+        out[f"f{facet}_occ"] = occ
+        out[f"f{facet}_u"] = u
+        out[f"f{facet}_out"] = oa
+        out[f"f{facet}_out_occ"] = oo
+        # interact_boundary on the same facet: velocity of the incoming mode
+        vel = rt.arr(m.velocity, 3 * na).reshape(na, 3)[ins]
+        action, oa2, ov, oo2, absorbed, shift = rd.interact(rm, facet, 300.0, 1.0, ins, vel, occ, seeds)
+        out[f"f{facet}_action"] = action
+        out[f"f{facet}_ivel"] = ov
+    return out
+
+
+def reservoir_vectors(problem, rm, rd, rng, dt):
+    s, m, g, i, rough, res = problem.views()
+    dens = problem.scalars()["particle_density"]
+    out = {}
+    for r, spec in enumerate(res):
+        init, oa, ot, u = rd.reservoir(rm, spec.facet, dens, dt, 1234 + r, 4000)
+        pts, uu = rd.sample_surface(spec.facet, 2000, 4321 + r)
+        out[f"r{r}_facet"] = spec.facet
+        out[f"r{r}_initial"] = init
+        out[f"r{r}_u"] = u
+        out[f"r{r}_mode"] = oa
+        out[f"r{r}_time"] = ot
+        out[f"r{r}_surf_u"] = uu
+        out[f"r{r}_surf_point"] = pts
+        # interact_boundary absorbs on a reservoir facet: absorbed energy known answer
+        ins = rng.integers(0, m.n_active, 300).astype(np.int32)
+        occ = rng.uniform(0.0, 3.0, 300)
+        vel = rt.arr(m.velocity, 3 * m.n_active).reshape(-1, 3)[ins]
+        action, _, _, _, absorbed, _ = rd.interact(rm, spec.facet, 300.0, 2.5, ins, vel, occ,
+                                                   rng.integers(1, 2 ** 62, 300, dtype=np.uint64))
+        out[f"r{r}_abs_in"] = ins
+        out[f"r{r}_abs_occ"] = occ
+        out[f"r{r}_abs_action"] = action
+        out[f"r{r}_abs_energy_w25"] = absorbed
+    return out
+
+
+def collision_vectors(problem, rm, rng, dt, local, lifetime_t):
+    s, m, g, i, rough, res = problem.views()
+    na = m.n_active
+    nc = g.n_cells
+    n = 12000
+    flat = rt.arr(m.active_flat, na, np.int32)
+    act = rng.integers(0, na, n).astype(np.int32)
+    cell = rng.integers(0, nc, n).astype(np.int32)
+    t = 298.0 + 4.0 * rng.random(n)
+    occ = np.array([rm.bose(t[k], flat[act[k]] // 6, flat[act[k]] % 6) for k in range(n)])
+    occ *= 1.0 + 0.02 * rng.normal(size=n)
+    occ = np.maximum(occ, 0.0)
+    centers, volumes = problem.cells()
+    sc = problem.scalars()
+    e, tc = R.refresh_temperatures(rm, volumes, 300.0, sc["particle_volume"], act, cell, occ)
+    new, residual = R.rta_apply(rm, tc, act, cell, tc[cell], occ, dt, sc["particle_volume"], local, lifetime_t)
+    return dict(active=act, cell=cell, occ=occ, cell_energy=e, cell_t=tc, occ_after=new, residual=residual,
+                dt=dt, local=local, lifetime_t=lifetime_t)
+
+
+def whole_run(key, n, steps, seeds, interval):
+    """Reference runs at reduced size: final cell temperatures and kappa per seed."""
+    temps, keff, kint = [], [], []
+    for seed in seeds:
+        tmp = tempfile.mkdtemp()
+        text = rt.config_text(WORKLOADS[key], **{"simulation.particle_count": n, "simulation.iterations": steps,
+                                                 "simulation.random_seed": seed,
+                                                 "simulation.convergence_write_interval": interval,
+                                                 "io.output_folder": f'"{tmp}/out"'})
+        path = os.path.join(tmp, "run.toml")
+        with open(path, "w") as f:
+            f.write(text)
+        problem = rt.Problem(text)
+        R.run(path, problem.material(), steps, threads=8)
+        rows = [line.split() for line in open(os.path.join(tmp, "out", "convergence.txt")) if not line.startswith("#")]
+        ncell = int(problem.scalars()["cells"])
+        # average the rows of the second half of the run
+        half = [r for r in rows if int(r[0]) > steps // 2]
+        tt = np.array([[float(x) for x in r[2:2 + ncell]] for r in half])
+        temps.append(tt.mean(axis=0))
+        kint.append(np.mean([float(r[2 + ncell + 1]) for r in half]))
+        keff.append(np.mean([float(r[2 + ncell + 2]) for r in half]))
+    return dict(temperature=np.array(temps), kappa_eff=np.array(keff), kappa_int=np.array(kint),
+                n=n, steps=steps, seeds=np.array(seeds), interval=interval)
+
+
+def main():
+    rng = np.random.default_rng(20261203)
+    p1, rm1, rd1, _ = problem_and_reference("c1")
+    p2, rm2, rd2, _ = problem_and_reference("c2")
+    np.savez_compressed(os.path.join(OUT, "material_si16.npz"), **material_vectors(p2, rm2, rng))
+    np.savez_compressed(os.path.join(OUT, "domain_c1.npz"), **domain_vectors(p1, rd1, rng))
+    np.savez_compressed(os.path.join(OUT, "domain_c2.npz"), **domain_vectors(p2, rd2, rng))
+    np.savez_compressed(os.path.join(OUT, "rough_c2.npz"), **rough_vectors(p2, rm2, rd2, rng, [4, 5]))
+    np.savez_compressed(os.path.join(OUT, "reservoir_c1.npz"), **reservoir_vectors(p1, rm1, rd1, rng, 1.0))
+    np.savez_compressed(os.path.join(OUT, "reservoir_c2.npz"), **reservoir_vectors(p2, rm2, rd2, rng, 0.5))
+    np.savez_compressed(os.path.join(OUT, "collision_c1.npz"), **collision_vectors(p1, rm1, rng, 1.0, 0, 300.0))
+    np.savez_compressed(os.path.join(OUT, "collision_c2.npz"), **collision_vectors(p2, rm2, rng, 0.5, 1, 300.0))
+    if "--runs" in sys.argv:
+        np.savez_compressed(os.path.join(OUT, "run_c1.npz"), **whole_run("c1", 20000, 400, list(range(1000, 1016)), 10))
+        np.savez_compressed(os.path.join(OUT, "run_c2.npz"), **whole_run("c2", 200000, 300, list(range(1000, 1016)), 10))
+    print("fixtures written to", OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,58 @@
+"""Invariants of the HIP step that hold at any size: energy closure, isothermal
+equilibrium, carrier conservation by collisions, containment, and the failure
+guards of the reference (forced from the host, no device fault)."""
+import numpy as np
+import pytest
+
+from phonomc_amd import runtime as rt
+
+pytestmark = pytest.mark.gpu
+
+
+def test_energy_balance_closes_every_step():
+    p = rt.Problem(rt.config_text("c2_inplane_film.toml", **{"simulation.particle_count": 400000}))
+    g = rt.Context(p)
+    for _ in range(12):
+        g.step(1)
+        led = g.ledger()
+        assert abs(led["step_balance_residual_ev"]) <= 1e-12 * led["thermal_energy_ev"]
+        assert abs(led["step_lifetime_residual_ev"]) <= 1e-12 * led["thermal_energy_ev"]
+
+
+def test_isothermal_box_stays_at_background():
+    text = rt.config_text("c1_cross_plane.toml", **{"simulation.particle_count": 100000,
+                                                    "boundary.boundary_values": "[300, 300, 0, 0, 0, 0]",
+                                                    "simulation.initial_temperature": "300"})
+    g = rt.Context(rt.Problem(text))
+    t0, _ = g.cells()
+    g.step(30)
+    t, _ = g.cells()
+    assert np.all(np.abs(t - 300.0) < 1e-6), t
+    assert np.all(np.abs(t0 - 300.0) < 1e-6)
+
+
+def test_full_size_population_and_sorting():
+    p = rt.Problem(rt.config_text("c2_inplane_film.toml"))
+    g = rt.Context(p)
+    g.step(8)  # ends on a cell-order compaction
+    led = g.ledger()
+    assert abs(led["live_carriers"] - 10_000_000) < 5000
+    assert led["total_absorbed"] > 0 and led["rough_events"] > 0
+    c = g.carriers()
+    live = c["mode"] >= 0
+    assert live.all() and len(live) == led["live_carriers"]
+    assert np.all(np.diff(c["cell"]) >= 0)
+    bmin, bmax = np.zeros(3), np.array([10000.0, 1000.0, 1000.0])
+    pos = c["position"][live]
+    assert np.all(pos >= bmin) and np.all(pos <= bmax)
+
+
+def test_excess_events_guard_latches_context():
+    """A 1 nm periodic cube with dt = 50 ps forces > 64 wall events per step."""
+    text = rt.config_text("c1_cross_plane.toml", **{"simulation.particle_count": 2000, "simulation.time_step": "50.0",
+                                                    "geometry.sizes": "[1, 1, 1]", "simulation.grid_xyz": "[1, 1, 2]"})
+    g = rt.Context(rt.Problem(text))
+    with pytest.raises(RuntimeError, match="exceeded 64 transport events"):
+        g.step(1)
+    with pytest.raises(RuntimeError, match="pmc error 8"):
+        g.step(1)

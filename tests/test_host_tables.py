@@ -1,0 +1,155 @@
+"""Host-built tables (material, domain, boundary tables, config) against the
+reference's known answers and closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from phonomc_amd import runtime as rt
+
+HBAR, KB = 6.582119569e-4, 8.617333262145e-5
+
+
+def test_material_queries_bit_exact(problems, golden):
+    p, g = problems("c2"), golden("material_si16.npz")
+    L = rt.lib()
+    s, m, geo, i, rough, res = p.views()
+    assert np.array_equal(rt.arr(m.active_flat, m.n_active, np.int32), g["active_flat"])
+    sc = p.scalars()
+    assert sc["zero_point_density"] == float(g["zero_point"])
+    assert sc["energy_normalization"] == float(g["normalization"])
+    for e, t in zip(g["lut_e"], g["lut_t"]):
+        assert L.pmcs_problem_temperature_from_energy(p.h, e) == t
+    for t, e in zip(g["e_t"], g["e_of_t"]):
+        assert L.pmcs_problem_energy_from_temperature(p.h, t) == e
+    flat = g["active_flat"]
+    for k in range(len(g["q_modes"])):
+        f = int(flat[g["q_modes"][k]])
+        assert L.pmcs_problem_bose(p.h, g["q_temps"][k], f) == g["bose"][k]
+        assert L.pmcs_problem_relaxation_weight(p.h, g["q_temps"][k], f, 0.5) == g["weight_dt05"][k]
+
+
+@pytest.mark.parametrize("key", ["c1", "c2"])
+def test_domain_bit_exact(problems, golden, key):
+    p, g = problems(key), golden(f"domain_{key}.npz")
+    f = p.facets()
+    for name in ["normals", "centroids", "areas", "shifts"]:
+        assert np.array_equal(f[name], g[name]), name
+    assert f["conditions"] == bytes(g["conditions"]).decode()
+    assert np.array_equal(f["faces"], g["faces"])
+    centers, volumes = p.cells()
+    assert np.array_equal(centers, g["centers"]) and np.array_equal(volumes, g["volumes"])
+    assert p.scalars()["volume"] == float(g["volume"])
+
+
+def test_synthetic_table_closed_forms(problems):
+    """Dispersion, velocity and linewidth of the synthetic table follow their formulas."""
+    p = problems("c1")
+    mv = p.material()
+    nq, nb = mv.qpoints, mv.branches
+    assert (nq, nb) == (4096, 6) and list(mv.mesh) == [16, 16, 16]
+    f = rt.arr(mv.frequency_thz, nq * nb).reshape(nq, nb)
+    v = rt.arr(mv.group_velocity, nq * nb * 3).reshape(nq, nb, 3)
+    qf = rt.arr(mv.qpoint_fractions, nq * 3).reshape(nq, 3)
+    a = mv.volume_a3 ** (1.0 / 3.0)
+    assert mv.volume_a3 == pytest.approx(5.431 ** 3 / 4, rel=1e-12)
+    k = 2 * math.pi / a * qf
+    kn = np.linalg.norm(k, axis=1)
+    kmax = math.sqrt(3) * math.pi / a
+    for br, v0 in [(0, 58.0), (1, 58.0), (2, 84.0)]:
+        w = 2 * v0 * kmax / math.pi * np.sin(0.5 * math.pi * kn / kmax)
+        np.testing.assert_allclose(2 * math.pi * f[:, br], w, rtol=1e-12, atol=1e-12)
+        # |v| equals dw/dk away from zone faces; sound velocity at small k
+        interior = np.all(np.abs(qf) < 0.5, axis=1) & (kn > 0)
+        np.testing.assert_allclose(np.linalg.norm(v[interior, br], axis=1),
+                                   v0 * np.cos(0.5 * math.pi * kn[interior] / kmax), rtol=1e-12)
+    assert np.all((f[:, 3:] >= 12.0) & (f[:, 3:] <= 15.0))
+    assert np.linalg.norm(v[:, 3:], axis=2).max() <= 10.0  # optical |v| <= 1 km/s
+    # velocity component normal to a zone face vanishes on that face
+    face = np.abs(qf) == 0.5
+    for ax in range(3):
+        assert np.all(v[face[:, ax], :, ax] == 0.0)
+    # tau^-1 = A w^4 + B w^2 T exp(-C/T), gamma = tau^-1 / (4 pi) in THz
+    T = rt.arr(mv.temperatures, mv.ntemps)
+    gam = rt.arr(mv.gamma, mv.ntemps * nq * nb).reshape(mv.ntemps, nq * nb)
+    w = 2 * math.pi * f.reshape(-1) * 1e12
+    for ti in [0, 30, 100]:
+        anh = 1.7e-19 * w ** 2 * T[ti] * math.exp(-137.0 / T[ti]) if T[ti] > 0 else 0 * w
+        expect = (1.3e-45 * w ** 4 + anh) * 1e-12 / (4 * math.pi)
+        np.testing.assert_allclose(gam[ti], expect, rtol=1e-12, atol=1e-300)
+
+
+def test_table_mirror_symmetry(problems):
+    """Every incoming mode on a z wall has an exact mirror partner (specular match)."""
+    p = problems("c2")
+    s, m, g, i, rough, res = p.views()
+    na = m.n_active
+    vel = rt.arr(m.velocity, 3 * na).reshape(na, 3)
+    for r in rough:
+        match = rt.arr(r.spec_match, na, np.int32)
+        inc = np.where(match >= 0)[0]
+        mirrored = vel[inc] * np.array([1, 1, -1])
+        np.testing.assert_array_equal(vel[match[inc]], mirrored)
+
+
+@pytest.mark.parametrize("key", ["c1", "c2"])
+def test_alias_tables_reconstruct_the_inward_flux_pmf(problems, key):
+    p = problems(key)
+    s, m, g, i, rough, res = p.views()
+    na = m.n_active
+    vel = rt.arr(m.velocity, 3 * na).reshape(na, 3)
+    for r in res:
+        prob = rt.arr(r.alias_prob, na)
+        alias = rt.arr(r.alias_index, na, np.int32)
+        pmf = prob.copy()
+        np.add.at(pmf, alias, 1.0 - prob)
+        pmf /= na
+        flux = np.maximum(0.0, -(vel @ np.array(list(r.normal))))
+        expect = flux / flux.sum()
+        np.testing.assert_allclose(pmf, expect, rtol=1e-9, atol=1e-15)
+        cdf = rt.arr(r.cdf, na)
+        np.testing.assert_allclose(np.diff(np.concatenate([[0.0], cdf])), expect, rtol=1e-9, atol=1e-14)
+
+
+def test_expected_reservoir_count_closed_form(problems):
+    """Expected injections per step = sum max(0, v.n_in) dt A rho / M (src/solver/ReservoirSampler.cpp:79-82)."""
+    p = problems("c1")
+    s, m, g, i, rough, res = p.views()
+    na = m.n_active
+    vel = rt.arr(m.velocity, 3 * na).reshape(na, 3)
+    sc = p.scalars()
+    for r in res:
+        f = p.facets()
+        area = f["areas"][r.facet]
+        flux = np.maximum(0.0, -(vel @ np.array(list(r.normal))))
+        expect = flux.sum() * s.time_step_ps * area * sc["particle_density"] / na
+        assert r.expected_count == pytest.approx(expect, rel=1e-12)
+
+
+def test_energy_weight_closed_form(problems):
+    p = problems("c2")
+    s, m, g, i, rough, res = p.views()
+    sc = p.scalars()
+    assert sc["particle_volume"] == sc["volume"] / sc["carriers"]
+    assert m.energy_weight == pytest.approx(m.n_active * sc["particle_volume"] / sc["energy_normalization"], rel=1e-15)
+    assert s.push_distance_a == pytest.approx(1e-10 * max(s.time_step_ps, 1.0))
+
+
+@pytest.mark.parametrize("bad,message", [
+    ("[bogus]\nx = 1\n", "Unknown TOML section: [bogus]"),
+    ("[simulation]\ngrid_xyz=[1,1,1]\nfoo = 1\n", "Unknown TOML key: simulation.foo"),
+    ("[simulation]\ngrid_xyz=[1,1,1]\ngrid_xyz=[1,1,1]\n", "Duplicate TOML key: simulation.grid_xyz"),
+    ("[simulation]\nparticle_count = 10\n", "Missing required key: simulation.grid_xyz."),
+    ("[simulation]\ngrid_xyz=[1,1,1]\ntime_step = -1\n", "time_step must be a finite positive value."),
+    ("x = 1\n", "Top-level TOML keys are not supported"),
+    ("[simulation]\ngrid_xyz=[1,1,1]\nlifetime_temperature = \"hot\"\n", "lifetime_temperature string value must be"),
+])
+def test_config_rejections_use_reference_messages(bad, message):
+    with pytest.raises(RuntimeError) as e:
+        rt.Problem(bad)
+    assert message in str(e.value)
+
+
+def test_unsupported_features_fail_loudly():
+    with pytest.raises(RuntimeError, match="callaway"):
+        rt.Problem(rt.config_text("c1_cross_plane.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))
