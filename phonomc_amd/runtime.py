@@ -362,19 +362,25 @@ def config_text(name, **overrides):
     section = None
     out = []
     done = set()
+    depth = 0  # bracket depth of a multi-line array value
     for line in lines:
         s = line.strip()
-        if s.startswith("[") and s.endswith("]") and "=" not in s:
+        if depth == 0 and s.startswith("[") and s.endswith("]") and "=" not in s:
             section = s[1:-1]
             out.append(line)
             continue
-        if "=" in s and section:
+        if depth == 0 and "=" in s and section:
             key = s.split("=", 1)[0].strip()
             full = f"{section}.{key}"
+            depth = s.count("[") - s.count("]")
             if full in overrides:
+                if depth != 0:
+                    raise KeyError(f"override of multi-line value {full} is not supported")
                 out.append(f"{key} = {overrides[full]}")
                 done.add(full)
                 continue
+        elif depth:
+            depth += s.count("[") - s.count("]")
         out.append(line)
     missing = set(overrides) - done
     if missing:
