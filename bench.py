@@ -29,14 +29,17 @@ sys.path.insert(0, ROOT)
 
 WORKLOAD = "c2_inplane_film.toml"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per carrier of each kernel class (reads + writes of the
-# carrier arrays the kernel must touch; tables and cell tallies excluded).
+# Algorithmic bytes per carrier of each carrier-pass kernel class (carrier
+# arrays the pass must read and write; tables and cell tallies excluded).
 BYTES_PER_CARRIER = {
-    "transport": 24 + 16,  # read pos(12) mode(4) occ(8); write pos(12) cell(4)
-    "rta_first": 16,       # read mode(4) cell(4) occ(8)
-    "rta_pass": 16,
-    "relax": 16 + 8,       # read mode, cell, occ; write occ
+    "transport": 16 + 16,  # read pos(12) mode(4); write pos(12) cell(4)
+    "sort": 32 + 32,       # read pos, mode, occ, cell, uid (32); write them in cell order (32)
+    "collide": 12 + 8,     # read mode(4) occ(8) of the cell's carriers; write occ(8)
 }
+# Whole-step algorithmic bytes per particle-step: fused transport + tally pass
+# (read and write pos 12, mode 4, occ 8, cell 4) and collision pass (read mode,
+# cell, occ; write occ).
+STEP_BYTES = 80
 
 
 def log(*a):
@@ -64,7 +67,16 @@ def cpu_baseline(problem_text, steps):
         f.write(text)
     problem = rt.Problem(text)
     if ref_ctypes.available():
-        secs, ps = ref_ctypes.run(path, problem.material(), steps, threads=cores)
+        # The reference solver prints progress on stdout; keep stdout for the JSON line.
+        sys.stdout.flush()
+        saved = os.dup(1)
+        os.dup2(2, 1)
+        try:
+            secs, ps = ref_ctypes.run(path, problem.material(), steps, threads=cores)
+        finally:
+            sys.stdout.flush()
+            os.dup2(saved, 1)
+            os.close(saved)
         kind = "reference"
     else:  # restatement when the reference build is absent
         from oracle import oracle_ctypes as O
@@ -131,9 +143,13 @@ def main():
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    step_gbs = value * STEP_BYTES / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dominant,
+                "bytes_per_carrier": BYTES_PER_CARRIER[dominant], "carriers_per_launch": live,
                 "avg_launch_ms": total_ms / launches,
+                "step": {"bytes_per_particle_step": STEP_BYTES, "achieved": step_gbs,
+                         "frac": step_gbs / HBM_PEAK_GBS},
                 "step_breakdown_ms": {k: v[0] / max(1, times["step"][1]) for k, v in times.items() if v[1]}}
     log(f"[bench] kernel times (ms/step): {json.dumps(roofline['step_breakdown_ms'])}")
 

@@ -147,7 +147,8 @@ typedef struct pmc_ledger {
     double thermal_energy_ev;
     int64_t carrier_steps;         /* sum of live carriers at step start */
     int64_t rough_events, rough_specular, rough_diffuse;
-    int64_t newton_passes;         /* RTA pseudo-temperature passes run */
+    int64_t newton_passes;         /* exact carrier sums of the pseudo-temperature solve */
+    int64_t newton_evals;          /* all evaluations of the pseudo-temperature function */
 } pmc_ledger;
 
 typedef struct pmc_error {
@@ -193,19 +194,34 @@ int pmc_read_carriers(pmc_ctx* ctx, int64_t* n, float* position, int32_t* mode, 
 /* Philox stream ids of slots [0, n) (initial carriers 0..N-1, injected ones numbered on). */
 int pmc_read_uids(pmc_ctx* ctx, int64_t n, uint32_t* uid);
 int pmc_last_error(pmc_ctx* ctx, pmc_error* error);
+
+/* Evaluation strategy of the RTA pseudo-temperature solve
+ * (src/solver/RtaCollisionOperator.cpp:91-165).  The reference sums
+ * hw*a*(n - n_BE(T)) over a cell's carriers for every trial T.  With the
+ * series on, one pass stores per cell the exact value at the cell temperature
+ * T0 plus Taylor terms of sum hw*a*n_BE(T) in u = T0/T - 1; trial
+ * temperatures within |u| <= series_radius whose truncation estimate stays
+ * below 1e-3 of the convergence tolerance use the series, all others the exact
+ * carrier sum.  series_on = 0 reproduces the reference's exact sums only. */
+typedef struct pmc_tuning {
+    int32_t series_on;             /* default 1 */
+    double series_radius;          /* default 0.01 */
+    int32_t reserved[6];
+} pmc_tuning;
+int pmc_set_tuning(pmc_ctx* ctx, const pmc_tuning* tuning);
+int pmc_get_tuning(pmc_ctx* ctx, pmc_tuning* tuning);
 /* Kernel classes of the per-kernel timers. */
 enum pmc_kernel_class {
     PMC_KERNEL_BEGIN = 0,      /* step set-up (+ containment check every 100 steps) */
-    PMC_KERNEL_TRANSPORT = 1,  /* free flight, boundary events, energy tally */
-    PMC_KERNEL_INJECT = 2,     /* reservoir injection and remaining flight */
-    PMC_KERNEL_REFRESH = 3,    /* cell temperatures and ledgers */
-    PMC_KERNEL_RTA_FIRST = 4,  /* first pseudo-temperature pass (bracket + Newton) */
-    PMC_KERNEL_RTA_PASS = 5,   /* later pseudo-temperature passes */
-    PMC_KERNEL_RTA_UPDATE = 6, /* per-cell Newton updates */
-    PMC_KERNEL_RELAX = 7,      /* relaxation + energy tally */
-    PMC_KERNEL_SORT = 8,       /* cell-order compaction */
-    PMC_KERNEL_STEP = 9,       /* whole steps */
-    PMC_KERNEL_CLASSES = 10
+    PMC_KERNEL_TRANSPORT = 1,  /* free flight of carriers that reach no wall, energy tally */
+    PMC_KERNEL_EVENTS = 2,     /* carriers with wall events (boundary conditions, rough walls) */
+    PMC_KERNEL_INJECT = 3,     /* reservoir injection and remaining flight */
+    PMC_KERNEL_REFRESH = 4,    /* cell temperatures, cell offsets and ledgers */
+    PMC_KERNEL_SORT = 5,       /* cell-order compaction of the live carriers */
+    PMC_KERNEL_COLLIDE = 6,    /* per cell: temperature, pseudo-temperature solve, relaxation */
+    PMC_KERNEL_OUTPUT = 7,     /* observables outside the step (heat flux) */
+    PMC_KERNEL_STEP = 8,       /* whole steps */
+    PMC_KERNEL_CLASSES = 9
 };
 /* Accumulated kernel time (ms) and launch counts per class since the last reset,
  * measured with HIP events on the context's stream while timing is enabled. */

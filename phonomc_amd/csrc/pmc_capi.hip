@@ -15,7 +15,7 @@
 
 using pmc_dev::DevState;
 using pmc_dev::FacetDev;
-using pmc_dev::NewtonCell;
+using pmc_dev::CellRta;
 using pmc_dev::ResDev;
 using pmc_dev::RoughDev;
 using pmc_dev::View;
@@ -41,8 +41,7 @@ struct pmc_ctx {
     bool latched = false;
     pmc_error error{};
     int64_t host_step = 0;
-    int sort_interval = 8;
-    int steps_since_sort = 0;
+    int error_interval = 8;   // steps between reads of the device error word
     int n_reservoirs = 0;
     double expected_injection = 0;
     bool timing = false;
@@ -89,6 +88,8 @@ int check_hip(pmc_ctx* ctx, hipError_t e, const char* what) {
     return fail(ctx, PMC_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// View with the current carrier buffer as source and the other one as the
+// destination of the cell sort.
 View& view(pmc_ctx* ctx) {
     View& v = ctx->view;
     const Carriers& c = ctx->buf[ctx->cur];
@@ -99,6 +100,14 @@ View& view(pmc_ctx* ctx) {
     v.occ = c.occ;
     v.cell = c.cell;
     v.uid = c.uid;
+    const Carriers& d = ctx->buf[1 - ctx->cur];
+    v.o_px = d.px;
+    v.o_py = d.py;
+    v.o_pz = d.pz;
+    v.o_mode = d.mode;
+    v.o_occ = d.occ;
+    v.o_cell = d.cell;
+    v.o_uid = d.uid;
     return v;
 }
 
@@ -134,8 +143,8 @@ int collect_error(pmc_ctx* ctx) {
                       i, x, y, z);
         break;
     case PMC_ERR_INVALID_STATE:
-        if (st.error_facet == -2)
-            std::snprintf(msg, sizeof msg, "RTA pseudo-temperature of cell %lld did not converge within the pass budget.", i);
+        if (st.error_facet == -3)
+            std::snprintf(msg, sizeof msg, "Live carrier count %lld does not match the boundary ledger.", i);
         else
             std::snprintf(msg, sizeof msg, "Invalid particle in grid energy statistics.");
         break;
@@ -150,15 +159,6 @@ int collect_error(pmc_ctx* ctx) {
     }
     std::snprintf(ctx->error.message, sizeof(ctx->error.message), "%s", msg);
     return st.error_code;
-}
-
-int sort_carriers(pmc_ctx* ctx) {
-    View& v = view(ctx);
-    Carriers& d = ctx->buf[1 - ctx->cur];
-    pmc_launch::sort_by_cell(v, ctx->stream, d.px, d.py, d.pz, d.mode, d.occ, d.cell, d.uid);
-    ctx->cur = 1 - ctx->cur;
-    ctx->steps_since_sort = 0;
-    return check_hip(ctx, hipGetLastError(), "sort");
 }
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
@@ -212,6 +212,8 @@ int pmc_create(const pmc_setup* setup, pmc_ctx** out) {
     v.lifetime_T = setup->lifetime_temperature;
     v.sampling = setup->mode_sampling;
     v.inversion = setup->temperature_inversion;
+    v.series_on = 1;
+    v.series_radius = 0.01;
     for (auto& b : ctx->buf) {
         const size_t n = static_cast<size_t>(setup->capacity);
         b.px = ctx->alloc<float>(n);
@@ -253,16 +255,28 @@ int pmc_upload_material(pmc_ctx* ctx, const pmc_mode_table* t) {
     v.vel = ctx->upload(vel.data(), vel.size());
     v.veld = ctx->upload(t->velocity, 3 * static_cast<size_t>(t->n_active));
     v.neq = ctx->upload(t->neq_background, t->n_active);
+    std::vector<double2> mtab(t->n_active);
+    for (int a = 0; a < t->n_active; ++a) mtab[a] = make_double2(t->omega[a], t->neq_background[a]);
+    v.mtab = ctx->upload(mtab.data(), mtab.size());
     v.ntemps = t->n_temps;
     v.temps = ctx->upload(t->temps, t->n_temps);
-    v.gamma = ctx->upload(t->gamma_total, static_cast<size_t>(t->n_temps) * t->n_flat);
+    // Rate table rows in active-mode order: one gather per carrier instead of two.
+    std::vector<double> gamma(static_cast<size_t>(t->n_temps) * t->n_active);
+    for (int r = 0; r < t->n_temps; ++r)
+        for (int a = 0; a < t->n_active; ++a) {
+            const int f = t->active_flat[a];
+            if (f < 0 || f >= t->n_flat) return fail(ctx, PMC_ERR_ARGUMENT, "active mode outside the flat table");
+            gamma[static_cast<size_t>(r) * t->n_active + a] = t->gamma_total[static_cast<size_t>(r) * t->n_flat + f];
+        }
+    v.gamma = ctx->upload(gamma.data(), gamma.size());
     v.w = t->energy_weight;
     v.e_bg = t->background_energy_density;
     v.zp = t->zero_point_density;
     v.lut_n = t->lut_n;
     v.lut_t = ctx->upload(t->lut_t, t->lut_n);
     v.lut_e = ctx->upload(t->lut_e, t->lut_n);
-    if (!v.active_flat || !v.omega || !v.vel || !v.veld || !v.neq || !v.temps || !v.gamma || !v.lut_t || !v.lut_e)
+    if (!v.active_flat || !v.omega || !v.vel || !v.veld || !v.neq || !v.mtab || !v.temps || !v.gamma || !v.lut_t ||
+        !v.lut_e)
         return fail(ctx, PMC_ERR_DEVICE, "material upload failed");
     ctx->has_material = true;
     return PMC_OK;
@@ -309,21 +323,25 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
     v.partial_rows = pmc_launch::partial_rows(v.capacity);
     v.partials = ctx->alloc<double>(static_cast<size_t>(v.partial_rows) * pmc_dev::kPartialCols);
     v.cell_T = ctx->alloc<double>(nc);
-    v.cell_acc = ctx->alloc<double>(8 * nc);
+    v.cell_acc = ctx->alloc<double>(pmc_dev::kAccCols * nc);
     v.cell_flux = ctx->alloc<double>(3 * nc);
     v.cell_count = ctx->alloc<int>(nc);
+    v.cell_start = ctx->alloc<int>(nc + 1);
     v.cell_cursor = ctx->alloc<int>(nc);
-    v.cell_glo = ctx->alloc<int>(nc);
-    v.cell_ghi = ctx->alloc<int>(nc);
-    v.cell_gt = ctx->alloc<double>(nc);
-    v.newton = ctx->alloc<NewtonCell>(nc);
+    v.crta = ctx->alloc<CellRta>(nc);
+    const size_t blocks = static_cast<size_t>(pmc_launch::wave_blocks(v.capacity));
+    v.events = ctx->alloc<int>(blocks * pmc_dev::kSlotsPerBlock);
+    v.block_events = ctx->alloc<int>(blocks);
     if (!v.facets || !v.voxel || !v.cell_vol || !v.cell_e || !v.cell_T || !v.cell_acc || !v.cell_tally || !v.partials ||
-        !v.cell_flux || !v.cell_count || !v.cell_cursor || !v.cell_glo || !v.cell_ghi || !v.cell_gt || !v.newton)
+        !v.cell_flux || !v.cell_count || !v.cell_start || !v.cell_cursor || !v.crta || !v.events || !v.block_events)
         return fail(ctx, PMC_ERR_DEVICE, "geometry upload failed");
-    ctx->note(hipMemset(v.cell_acc, 0, 8 * nc * sizeof(double)));
+    ctx->note(hipMemset(v.cell_acc, 0, pmc_dev::kAccCols * nc * sizeof(double)));
+    ctx->note(hipMemset(v.cell_start, 0, (nc + 1) * sizeof(int)));
+    ctx->note(hipMemset(v.cell_count, 0, nc * sizeof(int)));
+    ctx->note(hipMemset(v.block_events, 0, blocks * sizeof(int)));
     ctx->note(hipMemset(v.cell_tally, 0, pmc_dev::kTallyStride * nc * sizeof(double)));
     ctx->note(hipMemset(v.partials, 0, static_cast<size_t>(v.partial_rows) * pmc_dev::kPartialCols * sizeof(double)));
-    ctx->note(hipMemset(v.newton, 0, nc * sizeof(NewtonCell)));
+    ctx->note(hipMemset(v.crta, 0, nc * sizeof(CellRta)));
     ctx->has_geometry = true;
     return PMC_OK;
 }
@@ -391,13 +409,15 @@ static int finish_population(pmc_ctx* ctx, int64_t count) {
     st.n_slots_start = static_cast<int>(count);
     st.next_uid = static_cast<unsigned>(count);
     ctx->note(hipMemcpy(v.st, &st, sizeof(DevState), hipMemcpyHostToDevice));
-    int rc = sort_carriers(ctx);
-    if (rc) return rc;
+    // Cell sort, then the initial refresh (src/solver/Observables.cpp:12-21).
+    pmc_launch::count_all(v, ctx->stream);
+    pmc_launch::refresh(v, ctx->stream, 0);
+    pmc_launch::sort(v, ctx->stream);
+    ctx->cur = 1 - ctx->cur;
     View& w = view(ctx);
-    pmc_launch::clear_cells(w, ctx->stream);
-    pmc_launch::tally_all(w, ctx->stream);
-    pmc_launch::refresh(w, ctx->stream, 0);
-    rc = check_hip(ctx, hipGetLastError(), "initial refresh");
+    pmc_launch::collide(w, ctx->stream, 0);
+    pmc_launch::refresh(w, ctx->stream, 3);
+    int rc = check_hip(ctx, hipGetLastError(), "initial refresh");
     if (rc) return rc;
     ctx->initialized = true;
     return collect_error(ctx);
@@ -508,14 +528,14 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
             if (ctx->host_step % 100 == 0) pmc_launch::containment(v, st);
         });
         T.run(PMC_KERNEL_TRANSPORT, [&] { pmc_launch::transport(v, st); });
+        T.run(PMC_KERNEL_EVENTS, [&] { pmc_launch::events(v, st); });
         T.run(PMC_KERNEL_INJECT, [&] { pmc_launch::inject(v, st); });
         T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::refresh(v, st, 1); });
-        for (int j = 0; j < pmc_dev::kNewtonSlots; ++j) {
-            T.run(j == 0 ? PMC_KERNEL_RTA_FIRST : PMC_KERNEL_RTA_PASS, [&] { pmc_launch::rta_pass(v, st, j == 0 ? 1 : 0); });
-            T.run(PMC_KERNEL_RTA_UPDATE, [&] { pmc_launch::rta_update(v, st); });
-        }
-        T.run(PMC_KERNEL_RELAX, [&] { pmc_launch::relax(v, st); });
-        T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::refresh(v, st, 2); });
+        T.run(PMC_KERNEL_SORT, [&] { pmc_launch::sort(v, st); });
+        ctx->cur = 1 - ctx->cur;
+        View& w = view(ctx);
+        T.run(PMC_KERNEL_COLLIDE, [&] { pmc_launch::collide(w, st, 1); });
+        T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::refresh(w, st, 2); });
         int rc = check_hip(ctx, hipGetLastError(), "step launch");
         if (rc) {
             ctx->latched = true;
@@ -523,16 +543,9 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
         }
         T.collect();
         ctx->host_step++;
-        ctx->steps_since_sort++;
-        if (ctx->steps_since_sort >= ctx->sort_interval || s == nsteps - 1) {
+        if (ctx->host_step % ctx->error_interval == 0 && s != nsteps - 1) {
             rc = collect_error(ctx);
             if (rc) return rc;
-            if (ctx->steps_since_sort >= ctx->sort_interval) {
-                StepTimer S{ctx};
-                S.run(PMC_KERNEL_SORT, [&] { rc = sort_carriers(ctx); });
-                if (rc) return rc;
-                S.collect(false);
-            }
         }
     }
     return collect_error(ctx);
@@ -550,7 +563,9 @@ int pmc_read_cells(pmc_ctx* ctx, double* temperature, double* energy_density) {
 int pmc_read_heat_flux(pmc_ctx* ctx, double* flux) {
     if (!ctx || !ctx->initialized || !flux) return PMC_ERR_ARGUMENT;
     View& v = view(ctx);
-    pmc_launch::flux(v, ctx->stream);
+    StepTimer T{ctx};
+    T.run(PMC_KERNEL_OUTPUT, [&] { pmc_launch::flux(v, ctx->stream); });
+    T.collect(false);
     ctx->note(hipStreamSynchronize(ctx->stream));
     ctx->note(hipMemcpy(flux, v.cell_flux, 3 * v.ncells * sizeof(double), hipMemcpyDeviceToHost));
     return check_hip(ctx, hipGetLastError(), "heat flux");
@@ -582,6 +597,7 @@ int pmc_read_ledger(pmc_ctx* ctx, pmc_ledger* out) {
     out->rough_specular = st.rough_specular;
     out->rough_diffuse = st.rough_diffuse;
     out->newton_passes = st.newton_passes;
+    out->newton_evals = st.newton_evals;
     return check_hip(ctx, hipGetLastError(), "read ledger");
 }
 
@@ -621,6 +637,23 @@ int pmc_read_uids(pmc_ctx* ctx, int64_t n, uint32_t* uid) {
 int pmc_last_error(pmc_ctx* ctx, pmc_error* error) {
     if (!ctx || !error) return PMC_ERR_ARGUMENT;
     *error = ctx->error;
+    return PMC_OK;
+}
+
+int pmc_set_tuning(pmc_ctx* ctx, const pmc_tuning* t) {
+    if (!ctx || !t) return PMC_ERR_ARGUMENT;
+    if (!(t->series_radius >= 0.0) || !(t->series_radius <= 0.5))
+        return fail(ctx, PMC_ERR_ARGUMENT, "series_radius must lie in [0, 0.5]");
+    ctx->view.series_on = t->series_on ? 1 : 0;
+    ctx->view.series_radius = t->series_radius;
+    return PMC_OK;
+}
+
+int pmc_get_tuning(pmc_ctx* ctx, pmc_tuning* t) {
+    if (!ctx || !t) return PMC_ERR_ARGUMENT;
+    *t = pmc_tuning{};
+    t->series_on = ctx->view.series_on;
+    t->series_radius = ctx->view.series_radius;
     return PMC_OK;
 }
 

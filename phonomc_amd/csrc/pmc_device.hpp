@@ -21,13 +21,36 @@ enum Stream : uint32_t { kStreamTransport = 1, kStreamInject = 2, kStreamInit = 
 
 constexpr int kMaxReservoirs = 16;
 constexpr int kMaxFacetsDev = 64;
-constexpr int kNewtonSlots = 8;   // RTA pseudo-temperature passes per step (early exit when done)
 
-// Carrier passes: each wave walks kRowsPerWave rows of 64 consecutive slots.
+// Carrier passes: each wave walks kRowsPerWave rows of 64 consecutive slots,
+// loading kUnroll rows ahead of their use.
 constexpr int kRowsPerWave = 16;
+constexpr int kUnroll = 4;
 constexpr int kWavesPerBlock = 4;
 constexpr int kSlotsPerBlock = 64 * kRowsPerWave * kWavesPerBlock;
 constexpr int kInjectBlocks = 64;
+
+// Collision kernel: one block per cell over its contiguous (cell-sorted) carriers.
+constexpr int kCollideThreads = 512;
+constexpr int kCollideWaves = kCollideThreads / 64;
+
+// RTA pseudo-temperature sums per cell:
+//   D      = sum hw a (n - n_BE(T0))                 exact value at the cell temperature T0
+//   M_k    = sum hw a (-x0)^k P_k(n_BE(T0)) / k!    Taylor terms of sum hw a n_BE(T) in u = T0/T - 1
+//   S0     = sum hw a n                              f(0)
+//   ESCALE = sum hw a max(1, n)
+//   UHIGH  = sum hw a (n - max(0, 1/x_high - 1/2))   upper bound of f(high) (n_BE(x) > 1/x - 1/2)
+//   BAD    = carriers whose x0 is too small for the series
+constexpr int kSeriesOrder = 6;
+enum AccCol : int {
+    kAccD = 0,
+    kAccM1 = 1, // .. kAccM1 + kSeriesOrder - 1
+    kAccS0 = 1 + kSeriesOrder,
+    kAccEscale,
+    kAccUHigh,
+    kAccBad,
+    kAccCols
+};
 
 // Per-block ledger partials (one row per block, summed by the refresh kernel).
 enum Partial : int {
@@ -41,7 +64,9 @@ enum Partial : int {
     kPAbsorbedRes = 8, // + reservoir index
     kPartialCols = 8 + kMaxReservoirs
 };
-constexpr int kTallyStride = 4; // cell tally: energy, residual, count
+// Cell tally columns: deviational energy, lifetime residual, live carriers.
+constexpr int kTallyStride = 4;
+enum TallyCol : int { kTallyEnergy = 0, kTallyCount = 1, kTallyResidual = 2 };
 
 struct FacetDev {
     double n[3];
@@ -97,18 +122,19 @@ struct DevState {
     double thermal_e;
     long long carrier_steps;
     long long rough_events, rough_specular, rough_diffuse;
-    long long newton_passes;
-    int newton_active;    // cells still iterating
+    long long newton_passes;  // exact carrier sums of the pseudo-temperature solve
+    long long newton_evals;   // all evaluations (series or exact)
     int error_code;
     long long error_slot;
     float error_pos[3];
     int error_facet;
 };
 
-// Per-cell RTA pseudo-temperature iteration state (src/solver/RtaCollisionOperator.cpp:118-165).
-struct NewtonCell {
-    double low, high, t, value, slope, s0, escale, fhigh;
-    int iter, expand, phase, done;
+// Per-cell RTA constants of one step: temperature of the refresh (series
+// centre), initial bracket top, rate-table rows/weight and the pseudo-temperature.
+struct CellRta {
+    double t0, high, gt, tr;
+    int glo, ghi, pad0, pad1;
 };
 
 struct View {
@@ -128,9 +154,10 @@ struct View {
     const float4* vel;      // x,y,z,|v| FP32
     const double* veld;     // FP64 velocity, 3 per mode
     const double* neq;
+    const double2* mtab;    // (omega, n_eq(T_bg)) per active mode
     int ntemps;
     const double* temps;
-    const double* gamma;
+    const double* gamma;    // [ntemps][na] in active-mode order
     double w, e_bg, zp;
     int lut_n;
     const double* lut_t;
@@ -151,20 +178,28 @@ struct View {
     const ResDev* res;
     int nres;
     // cells
-    double* cell_tally;   // kTallyStride per cell: energy, residual, count
+    double* cell_tally;   // kTallyStride per cell
     double* partials;     // kPartialCols per block row
     int partial_rows;     // transport blocks + injection blocks
     double* cell_e;       // energy density of the latest refresh
     double* cell_T;
-    double* cell_acc;     // RTA accumulators, 8 per cell
+    double* cell_acc;     // kAccCols RTA sums per cell
     double* cell_flux;    // 3 per cell
-    int* cell_count;
-    int* cell_cursor;
-    int* cell_glo;        // gamma interpolation row below
-    int* cell_ghi;
-    double* cell_gt;
-    NewtonCell* newton;
+    int* cell_count;      // live carriers per cell after transport and injection
+    int* cell_start;      // [ncells+1] first slot of each cell in the sorted buffer
+    int* cell_cursor;     // scatter cursors
+    CellRta* crta;
+    int* events;          // per transport block: slots with a wall event this step
+    int* block_events;    // per transport block: number of such slots
     DevState* st;
+    // destination buffer of the cell sort
+    float* o_px;
+    float* o_py;
+    float* o_pz;
+    int* o_mode;
+    double* o_occ;
+    int* o_cell;
+    unsigned* o_uid;
     // controls
     double dt;
     float dtf;
@@ -174,6 +209,8 @@ struct View {
     double lifetime_T;
     int sampling;
     int inversion;
+    int series_on;        // evaluate pseudo-temperature sums from the per-cell series
+    double series_radius; // largest |T0/T - 1| at which the series is tried
 };
 
 } // namespace pmc_dev

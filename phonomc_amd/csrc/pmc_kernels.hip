@@ -91,22 +91,36 @@ __device__ __forceinline__ float4 carrier_velocity(const View& V, int mw) {
     return v;
 }
 
-__device__ __forceinline__ double lerp_gamma(const View& V, int cell, int flat) {
-    // src/material/Thermodynamics.cpp:108-133 with the interpolation row and
-    // weight precomputed per cell.
-    const int lo = V.cell_glo[cell], hi = V.cell_ghi[cell];
-    const double t = V.cell_gt[cell];
-    const double y0 = V.gamma[static_cast<size_t>(lo) * V.nflat + flat];
-    if (lo == hi) return y0;
-    const double y1 = V.gamma[static_cast<size_t>(hi) * V.nflat + flat];
-    return y0 * (1.0 - t) + y1 * t;
-}
-
-__device__ __forceinline__ double relax_weight(const View& V, int cell, int m) {
-    const double g = lerp_gamma(V, cell, V.active_flat[m]);
+__device__ __forceinline__ double relax_weight(const View& V, const CellRta& cr, int m) {
+    // PhononMaterial::mode_relaxation_weight (src/material/Thermodynamics.cpp:95-106)
+    // with mode_gamma's interpolation (:108-133); row and weight are per cell.
+    const double y0 = V.gamma[static_cast<size_t>(cr.glo) * V.na + m];
+    double g = y0;
+    if (cr.ghi != cr.glo) {
+        const double y1 = V.gamma[static_cast<size_t>(cr.ghi) * V.na + m];
+        g = y0 * (1.0 - cr.gt) + y1 * cr.gt;
+    }
     if (g == 0.0 || V.dt == 0.0) return 0.0;
     return -expm1(-(g * V.dt) * (4.0 * kPi));
 }
+
+// Coefficients of P_k(n) = sum_j c[k][j] n^j with P_0 = n and
+// P_{k+1} = n (n + 1) dP_k/dn, so that d^k n_BE / d beta^k = (-hw/kB)^k P_k(n_BE).
+struct BoseSeries {
+    double c[kSeriesOrder + 1][kSeriesOrder + 2];
+};
+constexpr BoseSeries make_bose_series() {
+    BoseSeries s{};
+    s.c[0][1] = 1.0;
+    for (int k = 0; k < kSeriesOrder; ++k)
+        for (int j = 1; j <= k + 1; ++j) {
+            const double d = j * s.c[k][j]; // coefficient of n^(j-1) in dP_k/dn
+            s.c[k + 1][j] += d;             // times n
+            s.c[k + 1][j + 1] += d;         // times n^2
+        }
+    return s;
+}
+constexpr BoseSeries kBoseSeries = make_bose_series();
 
 // ---------------------------------------------------------------- wave reductions
 __device__ __forceinline__ double wave_sum(double x) {
@@ -422,6 +436,95 @@ __device__ __forceinline__ void record_events(BlockPartials& P, const Tally& T) 
     }
 }
 
+// Claims destination slots in cell order: the lanes of one wave that share a
+// key take consecutive slots with one atomic on that key's cursor.  All lanes
+// of the wave must call it; returns -1 for inactive lanes.
+__device__ __forceinline__ int seg_reserve(bool active, int key, int* cursor) {
+    const int lane = __lane_id();
+    uint64_t pending = __ballot(active);
+    int dst = -1;
+    while (pending) {
+        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        const int lkey = __shfl(key, leader, 64);
+        const bool mine = ((pending >> lane) & 1ull) && key == lkey;
+        const uint64_t mask = __ballot(mine);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cursor[lkey], __popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (mine) dst = base + __popcll(mask & ((1ull << lane) - 1ull));
+        pending &= ~mask;
+    }
+    return dst;
+}
+
+// ---------------------------------------------------------------- per-block cell tables
+// Open-addressing table of the cells one block touches (LDS).  Carriers of a
+// cell-sorted block fall into a handful of cells, so counts and slot
+// reservations reach global memory once per (block, cell) instead of once per
+// carrier or row.  A cell that finds no entry within kHashProbes probes falls
+// back to a direct global atomic.
+constexpr int kHashSlots = 1024;
+constexpr int kHashProbes = 32;
+
+struct BlockHash {
+    int* key;
+    int* cnt;
+    int* base;
+    __device__ void init() {
+        for (int t = threadIdx.x; t < kHashSlots; t += blockDim.x) {
+            key[t] = -1;
+            cnt[t] = 0;
+        }
+        __syncthreads();
+    }
+    __device__ int slot(int cell) {
+        unsigned s = (static_cast<unsigned>(cell) * 2654435761u) >> 22;
+        for (int p = 0; p < kHashProbes; ++p) {
+            const int k = key[s];
+            if (k == cell) return static_cast<int>(s);
+            if (k == -1) {
+                const int old = atomicCAS(&key[s], -1, cell);
+                if (old == -1 || old == cell) return static_cast<int>(s);
+            }
+            s = (s + 1) & (kHashSlots - 1);
+        }
+        return -1;
+    }
+    // Adds the wave's active lanes to their cells' counts (all lanes call).
+    __device__ void count(bool active, int cell, int* global_count) {
+        int s = -1;
+        if (active) {
+            s = slot(cell);
+            if (s < 0) {
+                atomicAdd(&global_count[cell], 1);
+                active = false;
+            }
+        }
+        const int lane = __lane_id();
+        uint64_t pending = __ballot(active);
+        while (pending) {
+            const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+            const int ls = __shfl(s, leader, 64);
+            const uint64_t mask = __ballot(((pending >> lane) & 1ull) && s == ls);
+            if (lane == leader) atomicAdd(&cnt[ls], __popcll(mask));
+            pending &= ~mask;
+        }
+    }
+    // Adds the block's counts to the global per-cell counts.
+    __device__ void flush_counts(int* global_count) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < kHashSlots; t += blockDim.x)
+            if (key[t] >= 0 && cnt[t] > 0) atomicAdd(&global_count[key[t]], cnt[t]);
+    }
+};
+
+#define PMC_BLOCK_HASH(H)                                                                                              \
+    __shared__ int H##_key[kHashSlots];                                                                                \
+    __shared__ int H##_cnt[kHashSlots];                                                                                \
+    __shared__ int H##_base[kHashSlots];                                                                               \
+    BlockHash H{H##_key, H##_cnt, H##_base};                                                                           \
+    H.init()
+
 __global__ void k_begin(View V) {
     DevState* st = V.st;
     const int t = threadIdx.x;
@@ -429,7 +532,6 @@ __global__ void k_begin(View V) {
     if (t == 0) {
         st->n_slots_start = st->n_slots;
         st->carrier_steps += st->n_live;
-        st->newton_active = 0;
     }
     __syncthreads();
     if (t == 0) {
@@ -439,33 +541,126 @@ __global__ void k_begin(View V) {
     }
 }
 
+// Free flight of every carrier that reaches no wall within the step: the first
+// iteration of advance() when it ends without an event, bit for bit.  Carriers
+// that do reach a wall are listed per block for k_events and left untouched.
+// Counts carriers per new cell for the cell sort.
 __global__ void __launch_bounds__(256) k_transport(View V) {
+    __shared__ int ev_n;
+    if (threadIdx.x == 0) ev_n = 0;
+    PMC_BLOCK_HASH(H);
+    const int n = V.st->n_slots_start;
+    const int blk = xcd_block();
+    const int base = (blk * kWavesPerBlock + (threadIdx.x >> 6)) * 64 * kRowsPerWave;
+    const int lane = __lane_id();
+    int* queue = V.events + static_cast<size_t>(blk) * kSlotsPerBlock;
+    const float dt = V.dtf;
+    const bool moves = dt > 1e-14f;
+    for (int r0 = 0; r0 < kRowsPerWave; r0 += kUnroll) {
+        if (base + r0 * 64 >= n) break;
+        int mw[kUnroll];
+        float x[kUnroll], y[kUnroll], z[kUnroll];
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) {
+            const int i = base + (r0 + j) * 64 + lane;
+            mw[j] = i < n ? V.mode[i] : -1;
+        }
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) {
+            const int i = base + (r0 + j) * 64 + lane;
+            x[j] = y[j] = z[j] = 0.0f;
+            if (mw[j] >= 0) {
+                x[j] = V.px[i];
+                y[j] = V.py[i];
+                z[j] = V.pz[i];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) {
+            const int i = base + (r0 + j) * 64 + lane;
+            const bool alive = mw[j] >= 0;
+            bool event = false;
+            int cell = 0;
+            if (alive) {
+                float p[3] = {x[j], y[j], z[j]};
+                if (moves) {
+                    const float4 v = carrier_velocity(V, mw[j]);
+                    const float vv[3] = {v.x, v.y, v.z};
+                    float tbest = INFINITY;
+                    int plane = -1;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        if (vv[a] != 0.0f) {
+                            const int side = vv[a] > 0.0f ? 1 : 0;
+                            const float target = side ? V.fbmax[a] : V.fbmin[a];
+                            float t = (target - p[a]) / vv[a];
+                            t = t > 0.0f ? t : 0.0f;
+                            if (t < tbest) {
+                                tbest = t;
+                                plane = 2 * a + side;
+                            }
+                        }
+                    }
+                    if (plane >= 0 && !(tbest > dt)) {
+                        event = true;
+                    } else {
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            const float q = fmaf(vv[a], dt, p[a]);
+                            p[a] = fminf(fmaxf(q, V.fbmin[a]), V.fbmax[a]);
+                        }
+                    }
+                }
+                if (!event) {
+                    V.px[i] = p[0];
+                    V.py[i] = p[1];
+                    V.pz[i] = p[2];
+                    cell = grid_index(V, p[0], p[1], p[2]);
+                    V.cell[i] = cell;
+                }
+            }
+            const uint64_t em = __ballot(event);
+            if (em) {
+                const int leader = __ffsll(static_cast<long long>(em)) - 1;
+                int b = 0;
+                if (lane == leader) b = atomicAdd(&ev_n, __popcll(em));
+                b = __shfl(b, leader, 64);
+                if (event) queue[b + __popcll(em & ((1ull << lane) - 1ull))] = i;
+            }
+            H.count(alive && !event, cell, V.cell_count);
+        }
+    }
+    H.flush_counts(V.cell_count);
+    if (threadIdx.x == 0) V.block_events[blk] = ev_n;
+}
+
+// Carriers with wall events this step: the whole event loop of
+// TransportOperator::advance_particle (src/solver/TransportOperator.cpp:405-466)
+// for the slots k_transport listed.  One block per transport block.
+__global__ void __launch_bounds__(256) k_events(View V) {
     __shared__ double sp[kPartialCols];
     BlockPartials P{sp};
     P.zero();
-    const int n = V.st->n_slots_start;
-    const int base = wave_global() * 64 * kRowsPerWave;
-    const int lane = __lane_id();
-    SegAcc<1, kTallyStride> acc;
-    acc.reset();
-    for (int r = 0; r < kRowsPerWave; ++r) {
-        const int row = base + r * 64;
-        if (row >= n) break;
-        const int i = row + lane;
-        int mw = i < n ? V.mode[i] : -1;
-        const bool alive0 = mw >= 0;
+    PMC_BLOCK_HASH(H);
+    const int blk = xcd_block();
+    const int cnt = V.block_events[blk];
+    const int* queue = V.events + static_cast<size_t>(blk) * kSlotsPerBlock;
+    for (int k0 = 0; k0 < cnt; k0 += blockDim.x) {
+        const int k = k0 + threadIdx.x;
         Tally T{};
         T.absorbed_res = -1;
         bool alive = false;
         int cell = 0;
-        double e[1] = {0.0};
-        if (alive0) {
+        if (k < cnt) {
+            const int i = queue[k];
+            int mw = V.mode[i];
             float p[3] = {V.px[i], V.py[i], V.pz[i]};
             double occ = V.occ[i];
             bool mc = false, oc = false;
             const int rc = advance(V, p, mw, occ, V.uid[i], V.dtf, T, mc, oc);
             if (rc < 0) {
                 raise_error(V, -rc, i, p, -1);
+                V.mode[i] = -1; // keeps the cell counts equal to the live slots
             } else if (rc == 1) {
                 V.mode[i] = -1;
             } else {
@@ -477,16 +672,14 @@ __global__ void __launch_bounds__(256) k_transport(View V) {
                 if (oc) V.occ[i] = occ;
                 cell = grid_index(V, p[0], p[1], p[2]);
                 V.cell[i] = cell;
-                const int m = mw & kModeMask;
-                e[0] = (kHbar * V.omega[m] * (occ - V.neq[m])) * V.w;
                 if (!isfinite(occ)) raise_error(V, PMC_ERR_INVALID_STATE, i, p, -1);
             }
             record_events(P, T);
         }
-        acc.add(alive, cell, e, V.cell_tally);
+        H.count(alive, cell, V.cell_count);
     }
-    acc.flush(V.cell_tally);
-    P.write(V, blockIdx.x);
+    H.flush_counts(V.cell_count);
+    P.write(V, blk);
 }
 
 // Reservoir injection (ReservoirInjector::inject, src/solver/ReservoirInjector.cpp:62-132)
@@ -496,12 +689,11 @@ __global__ void __launch_bounds__(256) k_inject(View V, int row0) {
     __shared__ double sp[kPartialCols];
     BlockPartials P{sp};
     P.zero();
+    PMC_BLOCK_HASH(H);
     DevState* st = V.st;
     const int total = st->emit_total;
     const int base_slot = st->n_slots_start;
     const uint32_t step = static_cast<uint32_t>(st->step);
-    SegAcc<1, kTallyStride> acc;
-    acc.reset();
     for (int k0 = blockIdx.x * blockDim.x; k0 < total; k0 += gridDim.x * blockDim.x) {
         const int k = k0 + threadIdx.x;
         const bool active = k < total;
@@ -509,7 +701,6 @@ __global__ void __launch_bounds__(256) k_inject(View V, int row0) {
         T.absorbed_res = -1;
         bool alive = false;
         int cell = 0;
-        double e[1] = {0.0};
         const int slot = base_slot + k;
         if (active && slot >= V.capacity) {
             raise_error(V, PMC_ERR_CAPACITY, slot, nullptr, -1);
@@ -584,14 +775,12 @@ __global__ void __launch_bounds__(256) k_inject(View V, int row0) {
                 V.mode[slot] = mw;
                 V.occ[slot] = occ;
                 V.cell[slot] = cell;
-                const int mm = mw & kModeMask;
-                e[0] = (kHbar * V.omega[mm] * (occ - V.neq[mm])) * V.w;
             }
             record_events(P, T);
         }
-        acc.add(alive, cell, e, V.cell_tally);
+        H.count(alive, cell, V.cell_count);
     }
-    acc.flush(V.cell_tally);
+    H.flush_counts(V.cell_count);
     P.write(V, row0 + blockIdx.x);
 }
 
@@ -642,16 +831,22 @@ __device__ double block_sum(double x, double* red) {
     return r;
 }
 
-// Cell energy density -> temperature (GridStatistics::update_energy_density and
-// refresh_temperatures, src/solver/GridStatistics.cpp:95-235).  Phase 1 also
-// reduces the transport/injection ledger rows (BoundaryLedger::finish_transport,
-// src/solver/BoundaryLedger.cpp:86-115) and sets up the RTA bracket of
-// src/solver/RtaCollisionOperator.cpp:118-133; phase 2 closes the energy ledger.
-__global__ void k_refresh(View V, int phase) {
+// Ledger rows, carrier counts -> cell offsets, and cell energies -> temperatures
+// (GridStatistics::update_energy_density / refresh_temperatures,
+// src/solver/GridStatistics.cpp:95-235; bucket offsets of
+// CellParticleIndex::ensure, src/solver/CellParticleIndex.cpp:14-44).
+//   phase 0: initial counts -> offsets
+//   phase 1: transport/injection ledger rows (BoundaryLedger::finish_transport,
+//            src/solver/BoundaryLedger.cpp:86-115) + counts -> offsets
+//   phase 2: cell energies after relaxation -> temperatures, energy ledger closure
+//   phase 3: initial cell energies -> temperatures and thermal energy
+__global__ void __launch_bounds__(1024) k_refresh(View V, int phase) {
     DevState* st = V.st;
     __shared__ double red[1024];
+    __shared__ int scan[1024];
     if (phase == 1) {
         for (int c = 0; c < kPartialCols; ++c) {
+            if (c == kPAbsorbedRes + V.nres) break;
             double x = 0.0;
             for (int r = threadIdx.x; r < V.partial_rows; r += blockDim.x) x += V.partials[r * kPartialCols + c];
             const double s = block_sum(x, red);
@@ -671,45 +866,55 @@ __global__ void k_refresh(View V, int phase) {
             }
         }
     }
-    double local = 0.0, residual = 0.0;
-    for (int c = threadIdx.x; c < V.ncells; c += blockDim.x) {
-        double* tally = V.cell_tally + static_cast<size_t>(kTallyStride) * c;
-        const double e = tally[0] / V.cell_vol[c] + V.e_bg;
-        residual += tally[1];
-        tally[0] = 0.0;
-        tally[1] = 0.0;
-        tally[2] = 0.0;
-        V.cell_e[c] = e;
-        const double T = lut_inverse(V, e);
-        V.cell_T[c] = T;
-        local += (e - V.zp) * V.cell_vol[c];
-        if (phase == 1) {
-            const double tl = V.lifetime_local ? T : V.lifetime_T;
-            int lo, hi;
-            double t;
-            gamma_row(V, tl, lo, hi, t);
-            V.cell_glo[c] = lo;
-            V.cell_ghi[c] = hi;
-            V.cell_gt[c] = t;
-            NewtonCell nc{};
-            nc.low = 0.0;
-            nc.high = fmax(1000.0, 2.0 * fmax(1.0, T));
-            nc.t = T < nc.low ? nc.low : (T > nc.high ? nc.high : T);
-            V.newton[c] = nc;
-            for (int k = 0; k < 8; ++k) V.cell_acc[8 * c + k] = 0.0;
+    const int nc = V.ncells;
+    const int per = (nc + static_cast<int>(blockDim.x) - 1) / static_cast<int>(blockDim.x);
+    const int c0 = min(nc, static_cast<int>(threadIdx.x) * per), c1 = min(nc, c0 + per);
+    if (phase <= 1) {
+        int mine = 0;
+        for (int c = c0; c < c1; ++c) mine += V.cell_count[c];
+        scan[threadIdx.x] = mine;
+        __syncthreads();
+        for (int off = 1; off < static_cast<int>(blockDim.x); off <<= 1) {
+            const int add = static_cast<int>(threadIdx.x) >= off ? scan[threadIdx.x - off] : 0;
+            __syncthreads();
+            scan[threadIdx.x] += add;
+            __syncthreads();
         }
+        int offset = scan[threadIdx.x] - mine;
+        const int total = scan[blockDim.x - 1];
+        for (int c = c0; c < c1; ++c) {
+            V.cell_start[c] = offset;
+            V.cell_cursor[c] = offset;
+            offset += V.cell_count[c];
+            V.cell_count[c] = 0;
+        }
+        if (threadIdx.x == 0) {
+            V.cell_start[nc] = total;
+            if (phase == 1) {
+                const int expect =
+                    st->n_live - static_cast<int>(st->step_absorbed) + static_cast<int>(st->step_injected);
+                if (expect != total) raise_error(V, PMC_ERR_INVALID_STATE, total, nullptr, -3);
+                const int grown = st->n_slots_start + st->emit_total;
+                st->n_slots = grown < V.capacity ? grown : V.capacity;
+                st->next_uid += static_cast<unsigned>(st->emit_total);
+            }
+            st->n_live = total;
+        }
+        return;
+    }
+    double local = 0.0, residual = 0.0;
+    for (int c = c0; c < c1; ++c) {
+        double* tally = V.cell_tally + static_cast<size_t>(kTallyStride) * c;
+        const double e = tally[kTallyEnergy] / V.cell_vol[c] + V.e_bg;
+        residual += tally[kTallyResidual];
+        V.cell_e[c] = e;
+        V.cell_T[c] = lut_inverse(V, e);
+        local += (e - V.zp) * V.cell_vol[c];
     }
     const double energy = block_sum(local, red);
     const double res = block_sum(residual, red);
     if (threadIdx.x == 0) {
-        if (phase == 0) st->thermal_e = energy;
-        if (phase == 1) {
-            st->newton_active = V.ncells;
-            st->n_live = st->n_live - static_cast<int>(st->step_absorbed) + static_cast<int>(st->step_injected);
-            const int grown = st->n_slots_start + st->emit_total;
-            st->n_slots = grown < V.capacity ? grown : V.capacity;
-            st->next_uid += static_cast<unsigned>(st->emit_total);
-        }
+        if (phase == 3) st->thermal_e = energy;
         if (phase == 2) {
             // EnergyLedger::finish_step (src/solver/EnergyLedger.cpp:58-90) and
             // BoundaryLedger::commit_step (src/solver/BoundaryLedger.cpp:117-141).
@@ -725,203 +930,343 @@ __global__ void k_refresh(View V, int phase) {
             st->thermal_e = energy;
             st->step += 1;
         }
+        st->n_slots = st->n_live;
     }
 }
 
-// One pass of the per-cell pseudo-temperature iteration: evaluates
-// f(T) = sum hw*a*(n - n_BE(T)) and f'(T) at each cell's current candidate
-// (the evaluate lambda of src/solver/RtaCollisionOperator.cpp:91-116).  The first
-// pass also accumulates f(0), f(high) and the energy scale.
-__global__ void __launch_bounds__(256) k_rta_pass(View V, int first) {
-    DevState* st = V.st;
-    if (st->newton_active == 0) return;
-    const int n = st->n_slots;
+// Counting-sort scatter of the live carriers into cell order (destination
+// buffer), dropping absorbed slots: ParticleStorage::compact_alive
+// (src/solver/ParticleStorage.cpp:75-105) and the cell buckets of
+// CellParticleIndex in one pass.  Each block ranks its carriers per cell in
+// LDS, reserves one destination range per (block, cell) from the global cell
+// cursor, then streams the carriers out.
+__global__ void __launch_bounds__(256) k_sort(View V) {
+    PMC_BLOCK_HASH(H);
+    const int n = V.st->n_slots;
     const int base = wave_global() * 64 * kRowsPerWave;
     const int lane = __lane_id();
-    SegAcc<5, 8> acc5;
-    SegAcc<2, 8> acc2;
-    acc5.reset();
-    acc2.reset();
+    int tok[kRowsPerWave]; // (hash slot << 16 | rank), -2 - destination, or -1 (dead)
+#pragma unroll
     for (int r = 0; r < kRowsPerWave; ++r) {
-        const int row = base + r * 64;
-        if (row >= n) break;
-        const int i = row + lane;
+        const int i = base + r * 64 + lane;
         const int mw = i < n ? V.mode[i] : -1;
         bool active = mw >= 0;
-        int cell = 0;
-        double vals[5] = {0, 0, 0, 0, 0};
+        const int cell = active ? V.cell[i] : 0;
+        int s = -1;
+        tok[r] = -1;
         if (active) {
-            cell = V.cell[i];
-            const NewtonCell& nc = V.newton[cell];
-            active = !nc.done;
-            if (active) {
-                const int m = mw & kModeMask;
-                const double w = fmax(0.0, V.omega[m]);
-                const double hw = kHbar * w;
-                const double a = relax_weight(V, cell, m);
-                const double occ = V.occ[i];
-                if (hw > 0.0 && a > 0.0) {
-                    const double T = fmax(0.0, nc.t);
-                    const double neq = bose(T, w);
-                    vals[0] = hw * a * (occ - neq);
-                    if (T > 0.0) vals[1] = -(hw * a * (hw * neq * (neq + 1.0) / (kBoltz * T * T)));
-                    if (first) {
-                        vals[2] = hw * a * occ;                      // f(0)
-                        vals[3] = hw * a * (occ - bose(nc.high, w)); // f(high)
-                        vals[4] = hw * a * fmax(1.0, occ);           // energy scale
-                    }
-                } else {
-                    active = false;
-                }
+            s = H.slot(cell);
+            if (s < 0) {
+                tok[r] = -2 - atomicAdd(&V.cell_cursor[cell], 1);
+                active = false;
             }
         }
-        if (first) {
-            acc5.add(active, cell, vals, V.cell_acc);
-        } else {
-            const double v2[2] = {vals[0], vals[1]};
-            acc2.add(active, cell, v2, V.cell_acc);
+        uint64_t pending = __ballot(active);
+        while (pending) {
+            const int leader = __ffsll(static_cast<long long>(pending)) - 1;
+            const int ls = __shfl(s, leader, 64);
+            const bool mine = ((pending >> lane) & 1ull) && s == ls;
+            const uint64_t mask = __ballot(mine);
+            int b = 0;
+            if (lane == leader) b = atomicAdd(&H.cnt[ls], __popcll(mask));
+            b = __shfl(b, leader, 64);
+            if (mine) tok[r] = (ls << 16) | (b + __popcll(mask & ((1ull << lane) - 1ull)));
+            pending &= ~mask;
         }
     }
-    if (first) acc5.flush(V.cell_acc);
-    else acc2.flush(V.cell_acc);
-}
-
-// Per-cell bracket/Newton update: one step of the loop at
-// src/solver/RtaCollisionOperator.cpp:118-165 per pass.
-__global__ void k_rta_update(View V) {
-    DevState* st = V.st;
-    if (st->newton_active == 0) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= V.ncells) return;
-    NewtonCell nc = V.newton[c];
-    if (nc.done) return;
-    double* acc = V.cell_acc + 8 * static_cast<size_t>(c);
-    const double value = acc[0], slope = acc[1];
-    bool run_iter = false;
-    if (nc.phase == 0) {
-        nc.s0 = acc[2];
-        nc.fhigh = acc[3];
-        nc.escale = acc[4];
-        if (nc.fhigh > 0.0 && nc.expand < 20) {
-            nc.expand++;
-            nc.high *= 2.0;
-            nc.t = nc.high;
-            nc.phase = 2;
-        } else if (nc.s0 <= 0.0) {
-            nc.t = nc.low;
-            nc.done = 1;
-        } else if (nc.fhigh >= 0.0) {
-            nc.t = nc.high;
-            nc.done = 1;
-        } else {
-            nc.phase = 1;
-            run_iter = true; // value and slope at the clamped cell temperature
-        }
-    } else if (nc.phase == 2) {
-        nc.fhigh = value;
-        if (nc.fhigh > 0.0 && nc.expand < 20) {
-            nc.expand++;
-            nc.high *= 2.0;
-            nc.t = nc.high;
-        } else if (nc.s0 <= 0.0) {
-            nc.t = nc.low;
-            nc.done = 1;
-        } else if (nc.fhigh >= 0.0) {
-            nc.t = nc.high;
-            nc.done = 1;
-        } else {
-            const double T = V.cell_T[c];
-            nc.t = T < nc.low ? nc.low : (T > nc.high ? nc.high : T);
-            nc.phase = 1; // evaluated by the next pass
-        }
-    } else {
-        run_iter = true;
-    }
-    if (run_iter) {
-        if (fabs(value) <= 1e-13 * fmax(1.0, nc.escale)) {
-            nc.done = 1;
-        } else {
-            if (value > 0.0) nc.low = nc.t;
-            else nc.high = nc.t;
-            double cand = 0.5 * (nc.low + nc.high);
-            if (slope < 0.0 && isfinite(slope)) {
-                const double newton = nc.t - value / slope;
-                if (newton > nc.low && newton < nc.high && isfinite(newton)) cand = newton;
-            }
-            nc.t = cand;
-            nc.iter++;
-            if (nc.iter >= 40) nc.done = 1;
-        }
-    }
-    acc[0] = 0.0;
-    acc[1] = 0.0;
-    V.newton[c] = nc;
-}
-
-// Counts unfinished cells after an update.
-__global__ void k_rta_count(View V) {
-    DevState* st = V.st;
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
-    if (st->newton_active == 0) return;
-    for (int c = threadIdx.x; c < V.ncells; c += blockDim.x)
-        if (!V.newton[c].done) atomicAdd(&cnt, 1);
+    for (int t = threadIdx.x; t < kHashSlots; t += blockDim.x)
+        if (H.key[t] >= 0 && H.cnt[t] > 0) H.base[t] = atomicAdd(&V.cell_cursor[H.key[t]], H.cnt[t]);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        st->newton_passes += 1;
-        st->newton_active = cnt;
-    }
-}
-
-// Relaxation n <- max(0, n + a (n_BE(T_R) - n)) and residual
-// (src/solver/RtaCollisionOperator.cpp:167-183), fused with the energy tally of
-// the following temperature refresh.
-__global__ void __launch_bounds__(256) k_relax(View V) {
-    DevState* st = V.st;
-    const int n = st->n_slots;
-    const int base = wave_global() * 64 * kRowsPerWave;
-    const int lane = __lane_id();
-    SegAcc<3, kTallyStride> acc;
-    acc.reset();
+#pragma unroll
     for (int r = 0; r < kRowsPerWave; ++r) {
-        const int row = base + r * 64;
-        if (row >= n) break;
-        const int i = row + lane;
-        const int mw = i < n ? V.mode[i] : -1;
-        const bool alive = mw >= 0;
-        int cell = 0;
-        double vals[3] = {0.0, 0.0, 0.0};
-        if (alive) {
-            cell = V.cell[i];
-            const NewtonCell& nc = V.newton[cell];
-            if (!nc.done) raise_error(V, PMC_ERR_INVALID_STATE, cell, nullptr, -2);
-            const int m = mw & kModeMask;
-            const double w = fmax(0.0, V.omega[m]);
+        if (tok[r] == -1) continue;
+        const int i = base + r * 64 + lane;
+        const int dst = tok[r] >= 0 ? H.base[tok[r] >> 16] + (tok[r] & 0xffff) : -2 - tok[r];
+        V.o_px[dst] = V.px[i];
+        V.o_py[dst] = V.py[i];
+        V.o_pz[dst] = V.pz[i];
+        V.o_mode[dst] = V.mode[i];
+        V.o_occ[dst] = V.occ[i];
+        V.o_cell[dst] = V.cell[i];
+        V.o_uid[dst] = V.uid[i];
+    }
+}
+
+// Sums v[k] over the block; every thread receives the same totals (wave sums
+// combined in wave order).
+template <int K>
+__device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[k * kCollideWaves + wave] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < kCollideWaves; ++w) s += lds[k * kCollideWaves + w];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+// Adds one carrier's terms of the pseudo-temperature sums (the evaluate lambda
+// of src/solver/RtaCollisionOperator.cpp:91-116 and its f(0), f(high) and
+// energy-scale sums, :118-146), expanded around the cell temperature T0:
+// n_BE(T) = n0 + sum_k (-x0)^k P_k(n0)/k! u^k with u = T0/T - 1, x0 = hw/(kB T0).
+// Carriers the reference skips (hw <= 0 or a <= 0) add nothing.
+__device__ __forceinline__ void rta_accumulate(const View& V, const CellRta& cr, int m, double n,
+                                               double (&acc)[kAccCols]) {
+    const double w = fmax(0.0, V.mtab[m].x);
+    const double hw = kHbar * w;
+    const double a = relax_weight(V, cr, m);
+    if (!(hw > 0.0) || !(a > 0.0)) return;
+    const double g = hw * a;
+    const double t0 = fmax(0.0, cr.t0);
+    const double n0 = bose(t0, w);
+    acc[kAccD] += g * (n - n0);
+    if (t0 > 0.0) {
+        const double x0 = hw / (kBoltz * t0);
+        if (x0 >= 1e-6) {
+            double c = g;
+#pragma unroll
+            for (int k = 1; k <= kSeriesOrder; ++k) {
+                c *= -x0 / k;
+                double p = kBoseSeries.c[k][k + 1];
+#pragma unroll
+                for (int j = k; j >= 1; --j) p = p * n0 + kBoseSeries.c[k][j];
+                acc[kAccM1 + k - 1] += c * (p * n0);
+            }
+        } else {
+            acc[kAccBad] += 1.0;
+        }
+    }
+    acc[kAccS0] += g * n;
+    acc[kAccEscale] += g * fmax(1.0, n);
+    const double xh = hw / (kBoltz * cr.high);
+    const double lb = xh > 1e-9 ? fmax(0.0, 1.0 / xh - 0.5) : 0.0;
+    acc[kAccUHigh] += g * (n - lb);
+}
+
+// Exact pseudo-temperature sums of one cell over its carriers (the evaluate
+// lambda of src/solver/RtaCollisionOperator.cpp:91-116), whole block.
+__device__ void exact_sums(const View& V, const CellRta& cr, int begin, int end, double t, double& value,
+                           double& slope, double* lds) {
+    const double T = fmax(0.0, t);
+    double r[2] = {0.0, 0.0};
+    for (int i = begin + static_cast<int>(threadIdx.x); i < end; i += blockDim.x) {
+        const int m = V.mode[i] & kModeMask;
+        const double w = fmax(0.0, V.mtab[m].x);
+        const double hw = kHbar * w;
+        const double a = relax_weight(V, cr, m);
+        if (!(hw > 0.0) || !(a > 0.0)) continue;
+        const double neq = bose(T, w);
+        r[0] += hw * a * (V.occ[i] - neq);
+        if (T > 0.0) r[1] -= hw * a * (hw * neq * (neq + 1.0) / (kBoltz * T * T));
+    }
+    block_reduce<2>(r, lds);
+    value = r[0];
+    slope = r[1];
+}
+
+// Pseudo-temperature sums from the per-cell series at temperature t.  Returns
+// false (caller sums exactly) outside |u| <= radius or when the geometric tail
+// estimate of the truncated terms exceeds 1e-3 of the convergence tolerance.
+__device__ __forceinline__ bool series_sums(const double (&acc)[kAccCols], double t0, double t, double tol,
+                                            double radius, double& value, double& slope) {
+    if (!(t > 0.0)) return false;
+    const double u = t0 / t - 1.0;
+    if (!(fabs(u) <= radius)) return false;
+    double sum = 0.0, dsum = 0.0, up = 1.0, last = 0.0, prev = 0.0;
+#pragma unroll
+    for (int k = 1; k <= kSeriesOrder; ++k) {
+        const double mk = acc[kAccM1 + k - 1];
+        dsum += k * mk * up;
+        const double term = mk * up * u;
+        sum += term;
+        prev = last;
+        last = term;
+        up *= u;
+    }
+    if (last != 0.0) {
+        const double rho = fabs(last) / fabs(prev);
+        if (!(rho <= 0.5)) return false;
+        if (!(fabs(last) * rho / (1.0 - rho) <= 1e-3 * tol)) return false;
+    }
+    value = acc[kAccD] - sum;
+    slope = (t0 / (t * t)) * dsum;
+    return true;
+}
+
+// Collision step of one cell over its contiguous carriers in the cell-sorted
+// buffer, one block per cell, all sums deterministic:
+//  0. deviational energy -> cell temperature T0 (GridStatistics::refresh_temperatures);
+//  1. pseudo-temperature sums at T0 (exact value, series terms, f(0), bound of f(high), scale);
+//  2. bracket + safeguarded Newton for T_R (src/solver/RtaCollisionOperator.cpp:118-165),
+//     from the series where trusted, else exact carrier sums;
+//  3. relaxation n <- max(0, n + a (n_BE(T_R) - n)) with residual (:167-183) and the
+//     energy tally of the following refresh.
+// With full == 0 only step 0 runs (initial refresh): the cell energy is tallied.
+__global__ void __launch_bounds__(kCollideThreads) k_collide(View V, int full) {
+    __shared__ double lds[kAccCols * kCollideWaves];
+    const int c = blockIdx.x;
+    const int begin = V.cell_start[c], end = V.cell_start[c + 1];
+    const int stride = blockDim.x;
+    double* tally = V.cell_tally + static_cast<size_t>(kTallyStride) * c;
+    double e[1] = {0.0};
+    for (int i0 = begin + static_cast<int>(threadIdx.x); i0 < end; i0 += stride * kUnroll) {
+        int m[kUnroll];
+        double n[kUnroll];
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) {
+            const int i = i0 + j * stride;
+            m[j] = -1;
+            n[j] = 0.0;
+            if (i < end) {
+                m[j] = V.mode[i] & kModeMask;
+                n[j] = V.occ[i];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) {
+            if (m[j] < 0) continue;
+            const double2 mt = V.mtab[m[j]];
+            e[0] += (kHbar * mt.x * (n[j] - mt.y)) * V.w;
+        }
+    }
+    block_reduce<1>(e, lds);
+    if (!full || end <= begin) {
+        if (threadIdx.x == 0) {
+            tally[kTallyEnergy] = e[0];
+            tally[kTallyResidual] = 0.0;
+        }
+        return;
+    }
+    const double T = lut_inverse(V, e[0] / V.cell_vol[c] + V.e_bg);
+    CellRta cr{};
+    cr.t0 = T;
+    cr.high = fmax(1000.0, 2.0 * fmax(1.0, T));
+    gamma_row(V, V.lifetime_local ? T : V.lifetime_T, cr.glo, cr.ghi, cr.gt);
+
+    double acc[kAccCols];
+#pragma unroll
+    for (int k = 0; k < kAccCols; ++k) acc[k] = 0.0;
+    for (int i0 = begin + static_cast<int>(threadIdx.x); i0 < end; i0 += stride * kUnroll) {
+        int m[kUnroll];
+        double n[kUnroll];
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) {
+            const int i = i0 + j * stride;
+            m[j] = -1;
+            n[j] = 0.0;
+            if (i < end) {
+                m[j] = V.mode[i] & kModeMask;
+                n[j] = V.occ[i];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j)
+            if (m[j] >= 0) rta_accumulate(V, cr, m[j], n[j], acc);
+    }
+    block_reduce<kAccCols>(acc, lds);
+
+    const double s0 = acc[kAccS0], escale = acc[kAccEscale];
+    long long exact = 0, evals = 0;
+    double low = 0.0, high = cr.high;
+    bool known_high = false;
+    double f_high = 0.0, d = 0.0;
+    if (s0 > 0.0 && !(acc[kAccUHigh] < -1e-12 * escale)) {
+        exact_sums(V, cr, begin, end, high, f_high, d, lds);
+        ++exact;
+        for (int x = 0; f_high > 0.0 && x < 20; ++x) {
+            high *= 2.0;
+            exact_sums(V, cr, begin, end, high, f_high, d, lds);
+            ++exact;
+        }
+        known_high = true;
+    }
+    double t = T < low ? low : (high < T ? high : T);
+    if (s0 <= 0.0) {
+        t = low;
+    } else if (known_high && f_high >= 0.0) {
+        t = high;
+    } else {
+        const double tol = 1e-13 * fmax(1.0, escale);
+        const bool series = V.series_on && acc[kAccBad] == 0.0 && cr.t0 > 0.0;
+        for (int iter = 0; iter < 40; ++iter) {
+            double value = 0.0, slope = 0.0;
+            if (!(series && series_sums(acc, cr.t0, t, tol, V.series_radius, value, slope))) {
+                exact_sums(V, cr, begin, end, t, value, slope, lds);
+                ++exact;
+            }
+            ++evals;
+            if (fabs(value) <= tol) break;
+            if (value > 0.0) low = t;
+            else high = t;
+            double cand = 0.5 * (low + high);
+            if (slope < 0.0 && isfinite(slope)) {
+                const double nt = t - value / slope;
+                if (nt > low && nt < high && isfinite(nt)) cand = nt;
+            }
+            t = cand;
+        }
+    }
+    cr.tr = t;
+
+    double r[2] = {0.0, 0.0}; // energy after relaxation, lifetime residual
+    for (int i0 = begin + static_cast<int>(threadIdx.x); i0 < end; i0 += stride * kUnroll) {
+        int m[kUnroll];
+        double n[kUnroll];
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) {
+            const int i = i0 + j * stride;
+            m[j] = -1;
+            n[j] = 0.0;
+            if (i < end) {
+                m[j] = V.mode[i] & kModeMask;
+                n[j] = V.occ[i];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kUnroll; ++j) {
+            if (m[j] < 0) continue;
+            const int i = i0 + j * stride;
+            const double2 mt = V.mtab[m[j]];
+            const double w = fmax(0.0, mt.x);
             const double hw = kHbar * w;
-            const double a = relax_weight(V, cell, m);
-            const double old = V.occ[i];
-            const double neq = bose(nc.t, w);
+            const double a = relax_weight(V, cr, m[j]);
+            const double old = n[j];
+            const double neq = bose(t, w);
             const double nn = old + a * (neq - old);
-            vals[1] = (hw * nn - hw * old) * V.w;
             const double occ = fmax(0.0, nn);
             if (occ != old) V.occ[i] = occ;
-            vals[0] = (kHbar * V.omega[m] * (occ - V.neq[m])) * V.w;
-            vals[2] = 1.0;
+            r[0] += (kHbar * mt.x * (occ - mt.y)) * V.w;
+            r[1] += (hw * nn - hw * old) * V.w;
             if (!isfinite(occ)) raise_error(V, PMC_ERR_INVALID_STATE, i, nullptr, -1);
         }
-        acc.add(alive, cell, vals, V.cell_tally);
     }
-    acc.flush(V.cell_tally);
+    block_reduce<2>(r, lds);
+    if (threadIdx.x == 0) {
+        tally[kTallyEnergy] = r[0];
+        tally[kTallyResidual] = r[1];
+        V.crta[c] = cr;
+        if (exact)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->newton_passes),
+                      static_cast<unsigned long long>(exact));
+        if (evals)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->newton_evals),
+                      static_cast<unsigned long long>(evals));
+    }
 }
 
 __global__ void k_zero_tally(View V) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < kTallyStride * V.ncells; c += gridDim.x * blockDim.x)
         V.cell_tally[c] = 0.0;
-}
-
-__global__ void k_zero_counts(View V) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < V.ncells; c += gridDim.x * blockDim.x) V.cell_count[c] = 0;
 }
 
 __global__ void k_zero_flux(View V) {
@@ -1021,107 +1366,14 @@ __global__ void __launch_bounds__(256) k_init(View V, int count, int tmode, doub
     V.uid[i] = static_cast<unsigned>(i);
 }
 
-__global__ void __launch_bounds__(256) k_tally_all(View V) {
-    const int n = V.st->n_slots;
-    const int base = wave_global() * 64 * kRowsPerWave;
-    const int lane = __lane_id();
-    SegAcc<1, kTallyStride> acc;
-    acc.reset();
-    for (int r = 0; r < kRowsPerWave; ++r) {
-        const int row = base + r * 64;
-        if (row >= n) break;
-        const int i = row + lane;
-        const int mw = i < n ? V.mode[i] : -1;
-        const bool alive = mw >= 0;
-        int cell = 0;
-        double e[1] = {0.0};
-        if (alive) {
-            cell = V.cell[i];
-            const int m = mw & kModeMask;
-            e[0] = (kHbar * V.omega[m] * (V.occ[i] - V.neq[m])) * V.w;
-        }
-        acc.add(alive, cell, e, V.cell_tally);
-    }
-    acc.flush(V.cell_tally);
-}
-
-// ---------------------------------------------------------------- compaction
-__global__ void __launch_bounds__(256) k_histogram(View V) {
+// Live carriers per cell of an initial population (for the first cell sort).
+__global__ void __launch_bounds__(256) k_count_all(View V) {
+    PMC_BLOCK_HASH(H);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = V.st->n_slots;
     const int mw = i < n ? V.mode[i] : -1;
-    const bool alive = mw >= 0;
-    seg_count(alive, alive ? V.cell[i] : 0, V.cell_count);
-}
-
-__global__ void k_scan_cells(View V) {
-    // Exclusive scan of cell counts into cursors (single block).
-    __shared__ int partial[1024];
-    const int n = V.ncells;
-    const int per = (n + blockDim.x - 1) / blockDim.x;
-    const int begin = threadIdx.x * per;
-    int s = 0;
-    for (int c = begin; c < begin + per && c < n; ++c) s += V.cell_count[c];
-    partial[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < static_cast<int>(blockDim.x); ++t) {
-            const int x = partial[t];
-            partial[t] = acc;
-            acc += x;
-        }
-    }
-    __syncthreads();
-    int acc = partial[threadIdx.x];
-    for (int c = begin; c < begin + per && c < n; ++c) {
-        V.cell_cursor[c] = acc;
-        acc += V.cell_count[c];
-    }
-}
-
-struct Scatter {
-    float *px, *py, *pz;
-    int* mode;
-    double* occ;
-    int* cell;
-    unsigned* uid;
-};
-
-// Counting-sort scatter of live carriers into cell order (dst buffers).
-__global__ void __launch_bounds__(256) k_scatter(View V, Scatter D) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int n = V.st->n_slots;
-    const int mw = i < n ? V.mode[i] : -1;
-    const bool alive = mw >= 0;
-    const int cell = alive ? V.cell[i] : 0;
-    const int lane = __lane_id();
-    uint64_t pending = __ballot(alive);
-    int dst = -1;
-    while (pending) {
-        const int leader = __ffsll(static_cast<long long>(pending)) - 1;
-        const int lkey = __shfl(cell, leader, 64);
-        const bool mine = ((pending >> lane) & 1ull) && cell == lkey;
-        const uint64_t mask = __ballot(mine);
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&V.cell_cursor[lkey], __popcll(mask));
-        base = __shfl(base, leader, 64);
-        if (mine) dst = base + __popcll(mask & ((1ull << lane) - 1ull));
-        pending &= ~mask;
-    }
-    if (alive) {
-        D.px[dst] = V.px[i];
-        D.py[dst] = V.py[i];
-        D.pz[dst] = V.pz[i];
-        D.mode[dst] = mw;
-        D.occ[dst] = V.occ[i];
-        D.cell[dst] = cell;
-        D.uid[dst] = V.uid[i];
-    }
-}
-
-__global__ void k_after_sort(View V) {
-    if (threadIdx.x == 0) V.st->n_slots = V.st->n_live;
+    H.count(mw >= 0, mw >= 0 ? V.cell[i] : 0, V.cell_count);
+    H.flush_counts(V.cell_count);
 }
 
 } // namespace pmc_dev
@@ -1145,19 +1397,18 @@ void begin(const View& v, hipStream_t s) { hipLaunchKernelGGL(k_begin, dim3(1), 
 void transport(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_transport, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
 }
+void events(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_events, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+}
 void inject(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_inject, dim3(kInjectBlocks), dim3(256), 0, s, v, wave_blocks(v.capacity));
 }
 void refresh(const View& v, hipStream_t s, int phase) { hipLaunchKernelGGL(k_refresh, dim3(1), dim3(1024), 0, s, v, phase); }
-void rta_pass(const View& v, hipStream_t s, int first) {
-    hipLaunchKernelGGL(k_rta_pass, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v, first);
+void sort(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_sort, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
 }
-void rta_update(const View& v, hipStream_t s) {
-    hipLaunchKernelGGL(k_rta_update, dim3(cell_blocks(v.ncells)), dim3(256), 0, s, v);
-    hipLaunchKernelGGL(k_rta_count, dim3(1), dim3(1024), 0, s, v);
-}
-void relax(const View& v, hipStream_t s) {
-    hipLaunchKernelGGL(k_relax, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+void collide(const View& v, hipStream_t s, int full) {
+    hipLaunchKernelGGL(k_collide, dim3(v.ncells), dim3(kCollideThreads), 0, s, v, full);
 }
 void clear_cells(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_tally, dim3(cell_blocks(kTallyStride * v.ncells)), dim3(256), 0, s, v);
@@ -1175,16 +1426,8 @@ void init(const View& v, hipStream_t s, int count, int tmode, double t0, const d
     hipLaunchKernelGGL(k_init, dim3((count + 255) / 256), dim3(256), 0, s, v, count, tmode, t0, cold[0], cold[1],
                        cold[2], hot[0], hot[1], hot[2], tcold, thot, field);
 }
-void tally_all(const View& v, hipStream_t s) {
-    hipLaunchKernelGGL(k_tally_all, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
-}
-void sort_by_cell(const View& v, hipStream_t s, float* px, float* py, float* pz, int* mode, double* occ, int* cell,
-                  unsigned* uid) {
-    hipLaunchKernelGGL(k_zero_counts, dim3(cell_blocks(v.ncells)), dim3(256), 0, s, v);
-    hipLaunchKernelGGL(k_histogram, dim3(slot_blocks(v.capacity)), dim3(256), 0, s, v);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, v);
-    Scatter d{px, py, pz, mode, occ, cell, uid};
-    hipLaunchKernelGGL(k_scatter, dim3(slot_blocks(v.capacity)), dim3(256), 0, s, v, d);
-    hipLaunchKernelGGL(k_after_sort, dim3(1), dim3(64), 0, s, v);
+void count_all(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_count_all, dim3(slot_blocks(v.capacity)), dim3(256), 0, s, v);
 }
 } // namespace pmc_launch
+

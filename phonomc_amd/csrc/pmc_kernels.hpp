@@ -10,15 +10,13 @@ void begin(const pmc_dev::View& v, hipStream_t s);
 void transport(const pmc_dev::View& v, hipStream_t s);
 void inject(const pmc_dev::View& v, hipStream_t s);
 void refresh(const pmc_dev::View& v, hipStream_t s, int phase);
-void rta_pass(const pmc_dev::View& v, hipStream_t s, int first);
-void rta_update(const pmc_dev::View& v, hipStream_t s);
-void relax(const pmc_dev::View& v, hipStream_t s);
+void events(const pmc_dev::View& v, hipStream_t s);
+void sort(const pmc_dev::View& v, hipStream_t s);
+void collide(const pmc_dev::View& v, hipStream_t s, int full);
 void clear_cells(const pmc_dev::View& v, hipStream_t s);
 void flux(const pmc_dev::View& v, hipStream_t s);
 void containment(const pmc_dev::View& v, hipStream_t s);
 void init(const pmc_dev::View& v, hipStream_t s, int count, int tmode, double t0, const double* cold,
           const double* hot, double tcold, double thot, const double* field);
-void tally_all(const pmc_dev::View& v, hipStream_t s);
-void sort_by_cell(const pmc_dev::View& v, hipStream_t s, float* px, float* py, float* pz, int* mode, double* occ,
-                  int* cell, unsigned* uid);
+void count_all(const pmc_dev::View& v, hipStream_t s);
 } // namespace pmc_launch

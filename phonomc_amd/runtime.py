@@ -13,8 +13,7 @@ BUILD = os.path.join(PKG, "_build")
 CONFIGS = os.path.join(PKG, "configs")
 
 PMC_OK = 0
-KERNEL_CLASSES = ["begin", "transport", "inject", "refresh", "rta_first", "rta_pass", "rta_update", "relax", "sort",
-                  "step"]
+KERNEL_CLASSES = ["begin", "transport", "events", "inject", "refresh", "sort", "collide", "output", "step"]
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "excess transport events", 4: "boundary failure",
           5: "escaped carrier", 6: "invalid state", 7: "capacity exhausted", 8: "latched", 9: "unsupported"}
 
@@ -64,6 +63,10 @@ class Reservoir(C.Structure):
                 ("triangles", C.POINTER(C.c_double))]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("series_on", C.c_int32), ("series_radius", C.c_double), ("reserved", C.c_int32 * 6)]
+
+
 class Ledger(C.Structure):
     _fields_ = [("step", C.c_int64), ("live_carriers", C.c_int64), ("slots", C.c_int64),
                 ("step_absorbed", C.c_int64), ("step_injected", C.c_int64),
@@ -73,7 +76,8 @@ class Ledger(C.Structure):
                 ("total_absorbed_energy_ev", C.c_double), ("total_injected_energy_ev", C.c_double),
                 ("total_lifetime_residual_ev", C.c_double), ("total_balance_residual_ev", C.c_double),
                 ("thermal_energy_ev", C.c_double), ("carrier_steps", C.c_int64), ("rough_events", C.c_int64),
-                ("rough_specular", C.c_int64), ("rough_diffuse", C.c_int64), ("newton_passes", C.c_int64)]
+                ("rough_specular", C.c_int64), ("rough_diffuse", C.c_int64), ("newton_passes", C.c_int64),
+                ("newton_evals", C.c_int64)]
 
 
 class Error(C.Structure):
@@ -159,6 +163,8 @@ def lib():
         "pmc_kernel_times": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]),
         "pmc_reset_kernel_times": (C.c_int, [P]),
         "pmc_set_timing": (C.c_int, [P, C.c_int32]),
+        "pmc_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+        "pmc_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pmc_destroy": (None, [P]),
         "pmc_status_string": (C.c_char_p, [C.c_int32]),
     }
@@ -328,6 +334,21 @@ class Context:
 
     def set_timing(self, on=True):
         self._check(lib().pmc_set_timing(self.h, 1 if on else 0))
+
+    def tuning(self):
+        t = Tuning()
+        self._check(lib().pmc_get_tuning(self.h, C.byref(t)))
+        return {"series_on": t.series_on, "series_radius": t.series_radius}
+
+    def set_tuning(self, series_on=None, series_radius=None):
+        """Pseudo-temperature evaluation: per-cell series (default) or exact carrier sums only."""
+        t = Tuning()
+        self._check(lib().pmc_get_tuning(self.h, C.byref(t)))
+        if series_on is not None:
+            t.series_on = 1 if series_on else 0
+        if series_radius is not None:
+            t.series_radius = series_radius
+        self._check(lib().pmc_set_tuning(self.h, C.byref(t)))
 
     def kernel_times(self):
         """Per-class (total ms, launches) since the last reset."""

@@ -20,9 +20,10 @@ def by_uid(c):
     return {k: v[live][order] for k, v in c.items()}
 
 
-def run_pair(cfg, n, steps, sampling):
+def run_pair(cfg, n, steps, sampling, series=True):
     p = rt.Problem(rt.config_text(cfg, **{"simulation.particle_count": n}), mode_sampling=sampling)
     g = rt.Context(p)
+    g.set_tuning(series_on=series)
     o = O.Sim(p)
     o.initialize()
     return p, g, o
@@ -41,10 +42,15 @@ def test_initial_population_identical(cfg, n):
     assert g.ledger()["thermal_energy_ev"] == pytest.approx(o.ledger()["thermal_energy_ev"], rel=1e-13)
 
 
-@pytest.mark.parametrize("cfg,n,sampling", [("c1_cross_plane.toml", 20000, 1), ("c2_inplane_film.toml", 60000, 1),
-                                            ("c2_inplane_film.toml", 60000, 0)])
-def test_steps_match_oracle(cfg, n, sampling):
-    p, g, o = run_pair(cfg, n, 3, sampling)
+@pytest.mark.parametrize("cfg,n,sampling,series", [("c1_cross_plane.toml", 20000, 1, True),
+                                                   ("c1_cross_plane.toml", 20000, 1, False),
+                                                   ("c2_inplane_film.toml", 60000, 1, True),
+                                                   ("c2_inplane_film.toml", 60000, 1, False),
+                                                   ("c2_inplane_film.toml", 60000, 0, True)])
+def test_steps_match_oracle(cfg, n, sampling, series):
+    """series=False: every pseudo-temperature evaluation is an exact carrier sum,
+    the reference's own procedure; series=True: the per-cell Taylor series."""
+    p, g, o = run_pair(cfg, n, 3, sampling, series)
     scale = max(p.scalars()["volume"] ** (1.0 / 3.0), 1.0)
     for step in range(3):
         g.step(1)
@@ -65,5 +71,9 @@ def test_steps_match_oracle(cfg, n, sampling):
             assert lg[k] == lo[k], k
         for k in ["step_absorbed_energy_ev", "step_injected_energy_ev", "thermal_energy_ev"]:
             assert lg[k] == pytest.approx(lo[k], rel=1e-11, abs=1e-9), k
+    led = g.ledger()
+    assert led["newton_evals"] > 0
+    if not series:  # every evaluation was an exact carrier sum
+        assert led["newton_passes"] >= led["newton_evals"]
     qg, qo = g.heat_flux(), o.heat_flux()
     np.testing.assert_allclose(qg, qo, rtol=1e-5, atol=1e-6 * np.abs(qo).max())
