@@ -93,6 +93,28 @@ def cpu_baseline(problem_text, steps):
                       f"OpenMP threads={cores}", "seconds": secs}
 
 
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB (hipMemcpy blit kernel): the
+    calibration for HBM byte counters (FETCH_SIZE / WRITE_SIZE per dispatch)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    rc = hip.hipMemcpy(b, a, C.c_size_t(n), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    dt = time.perf_counter() - t0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    if rc:
+        raise RuntimeError(f"hipMemcpy failed: {rc}")
+    log(f"[bench] copy probe: {mib} MiB in {dt * 1e3:.3f} ms ({2 * n / dt / 1e9:.1f} GB/s read+write)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,11 +123,16 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one device copy of this size first (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (multi-GPU sharding is "
               f"a design note, not part of this build).", file=sys.stderr)
         return 2
+
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
 
     from phonomc_amd import runtime as rt
 

@@ -260,22 +260,31 @@ int pmc_upload_material(pmc_ctx* ctx, const pmc_mode_table* t) {
     v.mtab = ctx->upload(mtab.data(), mtab.size());
     v.ntemps = t->n_temps;
     v.temps = ctx->upload(t->temps, t->n_temps);
-    // Rate table rows in active-mode order: one gather per carrier instead of two.
-    std::vector<double> gamma(static_cast<size_t>(t->n_temps) * t->n_active);
-    for (int r = 0; r < t->n_temps; ++r)
+    // Rate table as row pairs in active-mode order: a carrier's two
+    // interpolation rows come in one 16-byte load, and the few pairs the cell
+    // temperatures use stay cache resident.
+    std::vector<double2> gamma2(static_cast<size_t>(t->n_temps) * t->n_active);
+    for (int a = 0; a < t->n_active; ++a) {
+        const int f = t->active_flat[a];
+        if (f < 0 || f >= t->n_flat) return fail(ctx, PMC_ERR_ARGUMENT, "active mode outside the flat table");
+    }
+    for (int r = 0; r < t->n_temps; ++r) {
+        const int r1 = std::min(r + 1, t->n_temps - 1);
         for (int a = 0; a < t->n_active; ++a) {
             const int f = t->active_flat[a];
-            if (f < 0 || f >= t->n_flat) return fail(ctx, PMC_ERR_ARGUMENT, "active mode outside the flat table");
-            gamma[static_cast<size_t>(r) * t->n_active + a] = t->gamma_total[static_cast<size_t>(r) * t->n_flat + f];
+            gamma2[static_cast<size_t>(r) * t->n_active + a] =
+                make_double2(t->gamma_total[static_cast<size_t>(r) * t->n_flat + f],
+                             t->gamma_total[static_cast<size_t>(r1) * t->n_flat + f]);
         }
-    v.gamma = ctx->upload(gamma.data(), gamma.size());
+    }
+    v.gamma2 = ctx->upload(gamma2.data(), gamma2.size());
     v.w = t->energy_weight;
     v.e_bg = t->background_energy_density;
     v.zp = t->zero_point_density;
     v.lut_n = t->lut_n;
     v.lut_t = ctx->upload(t->lut_t, t->lut_n);
     v.lut_e = ctx->upload(t->lut_e, t->lut_n);
-    if (!v.active_flat || !v.omega || !v.vel || !v.veld || !v.neq || !v.mtab || !v.temps || !v.gamma || !v.lut_t ||
+    if (!v.active_flat || !v.omega || !v.vel || !v.veld || !v.neq || !v.mtab || !v.temps || !v.gamma2 || !v.lut_t ||
         !v.lut_e)
         return fail(ctx, PMC_ERR_DEVICE, "material upload failed");
     ctx->has_material = true;
@@ -323,19 +332,25 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
     v.partial_rows = pmc_launch::partial_rows(v.capacity);
     v.partials = ctx->alloc<double>(static_cast<size_t>(v.partial_rows) * pmc_dev::kPartialCols);
     v.cell_T = ctx->alloc<double>(nc);
-    v.cell_acc = ctx->alloc<double>(pmc_dev::kAccCols * nc);
     v.cell_flux = ctx->alloc<double>(3 * nc);
     v.cell_count = ctx->alloc<int>(nc);
     v.cell_start = ctx->alloc<int>(nc + 1);
     v.cell_cursor = ctx->alloc<int>(nc);
     v.crta = ctx->alloc<CellRta>(nc);
+    v.chunk_start = ctx->alloc<int>(nc + 1);
+    v.max_chunks = static_cast<int>((v.capacity + pmc_dev::kChunk - 1) / pmc_dev::kChunk + nc);
+    v.chunk_cell = ctx->alloc<int>(static_cast<size_t>(v.max_chunks));
+    v.chunk_acc = ctx->alloc<double>(static_cast<size_t>(v.max_chunks) * pmc_dev::kAccCols);
+    v.chunk_out = ctx->alloc<double>(2 * static_cast<size_t>(v.max_chunks));
+    v.aw = ctx->alloc<double>(static_cast<size_t>(v.capacity));
     const size_t blocks = static_cast<size_t>(pmc_launch::wave_blocks(v.capacity));
     v.events = ctx->alloc<int>(blocks * pmc_dev::kSlotsPerBlock);
     v.block_events = ctx->alloc<int>(blocks);
-    if (!v.facets || !v.voxel || !v.cell_vol || !v.cell_e || !v.cell_T || !v.cell_acc || !v.cell_tally || !v.partials ||
-        !v.cell_flux || !v.cell_count || !v.cell_start || !v.cell_cursor || !v.crta || !v.events || !v.block_events)
+    if (!v.facets || !v.voxel || !v.cell_vol || !v.cell_e || !v.cell_T || !v.cell_tally || !v.partials ||
+        !v.cell_flux || !v.cell_count || !v.cell_start || !v.cell_cursor || !v.crta || !v.events || !v.block_events ||
+        !v.chunk_start || !v.chunk_cell || !v.chunk_acc || !v.chunk_out || !v.aw)
         return fail(ctx, PMC_ERR_DEVICE, "geometry upload failed");
-    ctx->note(hipMemset(v.cell_acc, 0, pmc_dev::kAccCols * nc * sizeof(double)));
+    ctx->note(hipMemset(v.chunk_start, 0, (nc + 1) * sizeof(int)));
     ctx->note(hipMemset(v.cell_start, 0, (nc + 1) * sizeof(int)));
     ctx->note(hipMemset(v.cell_count, 0, nc * sizeof(int)));
     ctx->note(hipMemset(v.block_events, 0, blocks * sizeof(int)));
@@ -415,7 +430,6 @@ static int finish_population(pmc_ctx* ctx, int64_t count) {
     pmc_launch::sort(v, ctx->stream);
     ctx->cur = 1 - ctx->cur;
     View& w = view(ctx);
-    pmc_launch::collide(w, ctx->stream, 0);
     pmc_launch::refresh(w, ctx->stream, 3);
     int rc = check_hip(ctx, hipGetLastError(), "initial refresh");
     if (rc) return rc;
@@ -534,7 +548,11 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
         T.run(PMC_KERNEL_SORT, [&] { pmc_launch::sort(v, st); });
         ctx->cur = 1 - ctx->cur;
         View& w = view(ctx);
-        T.run(PMC_KERNEL_COLLIDE, [&] { pmc_launch::collide(w, st, 1); });
+        T.run(PMC_KERNEL_COLLIDE, [&] {
+            pmc_launch::moments(w, st);
+            pmc_launch::newton(w, st);
+            pmc_launch::relax(w, st);
+        });
         T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::refresh(w, st, 2); });
         int rc = check_hip(ctx, hipGetLastError(), "step launch");
         if (rc) {

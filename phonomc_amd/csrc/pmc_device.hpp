@@ -25,14 +25,18 @@ constexpr int kMaxFacetsDev = 64;
 // Carrier passes: each wave walks kRowsPerWave rows of 64 consecutive slots,
 // loading kUnroll rows ahead of their use.
 constexpr int kRowsPerWave = 16;
+constexpr int kSortRows = 8;   // rows per wave of the cell sort (values held in registers)
 constexpr int kUnroll = 4;
 constexpr int kWavesPerBlock = 4;
 constexpr int kSlotsPerBlock = 64 * kRowsPerWave * kWavesPerBlock;
 constexpr int kInjectBlocks = 64;
 
-// Collision kernel: one block per cell over its contiguous (cell-sorted) carriers.
-constexpr int kCollideThreads = 512;
-constexpr int kCollideWaves = kCollideThreads / 64;
+// Collision kernels work on chunks of up to kChunk consecutive carriers of one
+// cell in the cell-sorted buffer (one block each) and write per-chunk partial
+// sums; per-cell results add the chunks of a cell in chunk order.
+constexpr int kChunk = 2048;
+constexpr int kChunkThreads = 256;
+constexpr int kChunkWaves = kChunkThreads / 64;
 
 // RTA pseudo-temperature sums per cell:
 //   D      = sum hw a (n - n_BE(T0))                 exact value at the cell temperature T0
@@ -133,8 +137,8 @@ struct DevState {
 // Per-cell RTA constants of one step: temperature of the refresh (series
 // centre), initial bracket top, rate-table rows/weight and the pseudo-temperature.
 struct CellRta {
-    double t0, high, gt, tr;
-    int glo, ghi, pad0, pad1;
+    double t0, high, gt, tr, inv_t0;
+    int glo, ghi;
 };
 
 struct View {
@@ -157,7 +161,7 @@ struct View {
     const double2* mtab;    // (omega, n_eq(T_bg)) per active mode
     int ntemps;
     const double* temps;
-    const double* gamma;    // [ntemps][na] in active-mode order
+    const double2* gamma2;  // [ntemps][na]: (gamma(T_r), gamma(T_r+1)) of each active mode
     double w, e_bg, zp;
     int lut_n;
     const double* lut_t;
@@ -183,12 +187,17 @@ struct View {
     int partial_rows;     // transport blocks + injection blocks
     double* cell_e;       // energy density of the latest refresh
     double* cell_T;
-    double* cell_acc;     // kAccCols RTA sums per cell
     double* cell_flux;    // 3 per cell
     int* cell_count;      // live carriers per cell after transport and injection
     int* cell_start;      // [ncells+1] first slot of each cell in the sorted buffer
     int* cell_cursor;     // scatter cursors
     CellRta* crta;
+    int* chunk_start;     // [ncells+1] first chunk of each cell
+    int* chunk_cell;      // [max_chunks] cell of each chunk
+    int max_chunks;
+    double* chunk_acc;    // kAccCols RTA sums per chunk
+    double* chunk_out;    // 2 per chunk: energy after relaxation, lifetime residual
+    double* aw;           // relaxation weight of each sorted carrier (moments -> relax)
     int* events;          // per transport block: slots with a wall event this step
     int* block_events;    // per transport block: number of such slots
     DevState* st;

@@ -94,12 +94,10 @@ __device__ __forceinline__ float4 carrier_velocity(const View& V, int mw) {
 __device__ __forceinline__ double relax_weight(const View& V, const CellRta& cr, int m) {
     // PhononMaterial::mode_relaxation_weight (src/material/Thermodynamics.cpp:95-106)
     // with mode_gamma's interpolation (:108-133); row and weight are per cell.
-    const double y0 = V.gamma[static_cast<size_t>(cr.glo) * V.na + m];
-    double g = y0;
-    if (cr.ghi != cr.glo) {
-        const double y1 = V.gamma[static_cast<size_t>(cr.ghi) * V.na + m];
-        g = y0 * (1.0 - cr.gt) + y1 * cr.gt;
-    }
+    // Row-pair table: (gamma[glo], gamma[glo + 1]) of a mode in one load.
+    const double2 y = V.gamma2[static_cast<size_t>(cr.glo) * V.na + m];
+    double g = y.x;
+    if (cr.ghi != cr.glo) g = y.x * (1.0 - cr.gt) + y.y * cr.gt;
     if (g == 0.0 || V.dt == 0.0) return 0.0;
     return -expm1(-(g * V.dt) * (4.0 * kPi));
 }
@@ -566,10 +564,10 @@ __global__ void __launch_bounds__(256) k_transport(View V) {
             mw[j] = i < n ? V.mode[i] : -1;
         }
 #pragma unroll
-        for (int j = 0; j < kUnroll; ++j) {
+        for (int j = 0; j < kUnroll; ++j) { // independent of the mode load: all slots below n hold data
             const int i = base + (r0 + j) * 64 + lane;
             x[j] = y[j] = z[j] = 0.0f;
-            if (mw[j] >= 0) {
+            if (i < n) {
                 x[j] = V.px[i];
                 y[j] = V.py[i];
                 z[j] = V.pz[i];
@@ -844,6 +842,7 @@ __global__ void __launch_bounds__(1024) k_refresh(View V, int phase) {
     DevState* st = V.st;
     __shared__ double red[1024];
     __shared__ int scan[1024];
+    __shared__ int scan2[1024];
     if (phase == 1) {
         for (int c = 0; c < kPartialCols; ++c) {
             if (c == kPAbsorbedRes + V.nres) break;
@@ -870,26 +869,42 @@ __global__ void __launch_bounds__(1024) k_refresh(View V, int phase) {
     const int per = (nc + static_cast<int>(blockDim.x) - 1) / static_cast<int>(blockDim.x);
     const int c0 = min(nc, static_cast<int>(threadIdx.x) * per), c1 = min(nc, c0 + per);
     if (phase <= 1) {
-        int mine = 0;
-        for (int c = c0; c < c1; ++c) mine += V.cell_count[c];
+        int mine = 0, mine_chunks = 0;
+        for (int c = c0; c < c1; ++c) {
+            mine += V.cell_count[c];
+            mine_chunks += (V.cell_count[c] + kChunk - 1) / kChunk;
+        }
         scan[threadIdx.x] = mine;
+        scan2[threadIdx.x] = mine_chunks;
         __syncthreads();
         for (int off = 1; off < static_cast<int>(blockDim.x); off <<= 1) {
-            const int add = static_cast<int>(threadIdx.x) >= off ? scan[threadIdx.x - off] : 0;
+            const bool take = static_cast<int>(threadIdx.x) >= off;
+            const int add = take ? scan[threadIdx.x - off] : 0;
+            const int add2 = take ? scan2[threadIdx.x - off] : 0;
             __syncthreads();
             scan[threadIdx.x] += add;
+            scan2[threadIdx.x] += add2;
             __syncthreads();
         }
         int offset = scan[threadIdx.x] - mine;
+        int coffset = scan2[threadIdx.x] - mine_chunks;
         const int total = scan[blockDim.x - 1];
+        const int total_chunks = scan2[blockDim.x - 1];
         for (int c = c0; c < c1; ++c) {
+            const int count = V.cell_count[c];
             V.cell_start[c] = offset;
             V.cell_cursor[c] = offset;
-            offset += V.cell_count[c];
+            V.chunk_start[c] = coffset;
+            const int nch = (count + kChunk - 1) / kChunk;
+            for (int k = 0; k < nch; ++k) V.chunk_cell[coffset + k] = c;
+            offset += count;
+            coffset += nch;
             V.cell_count[c] = 0;
+            V.cell_tally[static_cast<size_t>(kTallyStride) * c + kTallyEnergy] = 0.0;
         }
         if (threadIdx.x == 0) {
             V.cell_start[nc] = total;
+            V.chunk_start[nc] = total_chunks;
             if (phase == 1) {
                 const int expect =
                     st->n_live - static_cast<int>(st->step_absorbed) + static_cast<int>(st->step_injected);
@@ -904,9 +919,15 @@ __global__ void __launch_bounds__(1024) k_refresh(View V, int phase) {
     }
     double local = 0.0, residual = 0.0;
     for (int c = c0; c < c1; ++c) {
-        double* tally = V.cell_tally + static_cast<size_t>(kTallyStride) * c;
-        const double e = tally[kTallyEnergy] / V.cell_vol[c] + V.e_bg;
-        residual += tally[kTallyResidual];
+        double esum = V.cell_tally[static_cast<size_t>(kTallyStride) * c + kTallyEnergy];
+        if (phase == 2) { // chunk partials of the relaxation, in chunk order
+            esum = 0.0;
+            for (int k = V.chunk_start[c]; k < V.chunk_start[c + 1]; ++k) {
+                esum += V.chunk_out[2 * static_cast<size_t>(k)];
+                residual += V.chunk_out[2 * static_cast<size_t>(k) + 1];
+            }
+        }
+        const double e = esum / V.cell_vol[c] + V.e_bg;
         V.cell_e[c] = e;
         V.cell_T[c] = lut_inverse(V, e);
         local += (e - V.zp) * V.cell_vol[c];
@@ -941,25 +962,53 @@ __global__ void __launch_bounds__(1024) k_refresh(View V, int phase) {
 // LDS, reserves one destination range per (block, cell) from the global cell
 // cursor, then streams the carriers out.
 __global__ void __launch_bounds__(256) k_sort(View V) {
+    __shared__ double esum[kHashSlots];
+    for (int t = threadIdx.x; t < kHashSlots; t += blockDim.x) esum[t] = 0.0;
     PMC_BLOCK_HASH(H);
     const int n = V.st->n_slots;
-    const int base = wave_global() * 64 * kRowsPerWave;
+    const int base = (xcd_block() * kWavesPerBlock + (threadIdx.x >> 6)) * 64 * kSortRows;
     const int lane = __lane_id();
-    int tok[kRowsPerWave]; // (hash slot << 16 | rank), -2 - destination, or -1 (dead)
+    int mw[kSortRows], cl[kSortRows];
+    double oc[kSortRows];
 #pragma unroll
-    for (int r = 0; r < kRowsPerWave; ++r) {
+    for (int r = 0; r < kSortRows; ++r) {
         const int i = base + r * 64 + lane;
-        const int mw = i < n ? V.mode[i] : -1;
-        bool active = mw >= 0;
-        const int cell = active ? V.cell[i] : 0;
+        mw[r] = -1;
+        cl[r] = 0;
+        oc[r] = 0.0;
+        if (i < n) { // all slots below n hold data; dead ones are skipped below
+            mw[r] = V.mode[i];
+            cl[r] = V.cell[i];
+            oc[r] = V.occ[i];
+        }
+    }
+    int tok[kSortRows]; // (hash slot << 16 | rank), -2 - destination, or -1 (dead)
+    int run_slot = -1;  // lane-local running energy sum of one hash slot
+    double run_e = 0.0;
+#pragma unroll
+    for (int r = 0; r < kSortRows; ++r) {
+        bool active = mw[r] >= 0;
+        const int cell = cl[r];
+        double e = 0.0;
+        if (active) {
+            // Deviational energy for the refresh after transport (update_energy_density).
+            const double2 mt = V.mtab[mw[r] & kModeMask];
+            e = (kHbar * mt.x * (oc[r] - mt.y)) * V.w;
+        }
         int s = -1;
         tok[r] = -1;
         if (active) {
             s = H.slot(cell);
             if (s < 0) {
                 tok[r] = -2 - atomicAdd(&V.cell_cursor[cell], 1);
+                atomicAdd(&V.cell_tally[static_cast<size_t>(kTallyStride) * cell + kTallyEnergy], e);
                 active = false;
+            } else if (s != run_slot) {
+                if (run_slot >= 0) atomicAdd(&esum[run_slot], run_e);
+                run_slot = s;
+                run_e = 0.0;
             }
+            if (active) run_e += e;
         }
         uint64_t pending = __ballot(active);
         while (pending) {
@@ -974,42 +1023,46 @@ __global__ void __launch_bounds__(256) k_sort(View V) {
             pending &= ~mask;
         }
     }
+    if (run_slot >= 0) atomicAdd(&esum[run_slot], run_e);
     __syncthreads();
     for (int t = threadIdx.x; t < kHashSlots; t += blockDim.x)
-        if (H.key[t] >= 0 && H.cnt[t] > 0) H.base[t] = atomicAdd(&V.cell_cursor[H.key[t]], H.cnt[t]);
+        if (H.key[t] >= 0 && H.cnt[t] > 0) {
+            H.base[t] = atomicAdd(&V.cell_cursor[H.key[t]], H.cnt[t]);
+            atomicAdd(&V.cell_tally[static_cast<size_t>(kTallyStride) * H.key[t] + kTallyEnergy], esum[t]);
+        }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kRowsPerWave; ++r) {
+    for (int r = 0; r < kSortRows; ++r) {
         if (tok[r] == -1) continue;
         const int i = base + r * 64 + lane;
         const int dst = tok[r] >= 0 ? H.base[tok[r] >> 16] + (tok[r] & 0xffff) : -2 - tok[r];
         V.o_px[dst] = V.px[i];
         V.o_py[dst] = V.py[i];
         V.o_pz[dst] = V.pz[i];
-        V.o_mode[dst] = V.mode[i];
-        V.o_occ[dst] = V.occ[i];
-        V.o_cell[dst] = V.cell[i];
+        V.o_mode[dst] = mw[r];
+        V.o_occ[dst] = oc[r];
+        V.o_cell[dst] = cl[r];
         V.o_uid[dst] = V.uid[i];
     }
 }
 
-// Sums v[k] over the block; every thread receives the same totals (wave sums
-// combined in wave order).
-template <int K>
+// Sums v[k] over the block of W waves; every thread receives the same totals
+// (wave sums combined in wave order).
+template <int K, int W>
 __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
     const int wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
     if (__lane_id() == 0) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) lds[k * kCollideWaves + wave] = v[k];
+        for (int k = 0; k < K; ++k) lds[k * W + wave] = v[k];
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < kCollideWaves; ++w) s += lds[k * kCollideWaves + w];
+        for (int w = 0; w < W; ++w) s += lds[k * W + w];
         v[k] = s;
     }
     __syncthreads();
@@ -1018,29 +1071,37 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], double* lds) {
 // Adds one carrier's terms of the pseudo-temperature sums (the evaluate lambda
 // of src/solver/RtaCollisionOperator.cpp:91-116 and its f(0), f(high) and
 // energy-scale sums, :118-146), expanded around the cell temperature T0:
-// n_BE(T) = n0 + sum_k (-x0)^k P_k(n0)/k! u^k with u = T0/T - 1, x0 = hw/(kB T0).
-// Carriers the reference skips (hw <= 0 or a <= 0) add nothing.
-__device__ __forceinline__ void rta_accumulate(const View& V, const CellRta& cr, int m, double n,
-                                               double (&acc)[kAccCols]) {
-    const double w = fmax(0.0, V.mtab[m].x);
-    const double hw = kHbar * w;
+// n_BE(T) = n0 + sum_k (-x0)^k P_k(n0)/k! u^k with u = T0/T - 1, x0 = hw/(kB T0),
+// P_k in q = n0(n0+1), r = 2 n0 + 1.  Carriers the reference skips (hw <= 0 or
+// a <= 0) add nothing.  Returns the relaxation weight a.
+__device__ __forceinline__ double rta_accumulate(const View& V, const CellRta& cr, int m, double n,
+                                                 double (&acc)[kAccCols]) {
     const double a = relax_weight(V, cr, m);
-    if (!(hw > 0.0) || !(a > 0.0)) return;
+    const double w = fmax(0.0, V.omega[m]);
+    const double hw = kHbar * w;
+    if (!(hw > 0.0) || !(a > 0.0)) return a;
     const double g = hw * a;
     const double t0 = fmax(0.0, cr.t0);
     const double n0 = bose(t0, w);
     acc[kAccD] += g * (n - n0);
     if (t0 > 0.0) {
-        const double x0 = hw / (kBoltz * t0);
+        const double hk = hw * (1.0 / kBoltz); // hbar omega / kB
+        const double x0 = hk * cr.inv_t0;
         if (x0 >= 1e-6) {
+            static_assert(kSeriesOrder == 6, "closed forms below cover six terms");
+            const double q = n0 * (n0 + 1.0), r = 2.0 * n0 + 1.0;
+            const double P[6] = {q,
+                                 q * r,
+                                 q * (6.0 * q + 1.0),
+                                 q * r * (12.0 * q + 1.0),
+                                 q * ((120.0 * q + 30.0) * q + 1.0),
+                                 q * r * ((360.0 * q + 60.0) * q + 1.0)};
+            constexpr double kInvFact[6] = {1.0, 1.0 / 2.0, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0};
             double c = g;
 #pragma unroll
             for (int k = 1; k <= kSeriesOrder; ++k) {
-                c *= -x0 / k;
-                double p = kBoseSeries.c[k][k + 1];
-#pragma unroll
-                for (int j = k; j >= 1; --j) p = p * n0 + kBoseSeries.c[k][j];
-                acc[kAccM1 + k - 1] += c * (p * n0);
+                c *= -x0; // g (-x0)^k
+                acc[kAccM1 + k - 1] += (c * kInvFact[k - 1]) * P[k - 1];
             }
         } else {
             acc[kAccBad] += 1.0;
@@ -1048,30 +1109,91 @@ __device__ __forceinline__ void rta_accumulate(const View& V, const CellRta& cr,
     }
     acc[kAccS0] += g * n;
     acc[kAccEscale] += g * fmax(1.0, n);
-    const double xh = hw / (kBoltz * cr.high);
-    const double lb = xh > 1e-9 ? fmax(0.0, 1.0 / xh - 0.5) : 0.0;
+    const double inv_xh = cr.high / (hw * (1.0 / kBoltz)); // 1/x at T = high
+    const double lb = inv_xh < 1e9 ? fmax(0.0, inv_xh - 0.5) : 0.0;
     acc[kAccUHigh] += g * (n - lb);
+    return a;
+}
+
+// Per-cell RTA constants after the transport refresh: cell temperature from the
+// deviational energy tallied by the sort (GridStatistics::refresh_temperatures)
+// and the rate-table row of the lifetime temperature.
+__device__ __forceinline__ CellRta cell_rta(const View& V, int c) {
+    const double e = V.cell_tally[static_cast<size_t>(kTallyStride) * c + kTallyEnergy] / V.cell_vol[c] + V.e_bg;
+    const double T = lut_inverse(V, e);
+    CellRta cr{};
+    cr.t0 = T;
+    cr.high = fmax(1000.0, 2.0 * fmax(1.0, T));
+    cr.tr = T;
+    cr.inv_t0 = T > 0.0 ? 1.0 / T : 0.0;
+    gamma_row(V, V.lifetime_local ? T : V.lifetime_T, cr.glo, cr.ghi, cr.gt);
+    return cr;
+}
+
+// Per-cell constants of the collision step, once per cell after the sort.
+__global__ void k_cell_prep(View V) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < V.ncells) V.crta[c] = cell_rta(V, c);
+}
+
+// Pseudo-temperature sums of one chunk at the cell temperature (terms of
+// rta_accumulate), written as the chunk's partial row.
+__global__ void __launch_bounds__(kChunkThreads) k_moments(View V) {
+    __shared__ double lds[kAccCols * kChunkWaves];
+    const int b = blockIdx.x;
+    if (b >= V.chunk_start[V.ncells]) return;
+    const int c = V.chunk_cell[b];
+    const int begin = V.cell_start[c] + (b - V.chunk_start[c]) * kChunk;
+    const int end = min(begin + kChunk, V.cell_start[c + 1]);
+    const CellRta cr = V.crta[c];
+    double acc[kAccCols];
+#pragma unroll
+    for (int k = 0; k < kAccCols; ++k) acc[k] = 0.0;
+    constexpr int U = kChunk / kChunkThreads; // every carrier of the chunk in one round of loads
+    for (int i0 = begin + static_cast<int>(threadIdx.x); i0 < end; i0 += kChunkThreads * U) {
+        int m[U];
+        double n[U];
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            const int i = i0 + j * kChunkThreads;
+            m[j] = -1;
+            n[j] = 0.0;
+            if (i < end) {
+                m[j] = V.mode[i] & kModeMask;
+                n[j] = V.occ[i];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+            if (m[j] >= 0) V.aw[i0 + j * kChunkThreads] = rta_accumulate(V, cr, m[j], n[j], acc);
+    }
+    block_reduce<kAccCols, kChunkWaves>(acc, lds);
+    if (threadIdx.x == 0) {
+        double* out = V.chunk_acc + static_cast<size_t>(b) * kAccCols;
+#pragma unroll
+        for (int k = 0; k < kAccCols; ++k) out[k] = acc[k];
+    }
 }
 
 // Exact pseudo-temperature sums of one cell over its carriers (the evaluate
-// lambda of src/solver/RtaCollisionOperator.cpp:91-116), whole block.
+// lambda of src/solver/RtaCollisionOperator.cpp:91-116), one wave; the result
+// is identical in every lane.
 __device__ void exact_sums(const View& V, const CellRta& cr, int begin, int end, double t, double& value,
-                           double& slope, double* lds) {
+                           double& slope) {
     const double T = fmax(0.0, t);
-    double r[2] = {0.0, 0.0};
-    for (int i = begin + static_cast<int>(threadIdx.x); i < end; i += blockDim.x) {
+    double f = 0.0, s = 0.0;
+    for (int i = begin + __lane_id(); i < end; i += 64) {
         const int m = V.mode[i] & kModeMask;
         const double w = fmax(0.0, V.mtab[m].x);
         const double hw = kHbar * w;
-        const double a = relax_weight(V, cr, m);
+        const double a = V.aw[i];
         if (!(hw > 0.0) || !(a > 0.0)) continue;
         const double neq = bose(T, w);
-        r[0] += hw * a * (V.occ[i] - neq);
-        if (T > 0.0) r[1] -= hw * a * (hw * neq * (neq + 1.0) / (kBoltz * T * T));
+        f += hw * a * (V.occ[i] - neq);
+        if (T > 0.0) s -= hw * a * (hw * neq * (neq + 1.0) / (kBoltz * T * T));
     }
-    block_reduce<2>(r, lds);
-    value = r[0];
-    slope = r[1];
+    value = wave_sum(f);
+    slope = wave_sum(s);
 }
 
 // Pseudo-temperature sums from the per-cell series at temperature t.  Returns
@@ -1103,89 +1225,38 @@ __device__ __forceinline__ bool series_sums(const double (&acc)[kAccCols], doubl
     return true;
 }
 
-// Collision step of one cell over its contiguous carriers in the cell-sorted
-// buffer, one block per cell, all sums deterministic:
-//  0. deviational energy -> cell temperature T0 (GridStatistics::refresh_temperatures);
-//  1. pseudo-temperature sums at T0 (exact value, series terms, f(0), bound of f(high), scale);
-//  2. bracket + safeguarded Newton for T_R (src/solver/RtaCollisionOperator.cpp:118-165),
-//     from the series where trusted, else exact carrier sums;
-//  3. relaxation n <- max(0, n + a (n_BE(T_R) - n)) with residual (:167-183) and the
-//     energy tally of the following refresh.
-// With full == 0 only step 0 runs (initial refresh): the cell energy is tallied.
-__global__ void __launch_bounds__(kCollideThreads) k_collide(View V, int full) {
-    __shared__ double lds[kAccCols * kCollideWaves];
-    const int c = blockIdx.x;
+// Per-cell bracket + safeguarded Newton solve of sum hw a (n - n_BE(T_R)) = 0
+// (src/solver/RtaCollisionOperator.cpp:118-165), one wave per cell with every
+// lane running the same iteration.  f(0) and the energy scale are exact chunk
+// sums; f(high) is summed exactly only when its upper bound is not negative;
+// f(T) and f'(T) come from the series where trusted, otherwise from the cell's
+// carriers.
+__global__ void __launch_bounds__(256) k_newton(View V) {
+    const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (c >= V.ncells) return;
     const int begin = V.cell_start[c], end = V.cell_start[c + 1];
-    const int stride = blockDim.x;
-    double* tally = V.cell_tally + static_cast<size_t>(kTallyStride) * c;
-    double e[1] = {0.0};
-    for (int i0 = begin + static_cast<int>(threadIdx.x); i0 < end; i0 += stride * kUnroll) {
-        int m[kUnroll];
-        double n[kUnroll];
-#pragma unroll
-        for (int j = 0; j < kUnroll; ++j) {
-            const int i = i0 + j * stride;
-            m[j] = -1;
-            n[j] = 0.0;
-            if (i < end) {
-                m[j] = V.mode[i] & kModeMask;
-                n[j] = V.occ[i];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kUnroll; ++j) {
-            if (m[j] < 0) continue;
-            const double2 mt = V.mtab[m[j]];
-            e[0] += (kHbar * mt.x * (n[j] - mt.y)) * V.w;
-        }
-    }
-    block_reduce<1>(e, lds);
-    if (!full || end <= begin) {
-        if (threadIdx.x == 0) {
-            tally[kTallyEnergy] = e[0];
-            tally[kTallyResidual] = 0.0;
-        }
-        return;
-    }
-    const double T = lut_inverse(V, e[0] / V.cell_vol[c] + V.e_bg);
-    CellRta cr{};
-    cr.t0 = T;
-    cr.high = fmax(1000.0, 2.0 * fmax(1.0, T));
-    gamma_row(V, V.lifetime_local ? T : V.lifetime_T, cr.glo, cr.ghi, cr.gt);
-
+    if (end <= begin) return;
+    const CellRta cr = V.crta[c];
+    const int k0 = V.chunk_start[c], k1 = V.chunk_start[c + 1];
     double acc[kAccCols];
 #pragma unroll
-    for (int k = 0; k < kAccCols; ++k) acc[k] = 0.0;
-    for (int i0 = begin + static_cast<int>(threadIdx.x); i0 < end; i0 += stride * kUnroll) {
-        int m[kUnroll];
-        double n[kUnroll];
-#pragma unroll
-        for (int j = 0; j < kUnroll; ++j) {
-            const int i = i0 + j * stride;
-            m[j] = -1;
-            n[j] = 0.0;
-            if (i < end) {
-                m[j] = V.mode[i] & kModeMask;
-                n[j] = V.occ[i];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kUnroll; ++j)
-            if (m[j] >= 0) rta_accumulate(V, cr, m[j], n[j], acc);
+    for (int k = 0; k < kAccCols; ++k) {
+        double x = 0.0;
+        for (int ch = k0 + __lane_id(); ch < k1; ch += 64) x += V.chunk_acc[static_cast<size_t>(ch) * kAccCols + k];
+        acc[k] = wave_sum(x);
     }
-    block_reduce<kAccCols>(acc, lds);
-
+    const double T = cr.t0;
     const double s0 = acc[kAccS0], escale = acc[kAccEscale];
     long long exact = 0, evals = 0;
     double low = 0.0, high = cr.high;
     bool known_high = false;
     double f_high = 0.0, d = 0.0;
     if (s0 > 0.0 && !(acc[kAccUHigh] < -1e-12 * escale)) {
-        exact_sums(V, cr, begin, end, high, f_high, d, lds);
+        exact_sums(V, cr, begin, end, high, f_high, d);
         ++exact;
         for (int x = 0; f_high > 0.0 && x < 20; ++x) {
             high *= 2.0;
-            exact_sums(V, cr, begin, end, high, f_high, d, lds);
+            exact_sums(V, cr, begin, end, high, f_high, d);
             ++exact;
         }
         known_high = true;
@@ -1201,7 +1272,7 @@ __global__ void __launch_bounds__(kCollideThreads) k_collide(View V, int full) {
         for (int iter = 0; iter < 40; ++iter) {
             double value = 0.0, slope = 0.0;
             if (!(series && series_sums(acc, cr.t0, t, tol, V.series_radius, value, slope))) {
-                exact_sums(V, cr, begin, end, t, value, slope, lds);
+                exact_sums(V, cr, begin, end, t, value, slope);
                 ++exact;
             }
             ++evals;
@@ -1216,32 +1287,54 @@ __global__ void __launch_bounds__(kCollideThreads) k_collide(View V, int full) {
             t = cand;
         }
     }
-    cr.tr = t;
+    if (__lane_id() == 0) {
+        V.crta[c].tr = t;
+        if (exact)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->newton_passes),
+                      static_cast<unsigned long long>(exact));
+        if (evals)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->newton_evals),
+                      static_cast<unsigned long long>(evals));
+    }
+}
 
-    double r[2] = {0.0, 0.0}; // energy after relaxation, lifetime residual
-    for (int i0 = begin + static_cast<int>(threadIdx.x); i0 < end; i0 += stride * kUnroll) {
+// Relaxation n <- max(0, n + a (n_BE(T_R) - n)) and residual
+// (src/solver/RtaCollisionOperator.cpp:167-183) of one chunk, with the
+// deviational energy after relaxation for the following refresh.
+__global__ void __launch_bounds__(kChunkThreads) k_relax(View V) {
+    __shared__ double lds[2 * kChunkWaves];
+    const int b = blockIdx.x;
+    if (b >= V.chunk_start[V.ncells]) return;
+    const int c = V.chunk_cell[b];
+    const int begin = V.cell_start[c] + (b - V.chunk_start[c]) * kChunk;
+    const int end = min(begin + kChunk, V.cell_start[c + 1]);
+    const CellRta cr = V.crta[c];
+    double r[2] = {0.0, 0.0};
+    for (int i0 = begin + static_cast<int>(threadIdx.x); i0 < end; i0 += kChunkThreads * kUnroll) {
         int m[kUnroll];
-        double n[kUnroll];
+        double n[kUnroll], aj[kUnroll];
 #pragma unroll
         for (int j = 0; j < kUnroll; ++j) {
-            const int i = i0 + j * stride;
+            const int i = i0 + j * kChunkThreads;
             m[j] = -1;
             n[j] = 0.0;
+            aj[j] = 0.0;
             if (i < end) {
                 m[j] = V.mode[i] & kModeMask;
                 n[j] = V.occ[i];
+                aj[j] = V.aw[i];
             }
         }
 #pragma unroll
         for (int j = 0; j < kUnroll; ++j) {
             if (m[j] < 0) continue;
-            const int i = i0 + j * stride;
+            const int i = i0 + j * kChunkThreads;
             const double2 mt = V.mtab[m[j]];
             const double w = fmax(0.0, mt.x);
             const double hw = kHbar * w;
-            const double a = relax_weight(V, cr, m[j]);
+            const double a = aj[j];
             const double old = n[j];
-            const double neq = bose(t, w);
+            const double neq = bose(cr.tr, w);
             const double nn = old + a * (neq - old);
             const double occ = fmax(0.0, nn);
             if (occ != old) V.occ[i] = occ;
@@ -1250,17 +1343,10 @@ __global__ void __launch_bounds__(kCollideThreads) k_collide(View V, int full) {
             if (!isfinite(occ)) raise_error(V, PMC_ERR_INVALID_STATE, i, nullptr, -1);
         }
     }
-    block_reduce<2>(r, lds);
+    block_reduce<2, kChunkWaves>(r, lds);
     if (threadIdx.x == 0) {
-        tally[kTallyEnergy] = r[0];
-        tally[kTallyResidual] = r[1];
-        V.crta[c] = cr;
-        if (exact)
-            atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->newton_passes),
-                      static_cast<unsigned long long>(exact));
-        if (evals)
-            atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->newton_evals),
-                      static_cast<unsigned long long>(evals));
+        V.chunk_out[2 * static_cast<size_t>(b)] = r[0];
+        V.chunk_out[2 * static_cast<size_t>(b) + 1] = r[1];
     }
 }
 
@@ -1405,10 +1491,18 @@ void inject(const View& v, hipStream_t s) {
 }
 void refresh(const View& v, hipStream_t s, int phase) { hipLaunchKernelGGL(k_refresh, dim3(1), dim3(1024), 0, s, v, phase); }
 void sort(const View& v, hipStream_t s) {
-    hipLaunchKernelGGL(k_sort, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+    const int b = (v.capacity + 64 * kSortRows * kWavesPerBlock - 1) / (64 * kSortRows * kWavesPerBlock);
+    hipLaunchKernelGGL(k_sort, dim3((b + 7) & ~7), kWaveBlock, 0, s, v);
 }
-void collide(const View& v, hipStream_t s, int full) {
-    hipLaunchKernelGGL(k_collide, dim3(v.ncells), dim3(kCollideThreads), 0, s, v, full);
+void moments(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_prep, dim3(cell_blocks(v.ncells)), dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_moments, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v);
+}
+void newton(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_newton, dim3((v.ncells + 3) / 4), dim3(256), 0, s, v);
+}
+void relax(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_relax, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v);
 }
 void clear_cells(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_tally, dim3(cell_blocks(kTallyStride * v.ncells)), dim3(256), 0, s, v);

@@ -12,7 +12,9 @@ void inject(const pmc_dev::View& v, hipStream_t s);
 void refresh(const pmc_dev::View& v, hipStream_t s, int phase);
 void events(const pmc_dev::View& v, hipStream_t s);
 void sort(const pmc_dev::View& v, hipStream_t s);
-void collide(const pmc_dev::View& v, hipStream_t s, int full);
+void moments(const pmc_dev::View& v, hipStream_t s);
+void newton(const pmc_dev::View& v, hipStream_t s);
+void relax(const pmc_dev::View& v, hipStream_t s);
 void clear_cells(const pmc_dev::View& v, hipStream_t s);
 void flux(const pmc_dev::View& v, hipStream_t s);
 void containment(const pmc_dev::View& v, hipStream_t s);
