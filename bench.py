@@ -29,13 +29,16 @@ sys.path.insert(0, ROOT)
 
 WORKLOAD = "c2_inplane_film.toml"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per carrier of each carrier-pass kernel class (carrier
-# arrays the pass must read and write; tables and cell tallies excluded).
+# Algorithmic bytes per carrier of each carrier-pass kernel (carrier arrays the
+# pass must read and write in this design; tables and cell sums excluded).
 BYTES_PER_CARRIER = {
     "transport": 16 + 16,  # read pos(12) mode(4); write pos(12) cell(4)
     "sort": 32 + 32,       # read pos, mode, occ, cell, uid (32); write them in cell order (32)
-    "collide": 12 + 8,     # read mode(4) occ(8) of the cell's carriers; write occ(8)
+    "moments": 12 + 8,     # read mode(4) occ(8); write relaxation weight (8)
+    "relax": 20 + 8,       # read mode(4) occ(8) weight(8); write occ(8)
 }
+# Kernel names of each class in rocprofv3 output.
+KERNEL_NAMES = {"transport": "k_transport", "sort": "k_sort", "moments": "k_moments", "relax": "k_relax"}
 # Whole-step algorithmic bytes per particle-step: fused transport + tally pass
 # (read and write pos 12, mode 4, occ 8, cell 4) and collision pass (read mode,
 # cell, occ; write occ).
@@ -115,17 +118,57 @@ def copy_probe(mib):
     log(f"[bench] copy probe: {mib} MiB in {dt * 1e3:.3f} ms ({2 * n / dt / 1e9:.1f} GB/s read+write)")
 
 
+def pmc_summary(fetch_csv, write_csv, copy_mib, command):
+    """HBM bytes per launch of each carrier-pass kernel from two rocprofv3
+    --pmc runs (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), each scaled by what
+    the same counter reads for the device copy of copy_mib MiB in that run.
+    Writes profiles/traffic_<kernel>.json."""
+    import csv
+
+    def per_kernel(path, counter):
+        d = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            name = r["Kernel_Name"].split("(")[0].replace("pmc_dev::", "")
+            d.setdefault(name, []).append(float(r["Counter_Value"]))
+        return d
+
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    copy_bytes = float(copy_mib << 20)
+    fetch_scale = copy_bytes / (max(fetch["__amd_rocclr_copyBuffer"]) * 1024.0)
+    write_scale = copy_bytes / (max(write["__amd_rocclr_copyBuffer"]) * 1024.0)
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    for cls, name in KERNEL_NAMES.items():
+        if name not in fetch or name not in write:
+            continue
+        f = sum(fetch[name]) / len(fetch[name]) * 1024.0 * fetch_scale
+        w = sum(write[name]) / len(write[name]) * 1024.0 * write_scale
+        out = {"kernel": name, "dispatches": len(fetch[name]), "fetch_bytes": f, "write_bytes": w,
+               "bytes_per_launch": f + w, "fetch_scale": fetch_scale, "write_scale": write_scale,
+               "calibration": f"device copy of {copy_mib} MiB (__amd_rocclr_copyBuffer)", "command": command}
+        with open(os.path.join(ROOT, "profiles", f"traffic_{cls}.json"), "w") as fh:
+            json.dump(out, fh, indent=1)
+        log(f"[bench] {name}: {f / 2**20:.1f} MiB read + {w / 2**20:.1f} MiB written per launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--timing-steps", type=int, default=40)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this size first (byte-counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic_*.json from rocprofv3 counter CSVs and exit")
+    ap.add_argument("--pmc-command", default="", help="command recorded with --pmc-summary")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.copy_probe_mib or 1024, args.pmc_command)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (multi-GPU sharding is "
               f"a design note, not part of this build).", file=sys.stderr)

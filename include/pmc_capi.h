@@ -213,15 +213,17 @@ int pmc_get_tuning(pmc_ctx* ctx, pmc_tuning* tuning);
 /* Kernel classes of the per-kernel timers. */
 enum pmc_kernel_class {
     PMC_KERNEL_BEGIN = 0,      /* step set-up (+ containment check every 100 steps) */
-    PMC_KERNEL_TRANSPORT = 1,  /* free flight of carriers that reach no wall, energy tally */
+    PMC_KERNEL_TRANSPORT = 1,  /* free flight of carriers that reach no wall, cell counts */
     PMC_KERNEL_EVENTS = 2,     /* carriers with wall events (boundary conditions, rough walls) */
     PMC_KERNEL_INJECT = 3,     /* reservoir injection and remaining flight */
-    PMC_KERNEL_REFRESH = 4,    /* cell temperatures, cell offsets and ledgers */
-    PMC_KERNEL_SORT = 5,       /* cell-order compaction of the live carriers */
-    PMC_KERNEL_COLLIDE = 6,    /* per cell: temperature, pseudo-temperature solve, relaxation */
-    PMC_KERNEL_OUTPUT = 7,     /* observables outside the step (heat flux) */
-    PMC_KERNEL_STEP = 8,       /* whole steps */
-    PMC_KERNEL_CLASSES = 9
+    PMC_KERNEL_REFRESH = 4,    /* cell offsets, cell temperatures, per-cell constants, ledgers */
+    PMC_KERNEL_SORT = 5,       /* cell-order compaction of the live carriers + cell energy */
+    PMC_KERNEL_MOMENTS = 6,    /* per-chunk pseudo-temperature sums */
+    PMC_KERNEL_NEWTON = 7,     /* per-cell pseudo-temperature solve */
+    PMC_KERNEL_RELAX = 8,      /* per-chunk relaxation */
+    PMC_KERNEL_OUTPUT = 9,     /* observables outside the step (heat flux) */
+    PMC_KERNEL_STEP = 10,      /* whole steps */
+    PMC_KERNEL_CLASSES = 11
 };
 /* Accumulated kernel time (ms) and launch counts per class since the last reset,
  * measured with HIP events on the context's stream while timing is enabled. */

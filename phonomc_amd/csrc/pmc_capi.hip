@@ -548,11 +548,10 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
         T.run(PMC_KERNEL_SORT, [&] { pmc_launch::sort(v, st); });
         ctx->cur = 1 - ctx->cur;
         View& w = view(ctx);
-        T.run(PMC_KERNEL_COLLIDE, [&] {
-            pmc_launch::moments(w, st);
-            pmc_launch::newton(w, st);
-            pmc_launch::relax(w, st);
-        });
+        T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::cell_prep(w, st); });
+        T.run(PMC_KERNEL_MOMENTS, [&] { pmc_launch::moments(w, st); });
+        T.run(PMC_KERNEL_NEWTON, [&] { pmc_launch::newton(w, st); });
+        T.run(PMC_KERNEL_RELAX, [&] { pmc_launch::relax(w, st); });
         T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::refresh(w, st, 2); });
         int rc = check_hip(ctx, hipGetLastError(), "step launch");
         if (rc) {

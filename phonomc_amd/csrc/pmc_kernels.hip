@@ -970,16 +970,24 @@ __global__ void __launch_bounds__(256) k_sort(View V) {
     const int lane = __lane_id();
     int mw[kSortRows], cl[kSortRows];
     double oc[kSortRows];
+    float px[kSortRows], py[kSortRows], pz[kSortRows];
+    unsigned id[kSortRows];
 #pragma unroll
     for (int r = 0; r < kSortRows; ++r) {
         const int i = base + r * 64 + lane;
         mw[r] = -1;
         cl[r] = 0;
         oc[r] = 0.0;
+        px[r] = py[r] = pz[r] = 0.0f;
+        id[r] = 0;
         if (i < n) { // all slots below n hold data; dead ones are skipped below
             mw[r] = V.mode[i];
             cl[r] = V.cell[i];
             oc[r] = V.occ[i];
+            px[r] = V.px[i];
+            py[r] = V.py[i];
+            pz[r] = V.pz[i];
+            id[r] = V.uid[i];
         }
     }
     int tok[kSortRows]; // (hash slot << 16 | rank), -2 - destination, or -1 (dead)
@@ -1034,15 +1042,14 @@ __global__ void __launch_bounds__(256) k_sort(View V) {
 #pragma unroll
     for (int r = 0; r < kSortRows; ++r) {
         if (tok[r] == -1) continue;
-        const int i = base + r * 64 + lane;
         const int dst = tok[r] >= 0 ? H.base[tok[r] >> 16] + (tok[r] & 0xffff) : -2 - tok[r];
-        V.o_px[dst] = V.px[i];
-        V.o_py[dst] = V.py[i];
-        V.o_pz[dst] = V.pz[i];
+        V.o_px[dst] = px[r];
+        V.o_py[dst] = py[r];
+        V.o_pz[dst] = pz[r];
         V.o_mode[dst] = mw[r];
         V.o_occ[dst] = oc[r];
         V.o_cell[dst] = cl[r];
-        V.o_uid[dst] = V.uid[i];
+        V.o_uid[dst] = id[r];
     }
 }
 
@@ -1494,8 +1501,10 @@ void sort(const View& v, hipStream_t s) {
     const int b = (v.capacity + 64 * kSortRows * kWavesPerBlock - 1) / (64 * kSortRows * kWavesPerBlock);
     hipLaunchKernelGGL(k_sort, dim3((b + 7) & ~7), kWaveBlock, 0, s, v);
 }
-void moments(const View& v, hipStream_t s) {
+void cell_prep(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_cell_prep, dim3(cell_blocks(v.ncells)), dim3(256), 0, s, v);
+}
+void moments(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_moments, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v);
 }
 void newton(const View& v, hipStream_t s) {

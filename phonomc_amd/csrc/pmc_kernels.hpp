@@ -12,6 +12,7 @@ void inject(const pmc_dev::View& v, hipStream_t s);
 void refresh(const pmc_dev::View& v, hipStream_t s, int phase);
 void events(const pmc_dev::View& v, hipStream_t s);
 void sort(const pmc_dev::View& v, hipStream_t s);
+void cell_prep(const pmc_dev::View& v, hipStream_t s);
 void moments(const pmc_dev::View& v, hipStream_t s);
 void newton(const pmc_dev::View& v, hipStream_t s);
 void relax(const pmc_dev::View& v, hipStream_t s);

@@ -13,7 +13,8 @@ BUILD = os.path.join(PKG, "_build")
 CONFIGS = os.path.join(PKG, "configs")
 
 PMC_OK = 0
-KERNEL_CLASSES = ["begin", "transport", "events", "inject", "refresh", "sort", "collide", "output", "step"]
+KERNEL_CLASSES = ["begin", "transport", "events", "inject", "refresh", "sort", "moments", "newton", "relax", "output",
+                  "step"]
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "excess transport events", 4: "boundary failure",
           5: "escaped carrier", 6: "invalid state", 7: "capacity exhausted", 8: "latched", 9: "unsupported"}
 
