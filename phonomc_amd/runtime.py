@@ -377,6 +377,51 @@ class Context:
             pass
 
 
+class Solver:
+    """MonteCarloSolver facade (pmcs_solver): the time-step loop with the
+    reference's observables and output files.  Takes ownership of the problem."""
+
+    def __init__(self, problem, output_folder=None):
+        err = C.create_string_buffer(1024)
+        h = C.c_void_p()
+        ph, problem.h = problem.h, None
+        rc = lib().pmcs_solver_create(ph, output_folder.encode() if output_folder else None, C.byref(h), err, 1024)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        self.h = h
+        self.problem = Problem(handle=lib().pmcs_solver_problem(h), owner=self)
+        self.n_cells = int(self.problem.scalars()["cells"])
+
+    def run(self, steps):
+        err = C.create_string_buffer(1024)
+        if lib().pmcs_solver_run(self.h, steps, err, 1024) != 0:
+            raise RuntimeError(err.value.decode())
+
+    def timestep(self):
+        return lib().pmcs_solver_timestep(self.h)
+
+    def observables(self):
+        ke, ki, q = C.c_double(), C.c_double(), C.c_double()
+        t = np.zeros(self.n_cells)
+        lib().pmcs_solver_observables(self.h, C.byref(ke), C.byref(ki), C.byref(q), _ptr(t, C.c_double))
+        return {"kappa_eff": ke.value, "kappa_int": ki.value, "mean_heat_flux": q.value, "temperature": t}
+
+    def append_summary(self, path):
+        if lib().pmcs_solver_append_summary(self.h, path.encode()) != 0:
+            raise RuntimeError("summary write failed")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pmcs_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def config_text(name, **overrides):
     """Reads a bundled workload TOML and applies key = value overrides (section.key form)."""
     path = os.path.join(CONFIGS, name)
