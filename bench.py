@@ -247,6 +247,8 @@ def main():
                    "carriers": n0, "cells": int(problem.scalars()["cells"]), "time_step_ps": 0.5},
         "roofline": roofline,
         "cpu_baseline": baseline,
+        "rta": {"evaluations": led1["newton_evals"] - led0["newton_evals"],
+                "exact_carrier_sums": led1["newton_passes"] - led0["newton_passes"]},
     }
     if baseline:
         out["speedup_vs_cpu"] = value / baseline["value"]

@@ -372,7 +372,8 @@ void MonteCarloSolver::append_profile_summary(std::ostream& out) const {
     out << "step_loop_seconds = " << step_seconds_ << '\n';
     out << "particle_steps_per_second = " << (step_seconds_ > 0 ? static_cast<double>(l.carrier_steps) / step_seconds_ : 0.0)
         << '\n';
-    out << "rta_pseudo_temperature_passes = " << l.newton_passes << '\n';
+    out << "rta_pseudo_temperature_evaluations = " << l.newton_evals << '\n';
+    out << "rta_exact_carrier_sums = " << l.newton_passes << '\n';
     out << "total_energy_balance_residual_ev = " << l.total_balance_residual_ev << '\n';
 }
 
