@@ -149,6 +149,9 @@ typedef struct pmc_ledger {
     int64_t rough_events, rough_specular, rough_diffuse;
     int64_t newton_passes;         /* exact carrier sums of the pseudo-temperature solve */
     int64_t newton_evals;          /* all evaluations of the pseudo-temperature function */
+    double step_source_energy_ev;  /* heat-source energy deposited this step (EnergyLedger::record_source) */
+    double total_source_energy_ev;
+    int64_t source_updates;        /* occupation updates by the heat source */
 } pmc_ledger;
 
 typedef struct pmc_error {
@@ -158,6 +161,21 @@ typedef struct pmc_error {
     int32_t facet;
     char message[256];
 } pmc_error;
+
+/* Prescribed lattice heat source with the thermal spectrum; replaces
+ * PrescribedHeatSource (src/solver/PrescribedHeatSource.cpp:22-252), the time
+ * profile of MonteCarloSolver::local_heat_source_integrated_time_factor
+ * (src/solver/HeatSourceDriving.cpp:14-46) and PhononSourceOperator::apply
+ * (src/solver/PhononSourceOperator.cpp:120-341).  Applied as two half-step
+ * deposits around transport and collisions (src/MonteCarloSolver.cpp:366-377). */
+typedef struct pmc_heat_source {
+    int32_t n_cells;
+    const double* rate;            /* [n_cells] eV/ps */
+    const int32_t* mode_eligible;  /* [n_active] 1: the spectrum deposits into the mode */
+    int32_t square;                /* 0 constant, 1 square wave */
+    double start_ps, end_ps;       /* end < 0: no end */
+    double period_ps, on_duration_ps, duty_cycle, amplitude;
+} pmc_heat_source;
 
 /* Initial population (src/solver/Initialization.cpp:32-275). */
 typedef struct pmc_init {
@@ -174,6 +192,8 @@ int pmc_upload_material(pmc_ctx* ctx, const pmc_mode_table* table);
 int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* geometry);
 int pmc_upload_boundary_tables(pmc_ctx* ctx, int32_t n_rough, const pmc_rough_table* rough,
                                int32_t n_reservoirs, const pmc_reservoir* reservoirs);
+/* Enables the heat source (after material and geometry). */
+int pmc_upload_heat_source(pmc_ctx* ctx, const pmc_heat_source* source);
 /* Samples the initial population on the device (Initialization.cpp:48-113) and
  * performs the initial temperature refresh (Observables.cpp:12-21). */
 int pmc_initialize_carriers(pmc_ctx* ctx, const pmc_init* init);
@@ -221,9 +241,10 @@ enum pmc_kernel_class {
     PMC_KERNEL_MOMENTS = 6,    /* per-chunk pseudo-temperature sums */
     PMC_KERNEL_NEWTON = 7,     /* per-cell pseudo-temperature solve */
     PMC_KERNEL_RELAX = 8,      /* per-chunk relaxation */
-    PMC_KERNEL_OUTPUT = 9,     /* observables outside the step (heat flux) */
-    PMC_KERNEL_STEP = 10,      /* whole steps */
-    PMC_KERNEL_CLASSES = 11
+    PMC_KERNEL_SOURCE = 9,     /* heat-source half-step deposits */
+    PMC_KERNEL_OUTPUT = 10,    /* observables outside the step (heat flux) */
+    PMC_KERNEL_STEP = 11,      /* whole steps */
+    PMC_KERNEL_CLASSES = 12
 };
 /* Accumulated kernel time (ms) and launch counts per class since the last reset,
  * measured with HIP events on the context's stream while timing is enabled. */

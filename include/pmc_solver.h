@@ -52,6 +52,8 @@ int pmcs_problem_views(const pmcs_problem* problem, pmc_setup* setup, pmc_mode_t
                        pmc_init* init, int32_t* n_rough, const pmc_rough_table** rough, int32_t* n_reservoirs,
                        const pmc_reservoir** reservoirs);
 int pmcs_problem_material(const pmcs_problem* problem, pmcs_material_view* out);
+/* Heat-source view (valid while the problem lives): 1 if the input enables one, 0 if not. */
+int pmcs_problem_heat_source(const pmcs_problem* problem, pmc_heat_source* out);
 int pmcs_problem_scalars(const pmcs_problem* problem, double* out);
 /* Derived material queries, for checks against the reference's PhononMaterial. */
 double pmcs_problem_bose(const pmcs_problem* problem, double temperature, int32_t flat_mode);

@@ -350,6 +350,92 @@ double orc_rta(const pmc_mode_table* m, int ncells, const double* cell_T, int64_
     return residual;
 }
 
+// Thermal-spectrum branch of PhononSourceOperator::apply
+// (src/solver/PhononSourceOperator.cpp:120-341): cells in order, each cell's
+// eligible carriers in storage order; deposits every pending budget of a cell
+// with eligible carriers and zeroes it.
+int orc_source_deposit(const pmc_mode_table* m, int ncells, double* pending, const double* cell_T, double t_bg,
+                       const int32_t* elig, int64_t n, const int32_t* active, const int32_t* cell, double* occ,
+                       double* energy_out, int64_t* updates_out) {
+    *energy_out = 0.0;
+    *updates_out = 0;
+    double pending_total = 0.0;
+    for (int c = 0; c < ncells; ++c) pending_total += pending[c];
+    if (pending_total <= 0.0) return 0;
+    std::vector<std::vector<int64_t>> members(ncells);
+    for (int64_t i = 0; i < n; ++i) {
+        const int c = cell[i];
+        if (pending[c] <= 0.0) continue;
+        const double hw = kHbar * m->omega[active[i]];
+        if (std::isfinite(hw) && hw > 1e-14 && elig[active[i]]) members[c].push_back(i);
+    }
+    for (int c = 0; c < ncells; ++c) {
+        if (pending[c] <= 0.0 || members[c].empty()) continue; // the budget stays in the cell
+        const auto& idx = members[c];
+        const double target = pending[c] / m->energy_weight;
+        if (!std::isfinite(target)) return PMC_ERR_INVALID_STATE;
+        const double tb = std::isfinite(cell_T[c]) && cell_T[c] >= 0.0 ? cell_T[c] : t_bg;
+        auto energy = [&](double t, double* deriv) {
+            double e = 0.0, slope = 0.0;
+            for (int64_t i : idx) {
+                const double w = m->omega[active[i]];
+                const double hw = kHbar * std::max(0.0, w);
+                const double n0 = bose(tb, w), nn = bose(t, w);
+                e += hw * (nn - n0);
+                if (deriv && t > 0.0) slope += hw * (hw * nn * (nn + 1.0) / (kBoltz * t * t));
+            }
+            if (deriv) *deriv = slope;
+            return e;
+        };
+        double low = std::max(0.0, tb);
+        double high = std::max(low + 1.0, 1.05 * low + 1.0);
+        double he = energy(high, nullptr);
+        for (int x = 0; he < target && x < 60; ++x) {
+            high = 2.0 * high + 1.0;
+            he = energy(high, nullptr);
+        }
+        if (!(he >= target) || !std::isfinite(he)) return PMC_ERR_INVALID_STATE;
+        double t = 0.5 * (low + high);
+        const double tol = 1e-12 * std::max(1e-6, target);
+        for (int iter = 0; iter < 60; ++iter) {
+            double d = 0.0;
+            const double r = energy(t, &d) - target;
+            if (std::abs(r) <= tol) break;
+            if (r > 0.0) high = t;
+            else low = t;
+            double cand = 0.5 * (low + high);
+            if (d > 0.0 && std::isfinite(d)) {
+                const double nt = t - r / d;
+                if (nt > low && nt < high && std::isfinite(nt)) cand = nt;
+            }
+            t = cand;
+        }
+        std::vector<double> inc(idx.size());
+        double applied = 0.0, largest = -1.0;
+        size_t at = 0;
+        for (size_t k = 0; k < idx.size(); ++k) {
+            const double w = m->omega[active[idx[k]]];
+            inc[k] = bose(t, w) - bose(tb, w);
+            const double e = kHbar * w * inc[k];
+            applied += e;
+            if (e > largest) {
+                largest = e;
+                at = k;
+            }
+        }
+        inc[at] += (target - applied) / (kHbar * m->omega[active[idx[at]]]);
+        for (size_t k = 0; k < idx.size(); ++k) {
+            const double next = occ[idx[k]] + inc[k];
+            if (!std::isfinite(inc[k]) || !std::isfinite(next) || next < 0.0) return PMC_ERR_INVALID_STATE;
+        }
+        for (size_t k = 0; k < idx.size(); ++k) occ[idx[k]] += inc[k];
+        *energy_out += pending[c];
+        *updates_out += static_cast<int64_t>(idx.size());
+        pending[c] = 0.0;
+    }
+    return 0;
+}
+
 } // extern "C"
 
 // ---------------------------------------------------------------- whole-step simulator
@@ -375,6 +461,12 @@ struct orc_sim {
     std::vector<int> previous_absorbed;
     pmc_ledger ledger{};
     std::string error;
+    // prescribed heat source (thermal spectrum)
+    bool hs_on = false;
+    pmc_heat_source hs{};
+    std::vector<double> hs_rate, hs_pending;
+    std::vector<int> hs_elig;
+    double elapsed = 0.0;
 };
 
 namespace {
@@ -550,6 +642,38 @@ template <class T>
 void set_error(orc_sim* s, const T& msg) {
     s->error = msg;
 }
+
+// MonteCarloSolver::local_heat_source_integrated_time_factor (src/solver/HeatSourceDriving.cpp:14-46).
+double source_time_factor(const pmc_heat_source& h, double t0, double t1) {
+    const double b = std::max(t0, h.start_ps);
+    const double e = h.end_ps >= 0.0 ? std::min(t1, h.end_ps) : t1;
+    if (e <= b) return 0.0;
+    if (!h.square) return h.amplitude * (e - b);
+    const double on = h.on_duration_ps < 0.0 ? h.duty_cycle * h.period_ps : h.on_duration_ps;
+    if (on <= 0.0) return 0.0;
+    auto on_time = [&](double rel) {
+        if (!(rel > 0.0)) return 0.0;
+        const double cycles = std::floor(rel / h.period_ps);
+        return cycles * on + std::min(rel - cycles * h.period_ps, on);
+    };
+    return h.amplitude * std::max(0.0, on_time(e - h.start_ps) - on_time(b - h.start_ps));
+}
+
+// One heat-source application: PrescribedHeatSource::accrue
+// (src/solver/PrescribedHeatSource.cpp:232-252) then orc_source_deposit.
+int apply_source(orc_sim& s, double factor, double* energy_out, int64_t* updates_out) {
+    *energy_out = 0.0;
+    *updates_out = 0;
+    double pending_total = 0.0;
+    for (double p : s.hs_pending) pending_total += p;
+    if (factor <= 0.0 && pending_total <= 0.0) return 0;
+    const int nc = s.g.n_cells;
+    for (int c = 0; c < nc; ++c) s.hs_pending[c] += s.hs_rate[c] * factor;
+    std::vector<int32_t> act(s.mode.begin(), s.mode.end());
+    return orc_source_deposit(&s.m, nc, s.hs_pending.data(), s.cell_T.data(), s.setup.background_temperature,
+                              s.hs_elig.data(), static_cast<int64_t>(act.size()), act.data(), s.cell.data(),
+                              s.occ.data(), energy_out, updates_out);
+}
 } // namespace
 
 extern "C" {
@@ -661,12 +785,37 @@ int orc_seed(orc_sim* s, int64_t n, const float* position, const int32_t* mode, 
     return finish_population(s);
 }
 
+int orc_set_heat_source(orc_sim* s, const pmc_heat_source* h) {
+    if (!s || !h || h->n_cells != s->g.n_cells) return PMC_ERR_ARGUMENT;
+    s->hs = *h;
+    s->hs_rate.assign(h->rate, h->rate + h->n_cells);
+    s->hs_elig.assign(h->mode_eligible, h->mode_eligible + s->m.n_active);
+    s->hs_pending.assign(h->n_cells, 0.0);
+    s->hs_on = true;
+    return PMC_OK;
+}
+
 int orc_step(orc_sim* s, int32_t nsteps) {
     const double dt = s->setup.time_step_ps;
     const int nres = static_cast<int>(s->res.size());
     for (int it = 0; it < nsteps; ++it) {
         pmc_ledger& L = s->ledger;
         L.carrier_steps += static_cast<int64_t>(s->mode.size());
+        // First half of the symmetric source split (src/MonteCarloSolver.cpp:366-372).
+        const double mid = s->elapsed + 0.5 * dt, end = s->elapsed + dt;
+        double source_e = 0.0;
+        int64_t source_n = 0;
+        if (s->hs_on) {
+            double e = 0.0;
+            int64_t n = 0;
+            if (apply_source(*s, source_time_factor(s->hs, s->elapsed, mid), &e, &n)) {
+                set_error(s, "heat source failure");
+                return PMC_ERR_INVALID_STATE;
+            }
+            source_e += e;
+            source_n += n;
+            if (e > 0.0) refresh(*s);
+        }
         StepTally total;
         total.absorbed_by_res.assign(nres, 0);
         int err = 0;
@@ -778,10 +927,24 @@ int orc_step(orc_sim* s, int32_t nsteps) {
         const double residual = orc_rta(&s->m, s->g.n_cells, s->cell_T.data(), static_cast<int64_t>(act.size()),
                                         act.data(), s->cell.data(), s->occ.data(), dt, s->m.energy_weight,
                                         s->setup.lifetime_local, s->setup.lifetime_temperature, nullptr);
+        if (s->hs_on) { // second half (src/MonteCarloSolver.cpp:376-378)
+            double e = 0.0;
+            int64_t n = 0;
+            if (apply_source(*s, source_time_factor(s->hs, mid, end), &e, &n)) {
+                set_error(s, "heat source failure");
+                return PMC_ERR_INVALID_STATE;
+            }
+            source_e += e;
+            source_n += n;
+        }
         refresh(*s);
+        s->elapsed += dt;
         // Ledgers (src/solver/EnergyLedger.cpp:58-90, src/solver/BoundaryLedger.cpp:86-141).
         const double energy = thermal_energy(*s);
-        const double balance = energy - L.thermal_energy_ev - 0.0 - 0.0 - injected_e + total.absorbed_e - residual;
+        const double balance = energy - L.thermal_energy_ev - source_e - 0.0 - injected_e + total.absorbed_e - residual;
+        L.step_source_energy_ev = source_e;
+        L.total_source_energy_ev += source_e;
+        L.source_updates += source_n;
         L.step_absorbed = total.absorbed;
         L.step_injected = emissions;
         L.step_absorbed_energy_ev = total.absorbed_e;

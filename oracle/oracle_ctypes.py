@@ -54,6 +54,10 @@ def lib():
         L.orc_carriers.argtypes = [P, C.POINTER(d), C.POINTER(C.c_int32), C.POINTER(d), C.POINTER(C.c_int32),
                                    C.POINTER(C.c_uint32)]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_heat_source.argtypes = [P, C.POINTER(rt.HeatSource)]
+        L.orc_source_deposit.argtypes = [C.POINTER(rt.ModeTable), C.c_int, C.POINTER(d), C.POINTER(d), d,
+                                         C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(d), C.POINTER(d), C.POINTER(C.c_int64)]
         L.orc_last_error.restype = C.c_char_p
         L.orc_last_error.argtypes = [P]
         L.orc_destroy.argtypes = [P]
@@ -96,6 +100,9 @@ class Sim:
             raise RuntimeError(f"orc_create failed: {rc}")
         self.h = h
         self.n_cells = g.n_cells
+        hs = problem.heat_source()
+        if hs is not None:
+            self._check(lib().orc_set_heat_source(h, C.byref(hs)))
 
     def _check(self, rc):
         if rc:
@@ -197,6 +204,23 @@ def rta(problem, cell_temperature, active, cell, occupation, dt, local, lifetime
     res = lib().orc_rta(C.byref(m), len(ct), _dp(ct), len(a), _ip(a), _ip(c), _dp(o), dt, m.energy_weight, int(local),
                         lifetime_temperature, _dp(pseudo))
     return o, res, pseudo
+
+
+def source_deposit(problem, pending, cell_temperature, active, cell, occupation):
+    """Thermal-spectrum deposit of per-cell pending budgets (eV); returns occupations, energy, updates, pending."""
+    _, m, g, _, _, _ = problem.views()
+    hs = problem.heat_source()
+    pend = np.array(pending, dtype=np.float64)
+    ct = np.ascontiguousarray(cell_temperature, dtype=np.float64)
+    a = np.ascontiguousarray(active, dtype=np.int32)
+    c = np.ascontiguousarray(cell, dtype=np.int32)
+    o = np.array(occupation, dtype=np.float64)
+    e, n = C.c_double(), C.c_int64()
+    rc = lib().orc_source_deposit(C.byref(m), g.n_cells, _dp(pend), _dp(ct), problem.views()[0].background_temperature,
+                                  hs.mode_eligible, len(a), _ip(a), _ip(c), _dp(o), C.byref(e), C.byref(n))
+    if rc:
+        raise RuntimeError(f"orc_source_deposit: {rc}")
+    return o, e.value, n.value, pend
 
 
 def reservoir_mode(problem, reservoir_index, u, sampling):

@@ -265,6 +265,31 @@ def rta_apply(material, cell_temperature, active, cell, carrier_temperature, occ
     return occ, res.value
 
 
+def heat_source_rates(domain, ncells):
+    """PrescribedHeatSource::configure cell rates (eV/ps) of the domain's input."""
+    rates = np.zeros(ncells)
+    err = C.create_string_buffer(512)
+    if lib().ref_heat_source_rates(domain.h, _dp(rates), err, 512) != 0:
+        raise RuntimeError(err.value.decode())
+    return rates
+
+
+def source_apply(domain, material, integrated_time, cell_temperature, active, cell, occupation, particle_volume, t_bg):
+    """accrue(integrated_time) + PhononSourceOperator::apply on one carrier set."""
+    cell_temperature = np.ascontiguousarray(cell_temperature, dtype=np.float64)
+    active = np.ascontiguousarray(active, dtype=np.int32)
+    cell = np.ascontiguousarray(cell, dtype=np.int32)
+    occ = np.array(occupation, dtype=np.float64)
+    dep, upd = C.c_double(), C.c_longlong()
+    err = C.create_string_buffer(512)
+    rc = lib().ref_source_apply(domain.h, material.h, C.c_double(integrated_time), _dp(cell_temperature), len(active),
+                                _ip(active), _ip(cell), _dp(occ), C.c_double(particle_volume), C.c_double(t_bg),
+                                C.byref(dep), C.byref(upd), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return occ, dep.value, upd.value
+
+
 def run(toml_path, material_view, steps, threads=0, lookup_dt=0.1):
     mat = RefMaterial(material_view, lookup_dt)  # keeps the arrays struct alive
     secs, ps = C.c_double(), C.c_longlong()

@@ -6,6 +6,8 @@
 #include "MonteCarloSolver.h"
 #include "PhononMaterial.h"
 #include "SimulationConfig.h"
+#include "solver/PhononSourceOperator.h"
+#include "solver/PrescribedHeatSource.h"
 #include "SimulationDomain.h"
 #include "material/MaterialData.h"
 #include "solver/BackgroundCache.h"
@@ -368,6 +370,48 @@ int ref_rta_apply(void* material, int ncells, const double* cell_temperature, in
         phonomc::RtaCollisionOperator rta;
         *residual_ev = rta.apply(p, {&m}, {index.counts(), materials, temps, &index},
                                  {dt, particle_volume, local_lifetime != 0, lifetime_temperature});
+        for (int i = 0; i < n; ++i) occupation[i] = p.occupation[i];
+    });
+}
+
+int ref_heat_source_rates(void* domain, double* rates, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        phonomc::PrescribedHeatSource hs;
+        hs.configure(dom(domain).config, *dom(domain).domain);
+        if (!hs.enabled()) throw std::runtime_error("input enables no heat source");
+        const auto& r = hs.cell_rates_evps();
+        std::copy(r.begin(), r.end(), rates);
+    });
+}
+
+int ref_source_apply(void* domain, void* material, double integrated_time_ps, const double* cell_temperature,
+                     int n, const int* active, const int* cell, double* occupation, double particle_volume,
+                     double t_bg, double* deposited_ev, long long* updates, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& d = dom(domain);
+        const auto& m = mat(material);
+        const int ncells = d.domain->grid_count();
+        phonomc::PrescribedHeatSource hs;
+        hs.configure(d.config, *d.domain);
+        hs.accrue(integrated_time_ps);
+        phonomc::PhononSourceOperator op;
+        op.configure(d.config, {&m});
+        phonomc::ParticleStorage p;
+        p.reset(n);
+        for (int i = 0; i < n; ++i) {
+            p.modes[i] = m.active_mode_at(active[i]);
+            p.grid_ids[i] = cell[i];
+            p.occupation[i] = occupation[i];
+            p.material_ids[i] = 0;
+            p.alive[i] = 1;
+        }
+        phonomc::CellParticleIndex index;
+        index.ensure(p, ncells);
+        std::vector<int> materials(ncells, 0);
+        std::vector<double> temps(cell_temperature, cell_temperature + ncells);
+        const auto r = op.apply(p, {&m}, {materials, temps, &index}, hs, particle_volume, t_bg);
+        *deposited_ev = r.energy_ev;
+        *updates = r.occupation_updates;
         for (int i = 0; i < n; ++i) occupation[i] = p.occupation[i];
     });
 }

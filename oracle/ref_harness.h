@@ -99,6 +99,12 @@ int ref_refresh_temperatures(void* material, int ncells, const double* volumes, 
 
 /* RtaCollisionOperator::apply on a supplied carrier set; occupation is updated in
  * place and the residual returned through residual_ev. */
+/* PrescribedHeatSource::configure cell rates (eV/ps) for the input the domain was built from. */
+int ref_heat_source_rates(void* domain, double* rates, char* err, int errlen);
+/* PrescribedHeatSource::accrue(integrated_time) + PhononSourceOperator::apply on a carrier set. */
+int ref_source_apply(void* domain, void* material, double integrated_time_ps, const double* cell_temperature,
+                     int n, const int* active, const int* cell, double* occupation, double particle_volume,
+                     double t_bg, double* deposited_ev, long long* updates, char* err, int errlen);
 int ref_rta_apply(void* material, int ncells, const double* cell_temperature, int n,
                   const int* active, const int* cell, const double* carrier_temperature,
                   double* occupation, double dt, double particle_volume, int local_lifetime,

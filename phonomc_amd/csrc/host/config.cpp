@@ -355,7 +355,7 @@ SimConfig build(const std::map<std::string, std::string>& kv, const std::string&
     if (get("materials.folders")) c.has_layered_materials = true;
     if (auto v = get("mode_excitation.enabled")) c.has_mode_excitation = to_bool(*v);
 
-    // Heat source subset (src/config/ConfigParsing.cpp:657-766).
+    // Heat source (src/config/ConfigParsing.cpp:658-766).
     bool enabled_set = false;
     if (auto v = get("heat_source.enabled")) { c.heat_source_enabled = to_bool(*v); enabled_set = true; }
     if (auto v = get("heat_source.min")) c.heat_min = numbers(*v);
@@ -365,8 +365,25 @@ SimConfig build(const std::map<std::string, std::string>& kv, const std::string&
             throw std::runtime_error("Specify either total_power or density inputs, not both.");
         c.heat_total_power = to_double(*v);
     }
-    if (auto v = get("heat_source.spectrum"); v && lower(unquote(*v)) != "thermal") c.has_non_thermal_spectrum = true;
+    if (auto v = get("heat_source.spectrum")) {
+        const auto s = lower(unquote(*v));
+        if (s == "weighted" || s == "gaussian") c.has_non_thermal_spectrum = true;
+        else if (s != "thermal")
+            throw std::runtime_error("heat_source.spectrum must be 'thermal', 'weighted', or 'gaussian'.");
+    }
+    if (auto v = get("heat_source.branches")) {
+        for (double b : numbers(*v)) {
+            if (!std::isfinite(b) || b < 0 || b != std::floor(b) || b > 2147483647.0)
+                throw std::runtime_error("heat_source.branches must contain nonnegative integer indices.");
+            c.heat_branches.push_back(static_cast<int>(b));
+        }
+    }
+    if (get("heat_source.branch_weights") || get("heat_source.frequency_center") || get("heat_source.frequency_sigma"))
+        c.has_non_thermal_spectrum = true;
     if (auto v = get("heat_source.power_density")) c.heat_power_density = to_double(*v);
+    if (auto v = get("heat_source.power_densities")) c.heat_power_densities = numbers(*v);
+    if (auto v = get("heat_source.frequency_min")) c.heat_frequency_min = to_double(*v);
+    if (auto v = get("heat_source.frequency_max")) c.heat_frequency_max = to_double(*v);
     if (auto v = get("heat_source.profile")) {
         const auto p = lower(trim(unquote(*v)));
         if (p == "uniform") c.heat_profile = HeatProfile::Uniform;
@@ -374,23 +391,102 @@ SimConfig build(const std::map<std::string, std::string>& kv, const std::string&
         else throw std::runtime_error("heat_source.profile must be either 'uniform' or 'gaussian'.");
     }
     if (auto v = get("heat_source.center")) c.heat_center = numbers(*v);
+    if (auto v = get("heat_source.centers")) {
+        for (const auto& row : array_elements(*v)) {
+            const auto xyz = numbers(row);
+            if (xyz.size() != 3) throw std::runtime_error("heat_source.centers row must be [x,y,z].");
+            c.heat_centers.insert(c.heat_centers.end(), xyz.begin(), xyz.end());
+        }
+    }
     if (auto v = get("heat_source.sigma")) c.heat_sigma = numbers(*v);
-    if (get("heat_source.time_profile") || get("heat_source.centers") || get("heat_source.power_densities") ||
-        get("heat_source.half_width"))
-        c.has_time_profile = true;
-    if (!enabled_set) {
+    if (auto v = get("heat_source.half_width")) c.heat_half_width = numbers(*v);
+    if (auto v = get("heat_source.time_profile")) {
+        const auto p = lower(trim(unquote(*v)));
+        if (p == "constant") c.heat_time_square = false;
+        else if (p == "square") c.heat_time_square = true;
+        else throw std::runtime_error("heat_source.time_profile must be either 'constant' or 'square'.");
+    }
+    if (auto v = get("heat_source.start_time")) c.heat_time_start = to_double(*v);
+    if (auto v = get("heat_source.end_time")) c.heat_time_end = to_double(*v);
+    if (auto v = get("heat_source.period")) c.heat_period = to_double(*v);
+    if (auto v = get("heat_source.on_duration")) c.heat_on_duration = to_double(*v);
+    if (auto v = get("heat_source.duty_cycle")) c.heat_duty_cycle = to_double(*v);
+    if (auto v = get("heat_source.amplitude")) c.heat_amplitude = to_double(*v);
+    if (!enabled_set) { // infer_heat_source_enabled (src/config/ConfigParsing.cpp:503-513)
         const bool has_power = c.heat_total_power.has_value() || std::abs(c.heat_power_density) > 0.0;
         if (has_power) {
             c.heat_source_enabled = c.heat_profile == HeatProfile::Gaussian
-                                        ? (c.heat_center.size() == 3 && c.heat_sigma.size() == 3)
+                                        ? ((c.heat_center.size() == 3 || c.heat_centers.size() >= 3) &&
+                                           c.heat_sigma.size() == 3)
                                         : (c.heat_min.size() == 3 && c.heat_max.size() == 3);
         }
     }
-    for (auto* v : {&c.heat_min, &c.heat_max, &c.heat_center, &c.heat_sigma})
+    for (auto* v : {&c.heat_min, &c.heat_max, &c.heat_center, &c.heat_centers, &c.heat_sigma, &c.heat_half_width})
         for (double& x : *v) x *= 10.0;
     return c;
 }
 } // namespace
+
+// validate_heat_source_config (src/config/ConfigValidation.cpp:196-344), same messages.
+static void validate_heat_source(const SimConfig& c) {
+    if (c.heat_total_power) {
+        if (!std::isfinite(*c.heat_total_power) || !(*c.heat_total_power > 0.0))
+            throw std::runtime_error("heat_source.total_power must be finite and positive (W).");
+        if (c.heat_power_density != 0.0 || !c.heat_power_densities.empty())
+            throw std::runtime_error("heat_source.total_power cannot be combined with power_density or power_densities.");
+    } else if (!(c.heat_power_density > 0.0) || !std::isfinite(c.heat_power_density)) {
+        throw std::runtime_error("enabled heat_source requires positive total_power or power_density.");
+    }
+    for (const auto* values : {&c.heat_min, &c.heat_max, &c.heat_center, &c.heat_centers, &c.heat_sigma,
+                               &c.heat_half_width})
+        for (double v : *values)
+            if (!std::isfinite(v)) throw std::runtime_error("Heat-source spatial parameters must be finite.");
+    std::vector<int> branches = c.heat_branches;
+    std::sort(branches.begin(), branches.end());
+    if ((!branches.empty() && branches.front() < 0) || std::adjacent_find(branches.begin(), branches.end()) != branches.end())
+        throw std::runtime_error("heat_source.branches must contain unique nonnegative indices.");
+    for (double q : c.heat_power_densities)
+        if (!(q > 0.0) || !std::isfinite(q))
+            throw std::runtime_error("heat_source.power_densities values must be finite and positive.");
+    if (!(c.heat_frequency_min < 0.0) && !(std::isfinite(c.heat_frequency_min) && c.heat_frequency_min >= 0.0))
+        throw std::runtime_error("heat_source.frequency_min must be finite and non-negative if provided.");
+    if (!(c.heat_frequency_max < 0.0) && !(std::isfinite(c.heat_frequency_max) && c.heat_frequency_max >= 0.0))
+        throw std::runtime_error("heat_source.frequency_max must be finite and non-negative if provided.");
+    if (c.heat_frequency_min >= 0.0 && c.heat_frequency_max >= 0.0 && c.heat_frequency_max < c.heat_frequency_min)
+        throw std::runtime_error("heat_source.frequency_max must be no smaller than frequency_min.");
+    if (c.heat_profile == HeatProfile::Uniform) {
+        if (c.heat_min.size() != 3 || c.heat_max.size() != 3)
+            throw std::runtime_error("uniform heat_source requires min and max arrays with 3 values.");
+    } else {
+        const bool single = c.heat_center.size() == 3, multi = !c.heat_centers.empty();
+        if ((!single && !multi) || c.heat_sigma.size() != 3)
+            throw std::runtime_error("gaussian heat_source requires center=[x,y,z] or centers=[[x,y,z],...] and "
+                                     "sigma=[sx,sy,sz].");
+        if (multi && c.heat_centers.size() % 3 != 0)
+            throw std::runtime_error("heat_source.centers must contain one or more [x,y,z] rows.");
+        const size_t nsrc = multi ? c.heat_centers.size() / 3 : 1;
+        if (!c.heat_power_densities.empty() && c.heat_power_densities.size() != nsrc)
+            throw std::runtime_error("heat_source.power_densities must match the number of heat_source.centers.");
+        if (!c.heat_half_width.empty() && c.heat_half_width.size() != 3)
+            throw std::runtime_error("heat_source.half_width must contain 3 values if provided.");
+        if ((!c.heat_min.empty() || !c.heat_max.empty()) && (c.heat_min.size() != 3 || c.heat_max.size() != 3))
+            throw std::runtime_error("gaussian heat_source min/max clipping box must be omitted or contain 3 values "
+                                     "each.");
+    }
+    if (!std::isfinite(c.heat_amplitude) || c.heat_amplitude < 0.0)
+        throw std::runtime_error("heat_source.amplitude must be finite and non-negative.");
+    if (!std::isfinite(c.heat_on_duration)) throw std::runtime_error("heat_source.on_duration must be finite.");
+    if (!std::isfinite(c.heat_time_start)) throw std::runtime_error("heat_source.start_time must be finite.");
+    if (!std::isfinite(c.heat_time_end)) throw std::runtime_error("heat_source.end_time must be finite.");
+    if (!std::isfinite(c.heat_duty_cycle) || c.heat_duty_cycle < 0.0 || c.heat_duty_cycle > 1.0)
+        throw std::runtime_error("heat_source.duty_cycle must be in [0, 1].");
+    if (c.heat_time_square) {
+        if (!(c.heat_period > 0.0) || !std::isfinite(c.heat_period))
+            throw std::runtime_error("square heat_source.time_profile requires a finite positive period.");
+        if (c.heat_on_duration >= 0.0 && (!std::isfinite(c.heat_on_duration) || c.heat_on_duration > c.heat_period))
+            throw std::runtime_error("heat_source.on_duration must be finite and no larger than period.");
+    }
+}
 
 void validate_config(const SimConfig& c) {
     // Subset of src/config/ConfigValidation.cpp:457-502 and :475-501 with the same messages.
@@ -447,14 +543,7 @@ void validate_config(const SimConfig& c) {
              return !std::isfinite(v) || !(v > 0.0);
          })))
         throw std::runtime_error("geometry.sizes for box must contain three positive values.");
-    if (c.heat_source_enabled) {
-        if (!(c.heat_power_density > 0.0) && !c.heat_total_power)
-            throw std::runtime_error("heat_source power_density must be positive.");
-        if (c.heat_profile == HeatProfile::Gaussian && (c.heat_center.size() != 3 || c.heat_sigma.size() != 3))
-            throw std::runtime_error("gaussian heat_source requires center=[x,y,z] or centers and sigma=[sx,sy,sz].");
-        if (c.heat_profile == HeatProfile::Uniform && (c.heat_min.size() != 3 || c.heat_max.size() != 3))
-            throw std::runtime_error("uniform heat_source requires min=[x,y,z] and max=[x,y,z].");
-    }
+    if (c.heat_source_enabled) validate_heat_source(c);
 }
 
 SimConfig parse_toml_text(const std::string& text, const std::string& input_directory) {

@@ -48,12 +48,20 @@ struct SimConfig {
     std::string material_folder;
     std::string output_folder;
 
-    // Prescribed heat source (Angstrom inside).
+    // Prescribed heat source (Angstrom inside; include/SimulationConfig.h:103-139).
     bool heat_source_enabled = false;
     HeatProfile heat_profile = HeatProfile::Uniform;
     std::vector<double> heat_min, heat_max, heat_center, heat_sigma;
-    double heat_power_density = 0.0; // W/m^3
-    std::optional<double> heat_total_power;
+    std::vector<double> heat_centers;         // flattened [x,y,z] rows
+    std::vector<double> heat_half_width;
+    std::vector<double> heat_power_densities; // per Gaussian centre, W/m^3
+    double heat_power_density = 0.0;          // W/m^3
+    std::optional<double> heat_total_power;   // W
+    double heat_frequency_min = -1.0, heat_frequency_max = -1.0; // THz, <0: no cutoff
+    std::vector<int> heat_branches;
+    bool heat_time_square = false;
+    double heat_time_start = 0.0, heat_time_end = -1.0; // ps
+    double heat_period = 0.0, heat_on_duration = -1.0, heat_duty_cycle = 0.5, heat_amplitude = 1.0;
 
     // Keys parsed for compatibility whose features this build rejects at solver set-up.
     bool has_gradient_drive = false;
@@ -61,7 +69,6 @@ struct SimConfig {
     bool has_convergence_stop = false;
     bool has_mode_excitation = false;
     bool has_non_thermal_spectrum = false;
-    bool has_time_profile = false;
 };
 
 SimConfig parse_toml_file(const std::string& path);

@@ -15,7 +15,8 @@ namespace pmc {
 std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions& opt) {
     if (c.collision != CollisionModel::Rta)
         throw std::runtime_error("scattering.model = 'callaway' is not supported by this build (RTA only).");
-    if (c.heat_source_enabled) throw std::runtime_error("heat_source is not supported by this build.");
+    if (c.heat_source_enabled && c.has_non_thermal_spectrum)
+        throw std::runtime_error("heat_source.spectrum other than 'thermal' is not supported by this build.");
     if (c.has_gradient_drive) throw std::runtime_error("driving.temperature_gradient is not supported by this build.");
     if (c.has_layered_materials) throw std::runtime_error("materials.folders (layered materials) is not supported by this build.");
     if (c.has_convergence_stop) throw std::runtime_error("convergence.enabled requires gradient driving.");
@@ -51,6 +52,22 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     for (size_t f = 0; f < d.facets.size(); ++f) enabled[f] = d.facets[f].condition == 'R';
     p->rough = build_rough_tables(d, m, enabled, 0.0, p->facet_to_rough);
     p->reservoirs = build_reservoir_tables(d, m, p->particle_density, c.time_step, p->facet_to_reservoir);
+
+    // Prescribed heat source (src/solver/HeatSourceDriving.cpp:8-11).
+    p->heat = build_heat_source(c, d, m);
+    if (p->heat.enabled) {
+        auto& h = p->heat_view;
+        h.n_cells = d.cell_count();
+        h.rate = p->heat.rates.data();
+        h.mode_eligible = p->heat.mode_eligible.data();
+        h.square = p->heat.square ? 1 : 0;
+        h.start_ps = p->heat.start;
+        h.end_ps = p->heat.end;
+        h.period_ps = p->heat.period;
+        h.on_duration_ps = p->heat.on_duration;
+        h.duty_cycle = p->heat.duty_cycle;
+        h.amplitude = p->heat.amplitude;
+    }
 
     // C-ABI views.
     for (size_t f = 0; f < d.facets.size(); ++f) {
@@ -194,6 +211,7 @@ MonteCarloSolver::MonteCarloSolver(std::unique_ptr<Problem> problem, std::string
     throw_if_failed(pmc_upload_geometry(ctx_, &p.geometry_view));
     throw_if_failed(pmc_upload_boundary_tables(ctx_, static_cast<int>(p.rough_views.size()), p.rough_views.data(),
                                                static_cast<int>(p.reservoir_views.size()), p.reservoir_views.data()));
+    if (p.heat.enabled) throw_if_failed(pmc_upload_heat_source(ctx_, &p.heat_view));
     throw_if_failed(pmc_initialize_carriers(ctx_, &p.init));
     if (!folder_.empty()) {
         write_domain_summary(p.domain, p.config, folder_);

@@ -6,6 +6,7 @@
 // src/solver/ResultWriter.cpp:25-119).
 #pragma once
 #include "boundary_tables.hpp"
+#include "heat_source.hpp"
 #include "config.hpp"
 #include "domain.hpp"
 #include "material.hpp"
@@ -34,6 +35,8 @@ struct Problem {
     double background_energy_density = 0;
     std::vector<double> neq_background; // per active mode
     std::vector<double> omega_active, velocity_active; // per active mode (3 per mode for velocity)
+    HeatSource heat;
+    pmc_heat_source heat_view{};
     // C-ABI views (pointers into the vectors above)
     std::vector<pmc_facet> facets;
     std::vector<pmc_rough_table> rough_views;

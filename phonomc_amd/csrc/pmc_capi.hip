@@ -41,6 +41,8 @@ struct pmc_ctx {
     bool latched = false;
     pmc_error error{};
     int64_t host_step = 0;
+    double elapsed_ps = 0.0;   // accumulated like the reference's elapsed_time_
+    pmc_heat_source hs{};      // scalars of the time profile (pointers not kept)
     int error_interval = 8;   // steps between reads of the device error word
     int n_reservoirs = 0;
     double expected_injection = 0;
@@ -145,6 +147,12 @@ int collect_error(pmc_ctx* ctx) {
     case PMC_ERR_INVALID_STATE:
         if (st.error_facet == -3)
             std::snprintf(msg, sizeof msg, "Live carrier count %lld does not match the boundary ledger.", i);
+        else if (st.error_facet == -4)
+            std::snprintf(msg, sizeof msg, "Non-finite heat-source energy per carrier.");
+        else if (st.error_facet == -5)
+            std::snprintf(msg, sizeof msg, "Could not bracket the occupation-source pseudo-temperature.");
+        else if (st.error_facet == -6)
+            std::snprintf(msg, sizeof msg, "Heat source would produce an invalid occupation; cell budget retained.");
         else
             std::snprintf(msg, sizeof msg, "Invalid particle in grid energy statistics.");
         break;
@@ -214,6 +222,7 @@ int pmc_create(const pmc_setup* setup, pmc_ctx** out) {
     v.inversion = setup->temperature_inversion;
     v.series_on = 1;
     v.series_radius = 0.01;
+    v.t_bg = setup->background_temperature;
     for (auto& b : ctx->buf) {
         const size_t n = static_cast<size_t>(setup->capacity);
         b.px = ctx->alloc<float>(n);
@@ -415,6 +424,46 @@ int pmc_upload_boundary_tables(pmc_ctx* ctx, int32_t n_rough, const pmc_rough_ta
     return PMC_OK;
 }
 
+int pmc_upload_heat_source(pmc_ctx* ctx, const pmc_heat_source* h) {
+    if (!ctx || !h || !ctx->has_material || !ctx->has_geometry || h->n_cells != ctx->view.ncells || !h->rate ||
+        !h->mode_eligible)
+        return fail(ctx, PMC_ERR_ARGUMENT, "invalid heat source");
+    View& v = ctx->view;
+    const size_t nc = static_cast<size_t>(v.ncells);
+    v.hs_rate = ctx->upload(h->rate, nc);
+    v.hs_elig = ctx->upload(h->mode_eligible, static_cast<size_t>(v.na));
+    v.hs_pending = ctx->alloc<double>(nc);
+    v.hs_cell = ctx->alloc<double>(pmc_dev::kSrcCellCols * nc);
+    v.hs_acc = ctx->alloc<double>(static_cast<size_t>(v.max_chunks) * pmc_dev::kSrcCols);
+    v.chunk_src = ctx->alloc<double>(static_cast<size_t>(v.max_chunks));
+    if (!v.hs_rate || !v.hs_elig || !v.hs_pending || !v.hs_cell || !v.hs_acc || !v.chunk_src)
+        return fail(ctx, PMC_ERR_DEVICE, "heat source upload failed");
+    ctx->note(hipMemset(v.hs_pending, 0, nc * sizeof(double)));
+    ctx->note(hipMemset(v.hs_cell, 0, pmc_dev::kSrcCellCols * nc * sizeof(double)));
+    ctx->hs = *h;
+    ctx->hs.rate = nullptr;
+    ctx->hs.mode_eligible = nullptr;
+    v.hs_on = 1;
+    return check_hip(ctx, hipGetLastError(), "heat source upload");
+}
+
+// Integrated drive time over [t0, t1) (MonteCarloSolver::local_heat_source_integrated_time_factor,
+// src/solver/HeatSourceDriving.cpp:14-46).
+static double source_time_factor(const pmc_heat_source& h, double t0, double t1) {
+    const double b = std::max(t0, h.start_ps);
+    const double e = h.end_ps >= 0.0 ? std::min(t1, h.end_ps) : t1;
+    if (e <= b) return 0.0;
+    if (!h.square) return h.amplitude * (e - b);
+    const double on = h.on_duration_ps < 0.0 ? h.duty_cycle * h.period_ps : h.on_duration_ps;
+    if (on <= 0.0) return 0.0;
+    auto on_time = [&](double rel) {
+        if (!(rel > 0.0)) return 0.0;
+        const double cycles = std::floor(rel / h.period_ps);
+        return cycles * on + std::min(rel - cycles * h.period_ps, on);
+    };
+    return h.amplitude * std::max(0.0, on_time(e - h.start_ps) - on_time(b - h.start_ps));
+}
+
 static int finish_population(pmc_ctx* ctx, int64_t count) {
     View& v = view(ctx);
     DevState st{};
@@ -537,10 +586,13 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
         View& v = view(ctx);
         hipStream_t st = ctx->stream;
         StepTimer T{ctx};
+        const double mid = ctx->elapsed_ps + 0.5 * v.dt, end = ctx->elapsed_ps + v.dt;
         T.run(PMC_KERNEL_BEGIN, [&] {
             pmc_launch::begin(v, st);
             if (ctx->host_step % 100 == 0) pmc_launch::containment(v, st);
         });
+        if (v.hs_on)
+            T.run(PMC_KERNEL_SOURCE, [&] { pmc_launch::source(v, st, source_time_factor(ctx->hs, ctx->elapsed_ps, mid), 0); });
         T.run(PMC_KERNEL_TRANSPORT, [&] { pmc_launch::transport(v, st); });
         T.run(PMC_KERNEL_EVENTS, [&] { pmc_launch::events(v, st); });
         T.run(PMC_KERNEL_INJECT, [&] { pmc_launch::inject(v, st); });
@@ -552,6 +604,7 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
         T.run(PMC_KERNEL_MOMENTS, [&] { pmc_launch::moments(w, st); });
         T.run(PMC_KERNEL_NEWTON, [&] { pmc_launch::newton(w, st); });
         T.run(PMC_KERNEL_RELAX, [&] { pmc_launch::relax(w, st); });
+        if (w.hs_on) T.run(PMC_KERNEL_SOURCE, [&] { pmc_launch::source(w, st, source_time_factor(ctx->hs, mid, end), 1); });
         T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::refresh(w, st, 2); });
         int rc = check_hip(ctx, hipGetLastError(), "step launch");
         if (rc) {
@@ -560,6 +613,7 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
         }
         T.collect();
         ctx->host_step++;
+        ctx->elapsed_ps += v.dt;
         if (ctx->host_step % ctx->error_interval == 0 && s != nsteps - 1) {
             rc = collect_error(ctx);
             if (rc) return rc;
@@ -615,6 +669,9 @@ int pmc_read_ledger(pmc_ctx* ctx, pmc_ledger* out) {
     out->rough_diffuse = st.rough_diffuse;
     out->newton_passes = st.newton_passes;
     out->newton_evals = st.newton_evals;
+    out->step_source_energy_ev = st.step_source_e;
+    out->total_source_energy_ev = st.total_source_e;
+    out->source_updates = st.source_updates;
     return check_hip(ctx, hipGetLastError(), "read ledger");
 }
 

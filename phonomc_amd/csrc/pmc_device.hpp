@@ -68,6 +68,17 @@ enum Partial : int {
     kPAbsorbedRes = 8, // + reservoir index
     kPartialCols = 8 + kMaxReservoirs
 };
+// Heat-source sums per chunk: Taylor terms of sum hw n_BE(T) about the base
+// temperature (no constant term: f(T_b) = 0 by construction), eligible count,
+// carriers the series cannot take; then the increment pass's applied energy and
+// largest-energy carrier (energy, uid, slot).
+constexpr int kSrcM1 = 0;
+constexpr int kSrcCount = kSeriesOrder;
+constexpr int kSrcBad = kSeriesOrder + 1;
+constexpr int kSrcCols = kSeriesOrder + 2;
+constexpr int kSrcApplied = 0, kSrcMaxE = 1, kSrcMaxUid = 2, kSrcMaxSlot = 3, kSrcApplyCols = 4;
+constexpr int kSrcCellCols = 4; // base T, source T, target (raw eV), deposit (1) / keep pending (0)
+
 // Cell tally columns: deviational energy, lifetime residual, live carriers.
 constexpr int kTallyStride = 4;
 enum TallyCol : int { kTallyEnergy = 0, kTallyCount = 1, kTallyResidual = 2 };
@@ -126,6 +137,8 @@ struct DevState {
     double thermal_e;
     long long carrier_steps;
     long long rough_events, rough_specular, rough_diffuse;
+    double step_source_e, total_source_e;   // heat-source energy deposited (physical eV)
+    long long source_updates;               // occupation updates by the heat source
     long long newton_passes;  // exact carrier sums of the pseudo-temperature solve
     long long newton_evals;   // all evaluations (series or exact)
     int error_code;
@@ -197,7 +210,16 @@ struct View {
     int max_chunks;
     double* chunk_acc;    // kAccCols RTA sums per chunk
     double* chunk_out;    // 2 per chunk: energy after relaxation, lifetime residual
-    double* aw;           // relaxation weight of each sorted carrier (moments -> relax)
+    double* aw;           // relaxation weight of each sorted carrier (moments -> relax); source increments
+    // prescribed heat source (thermal spectrum)
+    int hs_on;
+    double t_bg;
+    const double* hs_rate;       // eV/ps per cell
+    const int* hs_elig;          // per active mode
+    double* hs_pending;          // eV per cell
+    double* hs_acc;              // kSrcCols per chunk
+    double* hs_cell;             // kSrcCellCols per cell: T_b, T_s, target, deposit flag
+    double* chunk_src;           // energy change of the second half-step deposit per chunk (tally units)
     int* events;          // per transport block: slots with a wall event this step
     int* block_events;    // per transport block: number of such slots
     DevState* st;

@@ -530,6 +530,7 @@ __global__ void k_begin(View V) {
     if (t == 0) {
         st->n_slots_start = st->n_slots;
         st->carrier_steps += st->n_live;
+        st->step_source_e = 0.0;
     }
     __syncthreads();
     if (t == 0) {
@@ -925,6 +926,7 @@ __global__ void __launch_bounds__(1024) k_refresh(View V, int phase) {
             for (int k = V.chunk_start[c]; k < V.chunk_start[c + 1]; ++k) {
                 esum += V.chunk_out[2 * static_cast<size_t>(k)];
                 residual += V.chunk_out[2 * static_cast<size_t>(k) + 1];
+                if (V.hs_on) esum += V.chunk_src[k];
             }
         }
         const double e = esum / V.cell_vol[c] + V.e_bg;
@@ -940,13 +942,15 @@ __global__ void __launch_bounds__(1024) k_refresh(View V, int phase) {
             // EnergyLedger::finish_step (src/solver/EnergyLedger.cpp:58-90) and
             // BoundaryLedger::commit_step (src/solver/BoundaryLedger.cpp:117-141).
             st->step_residual_e = res;
-            const double balance = energy - st->thermal_e - 0.0 - 0.0 - st->step_injected_e + st->step_absorbed_e - res;
+            const double balance =
+                energy - st->thermal_e - st->step_source_e - 0.0 - st->step_injected_e + st->step_absorbed_e - res;
             st->step_balance_e = balance;
             st->total_absorbed += st->step_absorbed;
             st->total_injected += st->step_injected;
             st->total_absorbed_e += st->step_absorbed_e;
             st->total_injected_e += st->step_injected_e;
             st->total_residual_e += res;
+            st->total_source_e += st->step_source_e;
             st->total_balance_e += balance;
             st->thermal_e = energy;
             st->step += 1;
@@ -1357,6 +1361,353 @@ __global__ void __launch_bounds__(kChunkThreads) k_relax(View V) {
     }
 }
 
+// ---------------------------------------------------------------- heat source
+// Prescribed lattice heat source, thermal spectrum: the deposit of
+// PhononSourceOperator::apply (src/solver/PhononSourceOperator.cpp:120-341) as
+// four kernels over the cell-sorted chunks.  half = 0 runs before transport
+// (base temperature: the previous refresh), half = 1 after relaxation (base:
+// the refresh after transport).
+
+__device__ __forceinline__ double source_base_T(const View& V, int c, int half) {
+    const double t = half == 0 ? V.cell_T[c] : V.crta[c].t0;
+    return (isfinite(t) && t >= 0.0) ? t : V.t_bg;
+}
+
+// Eligible carriers (the eligible lambda, :146-157): hw > 1e-14 eV and a mode
+// the spectrum filter keeps.
+__device__ __forceinline__ bool source_eligible(const View& V, int m, double& w, double& hw) {
+    w = V.mtab[m].x;
+    hw = kHbar * w;
+    return isfinite(hw) && hw > 1e-14 && V.hs_elig[m] != 0;
+}
+
+__device__ __forceinline__ int chunk_range(const View& V, int b, int& begin, int& end) {
+    const int c = V.chunk_cell[b];
+    begin = V.cell_start[c] + (b - V.chunk_start[c]) * kChunk;
+    end = min(begin + kChunk, V.cell_start[c + 1]);
+    return c;
+}
+
+// Per chunk: Taylor terms of sum hw n_BE(T) about the base temperature T_b of
+// the eligible carriers of cells with pending energy.
+__global__ void __launch_bounds__(kChunkThreads) k_source_sums(View V, double factor, int half) {
+    __shared__ double lds[kSrcCols * kChunkWaves];
+    const int b = blockIdx.x;
+    if (b >= V.chunk_start[V.ncells]) return;
+    int begin, end;
+    const int c = chunk_range(V, b, begin, end);
+    const double pend = V.hs_pending[c] + V.hs_rate[c] * factor;
+    double acc[kSrcCols];
+#pragma unroll
+    for (int k = 0; k < kSrcCols; ++k) acc[k] = 0.0;
+    if (pend > 0.0) {
+        const double tb = source_base_T(V, c, half);
+        const double inv = tb > 0.0 ? 1.0 / tb : 0.0;
+        for (int i = begin + static_cast<int>(threadIdx.x); i < end; i += kChunkThreads) {
+            const int m = V.mode[i] & kModeMask;
+            double w, hw;
+            if (!source_eligible(V, m, w, hw)) continue;
+            acc[kSrcCount] += 1.0;
+            const double x0 = hw * (1.0 / kBoltz) * inv;
+            if (!(tb > 0.0) || !(x0 >= 1e-6)) {
+                acc[kSrcBad] += 1.0;
+                continue;
+            }
+            const double n0 = bose(tb, w);
+            const double q = n0 * (n0 + 1.0), r = 2.0 * n0 + 1.0;
+            const double P[6] = {q,
+                                 q * r,
+                                 q * (6.0 * q + 1.0),
+                                 q * r * (12.0 * q + 1.0),
+                                 q * ((120.0 * q + 30.0) * q + 1.0),
+                                 q * r * ((360.0 * q + 60.0) * q + 1.0)};
+            constexpr double kInvFact[6] = {1.0, 1.0 / 2.0, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0, 1.0 / 720.0};
+            double cf = hw;
+#pragma unroll
+            for (int k = 1; k <= kSeriesOrder; ++k) {
+                cf *= -x0;
+                acc[kSrcM1 + k - 1] += (cf * kInvFact[k - 1]) * P[k - 1];
+            }
+        }
+        block_reduce<kSrcCols, kChunkWaves>(acc, lds);
+    }
+    if (threadIdx.x == 0) {
+        double* out = V.hs_acc + static_cast<size_t>(b) * kSrcCols;
+#pragma unroll
+        for (int k = 0; k < kSrcCols; ++k) out[k] = acc[k];
+    }
+}
+
+// Exact deposit energy sum_i hw (n_BE(T) - n_BE(T_b)) and its slope over the
+// cell's eligible carriers (thermal_increment_energy, :240-257); one wave.
+__device__ void source_exact(const View& V, int begin, int end, double tb, double t, double& energy,
+                             double& slope) {
+    double e = 0.0, s = 0.0;
+    for (int i = begin + __lane_id(); i < end; i += 64) {
+        const int m = V.mode[i] & kModeMask;
+        double w, hw;
+        if (!source_eligible(V, m, w, hw)) continue;
+        const double h = kHbar * fmax(0.0, w);
+        const double n0 = bose(tb, w), n = bose(t, w);
+        e += h * (n - n0);
+        if (t > 0.0) s += h * (h * n * (n + 1.0) / (kBoltz * t * t));
+    }
+    energy = wave_sum(e);
+    slope = wave_sum(s);
+}
+
+// Per cell (one wave): accrue the pending energy and solve for the source
+// pseudo-temperature T_s with sum hw (n_BE(T_s) - n_BE(T_b)) = pending / w to
+// 1e-12 * max(1e-6, target).  Newton from T_b on the series when it is trusted,
+// else the reference's bracket + safeguarded Newton on exact sums (:259-293).
+// Cells without eligible carriers keep their budget (:162-164).
+__global__ void __launch_bounds__(256) k_source_solve(View V, double factor, int half) {
+    const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (c >= V.ncells) return;
+    const bool lead = __lane_id() == 0;
+    double* cellv = V.hs_cell + static_cast<size_t>(kSrcCellCols) * c;
+    const double pend = V.hs_pending[c] + V.hs_rate[c] * factor;
+    if (!(pend > 0.0)) {
+        if (lead) {
+            V.hs_pending[c] = pend;
+            cellv[3] = 0.0;
+        }
+        return;
+    }
+    const int k0 = V.chunk_start[c], k1 = V.chunk_start[c + 1];
+    double acc[kSrcCols];
+#pragma unroll
+    for (int k = 0; k < kSrcCols; ++k) {
+        double x = 0.0;
+        for (int ch = k0 + __lane_id(); ch < k1; ch += 64) x += V.hs_acc[static_cast<size_t>(ch) * kSrcCols + k];
+        acc[k] = wave_sum(x);
+    }
+    if (!(acc[kSrcCount] > 0.0)) { // no eligible carrier: keep the budget locally
+        if (lead) {
+            V.hs_pending[c] = pend;
+            cellv[3] = 0.0;
+        }
+        return;
+    }
+    const int begin = V.cell_start[c], end = V.cell_start[c + 1];
+    const double tb = source_base_T(V, c, half);
+    const double target = pend / V.w;
+    if (!isfinite(target)) {
+        raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -4);
+        return;
+    }
+    const double tol = 1e-12 * fmax(1e-6, target);
+    long long exact = 0;
+    bool done = false;
+    double t = tb;
+    if (V.series_on && acc[kSrcBad] == 0.0 && tb > 0.0) {
+        for (int it = 0; it < 20; ++it) {
+            const double u = tb / t - 1.0;
+            if (!(fabs(u) <= V.series_radius)) break;
+            double sum = 0.0, dsum = 0.0, up = 1.0, last = 0.0, prev = 0.0;
+#pragma unroll
+            for (int k = 1; k <= kSeriesOrder; ++k) {
+                const double mk = acc[kSrcM1 + k - 1];
+                dsum += k * mk * up;
+                const double term = mk * up * u;
+                sum += term;
+                prev = last;
+                last = term;
+                up *= u;
+            }
+            if (last != 0.0) {
+                const double rho = fabs(last) / fabs(prev);
+                if (!(rho <= 0.5) || !(fabs(last) * rho / (1.0 - rho) <= 1e-3 * tol)) break;
+            }
+            const double value = sum - target;
+            const double slope = (tb / (t * t)) * dsum;
+            if (fabs(value) <= tol) {
+                done = true;
+                break;
+            }
+            if (!(slope > 0.0) || !isfinite(slope)) break;
+            t -= value / slope;
+            if (!(t >= tb) || !isfinite(t)) break;
+        }
+    }
+    if (!done) {
+        double low = fmax(0.0, tb);
+        double high = fmax(low + 1.0, 1.05 * low + 1.0);
+        double he, d;
+        source_exact(V, begin, end, tb, high, he, d);
+        ++exact;
+        for (int x = 0; he < target && x < 60; ++x) {
+            high = 2.0 * high + 1.0;
+            source_exact(V, begin, end, tb, high, he, d);
+            ++exact;
+        }
+        if (!(he >= target) || !isfinite(he)) {
+            raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -5);
+            return;
+        }
+        t = 0.5 * (low + high);
+        for (int iter = 0; iter < 60; ++iter) {
+            double e, slope;
+            source_exact(V, begin, end, tb, t, e, slope);
+            ++exact;
+            const double residual = e - target;
+            if (fabs(residual) <= tol) break;
+            if (residual > 0.0) high = t;
+            else low = t;
+            double cand = 0.5 * (low + high);
+            if (slope > 0.0 && isfinite(slope)) {
+                const double nt = t - residual / slope;
+                if (nt > low && nt < high && isfinite(nt)) cand = nt;
+            }
+            t = cand;
+        }
+    }
+    if (lead) {
+        cellv[0] = tb;
+        cellv[1] = t;
+        cellv[2] = target;
+        cellv[3] = 1.0;
+        V.hs_pending[c] = 0.0;
+        atomicAdd(&V.st->step_source_e, pend);
+        if (exact)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->newton_passes),
+                      static_cast<unsigned long long>(exact));
+    }
+}
+
+// Per chunk: increments n_BE(T_s) - n_BE(T_b) of the eligible carriers (kept in
+// the scratch array), their applied energy and the largest-energy carrier
+// (:296-311; ties go to the smallest carrier id).
+__global__ void __launch_bounds__(kChunkThreads) k_source_increments(View V) {
+    __shared__ double lds[3 * kChunkWaves];
+    const int b = blockIdx.x;
+    if (b >= V.chunk_start[V.ncells]) return;
+    int begin, end;
+    const int c = chunk_range(V, b, begin, end);
+    const double* cellv = V.hs_cell + static_cast<size_t>(kSrcCellCols) * c;
+    double* out = V.hs_acc + static_cast<size_t>(b) * kSrcCols;
+    if (cellv[3] == 0.0) return;
+    const double tb = cellv[0], ts = cellv[1];
+    double applied = 0.0, best = -1.0;
+    unsigned best_uid = 0xffffffffu;
+    int best_slot = -1;
+    for (int i = begin + static_cast<int>(threadIdx.x); i < end; i += kChunkThreads) {
+        const int m = V.mode[i] & kModeMask;
+        double w, hw;
+        double inc = 0.0;
+        if (source_eligible(V, m, w, hw)) {
+            inc = bose(ts, w) - bose(tb, w);
+            const double e = hw * inc;
+            applied += e;
+            const unsigned id = V.uid[i];
+            if (e > best || (e == best && id < best_uid)) {
+                best = e;
+                best_uid = id;
+                best_slot = i;
+            }
+        }
+        V.aw[i] = inc;
+    }
+    // Wave reduction of the applied sum and of the (energy, id) maximum.
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        applied += __shfl_xor(applied, off, 64);
+        const double ob = __shfl_xor(best, off, 64);
+        const unsigned ou = __shfl_xor(best_uid, off, 64);
+        const int os = __shfl_xor(best_slot, off, 64);
+        if (ob > best || (ob == best && ou < best_uid)) {
+            best = ob;
+            best_uid = ou;
+            best_slot = os;
+        }
+    }
+    const int wave = threadIdx.x >> 6;
+    if (__lane_id() == 0) {
+        lds[wave] = applied;
+        lds[kChunkWaves + wave] = best;
+        lds[2 * kChunkWaves + wave] = __longlong_as_double((static_cast<long long>(best_uid) << 32) |
+                                                           static_cast<unsigned>(best_slot));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, bb = -1.0;
+        unsigned bu = 0xffffffffu;
+        int bs = -1;
+        for (int k = 0; k < kChunkWaves; ++k) {
+            a += lds[k];
+            const long long packed = __double_as_longlong(lds[2 * kChunkWaves + k]);
+            const unsigned u = static_cast<unsigned>(packed >> 32);
+            const int s = static_cast<int>(packed & 0xffffffffll);
+            if (lds[kChunkWaves + k] > bb || (lds[kChunkWaves + k] == bb && u < bu)) {
+                bb = lds[kChunkWaves + k];
+                bu = u;
+                bs = s;
+            }
+        }
+        out[kSrcApplied] = a;
+        out[kSrcMaxE] = bb;
+        out[kSrcMaxUid] = static_cast<double>(bu);
+        out[kSrcMaxSlot] = static_cast<double>(bs);
+    }
+}
+
+// Per chunk: roundoff correction on the cell's largest-energy carrier so the
+// deposit equals the target exactly (:312-318), validation (:319-326) and the
+// occupation update; records the energy change for the following refresh.
+__global__ void __launch_bounds__(kChunkThreads) k_source_apply(View V, int half) {
+    __shared__ double lds[2 * kChunkWaves];
+    const int b = blockIdx.x;
+    if (b >= V.chunk_start[V.ncells]) return;
+    int begin, end;
+    const int c = chunk_range(V, b, begin, end);
+    const double* cellv = V.hs_cell + static_cast<size_t>(kSrcCellCols) * c;
+    if (cellv[3] == 0.0) {
+        if (half == 1 && threadIdx.x == 0) V.chunk_src[b] = 0.0;
+        return;
+    }
+    // Cell totals from the chunk rows, in chunk order.
+    double applied = 0.0, best = -1.0;
+    unsigned best_uid = 0xffffffffu;
+    int best_slot = -1;
+    for (int ch = V.chunk_start[c]; ch < V.chunk_start[c + 1]; ++ch) {
+        const double* row = V.hs_acc + static_cast<size_t>(ch) * kSrcCols;
+        applied += row[kSrcApplied];
+        const unsigned u = static_cast<unsigned>(row[kSrcMaxUid]);
+        if (row[kSrcMaxE] > best || (row[kSrcMaxE] == best && u < best_uid)) {
+            best = row[kSrcMaxE];
+            best_uid = u;
+            best_slot = static_cast<int>(row[kSrcMaxSlot]);
+        }
+    }
+    const double target = cellv[2];
+    double correction = 0.0;
+    if (best_slot >= begin && best_slot < end) {
+        const double hw = kHbar * V.mtab[V.mode[best_slot] & kModeMask].x;
+        correction = (target - applied) / hw;
+    }
+    double r[2] = {0.0, 0.0}; // energy change (tally units), updated carriers
+    for (int i = begin + static_cast<int>(threadIdx.x); i < end; i += kChunkThreads) {
+        const int m = V.mode[i] & kModeMask;
+        double w, hw;
+        if (!source_eligible(V, m, w, hw)) continue;
+        const double inc = V.aw[i] + (i == best_slot ? correction : 0.0);
+        const double next = V.occ[i] + inc;
+        if (!isfinite(inc) || !isfinite(next) || next < 0.0) {
+            raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -6);
+            continue;
+        }
+        V.occ[i] = next;
+        r[0] += (kHbar * w * inc) * V.w;
+        r[1] += 1.0;
+    }
+    block_reduce<2, kChunkWaves>(r, lds);
+    if (threadIdx.x == 0) {
+        if (half == 1) V.chunk_src[b] = r[0];
+        atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->source_updates),
+                  static_cast<unsigned long long>(r[1]));
+    }
+}
+
 __global__ void k_zero_tally(View V) {
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < kTallyStride * V.ncells; c += gridDim.x * blockDim.x)
         V.cell_tally[c] = 0.0;
@@ -1512,6 +1863,12 @@ void newton(const View& v, hipStream_t s) {
 }
 void relax(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_relax, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v);
+}
+void source(const View& v, hipStream_t s, double factor, int half) {
+    hipLaunchKernelGGL(k_source_sums, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v, factor, half);
+    hipLaunchKernelGGL(k_source_solve, dim3((v.ncells + 3) / 4), dim3(256), 0, s, v, factor, half);
+    hipLaunchKernelGGL(k_source_increments, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v);
+    hipLaunchKernelGGL(k_source_apply, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v, half);
 }
 void clear_cells(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_zero_tally, dim3(cell_blocks(kTallyStride * v.ncells)), dim3(256), 0, s, v);

@@ -13,8 +13,8 @@ BUILD = os.path.join(PKG, "_build")
 CONFIGS = os.path.join(PKG, "configs")
 
 PMC_OK = 0
-KERNEL_CLASSES = ["begin", "transport", "events", "inject", "refresh", "sort", "moments", "newton", "relax", "output",
-                  "step"]
+KERNEL_CLASSES = ["begin", "transport", "events", "inject", "refresh", "sort", "moments", "newton", "relax", "source",
+                  "output", "step"]
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "excess transport events", 4: "boundary failure",
           5: "escaped carrier", 6: "invalid state", 7: "capacity exhausted", 8: "latched", 9: "unsupported"}
 
@@ -64,6 +64,12 @@ class Reservoir(C.Structure):
                 ("triangles", C.POINTER(C.c_double))]
 
 
+class HeatSource(C.Structure):
+    _fields_ = [("n_cells", C.c_int32), ("rate", C.POINTER(C.c_double)), ("mode_eligible", C.POINTER(C.c_int32)),
+                ("square", C.c_int32), ("start_ps", C.c_double), ("end_ps", C.c_double), ("period_ps", C.c_double),
+                ("on_duration_ps", C.c_double), ("duty_cycle", C.c_double), ("amplitude", C.c_double)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("series_on", C.c_int32), ("series_radius", C.c_double), ("reserved", C.c_int32 * 6)]
 
@@ -78,7 +84,8 @@ class Ledger(C.Structure):
                 ("total_lifetime_residual_ev", C.c_double), ("total_balance_residual_ev", C.c_double),
                 ("thermal_energy_ev", C.c_double), ("carrier_steps", C.c_int64), ("rough_events", C.c_int64),
                 ("rough_specular", C.c_int64), ("rough_diffuse", C.c_int64), ("newton_passes", C.c_int64),
-                ("newton_evals", C.c_int64)]
+                ("newton_evals", C.c_int64), ("step_source_energy_ev", C.c_double),
+                ("total_source_energy_ev", C.c_double), ("source_updates", C.c_int64)]
 
 
 class Error(C.Structure):
@@ -165,6 +172,8 @@ def lib():
         "pmc_reset_kernel_times": (C.c_int, [P]),
         "pmc_set_timing": (C.c_int, [P, C.c_int32]),
         "pmc_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+        "pmc_upload_heat_source": (C.c_int, [P, C.POINTER(HeatSource)]),
+        "pmcs_problem_heat_source": (C.c_int, [P, C.POINTER(HeatSource)]),
         "pmc_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pmc_destroy": (None, [P]),
         "pmc_status_string": (C.c_char_p, [C.c_int32]),
@@ -226,6 +235,11 @@ class Problem:
                                  C.byref(ns), C.byref(v))
         return s, m, g, i, [r[k] for k in range(nr.value)], [v[k] for k in range(ns.value)]
 
+    def heat_source(self):
+        """The heat-source view, or None when the input has no heat source."""
+        hs = HeatSource()
+        return hs if lib().pmcs_problem_heat_source(self.h, C.byref(hs)) == 1 else None
+
     def material(self):
         mv = MaterialView()
         lib().pmcs_problem_material(self.h, C.byref(mv))
@@ -281,6 +295,9 @@ class Context:
         rough_arr = (RoughTable * max(1, len(rough)))(*rough)
         res_arr = (Reservoir * max(1, len(res)))(*res)
         self._check(L.pmc_upload_boundary_tables(h, len(rough), rough_arr, len(res), res_arr))
+        hs = problem.heat_source()
+        if hs is not None:
+            self._check(L.pmc_upload_heat_source(h, C.byref(hs)))
         self.n_cells = g.n_cells
         if initialize:
             self._check(L.pmc_initialize_carriers(h, C.byref(i)))
@@ -423,7 +440,8 @@ class Solver:
 
 
 def config_text(name, **overrides):
-    """Reads a bundled workload TOML and applies key = value overrides (section.key form)."""
+    """Reads a bundled workload TOML and applies key = value overrides (section.key form);
+    keys the file lacks are added to their section."""
     path = os.path.join(CONFIGS, name)
     lines = open(path).read().splitlines()
     section = None
@@ -449,7 +467,18 @@ def config_text(name, **overrides):
         elif depth:
             depth += s.count("[") - s.count("]")
         out.append(line)
-    missing = set(overrides) - done
-    if missing:
-        raise KeyError(f"override keys not present in {name}: {sorted(missing)}")
-    return "\n".join(out) + "\n"
+    # Keys the file does not set are added to their section (appended as a new
+    # section when the file has none of that name).
+    missing = sorted(set(overrides) - done)
+    text = "\n".join(out) + "\n"
+    for full in missing:
+        sec, key = full.split(".", 1)
+        header = f"[{sec}]"
+        if header in text.splitlines():
+            idx = text.splitlines().index(header)
+            rows = text.splitlines()
+            rows.insert(idx + 1, f"{key} = {overrides[full]}")
+            text = "\n".join(rows) + "\n"
+        else:
+            text += f"\n{header}\n{key} = {overrides[full]}\n"
+    return text

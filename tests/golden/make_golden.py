@@ -22,7 +22,7 @@ from oracle import ref_ctypes as R  # noqa: E402
 from phonomc_amd import runtime as rt  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
-WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml"}
+WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml"}
 
 
 def problem_and_reference(key, **overrides):
@@ -146,6 +146,26 @@ def collision_vectors(problem, rm, rng, dt, local, lifetime_t):
                 dt=dt, local=local, lifetime_t=lifetime_t)
 
 
+def source_vectors(problem, rm, rd, rng, factor):
+    """Heat-source cell rates and one deposit (accrue(factor) + PhononSourceOperator::apply)
+    on a random carrier set near the background temperature."""
+    s, m, g, i, rough, res = problem.views()
+    na, nc = m.n_active, g.n_cells
+    flat = rt.arr(m.active_flat, na, np.int32)
+    rates = R.heat_source_rates(rd, nc)
+    n = 8000
+    act = rng.integers(0, na, n).astype(np.int32)
+    cell = rng.integers(0, nc, n).astype(np.int32)
+    cell[:40] = 0  # make sure some cells are crowded
+    t = 299.0 + 2.0 * rng.random(n)
+    occ = np.array([rm.bose(t[k], flat[act[k]] // 6, flat[act[k]] % 6) for k in range(n)])
+    ct = 299.0 + 2.0 * rng.random(nc)
+    sc = problem.scalars()
+    out, dep, upd = R.source_apply(rd, rm, factor, ct, act, cell, occ, sc["particle_volume"], s.background_temperature)
+    return dict(rates=rates, factor=factor, active=act, cell=cell, occ=occ, cell_t=ct, occ_after=out, deposited=dep,
+                updates=upd)
+
+
 def whole_run(key, n, steps, seeds, interval):
     """Reference runs at reduced size: final cell temperatures and kappa per seed."""
     temps, keff, kint = [], [], []
@@ -184,6 +204,8 @@ def main():
     np.savez_compressed(os.path.join(OUT, "reservoir_c2.npz"), **reservoir_vectors(p2, rm2, rd2, rng, 0.5))
     np.savez_compressed(os.path.join(OUT, "collision_c1.npz"), **collision_vectors(p1, rm1, rng, 1.0, 0, 300.0))
     np.savez_compressed(os.path.join(OUT, "collision_c2.npz"), **collision_vectors(p2, rm2, rng, 0.5, 1, 300.0))
+    p3, rm3, rd3, _ = problem_and_reference("c1s")
+    np.savez_compressed(os.path.join(OUT, "source_c1s.npz"), **source_vectors(p3, rm3, rd3, rng, 0.5))
     if "--runs" in sys.argv:
         np.savez_compressed(os.path.join(OUT, "run_c1.npz"), **whole_run("c1", 20000, 400, list(range(1000, 1016)), 10))
         np.savez_compressed(os.path.join(OUT, "run_c2.npz"), **whole_run("c2", 200000, 300, list(range(1000, 1016)), 10))

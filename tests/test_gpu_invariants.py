@@ -19,6 +19,25 @@ def test_energy_balance_closes_every_step():
         assert abs(led["step_lifetime_residual_ev"]) <= 1e-12 * led["thermal_energy_ev"]
 
 
+def test_heat_source_deposits_its_power():
+    """Constant Gaussian source: every half-step deposits rate x dt/2 in each source
+    cell, the energy balance closes with the source term, and the film heats in
+    the middle."""
+    p = rt.Problem(rt.config_text("c1s_source.toml"))
+    hs = p.heat_source()
+    total_rate = np.ctypeslib.as_array(hs.rate, shape=(hs.n_cells,)).sum()
+    g = rt.Context(p)
+    steps, dt = 20, 1.0
+    for _ in range(steps):
+        g.step(1)
+        led = g.ledger()
+        assert abs(led["step_balance_residual_ev"]) <= 1e-12 * led["thermal_energy_ev"]
+        assert led["step_source_energy_ev"] == pytest.approx(total_rate * dt, rel=1e-12)
+    assert g.ledger()["total_source_energy_ev"] == pytest.approx(total_rate * dt * steps, rel=1e-12)
+    t, _ = g.cells()
+    assert t[9] > t[0] - 1.0 and t[10] > t[19]
+
+
 def test_isothermal_box_stays_at_background():
     text = rt.config_text("c1_cross_plane.toml", **{"simulation.particle_count": 100000,
                                                     "boundary.boundary_values": "[300, 300, 0, 0, 0, 0]",

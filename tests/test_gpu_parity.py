@@ -46,7 +46,9 @@ def test_initial_population_identical(cfg, n):
                                                    ("c1_cross_plane.toml", 20000, 1, False),
                                                    ("c2_inplane_film.toml", 60000, 1, True),
                                                    ("c2_inplane_film.toml", 60000, 1, False),
-                                                   ("c2_inplane_film.toml", 60000, 0, True)])
+                                                   ("c2_inplane_film.toml", 60000, 0, True),
+                                                   ("c1s_source.toml", 20000, 1, True),
+                                                   ("c1s_source.toml", 20000, 1, False)])
 def test_steps_match_oracle(cfg, n, sampling, series):
     """series=False: every pseudo-temperature evaluation is an exact carrier sum,
     the reference's own procedure; series=True: the per-cell Taylor series."""
@@ -67,9 +69,9 @@ def test_steps_match_oracle(cfg, n, sampling, series):
         np.testing.assert_allclose(eg, eo, rtol=1e-12)
         lg, lo = g.ledger(), o.ledger()
         for k in ["step_absorbed", "step_injected", "total_absorbed", "total_injected", "rough_events",
-                  "rough_specular", "rough_diffuse", "live_carriers"]:
+                  "rough_specular", "rough_diffuse", "live_carriers", "source_updates"]:
             assert lg[k] == lo[k], k
-        for k in ["step_absorbed_energy_ev", "step_injected_energy_ev", "thermal_energy_ev"]:
+        for k in ["step_absorbed_energy_ev", "step_injected_energy_ev", "thermal_energy_ev", "step_source_energy_ev"]:
             assert lg[k] == pytest.approx(lo[k], rel=1e-11, abs=1e-9), k
     led = g.ledger()
     assert led["newton_evals"] > 0

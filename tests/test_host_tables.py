@@ -153,3 +153,25 @@ def test_config_rejections_use_reference_messages(bad, message):
 def test_unsupported_features_fail_loudly():
     with pytest.raises(RuntimeError, match="callaway"):
         rt.Problem(rt.config_text("c1_cross_plane.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))
+
+
+def test_heat_source_rates_match_reference(golden):
+    """Cell powers of the Gaussian heat source equal PrescribedHeatSource::configure's, bit for bit."""
+    p = rt.Problem(rt.config_text("c1s_source.toml"), mode_sampling=1)
+    hs = p.heat_source()
+    assert hs is not None and hs.n_cells == 20
+    rates = np.ctypeslib.as_array(hs.rate, shape=(hs.n_cells,)).copy()
+    np.testing.assert_array_equal(rates, golden("source_c1s.npz")["rates"])
+    assert rt.Problem(rt.config_text("c1_cross_plane.toml")).heat_source() is None
+
+
+@pytest.mark.parametrize("bad,message", [
+    ({"heat_source.spectrum": '"weighted"', "heat_source.branch_weights": "[1, 1, 1, 1, 1, 1]"},
+     "spectrum other than 'thermal' is not supported"),
+    ({"heat_source.time_profile": '"square"'}, "square heat_source.time_profile requires a finite positive period."),
+    ({"heat_source.duty_cycle": "1.5"}, r"heat_source.duty_cycle must be in \[0, 1\]."),
+    ({"heat_source.power_density": "-1"}, "enabled heat_source requires positive total_power or power_density."),
+])
+def test_heat_source_rejections_use_reference_messages(bad, message):
+    with pytest.raises(RuntimeError, match=message):
+        rt.Problem(rt.config_text("c1s_source.toml", **bad))

@@ -76,6 +76,26 @@ def test_refresh_and_rta_match_reference(problems, golden, key):
     assert res == pytest.approx(float(g["residual"]), rel=1e-9, abs=1e-12)
 
 
+def test_source_deposit_matches_reference(golden):
+    """Thermal-spectrum heat-source deposit (bracket + Newton per cell, roundoff
+    correction on the largest carrier) against PhononSourceOperator::apply."""
+    from phonomc_amd import runtime as rt
+
+    g = golden("source_c1s.npz")
+    p = rt.Problem(rt.config_text("c1s_source.toml"), mode_sampling=1)
+    pending = g["rates"] * float(g["factor"])
+    occ, energy, updates, left = O.source_deposit(p, pending, g["cell_t"], g["active"], g["cell"], g["occ"])
+    np.testing.assert_allclose(occ, g["occ_after"], rtol=1e-12, atol=0)
+    assert energy == pytest.approx(float(g["deposited"]), rel=1e-14)
+    assert updates == int(g["updates"])
+    assert np.all(left == 0.0)
+    # the deposit is exact: sum of hw * increment * weight equals the budget
+    _, m, _, _, _, _ = p.views()
+    omega = np.ctypeslib.as_array(m.omega, shape=(m.n_active,))
+    dep = np.sum(6.582119569e-4 * omega[g["active"]] * (occ - g["occ"])) * m.energy_weight
+    assert dep == pytest.approx(float(g["deposited"]), rel=1e-11)
+
+
 def test_planted_mistake_is_caught(problems, golden):
     """A plausible slip (window sampled from the wrong uniform) fails the rough vectors."""
     p, g = problems("c2"), golden("rough_c2.npz")
