@@ -1,7 +1,7 @@
 """Writes the known-answer fixtures of tests/golden/ from the reference PhonoMC
 sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
 
-    make -f oracle/ref.mk REF=/root/reference && python tests/golden/make_golden.py
+    make -f oracle/ref.mk REF=/root/reference && python tests/golden/make_golden.py [--runs] [--source-runs]
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -206,6 +206,8 @@ def main():
     np.savez_compressed(os.path.join(OUT, "collision_c2.npz"), **collision_vectors(p2, rm2, rng, 0.5, 1, 300.0))
     p3, rm3, rd3, _ = problem_and_reference("c1s")
     np.savez_compressed(os.path.join(OUT, "source_c1s.npz"), **source_vectors(p3, rm3, rd3, rng, 0.5))
+    if "--source-runs" in sys.argv:
+        np.savez_compressed(os.path.join(OUT, "run_c1s.npz"), **whole_run("c1s", 20000, 400, list(range(1000, 1016)), 10))
     if "--runs" in sys.argv:
         np.savez_compressed(os.path.join(OUT, "run_c1.npz"), **whole_run("c1", 20000, 400, list(range(1000, 1016)), 10))
         np.savez_compressed(os.path.join(OUT, "run_c2.npz"), **whole_run("c2", 200000, 300, list(range(1000, 1016)), 10))
