@@ -1,0 +1,579 @@
+// Spectral neural operator for the warm start (include/pmc_surrogate.h):
+// a BF16 matrix-core GEMM with FP32 accumulation, elementwise kernels, and the
+// forward / backward / Adam passes composed from them on one HIP stream.
+#include "pmc_surrogate.h"
+#include "pmc_capi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace pmc_nn_dev {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (inputs are finite).
+__device__ __forceinline__ unsigned short to_bf16(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<unsigned short>(u >> 16);
+}
+
+struct GemmArgs {
+    int m, n, k;
+    const float* a;
+    int lda;
+    long long sa;
+    int ta;
+    const float* b;
+    int ldb;
+    long long sb;
+    int tb;
+    float* c;
+    int ldc;
+    long long sc;
+    float alpha, beta;
+};
+
+constexpr int kTM = 64, kTN = 64, kTK = 16, kPad = 4;
+
+// C = alpha op(A) op(B) + beta C for one 64x64 tile per workgroup (blockIdx.z =
+// batch).  Four waves each own a 32x32 quadrant as 2x2 MFMA 16x16x16 tiles.
+// Operands are staged through LDS as BF16 with k contiguous:
+//   A fragment of lane l: rows l%16, k = 4(l/16)..+3;  B: columns l%16, same k;
+//   result lane l holds rows 4(l/16)..+3 of column l%16.
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) unsigned short As[kTM][kTK + kPad];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[kTN][kTK + kPad];
+    const long long bz = blockIdx.z;
+    const float* A = g.a + bz * g.sa;
+    const float* B = g.b + bz * g.sb;
+    float* C = g.c + bz * g.sc;
+    const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.k; k0 += kTK) {
+#pragma unroll
+        for (int r = 0; r < (kTM * kTK) / 256; ++r) {
+            const int e = t + 256 * r;
+            int mi, ki;
+            if (g.ta) { mi = e % kTM; ki = e / kTM; }   // consecutive threads along m (contiguous for A^T)
+            else { ki = e % kTK; mi = e / kTK; }       // consecutive threads along k
+            const int gm = m0 + mi, gk = k0 + ki;
+            float v = 0.f;
+            if (gm < g.m && gk < g.k) v = g.ta ? A[static_cast<long long>(gk) * g.lda + gm] : A[static_cast<long long>(gm) * g.lda + gk];
+            As[mi][ki] = to_bf16(v);
+        }
+#pragma unroll
+        for (int r = 0; r < (kTN * kTK) / 256; ++r) {
+            const int e = t + 256 * r;
+            int ni, ki;
+            if (g.tb) { ki = e % kTK; ni = e / kTK; }   // B^T: contiguous along k
+            else { ni = e % kTN; ki = e / kTN; }       // B: contiguous along n
+            const int gn = n0 + ni, gk = k0 + ki;
+            float v = 0.f;
+            if (gn < g.n && gk < g.k) v = g.tb ? B[static_cast<long long>(gn) * g.ldb + gk] : B[static_cast<long long>(gk) * g.ldb + gn];
+            Bs[ni][ki] = to_bf16(v);
+        }
+        __syncthreads();
+        const int kq = 4 * (lane >> 4), r16 = lane & 15;
+        bf16x4 af[2], bfr[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x4*>(&As[wm + 16 * i + r16][kq]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) bfr[j] = *reinterpret_cast<const bf16x4*>(&Bs[wn + 16 * j + r16][kq]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    const int rq = 4 * (lane >> 4), cc = lane & 15;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = m0 + wm + 16 * i + rq + e, col = n0 + wn + 16 * j + cc;
+                if (row < g.m && col < g.n) {
+                    float* p = C + static_cast<long long>(row) * g.ldc + col;
+                    const float v = g.alpha * acc[i][j][e];
+                    *p = g.beta == 0.f ? v : v + g.beta * *p;
+                }
+            }
+}
+
+// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).
+__device__ __forceinline__ float gelu(float x) {
+    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+    return 0.5f * x * (1.f + tanhf(u));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+    const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+    const float th = tanhf(u);
+    return 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * 0.7978845608028654f * (1.f + 3.f * 0.044715f * x * x);
+}
+
+// pre[r][c] += bias[c]; out = act ? gelu(pre) : pre.
+__global__ void k_bias_act(float* pre, const float* bias, float* out, long long rows, int cols, int act) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= rows * cols) return;
+    const float v = pre[i] + bias[i % cols];
+    pre[i] = v;
+    out[i] = act ? gelu(v) : v;
+}
+
+// d *= gelu'(pre)
+__global__ void k_gelu_back(float* d, const float* pre, long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) d[i] *= gelu_grad(pre[i]);
+}
+
+// grad[c] = sum over rows of d[r][c] (cols <= 256).
+__global__ void k_colsum(const float* d, long long rows, int cols, float* grad) {
+    __shared__ float part[256];
+    for (int c = threadIdx.x; c < cols; c += blockDim.x) part[c] = 0.f;
+    __syncthreads();
+    const long long per = (rows + gridDim.x - 1) / gridDim.x;
+    const long long r0 = blockIdx.x * per, r1 = std::min(rows, r0 + per);
+    for (int c = threadIdx.x; c < cols; c += blockDim.x) {
+        float s = 0.f;
+        for (long long r = r0; r < r1; ++r) s += d[r * cols + c];
+        part[c] = s;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < cols; c += blockDim.x) atomicAdd(&grad[c], part[c]);
+}
+
+// dy = 2 (y - t) / n and the summed squared error.
+__global__ void k_mse(const float* y, const float* t, float* dy, long long n, float* sse) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    float e = 0.f;
+    if (i < n) {
+        const float r = y[i] - t[i];
+        dy[i] = 2.f * r / static_cast<float>(n);
+        e = r * r;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(sse, e);
+}
+
+// Adam with bias correction.
+__global__ void k_adam(float* p, const float* g, float* m, float* v, long long n, float lr, float b1, float b2,
+                       float c1, float c2) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+// Rows of a device data set by index: dst[j] = src[idx[j]] (row length len).
+__global__ void k_gather(const float* src, const int* idx, float* dst, int rows, long long len) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= rows * len) return;
+    const long long r = i / len, c = i % len;
+    dst[i] = src[static_cast<long long>(idx[r]) * len + c];
+}
+
+} // namespace pmc_nn_dev
+
+using namespace pmc_nn_dev;
+
+struct pmc_nn {
+    pmc_nn_shape sh{};
+    int S = 0, M = 0, W = 0, H = 0, L = 0, Cin = 0;
+    int64_t nparam = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    hipError_t err = hipSuccess;
+    // parameter offsets
+    int64_t oP = 0, obp = 0, oQ1 = 0, ob1 = 0, oQ2 = 0, ob2 = 0;
+    std::vector<int64_t> oR, oW, ob;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    float *D = nullptr, *G = nullptr;
+    // activations for max_batch
+    float *x = nullptr, *y = nullptr, *t = nullptr, *dy = nullptr, *sse = nullptr;
+    std::vector<float*> h, pre;          // h[0..L], pre[0..L-1]
+    float *Y = nullptr, *Z = nullptr;    // per layer spectral coefficients (kept for backward)
+    std::vector<float*> Ys, Zs;
+    float *preq = nullptr, *aq = nullptr;
+    float *dh = nullptr, *dh2 = nullptr, *dpre = nullptr, *dZ = nullptr, *dY = nullptr, *daq = nullptr;
+    // device data set for training
+    float *ds_in = nullptr, *ds_t = nullptr;
+    int* idx = nullptr;
+    int ds_n = 0;
+    int64_t adam_t = 0;
+
+    void note(hipError_t e) {
+        if (e != hipSuccess && err == hipSuccess) err = e;
+    }
+    template <class T>
+    T* alloc(size_t n) {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(1, n) * sizeof(T)) != hipSuccess) {
+            note(hipErrorOutOfMemory);
+            return nullptr;
+        }
+        allocs.push_back(p);
+        return static_cast<T*>(p);
+    }
+};
+
+namespace {
+
+void gemm(pmc_nn* nn, int m, int n, int k, int batch, float alpha, const float* a, int lda, long long sa, int ta,
+          const float* b, int ldb, long long sb, int tb, float beta, float* c, int ldc, long long sc) {
+    if (m <= 0 || n <= 0 || batch <= 0) return;
+    GemmArgs g{m, n, k, a, lda, sa, ta, b, ldb, sb, tb, c, ldc, sc, alpha, beta};
+    dim3 grid((n + kTN - 1) / kTN, (m + kTM - 1) / kTM, batch);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, nn->stream, g);
+}
+
+unsigned blocks(long long n, int t = 256) { return static_cast<unsigned>((n + t - 1) / t); }
+
+// Forward pass of `b` samples already in nn->x; leaves the prediction in nn->y.
+void forward(pmc_nn* nn, int b) {
+    const int S = nn->S, W = nn->W, M2 = 2 * nn->M, H = nn->H, L = nn->L;
+    const long long R = static_cast<long long>(b) * S;
+    float* P = nn->params;
+    gemm(nn, static_cast<int>(R), W, nn->Cin, 1, 1.f, nn->x, nn->Cin, 0, 0, P + nn->oP, W, 0, 0, 0.f, nn->h[0], W, 0);
+    hipLaunchKernelGGL(k_bias_act, dim3(blocks(R * W)), dim3(256), 0, nn->stream, nn->h[0], P + nn->obp, nn->h[0], R, W, 0);
+    for (int l = 0; l < L; ++l) {
+        float* Y = nn->Ys[l];
+        float* Z = nn->Zs[l];
+        // Y_b = D^T h_b  [2M x W]
+        gemm(nn, M2, W, S, b, 1.f, nn->D, M2, 0, 1, nn->h[l], W, static_cast<long long>(S) * W, 0, 0.f, Y, W,
+             static_cast<long long>(M2) * W);
+        // Z_m = Y_m R_m over the batch rows of each mode (rows b at stride 2M*W)
+        gemm(nn, b, 2 * W, 2 * W, nn->M, 1.f, Y, M2 * W, 2 * W, 0, P + nn->oR[l], 2 * W, 4LL * W * W, 0, 0.f, Z,
+             M2 * W, 2 * W);
+        // pre_b = G Z_b + h_b W_l
+        gemm(nn, S, W, M2, b, 1.f, nn->G, M2, 0, 0, Z, W, static_cast<long long>(M2) * W, 0, 0.f, nn->pre[l], W,
+             static_cast<long long>(S) * W);
+        gemm(nn, static_cast<int>(R), W, W, 1, 1.f, nn->h[l], W, 0, 0, P + nn->oW[l], W, 0, 0, 1.f, nn->pre[l], W, 0);
+        hipLaunchKernelGGL(k_bias_act, dim3(blocks(R * W)), dim3(256), 0, nn->stream, nn->pre[l], P + nn->ob[l],
+                           nn->h[l + 1], R, W, l < L - 1 ? 1 : 0);
+    }
+    gemm(nn, static_cast<int>(R), H, W, 1, 1.f, nn->h[L], W, 0, 0, P + nn->oQ1, H, 0, 0, 0.f, nn->preq, H, 0);
+    hipLaunchKernelGGL(k_bias_act, dim3(blocks(R * H)), dim3(256), 0, nn->stream, nn->preq, P + nn->ob1, nn->aq, R, H, 1);
+    gemm(nn, static_cast<int>(R), 1, H, 1, 1.f, nn->aq, H, 0, 0, P + nn->oQ2, 1, 0, 0, 0.f, nn->y, 1, 0);
+    hipLaunchKernelGGL(k_bias_act, dim3(blocks(R)), dim3(256), 0, nn->stream, nn->y, P + nn->ob2, nn->y, R, 1, 0);
+}
+
+// Loss and gradient of `b` samples (x, t on the device): grads overwritten.
+void backward(pmc_nn* nn, int b) {
+    const int S = nn->S, W = nn->W, M2 = 2 * nn->M, H = nn->H, L = nn->L;
+    const long long R = static_cast<long long>(b) * S;
+    float* P = nn->params;
+    float* Gd = nn->grads;
+    nn->note(hipMemsetAsync(Gd, 0, nn->nparam * sizeof(float), nn->stream));
+    nn->note(hipMemsetAsync(nn->sse, 0, sizeof(float), nn->stream));
+    hipLaunchKernelGGL(k_mse, dim3(blocks(R)), dim3(256), 0, nn->stream, nn->y, nn->t, nn->dy, R, nn->sse);
+    // output layer
+    gemm(nn, H, 1, static_cast<int>(R), 1, 1.f, nn->aq, H, 0, 1, nn->dy, 1, 0, 0, 0.f, Gd + nn->oQ2, 1, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, nn->stream, nn->dy, R, 1, Gd + nn->ob2);
+    gemm(nn, static_cast<int>(R), H, 1, 1, 1.f, nn->dy, 1, 0, 0, P + nn->oQ2, 1, 0, 1, 0.f, nn->daq, H, 0);
+    hipLaunchKernelGGL(k_gelu_back, dim3(blocks(R * H)), dim3(256), 0, nn->stream, nn->daq, nn->preq, R * H);
+    gemm(nn, W, H, static_cast<int>(R), 1, 1.f, nn->h[L], W, 0, 1, nn->daq, H, 0, 0, 0.f, Gd + nn->oQ1, H, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, nn->stream, nn->daq, R, H, Gd + nn->ob1);
+    gemm(nn, static_cast<int>(R), W, H, 1, 1.f, nn->daq, H, 0, 0, P + nn->oQ1, H, 0, 1, 0.f, nn->dh, W, 0);
+    for (int l = L - 1; l >= 0; --l) {
+        float* dpre = nn->dpre;
+        nn->note(hipMemcpyAsync(dpre, nn->dh, R * W * sizeof(float), hipMemcpyDeviceToDevice, nn->stream));
+        if (l < L - 1)
+            hipLaunchKernelGGL(k_gelu_back, dim3(blocks(R * W)), dim3(256), 0, nn->stream, dpre, nn->pre[l], R * W);
+        hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, nn->stream, dpre, R, W, Gd + nn->ob[l]);
+        // skip path
+        gemm(nn, W, W, static_cast<int>(R), 1, 1.f, nn->h[l], W, 0, 1, dpre, W, 0, 0, 0.f, Gd + nn->oW[l], W, 0);
+        gemm(nn, static_cast<int>(R), W, W, 1, 1.f, dpre, W, 0, 0, P + nn->oW[l], W, 0, 1, 0.f, nn->dh2, W, 0);
+        // spectral path: dZ_b = G^T dpre_b; dR_m = Y_m^T dZ_m; dY_m = dZ_m R_m^T; dh += D dY_b
+        gemm(nn, M2, W, S, b, 1.f, nn->G, M2, 0, 1, dpre, W, static_cast<long long>(S) * W, 0, 0.f, nn->dZ, W,
+             static_cast<long long>(M2) * W);
+        gemm(nn, 2 * W, 2 * W, b, nn->M, 1.f, nn->Ys[l], M2 * W, 2 * W, 1, nn->dZ, M2 * W, 2 * W, 0, 0.f,
+             Gd + nn->oR[l], 2 * W, 4LL * W * W);
+        gemm(nn, b, 2 * W, 2 * W, nn->M, 1.f, nn->dZ, M2 * W, 2 * W, 0, P + nn->oR[l], 2 * W, 4LL * W * W, 1, 0.f,
+             nn->dY, M2 * W, 2 * W);
+        gemm(nn, S, W, M2, b, 1.f, nn->D, M2, 0, 0, nn->dY, W, static_cast<long long>(M2) * W, 0, 1.f, nn->dh2, W,
+             static_cast<long long>(S) * W);
+        std::swap(nn->dh, nn->dh2);
+    }
+    gemm(nn, nn->Cin, W, static_cast<int>(R), 1, 1.f, nn->x, nn->Cin, 0, 1, nn->dh, W, 0, 0, 0.f, Gd + nn->oP, W, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, nn->stream, nn->dh, R, W, Gd + nn->obp);
+}
+
+int finish(pmc_nn* nn, const char* /*what*/) {
+    nn->note(hipStreamSynchronize(nn->stream));
+    nn->note(hipGetLastError());
+    return nn->err == hipSuccess ? PMC_OK : PMC_ERR_DEVICE;
+}
+
+// splitmix64 for parameter initialization and shuffles.
+uint64_t mix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double uniform(uint64_t& s) { return static_cast<double>(mix(s) >> 11) * 0x1.0p-53; }
+
+} // namespace
+
+extern "C" {
+
+int pmc_nn_create(const pmc_nn_shape* sh, pmc_nn** out) {
+    if (!sh || !out || sh->nx <= 0 || sh->nz <= 0 || sh->in_channels <= 0 || sh->width <= 0 || sh->width > 128 ||
+        sh->layers <= 0 || sh->modes_x <= 0 || sh->modes_z <= 0 || sh->modes_x > sh->nx || sh->modes_z > sh->nz ||
+        sh->hidden <= 0 || sh->hidden > 256 || sh->max_batch <= 0)
+        return PMC_ERR_ARGUMENT;
+    auto* nn = new pmc_nn();
+    nn->sh = *sh;
+    nn->S = sh->nx * sh->nz;
+    nn->M = (2 * sh->modes_x - 1) * sh->modes_z;
+    nn->W = sh->width;
+    nn->H = sh->hidden;
+    nn->L = sh->layers;
+    nn->Cin = sh->in_channels;
+    if (hipStreamCreateWithFlags(&nn->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete nn;
+        return PMC_ERR_DEVICE;
+    }
+    const int W = nn->W, H = nn->H, M2 = 2 * nn->M, S = nn->S;
+    int64_t o = 0;
+    nn->oP = o; o += static_cast<int64_t>(nn->Cin) * W;
+    nn->obp = o; o += W;
+    for (int l = 0; l < nn->L; ++l) {
+        nn->oR.push_back(o); o += static_cast<int64_t>(nn->M) * 4 * W * W;
+        nn->oW.push_back(o); o += static_cast<int64_t>(W) * W;
+        nn->ob.push_back(o); o += W;
+    }
+    nn->oQ1 = o; o += static_cast<int64_t>(W) * H;
+    nn->ob1 = o; o += H;
+    nn->oQ2 = o; o += H;
+    nn->ob2 = o; o += 1;
+    nn->nparam = o;
+    // Initialization: uniform(-a, a) with a = sqrt(3 / fan_in); zero biases.
+    std::vector<float> p(o, 0.f);
+    uint64_t seed = sh->seed;
+    auto fill = [&](int64_t off, int64_t n, double fan_in) {
+        const double a = std::sqrt(3.0 / fan_in);
+        for (int64_t i = 0; i < n; ++i) p[off + i] = static_cast<float>((2.0 * uniform(seed) - 1.0) * a);
+    };
+    fill(nn->oP, static_cast<int64_t>(nn->Cin) * W, nn->Cin);
+    for (int l = 0; l < nn->L; ++l) {
+        fill(nn->oR[l], static_cast<int64_t>(nn->M) * 4 * W * W, 2.0 * W * nn->M);
+        fill(nn->oW[l], static_cast<int64_t>(W) * W, W);
+    }
+    fill(nn->oQ1, static_cast<int64_t>(W) * H, W);
+    fill(nn->oQ2, H, H);
+    // Fourier basis of the kept modes.
+    std::vector<float> d(static_cast<size_t>(S) * M2), gm(static_cast<size_t>(S) * M2);
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int ix = 0; ix < sh->nx; ++ix)
+        for (int iz = 0; iz < sh->nz; ++iz) {
+            const int s = ix * sh->nz + iz;
+            int m = 0;
+            for (int kx = -(sh->modes_x - 1); kx < sh->modes_x; ++kx)
+                for (int kz = 0; kz < sh->modes_z; ++kz, ++m) {
+                    const double th = two_pi * (static_cast<double>(kx) * ix / sh->nx + static_cast<double>(kz) * iz / sh->nz);
+                    d[static_cast<size_t>(s) * M2 + 2 * m] = static_cast<float>(std::cos(th));
+                    d[static_cast<size_t>(s) * M2 + 2 * m + 1] = static_cast<float>(std::sin(th));
+                }
+        }
+    for (size_t i = 0; i < d.size(); ++i) gm[i] = d[i] / static_cast<float>(S);
+    const size_t B = sh->max_batch;
+    const size_t R = B * S;
+    nn->params = nn->alloc<float>(o);
+    nn->grads = nn->alloc<float>(o);
+    nn->adam_m = nn->alloc<float>(o);
+    nn->adam_v = nn->alloc<float>(o);
+    nn->D = nn->alloc<float>(d.size());
+    nn->G = nn->alloc<float>(d.size());
+    nn->x = nn->alloc<float>(R * nn->Cin);
+    nn->y = nn->alloc<float>(R);
+    nn->t = nn->alloc<float>(R);
+    nn->dy = nn->alloc<float>(R);
+    nn->sse = nn->alloc<float>(1);
+    for (int l = 0; l <= nn->L; ++l) nn->h.push_back(nn->alloc<float>(R * W));
+    for (int l = 0; l < nn->L; ++l) {
+        nn->pre.push_back(nn->alloc<float>(R * W));
+        nn->Ys.push_back(nn->alloc<float>(B * M2 * W));
+        nn->Zs.push_back(nn->alloc<float>(B * M2 * W));
+    }
+    nn->preq = nn->alloc<float>(R * H);
+    nn->aq = nn->alloc<float>(R * H);
+    nn->daq = nn->alloc<float>(R * H);
+    nn->dh = nn->alloc<float>(R * W);
+    nn->dh2 = nn->alloc<float>(R * W);
+    nn->dpre = nn->alloc<float>(R * W);
+    nn->dZ = nn->alloc<float>(B * M2 * W);
+    nn->dY = nn->alloc<float>(B * M2 * W);
+    nn->idx = nn->alloc<int>(B);
+    if (nn->err != hipSuccess) {
+        pmc_nn_destroy(nn);
+        return PMC_ERR_DEVICE;
+    }
+    nn->note(hipMemcpy(nn->params, p.data(), o * sizeof(float), hipMemcpyHostToDevice));
+    nn->note(hipMemset(nn->adam_m, 0, o * sizeof(float)));
+    nn->note(hipMemset(nn->adam_v, 0, o * sizeof(float)));
+    nn->note(hipMemcpy(nn->D, d.data(), d.size() * sizeof(float), hipMemcpyHostToDevice));
+    nn->note(hipMemcpy(nn->G, gm.data(), gm.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (nn->err != hipSuccess) {
+        pmc_nn_destroy(nn);
+        return PMC_ERR_DEVICE;
+    }
+    *out = nn;
+    return PMC_OK;
+}
+
+int64_t pmc_nn_param_count(const pmc_nn* nn) { return nn ? nn->nparam : 0; }
+
+int pmc_nn_get_params(pmc_nn* nn, float* params) {
+    if (!nn || !params) return PMC_ERR_ARGUMENT;
+    nn->note(hipMemcpyAsync(params, nn->params, nn->nparam * sizeof(float), hipMemcpyDeviceToHost, nn->stream));
+    return finish(nn, "get params");
+}
+
+int pmc_nn_set_params(pmc_nn* nn, const float* params) {
+    if (!nn || !params) return PMC_ERR_ARGUMENT;
+    nn->note(hipMemcpyAsync(nn->params, params, nn->nparam * sizeof(float), hipMemcpyHostToDevice, nn->stream));
+    return finish(nn, "set params");
+}
+
+int pmc_nn_basis(pmc_nn* nn, float* d) {
+    if (!nn || !d) return PMC_ERR_ARGUMENT;
+    nn->note(hipMemcpyAsync(d, nn->D, static_cast<size_t>(nn->S) * 2 * nn->M * sizeof(float), hipMemcpyDeviceToHost,
+                            nn->stream));
+    return finish(nn, "basis");
+}
+
+int pmc_nn_predict(pmc_nn* nn, int32_t batch, const float* in, float* out) {
+    if (!nn || !in || !out || batch <= 0) return PMC_ERR_ARGUMENT;
+    const int S = nn->S;
+    for (int b0 = 0; b0 < batch; b0 += nn->sh.max_batch) {
+        const int b = std::min(nn->sh.max_batch, batch - b0);
+        nn->note(hipMemcpyAsync(nn->x, in + static_cast<size_t>(b0) * S * nn->Cin,
+                                static_cast<size_t>(b) * S * nn->Cin * sizeof(float), hipMemcpyHostToDevice, nn->stream));
+        forward(nn, b);
+        nn->note(hipMemcpyAsync(out + static_cast<size_t>(b0) * S, nn->y, static_cast<size_t>(b) * S * sizeof(float),
+                                hipMemcpyDeviceToHost, nn->stream));
+    }
+    return finish(nn, "predict");
+}
+
+int pmc_nn_loss_grad(pmc_nn* nn, int32_t batch, const float* in, const float* target, float* loss, float* grad) {
+    if (!nn || !in || !target || !loss || batch <= 0 || batch > nn->sh.max_batch) return PMC_ERR_ARGUMENT;
+    const size_t R = static_cast<size_t>(batch) * nn->S;
+    nn->note(hipMemcpyAsync(nn->x, in, R * nn->Cin * sizeof(float), hipMemcpyHostToDevice, nn->stream));
+    nn->note(hipMemcpyAsync(nn->t, target, R * sizeof(float), hipMemcpyHostToDevice, nn->stream));
+    forward(nn, batch);
+    backward(nn, batch);
+    float sse = 0.f;
+    nn->note(hipMemcpyAsync(&sse, nn->sse, sizeof(float), hipMemcpyDeviceToHost, nn->stream));
+    if (grad) nn->note(hipMemcpyAsync(grad, nn->grads, nn->nparam * sizeof(float), hipMemcpyDeviceToHost, nn->stream));
+    const int rc = finish(nn, "loss/grad");
+    *loss = sse / static_cast<float>(R);
+    return rc;
+}
+
+int pmc_nn_train(pmc_nn* nn, int32_t n, const float* in, const float* target, int32_t epochs, int32_t batch, float lr,
+                 uint64_t seed, float* losses) {
+    if (!nn || !in || !target || n <= 0 || epochs < 0 || batch <= 0 || batch > nn->sh.max_batch) return PMC_ERR_ARGUMENT;
+    const int S = nn->S;
+    const size_t in_row = static_cast<size_t>(S) * nn->Cin;
+    if (nn->ds_n != n) {
+        nn->ds_in = nn->alloc<float>(static_cast<size_t>(n) * in_row);
+        nn->ds_t = nn->alloc<float>(static_cast<size_t>(n) * S);
+        nn->ds_n = n;
+        if (nn->err != hipSuccess) return PMC_ERR_DEVICE;
+    }
+    nn->note(hipMemcpyAsync(nn->ds_in, in, static_cast<size_t>(n) * in_row * sizeof(float), hipMemcpyHostToDevice,
+                            nn->stream));
+    nn->note(hipMemcpyAsync(nn->ds_t, target, static_cast<size_t>(n) * S * sizeof(float), hipMemcpyHostToDevice,
+                            nn->stream));
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    uint64_t s = seed;
+    const float b1 = 0.9f, b2 = 0.999f;
+    for (int e = 0; e < epochs; ++e) {
+        for (int i = n - 1; i > 0; --i) std::swap(order[i], order[mix(s) % static_cast<uint64_t>(i + 1)]);
+        double total = 0.0;
+        int nb = 0;
+        for (int b0 = 0; b0 + batch <= n; b0 += batch, ++nb) {
+            nn->note(hipMemcpyAsync(nn->idx, order.data() + b0, batch * sizeof(int), hipMemcpyHostToDevice, nn->stream));
+            hipLaunchKernelGGL(k_gather, dim3(blocks(static_cast<long long>(batch) * in_row)), dim3(256), 0, nn->stream,
+                               nn->ds_in, nn->idx, nn->x, batch, static_cast<long long>(in_row));
+            hipLaunchKernelGGL(k_gather, dim3(blocks(static_cast<long long>(batch) * S)), dim3(256), 0, nn->stream,
+                               nn->ds_t, nn->idx, nn->t, batch, static_cast<long long>(S));
+            forward(nn, batch);
+            backward(nn, batch);
+            ++nn->adam_t;
+            const float c1 = 1.f - std::pow(b1, static_cast<float>(nn->adam_t));
+            const float c2 = 1.f - std::pow(b2, static_cast<float>(nn->adam_t));
+            hipLaunchKernelGGL(k_adam, dim3(blocks(nn->nparam)), dim3(256), 0, nn->stream, nn->params, nn->grads,
+                               nn->adam_m, nn->adam_v, static_cast<long long>(nn->nparam), lr, b1, b2, c1, c2);
+            float sse = 0.f;
+            nn->note(hipMemcpyAsync(&sse, nn->sse, sizeof(float), hipMemcpyDeviceToHost, nn->stream));
+            nn->note(hipStreamSynchronize(nn->stream));
+            total += sse / (static_cast<double>(batch) * S);
+        }
+        if (losses) losses[e] = static_cast<float>(total / std::max(1, nb));
+    }
+    return finish(nn, "train");
+}
+
+void pmc_nn_destroy(pmc_nn* nn) {
+    if (!nn) return;
+    if (nn->stream) (void)hipStreamSynchronize(nn->stream);
+    for (void* p : nn->allocs) (void)hipFree(p);
+    if (nn->stream) (void)hipStreamDestroy(nn->stream);
+    delete nn;
+}
+
+int pmc_nn_gemm(int32_t m, int32_t n, int32_t k, int32_t batch, float alpha, const float* a, int32_t lda,
+                int64_t stride_a, int32_t ta, const float* b, int32_t ldb, int64_t stride_b, int32_t tb, float beta,
+                float* c, int32_t ldc, int64_t stride_c) {
+    if (m <= 0 || n <= 0 || k <= 0 || batch <= 0 || !a || !b || !c) return PMC_ERR_ARGUMENT;
+    pmc_nn tmp;
+    if (hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking) != hipSuccess) return PMC_ERR_DEVICE;
+    // Extents of each operand (last element + 1).
+    const size_t na = static_cast<size_t>(stride_a) * (batch - 1) + (ta ? static_cast<size_t>(k - 1) * lda + m
+                                                                         : static_cast<size_t>(m - 1) * lda + k);
+    const size_t nb = static_cast<size_t>(stride_b) * (batch - 1) + (tb ? static_cast<size_t>(n - 1) * ldb + k
+                                                                         : static_cast<size_t>(k - 1) * ldb + n);
+    const size_t nc = static_cast<size_t>(stride_c) * (batch - 1) + static_cast<size_t>(m - 1) * ldc + n;
+    float* da = tmp.alloc<float>(na);
+    float* db = tmp.alloc<float>(nb);
+    float* dc = tmp.alloc<float>(nc);
+    if (tmp.err == hipSuccess) {
+        tmp.note(hipMemcpy(da, a, na * sizeof(float), hipMemcpyHostToDevice));
+        tmp.note(hipMemcpy(db, b, nb * sizeof(float), hipMemcpyHostToDevice));
+        tmp.note(hipMemcpy(dc, c, nc * sizeof(float), hipMemcpyHostToDevice));
+        gemm(&tmp, m, n, k, batch, alpha, da, lda, stride_a, ta, db, ldb, stride_b, tb, beta, dc, ldc, stride_c);
+        tmp.note(hipStreamSynchronize(tmp.stream));
+        tmp.note(hipGetLastError());
+        tmp.note(hipMemcpy(c, dc, nc * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    const int rc = tmp.err == hipSuccess ? PMC_OK : PMC_ERR_DEVICE;
+    for (void* p : tmp.allocs) (void)hipFree(p);
+    (void)hipStreamDestroy(tmp.stream);
+    return rc;
+}
+
+} // extern "C"

@@ -16,7 +16,8 @@ def declared(header):
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(pmcs?_\w+)\s*\(", text, flags=re.M)))
 
 
-@pytest.mark.parametrize("header,lib", [("pmc_capi.h", "libpmc_device.so"), ("pmc_solver.h", "libpmc_host.so")])
+@pytest.mark.parametrize("header,lib", [("pmc_capi.h", "libpmc_device.so"), ("pmc_solver.h", "libpmc_host.so"),
+                                        ("pmc_surrogate.h", "libpmc_device.so")])
 def test_every_declared_symbol_is_exported(header, lib):
     names = declared(header)
     assert len(names) > 8
