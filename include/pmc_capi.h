@@ -152,6 +152,7 @@ typedef struct pmc_ledger {
     double step_source_energy_ev;  /* heat-source energy deposited this step (EnergyLedger::record_source) */
     double total_source_energy_ev;
     int64_t source_updates;        /* occupation updates by the heat source */
+    int64_t source_exact_passes;   /* exact carrier sums of the source pseudo-temperature solve */
 } pmc_ledger;
 
 typedef struct pmc_error {

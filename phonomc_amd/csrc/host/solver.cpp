@@ -392,6 +392,7 @@ void MonteCarloSolver::append_profile_summary(std::ostream& out) const {
         << '\n';
     out << "rta_pseudo_temperature_evaluations = " << l.newton_evals << '\n';
     out << "rta_exact_carrier_sums = " << l.newton_passes << '\n';
+    out << "source_exact_carrier_sums = " << l.source_exact_passes << '\n';
     out << "total_energy_balance_residual_ev = " << l.total_balance_residual_ev << '\n';
 }
 

@@ -672,6 +672,7 @@ int pmc_read_ledger(pmc_ctx* ctx, pmc_ledger* out) {
     out->step_source_energy_ev = st.step_source_e;
     out->total_source_energy_ev = st.total_source_e;
     out->source_updates = st.source_updates;
+    out->source_exact_passes = st.source_passes;
     return check_hip(ctx, hipGetLastError(), "read ledger");
 }
 

@@ -139,6 +139,7 @@ struct DevState {
     long long rough_events, rough_specular, rough_diffuse;
     double step_source_e, total_source_e;   // heat-source energy deposited (physical eV)
     long long source_updates;               // occupation updates by the heat source
+    long long source_passes;                // exact carrier sums of the source pseudo-temperature solve
     long long newton_passes;  // exact carrier sums of the pseudo-temperature solve
     long long newton_evals;   // all evaluations (series or exact)
     int error_code;

@@ -1501,8 +1501,12 @@ __global__ void __launch_bounds__(256) k_source_solve(View V, double factor, int
     bool done = false;
     double t = tb;
     if (V.series_on && acc[kSrcBad] == 0.0 && tb > 0.0) {
+        // Newton on d = T_s - T_b; u = T_b / T - 1 = -d / (T_b + d) keeps full
+        // precision for the millikelvin increments of a half-step deposit.
+        double d = 0.0;
         for (int it = 0; it < 20; ++it) {
-            const double u = tb / t - 1.0;
+            const double tt = tb + d;
+            const double u = -d / tt;
             if (!(fabs(u) <= V.series_radius)) break;
             double sum = 0.0, dsum = 0.0, up = 1.0, last = 0.0, prev = 0.0;
 #pragma unroll
@@ -1519,15 +1523,17 @@ __global__ void __launch_bounds__(256) k_source_solve(View V, double factor, int
                 const double rho = fabs(last) / fabs(prev);
                 if (!(rho <= 0.5) || !(fabs(last) * rho / (1.0 - rho) <= 1e-3 * tol)) break;
             }
+            // d/dT of sum_k M_k u^k is -(T_b / T^2) sum_k k M_k u^(k-1).
             const double value = sum - target;
-            const double slope = (tb / (t * t)) * dsum;
+            const double slope = -(tb / (tt * tt)) * dsum;
             if (fabs(value) <= tol) {
+                t = tt;
                 done = true;
                 break;
             }
             if (!(slope > 0.0) || !isfinite(slope)) break;
-            t -= value / slope;
-            if (!(t >= tb) || !isfinite(t)) break;
+            d -= value / slope;
+            if (!(d >= 0.0) || !isfinite(d)) break;
         }
     }
     if (!done) {
@@ -1570,7 +1576,7 @@ __global__ void __launch_bounds__(256) k_source_solve(View V, double factor, int
         V.hs_pending[c] = 0.0;
         atomicAdd(&V.st->step_source_e, pend);
         if (exact)
-            atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->newton_passes),
+            atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->source_passes),
                       static_cast<unsigned long long>(exact));
     }
 }

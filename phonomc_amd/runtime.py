@@ -85,7 +85,8 @@ class Ledger(C.Structure):
                 ("thermal_energy_ev", C.c_double), ("carrier_steps", C.c_int64), ("rough_events", C.c_int64),
                 ("rough_specular", C.c_int64), ("rough_diffuse", C.c_int64), ("newton_passes", C.c_int64),
                 ("newton_evals", C.c_int64), ("step_source_energy_ev", C.c_double),
-                ("total_source_energy_ev", C.c_double), ("source_updates", C.c_int64)]
+                ("total_source_energy_ev", C.c_double), ("source_updates", C.c_int64),
+                ("source_exact_passes", C.c_int64)]
 
 
 class Error(C.Structure):
@@ -276,7 +277,9 @@ class Problem:
 class Context:
     """Device context (pmc_ctx) built from a Problem's views."""
 
-    def __init__(self, problem, seed=None, mode_sampling=None, capacity=None, initialize=True):
+    def __init__(self, problem, seed=None, mode_sampling=None, capacity=None, initialize=True, initial_field=None):
+        """initial_field: per-cell initial temperature [n_cells] (K), the warm start
+        (pmc_init.temperature_mode = 2); otherwise the input's own initial temperature."""
         L = lib()
         s, m, g, i, rough, res = problem.views()
         if seed is not None:
@@ -299,6 +302,12 @@ class Context:
         if hs is not None:
             self._check(L.pmc_upload_heat_source(h, C.byref(hs)))
         self.n_cells = g.n_cells
+        if initial_field is not None:
+            self._field = np.ascontiguousarray(initial_field, dtype=np.float64)
+            if self._field.shape != (g.n_cells,):
+                raise ValueError(f"initial_field must hold {g.n_cells} cell temperatures")
+            i.temperature_mode = 2
+            i.cell_temperature = _ptr(self._field, C.c_double)
         if initialize:
             self._check(L.pmc_initialize_carriers(h, C.byref(i)))
 
