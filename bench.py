@@ -16,6 +16,15 @@ largest share of step time (HIP-event durations on the step's stream); the
 cpu_baseline object times the reference PhonoMC solver compiled from its own
 sources (oracle/_ref) on a bounded sample of the same workload on the host
 cores of this machine.
+
+The warm_start object is the second half of the metric: the spectral operator
+is trained on the device (BF16 MFMA) from the committed heated-film data set
+(phonomc_amd/data/heated_film_fields.npz), then plain and warm-started Monte
+Carlo of the hottest held-out case are timed to 0.5 K RMS of a long plain
+run's field (phonomc_amd/warmstart.py: evaluate).  Its roofline object is the
+operator's GEMM rate (algorithmic flops over HIP-event durations of the GEMM
+launches of forward + backward passes at the training batch) against the
+dense BF16 MFMA peak.
 """
 import argparse
 import json
@@ -43,6 +52,9 @@ KERNEL_NAMES = {"transport": "k_transport", "sort": "k_sort", "moments": "k_mome
 # (read and write pos 12, mode 4, occ 8, cell 4) and collision pass (read mode,
 # cell, occ; write occ).
 STEP_BYTES = 80
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+WARM_DATA = os.path.join(ROOT, "phonomc_amd", "data", "heated_film_fields.npz")
+WARM_TRAIN = 144  # the first 144 cases train, the rest are held out
 
 
 def log(*a):
@@ -152,6 +164,66 @@ def pmc_summary(fetch_csv, write_csv, copy_mib, command):
         log(f"[bench] {name}: {f / 2**20:.1f} MiB read + {w / 2**20:.1f} MiB written per launch")
 
 
+def gemm_rate(model, x, y, reps=50):
+    """Roofline object of the operator's GEMMs: forward + backward passes at the
+    training batch (8 cases), algorithmic flops over HIP-event GEMM durations."""
+    from phonomc_amd import warmstart as ws
+
+    op = model.op
+    op.set_timing(True)
+    op.gemm_stats(reset=True)
+    for _ in range(reps):
+        op.loss_grad(x[:8], y[:8] / ws.Y_SCALE)
+    flops, launches, gemm_ms = op.gemm_stats(reset=True)
+    op.set_timing(False)
+    tflops = flops / (gemm_ms / 1e3) / 1e12
+    return {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tflops / MFMA_BF16_PEAK_TFLOPS, "traffic": None, "kernel": "k_gemm",
+            "flops_per_pass": flops / reps, "launches_per_pass": launches / reps,
+            "gemm_ms_per_pass": gemm_ms / reps, "avg_launch_ms": gemm_ms / max(1, launches)}
+
+
+def warm_start_leg(epochs=None, evaluate=True):
+    import numpy as np
+
+    from phonomc_amd import warmstart as ws
+
+    params, x, y = ws.load_dataset(WARM_DATA)
+    model = ws.WarmStart()
+    t0 = time.perf_counter()
+    losses = model.fit(x[:WARM_TRAIN], y[:WARM_TRAIN], **({"epochs": epochs} if epochs else {}))
+    train_s = time.perf_counter() - t0
+    pred = model.predict(x[WARM_TRAIN:])
+    held = np.sqrt(np.mean((pred - y[WARM_TRAIN:]) ** 2, axis=1))
+    without = np.sqrt(np.mean(y[WARM_TRAIN:] ** 2, axis=1))
+    log(f"[bench] operator trained in {train_s:.1f} s: held-out RMS {held.mean():.3f} K "
+        f"(reservoir field {without.mean():.3f} K)")
+    roofline = gemm_rate(model, x, y)
+    log(f"[bench] operator GEMMs: {roofline['achieved']:.3f} TFLOP/s, {roofline['launches_per_pass']:.0f} launches "
+        f"per forward + backward, {roofline['avg_launch_ms'] * 1e3:.1f} us per launch")
+    if not evaluate:
+        return {"held_out_rms_k": float(held.mean()), "train_seconds": train_s, "train_epochs": int(len(losses)),
+                "roofline": roofline}
+    k = int(np.argmax(y[WARM_TRAIN:].max(axis=1)))
+    case = ws.cases_from_params(params[WARM_TRAIN:])[k]
+    ev = ws.evaluate(model, case, particles=1000000, log=log)
+    return {
+        "workload": "heated film 500x100 nm, 50x10 cells, Gaussian line source, reservoirs 295-305 K, "
+                    "1e6 carriers, dt 0.5 ps",
+        "tol_k": ev["tol_k"], "window_steps": ev["window_steps"], "particles": ev["particles"],
+        "case": ev["case"], "peak_rise_k": ev["peak_rise_k"],
+        "plain_steps_to_steady": None if ev["plain_windows"] is None else ev["plain_windows"] * ev["window_steps"],
+        "warm_steps_to_steady": None if ev["warm_windows"] is None else ev["warm_windows"] * ev["window_steps"],
+        "plain_seconds": ev["plain_seconds"], "warm_seconds": ev["warm_seconds"],
+        "speedup_steps": ev.get("speedup_steps"), "speedup_wall": ev.get("speedup_wall"),
+        "operator_rms_k": ev["surrogate_rms_k"], "uniform_start_rms_k": ev["uniform_rms_k"],
+        "plain_late_rms_k": ev["plain_late_rms_k"], "warm_late_rms_k": ev["warm_late_rms_k"],
+        "held_out_rms_k": float(held.mean()), "held_out_cases": int(len(held)),
+        "train_cases": WARM_TRAIN, "train_seconds": train_s, "train_epochs": int(len(losses)),
+        "roofline": roofline,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +232,9 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-warm-start", action="store_true", help="skip the warm-start leg")
+    ap.add_argument("--surrogate-only", type=int, metavar="EPOCHS", default=0,
+                    help="only train the operator for EPOCHS epochs and print its GEMM roofline (for profiling)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one device copy of this size first (byte-counter calibration)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -176,6 +251,9 @@ def main():
 
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
+    if args.surrogate_only:
+        print(json.dumps(warm_start_leg(epochs=args.surrogate_only, evaluate=False)), flush=True)
+        return 0
 
     from phonomc_amd import runtime as rt
 
@@ -229,8 +307,16 @@ def main():
         baseline = cpu_baseline(text, args.cpu_steps)
         log(f"[bench] CPU baseline {baseline['value']:.4e} particle-steps/s on {baseline['cores']} threads")
 
+    ctx.close()
+    warm = None
+    if not args.no_warm_start:
+        log("[bench] warm start: training the operator and timing plain vs warm-started runs")
+        warm = warm_start_leg()
+        log(f"[bench] warm start: {warm['plain_steps_to_steady']} vs {warm['warm_steps_to_steady']} steps to 0.5 K "
+            f"RMS ({warm['speedup_wall']:.1f}x wall); GEMM {warm['roofline']['achieved']:.2f} TFLOP/s")
+
     out = {
-        "metric": "particle-steps/s @1 GPU",
+        "metric": "particle-steps/s @1 GPU; time-to-steady T (0.5 K RMS) warm-start vs plain",
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -252,8 +338,8 @@ def main():
     }
     if baseline:
         out["speedup_vs_cpu"] = value / baseline["value"]
+    out["warm_start"] = warm
     print(json.dumps(out), flush=True)
-    ctx.close()
     return 0
 
 

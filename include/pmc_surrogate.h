@@ -50,6 +50,12 @@ int pmc_nn_loss_grad(pmc_nn* nn, int32_t batch, const float* in, const float* ta
  * losses[epochs] receives the mean training loss of each epoch. */
 int pmc_nn_train(pmc_nn* nn, int32_t n, const float* in, const float* target, int32_t epochs, int32_t batch,
                  float lr, uint64_t seed, float* losses);
+/* GEMM accounting of the operator's own launches: algorithmic flops
+ * (2 m n k per product, BF16 MFMA), launch count and, while timing is on,
+ * the summed HIP-event durations of those launches in ms.  reset != 0 zeroes
+ * the counters after reading them. */
+int pmc_nn_set_timing(pmc_nn* nn, int32_t on);
+int pmc_nn_gemm_stats(pmc_nn* nn, double* flops, int64_t* launches, double* ms, int32_t reset);
 void pmc_nn_destroy(pmc_nn* nn);
 
 /* C[b] = alpha op(A[b]) op(B[b]) + beta C[b] on host arrays through the device

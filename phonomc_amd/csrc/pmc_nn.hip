@@ -39,19 +39,27 @@ struct GemmArgs {
     int ldc;
     long long sc;
     float alpha, beta;
+    int splits;  // workgroups along k per tile (blockIdx.z = batch * splits + split)
+    int kchunk;  // k extent of one split, a multiple of kTK
+    float* ws;   // splits > 1: partial products [batch][splits][m][n]
 };
 
-constexpr int kTM = 64, kTN = 64, kTK = 16, kPad = 4;
+constexpr int kTM = 64, kTN = 64, kTK = 64, kPad = 4;
 
-// C = alpha op(A) op(B) + beta C for one 64x64 tile per workgroup (blockIdx.z =
-// batch).  Four waves each own a 32x32 quadrant as 2x2 MFMA 16x16x16 tiles.
-// Operands are staged through LDS as BF16 with k contiguous:
+// C = alpha op(A) op(B) + beta C for one 64x64 tile per workgroup.  Four
+// waves each own a 32x32 quadrant as 2x2 MFMA 16x16x16 tiles.  Operands are
+// staged through LDS as BF16 with k contiguous, 64 k per stage (16 loads in
+// flight per operand and thread):
 //   A fragment of lane l: rows l%16, k = 4(l/16)..+3;  B: columns l%16, same k;
 //   result lane l holds rows 4(l/16)..+3 of column l%16.
+// With splits > 1 each workgroup sums one k range and writes alpha * partial
+// to the workspace; k_splitk_reduce adds the splits in order.
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
     __shared__ __attribute__((aligned(16))) unsigned short As[kTM][kTK + kPad];
     __shared__ __attribute__((aligned(16))) unsigned short Bs[kTN][kTK + kPad];
-    const long long bz = blockIdx.z;
+    const long long bz = blockIdx.z / g.splits;
+    const int split = static_cast<int>(blockIdx.z % g.splits);
+    const int kbeg = split * g.kchunk, kend = min(g.k, kbeg + g.kchunk);
     const float* A = g.a + bz * g.sa;
     const float* B = g.b + bz * g.sb;
     float* C = g.c + bz * g.sc;
@@ -63,7 +71,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < g.k; k0 += kTK) {
+    for (int k0 = kbeg; k0 < kend; k0 += kTK) {
 #pragma unroll
         for (int r = 0; r < (kTM * kTK) / 256; ++r) {
             const int e = t + 256 * r;
@@ -72,7 +80,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
             else { ki = e % kTK; mi = e / kTK; }       // consecutive threads along k
             const int gm = m0 + mi, gk = k0 + ki;
             float v = 0.f;
-            if (gm < g.m && gk < g.k) v = g.ta ? A[static_cast<long long>(gk) * g.lda + gm] : A[static_cast<long long>(gm) * g.lda + gk];
+            if (gm < g.m && gk < kend) v = g.ta ? A[static_cast<long long>(gk) * g.lda + gm] : A[static_cast<long long>(gm) * g.lda + gk];
             As[mi][ki] = to_bf16(v);
         }
 #pragma unroll
@@ -83,20 +91,24 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
             else { ni = e % kTN; ki = e / kTN; }       // B: contiguous along n
             const int gn = n0 + ni, gk = k0 + ki;
             float v = 0.f;
-            if (gn < g.n && gk < g.k) v = g.tb ? B[static_cast<long long>(gn) * g.ldb + gk] : B[static_cast<long long>(gk) * g.ldb + gn];
+            if (gn < g.n && gk < kend) v = g.tb ? B[static_cast<long long>(gn) * g.ldb + gk] : B[static_cast<long long>(gk) * g.ldb + gn];
             Bs[ni][ki] = to_bf16(v);
         }
         __syncthreads();
         const int kq = 4 * (lane >> 4), r16 = lane & 15;
-        bf16x4 af[2], bfr[2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x4*>(&As[wm + 16 * i + r16][kq]);
+        for (int ks = 0; ks < kTK; ks += 16) {
+            bf16x4 af[2], bfr[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) bfr[j] = *reinterpret_cast<const bf16x4*>(&Bs[wn + 16 * j + r16][kq]);
+            for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x4*>(&As[wm + 16 * i + r16][ks + kq]);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) bfr[j] = *reinterpret_cast<const bf16x4*>(&Bs[wn + 16 * j + r16][ks + kq]);
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        }
         __syncthreads();
     }
     const int rq = 4 * (lane >> 4), cc = lane & 15;
@@ -108,11 +120,30 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
             for (int e = 0; e < 4; ++e) {
                 const int row = m0 + wm + 16 * i + rq + e, col = n0 + wn + 16 * j + cc;
                 if (row < g.m && col < g.n) {
-                    float* p = C + static_cast<long long>(row) * g.ldc + col;
                     const float v = g.alpha * acc[i][j][e];
-                    *p = g.beta == 0.f ? v : v + g.beta * *p;
+                    if (g.splits > 1) {
+                        g.ws[(static_cast<long long>(blockIdx.z) * g.m + row) * g.n + col] = v;
+                    } else {
+                        float* p = C + static_cast<long long>(row) * g.ldc + col;
+                        *p = g.beta == 0.f ? v : v + g.beta * *p;
+                    }
                 }
             }
+}
+
+// C[b] = sum over splits (in order) of the partials + beta C[b].
+__global__ void k_splitk_reduce(const float* ws, int splits, int m, int n, long long total, float* c, int ldc,
+                                long long sc, float beta) {
+    const long long mn = static_cast<long long>(m) * n;
+    for (long long e = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; e < total;
+         e += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const long long b = e / mn, rc = e % mn;
+        const float* p = ws + b * splits * mn + rc;
+        float v = 0.f;
+        for (int s = 0; s < splits; ++s) v += p[s * mn];
+        float* q = c + b * sc + (rc / n) * ldc + rc % n;
+        *q = beta == 0.f ? v : v + beta * *q;
+    }
 }
 
 // GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).
@@ -141,20 +172,25 @@ __global__ void k_gelu_back(float* d, const float* pre, long long n) {
     if (i < n) d[i] *= gelu_grad(pre[i]);
 }
 
-// grad[c] = sum over rows of d[r][c] (cols <= 256).
+// grad[c] += sum over rows of d[r][c] (cols <= blockDim.x = 256).  A block sums
+// a contiguous row range; thread t takes column t % cols of every
+// (blockDim.x / cols)-th row, and the row groups meet in LDS.
 __global__ void k_colsum(const float* d, long long rows, int cols, float* grad) {
     __shared__ float part[256];
-    for (int c = threadIdx.x; c < cols; c += blockDim.x) part[c] = 0.f;
-    __syncthreads();
+    const int groups = static_cast<int>(blockDim.x) / cols;
+    const int c = static_cast<int>(threadIdx.x) % cols, grp = static_cast<int>(threadIdx.x) / cols;
     const long long per = (rows + gridDim.x - 1) / gridDim.x;
     const long long r0 = blockIdx.x * per, r1 = std::min(rows, r0 + per);
-    for (int c = threadIdx.x; c < cols; c += blockDim.x) {
-        float s = 0.f;
-        for (long long r = r0; r < r1; ++r) s += d[r * cols + c];
-        part[c] = s;
-    }
+    float s = 0.f;
+    if (grp < groups)
+        for (long long r = r0 + grp; r < r1; r += groups) s += d[r * cols + c];
+    part[threadIdx.x] = s;
     __syncthreads();
-    for (int c = threadIdx.x; c < cols; c += blockDim.x) atomicAdd(&grad[c], part[c]);
+    if (static_cast<int>(threadIdx.x) < cols) {
+        float t = 0.f;
+        for (int k = 0; k < groups; ++k) t += part[k * cols + threadIdx.x];
+        atomicAdd(&grad[threadIdx.x], t);
+    }
 }
 
 // dy = 2 (y - t) / n and the summed squared error.
@@ -220,6 +256,27 @@ struct pmc_nn {
     int* idx = nullptr;
     int ds_n = 0;
     int64_t adam_t = 0;
+    float* ws = nullptr;  // split-k partial products
+    size_t ws_cap = 0;
+    // GEMM accounting: algorithmic flops (2 m n k per product) of every launch
+    // and, with timing on, their HIP-event durations on the operator's stream.
+    double gemm_flops = 0.0, gemm_ms = 0.0;
+    int64_t gemm_launches = 0;
+    bool timing = false;
+    static constexpr int kTimed = 64;
+    hipEvent_t ev[2 * kTimed] = {};
+    int ev_used = 0;
+
+    // Adds the durations of the recorded launches (waits for the last one).
+    void harvest() {
+        if (ev_used == 0) return;
+        note(hipEventSynchronize(ev[2 * ev_used - 1]));
+        for (int i = 0; i < ev_used; ++i) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]) == hipSuccess) gemm_ms += ms;
+        }
+        ev_used = 0;
+    }
 
     void note(hipError_t e) {
         if (e != hipSuccess && err == hipSuccess) err = e;
@@ -241,9 +298,45 @@ namespace {
 void gemm(pmc_nn* nn, int m, int n, int k, int batch, float alpha, const float* a, int lda, long long sa, int ta,
           const float* b, int ldb, long long sb, int tb, float beta, float* c, int ldc, long long sc) {
     if (m <= 0 || n <= 0 || batch <= 0) return;
-    GemmArgs g{m, n, k, a, lda, sa, ta, b, ldb, sb, tb, c, ldc, sc, alpha, beta};
-    dim3 grid((n + kTN - 1) / kTN, (m + kTM - 1) / kTM, batch);
+    // Split k when the tile grid leaves most of the 256 CUs idle: every stage of
+    // a workgroup is one exposed global-memory round trip at these sizes, so aim
+    // at about 512 workgroups of at most two stages (128 k) each.
+    const long long tiles = static_cast<long long>((n + kTN - 1) / kTN) * ((m + kTM - 1) / kTM) * batch;
+    int splits = 1, kchunk = std::max(k, 1);
+    if (tiles < 256 && k > 2 * kTK) {
+        splits = static_cast<int>(std::min<long long>((k + 2 * kTK - 1) / (2 * kTK), (512 + tiles - 1) / tiles));
+        if (splits > 1) {
+            kchunk = ((k + splits - 1) / splits + kTK - 1) / kTK * kTK;
+            splits = (k + kchunk - 1) / kchunk;
+        }
+    }
+    float* ws = nullptr;
+    if (splits > 1) {
+        const size_t need = static_cast<size_t>(splits) * batch * m * n;
+        if (need > nn->ws_cap) {
+            nn->ws = nn->alloc<float>(need);
+            nn->ws_cap = nn->ws ? need : 0;
+        }
+        ws = nn->ws;
+        if (!ws) return;
+    }
+    GemmArgs g{m, n, k, a, lda, sa, ta, b, ldb, sb, tb, c, ldc, sc, alpha, beta, splits, kchunk, ws};
+    dim3 grid((n + kTN - 1) / kTN, (m + kTM - 1) / kTM, static_cast<unsigned>(batch) * splits);
+    nn->gemm_flops += 2.0 * m * n * static_cast<double>(k) * batch;
+    ++nn->gemm_launches;
+    const bool timed = nn->timing && nn->ev[0] != nullptr;
+    if (timed) {
+        if (nn->ev_used == pmc_nn::kTimed) nn->harvest();
+        nn->note(hipEventRecord(nn->ev[2 * nn->ev_used], nn->stream));
+    }
     hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, nn->stream, g);
+    if (splits > 1) {
+        const long long total = static_cast<long long>(batch) * m * n;
+        const unsigned nb = static_cast<unsigned>(std::min<long long>((total + 255) / 256, 2048));
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(nb), dim3(256), 0, nn->stream, ws, splits, m, n, total, c, ldc, sc,
+                           beta);
+    }
+    if (timed) nn->note(hipEventRecord(nn->ev[2 * nn->ev_used++ + 1], nn->stream));
 }
 
 unsigned blocks(long long n, int t = 256) { return static_cast<unsigned>((n + t - 1) / t); }
@@ -319,6 +412,7 @@ void backward(pmc_nn* nn, int b) {
 }
 
 int finish(pmc_nn* nn, const char* /*what*/) {
+    nn->harvest();
     nn->note(hipStreamSynchronize(nn->stream));
     nn->note(hipGetLastError());
     return nn->err == hipSuccess ? PMC_OK : PMC_ERR_DEVICE;
@@ -538,9 +632,36 @@ int pmc_nn_train(pmc_nn* nn, int32_t n, const float* in, const float* target, in
     return finish(nn, "train");
 }
 
+int pmc_nn_set_timing(pmc_nn* nn, int32_t on) {
+    if (!nn) return PMC_ERR_ARGUMENT;
+    if (on && nn->ev[0] == nullptr)
+        for (auto& e : nn->ev)
+            if (hipEventCreate(&e) != hipSuccess) return PMC_ERR_DEVICE;
+    nn->harvest();
+    nn->timing = on != 0;
+    return PMC_OK;
+}
+
+int pmc_nn_gemm_stats(pmc_nn* nn, double* flops, int64_t* launches, double* ms, int32_t reset) {
+    if (!nn) return PMC_ERR_ARGUMENT;
+    nn->harvest();
+    nn->note(hipStreamSynchronize(nn->stream));
+    if (flops) *flops = nn->gemm_flops;
+    if (launches) *launches = nn->gemm_launches;
+    if (ms) *ms = nn->gemm_ms;
+    if (reset) {
+        nn->gemm_flops = 0.0;
+        nn->gemm_ms = 0.0;
+        nn->gemm_launches = 0;
+    }
+    return nn->err == hipSuccess ? PMC_OK : PMC_ERR_DEVICE;
+}
+
 void pmc_nn_destroy(pmc_nn* nn) {
     if (!nn) return;
     if (nn->stream) (void)hipStreamSynchronize(nn->stream);
+    for (auto& e : nn->ev)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : nn->allocs) (void)hipFree(p);
     if (nn->stream) (void)hipStreamDestroy(nn->stream);
     delete nn;

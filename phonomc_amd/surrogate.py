@@ -37,6 +37,9 @@ def _lib():
         L.pmc_nn_loss_grad.argtypes = [P, C.c_int32, f, f, f, f]
         L.pmc_nn_train.argtypes = [P, C.c_int32, f, f, C.c_int32, C.c_int32, C.c_float, C.c_uint64, f]
         L.pmc_nn_destroy.argtypes = [P]
+        L.pmc_nn_set_timing.argtypes = [P, C.c_int32]
+        L.pmc_nn_gemm_stats.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                        C.c_int32]
         L.pmc_nn_gemm.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, f, C.c_int32, C.c_int64,
                                   C.c_int32, f, C.c_int32, C.c_int64, C.c_int32, C.c_float, f, C.c_int32, C.c_int64]
         _bound = True
@@ -122,6 +125,17 @@ class Operator:
         if rc:
             raise RuntimeError(f"pmc_nn_train failed: {rc}")
         return losses
+
+    def set_timing(self, on=True):
+        if _lib().pmc_nn_set_timing(self.h, 1 if on else 0):
+            raise RuntimeError("pmc_nn_set_timing failed")
+
+    def gemm_stats(self, reset=False):
+        """(flops, launches, ms) of the operator's GEMM launches since the last reset."""
+        f, n, ms = C.c_double(), C.c_int64(), C.c_double()
+        if _lib().pmc_nn_gemm_stats(self.h, C.byref(f), C.byref(n), C.byref(ms), 1 if reset else 0):
+            raise RuntimeError("pmc_nn_gemm_stats failed")
+        return f.value, n.value, ms.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -1,6 +1,6 @@
 """Device kernels of the spectral neural operator: the BF16 matrix-core GEMM
 against numpy on BF16-rounded operands (every transpose combination, ragged
-sizes, batches), the loss and full gradient against the FP64 oracle
+sizes, batches, split k), the loss and full gradient against the FP64 oracle
 (oracle/surrogate_ref.py), and a short training run."""
 import numpy as np
 import pytest
@@ -18,7 +18,8 @@ def bf16(a):
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("m,n,k,batch", [(1, 1, 1, 1), (17, 33, 65, 1), (64, 64, 16, 3), (130, 70, 231, 2)])
+@pytest.mark.parametrize("m,n,k,batch", [(1, 1, 1, 1), (17, 33, 65, 1), (64, 64, 16, 3), (130, 70, 231, 2),
+                                         (32, 48, 4003, 2), (1, 64, 1000, 1)])  # the last two split k
 def test_gemm_matches_numpy(m, n, k, batch, ta, tb):
     rng = np.random.default_rng(m * 1000 + n * 10 + k)
     a = rng.normal(size=(batch, k, m) if ta else (batch, m, k)).astype(np.float32)
