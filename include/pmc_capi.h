@@ -132,6 +132,30 @@ typedef struct pmc_reservoir {
     const double* triangles;       /* [n_faces*9] */
 } pmc_reservoir;
 
+/* Linearized Callaway scattering about the fixed reference T_bg; replaces
+ * CarrierCollisionOperator::apply (src/solver/CarrierCollisionOperator.cpp:38-142)
+ * with CallawayOperator (src/material/CallawayOperator.cpp:62-201) on the
+ * CallawayData of PhononMaterial::callaway_data
+ * (src/material/PhononMaterialCollisionData.cpp:32-165).  Uploading it switches
+ * the context to the reference's linearized, conservative transport
+ * (include/SimulationConfig.h:148-156): linearized initial occupations
+ * n_eq(T_bg) + C(T_bg)(T - T_bg)/hw (src/material/Thermodynamics.cpp:27-37),
+ * cell temperature T_bg + (e - E_bg)/C_bg (src/solver/GridStatistics.cpp:193-225),
+ * discrete-mode reflection with signed occupations on every reflecting facet
+ * (src/solver/BoundaryInteraction.cpp:90-113).  The caller supplies linearized
+ * reservoir occupations in pmc_reservoir.occupation, strict-elastic rough tables
+ * and pmc_setup.refill_policy = 1 (fixed incoming flux).  All arrays by active
+ * index, evaluated at T_bg.  Call after pmc_upload_material and before the
+ * population is created. */
+typedef struct pmc_callaway_table {
+    const double* capacity;        /* [n_active] mode heat capacity, eV/K */
+    const double* wavevector;      /* [n_active*3] Cartesian q, 1/Angstrom */
+    const double* normal_rate;     /* [n_active] 4 pi gamma_N, 1/ps */
+    const double* resistive_rate;  /* [n_active] 4 pi (gamma_U + isotope/impurity/defect), 1/ps */
+    double heat_capacity_density;  /* C_bg, eV/(K Angstrom^3) (BackgroundCache.h:24-27) */
+    double background_energy_density; /* E_bg: exact crystal energy density at T_bg */
+} pmc_callaway_table;
+
 /* Per-step and cumulative ledgers; replaces BoundaryLedger/EnergyLedger
  * (src/solver/BoundaryLedger.cpp:30-141, src/solver/EnergyLedger.cpp:58-90). */
 typedef struct pmc_ledger {
@@ -194,6 +218,7 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* geometry);
 int pmc_upload_boundary_tables(pmc_ctx* ctx, int32_t n_rough, const pmc_rough_table* rough,
                                int32_t n_reservoirs, const pmc_reservoir* reservoirs);
 /* Enables the heat source (after material and geometry). */
+int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* table);
 int pmc_upload_heat_source(pmc_ctx* ctx, const pmc_heat_source* source);
 /* Samples the initial population on the device (Initialization.cpp:48-113) and
  * performs the initial temperature refresh (Observables.cpp:12-21). */
@@ -245,7 +270,8 @@ enum pmc_kernel_class {
     PMC_KERNEL_SOURCE = 9,     /* heat-source half-step deposits */
     PMC_KERNEL_OUTPUT = 10,    /* observables outside the step (heat flux) */
     PMC_KERNEL_STEP = 11,      /* whole steps */
-    PMC_KERNEL_CLASSES = 12
+    PMC_KERNEL_CALLAWAY = 12,  /* per-cell linearized Callaway collisions */
+    PMC_KERNEL_CLASSES = 13
 };
 /* Accumulated kernel time (ms) and launch counts per class since the last reset,
  * measured with HIP events on the context's stream while timing is enabled. */

@@ -54,6 +54,9 @@ int pmcs_problem_views(const pmcs_problem* problem, pmc_setup* setup, pmc_mode_t
 int pmcs_problem_material(const pmcs_problem* problem, pmcs_material_view* out);
 /* Heat-source view (valid while the problem lives): 1 if the input enables one, 0 if not. */
 int pmcs_problem_heat_source(const pmcs_problem* problem, pmc_heat_source* out);
+/* Linearized Callaway view (valid while the problem lives): 1 if the input selects
+ * scattering.model = 'callaway' (linearized), 0 if not. */
+int pmcs_problem_callaway(const pmcs_problem* problem, pmc_callaway_table* out);
 int pmcs_problem_scalars(const pmcs_problem* problem, double* out);
 /* Derived material queries, for checks against the reference's PhononMaterial. */
 double pmcs_problem_bose(const pmcs_problem* problem, double temperature, int32_t flat_mode);

@@ -178,6 +178,11 @@ int orc_grid_index(const pmc_geometry* g, const double p[3]) {
 
 int orc_rough_sample(const pmc_mode_table* m, const pmc_rough_table* r, int in_ai, double in_occ, const double* u,
                      int* out_ai, double* out_occ, int* kind) {
+    return orc_rough_sample_signed(m, r, in_ai, in_occ, u, out_ai, out_occ, kind, 0);
+}
+
+int orc_rough_sample_signed(const pmc_mode_table* m, const pmc_rough_table* r, int in_ai, double in_occ,
+                            const double* u, int* out_ai, double* out_occ, int* kind, int signed_weights) {
     int used = 0;
     *out_ai = in_ai;
     *out_occ = in_occ;
@@ -187,7 +192,7 @@ int orc_rough_sample(const pmc_mode_table* m, const pmc_rough_table* r, int in_a
     const double w_in = m->omega[in_ai], neq_in = m->neq_background[in_ai];
     auto mapped_to = [&](int o) {
         const double mapped = m->neq_background[o] + w_in * (in_occ - neq_in) / std::max(1e-30, m->omega[o]);
-        if (std::isfinite(mapped) && mapped >= 0.0) {
+        if (std::isfinite(mapped) && (signed_weights || mapped >= 0.0)) {
             *out_ai = o;
             *out_occ = mapped;
         }
@@ -438,6 +443,187 @@ int orc_source_deposit(const pmc_mode_table* m, int ncells, double* pending, con
 
 } // extern "C"
 
+// ---------------------------------------------------------------- linearized Callaway
+// CallawayOperator (src/material/CallawayOperator.cpp:9-201) restated on
+// explicit vectors in long double, as the reference computes it.
+namespace {
+using LVec = std::vector<long double>;
+
+long double ldot(const LVec& a, const LVec& b) {
+    long double s = 0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+long double lnorm(const LVec& a) { return std::sqrt(ldot(a, a)); }
+
+// gains (:28-64): sqrt(rate)-scaled basis, normalized, two-pass modified
+// Gram-Schmidt, skipped below 1e-12, scaled by sqrt(rate) again.
+std::vector<LVec> cw_gains(const std::vector<double>& rates, std::vector<LVec> basis) {
+    std::vector<LVec> orth;
+    for (auto v : basis) {
+        for (size_t i = 0; i < v.size(); ++i) v[i] *= std::sqrt(static_cast<long double>(rates[i]));
+        const long double original = lnorm(v);
+        if (original == 0) continue;
+        for (auto& x : v) x /= original;
+        for (int pass = 0; pass < 2; ++pass)
+            for (const auto& q : orth) {
+                const long double proj = ldot(q, v);
+                for (size_t i = 0; i < v.size(); ++i) v[i] -= proj * q[i];
+            }
+        const long double length = lnorm(v);
+        if (length < 1e-12L) continue;
+        for (auto& x : v) x /= length;
+        orth.push_back(std::move(v));
+    }
+    for (auto& q : orth)
+        for (size_t i = 0; i < q.size(); ++i) q[i] *= std::sqrt(static_cast<long double>(rates[i]));
+    return orth;
+}
+
+struct CwOp {
+    LVec scale, rate;
+    std::vector<LVec> ngain, rgain;
+    long double bound = 0;
+};
+
+// Constructor (:66-95).
+bool cw_build(CwOp& op, const std::vector<double>& omega, const std::vector<double>& cap,
+              const std::vector<V3>& q, const std::vector<double>& gn, const std::vector<double>& gr) {
+    const size_t n = omega.size();
+    op.scale.resize(n);
+    op.rate.resize(n);
+    std::vector<LVec> basis(4, LVec(n));
+    for (size_t i = 0; i < n; ++i) {
+        if (!std::isfinite(omega[i]) || omega[i] <= 0 || !std::isfinite(cap[i]) || cap[i] <= 0 ||
+            !std::isfinite(gn[i]) || gn[i] < 0 || !std::isfinite(gr[i]) || gr[i] < 0)
+            return false;
+        op.scale[i] = std::sqrt(static_cast<long double>(cap[i]));
+        basis[0][i] = op.scale[i];
+        for (int a = 0; a < 3; ++a) basis[a + 1][i] = op.scale[i] * q[i][a] / omega[i];
+        op.rate[i] = static_cast<long double>(gn[i]) + gr[i];
+        op.bound = std::max(op.bound, op.rate[i]);
+    }
+    op.ngain = cw_gains(gn, basis);
+    op.rgain = cw_gains(gr, {basis[0]});
+    return true;
+}
+
+// action (:97-111).
+LVec cw_action(const CwOp& op, const LVec& x) {
+    LVec r(x.size());
+    for (size_t i = 0; i < x.size(); ++i) r[i] = -op.rate[i] * x[i];
+    for (const auto* gain : {&op.ngain, &op.rgain})
+        for (const auto& u : *gain) {
+            const long double c = ldot(u, x);
+            for (size_t i = 0; i < x.size(); ++i) r[i] += u[i] * c;
+        }
+    return r;
+}
+
+// evolve (:135-201): 0, or -9 (work limit) / -10 (no convergence).
+int cw_evolve(const CwOp& op, std::vector<double>& energy, double dt) {
+    LVec x(energy.size());
+    long double magnitude = 0;
+    for (size_t i = 0; i < x.size(); ++i) {
+        x[i] = energy[i] / op.scale[i];
+        magnitude = std::max(magnitude, std::abs(x[i]));
+    }
+    if (magnitude == 0 || dt == 0 || op.bound == 0) return 0;
+    const long double steps_ld = std::max(1.0L, std::ceil(dt * op.bound / 4.0L));
+    if (!std::isfinite(steps_ld) || steps_ld > 100000 || steps_ld * x.size() > 200000000) return -9;
+    const size_t steps = static_cast<size_t>(steps_ld);
+    const long double h = static_cast<long double>(dt) / steps;
+    const long double scaled = h * op.bound;
+    for (auto& v : x) v /= magnitude;
+    const long double tol = 2e-14L / steps;
+    for (size_t st = 0; st < steps; ++st) {
+        LVec sum = x, term = x;
+        const long double reference = lnorm(x);
+        bool converged = false;
+        for (int k = 1; k <= 40; ++k) {
+            term = cw_action(op, term);
+            for (size_t i = 0; i < x.size(); ++i) {
+                term[i] *= h / k;
+                sum[i] += term[i];
+            }
+            if (scaled < k + 2) {
+                const long double tail = lnorm(term) * scaled / (k + 1) / (1.0L - scaled / (k + 2));
+                if (tail <= tol * reference) {
+                    converged = true;
+                    break;
+                }
+            }
+        }
+        if (!converged) return -10;
+        x = std::move(sum);
+    }
+    for (size_t i = 0; i < x.size(); ++i) energy[i] = static_cast<double>(x[i] * magnitude * op.scale[i]);
+    return 0;
+}
+} // namespace
+
+extern "C" {
+
+double orc_callaway_apply(const pmc_mode_table* m, const pmc_callaway_table* t, int ncells, int64_t n,
+                          const int32_t* active, const int32_t* cell, double* occ, double dt, double energy_weight,
+                          int* status) {
+    // CarrierCollisionOperator::apply (src/solver/CarrierCollisionOperator.cpp:38-142):
+    // each cell's carriers in index order are the quadrature; the new
+    // occupations are staged and committed together.
+    std::vector<std::vector<int64_t>> members(ncells);
+    for (int64_t i = 0; i < n; ++i) members[cell[i]].push_back(i);
+    std::vector<double> next(occ, occ + n);
+    std::vector<long double> residual(ncells, 0);
+    std::vector<int> err(ncells, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < ncells; ++c) {
+        const auto& idx = members[c];
+        if (idx.empty()) continue;
+        std::vector<double> omega, cap, gn, gr, energy;
+        std::vector<V3> q;
+        for (int64_t i : idx) {
+            const int a = active[i];
+            omega.push_back(m->omega[a]);
+            cap.push_back(t->capacity[a]);
+            q.push_back({t->wavevector[3 * a], t->wavevector[3 * a + 1], t->wavevector[3 * a + 2]});
+            gn.push_back(t->normal_rate[a]);
+            gr.push_back(t->resistive_rate[a]);
+            energy.push_back(6.582119569e-4 * m->omega[a] * (occ[i] - m->neq_background[a]));
+        }
+        if (dt == 0) continue;
+        CwOp op;
+        if (!cw_build(op, omega, cap, q, gn, gr)) {
+            err[c] = -11;
+            continue;
+        }
+        const int rc = cw_evolve(op, energy, dt);
+        if (rc) {
+            err[c] = rc;
+            continue;
+        }
+        for (size_t j = 0; j < idx.size(); ++j) {
+            const int64_t i = idx[j];
+            const int a = active[i];
+            const double hw = 6.582119569e-4 * m->omega[a];
+            next[i] = m->neq_background[a] + energy[j] / hw;
+            residual[c] += hw * (static_cast<long double>(next[i]) - occ[i]);
+        }
+        residual[c] *= energy_weight;
+    }
+    *status = 0;
+    for (int e : err)
+        if (e) {
+            *status = e;
+            return 0.0;
+        }
+    long double sum = 0;
+    for (auto r : residual) sum += r;
+    std::copy(next.begin(), next.end(), occ);
+    return static_cast<double>(sum);
+}
+
+} // extern "C"
+
 // ---------------------------------------------------------------- whole-step simulator
 struct orc_sim {
     pmc_setup setup;
@@ -467,6 +653,10 @@ struct orc_sim {
     std::vector<double> hs_rate, hs_pending;
     std::vector<int> hs_elig;
     double elapsed = 0.0;
+    // linearized Callaway transport
+    bool callaway = false;
+    pmc_callaway_table cw{};
+    std::vector<double> cw_cap, cw_q, cw_gn, cw_gr;
 };
 
 namespace {
@@ -529,6 +719,9 @@ bool update_cache(orc_sim& s, int64_t i) {
     return true;
 }
 
+// advance() result of a failed discrete-mode reflection (conservative transport).
+constexpr int kOrcModeMapping = 64;
+
 struct StepTally {
     std::vector<int> absorbed_by_res;
     double absorbed_e = 0;
@@ -570,22 +763,32 @@ int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
         }
         if (F.condition == 'P' && F.periodic_pair >= 0) {
             s.pos[i] = add(s.pos[i], V3{F.shift[0], F.shift[1], F.shift[2]});
-        } else if (F.condition == 'R' && F.rough_table >= 0) {
+        } else if (F.rough_table >= 0 && (F.condition == 'R' || s.callaway)) {
+            // Conservative transport reflects every reflecting facet through its
+            // table with signed occupations and needs an inward outgoing mode
+            // (src/solver/BoundaryInteraction.cpp:90-113).
             U.ev = static_cast<uint32_t>(ev);
             U.used = 0;
             double u[4];
             for (int k = 0; k < 4; ++k) u[k] = U.next();
             int out, kind;
             double o;
-            orc_rough_sample(&s.m, &s.rough[F.rough_table], a, s.occ[i], u, &out, &o, &kind);
+            orc_rough_sample_signed(&s.m, &s.rough[F.rough_table], a, s.occ[i], u, &out, &o, &kind,
+                                    s.callaway ? 1 : 0);
             tally.rough_events++;
             if (kind == 1) tally.rough_spec++;
             if (kind == 2) tally.rough_diff++;
             V3 v = table_velocity(s, out);
-            if (dot(v, n) > 0.0) v = sub(v, mul(n, 2.0 * dot(v, n)));
+            if (s.callaway) {
+                if (!std::isfinite(o) || !(s.m.omega[out] > 0.0) || !(dot(v, n) < 0.0)) return -kOrcModeMapping;
+            } else if (dot(v, n) > 0.0) {
+                v = sub(v, mul(n, 2.0 * dot(v, n)));
+            }
             s.mode[i] = out;
             s.vel[i] = v;
             s.occ[i] = o;
+        } else if (s.callaway) {
+            return -kOrcModeMapping;
         } else if (F.condition == 'R') {
             return -PMC_ERR_UNSUPPORTED;
         } else {
@@ -630,6 +833,10 @@ void refresh(orc_sim& s) {
     std::vector<int32_t> act(s.mode.begin(), s.mode.end());
     orc_refresh(&s.m, nc, s.g.cell_volume, static_cast<int64_t>(s.mode.size()), act.data(), s.cell.data(),
                 s.occ.data(), s.cell_e.data(), s.cell_T.data());
+    if (s.callaway) // T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225)
+        for (int c = 0; c < nc; ++c)
+            s.cell_T[c] = s.setup.background_temperature +
+                          (s.cell_e[c] - s.cw.background_energy_density) / s.cw.heat_capacity_density;
 }
 
 double thermal_energy(const orc_sim& s) {
@@ -767,7 +974,14 @@ int orc_initialize(orc_sim* s, const pmc_init* init) {
         if (init->temperature_mode == 2) T = init->cell_temperature[orc_grid_index(&s->g, pf.data())];
         s->pos[i] = pf;
         s->mode[i] = m;
-        s->occ[i] = bose(T, s->m.omega[m]);
+        if (s->callaway) { // PhononMaterial::linearized_bose_occupation (Thermodynamics.cpp:27-37)
+            const double hw = kHbar * s->m.omega[m];
+            s->occ[i] = hw > 0.0 ? s->m.neq_background[m] +
+                                       s->cw_cap[m] * (T - s->setup.background_temperature) / hw
+                                 : 0.0;
+        } else {
+            s->occ[i] = bose(T, s->m.omega[m]);
+        }
         s->uid[i] = static_cast<uint32_t>(i);
     }
     return finish_population(s);
@@ -783,6 +997,22 @@ int orc_seed(orc_sim* s, int64_t n, const float* position, const int32_t* mode, 
         s->uid[i] = static_cast<uint32_t>(i);
     }
     return finish_population(s);
+}
+
+int orc_set_callaway(orc_sim* s, const pmc_callaway_table* t) {
+    if (!s || !t || !t->capacity || !t->wavevector || !t->normal_rate || !t->resistive_rate) return PMC_ERR_ARGUMENT;
+    const size_t na = static_cast<size_t>(s->m.n_active);
+    s->cw_cap.assign(t->capacity, t->capacity + na);
+    s->cw_q.assign(t->wavevector, t->wavevector + 3 * na);
+    s->cw_gn.assign(t->normal_rate, t->normal_rate + na);
+    s->cw_gr.assign(t->resistive_rate, t->resistive_rate + na);
+    s->cw = *t;
+    s->cw.capacity = s->cw_cap.data();
+    s->cw.wavevector = s->cw_q.data();
+    s->cw.normal_rate = s->cw_gn.data();
+    s->cw.resistive_rate = s->cw_gr.data();
+    s->callaway = true;
+    return PMC_OK;
 }
 
 int orc_set_heat_source(orc_sim* s, const pmc_heat_source* h) {
@@ -845,12 +1075,27 @@ int orc_step(orc_sim* s, int32_t nsteps) {
             }
         }
         if (err) {
-            set_error(s, "transport failure");
-            return err;
+            set_error(s, err == kOrcModeMapping ? "Full-matrix boundary mode mapping failed while advancing particles."
+                                                : "transport failure");
+            return err == kOrcModeMapping ? PMC_ERR_BOUNDARY : err;
         }
         compact(*s);
-        // Injection (src/solver/ReservoirInjector.cpp:62-132), one-to-one refill.
+        // Injection (src/solver/ReservoirInjector.cpp:62-132): one-to-one refill, or
+        // floor(expected) + Bernoulli(frac) (ReservoirSampler.cpp:117-125) with the
+        // device's Philox stream 4, counter (reservoir, step).
         std::vector<int> counts(s->previous_absorbed);
+        if (s->setup.refill_policy == 1)
+            for (int r = 0; r < nres; ++r) {
+                const double expected = s->res[r].expected_count;
+                int count = static_cast<int>(std::floor(expected));
+                const double fraction = expected - count;
+                if (fraction > 0.0) {
+                    uint32_t w[4];
+                    orc_philox(static_cast<uint32_t>(r), static_cast<uint32_t>(s->step), 0, 4u, s->k0, s->k1, w);
+                    if (orc_u53(w[0], w[1]) < fraction) ++count;
+                }
+                counts[r] = count;
+            }
         int emissions = 0;
         for (int c : counts) emissions += c;
         const int64_t base = static_cast<int64_t>(s->mode.size());
@@ -907,7 +1152,7 @@ int orc_step(orc_sim* s, int32_t nsteps) {
                     const int rc = advance(*s, i, remain, local);
                     if (rc < 0) {
                         set_error(s, "transport failure");
-                        return -rc;
+                        return -rc == kOrcModeMapping ? PMC_ERR_BOUNDARY : -rc;
                     }
                 }
                 if (s->alive[i]) s->cell[i] = orc_grid_index(&s->g, s->pos[i].data());
@@ -924,9 +1169,20 @@ int orc_step(orc_sim* s, int32_t nsteps) {
         // Refresh, RTA, refresh (src/MonteCarloSolver.cpp:376-381).
         refresh(*s);
         std::vector<int32_t> act(s->mode.begin(), s->mode.end());
-        const double residual = orc_rta(&s->m, s->g.n_cells, s->cell_T.data(), static_cast<int64_t>(act.size()),
-                                        act.data(), s->cell.data(), s->occ.data(), dt, s->m.energy_weight,
-                                        s->setup.lifetime_local, s->setup.lifetime_temperature, nullptr);
+        double residual;
+        if (s->callaway) {
+            int status = 0;
+            residual = orc_callaway_apply(&s->m, &s->cw, s->g.n_cells, static_cast<int64_t>(act.size()), act.data(),
+                                          s->cell.data(), s->occ.data(), dt, s->m.energy_weight, &status);
+            if (status) {
+                set_error(s, "Callaway collision failure");
+                return PMC_ERR_INVALID_STATE;
+            }
+        } else {
+            residual = orc_rta(&s->m, s->g.n_cells, s->cell_T.data(), static_cast<int64_t>(act.size()), act.data(),
+                               s->cell.data(), s->occ.data(), dt, s->m.energy_weight, s->setup.lifetime_local,
+                               s->setup.lifetime_temperature, nullptr);
+        }
         if (s->hs_on) { // second half (src/MonteCarloSolver.cpp:376-378)
             double e = 0.0;
             int64_t n = 0;

@@ -32,6 +32,19 @@ int orc_grid_index(const pmc_geometry* g, const double pos[3]);
  * explicit uniforms u[0..2] in draw order; returns the number consumed. */
 int orc_rough_sample(const pmc_mode_table* m, const pmc_rough_table* r, int in_active, double in_occupation,
                      const double* u, int* out_active, double* out_occupation, int* kind);
+/* The same with signed_weights (RoughBoundarySampler::sample's last argument):
+ * mapped occupations may be negative under linearized transport. */
+int orc_rough_sample_signed(const pmc_mode_table* m, const pmc_rough_table* r, int in_active, double in_occupation,
+                            const double* u, int* out_active, double* out_occupation, int* kind, int signed_weights);
+
+/* Linearized Callaway collisions of a carrier set (CarrierCollisionOperator::apply,
+ * src/solver/CarrierCollisionOperator.cpp:38-142, with CallawayOperator,
+ * src/material/CallawayOperator.cpp:9-201, in long double): occupations updated in
+ * place, returns the lifetime residual (eV); status 0, or -9 work limit, -10 no
+ * convergence, -11 invalid table. */
+double orc_callaway_apply(const pmc_mode_table* m, const pmc_callaway_table* t, int ncells, int64_t n,
+                          const int32_t* active, const int32_t* cell, double* occupation, double dt,
+                          double energy_weight, int* status);
 
 /* Reservoir mode/time draw (src/solver/ReservoirSampler.cpp:127-146 for
  * sampling = 1; Walker alias for sampling = 0) and surface point
@@ -67,6 +80,11 @@ int orc_source_deposit(const pmc_mode_table* m, int ncells, double* pending, con
                        double* energy, int64_t* updates);
 /* Prescribed heat source, thermal spectrum (src/solver/PhononSourceOperator.cpp:120-341). */
 int orc_set_heat_source(orc_sim* sim, const pmc_heat_source* source);
+/* Linearized Callaway transport (before initialization): linearized initial
+ * occupations, linear cell temperature, conservative reflections and Callaway
+ * collisions; the reservoir occupations and refill policy come from the setup
+ * and reservoir tables. */
+int orc_set_callaway(orc_sim* sim, const pmc_callaway_table* table);
 /* Live carriers sorted by uid. */
 int64_t orc_carrier_count(orc_sim* sim);
 int orc_carriers(orc_sim* sim, double* position, int32_t* mode, double* occupation, int32_t* cell, uint32_t* uid);

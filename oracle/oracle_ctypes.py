@@ -55,6 +55,11 @@ def lib():
                                    C.POINTER(C.c_uint32)]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_heat_source.argtypes = [P, C.POINTER(rt.HeatSource)]
+        L.orc_set_callaway.argtypes = [P, C.POINTER(rt.CallawayTable)]
+        L.orc_callaway_apply.restype = d
+        L.orc_callaway_apply.argtypes = [C.POINTER(rt.ModeTable), C.POINTER(rt.CallawayTable), C.c_int, C.c_int64,
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(d), d, d,
+                                         C.POINTER(C.c_int)]
         L.orc_source_deposit.argtypes = [C.POINTER(rt.ModeTable), C.c_int, C.POINTER(d), C.POINTER(d), d,
                                          C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(d), C.POINTER(d), C.POINTER(C.c_int64)]
@@ -100,6 +105,9 @@ class Sim:
             raise RuntimeError(f"orc_create failed: {rc}")
         self.h = h
         self.n_cells = g.n_cells
+        cw = problem.callaway()
+        if cw is not None:
+            self._check(lib().orc_set_callaway(h, C.byref(cw)))
         hs = problem.heat_source()
         if hs is not None:
             self._check(lib().orc_set_heat_source(h, C.byref(hs)))
@@ -233,3 +241,19 @@ def surface_point(problem, reservoir_index, u_face, r1, r2):
     pt = np.zeros(3)
     lib().orc_surface_point(C.byref(res[reservoir_index]), u_face, r1, r2, _dp(pt))
     return pt
+
+
+def callaway_apply(problem, active, cell, occupation, dt):
+    """Linearized Callaway collisions (orc_callaway_apply) of a carrier set:
+    returns (new occupations, lifetime residual eV)."""
+    _, m, g, _, _, _ = problem.views()
+    cw = problem.callaway()
+    a = np.ascontiguousarray(active, dtype=np.int32)
+    c = np.ascontiguousarray(cell, dtype=np.int32)
+    occ = np.array(occupation, dtype=np.float64)
+    status = C.c_int()
+    res = lib().orc_callaway_apply(C.byref(m), C.byref(cw), g.n_cells, len(a), _ip(a), _ip(c), _dp(occ), dt,
+                                   m.energy_weight, C.byref(status))
+    if status.value:
+        raise RuntimeError(f"orc_callaway_apply failed: {status.value}")
+    return occ, res

@@ -61,8 +61,9 @@ def _ip(a):
 
 
 class RefMaterial:
-    def __init__(self, view, lookup_dt=0.1):
-        """view: a pmcs_material_view (phonomc_amd.runtime.MaterialView) or compatible struct."""
+    def __init__(self, view, lookup_dt=0.1, include_stationary=False):
+        """view: a pmcs_material_view (phonomc_amd.runtime.MaterialView) or compatible struct.
+        include_stationary: the constructor flag of conservative transport (Callaway)."""
         m = MaterialArrays()
         m.volume_a3 = view.volume_a3
         for k in range(9):
@@ -83,7 +84,11 @@ class RefMaterial:
         self.arrays = m
         err = C.create_string_buffer(512)
         h = C.c_void_p()
-        if lib().ref_material_create(C.byref(m), C.byref(h), err, 512) != 0:
+        if include_stationary:
+            rc = lib().ref_material_create_stationary(C.byref(m), 1, C.byref(h), err, 512)
+        else:
+            rc = lib().ref_material_create(C.byref(m), C.byref(h), err, 512)
+        if rc != 0:
             raise RuntimeError(err.value.decode())
         self.h = h
 
@@ -299,3 +304,31 @@ def run(toml_path, material_view, steps, threads=0, lookup_dt=0.1):
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return secs.value, ps.value
+
+
+def callaway_data(material, temperature):
+    """PhononMaterial::callaway_data(T) by active index."""
+    n = lib().ref_material_active_count(material.h)
+    out = {k: np.zeros(n) for k in ("omega", "capacity", "normal_rate", "resistive_rate")}
+    q = np.zeros(3 * n)
+    err = C.create_string_buffer(512)
+    rc = lib().ref_callaway_data(material.h, C.c_double(temperature), _dp(out["omega"]), _dp(out["capacity"]), _dp(q),
+                                 _dp(out["normal_rate"]), _dp(out["resistive_rate"]), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    out["wavevector"] = q.reshape(n, 3)
+    return out
+
+
+def callaway_apply(material, t_bg, ncells, active, cell, occupation, dt, particle_volume):
+    """CarrierCollisionOperator::configure(T_bg) + apply: (new occupations, residual eV)."""
+    active = np.ascontiguousarray(active, dtype=np.int32)
+    cell = np.ascontiguousarray(cell, dtype=np.int32)
+    occ = np.array(occupation, dtype=np.float64)
+    res = C.c_double()
+    err = C.create_string_buffer(512)
+    rc = lib().ref_callaway_apply(material.h, C.c_double(t_bg), int(ncells), len(active), _ip(active), _ip(cell),
+                                  _dp(occ), C.c_double(dt), C.c_double(particle_volume), C.byref(res), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return occ, res.value

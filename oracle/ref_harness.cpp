@@ -18,6 +18,7 @@
 #include "solver/ReservoirSampler.h"
 #include "solver/RoughBoundarySampler.h"
 #include "solver/RtaCollisionOperator.h"
+#include "solver/CarrierCollisionOperator.h"
 
 #include <chrono>
 #include <cstring>
@@ -110,6 +111,53 @@ extern "C" {
 int ref_material_create(const ref_material_arrays* arrays, void** out, char* err, int errlen) {
     return guarded(err, errlen, [&] {
         *out = new PhononMaterial(to_material_data(*arrays), arrays->temperature_lookup_dt);
+    });
+}
+
+int ref_material_create_stationary(const ref_material_arrays* arrays, int include_stationary, void** out, char* err,
+                                   int errlen) {
+    return guarded(err, errlen, [&] {
+        *out = new PhononMaterial(to_material_data(*arrays), arrays->temperature_lookup_dt, include_stationary != 0);
+    });
+}
+
+int ref_callaway_data(void* material, double temperature, double* omega, double* capacity, double* wavevector,
+                      double* normal_rate, double* resistive_rate, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto d = mat(material).callaway_data(temperature);
+        for (size_t a = 0; a < d.omega.size(); ++a) {
+            omega[a] = d.omega[a];
+            capacity[a] = d.capacity[a];
+            for (int k = 0; k < 3; ++k) wavevector[3 * a + k] = d.wavevector[a][k];
+            normal_rate[a] = d.normal_rate[a];
+            resistive_rate[a] = d.resistive_rate[a];
+        }
+    });
+}
+
+int ref_callaway_apply(void* material, double t_bg, int ncells, int n, const int* active, const int* cell,
+                       double* occupation, double dt, double particle_volume, double* residual_ev, char* err,
+                       int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& m = mat(material);
+        phonomc::ParticleStorage p;
+        p.reset(n);
+        for (int i = 0; i < n; ++i) {
+            p.modes[i] = m.active_mode_at(active[i]);
+            p.grid_ids[i] = cell[i];
+            p.occupation[i] = occupation[i];
+            p.temperatures[i] = t_bg;
+            p.material_ids[i] = 0;
+        }
+        p.material_spatial_volumes_a3 = {particle_volume};
+        phonomc::CellParticleIndex index;
+        index.ensure(p, ncells);
+        std::vector<int> materials(ncells, 0);
+        std::vector<double> temps(ncells, t_bg);
+        phonomc::CarrierCollisionOperator op;
+        op.configure({&m}, t_bg);
+        *residual_ev = op.apply(p, {&m}, {index.counts(), materials, temps, &index}, dt, particle_volume);
+        for (int i = 0; i < n; ++i) occupation[i] = p.occupation[i];
     });
 }
 
@@ -426,7 +474,10 @@ int ref_run(const char* toml_path, const ref_material_arrays* arrays, int steps,
 #endif
         SimulationConfig config = load_simulation_config(toml_path);
         SimulationDomain domain(config);
-        PhononMaterial material(to_material_data(*arrays), config.temperature_lookup_dt);
+        // Constructor flags of MaterialFactory (src/material/MaterialFactory.cpp:51-55).
+        PhononMaterial material(to_material_data(*arrays), config.temperature_lookup_dt,
+                                uses_conservative_transport(config) ||
+                                    config.heat_source_spectrum == HeatSourceSpectrum::Gaussian);
         std::vector<std::reference_wrapper<const PhononMaterial>> materials{std::cref(material)};
         MonteCarloSolver solver(config, domain, materials);
         long long carried = 0;

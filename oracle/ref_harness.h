@@ -35,6 +35,19 @@ typedef struct ref_material_arrays {
 } ref_material_arrays;
 
 int ref_material_create(const ref_material_arrays* arrays, void** out, char* err, int errlen);
+/* With the constructor flag MaterialFactory passes under conservative transport
+ * (src/material/MaterialFactory.cpp:51-55): stationary modes join the active bank. */
+int ref_material_create_stationary(const ref_material_arrays* arrays, int include_stationary, void** out, char* err,
+                                   int errlen);
+/* PhononMaterial::callaway_data(T) (src/material/PhononMaterialCollisionData.cpp:32-165)
+ * by active index: omega, capacity, wavevector (3 per mode), normal and resistive rates. */
+int ref_callaway_data(void* material, double temperature, double* omega, double* capacity, double* wavevector,
+                      double* normal_rate, double* resistive_rate, char* err, int errlen);
+/* CarrierCollisionOperator::configure(T_bg) + apply on a carrier set: occupations
+ * updated in place, *residual_ev = the returned lifetime residual. */
+int ref_callaway_apply(void* material, double t_bg, int ncells, int n, const int* active, const int* cell,
+                       double* occupation, double dt, double particle_volume, double* residual_ev, char* err,
+                       int errlen);
 void ref_material_destroy(void* material);
 int ref_material_active_count(void* material);
 /* Active-mode list in the reference's sampling order (src/PhononMaterial.cpp:136-156). */

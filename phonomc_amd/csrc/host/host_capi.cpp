@@ -100,6 +100,13 @@ int pmcs_problem_heat_source(const pmcs_problem* pr, pmc_heat_source* out) {
     return p.heat.enabled ? 1 : 0;
 }
 
+int pmcs_problem_callaway(const pmcs_problem* pr, pmc_callaway_table* out) {
+    if (!pr) return -1;
+    const auto& p = *(pr->operator->());
+    if (out) *out = p.callaway_view;
+    return p.linearized ? 1 : 0;
+}
+
 int pmcs_problem_material(const pmcs_problem* pr, pmcs_material_view* out) {
     if (!pr || !out) return PMC_ERR_ARGUMENT;
     fill_material((*pr)->raw, (*pr)->config.temperature_lookup_dt, out);

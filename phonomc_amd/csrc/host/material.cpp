@@ -7,6 +7,7 @@
 #include <filesystem>
 #include <fstream>
 #include <limits>
+#include <numeric>
 #include <stdexcept>
 
 namespace pmc {
@@ -228,11 +229,19 @@ double ModeTables::temperature_from_energy_density(double e) const {
     return interp_clamped(lut_e, lut_t, e);
 }
 
-ModeTables build_mode_tables(const RawMaterial& raw, double lookup_dt) {
+ModeTables build_mode_tables(const RawMaterial& raw, double lookup_dt, bool include_stationary) {
     if (!std::isfinite(lookup_dt) || lookup_dt <= 0.0)
         throw std::invalid_argument("Temperature lookup spacing must be finite and positive");
     validate_raw_material(raw);
+    // N/U data: the reference completes the equal-weight first-BZ quadrature at
+    // construction (src/PhononMaterial.cpp:38-40).  A refinement above one
+    // replicates zone-boundary q points, which this build does not do.
+    if (!raw.gamma_normal.empty() && callaway_bz_refinement(raw) != 1)
+        throw std::runtime_error("N/U material needs equal-weight Brillouin-zone completion (q points on the zone "
+                                 "boundary), which this build does not support; use a mesh without zone-boundary "
+                                 "points (an odd mesh).");
     ModeTables t;
+    t.include_stationary = include_stationary;
     t.qpoints = raw.qpoints;
     t.branches = raw.branches;
     t.n_flat = raw.qpoints * raw.branches;
@@ -244,10 +253,13 @@ ModeTables build_mode_tables(const RawMaterial& raw, double lookup_dt) {
     t.velocity_flat = raw.group_velocity;
     t.temps = raw.temperatures;
     t.gamma_total = raw.gamma;
+    t.gamma_table = raw.gamma;
+    t.gamma_extra.assign(raw.gamma.size(), 0.0);
     if (!raw.gamma_isotope.empty()) {
         for (size_t i = 0; i < t.gamma_total.size(); ++i) {
             const double extra = raw.gamma_isotope[raw.gamma_isotope.size() == static_cast<size_t>(nm) ? i % nm : i];
-            t.gamma_total[i] = raw.gamma[i] + (0.0 + extra);
+            t.gamma_extra[i] = 0.0 + extra;
+            t.gamma_total[i] = raw.gamma[i] + t.gamma_extra[i];
         }
     }
     t.gamma_normal = raw.gamma_normal;
@@ -270,13 +282,15 @@ ModeTables build_mode_tables(const RawMaterial& raw, double lookup_dt) {
                                           (v[2] * kg[2]) * (v[2] * kg[2]));
         }
     }
-    // Active modes: |v| > 1e-12, q-major then branch (src/PhononMaterial.cpp:136-156).
+    // Active modes: |v| > 1e-12 (or omega > 0 with stationary modes), q-major
+    // then branch (src/PhononMaterial.cpp:140-160).
     t.flat_to_active.assign(nm, -1);
     for (int q = 0; q < raw.qpoints; ++q)
         for (int b = 0; b < raw.branches; ++b) {
             const int fi = q * raw.branches + b;
             const auto& v = t.velocity_flat[fi];
-            if (std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) > 1e-12) {
+            if (include_stationary ? t.omega_flat[fi] > 0.0
+                                   : std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) > 1e-12) {
                 t.flat_to_active[fi] = static_cast<int>(t.active_flat.size());
                 t.active_flat.push_back(fi);
             }
@@ -307,6 +321,140 @@ ModeTables build_mode_tables(const RawMaterial& raw, double lookup_dt) {
         t.lut_e[i] = e;
     }
     return t;
+}
+
+namespace {
+double det3(const std::array<std::array<double, 3>, 3>& b) {
+    return b[0][0] * (b[1][1] * b[2][2] - b[1][2] * b[2][1]) - b[0][1] * (b[1][0] * b[2][2] - b[1][2] * b[2][0]) +
+           b[0][2] * (b[1][0] * b[2][1] - b[1][1] * b[2][0]);
+}
+
+std::array<std::array<double, 3>, 3> inverse3(const std::array<std::array<double, 3>, 3>& b, double det) {
+    std::array<std::array<double, 3>, 3> inv{};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            inv[i][j] = (b[(j + 1) % 3][(i + 1) % 3] * b[(j + 2) % 3][(i + 2) % 3] -
+                         b[(j + 1) % 3][(i + 2) % 3] * b[(j + 2) % 3][(i + 1) % 3]) /
+                        det;
+    return inv;
+}
+
+double length2(const std::array<std::array<double, 3>, 3>& b, const std::array<double, 3>& q) {
+    double r = 0;
+    for (int i = 0; i < 3; ++i) {
+        double x = 0;
+        for (int j = 0; j < 3; ++j) x += b[i][j] * q[j];
+        r += x * x;
+    }
+    return r;
+}
+} // namespace
+
+unsigned callaway_bz_refinement(const RawMaterial& d) {
+    // src/material/BzQuadrature.cpp:11-110: complete Gamma-centred mesh, no
+    // duplicates, first-BZ representatives; refinement = lcm of the numbers of
+    // equally short representatives.
+    const auto& b = d.reciprocal;
+    const double det = det3(b);
+    if (!std::isfinite(det) || det == 0) throw std::invalid_argument("Callaway reciprocal lattice is singular.");
+    const auto inv = inverse3(b, det);
+    std::vector<std::array<int, 3>> keys;
+    keys.reserve(d.qpoint_fractions.size());
+    unsigned refinement = 1;
+    for (const auto& q : d.qpoint_fractions) {
+        std::array<int, 3> key{};
+        for (int a = 0; a < 3; ++a) {
+            const double x = (q[a] - std::floor(q[a])) * d.mesh[a];
+            const long index = std::lround(x);
+            if (std::abs(x - index) > 2e-6 * d.mesh[a])
+                throw std::invalid_argument("Callaway requires a complete Gamma-centred full q mesh.");
+            key[a] = static_cast<int>(index % d.mesh[a]);
+        }
+        keys.push_back(key);
+        const double original = length2(b, q);
+        int lo[3], hi[3];
+        double count = 1;
+        for (int a = 0; a < 3; ++a) {
+            double nrm = 0;
+            for (double x : inv[a]) nrm += x * x;
+            const double extent = std::sqrt(original * nrm);
+            if (!std::isfinite(extent) || extent > 100 || std::abs(q[a]) > 100)
+                throw std::invalid_argument("Callaway first-BZ validation exceeds supported lattice conditioning.");
+            lo[a] = static_cast<int>(std::ceil(q[a] - extent - 1e-10));
+            hi[a] = static_cast<int>(std::floor(q[a] + extent + 1e-10));
+            count *= hi[a] - lo[a] + 1;
+        }
+        if (count > 100000) throw std::invalid_argument("Callaway first-BZ validation work limit exceeded.");
+        std::vector<double> lengths;
+        double shortest = original;
+        for (int i = lo[0]; i <= hi[0]; ++i)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+                for (int k = lo[2]; k <= hi[2]; ++k) {
+                    const double r = length2(b, {q[0] - i, q[1] - j, q[2] - k});
+                    shortest = std::min(shortest, r);
+                    lengths.push_back(r);
+                }
+        const double tolerance = 1e-10 * std::max(shortest, 1e-20);
+        if (original > shortest + tolerance)
+            throw std::invalid_argument("Callaway qpoint is outside the first Brillouin zone; supply first-BZ "
+                                        "representatives.");
+        unsigned m = 0;
+        for (double r : lengths)
+            if (std::abs(r - shortest) <= tolerance) ++m;
+        if (m == 0) throw std::runtime_error("Empty Callaway BZ representative set.");
+        unsigned g = refinement, h = m;
+        while (h) {
+            const unsigned t = g % h;
+            g = h;
+            h = t;
+        }
+        refinement = refinement / g * m;
+    }
+    std::sort(keys.begin(), keys.end());
+    if (std::adjacent_find(keys.begin(), keys.end()) != keys.end())
+        throw std::invalid_argument("Callaway input q mesh contains duplicate points modulo reciprocal translations.");
+    return refinement;
+}
+
+CallawayTable callaway_data(const ModeTables& t, const RawMaterial& raw, double temperature) {
+    // src/material/PhononMaterialCollisionData.cpp:32-165, with the mesh and
+    // first-BZ checks of callaway_bz_refinement.
+    if (t.gamma_normal.empty() || t.gamma_umklapp.empty())
+        throw std::invalid_argument("Callaway requires gamma_N and gamma_U in kappa-fbz.hdf5; regenerate with phono3py "
+                                    "--br --nu and expand consistently. No N/U split is inferred.");
+    if (!std::isfinite(temperature) || temperature <= 0 || temperature < t.temps.front() ||
+        temperature > t.temps.back())
+        throw std::invalid_argument("Callaway reference temperature is outside the N/U data range.");
+    for (size_t i = 0; i < t.gamma_table.size(); ++i) {
+        const double sum = t.gamma_normal[i] + t.gamma_umklapp[i];
+        if (!std::isfinite(sum) || std::abs(sum - t.gamma_table[i]) > 1e-7 * std::max(sum, t.gamma_table[i]) + 1e-14)
+            throw std::invalid_argument("Callaway data inconsistent: gamma_N + gamma_U != gamma. Supply unmodified, "
+                                        "same-run linewidths in the same units.");
+    }
+    if (callaway_bz_refinement(raw) != 1)
+        throw std::invalid_argument("Incomplete equal-weight Callaway BZ quadrature.");
+    size_t hi = static_cast<size_t>(std::lower_bound(t.temps.begin(), t.temps.end(), temperature) - t.temps.begin());
+    const size_t lo = hi > 0 ? hi - 1 : hi;
+    const double fraction = hi == lo ? 0 : (temperature - t.temps[lo]) / (t.temps[hi] - t.temps[lo]);
+    const size_t nm = static_cast<size_t>(t.n_flat);
+    size_t positive = 0;
+    for (double w : t.omega_flat)
+        if (w > 0) ++positive;
+    if (positive != t.active_flat.size())
+        throw std::invalid_argument("Callaway requires all positive-frequency modes, including stationary modes.");
+    CallawayTable c;
+    const auto rate = [&](const std::vector<double>& table, size_t flat) {
+        return 4 * std::acos(-1.0) * ((1 - fraction) * table[lo * nm + flat] + fraction * table[hi * nm + flat]);
+    };
+    for (int a = 0; a < t.n_active; ++a) {
+        const size_t flat = static_cast<size_t>(t.active_flat[a]);
+        c.omega.push_back(t.omega_flat[flat]);
+        c.capacity.push_back(t.heat_capacity(temperature, static_cast<int>(flat)));
+        c.wavevector.push_back(t.wavevector_flat[flat]);
+        c.normal_rate.push_back(rate(t.gamma_normal, flat));
+        c.resistive_rate.push_back(rate(t.gamma_umklapp, flat) + rate(t.gamma_extra, flat));
+    }
+    return c;
 }
 
 namespace {

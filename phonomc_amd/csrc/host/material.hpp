@@ -57,8 +57,11 @@ struct ModeTables {
     std::vector<double> knorm_flat, window_flat;
     std::vector<std::array<double, 3>> wavevector_flat;
     std::vector<double> temps;               // gamma temperature samples
-    std::vector<double> gamma_total;         // nT*n_flat
+    std::vector<double> gamma_total;         // nT*n_flat: gamma + resistive extras
+    std::vector<double> gamma_table;         // nT*n_flat: gamma as supplied
+    std::vector<double> gamma_extra;         // nT*n_flat: isotope (and other resistive) linewidths
     std::vector<double> gamma_normal, gamma_umklapp; // optional
+    bool include_stationary = false;         // active = omega > 0 instead of |v| > 1e-12
     std::vector<double> lut_t, lut_e;        // temperature lookup table
     double zero_point_density = 0;           // eV/A^3
     double energy_normalization = 1;         // max(1, Nq*Vuc)
@@ -73,7 +76,24 @@ struct ModeTables {
     double temperature_from_energy_density(double e) const;
 };
 
-ModeTables build_mode_tables(const RawMaterial& raw, double temperature_lookup_dt);
+// include_stationary: the active bank holds every positive-frequency mode, as the
+// reference's MaterialFactory requests for conservative transport
+// (src/material/MaterialFactory.cpp:51-55, src/PhononMaterial.cpp:140-160).
+ModeTables build_mode_tables(const RawMaterial& raw, double temperature_lookup_dt, bool include_stationary = false);
+
+// Callaway quadrature data by active index at a fixed reference temperature:
+// PhononMaterial::callaway_data (src/material/PhononMaterialCollisionData.cpp:32-165).
+struct CallawayTable {
+    std::vector<double> omega, capacity, normal_rate, resistive_rate; // rad/ps, eV/K, 1/ps, 1/ps
+    std::vector<std::array<double, 3>> wavevector;                    // 1/Angstrom
+};
+CallawayTable callaway_data(const ModeTables& t, const RawMaterial& raw, double temperature);
+
+// Mesh and first-Brillouin-zone checks of complete_callaway_bz
+// (src/material/BzQuadrature.cpp:11-110) for materials with N/U linewidths.
+// Returns the equal-weight refinement the reference would apply (1 when every
+// q point has a unique shortest representative).
+unsigned callaway_bz_refinement(const RawMaterial& raw);
 
 // Loads a material named by io.material_folder: "synthetic:si16[:callaway]" or a
 // directory holding material.pmcmat (the raw binary layout written by save_raw_material).

@@ -13,8 +13,15 @@
 namespace pmc {
 
 std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions& opt) {
-    if (c.collision != CollisionModel::Rta)
-        throw std::runtime_error("scattering.model = 'callaway' is not supported by this build (RTA only).");
+    if (c.collision == CollisionModel::Callaway && c.callaway_nonlinear)
+        throw std::runtime_error("scattering.callaway_formulation = 'nonlinear' is not supported by this build.");
+    // Linearized Callaway: fixed-reference, conservative transport
+    // (include/SimulationConfig.h:148-156).
+    const bool linearized = c.collision == CollisionModel::Callaway;
+    if (linearized && (!std::isfinite(c.background_temperature) || c.background_temperature <= 0))
+        throw std::runtime_error("Callaway requires a positive fixed background_temperature.");
+    if (linearized && c.heat_source_enabled)
+        throw std::runtime_error("heat_source with scattering.model = 'callaway' is not supported by this build.");
     if (c.heat_source_enabled && c.has_non_thermal_spectrum)
         throw std::runtime_error("heat_source.spectrum other than 'thermal' is not supported by this build.");
     if (c.has_gradient_drive) throw std::runtime_error("driving.temperature_gradient is not supported by this build.");
@@ -27,7 +34,10 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     auto p = std::make_unique<Problem>();
     p->config = c;
     p->raw = load_material(c.material_folder, c.input_directory);
-    p->modes = build_mode_tables(p->raw, c.temperature_lookup_dt);
+    // Stationary modes join the active bank under conservative transport
+    // (src/material/MaterialFactory.cpp:51-55).
+    p->modes = build_mode_tables(p->raw, c.temperature_lookup_dt, linearized);
+    p->linearized = linearized;
     p->domain = build_domain(c);
     const auto& m = p->modes;
     const auto& d = p->domain;
@@ -37,7 +47,30 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     p->particle_volume = d.volume / static_cast<double>(n);
     // include/solver/BackgroundCache.h:18-21.
     p->energy_weight = static_cast<double>(m.n_active) * p->particle_volume / m.energy_normalization;
-    p->background_energy_density = m.energy_density_from_temperature(c.background_temperature);
+    // BackgroundCache (include/solver/BackgroundCache.h:13-30): the exact crystal
+    // energy density and the heat-capacity density over all modes when linearized.
+    p->background_energy_density = linearized ? m.crystal_energy_density(c.background_temperature)
+                                              : m.energy_density_from_temperature(c.background_temperature);
+    if (linearized) {
+        double hc = 0.0;
+        for (int q = 0; q < m.qpoints; ++q)
+            for (int b = 0; b < m.branches; ++b)
+                hc += m.heat_capacity(c.background_temperature, q * m.branches + b) / m.energy_normalization;
+        if (!(hc > 0.0) || !std::isfinite(hc))
+            throw std::invalid_argument("Linearized temperature requires positive heat capacity.");
+        p->heat_capacity_density = hc;
+        p->callaway = callaway_data(m, p->raw, c.background_temperature);
+        p->callaway_wavevector.resize(3 * static_cast<size_t>(m.n_active));
+        for (int a = 0; a < m.n_active; ++a)
+            for (int k = 0; k < 3; ++k) p->callaway_wavevector[3 * a + k] = p->callaway.wavevector[a][k];
+        auto& cv = p->callaway_view;
+        cv.capacity = p->callaway.capacity.data();
+        cv.wavevector = p->callaway_wavevector.data();
+        cv.normal_rate = p->callaway.normal_rate.data();
+        cv.resistive_rate = p->callaway.resistive_rate.data();
+        cv.heat_capacity_density = p->heat_capacity_density;
+        cv.background_energy_density = p->background_energy_density;
+    }
     p->neq_background.resize(m.n_active);
     p->omega_active.resize(m.n_active);
     p->velocity_active.resize(3 * static_cast<size_t>(m.n_active));
@@ -47,11 +80,29 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         p->omega_active[a] = m.omega_flat[f];
         for (int k = 0; k < 3; ++k) p->velocity_active[3 * a + k] = m.velocity_flat[f][k];
     }
-    // Rough tables on every 'R' facet (src/solver/RoughBoundaryInitialization.cpp:9-25).
+    // Rough tables on every 'R' facet, or on every reflecting facet with strict
+    // elastic tables at T_bg under conservative transport
+    // (src/solver/RoughBoundaryInitialization.cpp:9-25).
     std::vector<bool> enabled(d.facets.size());
-    for (size_t f = 0; f < d.facets.size(); ++f) enabled[f] = d.facets[f].condition == 'R';
-    p->rough = build_rough_tables(d, m, enabled, 0.0, p->facet_to_rough);
+    for (size_t f = 0; f < d.facets.size(); ++f) {
+        const auto& fc = d.facets[f];
+        const bool reflecting = fc.condition != 'T' && !(fc.condition == 'P' && fc.periodic_pair >= 0);
+        enabled[f] = linearized ? reflecting : fc.condition == 'R';
+    }
+    p->rough = build_rough_tables(d, m, enabled, linearized ? c.background_temperature : 0.0, p->facet_to_rough);
     p->reservoirs = build_reservoir_tables(d, m, p->particle_density, c.time_step, p->facet_to_reservoir);
+    if (linearized) {
+        // Linearized reservoir occupations (src/solver/ReservoirInjector.cpp:104-109).
+        for (auto& r : p->reservoirs)
+            for (int a = 0; a < m.n_active; ++a) {
+                const int f = m.active_flat[a];
+                const double hw = kHbarEvPs * m.omega_flat[f];
+                r.occupation[a] = hw > 0.0 ? m.bose(c.background_temperature, f) +
+                                                 m.heat_capacity(c.background_temperature, f) *
+                                                     (r.temperature - c.background_temperature) / hw
+                                           : 0.0;
+            }
+    }
 
     // Prescribed heat source (src/solver/HeatSourceDriving.cpp:8-11).
     p->heat = build_heat_source(c, d, m);
@@ -155,7 +206,9 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     s.background_temperature = c.background_temperature;
     s.lifetime_local = c.lifetime_local ? 1 : 0;
     s.lifetime_temperature = c.lifetime_temperature;
-    s.refill_policy = 0;
+    // Fixed incoming flux under conservative transport, else one-to-one refill
+    // (src/solver/ReservoirInitialization.cpp:30-33).
+    s.refill_policy = linearized ? 1 : 0;
     s.mode_sampling = opt.mode_sampling;
     s.temperature_inversion = 0;
 
@@ -211,6 +264,7 @@ MonteCarloSolver::MonteCarloSolver(std::unique_ptr<Problem> problem, std::string
     throw_if_failed(pmc_upload_geometry(ctx_, &p.geometry_view));
     throw_if_failed(pmc_upload_boundary_tables(ctx_, static_cast<int>(p.rough_views.size()), p.rough_views.data(),
                                                static_cast<int>(p.reservoir_views.size()), p.reservoir_views.data()));
+    if (p.linearized) throw_if_failed(pmc_upload_callaway(ctx_, &p.callaway_view));
     if (p.heat.enabled) throw_if_failed(pmc_upload_heat_source(ctx_, &p.heat_view));
     throw_if_failed(pmc_initialize_carriers(ctx_, &p.init));
     if (!folder_.empty()) {

@@ -37,6 +37,12 @@ struct Problem {
     std::vector<double> omega_active, velocity_active; // per active mode (3 per mode for velocity)
     HeatSource heat;
     pmc_heat_source heat_view{};
+    // Linearized Callaway transport (scattering.model = 'callaway').
+    bool linearized = false;
+    CallawayTable callaway;
+    std::vector<double> callaway_wavevector; // 3 per active mode
+    double heat_capacity_density = 0;
+    pmc_callaway_table callaway_view{};
     // C-ABI views (pointers into the vectors above)
     std::vector<pmc_facet> facets;
     std::vector<pmc_rough_table> rough_views;

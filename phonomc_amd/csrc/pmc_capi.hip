@@ -153,6 +153,10 @@ int collect_error(pmc_ctx* ctx) {
             std::snprintf(msg, sizeof msg, "Could not bracket the occupation-source pseudo-temperature.");
         else if (st.error_facet == -6)
             std::snprintf(msg, sizeof msg, "Heat source would produce an invalid occupation; cell budget retained.");
+        else if (st.error_facet == -9)
+            std::snprintf(msg, sizeof msg, "Callaway exponential work limit exceeded; reduce time_step.");
+        else if (st.error_facet == -10)
+            std::snprintf(msg, sizeof msg, "Callaway exponential did not converge.");
         else
             std::snprintf(msg, sizeof msg, "Invalid particle in grid energy statistics.");
         break;
@@ -160,10 +164,24 @@ int collect_error(pmc_ctx* ctx) {
         std::snprintf(msg, sizeof msg, "Carrier slot capacity exhausted while injecting into slot %lld.", i);
         break;
     case PMC_ERR_UNSUPPORTED:
-        std::snprintf(msg, sizeof msg, "Rough facet without a reflection table reached by particle %lld.", i);
+        if (st.error_facet == -8)
+            std::snprintf(msg, sizeof msg,
+                          "Cell %lld holds more carriers than the Callaway kernel takes per cell (%d); use more "
+                          "cells or fewer carriers.",
+                          i, pmc_dev::kCwThreads * pmc_dev::kCwPer);
+        else
+            std::snprintf(msg, sizeof msg, "Rough facet without a reflection table reached by particle %lld.", i);
         break;
     default:
-        std::snprintf(msg, sizeof msg, "Boundary intersection failed for particle %lld at (%f, %f, %f).", i, x, y, z);
+        if (st.error_facet == -7)
+            std::snprintf(msg, sizeof msg,
+                          "Full-matrix boundary mode mapping failed while advancing particles for particle %lld. An "
+                          "inward discrete material mode and finite signed energy weight are required; no "
+                          "velocity-only reflection was applied.",
+                          i);
+        else
+            std::snprintf(msg, sizeof msg, "Boundary intersection failed for particle %lld at (%f, %f, %f).", i, x,
+                          y, z);
     }
     std::snprintf(ctx->error.message, sizeof(ctx->error.message), "%s", msg);
     return st.error_code;
@@ -223,6 +241,7 @@ int pmc_create(const pmc_setup* setup, pmc_ctx** out) {
     v.series_on = 1;
     v.series_radius = 0.01;
     v.t_bg = setup->background_temperature;
+    v.refill_fixed = setup->refill_policy == 1 ? 1 : 0;
     for (auto& b : ctx->buf) {
         const size_t n = static_cast<size_t>(setup->capacity);
         b.px = ctx->alloc<float>(n);
@@ -486,6 +505,55 @@ static int finish_population(pmc_ctx* ctx, int64_t count) {
     return collect_error(ctx);
 }
 
+int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* t) {
+    if (!ctx) return PMC_ERR_ARGUMENT;
+    if (!t || !t->capacity || !t->wavevector || !t->normal_rate || !t->resistive_rate)
+        return fail(ctx, PMC_ERR_ARGUMENT, "Callaway table arrays missing");
+    if (!ctx->has_material) return fail(ctx, PMC_ERR_ARGUMENT, "material must be uploaded before the Callaway table");
+    if (ctx->initialized)
+        return fail(ctx, PMC_ERR_ARGUMENT, "the Callaway table must be uploaded before the population is created");
+    if (!(t->heat_capacity_density > 0.0) || !std::isfinite(t->heat_capacity_density) ||
+        !std::isfinite(t->background_energy_density))
+        return fail(ctx, PMC_ERR_ARGUMENT, "Linearized temperature requires positive heat capacity.");
+    View& v = ctx->view;
+    const int na = v.na;
+    std::vector<double> omega(na);
+    ctx->note(hipMemcpy(omega.data(), v.omega, na * sizeof(double), hipMemcpyDeviceToHost));
+    int rc = check_hip(ctx, hipGetLastError(), "Callaway table");
+    if (rc) return rc;
+    // Packed per-mode constants, 64 B per mode: (g_N, g_R, B0, B1), (B2, B3, hw, 0)
+    // with B = sqrt(C) (1, q / w) (CallawayOperator::CallawayOperator,
+    // src/material/CallawayOperator.cpp:62-95, and its checks).
+    std::vector<double> pack(8 * static_cast<size_t>(na), 0.0);
+    for (int a = 0; a < na; ++a) {
+        const double w = omega[a], cap = t->capacity[a], gn = t->normal_rate[a], gr = t->resistive_rate[a];
+        if (!std::isfinite(w) || w <= 0 || !std::isfinite(cap) || cap <= 0 || !std::isfinite(gn) || gn < 0 ||
+            !std::isfinite(gr) || gr < 0)
+            return fail(ctx, PMC_ERR_ARGUMENT,
+                        "Callaway needs positive frequencies/capacities and finite nonnegative rates.");
+        const double b0 = std::sqrt(cap);
+        double* p = pack.data() + 8 * static_cast<size_t>(a);
+        p[0] = gn;
+        p[1] = gr;
+        p[2] = b0;
+        for (int k = 0; k < 3; ++k) {
+            const double q = t->wavevector[3 * a + k];
+            if (!std::isfinite(q)) return fail(ctx, PMC_ERR_ARGUMENT, "Nonfinite Callaway wavevector.");
+            p[3 + k] = b0 * q / w;
+        }
+        p[6] = 6.582119569e-4 * w;
+    }
+    v.cw_cap = ctx->upload(t->capacity, na);
+    v.cw_q = ctx->upload(t->wavevector, 3 * static_cast<size_t>(na));
+    v.cw_gn = ctx->upload(t->normal_rate, na);
+    v.cw_gr = ctx->upload(t->resistive_rate, na);
+    v.cw_pack = reinterpret_cast<const double4*>(ctx->upload(pack.data(), pack.size()));
+    v.cw_cbg = t->heat_capacity_density;
+    v.e_bg = t->background_energy_density;
+    v.callaway = 1;
+    return check_hip(ctx, hipGetLastError(), "Callaway table");
+}
+
 int pmc_initialize_carriers(pmc_ctx* ctx, const pmc_init* init) {
     if (!ctx || !init || !ctx->has_material || !ctx->has_geometry || !ctx->has_tables)
         return fail(ctx, PMC_ERR_ARGUMENT, "tables must be uploaded before initialization");
@@ -601,9 +669,13 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
         ctx->cur = 1 - ctx->cur;
         View& w = view(ctx);
         T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::cell_prep(w, st); });
-        T.run(PMC_KERNEL_MOMENTS, [&] { pmc_launch::moments(w, st); });
-        T.run(PMC_KERNEL_NEWTON, [&] { pmc_launch::newton(w, st); });
-        T.run(PMC_KERNEL_RELAX, [&] { pmc_launch::relax(w, st); });
+        if (w.callaway) {
+            T.run(PMC_KERNEL_CALLAWAY, [&] { pmc_launch::callaway(w, st); });
+        } else {
+            T.run(PMC_KERNEL_MOMENTS, [&] { pmc_launch::moments(w, st); });
+            T.run(PMC_KERNEL_NEWTON, [&] { pmc_launch::newton(w, st); });
+            T.run(PMC_KERNEL_RELAX, [&] { pmc_launch::relax(w, st); });
+        }
         if (w.hs_on) T.run(PMC_KERNEL_SOURCE, [&] { pmc_launch::source(w, st, source_time_factor(ctx->hs, mid, end), 1); });
         T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::refresh(w, st, 2); });
         int rc = check_hip(ctx, hipGetLastError(), "step launch");

@@ -243,6 +243,19 @@ struct View {
     int inversion;
     int series_on;        // evaluate pseudo-temperature sums from the per-cell series
     double series_radius; // largest |T0/T - 1| at which the series is tried
+    int refill_fixed;     // reservoirs emit floor(expected) + Bernoulli(frac) carriers per step
+    // linearized Callaway transport (fixed reference T_bg)
+    int callaway;
+    const double* cw_cap;   // mode heat capacity at T_bg, eV/K
+    const double* cw_q;     // Cartesian wavevector, 3 per mode, 1/A
+    const double* cw_gn;    // normal rate 1/ps
+    const double* cw_gr;    // resistive rate 1/ps
+    const double4* cw_pack; // 2 per mode: (g_N, g_R, B0, B1), (B2, B3, hw, 0); B = sqrt(C) (1, q / w)
+    double cw_cbg;          // heat-capacity density at T_bg, eV/(K A^3)
 };
+
+// Linearized Callaway kernel: one workgroup of kCwThreads per cell, up to
+// kCwPer carriers per thread held in registers.
+constexpr int kCwThreads = 512, kCwWaves = kCwThreads / 64, kCwPer = 16;
 
 } // namespace pmc_dev

@@ -172,6 +172,15 @@ __device__ void raise_error(const View& V, int code, long long slot, const float
     }
 }
 
+// advance() result for a discrete-mode reflection that found no inward mode
+// (conservative transport); reported as PMC_ERR_BOUNDARY with facet -7.
+constexpr int kErrModeMapping = 64;
+
+__device__ __forceinline__ void raise_advance_error(const View& V, int rc, long long slot, const float* p) {
+    if (rc == -kErrModeMapping) raise_error(V, PMC_ERR_BOUNDARY, slot, p, -7);
+    else raise_error(V, -rc, slot, p, -1);
+}
+
 // Logical block index grouping consecutive carrier chunks on one XCD (blocks are
 // dispatched round-robin over the 8 XCDs of an MI355X).
 __device__ __forceinline__ int xcd_block() {
@@ -187,8 +196,10 @@ __device__ __forceinline__ int carrier_slot() { return xcd_block() * blockDim.x 
 // with sample_diffuse (:373-456) and flux_weighted_window_index
 // (include/solver/BoundaryFluxSampling.h:64-88).  Returns the outgoing active mode
 // and occupation; kind = 1 specular, 2 diffuse, 0 unchanged.
+// signed_w: deviational occupations may go negative (the signed_weights
+// argument of RoughBoundarySampler::sample, linearized transport).
 __device__ void rough_sample(const View& V, const RoughDev& R, int in_ai, double in_occ, EventUniforms& U,
-                             int& out_ai, double& out_occ, int& kind) {
+                             int& out_ai, double& out_occ, int& kind, bool signed_w) {
     out_ai = in_ai;
     out_occ = in_occ;
     kind = 0;
@@ -201,7 +212,7 @@ __device__ void rough_sample(const View& V, const RoughDev& R, int in_ai, double
         if (o >= 0 && o < V.na) {
             const double w_out = V.omega[o];
             const double mapped = V.neq[o] + w_in * (in_occ - neq_in) / fmax(1e-30, w_out);
-            if (isfinite(mapped) && mapped >= 0.0) {
+            if (isfinite(mapped) && (signed_w || mapped >= 0.0)) {
                 out_ai = o;
                 out_occ = mapped;
             }
@@ -257,7 +268,7 @@ __device__ void rough_sample(const View& V, const RoughDev& R, int in_ai, double
     kind = 2;
     const double w_out = V.omega[o];
     const double mapped = V.neq[o] + w_in * (in_occ - neq_in) / fmax(1e-30, w_out);
-    if (isfinite(mapped) && mapped >= 0.0) {
+    if (isfinite(mapped) && (signed_w || mapped >= 0.0)) {
         out_ai = o;
         out_occ = mapped;
     }
@@ -335,7 +346,9 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             }
             continue;
         }
-        if (F.cond == 'R' && F.rough >= 0) {
+        // Conservative transport reflects through the table on every reflecting
+        // facet; otherwise only rough facets have one.
+        if (F.rough >= 0 && (F.cond == 'R' || V.callaway)) {
             if (ev == 0 || U.uid != uid) {
                 U.uid = uid;
                 U.step = step;
@@ -346,16 +359,22 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             U.used = 0;
             int out_ai, kind;
             double out_occ;
-            rough_sample(V, V.rough[F.rough], m, occ, U, out_ai, out_occ, kind);
+            rough_sample(V, V.rough[F.rough], m, occ, U, out_ai, out_occ, kind, V.callaway != 0);
             T.rough_events++;
             if (kind == 1) T.rough_spec++;
             else if (kind == 2) T.rough_diff++;
             // New velocity is the table velocity of the outgoing mode, mirrored if it
-            // points out of the facet.
+            // points out of the facet; under conservative transport it must point in
+            // (src/solver/BoundaryInteraction.cpp:90-113).
             int nw = out_ai;
             const double* vo = V.veld + 3 * static_cast<size_t>(out_ai);
             const double vn = vo[0] * F.n[0] + vo[1] * F.n[1] + vo[2] * F.n[2];
-            if (vn > 0.0) nw |= (1 << ha) << kFlipShift;
+            if (V.callaway) {
+                const double wo = V.omega[out_ai];
+                if (!isfinite(out_occ) || !(wo > 0.0) || !(vn < 0.0)) return -kErrModeMapping;
+            } else if (vn > 0.0) {
+                nw |= (1 << ha) << kFlipShift;
+            }
             if (nw != mw) mode_changed = true;
             if (out_occ != occ) occ_changed = true;
             mw = nw;
@@ -363,6 +382,7 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             continue;
         }
         if (F.cond == 'R') return -PMC_ERR_UNSUPPORTED; // table-less rough facet
+        if (V.callaway) return -kErrModeMapping;    // reflecting facet without a table
         // Specular mirror (periodic facet without a partner).
         mw ^= (1 << ha) << kFlipShift;
         mode_changed = true;
@@ -523,10 +543,31 @@ struct BlockHash {
     BlockHash H{H##_key, H##_cnt, H##_base};                                                                           \
     H.init()
 
+// Emission counts of the step: the previous step's absorptions per reservoir
+// (one-to-one refill), or floor(expected) plus one with probability
+// frac(expected) (ReservoirSampler::sample_flux_emission_count,
+// src/solver/ReservoirSampler.cpp:117-125; Philox stream 4, counter (reservoir, step)).
 __global__ void k_begin(View V) {
     DevState* st = V.st;
     const int t = threadIdx.x;
-    if (t < kMaxReservoirs) st->emit_count[t] = t < V.nres ? st->absorbed_now[t] : 0;
+    if (t < kMaxReservoirs) {
+        int count = 0;
+        if (t < V.nres) {
+            if (V.refill_fixed) {
+                const double expected = V.res[t].expected;
+                count = static_cast<int>(floor(expected));
+                const double fraction = expected - count;
+                if (fraction > 0.0) {
+                    const uint4 w = philox(static_cast<uint32_t>(t), static_cast<uint32_t>(st->step), 0, kStreamFlux,
+                                           V.key0, V.key1);
+                    if (u53(w.x, w.y) < fraction) ++count;
+                }
+            } else {
+                count = st->absorbed_now[t];
+            }
+        }
+        st->emit_count[t] = count;
+    }
     if (t == 0) {
         st->n_slots_start = st->n_slots;
         st->carrier_steps += st->n_live;
@@ -658,7 +699,7 @@ __global__ void __launch_bounds__(256) k_events(View V) {
             bool mc = false, oc = false;
             const int rc = advance(V, p, mw, occ, V.uid[i], V.dtf, T, mc, oc);
             if (rc < 0) {
-                raise_error(V, -rc, i, p, -1);
+                raise_advance_error(V, rc, i, p);
                 V.mode[i] = -1; // keeps the cell counts equal to the live slots
             } else if (rc == 1) {
                 V.mode[i] = -1;
@@ -761,7 +802,7 @@ __global__ void __launch_bounds__(256) k_inject(View V, int row0) {
             if (remain > 1e-14) rc = advance(V, p, mw, occ, uid, static_cast<float>(remain), T, mc, oc);
             V.uid[slot] = uid;
             if (rc < 0) {
-                raise_error(V, -rc, slot, p, -1);
+                raise_advance_error(V, rc, slot, p);
                 V.mode[slot] = -1;
             } else if (rc == 1) {
                 V.mode[slot] = -1;
@@ -799,6 +840,12 @@ __device__ double lut_inverse(const View& V, double e) {
     if (fabs(x1 - x0) <= 1e-18) return y0;
     const double t = (e - x0) / (x1 - x0);
     return y0 * (1.0 - t) + y1 * t;
+}
+
+// Cell temperature of an energy density: the lookup table, or under
+// linearized transport T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225).
+__device__ __forceinline__ double cell_temperature(const View& V, double e) {
+    return V.callaway ? V.t_bg + (e - V.e_bg) / V.cw_cbg : lut_inverse(V, e);
 }
 
 __device__ void gamma_row(const View& V, double temp, int& lo, int& hi, double& t) {
@@ -931,7 +978,7 @@ __global__ void __launch_bounds__(1024) k_refresh(View V, int phase) {
         }
         const double e = esum / V.cell_vol[c] + V.e_bg;
         V.cell_e[c] = e;
-        V.cell_T[c] = lut_inverse(V, e);
+        V.cell_T[c] = cell_temperature(V, e);
         local += (e - V.zp) * V.cell_vol[c];
     }
     const double energy = block_sum(local, red);
@@ -1131,7 +1178,7 @@ __device__ __forceinline__ double rta_accumulate(const View& V, const CellRta& c
 // and the rate-table row of the lifetime temperature.
 __device__ __forceinline__ CellRta cell_rta(const View& V, int c) {
     const double e = V.cell_tally[static_cast<size_t>(kTallyStride) * c + kTallyEnergy] / V.cell_vol[c] + V.e_bg;
-    const double T = lut_inverse(V, e);
+    const double T = cell_temperature(V, e);
     CellRta cr{};
     cr.t0 = T;
     cr.high = fmax(1000.0, 2.0 * fmax(1.0, T));
@@ -1358,6 +1405,299 @@ __global__ void __launch_bounds__(kChunkThreads) k_relax(View V) {
     if (threadIdx.x == 0) {
         V.chunk_out[2 * static_cast<size_t>(b)] = r[0];
         V.chunk_out[2 * static_cast<size_t>(b) + 1] = r[1];
+    }
+}
+
+// ---------------------------------------------------------------- Callaway
+// Linearized Callaway collisions of one cell: CarrierCollisionOperator::apply
+// (src/solver/CarrierCollisionOperator.cpp:38-142) with CallawayOperator
+// (src/material/CallawayOperator.cpp:62-201); the cell's carriers are the
+// quadrature.  In x_i = e_i / sqrt(C_i), e_i = hw_i (n_i - n_eq,i), the generator
+// is L x = -(g_N + g_R) x + U_N U_N^T x + U_R U_R^T x, where U_N spans
+// sqrt(g_N) B_k over k = 0..3 (energy and the three crystal-momentum components,
+// B_i = sqrt(C_i) (1, q_i / w_i)) and U_R spans sqrt(g_R) B_0.  Writing the
+// orthonormal vectors as combinations of the scaled basis, the gain terms become
+// g_N,i B_i . (P_N m_N) + g_R,i B0_i m_R / G_R with the moments
+// m_N = sum_i g_N,i B_i x_i, m_R = sum_i g_R,i B0_i x_i, the Gram matrix
+// G_N = sum_i g_N,i B_i B_i^T and P_N = sum_k c_k c_k^T from modified
+// Gram-Schmidt (two passes) of the normalized basis in G_N's inner product: the
+// reference's orthogonalization of the n-vectors carried out in the 4-dim
+// coefficient space.  A direction is skipped when its basis vector vanishes or
+// its squared length after projection is below 1e-14 (lengths from a Gram matrix
+// resolve to about 1e-8; the reference skips below 1e-12 on explicit vectors).
+// exp(dt L) is the reference's Taylor scheme: ceil(dt max rate / 4) substeps,
+// terms until the geometric tail bound is at most 2e-14/steps of |x|.  Every
+// Taylor term is one pass over the registers with one block reduction.
+// Block sums of K values for many waves: wave sums to LDS, K threads add the
+// W wave rows (in wave order), every thread reads the K totals.  lds holds
+// K * W + K doubles.
+template <int K, int W>
+__device__ __forceinline__ void block_reduce_wide(double (&v)[K], double* lds) {
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) lds[k * W + wave] = v[k];
+    }
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < K) {
+        double s = 0.0;
+        for (int w = 0; w < W; ++w) s += lds[threadIdx.x * W + w];
+        lds[K * W + threadIdx.x] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = lds[K * W + k];
+    __syncthreads();
+}
+
+__device__ __forceinline__ void block_max2(double& a, double& b, double* lds) {
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmax(a, __shfl_xor(a, off, 64));
+        b = fmax(b, __shfl_xor(b, off, 64));
+    }
+    if (__lane_id() == 0) {
+        lds[wave] = a;
+        lds[kCwWaves + wave] = b;
+    }
+    __syncthreads();
+    a = 0.0;
+    b = 0.0;
+    for (int w = 0; w < kCwWaves; ++w) {
+        a = fmax(a, lds[w]);
+        b = fmax(b, lds[kCwWaves + w]);
+    }
+    __syncthreads();
+}
+
+// Coefficients of the kept orthonormal directions (modified Gram-Schmidt with
+// two passes in the inner product <c, d> = c^T G d, kept directions in basis
+// order) summed into P = sum c c^T.
+__device__ __forceinline__ double cw_inner(const double (&G)[4][4], const double (&a)[4], const double (&b)[4]) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double gb = 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) gb += G[i][j] * b[j];
+        s += a[i] * gb;
+    }
+    return s;
+}
+
+__device__ void cw_projector(const double (&G)[4][4], double (&P)[4][4]) {
+    double Q[4][4];
+    bool keep[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        keep[k] = false;
+        double c[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Q[k][i] = 0.0;
+        if (!(G[k][k] > 0.0)) continue; // inactive mechanism or absent momentum component
+        c[k] = 1.0 / sqrt(G[k][k]);
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+            for (int q = 0; q < k; ++q) {
+                if (!keep[q]) continue;
+                const double proj = cw_inner(G, Q[q], c);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) c[i] -= proj * Q[q][i];
+            }
+        const double len2 = cw_inner(G, c, c);
+        if (!(len2 >= 1e-14)) continue; // dependent direction
+        const double inv = 1.0 / sqrt(len2);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Q[k][i] = c[i] * inv;
+        keep[k] = true;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (keep[k]) v += Q[k][i] * Q[k][j];
+            P[i][j] = v;
+        }
+}
+
+__global__ void __launch_bounds__(kCwThreads) k_callaway(View V) {
+    __shared__ double lds[18 * kCwWaves + 18];
+    __shared__ double Psh[17]; // P_N row-major and 1/G_R
+    const int c = blockIdx.x;
+    const int begin = V.cell_start[c], end = V.cell_start[c + 1];
+    const int n = end - begin;
+    const int k0 = V.chunk_start[c], k1 = V.chunk_start[c + 1];
+    for (int k = k0 + static_cast<int>(threadIdx.x); k < k1; k += kCwThreads) {
+        V.chunk_out[2 * static_cast<size_t>(k)] = 0.0;
+        V.chunk_out[2 * static_cast<size_t>(k) + 1] = 0.0;
+    }
+    if (n <= 0) return;
+    if (n > kCwThreads * kCwPer) {
+        if (threadIdx.x == 0) raise_error(V, PMC_ERR_UNSUPPORTED, c, nullptr, -8);
+        return;
+    }
+    const int t = threadIdx.x;
+    int md[kCwPer];
+    double x[kCwPer], s[kCwPer];
+    // Setup: x = e / sqrt(C), Gram matrices, largest rate and |x|, moments of x.
+    double acc[17];
+#pragma unroll
+    for (int k = 0; k < 17; ++k) acc[k] = 0.0;
+    double rmax = 0.0, xmax = 0.0;
+#pragma unroll
+    for (int r = 0; r < kCwPer; ++r) {
+        const int j = t + r * kCwThreads;
+        md[r] = -1;
+        x[r] = 0.0;
+        if (j < n) {
+            const int m = V.mode[begin + j] & kModeMask;
+            md[r] = m;
+            const double4 p0 = V.cw_pack[2 * m], p1 = V.cw_pack[2 * m + 1];
+            const double occ = V.occ[begin + j];
+            const double e = p1.z * (occ - V.mtab[m].y);
+            if (!isfinite(e)) raise_error(V, PMC_ERR_INVALID_STATE, begin + j, nullptr, -1);
+            const double xi = e / p0.z;
+            x[r] = xi;
+            const double b[4] = {p0.z, p0.w, p1.x, p1.y};
+            int q = 0;
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int d = a; d < 4; ++d) acc[q++] += p0.x * b[a] * b[d];
+            acc[10] += p0.y * b[0] * b[0];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) acc[11 + a] += p0.x * b[a] * xi;
+            acc[15] += p0.y * b[0] * xi;
+            acc[16] += xi * xi;
+            rmax = fmax(rmax, p0.x + p0.y);
+            xmax = fmax(xmax, fabs(xi));
+        }
+    }
+    block_reduce_wide<17, kCwWaves>(acc, lds);
+    block_max2(rmax, xmax, lds);
+    if (t == 0) {
+        double G[4][4], P[4][4];
+        int q = 0;
+        for (int a = 0; a < 4; ++a)
+            for (int d = a; d < 4; ++d) {
+                G[a][d] = acc[q];
+                G[d][a] = acc[q];
+                ++q;
+            }
+        cw_projector(G, P);
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 4; ++d) Psh[4 * a + d] = P[a][d];
+        Psh[16] = acc[10] > 0.0 ? 1.0 / acc[10] : 0.0;
+    }
+    __syncthreads();
+    const double gr_inv = Psh[16];
+    const double dt = V.dt;
+    const bool evolve = xmax > 0.0 && dt > 0.0 && rmax > 0.0;
+    double scale = 1.0;
+    if (evolve) {
+        const double steps_d = fmax(1.0, ceil(dt * rmax / 4.0));
+        if (!(steps_d <= 100000.0) || steps_d * n > 200000000.0) {
+            if (t == 0) raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -9);
+            return;
+        }
+        const int steps = static_cast<int>(steps_d);
+        const double h = dt / steps, scaled_norm = h * rmax, tol = 2e-14 / steps;
+        scale = xmax;
+        const double inv = 1.0 / xmax;
+        double mN[4], mR = acc[15] * inv, xx = acc[16] * inv * inv;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) mN[a] = acc[11 + a] * inv;
+#pragma unroll
+        for (int r = 0; r < kCwPer; ++r) x[r] *= inv;
+        for (int st = 0; st < steps; ++st) {
+            const double reference = sqrt(xx);
+#pragma unroll
+            for (int r = 0; r < kCwPer; ++r) s[r] = x[r];
+            double sN[4] = {mN[0], mN[1], mN[2], mN[3]}, sR = mR; // moments of the running sum
+            bool converged = false;
+            double ss = xx;
+            for (int k = 1; k <= 40; ++k) {
+                double a4[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    a4[i] = Psh[4 * i] * mN[0] + Psh[4 * i + 1] * mN[1] + Psh[4 * i + 2] * mN[2] + Psh[4 * i + 3] * mN[3];
+                const double rr = mR * gr_inv, hk = h / k;
+                double red[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int r = 0; r < kCwPer; ++r) {
+                    if (md[r] < 0) continue;
+                    const double4 p0 = V.cw_pack[2 * md[r]], p1 = V.cw_pack[2 * md[r] + 1];
+                    const double act = -(p0.x + p0.y) * x[r] +
+                                       p0.x * (p0.z * a4[0] + p0.w * a4[1] + p1.x * a4[2] + p1.y * a4[3]) +
+                                       p0.y * p0.z * rr;
+                    const double tn = act * hk;
+                    x[r] = tn;
+                    s[r] += tn;
+                    red[0] += p0.x * p0.z * tn;
+                    red[1] += p0.x * p0.w * tn;
+                    red[2] += p0.x * p1.x * tn;
+                    red[3] += p0.x * p1.y * tn;
+                    red[4] += p0.y * p0.z * tn;
+                    red[5] += tn * tn;
+                    red[6] += s[r] * s[r];
+                }
+                block_reduce_wide<7, kCwWaves>(red, lds);
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    mN[a] = red[a];
+                    sN[a] += red[a];
+                }
+                mR = red[4];
+                sR += red[4];
+                ss = red[6];
+                if (scaled_norm < k + 2) {
+                    const double tail = sqrt(red[5]) * scaled_norm / (k + 1) / (1.0 - scaled_norm / (k + 2));
+                    if (tail <= tol * reference) {
+                        converged = true;
+                        break;
+                    }
+                }
+            }
+            if (!converged) {
+                if (t == 0) raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -10);
+                return;
+            }
+#pragma unroll
+            for (int r = 0; r < kCwPer; ++r) x[r] = s[r];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) mN[a] = sN[a];
+            mR = sR;
+            xx = ss;
+        }
+    }
+    // Occupations from the evolved energies: n = n_eq + (x |x|max sqrt(C)) / hw.
+    double out[2] = {0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < kCwPer; ++r) {
+        if (md[r] < 0) continue;
+        const int i = begin + t + r * kCwThreads;
+        const int m = md[r];
+        const double4 p0 = V.cw_pack[2 * m], p1 = V.cw_pack[2 * m + 1];
+        const double2 mt = V.mtab[m];
+        const double old = V.occ[i];
+        const double evolved = evolve ? x[r] * scale * p0.z : p1.z * (old - mt.y);
+        const double next = mt.y + evolved / p1.z;
+        if (!isfinite(next)) raise_error(V, PMC_ERR_INVALID_STATE, i, nullptr, -1);
+        V.occ[i] = next;
+        out[0] += (kHbar * mt.x * (next - mt.y)) * V.w;
+        out[1] += (p1.z * next - p1.z * old) * V.w;
+    }
+    block_reduce_wide<2, kCwWaves>(out, lds);
+    if (t == 0) {
+        V.chunk_out[2 * static_cast<size_t>(k0)] = out[0];
+        V.chunk_out[2 * static_cast<size_t>(k0) + 1] = out[1];
     }
 }
 
@@ -1811,7 +2151,14 @@ __global__ void __launch_bounds__(256) k_init(View V, int count, int tmode, doub
     V.py[i] = fp[1];
     V.pz[i] = fp[2];
     V.mode[i] = m;
-    V.occ[i] = bose(T, V.omega[m]);
+    // Linearized transport: n_eq(T_bg) + C(T_bg) (T - T_bg) / hw
+    // (PhononMaterial::linearized_bose_occupation, src/material/Thermodynamics.cpp:27-37).
+    if (V.callaway) {
+        const double hw = kHbar * V.omega[m];
+        V.occ[i] = hw > 0.0 ? V.neq[m] + V.cw_cap[m] * (T - V.t_bg) / hw : 0.0;
+    } else {
+        V.occ[i] = bose(T, V.omega[m]);
+    }
     V.cell[i] = cell;
     V.uid[i] = static_cast<unsigned>(i);
 }
@@ -1869,6 +2216,9 @@ void newton(const View& v, hipStream_t s) {
 }
 void relax(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_relax, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v);
+}
+void callaway(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_callaway, dim3(v.ncells), dim3(kCwThreads), 0, s, v);
 }
 void source(const View& v, hipStream_t s, double factor, int half) {
     hipLaunchKernelGGL(k_source_sums, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v, factor, half);

@@ -14,7 +14,7 @@ CONFIGS = os.path.join(PKG, "configs")
 
 PMC_OK = 0
 KERNEL_CLASSES = ["begin", "transport", "events", "inject", "refresh", "sort", "moments", "newton", "relax", "source",
-                  "output", "step"]
+                  "output", "step", "callaway"]
 STATUS = {0: "ok", 1: "invalid argument", 2: "device error", 3: "excess transport events", 4: "boundary failure",
           5: "escaped carrier", 6: "invalid state", 7: "capacity exhausted", 8: "latched", 9: "unsupported"}
 
@@ -68,6 +68,12 @@ class HeatSource(C.Structure):
     _fields_ = [("n_cells", C.c_int32), ("rate", C.POINTER(C.c_double)), ("mode_eligible", C.POINTER(C.c_int32)),
                 ("square", C.c_int32), ("start_ps", C.c_double), ("end_ps", C.c_double), ("period_ps", C.c_double),
                 ("on_duration_ps", C.c_double), ("duty_cycle", C.c_double), ("amplitude", C.c_double)]
+
+
+class CallawayTable(C.Structure):
+    _fields_ = [("capacity", C.POINTER(C.c_double)), ("wavevector", C.POINTER(C.c_double)),
+                ("normal_rate", C.POINTER(C.c_double)), ("resistive_rate", C.POINTER(C.c_double)),
+                ("heat_capacity_density", C.c_double), ("background_energy_density", C.c_double)]
 
 
 class Tuning(C.Structure):
@@ -175,6 +181,8 @@ def lib():
         "pmc_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pmc_upload_heat_source": (C.c_int, [P, C.POINTER(HeatSource)]),
         "pmcs_problem_heat_source": (C.c_int, [P, C.POINTER(HeatSource)]),
+        "pmcs_problem_callaway": (C.c_int, [P, C.POINTER(CallawayTable)]),
+        "pmc_upload_callaway": (C.c_int, [P, C.POINTER(CallawayTable)]),
         "pmc_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pmc_destroy": (None, [P]),
         "pmc_status_string": (C.c_char_p, [C.c_int32]),
@@ -241,6 +249,11 @@ class Problem:
         hs = HeatSource()
         return hs if lib().pmcs_problem_heat_source(self.h, C.byref(hs)) == 1 else None
 
+    def callaway(self):
+        """The linearized Callaway view, or None under RTA."""
+        t = CallawayTable()
+        return t if lib().pmcs_problem_callaway(self.h, C.byref(t)) == 1 else None
+
     def material(self):
         mv = MaterialView()
         lib().pmcs_problem_material(self.h, C.byref(mv))
@@ -298,6 +311,9 @@ class Context:
         rough_arr = (RoughTable * max(1, len(rough)))(*rough)
         res_arr = (Reservoir * max(1, len(res)))(*res)
         self._check(L.pmc_upload_boundary_tables(h, len(rough), rough_arr, len(res), res_arr))
+        cw = problem.callaway()
+        if cw is not None:
+            self._check(L.pmc_upload_callaway(h, C.byref(cw)))
         hs = problem.heat_source()
         if hs is not None:
             self._check(L.pmc_upload_heat_source(h, C.byref(hs)))

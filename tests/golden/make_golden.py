@@ -2,6 +2,7 @@
 sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
 
     make -f oracle/ref.mk REF=/root/reference && python tests/golden/make_golden.py [--runs] [--source-runs]
+    python tests/golden/make_golden.py --callaway [--callaway-runs]   (Callaway fixtures only)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -22,7 +23,8 @@ from oracle import ref_ctypes as R  # noqa: E402
 from phonomc_amd import runtime as rt  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
-WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml"}
+WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml",
+             "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml"}
 
 
 def problem_and_reference(key, **overrides):
@@ -166,6 +168,43 @@ def source_vectors(problem, rm, rd, rng, factor):
                 updates=upd)
 
 
+def callaway_vectors(problem, rng):
+    """PhononMaterial::callaway_data(T_bg) of the reference built with stationary
+    modes (SHA-256 of each array plus 64 sample rows) and CarrierCollisionOperator
+    applied to random carrier sets at three time steps (the last takes two
+    Taylor substeps)."""
+    import hashlib
+    s, m, g, i, rough, res = problem.views()
+    rm = R.RefMaterial(problem.material(), include_stationary=True)
+    na, nc = m.n_active, g.n_cells
+    t_bg = s.background_temperature
+    d = R.callaway_data(rm, t_bg)
+    out = {f"digest_{k}": np.array(hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest())
+           for k, v in d.items()}
+    rows = rng.choice(na, 64, replace=False)
+    out["rows"] = rows
+    for k, v in d.items():
+        out[f"rows_{k}"] = v[rows]
+    flat = rt.arr(m.active_flat, na, np.int32)
+    neq = rt.arr(m.neq_background, na)
+    hw = 6.582119569e-4 * d["omega"]
+    vp = problem.scalars()["particle_volume"]
+    for tag, dt in (("a", 1.0), ("b", 3.0), ("c", 12.0)):
+        n = 5000
+        act = rng.integers(0, na, n).astype(np.int32)
+        cell = rng.integers(0, nc, n).astype(np.int32)
+        t = 298.0 + 4.0 * rng.random(n)
+        occ = neq[act] + d["capacity"][act] * (t - t_bg) / hw[act]  # linearized occupations
+        occ = occ + 0.01 * rng.normal(size=n) * (neq[act] + 0.1)
+        new, residual = R.callaway_apply(rm, t_bg, nc, act, cell, occ, dt, vp)
+        out.update({f"{tag}_dt": dt, f"{tag}_active": act, f"{tag}_cell": cell, f"{tag}_occ": occ,
+                    f"{tag}_occ_after": new, f"{tag}_residual": residual})
+    out["active_count"] = na
+    out["flat"] = flat
+    rm.close()
+    return out
+
+
 def whole_run(key, n, steps, seeds, interval):
     """Reference runs at reduced size: final cell temperatures and kappa per seed."""
     temps, keff, kint = [], [], []
@@ -193,6 +232,16 @@ def whole_run(key, n, steps, seeds, interval):
 
 
 def main():
+    if "--callaway" in sys.argv:
+        rng = np.random.default_rng(20261204)
+        text = rt.config_text(WORKLOADS["c1c"])
+        np.savez_compressed(os.path.join(OUT, "callaway_c1c.npz"),
+                            **callaway_vectors(rt.Problem(text, mode_sampling=1), rng))
+        if "--callaway-runs" in sys.argv:
+            np.savez_compressed(os.path.join(OUT, "run_c1c.npz"),
+                                **whole_run("c1c", 20000, 400, list(range(1000, 1016)), 10))
+        print("Callaway fixtures written to", OUT)
+        return
     rng = np.random.default_rng(20261203)
     p1, rm1, rd1, _ = problem_and_reference("c1")
     p2, rm2, rd2, _ = problem_and_reference("c2")

@@ -151,8 +151,13 @@ def test_config_rejections_use_reference_messages(bad, message):
 
 
 def test_unsupported_features_fail_loudly():
-    with pytest.raises(RuntimeError, match="callaway"):
+    # Callaway on a table without an N/U split: the reference's callaway_data message.
+    with pytest.raises(RuntimeError, match="Callaway requires gamma_N and gamma_U"):
         rt.Problem(rt.config_text("c1_cross_plane.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))
+    # Callaway with a heat source is outside this build.
+    with pytest.raises(RuntimeError, match="heat_source with scattering.model = 'callaway'"):
+        rt.Problem(rt.config_text("c1s_source.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]")
+                   .replace("synthetic:si16", "synthetic:si15:callaway"))
 
 
 def test_heat_source_rates_match_reference(golden):
