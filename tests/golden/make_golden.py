@@ -240,6 +240,8 @@ def main():
         if "--callaway-runs" in sys.argv:
             np.savez_compressed(os.path.join(OUT, "run_c1c.npz"),
                                 **whole_run("c1c", 20000, 400, list(range(1000, 1016)), 10))
+            np.savez_compressed(os.path.join(OUT, "run_c2c.npz"),
+                                **whole_run("c2c", 200000, 300, list(range(1000, 1016)), 10))
         print("Callaway fixtures written to", OUT)
         return
     rng = np.random.default_rng(20261203)

@@ -1,6 +1,6 @@
 """Whole-run statistics of the device step against the reference solver.
 
-tests/golden/run_c1.npz, run_c2.npz and run_c1s.npz hold 16 runs of the
+tests/golden/run_c1.npz, run_c2.npz, run_c1s.npz, run_c1c.npz and run_c2c.npz hold 16 runs of the
 reference PhonoMC build per workload (seeds 1000..1015, written by tests/golden/make_golden.py):
 second-half averages of every cell temperature and of kappa_int / kappa_eff as
 the reference writes them to convergence.txt.  The same TOML inputs and seeds
@@ -23,7 +23,8 @@ from phonomc_amd import runtime as rt
 
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
-WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml"}
+WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml",
+             "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml"}
 
 
 def second_half(folder, ncell, steps):
@@ -61,9 +62,10 @@ def z_score(a, b):
     return (a.mean(axis=0) - b.mean(axis=0)) / se
 
 
-@pytest.mark.parametrize("key", ["c1", "c2", "c1s"])
+@pytest.mark.parametrize("key", ["c1", "c2", "c1s", "c1c", "c2c"])
 def test_whole_run_statistics_match_reference(key):
-    """c1s adds the Gaussian heat source (kappa is not estimated with a source)."""
+    """c1s adds the Gaussian heat source (kappa is not estimated with a source);
+    c1c and c2c run linearized Callaway scattering (c2c with strict-elastic rough walls)."""
     golden = np.load(os.path.join(HERE, "golden", f"run_{key}.npz"))
     temps, keff, kint = gpu_runs(key, golden)
     zt = z_score(temps, golden["temperature"])
