@@ -29,6 +29,7 @@ dense BF16 MFMA peak.
 import argparse
 import json
 import os
+import re
 import sys
 import tempfile
 import time
@@ -45,9 +46,11 @@ BYTES_PER_CARRIER = {
     "sort": 32 + 32,       # read pos, mode, occ, cell, uid (32); write them in cell order (32)
     "moments": 12 + 8,     # read mode(4) occ(8); write relaxation weight (8)
     "relax": 20 + 8,       # read mode(4) occ(8) weight(8); write occ(8)
+    "callaway": 12 + 8,    # read mode(4) occ(8); write occ(8) (the Taylor terms stay in registers)
 }
 # Kernel names of each class in rocprofv3 output.
-KERNEL_NAMES = {"transport": "k_transport", "sort": "k_sort", "moments": "k_moments", "relax": "k_relax"}
+KERNEL_NAMES = {"transport": "k_transport", "sort": "k_sort", "moments": "k_moments", "relax": "k_relax",
+                "callaway": "k_callaway"}
 # Whole-step algorithmic bytes per particle-step: fused transport + tally pass
 # (read and write pos 12, mode 4, occ 8, cell 4) and collision pass (read mode,
 # cell, occ; write occ).
@@ -69,15 +72,15 @@ def host_cores():
     return max(1, min(16, n))
 
 
-def cpu_baseline(problem_text, steps):
+def cpu_baseline(problem_text, steps, workload=WORKLOAD, carriers=1e7):
     """Times the reference solver (oracle/_ref) on the full workload for `steps` steps."""
     from oracle import ref_ctypes
     from phonomc_amd import runtime as rt
 
     cores = host_cores()
     tmp = tempfile.mkdtemp(prefix="pmc_ref_")
-    text = problem_text.replace('output_folder = "results/c2_inplane_film"', f'output_folder = "{tmp}/out"')
-    path = os.path.join(tmp, "c2.toml")
+    text = re.sub(r'output_folder = "[^"]*"', f'output_folder = "{tmp}/out"', problem_text)
+    path = os.path.join(tmp, "workload.toml")
     with open(path, "w") as f:
         f.write(text)
     problem = rt.Problem(text)
@@ -104,7 +107,7 @@ def cpu_baseline(problem_text, steps):
         ps = sim.ledger()["carrier_steps"] - before
         kind = "port"
     return {"value": ps / secs, "unit": "particle-steps/s", "cores": cores, "kind": kind,
-            "sample": f"{WORKLOAD} at 1e7 carriers, {steps} steps of run_timestep after initialization, "
+            "sample": f"{workload} at {carriers:.3g} carriers, {steps} steps of run_timestep after initialization, "
                       f"OpenMP threads={cores}", "seconds": secs}
 
 
@@ -233,6 +236,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the warm-start leg")
+    ap.add_argument("--workload", default=WORKLOAD, help="bundled TOML input (phonomc_amd/configs) to time")
+    ap.add_argument("--particles", type=int, default=0, help="override simulation.particle_count")
     ap.add_argument("--surrogate-only", type=int, metavar="EPOCHS", default=0,
                     help="only train the operator for EPOCHS epochs and print its GEMM roofline (for profiling)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -257,11 +262,11 @@ def main():
 
     from phonomc_amd import runtime as rt
 
-    text = rt.config_text(WORKLOAD)
+    text = rt.config_text(args.workload, **({"simulation.particle_count": args.particles} if args.particles else {}))
     problem = rt.Problem(text)
     ctx = rt.Context(problem)
     n0 = ctx.ledger()["live_carriers"]
-    log(f"[bench] {WORKLOAD}: {n0} carriers initialized; warmup {args.warmup} steps")
+    log(f"[bench] {args.workload}: {n0} carriers initialized; warmup {args.warmup} steps")
     ctx.step(args.warmup)
     led0 = ctx.ledger()  # synchronizes the device
     t0 = time.perf_counter()
@@ -304,7 +309,7 @@ def main():
     baseline = None
     if not args.no_cpu_baseline:
         log(f"[bench] CPU baseline: reference solver, {args.cpu_steps} steps")
-        baseline = cpu_baseline(text, args.cpu_steps)
+        baseline = cpu_baseline(text, args.cpu_steps, args.workload, n0)
         log(f"[bench] CPU baseline {baseline['value']:.4e} particle-steps/s on {baseline['cores']} threads")
 
     ctx.close()
@@ -328,9 +333,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 flight / f64 tallies",
         "data": "synthetic (seeded Si-like 16^3x6 phonon table, Philox-sampled carriers)",
-        "config": {"workload": "in-plane film 1000x100 nm, 50x10 cells, rough 1 nm walls, 302/298 K reservoirs, "
-                               "local lifetimes, 1e7 carriers, dt 0.5 ps",
-                   "carriers": n0, "cells": int(problem.scalars()["cells"]), "time_step_ps": 0.5},
+        "config": {"workload": ("in-plane film 1000x100 nm, 50x10 cells, rough 1 nm walls, 302/298 K reservoirs, "
+                                "local lifetimes, 1e7 carriers, dt 0.5 ps") if args.workload == WORKLOAD and
+                               not args.particles else f"{args.workload} ({n0} carriers)",
+                   "carriers": n0, "cells": int(problem.scalars()["cells"]),
+                   "time_step_ps": problem.views()[0].time_step_ps},
         "roofline": roofline,
         "cpu_baseline": baseline,
         "rta": {"evaluations": led1["newton_evals"] - led0["newton_evals"],
