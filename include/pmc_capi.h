@@ -83,10 +83,46 @@ typedef struct pmc_facet {
     int32_t periodic_pair;         /* facet or -1 */
 } pmc_facet;
 
+/* Node of a bounding-volume hierarchy over triangles or tetrahedra.  Boxes are
+ * FP32, rounded outward.  An interior node (count == 0) has its left child next
+ * in the array and its right child at `index`; a leaf holds `count` items
+ * order[index .. index+count). */
+typedef struct pmc_bvh_node {
+    float bmin[3], bmax[3];
+    int32_t index;
+    int32_t count;
+} pmc_bvh_node;
+
+/* Closed triangle surface of a mesh model (geometry kind 1); replaces
+ * SurfaceMesh::trace_boundary_intersection (src/SurfaceMesh.cpp:1164-1270),
+ * contains_point (:1126-1151, with the star tetrahedra of :783-817) and
+ * sample_volume_points (src/geometry/SurfaceSampling.cpp:113-155). */
+typedef struct pmc_mesh {
+    int32_t n_vertices;
+    const double* vertices;        /* [n_vertices*3] Angstrom */
+    int32_t n_faces;
+    const int32_t* faces;          /* [n_faces*3] outward winding */
+    const double* face_normal;     /* [n_faces*3] outward unit normal */
+    const double* face_k;          /* [n_faces] plane n.x + k = 0 */
+    const double* face_bounds;     /* [n_faces*6] min x,y,z then max x,y,z */
+    const int32_t* face_facet;     /* [n_faces] facet of each face */
+    int32_t n_face_nodes;
+    const pmc_bvh_node* face_nodes;
+    const int32_t* face_order;     /* [n_faces] */
+    int32_t n_simplices;           /* 0: membership by ray casting alone */
+    const double* simplex_vertices;/* [n_simplices*12] shared apex, then the face's three vertices */
+    const double* simplex_inverse; /* [n_simplices*9] row-major inverse of [a-o | b-o | c-o] */
+    const double* simplex_bounds;  /* [n_simplices*6] */
+    const double* simplex_volume;  /* [n_simplices] */
+    int32_t n_simplex_nodes;
+    const pmc_bvh_node* simplex_nodes;
+    const int32_t* simplex_order;  /* [n_simplices] */
+} pmc_mesh;
+
 /* Geometry and cell grid; replaces SimulationDomain::trace_boundary_intersection
  * (src/SimulationDomain.cpp:342-450) and fast_grid_index (:92-126). */
 typedef struct pmc_geometry {
-    int32_t kind;                  /* 0: analytic axis-aligned box */
+    int32_t kind;                  /* 0: analytic axis-aligned box, 1: triangle mesh */
     double bmin[3], bmax[3];
     int32_t n_facets;
     const pmc_facet* facets;
@@ -96,6 +132,7 @@ typedef struct pmc_geometry {
     const int32_t* voxel_to_cell;  /* [grid product] */
     int32_t n_cells;
     const double* cell_volume;     /* [n_cells] A^3 */
+    const pmc_mesh* mesh;          /* kind 1 only */
 } pmc_geometry;
 
 /* Rough-facet reflection table; replaces RoughBoundarySampler

@@ -57,6 +57,11 @@ int pmcs_problem_heat_source(const pmcs_problem* problem, pmc_heat_source* out);
 /* Linearized Callaway view (valid while the problem lives): 1 if the input selects
  * scattering.model = 'callaway' (linearized), 0 if not. */
 int pmcs_problem_callaway(const pmcs_problem* problem, pmc_callaway_table* out);
+/* Writes the procedural FinFET surface (a substrate box with one fin centred on
+ * its top, lattice faces as triangles; sizes and lattice step in nm) as a binary
+ * STL; returns the triangle count, or -1 with err filled. */
+int pmcs_write_finfet_stl(const char* path, const double substrate_nm[3], const double fin_nm[3],
+                          const double step_nm[3], char* err, int errlen);
 int pmcs_problem_scalars(const pmcs_problem* problem, double* out);
 /* Derived material queries, for checks against the reference's PhononMaterial. */
 double pmcs_problem_bose(const pmcs_problem* problem, double temperature, int32_t flat_mode);

@@ -176,6 +176,203 @@ int orc_grid_index(const pmc_geometry* g, const double p[3]) {
     return g->voxel_to_cell[(idx[0] * g->grid[1] + idx[1]) * g->grid[2] + idx[2]];
 }
 
+namespace {
+V3 cross(const V3& a, const V3& b) {
+    return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+V3 mesh_vertex(const pmc_mesh* M, int face, int k) {
+    const int v = M->faces[3 * face + k];
+    return {M->vertices[3 * v], M->vertices[3 * v + 1], M->vertices[3 * v + 2]};
+}
+// Moller-Trumbore with 1e-10 tolerances (src/SurfaceMesh.cpp:162-186).
+bool ray_triangle(const V3& o, const V3& d, const V3& a, const V3& b, const V3& c, double& t) {
+    const double eps = 1e-10;
+    const V3 e1 = sub(b, a), e2 = sub(c, a);
+    const V3 p = cross(d, e2);
+    const double det = dot(e1, p);
+    if (std::abs(det) < eps) return false;
+    const double inv = 1.0 / det;
+    const V3 s = sub(o, a);
+    const double u = dot(s, p) * inv;
+    if (u < -eps || u > 1.0 + eps) return false;
+    const V3 q = cross(s, e1);
+    const double v = dot(d, q) * inv;
+    if (v < -eps || (u + v) > 1.0 + eps) return false;
+    t = dot(e2, q) * inv;
+    return t > eps;
+}
+// Voronoi-region closest point (src/SurfaceMesh.cpp:111-159).
+V3 closest_point_on_triangle(const V3& p, const V3& a, const V3& b, const V3& c) {
+    const V3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+    const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) return a;
+    const V3 bp = sub(p, b);
+    const double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) return b;
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) return add(a, mul(ab, d1 / (d1 - d3)));
+    const V3 cp = sub(p, c);
+    const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) return c;
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return add(a, mul(ac, d2 / (d2 - d6)));
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0)
+        return add(b, mul(sub(c, b), (d4 - d3) / ((d4 - d3) + (d5 - d6))));
+    const double denom = 1.0 / (va + vb + vc);
+    return add(add(a, mul(ab, vb * denom)), mul(ac, vc * denom));
+}
+// +x ray cast, crossings closer than 1e-9 counted once (src/SurfaceMesh.cpp:1100-1123).
+bool ray_cast_inside(const pmc_mesh* M, const V3& p) {
+    std::vector<double> hits;
+    for (int f = 0; f < M->n_faces; ++f) {
+        double t = 0;
+        if (ray_triangle(p, {1.0, 0.0, 0.0}, mesh_vertex(M, f, 0), mesh_vertex(M, f, 1), mesh_vertex(M, f, 2), t))
+            hits.push_back(t);
+    }
+    if (hits.empty()) return false;
+    std::sort(hits.begin(), hits.end());
+    int unique = 1;
+    for (size_t i = 1; i < hits.size(); ++i)
+        if (std::abs(hits[i] - hits[i - 1]) > 1e-9) ++unique;
+    return unique % 2 == 1;
+}
+} // namespace
+
+int orc_trace_mesh(const pmc_geometry* g, const double pos[3], const double vel[3], double point[3], double* t) {
+    // SurfaceMesh::trace_boundary_intersection (src/SurfaceMesh.cpp:1164-1270): the
+    // nearest face plane crossing with t > 1e-10 whose point lies within 1e-9 of the
+    // face's box and 1e-7 of the triangle.  Faces are visited in index order with a
+    // strict improvement test; the reference visits them in BVH order, which can
+    // only differ on an exact tie in t between faces of two facets.
+    const pmc_mesh* M = g->mesh;
+    const double eps = 1e-10;
+    const V3 x{pos[0], pos[1], pos[2]}, v{vel[0], vel[1], vel[2]};
+    double best_t = std::numeric_limits<double>::infinity();
+    int best = -1;
+    V3 best_p = x;
+    for (int f = 0; f < M->n_faces; ++f) {
+        const V3 n{M->face_normal[3 * f], M->face_normal[3 * f + 1], M->face_normal[3 * f + 2]};
+        const double denom = dot(v, n);
+        if (std::abs(denom) < eps) continue;
+        const double th = -(dot(x, n) + M->face_k[f]) / denom;
+        if (th <= eps || th >= best_t) continue;
+        const V3 p = add(x, mul(v, th));
+        const double* bb = M->face_bounds + 6 * f;
+        if (p[0] < bb[0] - 1e-9 || p[1] < bb[1] - 1e-9 || p[2] < bb[2] - 1e-9 || p[0] > bb[3] + 1e-9 ||
+            p[1] > bb[4] + 1e-9 || p[2] > bb[5] + 1e-9)
+            continue;
+        const V3 cp = closest_point_on_triangle(p, mesh_vertex(M, f, 0), mesh_vertex(M, f, 1), mesh_vertex(M, f, 2));
+        if (norm(sub(cp, p)) > 1e-7) continue;
+        best_t = th;
+        best_p = p;
+        best = f;
+    }
+    for (int a = 0; a < 3; ++a) point[a] = best_p[a];
+    if (best < 0) {
+        *t = std::numeric_limits<double>::infinity();
+        return -1;
+    }
+    *t = best_t;
+    return M->face_facet[best];
+}
+
+int orc_contains_mesh(const pmc_geometry* g, const double pos[3]) {
+    // SurfaceMesh::contains_point (src/SurfaceMesh.cpp:1126-1151): star tetrahedra,
+    // inclusive to 1e-10 in barycentric coordinates, then the +x ray cast with
+    // crossings closer than 1e-9 counted once (:1100-1123).
+    const pmc_mesh* M = g->mesh;
+    const V3 p{pos[0], pos[1], pos[2]};
+    const double tol = 1e-10;
+    for (int s = 0; s < M->n_simplices; ++s) {
+        const double* b = M->simplex_bounds + 6 * s;
+        if (p[0] < b[0] - tol || p[1] < b[1] - tol || p[2] < b[2] - tol || p[0] > b[3] + tol || p[1] > b[4] + tol ||
+            p[2] > b[5] + tol)
+            continue;
+        const double* o = M->simplex_vertices + 12 * s;
+        const double* A = M->simplex_inverse + 9 * s;
+        const V3 q{p[0] - o[0], p[1] - o[1], p[2] - o[2]};
+        const double l1 = A[0] * q[0] + A[1] * q[1] + A[2] * q[2];
+        const double l2 = A[3] * q[0] + A[4] * q[1] + A[5] * q[2];
+        const double l3 = A[6] * q[0] + A[7] * q[1] + A[8] * q[2];
+        const double l0 = 1.0 - l1 - l2 - l3;
+        if (l0 >= -tol && l1 >= -tol && l2 >= -tol && l3 >= -tol && l0 <= 1.0 + tol && l1 <= 1.0 + tol &&
+            l2 <= 1.0 + tol && l3 <= 1.0 + tol)
+            return 1;
+    }
+    return ray_cast_inside(M, p) ? 1 : 0;
+}
+
+int orc_nearest_facet(const pmc_geometry* g, const double pos[3]) {
+    // SurfaceMesh::nearest_facet / nearest_face (src/SurfaceMesh.cpp:996-1022).
+    const pmc_mesh* M = g->mesh;
+    const V3 p{pos[0], pos[1], pos[2]};
+    double best = std::numeric_limits<double>::infinity();
+    int idx = -1;
+    for (int f = 0; f < M->n_faces; ++f) {
+        const V3 cp = closest_point_on_triangle(p, mesh_vertex(M, f, 0), mesh_vertex(M, f, 1), mesh_vertex(M, f, 2));
+        const double d = norm(sub(p, cp));
+        if (d < best) {
+            best = d;
+            idx = f;
+        }
+    }
+    return idx < 0 ? -1 : M->face_facet[idx];
+}
+
+int orc_sample_volume(const pmc_geometry* g, uint32_t k0, uint32_t k1, uint32_t id, double point[3], double* u_mode) {
+    // SurfaceMesh::sample_volume_points (src/geometry/SurfaceSampling.cpp:113-155) on
+    // Philox draws: a star tetrahedron by volume, then Dirichlet(1,1,1,1) weights
+    // from normalized exponentials.  Without a star decomposition, rejection in the
+    // bounding box against the ray cast (:81-106), up to 256 trials per point.
+    // Returns 0, or -1 when no trial was accepted.
+    const pmc_mesh* M = g->mesh;
+    uint32_t a[4], b[4], c[4];
+    orc_philox(id, 0, 0, 3u, k0, k1, a);
+    orc_philox(id, 0, 1, 3u, k0, k1, b);
+    *u_mode = orc_u53(b[2], b[3]);
+    if (M->n_simplices > 0) {
+        orc_philox(id, 0, 2, 3u, k0, k1, c);
+        double total = 0.0;
+        for (int s = 0; s < M->n_simplices; ++s) total += M->simplex_volume[s];
+        const double target = orc_u53(a[0], a[1]) * total;
+        double acc = 0.0;
+        int pick = M->n_simplices - 1;
+        for (int s = 0; s < M->n_simplices; ++s) {
+            acc += M->simplex_volume[s];
+            if (acc > target) {
+                pick = s;
+                break;
+            }
+        }
+        const double u[4] = {orc_u53(a[2], a[3]), orc_u53(b[0], b[1]), orc_u53(c[0], c[1]), orc_u53(c[2], c[3])};
+        double w[4], sum = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            w[k] = -std::log(std::max(u[k], 1e-15));
+            sum += w[k];
+        }
+        const double* v = M->simplex_vertices + 12 * pick;
+        for (int d = 0; d < 3; ++d) {
+            double x = 0.0;
+            for (int k = 0; k < 4; ++k) x += (w[k] / sum) * v[3 * k + d];
+            point[d] = x;
+        }
+        return 0;
+    }
+    for (uint32_t trial = 0; trial < 256; ++trial) {
+        orc_philox(id, 1 + trial, 0, 3u, k0, k1, a);
+        orc_philox(id, 1 + trial, 1, 3u, k0, k1, b);
+        const double u[3] = {orc_u53(a[0], a[1]), orc_u53(a[2], a[3]), orc_u53(b[0], b[1])};
+        V3 p;
+        for (int d = 0; d < 3; ++d) p[d] = g->bmin[d] + (g->bmax[d] - g->bmin[d]) * u[d];
+        if (ray_cast_inside(M, p)) {
+            for (int d = 0; d < 3; ++d) point[d] = p[d];
+            return 0;
+        }
+    }
+    return -1;
+}
+
 int orc_rough_sample(const pmc_mode_table* m, const pmc_rough_table* r, int in_ai, double in_occ, const double* u,
                      int* out_ai, double* out_occ, int* kind) {
     return orc_rough_sample_signed(m, r, in_ai, in_occ, u, out_ai, out_occ, kind, 0);
@@ -674,34 +871,50 @@ bool update_cache(orc_sim& s, int64_t i) {
         s.ctime[i] = std::numeric_limits<double>::infinity();
         return true;
     }
+    const bool mesh = s.g.kind == 1;
+    auto trace = [&](const double* x, double* cp, double* t) {
+        return mesh ? orc_trace_mesh(&s.g, x, s.vel[i].data(), cp, t) : orc_trace_box(&s.g, x, s.vel[i].data(), cp, t);
+    };
     double cp[3], t = 0;
-    int f = orc_trace_box(&s.g, s.pos[i].data(), s.vel[i].data(), cp, &t);
+    int f = trace(s.pos[i].data(), cp, &t);
     if (f < 0 || !std::isfinite(t)) {
         const V3 original = s.pos[i];
         double ext = 1.0;
         for (int a = 0; a < 3; ++a) ext = std::max(ext, std::abs(s.g.bmax[a] - s.g.bmin[a]));
         const double eps = std::max(s.setup.push_distance_a, 1e-10 * ext);
-        // nearest box plane
-        int best = 0;
-        double bd = std::numeric_limits<double>::infinity();
-        for (int p = 0; p < 6; ++p) {
-            const int a = p / 2;
-            const double d = std::abs(original[a] - (p % 2 ? s.g.bmax[a] : s.g.bmin[a]));
-            if (d < bd) {
-                bd = d;
-                best = p;
+        int nearest = -1;
+        if (mesh) {
+            nearest = orc_nearest_facet(&s.g, original.data());
+        } else { // nearest box plane
+            int best = 0;
+            double bd = std::numeric_limits<double>::infinity();
+            for (int p = 0; p < 6; ++p) {
+                const int a = p / 2;
+                const double d = std::abs(original[a] - (p % 2 ? s.g.bmax[a] : s.g.bmin[a]));
+                if (d < bd) {
+                    bd = d;
+                    best = p;
+                }
             }
+            nearest = s.g.box_plane_facet[best];
         }
-        const pmc_facet& F = s.facets[s.g.box_plane_facet[best]];
-        const V3 n{F.normal[0], F.normal[1], F.normal[2]};
         const V3 dir = mul(s.vel[i], 1.0 / speed);
-        const V3 cands[4] = {add(original, mul(n, eps)), add(original, mul(n, -eps)), add(original, mul(dir, eps)),
-                             add(original, mul(dir, -eps))};
+        V3 cands[4] = {original, original, add(original, mul(dir, eps)), add(original, mul(dir, -eps))};
+        if (nearest >= 0) {
+            const pmc_facet& F = s.facets[nearest];
+            const V3 n{F.normal[0], F.normal[1], F.normal[2]};
+            cands[0] = add(original, mul(n, eps));
+            cands[1] = add(original, mul(n, -eps));
+        }
         for (const V3& c : cands) {
             bool inside = true;
-            for (int a = 0; a < 3; ++a) inside = inside && c[a] >= s.g.bmin[a] && c[a] <= s.g.bmax[a];
+            if (mesh) {
+                inside = orc_contains_mesh(&s.g, c.data()) != 0;
+            } else {
+                for (int a = 0; a < 3; ++a) inside = inside && c[a] >= s.g.bmin[a] && c[a] <= s.g.bmax[a];
+            }
             if (!inside) continue;
-            f = orc_trace_box(&s.g, c.data(), s.vel[i].data(), cp, &t);
+            f = trace(c.data(), cp, &t);
             if (f >= 0 && std::isfinite(t)) {
                 s.pos[i] = c;
                 break;
@@ -895,7 +1108,8 @@ void orc_set_threads(int threads) {
 
 int orc_create(const pmc_setup* setup, const pmc_mode_table* modes, const pmc_geometry* geometry, int32_t n_rough,
                const pmc_rough_table* rough, int32_t n_res, const pmc_reservoir* res, orc_sim** out) {
-    if (!setup || !modes || !geometry || !out || geometry->kind != 0) return PMC_ERR_ARGUMENT;
+    if (!setup || !modes || !geometry || !out) return PMC_ERR_ARGUMENT;
+    if (geometry->kind != 0 && !(geometry->kind == 1 && geometry->mesh)) return PMC_ERR_ARGUMENT;
     auto* s = new orc_sim();
     s->setup = *setup;
     s->m = *modes;
@@ -958,7 +1172,12 @@ int orc_initialize(orc_sim* s, const pmc_init* init) {
         const double u[4] = {orc_u53(a[0], a[1]), orc_u53(a[2], a[3]), orc_u53(b[0], b[1]), orc_u53(b[2], b[3])};
         V3 p;
         for (int k = 0; k < 3; ++k) p[k] = s->g.bmin[k] + (s->g.bmax[k] - s->g.bmin[k]) * u[k];
-        const int m = std::min(static_cast<int>(u[3] * na), na - 1);
+        double u_mode = u[3];
+        if (s->g.kind == 1 && orc_sample_volume(&s->g, s->k0, s->k1, static_cast<uint32_t>(i), p.data(), &u_mode)) {
+            set_error(s, "sample_volume_points_naive: insufficient accepted samples.");
+            return PMC_ERR_BOUNDARY;
+        }
+        const int m = std::min(static_cast<int>(u_mode * na), na - 1);
         double T = init->uniform_temperature;
         if (init->temperature_mode == 1) {
             const V3 ax{init->hot_point[0] - init->cold_point[0], init->hot_point[1] - init->cold_point[1],
@@ -1030,6 +1249,14 @@ int orc_step(orc_sim* s, int32_t nsteps) {
     const int nres = static_cast<int>(s->res.size());
     for (int it = 0; it < nsteps; ++it) {
         pmc_ledger& L = s->ledger;
+        if (s->g.kind == 1 && s->step % 100 == 0) {
+            // TransportOperator::validate_containment (src/solver/TransportOperator.cpp:364-402).
+            for (size_t i = 0; i < s->pos.size(); ++i) {
+                if (orc_contains_mesh(&s->g, s->pos[i].data())) continue;
+                set_error(s, "Particle " + std::to_string(i) + " escaped the domain.");
+                return PMC_ERR_ESCAPED;
+            }
+        }
         L.carrier_steps += static_cast<int64_t>(s->mode.size());
         // First half of the symmetric source split (src/MonteCarloSolver.cpp:366-372).
         const double mid = s->elapsed + 0.5 * dt, end = s->elapsed + dt;

@@ -27,6 +27,12 @@ double orc_u53(uint32_t lo, uint32_t hi);
 int orc_trace_box(const pmc_geometry* g, const double pos[3], const double vel[3], double point[3], double* t);
 /* fast_grid_index (src/SimulationDomain.cpp:92-126). */
 int orc_grid_index(const pmc_geometry* g, const double pos[3]);
+/* Mesh geometry (kind 1): SurfaceMesh::trace_boundary_intersection, contains_point,
+ * nearest_facet and sample_volume_points on Philox draws (oracle.cpp). */
+int orc_trace_mesh(const pmc_geometry* g, const double pos[3], const double vel[3], double point[3], double* t);
+int orc_contains_mesh(const pmc_geometry* g, const double pos[3]);
+int orc_nearest_facet(const pmc_geometry* g, const double pos[3]);
+int orc_sample_volume(const pmc_geometry* g, uint32_t k0, uint32_t k1, uint32_t id, double point[3], double* u_mode);
 
 /* RoughBoundarySampler::sample (src/solver/RoughBoundarySampler.cpp:459-533) fed
  * explicit uniforms u[0..2] in draw order; returns the number consumed. */

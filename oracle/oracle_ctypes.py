@@ -32,6 +32,11 @@ def lib():
         L.orc_philox.argtypes = [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)]
         L.orc_trace_box.argtypes = [C.POINTER(rt.Geometry), C.POINTER(d), C.POINTER(d), C.POINTER(d), C.POINTER(d)]
         L.orc_grid_index.argtypes = [C.POINTER(rt.Geometry), C.POINTER(d)]
+        L.orc_trace_mesh.argtypes = [C.POINTER(rt.Geometry), C.POINTER(d), C.POINTER(d), C.POINTER(d), C.POINTER(d)]
+        L.orc_contains_mesh.argtypes = [C.POINTER(rt.Geometry), C.POINTER(d)]
+        L.orc_nearest_facet.argtypes = [C.POINTER(rt.Geometry), C.POINTER(d)]
+        L.orc_sample_volume.argtypes = [C.POINTER(rt.Geometry), C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(d),
+                                        C.POINTER(d)]
         L.orc_rough_sample.argtypes = [C.POINTER(rt.ModeTable), C.POINTER(rt.RoughTable), C.c_int, d, C.POINTER(d),
                                        C.POINTER(C.c_int), C.POINTER(d), C.POINTER(C.c_int)]
         L.orc_reservoir_mode.argtypes = [C.POINTER(rt.Reservoir), C.c_int, d, C.c_int]
@@ -182,6 +187,46 @@ def trace_box(problem, pos, vel):
     pt, t = np.zeros(3), C.c_double()
     f = lib().orc_trace_box(C.byref(g), _dp(p), _dp(v), _dp(pt), C.byref(t))
     return f, t.value, pt
+
+
+def trace_mesh(problem, pos, vel):
+    """orc_trace_mesh over rows of pos/vel: (points [n,3], t [n], facet [n])."""
+    _, _, g, _, _, _ = problem.views()
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+    n = len(pos)
+    pts, ts, fs = np.zeros((n, 3)), np.zeros(n), np.zeros(n, dtype=np.int32)
+    t = C.c_double()
+    for i in range(n):
+        pt = np.zeros(3)
+        fs[i] = lib().orc_trace_mesh(C.byref(g), _dp(pos[i]), _dp(vel[i]), _dp(pt), C.byref(t))
+        pts[i], ts[i] = pt, t.value
+    return pts, ts, fs
+
+
+def contains_mesh(problem, pos):
+    _, _, g, _, _, _ = problem.views()
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    return np.array([lib().orc_contains_mesh(C.byref(g), _dp(np.ascontiguousarray(p))) for p in pos], dtype=bool)
+
+
+def nearest_facet(problem, pos):
+    _, _, g, _, _, _ = problem.views()
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+    return np.array([lib().orc_nearest_facet(C.byref(g), _dp(np.ascontiguousarray(p))) for p in pos])
+
+
+def sample_volume(problem, key0, key1, ids):
+    """orc_sample_volume for carrier ids: (points [n,3], mode uniforms [n])."""
+    _, _, g, _, _, _ = problem.views()
+    pts, um = np.zeros((len(ids), 3)), np.zeros(len(ids))
+    u = C.c_double()
+    for k, i in enumerate(ids):
+        pt = np.zeros(3)
+        if lib().orc_sample_volume(C.byref(g), key0, key1, int(i), _dp(pt), C.byref(u)):
+            raise RuntimeError("no accepted volume sample")
+        pts[k], um[k] = pt, u.value
+    return pts, um
 
 
 def grid_index(problem, pos):

@@ -177,6 +177,18 @@ class RefDomain:
         lib().ref_domain_grid_index(self.h, n, _dp(pos), _ip(c))
         return c
 
+    def contains(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        out = np.zeros(len(pos), dtype=np.int32)
+        lib().ref_domain_contains(self.h, len(pos), _dp(pos), _ip(out))
+        return out.astype(bool)
+
+    def nearest_facet(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        out = np.zeros(len(pos), dtype=np.int32)
+        lib().ref_domain_nearest_facet(self.h, len(pos), _dp(pos), _ip(out))
+        return out
+
     def sample_surface(self, facet, n, seed):
         pts, u = np.zeros(3 * n), np.zeros(3 * n)
         err = C.create_string_buffer(512)

@@ -270,6 +270,16 @@ void ref_domain_grid_index(void* domain, int n, const double* pos, int* cell) {
     for (int i = 0; i < n; ++i) cell[i] = g.fast_grid_index({pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
 }
 
+void ref_domain_contains(void* domain, int n, const double* pos, int* inside) {
+    const auto& g = *dom(domain).domain;
+    for (int i = 0; i < n; ++i) inside[i] = g.contains_point({pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]}) ? 1 : 0;
+}
+
+void ref_domain_nearest_facet(void* domain, int n, const double* pos, int* facet) {
+    const auto& g = *dom(domain).domain;
+    for (int i = 0; i < n; ++i) facet[i] = g.mesh().nearest_facet({pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]});
+}
+
 int ref_domain_sample_surface(void* domain, int facet, int n, uint64_t seed, double* points,
                               double* uniforms, char* err, int errlen) {
     return guarded(err, errlen, [&] {

@@ -75,6 +75,9 @@ void ref_domain_bounds(void* domain, double* bmin, double* bmax, double* volume)
 void ref_domain_trace(void* domain, int n, const double* pos, const double* vel, double* point,
                       double* t, int* facet);
 void ref_domain_grid_index(void* domain, int n, const double* pos, int* cell);
+/* SimulationDomain::contains_point and SurfaceMesh::nearest_facet per point. */
+void ref_domain_contains(void* domain, int n, const double* pos, int* inside);
+void ref_domain_nearest_facet(void* domain, int n, const double* pos, int* facet);
 /* sample_surface_points on one facet (src/geometry/SurfaceSampling.cpp:20-73) with an
  * mt19937_64 seeded by seed; also returns the three canonical uniforms it consumed per
  * point (face pick, r1, r2) so a restatement can be fed the same values. */

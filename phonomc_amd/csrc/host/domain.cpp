@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <deque>
 #include <filesystem>
 #include <fstream>
 #include <limits>
@@ -116,14 +117,60 @@ int Domain::fast_grid_index(const Vec3& p) const {
     return voxel_to_cell[cell];
 }
 
+namespace {
+// Mesh model (src/SimulationDomain.cpp:129-200): file resolved against the
+// working directory, then the input directory; STL coordinates are nm and
+// become Angstrom; the surface is shifted so its bounding box starts at 0.
+void build_mesh_domain(Domain& d, const SimConfig& c) {
+    namespace fs = std::filesystem;
+    fs::path path(c.model);
+    if (!path.is_absolute()) {
+        const fs::path p1 = fs::current_path() / path, p2 = fs::path(c.input_directory) / path;
+        path = fs::exists(p1) ? p1 : p2;
+    }
+    path = path.lexically_normal();
+    if (!fs::exists(path)) throw std::runtime_error("Model file does not exist: " + path.string());
+    MeshData data = read_mesh_file(path.string());
+    std::string ext = path.extension().string();
+    for (auto& ch : ext) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+    if (ext == ".stl")
+        for (auto& v : data.vertices)
+            for (auto& x : v) x *= 10.0;
+    d.mesh.merge_coplanar = c.merge_coplanar_facets;
+    d.mesh.set(std::move(data.vertices), std::move(data.faces));
+    d.mesh.shift_to_origin();
+    d.is_box = false;
+    d.vertices = d.mesh.vertices;
+    d.faces = d.mesh.faces;
+    d.face_area = d.mesh.face_area;
+    d.face_normal = d.mesh.face_normal;
+    d.bmin = d.mesh.bmin;
+    d.bmax = d.mesh.bmax;
+    d.volume = d.mesh.volume;
+    d.facets.clear();
+    for (const auto& mf : d.mesh.facets) {
+        Facet f;
+        f.normal = mf.normal;
+        f.centroid = mf.centroid;
+        f.area = mf.area;
+        f.faces = mf.faces;
+        d.facets.push_back(f);
+    }
+    d.box_plane_facet = {-1, -1, -1, -1, -1, -1};
+}
+} // namespace
+
 Domain build_domain(const SimConfig& c) {
     Domain d;
-    if (c.model != "box")
-        throw std::runtime_error("Mesh geometry '" + c.model + "' requires the mesh domain builder (not a box).");
-    if (c.sizes.size() != 3 || std::any_of(c.sizes.begin(), c.sizes.end(), [](double s) { return !std::isfinite(s) || s <= 0; }))
-        throw std::runtime_error("Box requires three finite positive sizes.");
-    build_box_mesh(d, {c.sizes[0], c.sizes[1], c.sizes[2]});
-    d.is_box = true;
+    if (c.model != "box") {
+        build_mesh_domain(d, c);
+    } else {
+        if (c.sizes.size() != 3 ||
+            std::any_of(c.sizes.begin(), c.sizes.end(), [](double s) { return !std::isfinite(s) || s <= 0; }))
+            throw std::runtime_error("Box requires three finite positive sizes.");
+        build_box_mesh(d, {c.sizes[0], c.sizes[1], c.sizes[2]});
+        d.is_box = true;
+    }
 
     // Boundary conditions (src/SimulationDomain.cpp:474-515): unselected facets are R, eta = 0.
     if (c.boundary_conditions.size() != c.boundary_boxes.size() || c.boundary_values.size() != c.boundary_boxes.size())
@@ -145,15 +192,43 @@ Domain build_domain(const SimConfig& c) {
             d.facets[f].value = value;
         }
     }
-    // Periodic pairs (src/SimulationDomain.cpp:518-623); one facet per region on a box.
+    // Periodic pairs (src/SimulationDomain.cpp:518-623): one facet per region, or
+    // nearest centroid after the mean shift of the two regions.
     if (c.periodic_pairs.size() % 2 != 0) throw std::runtime_error("periodic_pair must have even number of regions.");
     for (size_t i = 0; i < c.periodic_pairs.size(); i += 2) {
         const auto ma = d.facets_in_region(c.periodic_pairs[i]);
         const auto mb = d.facets_in_region(c.periodic_pairs[i + 1]);
         if (ma.empty() || mb.empty()) throw std::runtime_error("Connected facet mapping failed for periodic_pair regions.");
         if (ma.size() != mb.size()) throw std::runtime_error("periodic_pair region pair must match the same number of facets.");
-        if (ma.size() != 1) throw std::runtime_error("Multi-facet periodic regions require the mesh domain builder.");
-        const int a = ma[0], b = mb[0];
+        std::vector<std::pair<int, int>> pairs;
+        if (ma.size() == 1) {
+            pairs.push_back({ma[0], mb[0]});
+        } else {
+            Vec3 mean_a{0, 0, 0}, mean_b{0, 0, 0};
+            for (int f : ma) mean_a = add(mean_a, d.facets[f].centroid);
+            for (int f : mb) mean_b = add(mean_b, d.facets[f].centroid);
+            mean_a = mul(mean_a, 1.0 / static_cast<double>(ma.size()));
+            mean_b = mul(mean_b, 1.0 / static_cast<double>(mb.size()));
+            const Vec3 guess = sub(mean_b, mean_a);
+            std::vector<int> left = mb;
+            for (int fa : ma) {
+                const Vec3 target = add(d.facets[fa].centroid, guess);
+                double best = std::numeric_limits<double>::infinity();
+                int bj = -1;
+                for (int j = 0; j < static_cast<int>(left.size()); ++j) {
+                    const Vec3 diff = sub(d.facets[left[j]].centroid, target);
+                    const double d2 = dot(diff, diff);
+                    if (d2 < best) {
+                        best = d2;
+                        bj = j;
+                    }
+                }
+                if (bj < 0) throw std::runtime_error("Failed to build one-to-one mapping for periodic_pair facets.");
+                pairs.push_back({fa, left[bj]});
+                left.erase(left.begin() + bj);
+            }
+        }
+        for (const auto& [a, b] : pairs) {
         if (d.facets[a].condition != 'P' || d.facets[b].condition != 'P')
             throw std::runtime_error("Connected facets must both be periodic ('P').");
         if ((d.facets[a].periodic_pair >= 0 && d.facets[a].periodic_pair != b) ||
@@ -163,16 +238,26 @@ Domain build_domain(const SimConfig& c) {
             throw std::runtime_error("Connected facets normals do not match opposite direction.");
         const double aa = d.facets[a].area, ab = d.facets[b].area;
         if (std::abs(aa - ab) / std::max({1.0, aa, ab}) > 1e-3) throw std::runtime_error("Connected facets area mismatch.");
+        if (!d.is_box) {
+            const auto la = d.mesh.boundary_edge_lengths(a), lb = d.mesh.boundary_edge_lengths(b);
+            if (la.size() != lb.size()) throw std::runtime_error("Connected facets boundary edge count mismatch.");
+            for (size_t k = 0; k < la.size(); ++k)
+                if (std::abs(la[k] - lb[k]) / std::max({1.0, la[k], lb[k]}) > 1e-3)
+                    throw std::runtime_error("Connected facets boundary edge-length spectrum mismatch.");
+        }
         d.facets[a].periodic_pair = b;
         d.facets[b].periodic_pair = a;
         d.facets[a].shift = sub(d.facets[b].centroid, d.facets[a].centroid);
         d.facets[b].shift = sub(d.facets[a].centroid, d.facets[b].centroid);
+        }
     }
     for (int f = 0; f < static_cast<int>(d.facets.size()); ++f)
         if (d.facets[f].condition == 'P' && d.facets[f].periodic_pair < 0)
             throw std::runtime_error("Periodic boundary facet " + std::to_string(f) + " is not assigned to a periodic pair.");
 
-    // Grid (src/SimulationDomain.cpp:626-766); every box voxel is a cell.
+    // Grid (src/SimulationDomain.cpp:626-766): voxels whose centre lies in the
+    // solid become cells (every voxel of a box); the others take the cell of the
+    // nearest such voxel by a multi-source six-neighbour breadth-first fill.
     const int nx = c.grid[0], ny = c.grid[1], nz = c.grid[2];
     if (nx <= 0 || ny <= 0 || nz <= 0) throw std::runtime_error("Grid sizes must be positive.");
     d.grid = {nx, ny, nz};
@@ -185,9 +270,31 @@ Domain build_domain(const SimConfig& c) {
                 const Vec3 p{d.bmin[0] + (static_cast<double>(ix) + 0.5) * ext[0] / static_cast<double>(nx),
                              d.bmin[1] + (static_cast<double>(iy) + 0.5) * ext[1] / static_cast<double>(ny),
                              d.bmin[2] + (static_cast<double>(iz) + 0.5) * ext[2] / static_cast<double>(nz)};
-                d.voxel_to_cell[(ix * ny + iy) * nz + iz] = static_cast<int>(d.cell_center.size());
-                d.cell_center.push_back(p);
+                if (d.is_box || d.mesh.contains_point(p)) {
+                    d.voxel_to_cell[(ix * ny + iy) * nz + iz] = static_cast<int>(d.cell_center.size());
+                    d.cell_center.push_back(p);
+                }
             }
+    if (d.cell_center.empty()) throw std::runtime_error("No valid grid centers found inside mesh.");
+    std::deque<int> queue;
+    for (int v = 0; v < static_cast<int>(d.voxel_to_cell.size()); ++v)
+        if (d.voxel_to_cell[v] >= 0) queue.push_back(v);
+    const int yz = ny * nz;
+    while (!queue.empty()) {
+        const int v = queue.front();
+        queue.pop_front();
+        const int ix = v / yz, iy = (v % yz) / nz, iz = v % nz;
+        const int src = d.voxel_to_cell[v];
+        const int nbr[6][3] = {{ix - 1, iy, iz}, {ix + 1, iy, iz}, {ix, iy - 1, iz},
+                               {ix, iy + 1, iz}, {ix, iy, iz - 1}, {ix, iy, iz + 1}};
+        for (const auto& q : nbr) {
+            if (q[0] < 0 || q[0] >= nx || q[1] < 0 || q[1] >= ny || q[2] < 0 || q[2] >= nz) continue;
+            const int w = (q[0] * ny + q[1]) * nz + q[2];
+            if (d.voxel_to_cell[w] >= 0) continue;
+            d.voxel_to_cell[w] = src;
+            queue.push_back(w);
+        }
+    }
     d.cell_volume.assign(d.cell_center.size(), d.volume / static_cast<double>(d.cell_center.size()));
     return d;
 }

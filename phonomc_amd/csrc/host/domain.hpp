@@ -3,6 +3,7 @@
 // src/SurfaceMesh.cpp:232-760).
 #pragma once
 #include "config.hpp"
+#include "mesh.hpp"
 
 #include <array>
 #include <string>
@@ -24,6 +25,7 @@ struct Facet {
 
 struct Domain {
     bool is_box = true;
+    SurfaceMesh mesh;                      // surface of a mesh model (empty for the box)
     Vec3 bmin{}, bmax{};
     double volume = 0;
     std::vector<Vec3> vertices;

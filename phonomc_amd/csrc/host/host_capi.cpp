@@ -100,6 +100,24 @@ int pmcs_problem_heat_source(const pmcs_problem* pr, pmc_heat_source* out) {
     return p.heat.enabled ? 1 : 0;
 }
 
+int pmcs_write_finfet_stl(const char* path, const double substrate_nm[3], const double fin_nm[3],
+                          const double step_nm[3], char* err, int errlen) {
+    try {
+        pmc::FinfetSpec spec;
+        for (int a = 0; a < 3; ++a) {
+            spec.substrate[a] = substrate_nm[a];
+            spec.fin[a] = fin_nm[a];
+            spec.step[a] = step_nm[a];
+        }
+        const auto mesh = pmc::finfet_mesh(spec);
+        pmc::write_binary_stl(path, mesh, "phonomc_amd procedural FinFET");
+        return static_cast<int>(mesh.faces.size());
+    } catch (const std::exception& e) {
+        if (err && errlen > 0) std::snprintf(err, static_cast<size_t>(errlen), "%s", e.what());
+        return -1;
+    }
+}
+
 int pmcs_problem_callaway(const pmcs_problem* pr, pmc_callaway_table* out) {
     if (!pr) return -1;
     const auto& p = *(pr->operator->());

@@ -163,6 +163,62 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     g.voxel_to_cell = d.voxel_to_cell.data();
     g.n_cells = d.cell_count();
     g.cell_volume = d.cell_volume.data();
+    g.mesh = nullptr;
+    if (!d.is_box) {
+        // Mesh model: faces, face and simplex hierarchies, star tetrahedra.
+        const SurfaceMesh& sm = d.mesh;
+        auto& a = p->mesh;
+        const int nf = static_cast<int>(sm.faces.size());
+        std::vector<std::array<Vec3, 2>> boxes(nf);
+        for (const auto& v : sm.vertices) a.vertices.insert(a.vertices.end(), v.begin(), v.end());
+        for (int f = 0; f < nf; ++f) {
+            for (int k = 0; k < 3; ++k) {
+                a.faces.push_back(sm.faces[f][k]);
+                a.face_normal.push_back(sm.face_normal[f][k]);
+            }
+            a.face_k.push_back(sm.face_k[f]);
+            for (int k = 0; k < 3; ++k) a.face_bounds.push_back(sm.face_bounds[f][0][k]);
+            for (int k = 0; k < 3; ++k) a.face_bounds.push_back(sm.face_bounds[f][1][k]);
+            a.face_facet.push_back(sm.face_to_facet[f]);
+            boxes[f] = sm.face_bounds[f];
+        }
+        a.face_bvh = build_bvh(boxes, sm.face_centroid, 4);
+        const auto& star = sm.simplices();
+        std::vector<std::array<Vec3, 2>> sboxes;
+        std::vector<Vec3> scent;
+        for (const auto& s : star) {
+            for (const auto& v : s.v) a.simplex_vertices.insert(a.simplex_vertices.end(), v.begin(), v.end());
+            for (const auto& r : s.inv) a.simplex_inverse.insert(a.simplex_inverse.end(), r.begin(), r.end());
+            for (const auto& b : s.bounds) a.simplex_bounds.insert(a.simplex_bounds.end(), b.begin(), b.end());
+            a.simplex_volume.push_back(s.volume);
+            sboxes.push_back(s.bounds);
+            scent.push_back({0.5 * (s.bounds[0][0] + s.bounds[1][0]), 0.5 * (s.bounds[0][1] + s.bounds[1][1]),
+                             0.5 * (s.bounds[0][2] + s.bounds[1][2])});
+        }
+        a.simplex_bvh = build_bvh(sboxes, scent, 4);
+        auto& mv = p->mesh_view;
+        mv.n_vertices = static_cast<int>(sm.vertices.size());
+        mv.vertices = a.vertices.data();
+        mv.n_faces = nf;
+        mv.faces = a.faces.data();
+        mv.face_normal = a.face_normal.data();
+        mv.face_k = a.face_k.data();
+        mv.face_bounds = a.face_bounds.data();
+        mv.face_facet = a.face_facet.data();
+        mv.n_face_nodes = static_cast<int>(a.face_bvh.nodes.size());
+        mv.face_nodes = a.face_bvh.nodes.data();
+        mv.face_order = a.face_bvh.order.data();
+        mv.n_simplices = static_cast<int>(star.size());
+        mv.simplex_vertices = a.simplex_vertices.data();
+        mv.simplex_inverse = a.simplex_inverse.data();
+        mv.simplex_bounds = a.simplex_bounds.data();
+        mv.simplex_volume = a.simplex_volume.data();
+        mv.n_simplex_nodes = static_cast<int>(a.simplex_bvh.nodes.size());
+        mv.simplex_nodes = a.simplex_bvh.nodes.data();
+        mv.simplex_order = a.simplex_bvh.order.data();
+        g.kind = 1;
+        g.mesh = &p->mesh_view;
+    }
     for (const auto& r : p->rough) {
         pmc_rough_table v{};
         v.specularity = r.specularity.data();

@@ -43,6 +43,14 @@ struct Problem {
     std::vector<double> callaway_wavevector; // 3 per active mode
     double heat_capacity_density = 0;
     pmc_callaway_table callaway_view{};
+    // Mesh model arrays behind pmc_mesh (geometry kind 1).
+    struct MeshArrays {
+        std::vector<double> vertices, face_normal, face_k, face_bounds;
+        std::vector<int32_t> faces, face_facet;
+        Bvh face_bvh, simplex_bvh;
+        std::vector<double> simplex_vertices, simplex_inverse, simplex_bounds, simplex_volume;
+    } mesh;
+    pmc_mesh mesh_view{};
     // C-ABI views (pointers into the vectors above)
     std::vector<pmc_facet> facets;
     std::vector<pmc_rough_table> rough_views;

@@ -40,11 +40,27 @@ class Facet(C.Structure):
                 ("rough_table", C.c_int32), ("reservoir", C.c_int32), ("periodic_pair", C.c_int32)]
 
 
+class BvhNode(C.Structure):
+    _fields_ = [("bmin", C.c_float * 3), ("bmax", C.c_float * 3), ("index", C.c_int32), ("count", C.c_int32)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("n_vertices", C.c_int32), ("vertices", C.POINTER(C.c_double)), ("n_faces", C.c_int32),
+                ("faces", C.POINTER(C.c_int32)), ("face_normal", C.POINTER(C.c_double)),
+                ("face_k", C.POINTER(C.c_double)), ("face_bounds", C.POINTER(C.c_double)),
+                ("face_facet", C.POINTER(C.c_int32)), ("n_face_nodes", C.c_int32),
+                ("face_nodes", C.POINTER(BvhNode)), ("face_order", C.POINTER(C.c_int32)),
+                ("n_simplices", C.c_int32), ("simplex_vertices", C.POINTER(C.c_double)),
+                ("simplex_inverse", C.POINTER(C.c_double)), ("simplex_bounds", C.POINTER(C.c_double)),
+                ("simplex_volume", C.POINTER(C.c_double)), ("n_simplex_nodes", C.c_int32),
+                ("simplex_nodes", C.POINTER(BvhNode)), ("simplex_order", C.POINTER(C.c_int32))]
+
+
 class Geometry(C.Structure):
     _fields_ = [("kind", C.c_int32), ("bmin", C.c_double * 3), ("bmax", C.c_double * 3), ("n_facets", C.c_int32),
                 ("facets", C.POINTER(Facet)), ("box_plane_facet", C.c_int32 * 6), ("grid", C.c_int32 * 3),
                 ("inv_cell", C.c_double * 3), ("voxel_to_cell", C.POINTER(C.c_int32)), ("n_cells", C.c_int32),
-                ("cell_volume", C.POINTER(C.c_double))]
+                ("cell_volume", C.POINTER(C.c_double)), ("mesh", C.POINTER(Mesh))]
 
 
 class RoughTable(C.Structure):
@@ -182,6 +198,8 @@ def lib():
         "pmc_upload_heat_source": (C.c_int, [P, C.POINTER(HeatSource)]),
         "pmcs_problem_heat_source": (C.c_int, [P, C.POINTER(HeatSource)]),
         "pmcs_problem_callaway": (C.c_int, [P, C.POINTER(CallawayTable)]),
+        "pmcs_write_finfet_stl": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double), C.c_char_p, C.c_int]),
         "pmc_upload_callaway": (C.c_int, [P, C.POINTER(CallawayTable)]),
         "pmc_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pmc_destroy": (None, [P]),
@@ -462,6 +480,16 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+def write_finfet_stl(path, substrate=(200.0, 100.0, 50.0), fin=(10.0, 40.0, 100.0), step=(5.0, 10.0, 5.0)):
+    """Procedural FinFET surface as a binary STL (nm); returns the triangle count."""
+    arrs = [(C.c_double * 3)(*v) for v in (substrate, fin, step)]
+    err = C.create_string_buffer(512)
+    n = lib().pmcs_write_finfet_stl(path.encode(), arrs[0], arrs[1], arrs[2], err, 512)
+    if n < 0:
+        raise RuntimeError(err.value.decode())
+    return n
 
 
 def config_text(name, **overrides):
