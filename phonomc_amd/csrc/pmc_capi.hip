@@ -179,9 +179,13 @@ int collect_error(pmc_ctx* ctx) {
                           "inward discrete material mode and finite signed energy weight are required; no "
                           "velocity-only reflection was applied.",
                           i);
+        else if (st.error_facet == -11)
+            std::snprintf(msg, sizeof msg, "sample_volume_points_naive: insufficient accepted samples.");
         else
-            std::snprintf(msg, sizeof msg, "Boundary intersection failed for particle %lld at (%f, %f, %f).", i, x,
-                          y, z);
+            std::snprintf(msg, sizeof msg,
+                          "Boundary intersection failed while advancing particles for particle %lld at (%f, %f, %f). "
+                          "Check that the STL is closed and manifold.",
+                          i, x, y, z);
     }
     std::snprintf(ctx->error.message, sizeof(ctx->error.message), "%s", msg);
     return st.error_code;
@@ -191,6 +195,108 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
     return ms;
+}
+} // namespace
+
+namespace {
+// Mesh upload: faces and star tetrahedra in BVH leaf order, node boxes padded
+// by the mesh tolerance and rounded outward.
+int bvh_depth(const pmc_bvh_node* nodes, int n, int node, int depth) {
+    if (node < 0 || node >= n) return 1 << 20;
+    if (nodes[node].count > 0) return depth;
+    return std::max(bvh_depth(nodes, n, node + 1, depth + 1), bvh_depth(nodes, n, nodes[node].index, depth + 1));
+}
+
+std::vector<pmc_dev::BvhDev> padded_nodes(const pmc_bvh_node* nodes, int n, double pad) {
+    std::vector<pmc_dev::BvhDev> out(n);
+    for (int i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            out[i].bmin[a] = std::nextafter(static_cast<float>(static_cast<double>(nodes[i].bmin[a]) - pad), -INFINITY);
+            out[i].bmax[a] = std::nextafter(static_cast<float>(static_cast<double>(nodes[i].bmax[a]) + pad), INFINITY);
+        }
+        out[i].index = nodes[i].index;
+        out[i].count = nodes[i].count;
+    }
+    return out;
+}
+
+int upload_mesh(pmc_ctx* ctx, const pmc_geometry* g) {
+    View& v = ctx->view;
+    const pmc_mesh& M = *g->mesh;
+    if (M.n_faces <= 0 || M.n_face_nodes <= 0 || !M.face_nodes || !M.face_order)
+        return fail(ctx, PMC_ERR_ARGUMENT, "mesh without faces or face hierarchy");
+    if (bvh_depth(M.face_nodes, M.n_face_nodes, 0, 1) > pmc_dev::kBvhStack ||
+        (M.n_simplices > 0 && bvh_depth(M.simplex_nodes, M.n_simplex_nodes, 0, 1) > pmc_dev::kBvhStack))
+        return fail(ctx, PMC_ERR_ARGUMENT, "mesh hierarchy deeper than the traversal stack");
+    // FP32 positions carry about 2^-24 of the largest coordinate; eight of those
+    // quanta are taken as on the surface.
+    double reach = 0.0;
+    for (int a = 0; a < 3; ++a) reach = std::max({reach, std::abs(g->bmin[a]), std::abs(g->bmax[a])});
+    v.mesh_tol = std::max(1e-7, 8.0 * std::ldexp(reach, -24));
+    const int nf = M.n_faces;
+    std::vector<double4> plane(nf);
+    std::vector<double> tri(9 * static_cast<size_t>(nf)), box(6 * static_cast<size_t>(nf));
+    std::vector<int> facet(nf);
+    for (int j = 0; j < nf; ++j) {
+        const int f = M.face_order[j];
+        if (f < 0 || f >= nf) return fail(ctx, PMC_ERR_ARGUMENT, "face order out of range");
+        plane[j] = make_double4(M.face_normal[3 * f], M.face_normal[3 * f + 1], M.face_normal[3 * f + 2], M.face_k[f]);
+        for (int k = 0; k < 3; ++k) {
+            const int vi = M.faces[3 * f + k];
+            if (vi < 0 || vi >= M.n_vertices) return fail(ctx, PMC_ERR_ARGUMENT, "face vertex out of range");
+            for (int a = 0; a < 3; ++a) tri[9 * j + 3 * k + a] = M.vertices[3 * vi + a];
+        }
+        for (int k = 0; k < 6; ++k) box[6 * j + k] = M.face_bounds[6 * f + k];
+        facet[j] = M.face_facet[f];
+        if (facet[j] < 0 || facet[j] >= g->n_facets) return fail(ctx, PMC_ERR_ARGUMENT, "face without facet");
+    }
+    for (int i = 0; i < M.n_face_nodes; ++i) {
+        const pmc_bvh_node& n = M.face_nodes[i];
+        const bool ok = n.count > 0 ? (n.index >= 0 && n.index + n.count <= nf)
+                                    : (n.count == 0 && i + 1 < M.n_face_nodes && n.index > i && n.index < M.n_face_nodes);
+        if (!ok) return fail(ctx, PMC_ERR_ARGUMENT, "malformed face hierarchy");
+    }
+    v.nfaces = nf;
+    v.face_plane = ctx->upload(plane.data(), plane.size());
+    v.face_tri = ctx->upload(tri.data(), tri.size());
+    v.face_box = ctx->upload(box.data(), box.size());
+    v.face_facet = ctx->upload(facet.data(), facet.size());
+    const auto fn = padded_nodes(M.face_nodes, M.n_face_nodes, v.mesh_tol);
+    v.face_nodes = ctx->upload(fn.data(), fn.size());
+    v.nsimplices = M.n_simplices;
+    if (M.n_simplices > 0) {
+        const int ns = M.n_simplices;
+        std::vector<double> sv(12 * static_cast<size_t>(ns)), si(9 * static_cast<size_t>(ns)),
+            sb(6 * static_cast<size_t>(ns)), cdf(ns);
+        std::vector<int> pos(ns, -1);
+        for (int j = 0; j < ns; ++j) {
+            const int t = M.simplex_order[j];
+            if (t < 0 || t >= ns || pos[t] >= 0) return fail(ctx, PMC_ERR_ARGUMENT, "simplex order is not a permutation");
+            pos[t] = j;
+            for (int k = 0; k < 12; ++k) sv[12 * j + k] = M.simplex_vertices[12 * t + k];
+            for (int k = 0; k < 9; ++k) si[9 * j + k] = M.simplex_inverse[9 * t + k];
+            for (int k = 0; k < 6; ++k) sb[6 * j + k] = M.simplex_bounds[6 * t + k];
+        }
+        double acc = 0.0;
+        for (int t = 0; t < ns; ++t) cdf[t] = (acc += M.simplex_volume[t]);
+        for (int i = 0; i < M.n_simplex_nodes; ++i) {
+            const pmc_bvh_node& n = M.simplex_nodes[i];
+            const bool ok = n.count > 0 ? (n.index >= 0 && n.index + n.count <= ns)
+                                        : (n.count == 0 && i + 1 < M.n_simplex_nodes && n.index > i &&
+                                           n.index < M.n_simplex_nodes);
+            if (!ok) return fail(ctx, PMC_ERR_ARGUMENT, "malformed simplex hierarchy");
+        }
+        v.simplex_v = ctx->upload(sv.data(), sv.size());
+        v.simplex_inv = ctx->upload(si.data(), si.size());
+        v.simplex_box = ctx->upload(sb.data(), sb.size());
+        v.simplex_cdf = ctx->upload(cdf.data(), cdf.size());
+        v.simplex_pos = ctx->upload(pos.data(), pos.size());
+        // Tetrahedron boxes are tested with the 1e-10 barycentric tolerance only;
+        // the padding covers it.
+        const auto sn = padded_nodes(M.simplex_nodes, M.n_simplex_nodes, 1e-6);
+        v.simplex_nodes = ctx->upload(sn.data(), sn.size());
+    }
+    return PMC_OK;
 }
 } // namespace
 
@@ -322,8 +428,10 @@ int pmc_upload_material(pmc_ctx* ctx, const pmc_mode_table* t) {
 int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
     if (!ctx || !g || g->n_facets <= 0 || g->n_facets > pmc_dev::kMaxFacetsDev || g->n_cells <= 0)
         return fail(ctx, PMC_ERR_ARGUMENT, "invalid geometry");
-    if (g->kind != 0) return fail(ctx, PMC_ERR_UNSUPPORTED, "only the analytic box geometry is supported");
+    if (g->kind != 0 && !(g->kind == 1 && g->mesh))
+        return fail(ctx, PMC_ERR_UNSUPPORTED, "geometry must be the analytic box or a triangle mesh");
     View& v = ctx->view;
+    v.geom_kind = g->kind;
     for (int a = 0; a < 3; ++a) {
         v.bmin[a] = g->bmin[a];
         v.bmax[a] = g->bmax[a];
@@ -333,6 +441,10 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
         v.inv_cell[a] = g->inv_cell[a];
     }
     for (int k = 0; k < 6; ++k) {
+        if (g->kind == 1) {
+            v.plane_facet[k] = -1;
+            continue;
+        }
         if (g->box_plane_facet[k] < 0 || g->box_plane_facet[k] >= g->n_facets)
             return fail(ctx, PMC_ERR_ARGUMENT, "box plane without facet");
         v.plane_facet[k] = g->box_plane_facet[k];
@@ -347,6 +459,27 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
         f[i].rough = g->facets[i].rough_table;
         f[i].res = g->facets[i].reservoir;
         f[i].pair = g->facets[i].periodic_pair;
+        // Carriers keep a table mode plus per-axis velocity signs, so a mirror
+        // reflection needs an axis-aligned facet.
+        f[i].axis = -1;
+        for (int a = 0; a < 3; ++a) {
+            const double o1 = f[i].n[(a + 1) % 3], o2 = f[i].n[(a + 2) % 3];
+            if (std::abs(std::abs(f[i].n[a]) - 1.0) <= 1e-9 && std::abs(o1) <= 1e-9 && std::abs(o2) <= 1e-9)
+                f[i].axis = a;
+        }
+        const bool mirrors = f[i].cond == 'R' || (f[i].cond == 'P' && f[i].pair < 0);
+        if (mirrors && f[i].axis < 0) {
+            char msg[160];
+            std::snprintf(msg, sizeof msg,
+                          "facet %d reflects carriers but is not axis-aligned; the device takes mirror "
+                          "reflections on axis-aligned facets only",
+                          i);
+            return fail(ctx, PMC_ERR_UNSUPPORTED, msg);
+        }
+    }
+    if (g->kind == 1) {
+        const int rc = upload_mesh(ctx, g);
+        if (rc) return rc;
     }
     v.nfacets = g->n_facets;
     v.facets = ctx->upload(f.data(), f.size());

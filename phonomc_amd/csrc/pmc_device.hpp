@@ -87,6 +87,14 @@ struct FacetDev {
     double n[3];
     double shift[3];
     int cond, rough, res, pair;
+    int axis; // axis of the normal of an axis-aligned facet, else -1 (mirror reflections need one)
+};
+
+// Bounding-volume hierarchy node (pmc_bvh_node layout): boxes padded by the
+// mesh tolerance at upload; left child next, right child at `index`.
+struct BvhDev {
+    float bmin[3], bmax[3];
+    int index, count;
 };
 
 struct RoughDev {
@@ -191,6 +199,22 @@ struct View {
     const int* voxel;
     int ncells;
     const double* cell_vol;
+    // triangle mesh (geom_kind 1); faces and star tetrahedra in BVH leaf order
+    int geom_kind;
+    double mesh_tol;          // FP32 positions within this distance of the surface are on it (A)
+    int nfaces;
+    const double4* face_plane; // outward normal, k: n.x + k = 0
+    const double* face_tri;    // 9 per face
+    const double* face_box;    // 6 per face: min, max
+    const int* face_facet;
+    const BvhDev* face_nodes;
+    int nsimplices;
+    const double* simplex_v;   // 12 per tetrahedron: apex, face vertices
+    const double* simplex_inv; // 9 per tetrahedron
+    const double* simplex_box; // 6 per tetrahedron
+    const double* simplex_cdf; // cumulative volume (A^3), original order
+    const int* simplex_pos;    // original index -> BVH order
+    const BvhDev* simplex_nodes;
     // boundary tables
     const RoughDev* rough;
     const ResDev* res;
@@ -257,5 +281,8 @@ struct View {
 // Linearized Callaway kernel: one workgroup of kCwThreads per cell, up to
 // kCwPer carriers per thread held in registers.
 constexpr int kCwThreads = 512, kCwWaves = kCwThreads / 64, kCwPer = 16;
+
+// Deepest BVH the traversal stacks take (hierarchies are checked at upload).
+constexpr int kBvhStack = 48;
 
 } // namespace pmc_dev

@@ -91,6 +91,281 @@ __device__ __forceinline__ float4 carrier_velocity(const View& V, int mw) {
     return v;
 }
 
+// ---------------------------------------------------------------- triangle mesh
+// Squared distance from p to the triangle T (9 doubles): the Voronoi-region
+// closest point of Ericson, Real-Time Collision Detection 5.1.5, the method of
+// src/SurfaceMesh.cpp:111-159.
+__device__ double tri_dist2(const double p[3], const double* T) {
+    const double a[3] = {T[0], T[1], T[2]};
+    double ab[3], ac[3], ap[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ab[k] = T[3 + k] - a[k];
+        ac[k] = T[6 + k] - a[k];
+        ap[k] = p[k] - a[k];
+    }
+    auto d3 = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    const double d1 = d3(ab, ap), d2 = d3(ac, ap);
+    double v = 0.0, w = 0.0; // closest point a + v ab + w ac
+    const double bp[3] = {p[0] - T[3], p[1] - T[4], p[2] - T[5]};
+    const double cp[3] = {p[0] - T[6], p[1] - T[7], p[2] - T[8]};
+    const double d3b = d3(ab, bp), d4 = d3(ac, bp), d5 = d3(ab, cp), d6 = d3(ac, cp);
+    const double vc = d1 * d4 - d3b * d2, vb = d5 * d2 - d1 * d6, va = d3b * d6 - d5 * d4;
+    if (d1 <= 0.0 && d2 <= 0.0) {
+    } else if (d3b >= 0.0 && d4 <= d3b) {
+        v = 1.0;
+    } else if (vc <= 0.0 && d1 >= 0.0 && d3b <= 0.0) {
+        v = d1 / (d1 - d3b);
+    } else if (d6 >= 0.0 && d5 <= d6) {
+        w = 1.0;
+    } else if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        w = d2 / (d2 - d6);
+    } else if (va <= 0.0 && (d4 - d3b) >= 0.0 && (d5 - d6) >= 0.0) {
+        w = (d4 - d3b) / ((d4 - d3b) + (d5 - d6));
+        v = 1.0 - w;
+    } else {
+        const double den = 1.0 / (va + vb + vc);
+        v = vb * den;
+        w = vc * den;
+    }
+    double r2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double d = ap[k] - v * ab[k] - w * ac[k];
+        r2 += d * d;
+    }
+    return r2;
+}
+
+// Slab test of the ray p + v t, t in [0, tmax], against a node box.
+__device__ __forceinline__ bool ray_box(const BvhDev& N, const double p[3], const double v[3], const double inv[3],
+                                        double tmax) {
+    double t0 = 0.0, t1 = tmax;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double lo = N.bmin[a], hi = N.bmax[a];
+        if (v[a] == 0.0) {
+            if (p[a] < lo || p[a] > hi) return false;
+            continue;
+        }
+        double ta = (lo - p[a]) * inv[a], tb = (hi - p[a]) * inv[a];
+        if (ta > tb) {
+            const double x = ta;
+            ta = tb;
+            tb = x;
+        }
+        t0 = fmax(t0, ta);
+        t1 = fmin(t1, tb);
+    }
+    return t0 <= t1;
+}
+
+__device__ __forceinline__ bool point_box(const BvhDev& N, const double p[3]) {
+    return p[0] >= N.bmin[0] && p[0] <= N.bmax[0] && p[1] >= N.bmin[1] && p[1] <= N.bmax[1] && p[2] >= N.bmin[2] &&
+           p[2] <= N.bmax[2];
+}
+
+// Nearest face the carrier leaves through along p + v t, t >= 0: the
+// restatement of SurfaceMesh::trace_boundary_intersection
+// (src/SurfaceMesh.cpp:1164-1270) for FP32 positions.  Only faces with
+// v.n > 1e-10 count (from inside a closed surface the first crossing is always
+// one of them); a point up to mesh_tol beyond such a face's plane is on it
+// (t = 0); a crossing counts when it lies within mesh_tol of the triangle; an
+// exact tie goes to the lower face.  Returns the face (BVH order) or -1.
+__device__ int mesh_trace(const View& V, const double p[3], const double v[3], double& t_hit) {
+    const double inv[3] = {1.0 / v[0], 1.0 / v[1], 1.0 / v[2]};
+    const double tol = V.mesh_tol, tol2 = tol * tol;
+    int stack[kBvhStack];
+    int sp = 0, node = 0, bf = -1;
+    double best = INFINITY;
+    while (true) {
+        const BvhDev N = V.face_nodes[node];
+        if (ray_box(N, p, v, inv, best)) {
+            if (N.count == 0) {
+                if (sp < kBvhStack) stack[sp++] = N.index; // depth is checked at upload
+                node = node + 1;
+                continue;
+            }
+            for (int f = N.index; f < N.index + N.count; ++f) {
+                const double4 P = V.face_plane[f];
+                const double den = v[0] * P.x + v[1] * P.y + v[2] * P.z;
+                if (!(den > 1e-10)) continue;
+                const double s = p[0] * P.x + p[1] * P.y + p[2] * P.z + P.w;
+                double t = -s / den;
+                if (t < 0.0) {
+                    if (s > tol) continue;
+                    t = 0.0;
+                }
+                if (t > best || (t == best && f > bf)) continue;
+                const double q[3] = {p[0] + v[0] * t, p[1] + v[1] * t, p[2] + v[2] * t};
+                const double* B = V.face_box + 6 * f;
+                if (q[0] < B[0] - tol || q[1] < B[1] - tol || q[2] < B[2] - tol || q[0] > B[3] + tol ||
+                    q[1] > B[4] + tol || q[2] > B[5] + tol)
+                    continue;
+                if (tri_dist2(q, V.face_tri + 9 * f) > tol2) continue;
+                best = t;
+                bf = f;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    t_hit = best;
+    return bf;
+}
+
+// Whether mesh_trace would find a face within t <= tmax: the same face tests,
+// returning at the first one found.
+__device__ bool mesh_reaches(const View& V, const double p[3], const double v[3], double tmax) {
+    const double inv[3] = {1.0 / v[0], 1.0 / v[1], 1.0 / v[2]};
+    const double tol = V.mesh_tol, tol2 = tol * tol;
+    int stack[kBvhStack];
+    int sp = 0, node = 0;
+    while (true) {
+        const BvhDev N = V.face_nodes[node];
+        if (ray_box(N, p, v, inv, tmax)) {
+            if (N.count == 0) {
+                if (sp < kBvhStack) stack[sp++] = N.index;
+                node = node + 1;
+                continue;
+            }
+            for (int f = N.index; f < N.index + N.count; ++f) {
+                const double4 P = V.face_plane[f];
+                const double den = v[0] * P.x + v[1] * P.y + v[2] * P.z;
+                if (!(den > 1e-10)) continue;
+                const double s = p[0] * P.x + p[1] * P.y + p[2] * P.z + P.w;
+                double t = -s / den;
+                if (t < 0.0) {
+                    if (s > tol) continue;
+                    t = 0.0;
+                }
+                if (t > tmax) continue;
+                const double q[3] = {p[0] + v[0] * t, p[1] + v[1] * t, p[2] + v[2] * t};
+                const double* B = V.face_box + 6 * f;
+                if (q[0] < B[0] - tol || q[1] < B[1] - tol || q[2] < B[2] - tol || q[0] > B[3] + tol ||
+                    q[1] > B[4] + tol || q[2] > B[5] + tol)
+                    continue;
+                if (tri_dist2(q, V.face_tri + 9 * f) <= tol2) return true;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    return false;
+}
+
+// Odd number of distinct +x crossings (SurfaceMesh::contains_point_ray_cast,
+// src/SurfaceMesh.cpp:1100-1123: Moller-Trumbore with 1e-10 tolerances,
+// crossings closer than 1e-9 counted once).
+__device__ bool mesh_ray_cast(const View& V, const double p[3]) {
+    constexpr int kMaxHits = 32;
+    double hits[kMaxHits];
+    int nh = 0;
+    const double v[3] = {1.0, 0.0, 0.0}, inv[3] = {1.0, INFINITY, INFINITY};
+    int stack[kBvhStack];
+    int sp = 0, node = 0;
+    while (true) {
+        const BvhDev N = V.face_nodes[node];
+        if (ray_box(N, p, v, inv, INFINITY)) {
+            if (N.count == 0) {
+                if (sp < kBvhStack) stack[sp++] = N.index;
+                node = node + 1;
+                continue;
+            }
+            for (int f = N.index; f < N.index + N.count; ++f) {
+                const double* T = V.face_tri + 9 * f;
+                const double e1[3] = {T[3] - T[0], T[4] - T[1], T[5] - T[2]};
+                const double e2[3] = {T[6] - T[0], T[7] - T[1], T[8] - T[2]};
+                const double pv[3] = {0.0, -e2[2], e2[1]}; // d x e2 with d = +x
+                const double det = e1[1] * pv[1] + e1[2] * pv[2];
+                if (fabs(det) < 1e-10) continue;
+                const double id = 1.0 / det;
+                const double sv[3] = {p[0] - T[0], p[1] - T[1], p[2] - T[2]};
+                const double u = (sv[1] * pv[1] + sv[2] * pv[2]) * id;
+                if (u < -1e-10 || u > 1.0 + 1e-10) continue;
+                const double qv[3] = {sv[1] * e1[2] - sv[2] * e1[1], sv[2] * e1[0] - sv[0] * e1[2],
+                                      sv[0] * e1[1] - sv[1] * e1[0]};
+                const double w = qv[0] * id;
+                if (w < -1e-10 || u + w > 1.0 + 1e-10) continue;
+                const double t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * id;
+                if (!(t > 1e-10)) continue;
+                if (nh == kMaxHits) return true; // a degenerate stack of crossings: not an escape
+                int k = nh++;
+                while (k > 0 && hits[k - 1] > t) {
+                    hits[k] = hits[k - 1];
+                    --k;
+                }
+                hits[k] = t;
+            }
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    if (nh == 0) return false;
+    int unique = 1;
+    for (int k = 1; k < nh; ++k)
+        if (fabs(hits[k] - hits[k - 1]) > 1e-9) ++unique;
+    return unique & 1;
+}
+
+// Membership of an FP32 position: inside a star tetrahedron (1e-10 barycentric
+// tolerance, src/SurfaceMesh.cpp:1126-1151), else the ray cast, else within
+// mesh_tol of a face (a carrier that FP32 rounding put just outside the surface
+// it sits on).
+__device__ bool mesh_contains(const View& V, const double p[3]) {
+    int stack[kBvhStack];
+    if (V.nsimplices > 0) {
+        int sp = 0, node = 0;
+        const double tol = 1e-10;
+        while (true) {
+            const BvhDev N = V.simplex_nodes[node];
+            if (point_box(N, p)) {
+                if (N.count == 0) {
+                    if (sp < kBvhStack) stack[sp++] = N.index;
+                    node = node + 1;
+                    continue;
+                }
+                for (int j = N.index; j < N.index + N.count; ++j) {
+                    const double* B = V.simplex_box + 6 * j;
+                    if (p[0] < B[0] - tol || p[1] < B[1] - tol || p[2] < B[2] - tol || p[0] > B[3] + tol ||
+                        p[1] > B[4] + tol || p[2] > B[5] + tol)
+                        continue;
+                    const double* o = V.simplex_v + 12 * j;
+                    const double* A = V.simplex_inv + 9 * j;
+                    const double q[3] = {p[0] - o[0], p[1] - o[1], p[2] - o[2]};
+                    const double l1 = A[0] * q[0] + A[1] * q[1] + A[2] * q[2];
+                    const double l2 = A[3] * q[0] + A[4] * q[1] + A[5] * q[2];
+                    const double l3 = A[6] * q[0] + A[7] * q[1] + A[8] * q[2];
+                    const double l0 = 1.0 - l1 - l2 - l3;
+                    if (l0 >= -tol && l1 >= -tol && l2 >= -tol && l3 >= -tol && l0 <= 1.0 + tol && l1 <= 1.0 + tol &&
+                        l2 <= 1.0 + tol && l3 <= 1.0 + tol)
+                        return true;
+                }
+            }
+            if (sp == 0) break;
+            node = stack[--sp];
+        }
+    }
+    if (mesh_ray_cast(V, p)) return true;
+    const double tol2 = V.mesh_tol * V.mesh_tol;
+    int sp = 0, node = 0;
+    while (true) {
+        const BvhDev N = V.face_nodes[node];
+        if (point_box(N, p)) {
+            if (N.count == 0) {
+                if (sp < kBvhStack) stack[sp++] = N.index;
+                node = node + 1;
+                continue;
+            }
+            for (int f = N.index; f < N.index + N.count; ++f)
+                if (tri_dist2(p, V.face_tri + 9 * f) <= tol2) return true;
+        }
+        if (sp == 0) break;
+        node = stack[--sp];
+    }
+    return false;
+}
+
 __device__ __forceinline__ double relax_weight(const View& V, const CellRta& cr, int m) {
     // PhononMaterial::mode_relaxation_weight (src/material/Thermodynamics.cpp:95-106)
     // with mode_gamma's interpolation (:108-133); row and weight are per cell.
@@ -285,11 +560,13 @@ struct Tally {
 // Free flight for `remaining` ps with up to 64 wall events
 // (TransportOperator::advance_particle, src/solver/TransportOperator.cpp:405-466,
 // against the analytic box of SimulationDomain::trace_boundary_intersection,
-// src/SimulationDomain.cpp:342-450, and interact_boundary,
-// src/solver/BoundaryInteraction.cpp:53-162).  FP32 positions: a carrier on a
-// wall always moves away from it (planes are chosen by velocity sign), so no
-// push distance is needed; positions are clamped into the closed box.
+// src/SimulationDomain.cpp:342-450, or the triangle mesh (mesh_trace), and
+// interact_boundary, src/solver/BoundaryInteraction.cpp:53-162).  FP32
+// positions: a carrier on a wall always moves away from it (box planes are
+// chosen by velocity sign, mesh faces must be left through), so no push
+// distance is needed; positions are clamped into the bounding box.
 // Returns 0 alive, 1 absorbed, or a negative error.
+template <bool kMesh>
 __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned uid, float remaining, Tally& T,
                        bool& mode_changed, bool& occ_changed) {
     const uint32_t step = static_cast<uint32_t>(V.st->step);
@@ -298,38 +575,64 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
         if (!(remaining > 1e-14f)) return 0;
         const float4 v = carrier_velocity(V, mw);
         const float vv[3] = {v.x, v.y, v.z};
-        float tbest = INFINITY;
-        int plane = -1;
+        int facet, ha;
+        if constexpr (kMesh) {
+            // Triangle mesh: FP64 trace from the FP32 position.
+            const double* vt = V.veld + 3 * static_cast<size_t>(mw & kModeMask);
+            const double vd[3] = {(mw & (1 << kFlipShift)) ? -vt[0] : vt[0], (mw & (2 << kFlipShift)) ? -vt[1] : vt[1],
+                                  (mw & (4 << kFlipShift)) ? -vt[2] : vt[2]};
+            if (vd[0] == 0.0 && vd[1] == 0.0 && vd[2] == 0.0) return 0; // stationary mode
+            const double pd[3] = {p[0], p[1], p[2]};
+            double th;
+            const int f = mesh_trace(V, pd, vd, th);
+            if (f < 0) return -PMC_ERR_BOUNDARY;
+            if (th > static_cast<double>(remaining)) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            if (vv[a] != 0.0f) {
-                const int side = vv[a] > 0.0f ? 1 : 0;
-                const float target = side ? V.fbmax[a] : V.fbmin[a];
-                float t = (target - p[a]) / vv[a];
-                t = t > 0.0f ? t : 0.0f;
-                if (t < tbest) {
-                    tbest = t;
-                    plane = 2 * a + side;
+                for (int a = 0; a < 3; ++a) {
+                    float x = fmaf(vv[a], remaining, p[a]);
+                    p[a] = fminf(fmaxf(x, V.fbmin[a]), V.fbmax[a]);
                 }
+                return 0;
             }
-        }
-        if (plane < 0 || tbest > remaining) {
+            remaining -= static_cast<float>(th);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) p[a] = static_cast<float>(fma(vd[a], th, pd[a]));
+            facet = V.face_facet[f];
+            ha = V.facets[facet].axis;
+        } else {
+            float tbest = INFINITY;
+            int plane = -1;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                float x = fmaf(vv[a], remaining, p[a]);
+                if (vv[a] != 0.0f) {
+                    const int side = vv[a] > 0.0f ? 1 : 0;
+                    const float target = side ? V.fbmax[a] : V.fbmin[a];
+                    float t = (target - p[a]) / vv[a];
+                    t = t > 0.0f ? t : 0.0f;
+                    if (t < tbest) {
+                        tbest = t;
+                        plane = 2 * a + side;
+                    }
+                }
+            }
+            if (plane < 0 || tbest > remaining) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    float x = fmaf(vv[a], remaining, p[a]);
+                    p[a] = fminf(fmaxf(x, V.fbmin[a]), V.fbmax[a]);
+                }
+                return 0;
+            }
+            remaining -= tbest;
+            ha = plane >> 1;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float x = fmaf(vv[a], tbest, p[a]);
                 p[a] = fminf(fmaxf(x, V.fbmin[a]), V.fbmax[a]);
             }
-            return 0;
+            p[ha] = (plane & 1) ? V.fbmax[ha] : V.fbmin[ha];
+            facet = V.plane_facet[plane];
         }
-        remaining -= tbest;
-        const int ha = plane >> 1;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            float x = fmaf(vv[a], tbest, p[a]);
-            p[a] = fminf(fmaxf(x, V.fbmin[a]), V.fbmax[a]);
-        }
-        p[ha] = (plane & 1) ? V.fbmax[ha] : V.fbmin[ha];
-        const int facet = V.plane_facet[plane];
         const FacetDev& F = V.facets[facet];
         const int m = mw & kModeMask;
         if (F.cond == 'T') {
@@ -585,6 +888,7 @@ __global__ void k_begin(View V) {
 // iteration of advance() when it ends without an event, bit for bit.  Carriers
 // that do reach a wall are listed per block for k_events and left untouched.
 // Counts carriers per new cell for the cell sort.
+template <bool kMesh>
 __global__ void __launch_bounds__(256) k_transport(View V) {
     __shared__ int ev_n;
     if (threadIdx.x == 0) ev_n = 0;
@@ -626,24 +930,34 @@ __global__ void __launch_bounds__(256) k_transport(View V) {
                 if (moves) {
                     const float4 v = carrier_velocity(V, mw[j]);
                     const float vv[3] = {v.x, v.y, v.z};
-                    float tbest = INFINITY;
-                    int plane = -1;
+                    if constexpr (kMesh) {
+                        const double* vt = V.veld + 3 * static_cast<size_t>(mw[j] & kModeMask);
+                        const int w = mw[j];
+                        const double vd[3] = {(w & (1 << kFlipShift)) ? -vt[0] : vt[0],
+                                              (w & (2 << kFlipShift)) ? -vt[1] : vt[1],
+                                              (w & (4 << kFlipShift)) ? -vt[2] : vt[2]};
+                        const double pd[3] = {p[0], p[1], p[2]};
+                        event = !(vd[0] == 0.0 && vd[1] == 0.0 && vd[2] == 0.0) &&
+                                mesh_reaches(V, pd, vd, static_cast<double>(dt));
+                    } else {
+                        float tbest = INFINITY;
+                        int plane = -1;
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        if (vv[a] != 0.0f) {
-                            const int side = vv[a] > 0.0f ? 1 : 0;
-                            const float target = side ? V.fbmax[a] : V.fbmin[a];
-                            float t = (target - p[a]) / vv[a];
-                            t = t > 0.0f ? t : 0.0f;
-                            if (t < tbest) {
-                                tbest = t;
-                                plane = 2 * a + side;
+                        for (int a = 0; a < 3; ++a) {
+                            if (vv[a] != 0.0f) {
+                                const int side = vv[a] > 0.0f ? 1 : 0;
+                                const float target = side ? V.fbmax[a] : V.fbmin[a];
+                                float t = (target - p[a]) / vv[a];
+                                t = t > 0.0f ? t : 0.0f;
+                                if (t < tbest) {
+                                    tbest = t;
+                                    plane = 2 * a + side;
+                                }
                             }
                         }
+                        event = plane >= 0 && !(tbest > dt);
                     }
-                    if (plane >= 0 && !(tbest > dt)) {
-                        event = true;
-                    } else {
+                    if (!event) {
 #pragma unroll
                         for (int a = 0; a < 3; ++a) {
                             const float q = fmaf(vv[a], dt, p[a]);
@@ -677,6 +991,7 @@ __global__ void __launch_bounds__(256) k_transport(View V) {
 // Carriers with wall events this step: the whole event loop of
 // TransportOperator::advance_particle (src/solver/TransportOperator.cpp:405-466)
 // for the slots k_transport listed.  One block per transport block.
+template <bool kMesh>
 __global__ void __launch_bounds__(256) k_events(View V) {
     __shared__ double sp[kPartialCols];
     BlockPartials P{sp};
@@ -697,7 +1012,7 @@ __global__ void __launch_bounds__(256) k_events(View V) {
             float p[3] = {V.px[i], V.py[i], V.pz[i]};
             double occ = V.occ[i];
             bool mc = false, oc = false;
-            const int rc = advance(V, p, mw, occ, V.uid[i], V.dtf, T, mc, oc);
+            const int rc = advance<kMesh>(V, p, mw, occ, V.uid[i], V.dtf, T, mc, oc);
             if (rc < 0) {
                 raise_advance_error(V, rc, i, p);
                 V.mode[i] = -1; // keeps the cell counts equal to the live slots
@@ -725,6 +1040,7 @@ __global__ void __launch_bounds__(256) k_events(View V) {
 // Reservoir injection (ReservoirInjector::inject, src/solver/ReservoirInjector.cpp:62-132)
 // followed by the remaining flight of each injected carrier
 // (TransportOperator::advance_injected, src/solver/TransportOperator.cpp:134-164).
+template <bool kMesh>
 __global__ void __launch_bounds__(256) k_inject(View V, int row0) {
     __shared__ double sp[kPartialCols];
     BlockPartials P{sp};
@@ -769,8 +1085,12 @@ __global__ void __launch_bounds__(256) k_inject(View V, int row0) {
                 m = (x - j) < R.aprob[j] ? j : R.aidx[j];
             }
             const double t_emit = V.dt * u_time;
-            int f = 0; // std::discrete_distribution: lower_bound on the area CDF
-            while (f < R.nfaces - 1 && R.face_cdf[f] < u_face) ++f;
+            int f = 0, fh = R.nfaces - 1; // std::discrete_distribution: lower_bound on the area CDF
+            while (f < fh) {
+                const int mid = (f + fh) >> 1;
+                if (R.face_cdf[mid] < u_face) f = mid + 1;
+                else fh = mid;
+            }
             const double* tri = R.tri + 9 * f;
             const double sr1 = sqrt(r1);
             const double b0 = 1.0 - sr1, b1 = sr1 * (1.0 - r2), b2 = sr1 * r2;
@@ -799,7 +1119,7 @@ __global__ void __launch_bounds__(256) k_inject(View V, int row0) {
             const double remain = V.dt - t_emit;
             int rc = 0;
             bool mc = false, oc = false;
-            if (remain > 1e-14) rc = advance(V, p, mw, occ, uid, static_cast<float>(remain), T, mc, oc);
+            if (remain > 1e-14) rc = advance<kMesh>(V, p, mw, occ, uid, static_cast<float>(remain), T, mc, oc);
             V.uid[slot] = uid;
             if (rc < 0) {
                 raise_advance_error(V, rc, slot, p);
@@ -2112,12 +2432,58 @@ __global__ void __launch_bounds__(256) k_containment(View V) {
             raise_error(V, PMC_ERR_ESCAPED, i, p, -1);
             return;
         }
+    if (V.geom_kind == 1) {
+        const double pd[3] = {p[0], p[1], p[2]};
+        if (!mesh_contains(V, pd)) raise_error(V, PMC_ERR_ESCAPED, i, p, -1);
+    }
 }
 
 // ---------------------------------------------------------------- initialization
 // Initial population (src/solver/Initialization.cpp:48-113 and :181-265):
 // uniform positions in the box, uniform active modes, Bose occupations at the
 // initial temperature of the carrier's position.
+// Uniform point in a mesh (SurfaceMesh::sample_volume_points,
+// src/geometry/SurfaceSampling.cpp:113-155): a star tetrahedron by volume, then
+// Dirichlet(1,1,1,1) weights from normalized exponentials, on the carrier's
+// Philox words a, b and a third one.  Without a star decomposition, rejection in
+// the bounding box against the ray cast (:81-106), up to 256 trials.
+__device__ bool mesh_volume_point(const View& V, uint32_t id, const uint4& a, const uint4& b, double p[3]) {
+    if (V.nsimplices > 0) {
+        const uint4 c = philox(id, 0, 2, kStreamInit, V.key0, V.key1);
+        const double target = u53(a.x, a.y) * V.simplex_cdf[V.nsimplices - 1];
+        int lo = 0, hi = V.nsimplices - 1; // first simplex whose cumulative volume exceeds target
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (V.simplex_cdf[mid] > target) hi = mid;
+            else lo = mid + 1;
+        }
+        const double u[4] = {u53(a.z, a.w), u53(b.x, b.y), u53(c.x, c.y), u53(c.z, c.w)};
+        double w[4], sum = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            w[k] = -log(fmax(u[k], 1e-15));
+            sum += w[k];
+        }
+        const double* v = V.simplex_v + 12 * V.simplex_pos[lo];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            double x = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) x += (w[k] / sum) * v[3 * k + d];
+            p[d] = x;
+        }
+        return true;
+    }
+    for (uint32_t trial = 0; trial < 256; ++trial) {
+        const uint4 ta = philox(id, 1 + trial, 0, kStreamInit, V.key0, V.key1);
+        const uint4 tb = philox(id, 1 + trial, 1, kStreamInit, V.key0, V.key1);
+        const double u[3] = {u53(ta.x, ta.y), u53(ta.z, ta.w), u53(tb.x, tb.y)};
+        for (int d = 0; d < 3; ++d) p[d] = V.bmin[d] + (V.bmax[d] - V.bmin[d]) * u[d];
+        if (mesh_ray_cast(V, p)) return true;
+    }
+    return false;
+}
+
 __global__ void __launch_bounds__(256) k_init(View V, int count, int tmode, double t0, double c0, double c1, double c2,
                                               double h0, double h1, double h2, double tcold, double thot,
                                               const double* field) {
@@ -2128,6 +2494,13 @@ __global__ void __launch_bounds__(256) k_init(View V, int count, int tmode, doub
     const double u[4] = {u53(a.x, a.y), u53(a.z, a.w), u53(b.x, b.y), u53(b.z, b.w)};
     double p[3];
     for (int k = 0; k < 3; ++k) p[k] = V.bmin[k] + (V.bmax[k] - V.bmin[k]) * u[k];
+    if (V.geom_kind == 1 && !mesh_volume_point(V, static_cast<uint32_t>(i), a, b, p)) {
+        raise_error(V, PMC_ERR_BOUNDARY, i, nullptr, -11);
+        V.mode[i] = -1;
+        V.uid[i] = static_cast<unsigned>(i);
+        V.cell[i] = 0;
+        return;
+    }
     float fp[3];
     for (int k = 0; k < 3; ++k) {
         fp[k] = static_cast<float>(p[k]);
@@ -2192,13 +2565,17 @@ static const dim3 kWaveBlock(64 * kWavesPerBlock);
 
 void begin(const View& v, hipStream_t s) { hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, s, v); }
 void transport(const View& v, hipStream_t s) {
-    hipLaunchKernelGGL(k_transport, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+    if (v.geom_kind == 1) hipLaunchKernelGGL(k_transport<true>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+    else hipLaunchKernelGGL(k_transport<false>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
 }
 void events(const View& v, hipStream_t s) {
-    hipLaunchKernelGGL(k_events, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+    if (v.geom_kind == 1) hipLaunchKernelGGL(k_events<true>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+    else hipLaunchKernelGGL(k_events<false>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
 }
 void inject(const View& v, hipStream_t s) {
-    hipLaunchKernelGGL(k_inject, dim3(kInjectBlocks), dim3(256), 0, s, v, wave_blocks(v.capacity));
+    if (v.geom_kind == 1)
+        hipLaunchKernelGGL(k_inject<true>, dim3(kInjectBlocks), dim3(256), 0, s, v, wave_blocks(v.capacity));
+    else hipLaunchKernelGGL(k_inject<false>, dim3(kInjectBlocks), dim3(256), 0, s, v, wave_blocks(v.capacity));
 }
 void refresh(const View& v, hipStream_t s, int phase) { hipLaunchKernelGGL(k_refresh, dim3(1), dim3(1024), 0, s, v, phase); }
 void sort(const View& v, hipStream_t s) {

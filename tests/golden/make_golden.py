@@ -3,6 +3,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
 
     make -f oracle/ref.mk REF=/root/reference && python tests/golden/make_golden.py [--runs] [--source-runs]
     python tests/golden/make_golden.py --callaway [--callaway-runs]   (Callaway fixtures only)
+    python tests/golden/make_golden.py --mesh [--mesh-runs]           (FinFET mesh fixtures only)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -11,6 +12,8 @@ from a copy of its mt19937_64 engine) and the reference outputs.  Only data is
 written; nothing of the reference's code is copied.
 """
 import os
+import re
+import shutil
 import sys
 import tempfile
 
@@ -24,16 +27,24 @@ from phonomc_amd import runtime as rt  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml",
-             "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml"}
+             "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml"}
+
+
+def write_input(text, folder, name):
+    """Writes a TOML input into folder, with the mesh file it names beside it."""
+    model = re.search(r'^model\s*=\s*"([^"]+)"', text, re.M)
+    if model and model.group(1) != "box":
+        shutil.copy(os.path.join(rt.CONFIGS, model.group(1)), folder)
+    path = os.path.join(folder, name)
+    with open(path, "w") as f:
+        f.write(text)
+    return path
 
 
 def problem_and_reference(key, **overrides):
     text = rt.config_text(WORKLOADS[key], **overrides)
-    problem = rt.Problem(text, mode_sampling=1)
-    tmp = tempfile.mkdtemp()
-    path = os.path.join(tmp, f"{key}.toml")
-    with open(path, "w") as f:
-        f.write(text)
+    problem = rt.Problem(text, input_directory=rt.CONFIGS, mode_sampling=1)
+    path = write_input(text, tempfile.mkdtemp(), f"{key}.toml")
     return problem, R.RefMaterial(problem.material()), R.RefDomain(path), path
 
 
@@ -205,6 +216,37 @@ def callaway_vectors(problem, rng):
     return out
 
 
+def mesh_vectors(problem, rd, rng):
+    """Mesh model: facets and grid, and the reference's trace, containment and
+    nearest facet on interior rays, rays from surface points and points on,
+    near and away from the surface."""
+    out = domain_vectors(problem, rd, rng)
+    bmin, bmax, _ = rd.bounds()
+    box = bmin + rng.random((20000, 3)) * (bmax - bmin)
+    inner = box[rd.contains(box)][:1500]
+    d = rng.normal(size=(len(inner), 3))
+    vel = d / np.linalg.norm(d, axis=1)[:, None] * rng.uniform(5.0, 80.0, (len(inner), 1))
+    vel[:100, 0] = 0.0
+    vel[100:200, 1:] = 0.0
+    sp, sv = [], []
+    f = rd.facets()
+    for k in range(len(f["areas"])):
+        pts, _ = rd.sample_surface(k, 100, 7 + k)
+        v = rng.normal(size=(100, 3)) * 30.0
+        v *= np.where(v @ f["normals"][k] > 0, -1.0, 1.0)[:, None]  # into the body
+        sp.append(pts)
+        sv.append(v)
+    sp, sv = np.concatenate(sp), np.concatenate(sv)
+    pos, vv = np.concatenate([inner, sp]), np.concatenate([vel, sv])
+    pt, t, facet = rd.trace(pos, vv)
+    near = np.concatenate([sp + 1e-11 * rng.normal(size=sp.shape), sp + 1e-8 * rng.normal(size=sp.shape),
+                           sp + 1e-3 * rng.normal(size=sp.shape), box[:1500]])
+    out.update(mesh_trace_pos=pos, mesh_trace_vel=vv, mesh_trace_point=pt, mesh_trace_t=t, mesh_trace_facet=facet,
+               mesh_contains_pos=near, mesh_contains=rd.contains(near).astype(np.uint8),
+               mesh_nearest_pos=near[::4], mesh_nearest=rd.nearest_facet(near[::4]), mesh_interior_fraction=len(inner))
+    return out
+
+
 def whole_run(key, n, steps, seeds, interval):
     """Reference runs at reduced size: final cell temperatures and kappa per seed."""
     temps, keff, kint = [], [], []
@@ -214,10 +256,8 @@ def whole_run(key, n, steps, seeds, interval):
                                                  "simulation.random_seed": seed,
                                                  "simulation.convergence_write_interval": interval,
                                                  "io.output_folder": f'"{tmp}/out"'})
-        path = os.path.join(tmp, "run.toml")
-        with open(path, "w") as f:
-            f.write(text)
-        problem = rt.Problem(text)
+        path = write_input(text, tmp, "run.toml")
+        problem = rt.Problem(text, input_directory=rt.CONFIGS)
         R.run(path, problem.material(), steps, threads=8)
         rows = [line.split() for line in open(os.path.join(tmp, "out", "convergence.txt")) if not line.startswith("#")]
         ncell = int(problem.scalars()["cells"])
@@ -232,6 +272,16 @@ def whole_run(key, n, steps, seeds, interval):
 
 
 def main():
+    if "--mesh" in sys.argv:
+        rng = np.random.default_rng(20261205)
+        p, rm, rd, _ = problem_and_reference("finfet")
+        np.savez_compressed(os.path.join(OUT, "domain_finfet.npz"), **mesh_vectors(p, rd, rng))
+        np.savez_compressed(os.path.join(OUT, "reservoir_finfet.npz"), **reservoir_vectors(p, rm, rd, rng, 0.2))
+        if "--mesh-runs" in sys.argv:
+            np.savez_compressed(os.path.join(OUT, "run_finfet.npz"),
+                                **whole_run("finfet", 100000, 400, list(range(1000, 1016)), 10))
+        print("mesh fixtures written to", OUT)
+        return
     if "--callaway" in sys.argv:
         rng = np.random.default_rng(20261204)
         text = rt.config_text(WORKLOADS["c1c"])

@@ -75,3 +75,30 @@ def test_excess_events_guard_latches_context():
         g.step(1)
     with pytest.raises(RuntimeError, match="pmc error 8"):
         g.step(1)
+
+
+def test_finfet_mesh_run_keeps_carriers_inside_and_closes_energy():
+    """FinFET mesh at 1e6 carriers for 201 steps: the device containment check
+    (star tetrahedra, ray cast) runs at steps 0, 100 and 200 and finds no
+    escaped carrier; every carrier lies inside the bounding box and in a cell;
+    the energy balance closes with the heat source every step; reservoir and
+    rough-wall events occur; the fin top heats above the substrate."""
+    p = rt.Problem(rt.config_text("finfet.toml"), input_directory=rt.CONFIGS)
+    g = rt.Context(p)
+    for _ in range(201):
+        g.step(1)
+        led = g.ledger()
+        assert abs(led["step_balance_residual_ev"]) <= 1e-12 * led["thermal_energy_ev"]
+    assert led["total_absorbed"] > 0 and led["total_injected"] > 0 and led["rough_events"] > 0
+    assert led["source_updates"] > 0
+    c = g.carriers()
+    live = c["mode"] >= 0
+    pos = c["position"][live]
+    s, m, geo, i, rough, res = p.views()
+    assert (pos >= np.array(geo.bmin[:]) - 1e-3).all() and (pos <= np.array(geo.bmax[:]) + 1e-3).all()
+    assert ((c["cell"][live] >= 0) & (c["cell"][live] < int(p.scalars()["cells"]))).all()
+    t, _ = g.cells()
+    centers, _ = p.cells()
+    fin_top = (centers[:, 2] > 1300.0)
+    substrate = centers[:, 2] < 300.0
+    assert t[fin_top].mean() > t[substrate].mean() + 0.5

@@ -21,7 +21,8 @@ def by_uid(c):
 
 
 def run_pair(cfg, n, steps, sampling, series=True):
-    p = rt.Problem(rt.config_text(cfg, **{"simulation.particle_count": n}), mode_sampling=sampling)
+    p = rt.Problem(rt.config_text(cfg, **{"simulation.particle_count": n}), input_directory=rt.CONFIGS,
+                   mode_sampling=sampling)
     g = rt.Context(p)
     g.set_tuning(series_on=series)
     o = O.Sim(p)
@@ -30,7 +31,7 @@ def run_pair(cfg, n, steps, sampling, series=True):
 
 
 @pytest.mark.parametrize("cfg,n", [("c1_cross_plane.toml", 20000), ("c2_inplane_film.toml", 60000),
-                                   ("c1c_callaway.toml", 20000)])
+                                   ("c1c_callaway.toml", 20000), ("finfet.toml", 20000)])
 def test_initial_population_identical(cfg, n):
     p, g, o = run_pair(cfg, n, 0, 1)
     a, b = by_uid(g.carriers()), o.carriers()
@@ -73,10 +74,15 @@ def test_initial_population_from_cell_field():
                                                    ("c1s_source.toml", 20000, 1, True),
                                                    ("c1s_source.toml", 20000, 1, False),
                                                    ("c1c_callaway.toml", 20000, 1, True),
-                                                   ("c2c_callaway.toml", 60000, 1, True)])
+                                                   ("c2c_callaway.toml", 60000, 1, True),
+                                                   ("finfet.toml", 20000, 1, True),
+                                                   ("finfet.toml", 20000, 0, False)])
 def test_steps_match_oracle(cfg, n, sampling, series):
     """series=False: every pseudo-temperature evaluation is an exact carrier sum,
     the reference's own procedure; series=True: the per-cell Taylor series.
+    finfet traces the 4080-triangle FinFET surface (device BVH against the
+    oracle's restatement of the reference's trace), with a periodic substrate,
+    a bottom reservoir, rough walls and a Gaussian heat source in the fin.
     The callaway inputs run linearized Callaway collisions (the device's Gram-space
     FP64 evaluation against the oracle's long-double restatement of the
     reference), fixed-flux reservoirs and, in c2c, strict-elastic discrete-mode

@@ -1,6 +1,6 @@
 """Whole-run statistics of the device step against the reference solver.
 
-tests/golden/run_c1.npz, run_c2.npz, run_c1s.npz, run_c1c.npz and run_c2c.npz hold 16 runs of the
+tests/golden/run_c1.npz, run_c2.npz, run_c1s.npz, run_c1c.npz, run_c2c.npz and run_finfet.npz hold 16 runs of the
 reference PhonoMC build per workload (seeds 1000..1015, written by tests/golden/make_golden.py):
 second-half averages of every cell temperature and of kappa_int / kappa_eff as
 the reference writes them to convergence.txt.  The same TOML inputs and seeds
@@ -24,7 +24,7 @@ from phonomc_amd import runtime as rt
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml",
-             "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml"}
+             "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml"}
 
 
 def second_half(folder, ncell, steps):
@@ -46,7 +46,7 @@ def gpu_runs(key, golden):
                                                  "simulation.random_seed": int(seed),
                                                  "simulation.convergence_write_interval": interval,
                                                  "io.output_folder": f'"{tmp}/out"'})
-        solver = rt.Solver(rt.Problem(text), os.path.join(tmp, "out"))
+        solver = rt.Solver(rt.Problem(text, input_directory=rt.CONFIGS), os.path.join(tmp, "out"))
         solver.run(steps)
         ncell = solver.n_cells
         solver.close()
@@ -62,10 +62,11 @@ def z_score(a, b):
     return (a.mean(axis=0) - b.mean(axis=0)) / se
 
 
-@pytest.mark.parametrize("key", ["c1", "c2", "c1s", "c1c", "c2c"])
+@pytest.mark.parametrize("key", ["c1", "c2", "c1s", "c1c", "c2c", "finfet"])
 def test_whole_run_statistics_match_reference(key):
     """c1s adds the Gaussian heat source (kappa is not estimated with a source);
-    c1c and c2c run linearized Callaway scattering (c2c with strict-elastic rough walls)."""
+    c1c and c2c run linearized Callaway scattering (c2c with strict-elastic rough walls);
+    finfet runs the 4080-triangle FinFET mesh with a heat source in the fin."""
     golden = np.load(os.path.join(HERE, "golden", f"run_{key}.npz"))
     temps, keff, kint = gpu_runs(key, golden)
     zt = z_score(temps, golden["temperature"])
