@@ -58,8 +58,13 @@ def gpu_runs(key, golden):
 
 
 def z_score(a, b):
+    """Per-quantity z over the seeds.  A quantity with no spread in either set
+    (a cell the heat has not reached keeps T_bg exactly) scores 0 when the two
+    means agree and inf when they do not."""
     se = np.sqrt(a.var(axis=0, ddof=1) / len(a) + b.var(axis=0, ddof=1) / len(b))
-    return (a.mean(axis=0) - b.mean(axis=0)) / se
+    diff = a.mean(axis=0) - b.mean(axis=0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        return np.where(se > 0, diff / np.where(se > 0, se, 1.0), np.where(diff == 0, 0.0, np.inf))
 
 
 @pytest.mark.parametrize("key", ["c1", "c2", "c1s", "c1c", "c2c", "finfet"])
@@ -70,7 +75,9 @@ def test_whole_run_statistics_match_reference(key):
     golden = np.load(os.path.join(HERE, "golden", f"run_{key}.npz"))
     temps, keff, kint = gpu_runs(key, golden)
     zt = z_score(temps, golden["temperature"])
-    msg = f"{key}: cell T max|z| {np.abs(zt).max():.2f}, mean z^2 {np.mean(zt ** 2):.2f}"
+    spread = (temps.std(axis=0) > 0) | (golden["temperature"].std(axis=0) > 0)
+    msg = (f"{key}: cell T max|z| {np.abs(zt).max():.2f}, mean z^2 {np.mean(zt[spread] ** 2):.2f} over "
+           f"{spread.sum()} cells with spread ({(~spread).sum()} cells at T_bg in every run)")
     if np.any(golden["kappa_eff"] != 0.0):
         zke = z_score(keff, golden["kappa_eff"])
         zki = z_score(kint, golden["kappa_int"])
@@ -79,7 +86,7 @@ def test_whole_run_statistics_match_reference(key):
         assert abs(zke) <= 3.0 and abs(zki) <= 3.0
     print(msg)
     assert np.abs(zt).max() <= 4.5
-    assert np.mean(zt ** 2) <= 2.5
+    assert np.mean(zt[spread] ** 2) <= 2.5
 
 
 def test_fp32_flight_error_against_fp64_oracle():
