@@ -30,6 +30,7 @@ import argparse
 import json
 import os
 import re
+import shutil
 import sys
 import tempfile
 import time
@@ -83,7 +84,10 @@ def cpu_baseline(problem_text, steps, workload=WORKLOAD, carriers=1e7):
     path = os.path.join(tmp, "workload.toml")
     with open(path, "w") as f:
         f.write(text)
-    problem = rt.Problem(text)
+    model = re.search(r'^model\s*=\s*"([^"]+)"', text, re.M)
+    if model and model.group(1) != "box":  # the mesh file beside the input, where the reference looks for it
+        shutil.copy(os.path.join(rt.CONFIGS, model.group(1)), tmp)
+    problem = rt.Problem(text, input_directory=rt.CONFIGS)
     if ref_ctypes.available():
         # The reference solver prints progress on stdout; keep stdout for the JSON line.
         sys.stdout.flush()
@@ -263,7 +267,7 @@ def main():
     from phonomc_amd import runtime as rt
 
     text = rt.config_text(args.workload, **({"simulation.particle_count": args.particles} if args.particles else {}))
-    problem = rt.Problem(text)
+    problem = rt.Problem(text, input_directory=rt.CONFIGS)
     ctx = rt.Context(problem)
     n0 = ctx.ledger()["live_carriers"]
     log(f"[bench] {args.workload}: {n0} carriers initialized; warmup {args.warmup} steps")
