@@ -117,6 +117,14 @@ typedef struct pmc_mesh {
     int32_t n_simplex_nodes;
     const pmc_bvh_node* simplex_nodes;
     const int32_t* simplex_order;  /* [n_simplices] */
+    /* Distance lower bounds on a voxel grid from bmin with edge field_h:
+     * field[(i*dims[1] + j)*dims[2] + k] * field_unit <= the distance from any
+     * point of voxel (i,j,k) to every face (255: at least 255 * field_unit).
+     * Optional (field == NULL): a carrier whose flight is shorter than the
+     * bound skips the face search. */
+    int32_t field_dims[3];
+    double field_h, field_unit;
+    const uint8_t* field;
 } pmc_mesh;
 
 /* Geometry and cell grid; replaces SimulationDomain::trace_boundary_intersection

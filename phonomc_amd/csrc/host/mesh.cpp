@@ -564,6 +564,55 @@ std::vector<double> SurfaceMesh::boundary_edge_lengths(int facet) const {
     return out;
 }
 
+// ---------------------------------------------------------------- distance field
+DistanceField build_distance_field(const SurfaceMesh& m, const Vec3& lo, const Vec3& hi, long max_voxels,
+                                   double unit) {
+    DistanceField f;
+    f.unit = unit;
+    const Vec3 ext = sub(hi, lo);
+    const double big = std::max({ext[0], ext[1], ext[2], 1e-9});
+    double prod = 1.0;
+    for (int a = 0; a < 3; ++a) prod *= std::max(ext[a], 1e-3 * big);
+    f.h = std::max(std::cbrt(prod / static_cast<double>(max_voxels)), 1e-6 * big);
+    for (int a = 0; a < 3; ++a) f.dims[a] = std::max(1, static_cast<int>(std::ceil(ext[a] / f.h)));
+    while (static_cast<long>(f.dims[0]) * f.dims[1] * f.dims[2] > max_voxels) {
+        f.h *= 1.05;
+        for (int a = 0; a < 3; ++a) f.dims[a] = std::max(1, static_cast<int>(std::ceil(ext[a] / f.h)));
+    }
+    const double half_diag = 0.5 * f.h * std::sqrt(3.0);
+    const double cap = 255.0 * unit, reach = cap + half_diag;
+    const int nx = f.dims[0], ny = f.dims[1], nz = f.dims[2];
+    f.bound.assign(static_cast<size_t>(nx) * ny * nz, 255);
+    const int nf = static_cast<int>(m.faces.size());
+    // Voxel range of each face's box grown by `reach`; a voxel no face range
+    // covers is at least `reach` from every face, so its bound is at least cap.
+    std::vector<std::array<int, 6>> range(nf);
+    for (int t = 0; t < nf; ++t)
+        for (int a = 0; a < 3; ++a) {
+            const double b0 = (m.face_bounds[t][0][a] - reach - lo[a]) / f.h - 0.5;
+            const double b1 = (m.face_bounds[t][1][a] + reach - lo[a]) / f.h - 0.5;
+            range[t][a] = std::clamp(static_cast<int>(std::ceil(b0)), 0, f.dims[a] - 1);
+            range[t][3 + a] = std::clamp(static_cast<int>(std::floor(b1)), -1, f.dims[a] - 1);
+        }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < nx; ++i) { // one x slab per iteration: no two threads write one voxel
+        for (int t = 0; t < nf; ++t) {
+            if (i < range[t][0] || i > range[t][3]) continue;
+            const auto& tri = m.faces[t];
+            const Vec3 &a = m.vertices[tri[0]], &b = m.vertices[tri[1]], &c = m.vertices[tri[2]];
+            for (int j = range[t][1]; j <= range[t][4]; ++j)
+                for (int k = range[t][2]; k <= range[t][5]; ++k) {
+                    const Vec3 centre{lo[0] + (i + 0.5) * f.h, lo[1] + (j + 0.5) * f.h, lo[2] + (k + 0.5) * f.h};
+                    const double d = norm(sub(closest_point_on_triangle(centre, a, b, c), centre)) - half_diag;
+                    const double q = std::floor(std::max(0.0, d) / unit);
+                    uint8_t& cell = f.bound[(static_cast<size_t>(i) * ny + j) * nz + k];
+                    if (q < cell) cell = static_cast<uint8_t>(q);
+                }
+        }
+    }
+    return f;
+}
+
 // ---------------------------------------------------------------- device BVH
 Bvh build_bvh(const std::vector<std::array<Vec3, 2>>& boxes, const std::vector<Vec3>& centroids, int leaf) {
     constexpr int kBins = 12;

@@ -98,6 +98,17 @@ struct Bvh {
 };
 Bvh build_bvh(const std::vector<std::array<Vec3, 2>>& boxes, const std::vector<Vec3>& centroids, int leaf);
 
+// Lower bounds of the distance to the surface on a voxel grid over [lo, hi]
+// (pmc_mesh field layout): at most max_voxels voxels, bounds quantized down to
+// multiples of unit and saturated at 255 units.
+struct DistanceField {
+    std::array<int, 3> dims{1, 1, 1};
+    double h = 0, unit = 0;
+    std::vector<uint8_t> bound;
+};
+DistanceField build_distance_field(const SurfaceMesh& mesh, const Vec3& lo, const Vec3& hi, long max_voxels,
+                                   double unit);
+
 // Procedural FinFET on a lattice: a substrate box with one fin on its top face,
 // every exposed lattice face split into two triangles (watertight, manifold).
 struct FinfetSpec {

@@ -216,6 +216,12 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         mv.n_simplex_nodes = static_cast<int>(a.simplex_bvh.nodes.size());
         mv.simplex_nodes = a.simplex_bvh.nodes.data();
         mv.simplex_order = a.simplex_bvh.order.data();
+        // 4M voxels of 1-byte bounds in quarter-angstrom units (0 to 63.75 A).
+        a.field = build_distance_field(sm, d.bmin, d.bmax, 4L << 20, 0.25);
+        for (int k = 0; k < 3; ++k) mv.field_dims[k] = a.field.dims[k];
+        mv.field_h = a.field.h;
+        mv.field_unit = a.field.unit;
+        mv.field = a.field.bound.data();
         g.kind = 1;
         g.mesh = &p->mesh_view;
     }

@@ -49,6 +49,7 @@ struct Problem {
         std::vector<int32_t> faces, face_facet;
         Bvh face_bvh, simplex_bvh;
         std::vector<double> simplex_vertices, simplex_inverse, simplex_bounds, simplex_volume;
+        DistanceField field;
     } mesh;
     pmc_mesh mesh_view{};
     // C-ABI views (pointers into the vectors above)

@@ -201,23 +201,38 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 namespace {
 // Mesh upload: faces and star tetrahedra in BVH leaf order, node boxes padded
 // by the mesh tolerance and rounded outward.
-int bvh_depth(const pmc_bvh_node* nodes, int n, int node, int depth) {
-    if (node < 0 || node >= n) return 1 << 20;
-    if (nodes[node].count > 0) return depth;
-    return std::max(bvh_depth(nodes, n, node + 1, depth + 1), bvh_depth(nodes, n, nodes[node].index, depth + 1));
-}
-
-std::vector<pmc_dev::BvhDev> padded_nodes(const pmc_bvh_node* nodes, int n, double pad) {
-    std::vector<pmc_dev::BvhDev> out(n);
-    for (int i = 0; i < n; ++i) {
-        for (int a = 0; a < 3; ++a) {
-            out[i].bmin[a] = std::nextafter(static_cast<float>(static_cast<double>(nodes[i].bmin[a]) - pad), -INFINITY);
-            out[i].bmax[a] = std::nextafter(static_cast<float>(static_cast<double>(nodes[i].bmax[a]) + pad), INFINITY);
+// Checks a pmc_bvh_node array over `items` and threads it: padded boxes, and
+// for interior nodes the first node after the subtree in place of the right
+// child.  Returns false on a malformed hierarchy.
+bool threaded_nodes(const pmc_bvh_node* nodes, int n, int items, double pad, std::vector<pmc_dev::BvhDev>& out) {
+    if (n <= 0 || !nodes) return false;
+    out.assign(n, {});
+    std::vector<int> end(n + 1, 0);
+    for (int i = n - 1; i >= 0; --i) { // right children lie after their parent
+        const pmc_bvh_node& nd = nodes[i];
+        if (nd.count > 0) {
+            if (nd.index < 0 || nd.index + nd.count > items) return false;
+            end[i] = i + 1;
+        } else {
+            if (nd.count != 0 || i + 1 >= n || nd.index <= i + 1 || nd.index >= n) return false;
+            if (end[i + 1] != nd.index) return false; // the left subtree ends where the right child starts
+            end[i] = end[nd.index];
         }
-        out[i].index = nodes[i].index;
-        out[i].count = nodes[i].count;
+        for (int a = 0; a < 3; ++a) {
+            out[i].bmin[a] = std::nextafter(static_cast<float>(static_cast<double>(nd.bmin[a]) - pad), -INFINITY);
+            out[i].bmax[a] = std::nextafter(static_cast<float>(static_cast<double>(nd.bmax[a]) + pad), INFINITY);
+        }
+        out[i].index = nd.count > 0 ? nd.index : end[i];
+        out[i].count = nd.count;
     }
-    return out;
+    if (end[0] != n) return false; // the root's subtree is the whole array
+    std::vector<int> seen(items, 0); // every item in exactly one leaf
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < nodes[i].count; ++k)
+            if (seen[nodes[i].index + k]++) return false;
+    for (int v : seen)
+        if (v != 1) return false;
+    return true;
 }
 
 int upload_mesh(pmc_ctx* ctx, const pmc_geometry* g) {
@@ -225,9 +240,6 @@ int upload_mesh(pmc_ctx* ctx, const pmc_geometry* g) {
     const pmc_mesh& M = *g->mesh;
     if (M.n_faces <= 0 || M.n_face_nodes <= 0 || !M.face_nodes || !M.face_order)
         return fail(ctx, PMC_ERR_ARGUMENT, "mesh without faces or face hierarchy");
-    if (bvh_depth(M.face_nodes, M.n_face_nodes, 0, 1) > pmc_dev::kBvhStack ||
-        (M.n_simplices > 0 && bvh_depth(M.simplex_nodes, M.n_simplex_nodes, 0, 1) > pmc_dev::kBvhStack))
-        return fail(ctx, PMC_ERR_ARGUMENT, "mesh hierarchy deeper than the traversal stack");
     // FP32 positions carry about 2^-24 of the largest coordinate; eight of those
     // quanta are taken as on the surface.
     double reach = 0.0;
@@ -250,19 +262,29 @@ int upload_mesh(pmc_ctx* ctx, const pmc_geometry* g) {
         facet[j] = M.face_facet[f];
         if (facet[j] < 0 || facet[j] >= g->n_facets) return fail(ctx, PMC_ERR_ARGUMENT, "face without facet");
     }
-    for (int i = 0; i < M.n_face_nodes; ++i) {
-        const pmc_bvh_node& n = M.face_nodes[i];
-        const bool ok = n.count > 0 ? (n.index >= 0 && n.index + n.count <= nf)
-                                    : (n.count == 0 && i + 1 < M.n_face_nodes && n.index > i && n.index < M.n_face_nodes);
-        if (!ok) return fail(ctx, PMC_ERR_ARGUMENT, "malformed face hierarchy");
-    }
+    std::vector<pmc_dev::BvhDev> fn;
+    if (!threaded_nodes(M.face_nodes, M.n_face_nodes, nf, v.mesh_tol, fn))
+        return fail(ctx, PMC_ERR_ARGUMENT, "malformed face hierarchy");
     v.nfaces = nf;
     v.face_plane = ctx->upload(plane.data(), plane.size());
     v.face_tri = ctx->upload(tri.data(), tri.size());
     v.face_box = ctx->upload(box.data(), box.size());
     v.face_facet = ctx->upload(facet.data(), facet.size());
-    const auto fn = padded_nodes(M.face_nodes, M.n_face_nodes, v.mesh_tol);
     v.face_nodes = ctx->upload(fn.data(), fn.size());
+    v.n_face_nodes = M.n_face_nodes;
+    v.field = nullptr;
+    if (M.field) {
+        const size_t nv = static_cast<size_t>(M.field_dims[0]) * M.field_dims[1] * M.field_dims[2];
+        if (M.field_dims[0] <= 0 || M.field_dims[1] <= 0 || M.field_dims[2] <= 0 || nv > (size_t(1) << 30) ||
+            !(M.field_h > 0.0) || !(M.field_unit > 0.0))
+            return fail(ctx, PMC_ERR_ARGUMENT, "malformed distance field");
+        v.field = ctx->upload(reinterpret_cast<const unsigned char*>(M.field), nv);
+        for (int a = 0; a < 3; ++a) v.field_dims[a] = M.field_dims[a];
+        // Rounded so the device bound never exceeds the host's: a shorter unit
+        // and a longer voxel edge.
+        v.field_inv_h = std::nextafter(static_cast<float>(1.0 / M.field_h), 0.0f);
+        v.field_unit = std::nextafter(static_cast<float>(M.field_unit), 0.0f);
+    }
     v.nsimplices = M.n_simplices;
     if (M.n_simplices > 0) {
         const int ns = M.n_simplices;
@@ -279,22 +301,18 @@ int upload_mesh(pmc_ctx* ctx, const pmc_geometry* g) {
         }
         double acc = 0.0;
         for (int t = 0; t < ns; ++t) cdf[t] = (acc += M.simplex_volume[t]);
-        for (int i = 0; i < M.n_simplex_nodes; ++i) {
-            const pmc_bvh_node& n = M.simplex_nodes[i];
-            const bool ok = n.count > 0 ? (n.index >= 0 && n.index + n.count <= ns)
-                                        : (n.count == 0 && i + 1 < M.n_simplex_nodes && n.index > i &&
-                                           n.index < M.n_simplex_nodes);
-            if (!ok) return fail(ctx, PMC_ERR_ARGUMENT, "malformed simplex hierarchy");
-        }
+        std::vector<pmc_dev::BvhDev> sn;
+        // Tetrahedron boxes are tested with the 1e-10 barycentric tolerance only;
+        // the padding covers it.
+        if (!threaded_nodes(M.simplex_nodes, M.n_simplex_nodes, ns, 1e-6, sn))
+            return fail(ctx, PMC_ERR_ARGUMENT, "malformed simplex hierarchy");
         v.simplex_v = ctx->upload(sv.data(), sv.size());
         v.simplex_inv = ctx->upload(si.data(), si.size());
         v.simplex_box = ctx->upload(sb.data(), sb.size());
         v.simplex_cdf = ctx->upload(cdf.data(), cdf.size());
         v.simplex_pos = ctx->upload(pos.data(), pos.size());
-        // Tetrahedron boxes are tested with the 1e-10 barycentric tolerance only;
-        // the padding covers it.
-        const auto sn = padded_nodes(M.simplex_nodes, M.n_simplex_nodes, 1e-6);
         v.simplex_nodes = ctx->upload(sn.data(), sn.size());
+        v.n_simplex_nodes = M.n_simplex_nodes;
     }
     return PMC_OK;
 }
@@ -490,7 +508,8 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
     const size_t nc = static_cast<size_t>(g->n_cells);
     v.cell_e = ctx->alloc<double>(nc);
     v.cell_tally = ctx->alloc<double>(pmc_dev::kTallyStride * nc);
-    v.partial_rows = pmc_launch::partial_rows(v.capacity);
+    v.event_split = g->kind == 1 ? pmc_launch::event_split(static_cast<int>(v.capacity)) : 1;
+    v.partial_rows = pmc_launch::partial_rows(v);
     v.partials = ctx->alloc<double>(static_cast<size_t>(v.partial_rows) * pmc_dev::kPartialCols);
     v.cell_T = ctx->alloc<double>(nc);
     v.cell_flux = ctx->alloc<double>(3 * nc);

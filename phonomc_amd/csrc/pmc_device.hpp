@@ -90,8 +90,10 @@ struct FacetDev {
     int axis; // axis of the normal of an axis-aligned facet, else -1 (mirror reflections need one)
 };
 
-// Bounding-volume hierarchy node (pmc_bvh_node layout): boxes padded by the
-// mesh tolerance at upload; left child next, right child at `index`.
+// Threaded bounding-volume hierarchy node: boxes padded by the mesh tolerance at
+// upload, nodes in depth-first order.  An interior node (count 0) has its left
+// child next and `index` = the first node after its subtree, where a traversal
+// that misses the box continues; a leaf holds items index .. index+count-1.
 struct BvhDev {
     float bmin[3], bmax[3];
     int index, count;
@@ -208,6 +210,11 @@ struct View {
     const double* face_box;    // 6 per face: min, max
     const int* face_facet;
     const BvhDev* face_nodes;
+    int n_face_nodes;
+    const unsigned char* field; // distance lower bounds (pmc_mesh field), or null
+    int field_dims[3];
+    float field_inv_h, field_unit;
+    int event_split;            // k_events blocks per transport block on a mesh
     int nsimplices;
     const double* simplex_v;   // 12 per tetrahedron: apex, face vertices
     const double* simplex_inv; // 9 per tetrahedron
@@ -215,6 +222,7 @@ struct View {
     const double* simplex_cdf; // cumulative volume (A^3), original order
     const int* simplex_pos;    // original index -> BVH order
     const BvhDev* simplex_nodes;
+    int n_simplex_nodes;
     // boundary tables
     const RoughDev* rough;
     const ResDev* res;
@@ -222,7 +230,7 @@ struct View {
     // cells
     double* cell_tally;   // kTallyStride per cell
     double* partials;     // kPartialCols per block row
-    int partial_rows;     // transport blocks + injection blocks
+    int partial_rows;     // event blocks + injection blocks
     double* cell_e;       // energy density of the latest refresh
     double* cell_T;
     double* cell_flux;    // 3 per cell
@@ -282,7 +290,5 @@ struct View {
 // kCwPer carriers per thread held in registers.
 constexpr int kCwThreads = 512, kCwWaves = kCwThreads / 64, kCwPer = 16;
 
-// Deepest BVH the traversal stacks take (hierarchies are checked at upload).
-constexpr int kBvhStack = 48;
 
 } // namespace pmc_dev

@@ -137,25 +137,26 @@ __device__ double tri_dist2(const double p[3], const double* T) {
     return r2;
 }
 
-// Slab test of the ray p + v t, t in [0, tmax], against a node box.
-__device__ __forceinline__ bool ray_box(const BvhDev& N, const double p[3], const double v[3], const double inv[3],
-                                        double tmax) {
-    double t0 = 0.0, t1 = tmax;
+// Slab test of the ray p + v t, t in [0, tmax], against a node box, in FP32:
+// node boxes are padded by mesh_tol, eight FP32 quanta of the coordinates, which
+// covers the rounding of this test.
+__device__ __forceinline__ bool ray_box(const BvhDev& N, const float p[3], const float v[3], const float inv[3],
+                                        float tmax) {
+    float t0 = 0.0f, t1 = tmax;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const double lo = N.bmin[a], hi = N.bmax[a];
-        if (v[a] == 0.0) {
-            if (p[a] < lo || p[a] > hi) return false;
+        if (v[a] == 0.0f) {
+            if (p[a] < N.bmin[a] || p[a] > N.bmax[a]) return false;
             continue;
         }
-        double ta = (lo - p[a]) * inv[a], tb = (hi - p[a]) * inv[a];
+        float ta = (N.bmin[a] - p[a]) * inv[a], tb = (N.bmax[a] - p[a]) * inv[a];
         if (ta > tb) {
-            const double x = ta;
+            const float x = ta;
             ta = tb;
             tb = x;
         }
-        t0 = fmax(t0, ta);
-        t1 = fmin(t1, tb);
+        t0 = fmaxf(t0, ta);
+        t1 = fminf(t1, tb);
     }
     return t0 <= t1;
 }
@@ -165,27 +166,29 @@ __device__ __forceinline__ bool point_box(const BvhDev& N, const double p[3]) {
            p[2] <= N.bmax[2];
 }
 
-// Nearest face the carrier leaves through along p + v t, t >= 0: the
+// Nearest face the carrier leaves through along p + v t, 0 <= t <= tmax: the
 // restatement of SurfaceMesh::trace_boundary_intersection
-// (src/SurfaceMesh.cpp:1164-1270) for FP32 positions.  Only faces with
-// v.n > 1e-10 count (from inside a closed surface the first crossing is always
-// one of them); a point up to mesh_tol beyond such a face's plane is on it
-// (t = 0); a crossing counts when it lies within mesh_tol of the triangle; an
-// exact tie goes to the lower face.  Returns the face (BVH order) or -1.
-__device__ int mesh_trace(const View& V, const double p[3], const double v[3], double& t_hit) {
-    const double inv[3] = {1.0 / v[0], 1.0 / v[1], 1.0 / v[2]};
+// (src/SurfaceMesh.cpp:1164-1270) for FP32 positions, limited to one flight.
+// Only faces with v.n > 1e-10 count (from inside a closed surface the first
+// crossing is always one of them); a point up to mesh_tol beyond such a face's
+// plane is on it (t = 0); a crossing counts when it lies within mesh_tol of the
+// triangle; an exact tie goes to the lower face.  Returns the face (BVH order),
+// or -1 when no face lies within the flight.
+__device__ int mesh_trace(const View& V, const double p[3], const double v[3], double tmax, double& t_hit) {
+    const float pf[3] = {static_cast<float>(p[0]), static_cast<float>(p[1]), static_cast<float>(p[2])};
+    const float vf[3] = {static_cast<float>(v[0]), static_cast<float>(v[1]), static_cast<float>(v[2])};
+    const float inv[3] = {1.0f / vf[0], 1.0f / vf[1], 1.0f / vf[2]};
     const double tol = V.mesh_tol, tol2 = tol * tol;
-    int stack[kBvhStack];
-    int sp = 0, node = 0, bf = -1;
-    double best = INFINITY;
-    while (true) {
+    int node = 0, bf = -1;
+    double best = tmax;
+    while (node < V.n_face_nodes) { // threaded: a missed box skips its subtree
         const BvhDev N = V.face_nodes[node];
-        if (ray_box(N, p, v, inv, best)) {
-            if (N.count == 0) {
-                if (sp < kBvhStack) stack[sp++] = N.index; // depth is checked at upload
-                node = node + 1;
-                continue;
-            }
+        // best rounded up, so the FP32 bound never cuts a box the FP64 one keeps
+        if (!ray_box(N, pf, vf, inv, static_cast<float>(best) * 1.000001f + 1e-30f)) {
+            node = N.count ? node + 1 : N.index;
+            continue;
+        }
+        if (N.count) {
             for (int f = N.index; f < N.index + N.count; ++f) {
                 const double4 P = V.face_plane[f];
                 const double den = v[0] * P.x + v[1] * P.y + v[2] * P.z;
@@ -196,7 +199,7 @@ __device__ int mesh_trace(const View& V, const double p[3], const double v[3], d
                     if (s > tol) continue;
                     t = 0.0;
                 }
-                if (t > best || (t == best && f > bf)) continue;
+                if (t > best || (t == best && bf >= 0 && f > bf)) continue;
                 const double q[3] = {p[0] + v[0] * t, p[1] + v[1] * t, p[2] + v[2] * t};
                 const double* B = V.face_box + 6 * f;
                 if (q[0] < B[0] - tol || q[1] < B[1] - tol || q[2] < B[2] - tol || q[0] > B[3] + tol ||
@@ -207,51 +210,30 @@ __device__ int mesh_trace(const View& V, const double p[3], const double v[3], d
                 bf = f;
             }
         }
-        if (sp == 0) break;
-        node = stack[--sp];
+        ++node;
     }
     t_hit = best;
     return bf;
 }
 
-// Whether mesh_trace would find a face within t <= tmax: the same face tests,
-// returning at the first one found.
-__device__ bool mesh_reaches(const View& V, const double p[3], const double v[3], double tmax) {
-    const double inv[3] = {1.0 / v[0], 1.0 / v[1], 1.0 / v[2]};
-    const double tol = V.mesh_tol, tol2 = tol * tol;
-    int stack[kBvhStack];
-    int sp = 0, node = 0;
-    while (true) {
-        const BvhDev N = V.face_nodes[node];
-        if (ray_box(N, p, v, inv, tmax)) {
-            if (N.count == 0) {
-                if (sp < kBvhStack) stack[sp++] = N.index;
-                node = node + 1;
-                continue;
-            }
-            for (int f = N.index; f < N.index + N.count; ++f) {
-                const double4 P = V.face_plane[f];
-                const double den = v[0] * P.x + v[1] * P.y + v[2] * P.z;
-                if (!(den > 1e-10)) continue;
-                const double s = p[0] * P.x + p[1] * P.y + p[2] * P.z + P.w;
-                double t = -s / den;
-                if (t < 0.0) {
-                    if (s > tol) continue;
-                    t = 0.0;
-                }
-                if (t > tmax) continue;
-                const double q[3] = {p[0] + v[0] * t, p[1] + v[1] * t, p[2] + v[2] * t};
-                const double* B = V.face_box + 6 * f;
-                if (q[0] < B[0] - tol || q[1] < B[1] - tol || q[2] < B[2] - tol || q[0] > B[3] + tol ||
-                    q[1] > B[4] + tol || q[2] > B[5] + tol)
-                    continue;
-                if (tri_dist2(q, V.face_tri + 9 * f) <= tol2) return true;
-            }
-        }
-        if (sp == 0) break;
-        node = stack[--sp];
+// Lower bound of the distance from p to every face, from the distance field
+// (0 without one).  FP32 voxel lookup; callers keep a margin of 2 mesh_tol.
+__device__ __forceinline__ float mesh_clearance(const View& V, const float p[3]) {
+    if (!V.field) return 0.0f;
+    int k[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int i = static_cast<int>((p[a] - V.fbmin[a]) * V.field_inv_h);
+        k[a] = i < 0 ? 0 : (i >= V.field_dims[a] ? V.field_dims[a] - 1 : i);
     }
-    return false;
+    return V.field[(static_cast<size_t>(k[0]) * V.field_dims[1] + k[1]) * V.field_dims[2] + k[2]] * V.field_unit;
+}
+
+// Whether a flight of `t` ps at FP32 speed `speed` from p stays clear of every
+// face by the distance field (then no face search is needed).
+__device__ __forceinline__ bool mesh_clear(const View& V, const float p[3], float speed, float t) {
+    const float reach = speed * t * 1.00001f + static_cast<float>(2.0 * V.mesh_tol);
+    return reach < mesh_clearance(V, p);
 }
 
 // Odd number of distinct +x crossings (SurfaceMesh::contains_point_ray_cast,
@@ -261,45 +243,41 @@ __device__ bool mesh_ray_cast(const View& V, const double p[3]) {
     constexpr int kMaxHits = 32;
     double hits[kMaxHits];
     int nh = 0;
-    const double v[3] = {1.0, 0.0, 0.0}, inv[3] = {1.0, INFINITY, INFINITY};
-    int stack[kBvhStack];
-    int sp = 0, node = 0;
-    while (true) {
+    const float pf[3] = {static_cast<float>(p[0]), static_cast<float>(p[1]), static_cast<float>(p[2])};
+    const float vf[3] = {1.0f, 0.0f, 0.0f}, inv[3] = {1.0f, INFINITY, INFINITY};
+    int node = 0;
+    while (node < V.n_face_nodes) {
         const BvhDev N = V.face_nodes[node];
-        if (ray_box(N, p, v, inv, INFINITY)) {
-            if (N.count == 0) {
-                if (sp < kBvhStack) stack[sp++] = N.index;
-                node = node + 1;
-                continue;
-            }
-            for (int f = N.index; f < N.index + N.count; ++f) {
-                const double* T = V.face_tri + 9 * f;
-                const double e1[3] = {T[3] - T[0], T[4] - T[1], T[5] - T[2]};
-                const double e2[3] = {T[6] - T[0], T[7] - T[1], T[8] - T[2]};
-                const double pv[3] = {0.0, -e2[2], e2[1]}; // d x e2 with d = +x
-                const double det = e1[1] * pv[1] + e1[2] * pv[2];
-                if (fabs(det) < 1e-10) continue;
-                const double id = 1.0 / det;
-                const double sv[3] = {p[0] - T[0], p[1] - T[1], p[2] - T[2]};
-                const double u = (sv[1] * pv[1] + sv[2] * pv[2]) * id;
-                if (u < -1e-10 || u > 1.0 + 1e-10) continue;
-                const double qv[3] = {sv[1] * e1[2] - sv[2] * e1[1], sv[2] * e1[0] - sv[0] * e1[2],
-                                      sv[0] * e1[1] - sv[1] * e1[0]};
-                const double w = qv[0] * id;
-                if (w < -1e-10 || u + w > 1.0 + 1e-10) continue;
-                const double t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * id;
-                if (!(t > 1e-10)) continue;
-                if (nh == kMaxHits) return true; // a degenerate stack of crossings: not an escape
-                int k = nh++;
-                while (k > 0 && hits[k - 1] > t) {
-                    hits[k] = hits[k - 1];
-                    --k;
-                }
-                hits[k] = t;
-            }
+        if (!ray_box(N, pf, vf, inv, INFINITY)) {
+            node = N.count ? node + 1 : N.index;
+            continue;
         }
-        if (sp == 0) break;
-        node = stack[--sp];
+        for (int f = N.index; f < N.index + N.count; ++f) { // no iterations on an interior node
+            const double* T = V.face_tri + 9 * f;
+            const double e1[3] = {T[3] - T[0], T[4] - T[1], T[5] - T[2]};
+            const double e2[3] = {T[6] - T[0], T[7] - T[1], T[8] - T[2]};
+            const double pv[3] = {0.0, -e2[2], e2[1]}; // d x e2 with d = +x
+            const double det = e1[1] * pv[1] + e1[2] * pv[2];
+            if (fabs(det) < 1e-10) continue;
+            const double id = 1.0 / det;
+            const double sv[3] = {p[0] - T[0], p[1] - T[1], p[2] - T[2]};
+            const double u = (sv[1] * pv[1] + sv[2] * pv[2]) * id;
+            if (u < -1e-10 || u > 1.0 + 1e-10) continue;
+            const double qv[3] = {sv[1] * e1[2] - sv[2] * e1[1], sv[2] * e1[0] - sv[0] * e1[2],
+                                  sv[0] * e1[1] - sv[1] * e1[0]};
+            const double w = qv[0] * id;
+            if (w < -1e-10 || u + w > 1.0 + 1e-10) continue;
+            const double t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * id;
+            if (!(t > 1e-10)) continue;
+            if (nh == kMaxHits) return true; // a degenerate stack of crossings: not an escape
+            int k = nh++;
+            while (k > 0 && hits[k - 1] > t) {
+                hits[k] = hits[k - 1];
+                --k;
+            }
+            hits[k] = t;
+        }
+        ++node;
     }
     if (nh == 0) return false;
     int unique = 1;
@@ -313,55 +291,46 @@ __device__ bool mesh_ray_cast(const View& V, const double p[3]) {
 // mesh_tol of a face (a carrier that FP32 rounding put just outside the surface
 // it sits on).
 __device__ bool mesh_contains(const View& V, const double p[3]) {
-    int stack[kBvhStack];
     if (V.nsimplices > 0) {
-        int sp = 0, node = 0;
         const double tol = 1e-10;
-        while (true) {
+        int node = 0;
+        while (node < V.n_simplex_nodes) {
             const BvhDev N = V.simplex_nodes[node];
-            if (point_box(N, p)) {
-                if (N.count == 0) {
-                    if (sp < kBvhStack) stack[sp++] = N.index;
-                    node = node + 1;
-                    continue;
-                }
-                for (int j = N.index; j < N.index + N.count; ++j) {
-                    const double* B = V.simplex_box + 6 * j;
-                    if (p[0] < B[0] - tol || p[1] < B[1] - tol || p[2] < B[2] - tol || p[0] > B[3] + tol ||
-                        p[1] > B[4] + tol || p[2] > B[5] + tol)
-                        continue;
-                    const double* o = V.simplex_v + 12 * j;
-                    const double* A = V.simplex_inv + 9 * j;
-                    const double q[3] = {p[0] - o[0], p[1] - o[1], p[2] - o[2]};
-                    const double l1 = A[0] * q[0] + A[1] * q[1] + A[2] * q[2];
-                    const double l2 = A[3] * q[0] + A[4] * q[1] + A[5] * q[2];
-                    const double l3 = A[6] * q[0] + A[7] * q[1] + A[8] * q[2];
-                    const double l0 = 1.0 - l1 - l2 - l3;
-                    if (l0 >= -tol && l1 >= -tol && l2 >= -tol && l3 >= -tol && l0 <= 1.0 + tol && l1 <= 1.0 + tol &&
-                        l2 <= 1.0 + tol && l3 <= 1.0 + tol)
-                        return true;
-                }
+            if (!point_box(N, p)) {
+                node = N.count ? node + 1 : N.index;
+                continue;
             }
-            if (sp == 0) break;
-            node = stack[--sp];
+            for (int j = N.index; j < N.index + N.count; ++j) {
+                const double* B = V.simplex_box + 6 * j;
+                if (p[0] < B[0] - tol || p[1] < B[1] - tol || p[2] < B[2] - tol || p[0] > B[3] + tol ||
+                    p[1] > B[4] + tol || p[2] > B[5] + tol)
+                    continue;
+                const double* o = V.simplex_v + 12 * j;
+                const double* A = V.simplex_inv + 9 * j;
+                const double q[3] = {p[0] - o[0], p[1] - o[1], p[2] - o[2]};
+                const double l1 = A[0] * q[0] + A[1] * q[1] + A[2] * q[2];
+                const double l2 = A[3] * q[0] + A[4] * q[1] + A[5] * q[2];
+                const double l3 = A[6] * q[0] + A[7] * q[1] + A[8] * q[2];
+                const double l0 = 1.0 - l1 - l2 - l3;
+                if (l0 >= -tol && l1 >= -tol && l2 >= -tol && l3 >= -tol && l0 <= 1.0 + tol && l1 <= 1.0 + tol &&
+                    l2 <= 1.0 + tol && l3 <= 1.0 + tol)
+                    return true;
+            }
+            ++node;
         }
     }
     if (mesh_ray_cast(V, p)) return true;
     const double tol2 = V.mesh_tol * V.mesh_tol;
-    int sp = 0, node = 0;
-    while (true) {
+    int node = 0;
+    while (node < V.n_face_nodes) {
         const BvhDev N = V.face_nodes[node];
-        if (point_box(N, p)) {
-            if (N.count == 0) {
-                if (sp < kBvhStack) stack[sp++] = N.index;
-                node = node + 1;
-                continue;
-            }
-            for (int f = N.index; f < N.index + N.count; ++f)
-                if (tri_dist2(p, V.face_tri + 9 * f) <= tol2) return true;
+        if (!point_box(N, p)) {
+            node = N.count ? node + 1 : N.index;
+            continue;
         }
-        if (sp == 0) break;
-        node = stack[--sp];
+        for (int f = N.index; f < N.index + N.count; ++f)
+            if (tri_dist2(p, V.face_tri + 9 * f) <= tol2) return true;
+        ++node;
     }
     return false;
 }
@@ -581,12 +550,12 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             const double* vt = V.veld + 3 * static_cast<size_t>(mw & kModeMask);
             const double vd[3] = {(mw & (1 << kFlipShift)) ? -vt[0] : vt[0], (mw & (2 << kFlipShift)) ? -vt[1] : vt[1],
                                   (mw & (4 << kFlipShift)) ? -vt[2] : vt[2]};
-            if (vd[0] == 0.0 && vd[1] == 0.0 && vd[2] == 0.0) return 0; // stationary mode
+            if (v.w == 0.0f) return 0; // stationary mode
             const double pd[3] = {p[0], p[1], p[2]};
-            double th;
-            const int f = mesh_trace(V, pd, vd, th);
-            if (f < 0) return -PMC_ERR_BOUNDARY;
-            if (th > static_cast<double>(remaining)) {
+            double th = INFINITY;
+            const int f = mesh_clear(V, p, v.w, remaining) ? -1
+                                                           : mesh_trace(V, pd, vd, static_cast<double>(remaining), th);
+            if (f < 0) { // no face within the flight
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     float x = fmaf(vv[a], remaining, p[a]);
@@ -931,14 +900,9 @@ __global__ void __launch_bounds__(256) k_transport(View V) {
                     const float4 v = carrier_velocity(V, mw[j]);
                     const float vv[3] = {v.x, v.y, v.z};
                     if constexpr (kMesh) {
-                        const double* vt = V.veld + 3 * static_cast<size_t>(mw[j] & kModeMask);
-                        const int w = mw[j];
-                        const double vd[3] = {(w & (1 << kFlipShift)) ? -vt[0] : vt[0],
-                                              (w & (2 << kFlipShift)) ? -vt[1] : vt[1],
-                                              (w & (4 << kFlipShift)) ? -vt[2] : vt[2]};
-                        const double pd[3] = {p[0], p[1], p[2]};
-                        event = !(vd[0] == 0.0 && vd[1] == 0.0 && vd[2] == 0.0) &&
-                                mesh_reaches(V, pd, vd, static_cast<double>(dt));
+                        // Carriers the distance field cannot clear go to k_events,
+                        // which searches the faces with whole waves of such carriers.
+                        event = v.w != 0.0f && !mesh_clear(V, p, v.w, dt);
                     } else {
                         float tbest = INFINITY;
                         int plane = -1;
@@ -993,14 +957,19 @@ __global__ void __launch_bounds__(256) k_transport(View V) {
 // for the slots k_transport listed.  One block per transport block.
 template <bool kMesh>
 __global__ void __launch_bounds__(256) k_events(View V) {
+    // On a mesh, event_split blocks share one transport block's list when there
+    // are few transport blocks: the face searches are latency-bound and need
+    // the extra waves.
+    const int kSplit = kMesh ? V.event_split : 1;
     __shared__ double sp[kPartialCols];
     BlockPartials P{sp};
     P.zero();
     PMC_BLOCK_HASH(H);
-    const int blk = xcd_block();
+    const int row = xcd_block();
+    const int blk = row / kSplit, part = row % kSplit;
     const int cnt = V.block_events[blk];
     const int* queue = V.events + static_cast<size_t>(blk) * kSlotsPerBlock;
-    for (int k0 = 0; k0 < cnt; k0 += blockDim.x) {
+    for (int k0 = part * blockDim.x; k0 < cnt; k0 += kSplit * blockDim.x) {
         const int k = k0 + threadIdx.x;
         Tally T{};
         T.absorbed_res = -1;
@@ -1034,7 +1003,7 @@ __global__ void __launch_bounds__(256) k_events(View V) {
         H.count(alive, cell, V.cell_count);
     }
     H.flush_counts(V.cell_count);
-    P.write(V, blk);
+    P.write(V, row);
 }
 
 // Reservoir injection (ReservoirInjector::inject, src/solver/ReservoirInjector.cpp:62-132)
@@ -2557,7 +2526,14 @@ int wave_blocks(int capacity) {
     return (b + 7) & ~7;
 }
 
-int partial_rows(int capacity) { return wave_blocks(capacity) + kInjectBlocks; }
+int event_split(int capacity) {
+    const int b = wave_blocks(capacity);
+    return std::max(1, std::min(8, 2048 / b));
+}
+
+int event_rows(const View& v) { return wave_blocks(v.capacity) * (v.geom_kind == 1 ? v.event_split : 1); }
+
+int partial_rows(const View& v) { return event_rows(v) + kInjectBlocks; }
 
 static int slot_blocks(int capacity) { return (capacity + 255) / 256; }
 static int cell_blocks(int ncells) { return (ncells + 255) / 256; }
@@ -2569,13 +2545,12 @@ void transport(const View& v, hipStream_t s) {
     else hipLaunchKernelGGL(k_transport<false>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
 }
 void events(const View& v, hipStream_t s) {
-    if (v.geom_kind == 1) hipLaunchKernelGGL(k_events<true>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
-    else hipLaunchKernelGGL(k_events<false>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+    if (v.geom_kind == 1) hipLaunchKernelGGL(k_events<true>, dim3(event_rows(v)), kWaveBlock, 0, s, v);
+    else hipLaunchKernelGGL(k_events<false>, dim3(event_rows(v)), kWaveBlock, 0, s, v);
 }
 void inject(const View& v, hipStream_t s) {
-    if (v.geom_kind == 1)
-        hipLaunchKernelGGL(k_inject<true>, dim3(kInjectBlocks), dim3(256), 0, s, v, wave_blocks(v.capacity));
-    else hipLaunchKernelGGL(k_inject<false>, dim3(kInjectBlocks), dim3(256), 0, s, v, wave_blocks(v.capacity));
+    if (v.geom_kind == 1) hipLaunchKernelGGL(k_inject<true>, dim3(kInjectBlocks), dim3(256), 0, s, v, event_rows(v));
+    else hipLaunchKernelGGL(k_inject<false>, dim3(kInjectBlocks), dim3(256), 0, s, v, event_rows(v));
 }
 void refresh(const View& v, hipStream_t s, int phase) { hipLaunchKernelGGL(k_refresh, dim3(1), dim3(1024), 0, s, v, phase); }
 void sort(const View& v, hipStream_t s) {

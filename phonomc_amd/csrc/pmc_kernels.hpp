@@ -5,7 +5,8 @@
 
 namespace pmc_launch {
 int wave_blocks(int capacity);
-int partial_rows(int capacity);
+int event_split(int capacity);
+int partial_rows(const pmc_dev::View& v);
 void begin(const pmc_dev::View& v, hipStream_t s);
 void transport(const pmc_dev::View& v, hipStream_t s);
 void inject(const pmc_dev::View& v, hipStream_t s);

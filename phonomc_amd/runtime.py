@@ -53,7 +53,9 @@ class Mesh(C.Structure):
                 ("n_simplices", C.c_int32), ("simplex_vertices", C.POINTER(C.c_double)),
                 ("simplex_inverse", C.POINTER(C.c_double)), ("simplex_bounds", C.POINTER(C.c_double)),
                 ("simplex_volume", C.POINTER(C.c_double)), ("n_simplex_nodes", C.c_int32),
-                ("simplex_nodes", C.POINTER(BvhNode)), ("simplex_order", C.POINTER(C.c_int32))]
+                ("simplex_nodes", C.POINTER(BvhNode)), ("simplex_order", C.POINTER(C.c_int32)),
+                ("field_dims", C.c_int32 * 3), ("field_h", C.c_double), ("field_unit", C.c_double),
+                ("field", C.POINTER(C.c_uint8))]
 
 
 class Geometry(C.Structure):

@@ -169,3 +169,59 @@ def test_volume_sampling_is_uniform(finfet):
     expect = len(sub) / 1000.0
     chi2 = ((counts - expect) ** 2 / expect).sum()
     assert abs(chi2 - 999) < 5 * np.sqrt(2 * 999), chi2
+
+
+def point_triangle_distance(p, a, b, c):
+    """Distance from point p to each triangle (a, b, c rows): Voronoi regions
+    of the triangle (Ericson 5.1.5), vectorized over triangles."""
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = (ab * ap).sum(1), (ac * ap).sum(1)
+    bp, cp = p - b, p - c
+    d3, d4 = (ab * bp).sum(1), (ac * bp).sum(1)
+    d5, d6 = (ab * cp).sum(1), (ac * cp).sum(1)
+    vc, vb, va = d1 * d4 - d3 * d2, d5 * d2 - d1 * d6, d3 * d6 - d5 * d4
+    with np.errstate(divide="ignore", invalid="ignore"):
+        den = 1.0 / (va + vb + vc)
+        v, w = vb * den, vc * den
+        cases = [
+            (d1 <= 0) & (d2 <= 0), (d3 >= 0) & (d4 <= d3), (vc <= 0) & (d1 >= 0) & (d3 <= 0),
+            (d6 >= 0) & (d5 <= d6), (vb <= 0) & (d2 >= 0) & (d6 <= 0),
+            (va <= 0) & ((d4 - d3) >= 0) & ((d5 - d6) >= 0)]
+        vals = [(0, 0), (1, 0), (d1 / (d1 - d3), 0), (0, 1), (0, d2 / (d2 - d6)),
+                (1 - (d4 - d3) / ((d4 - d3) + (d5 - d6)), (d4 - d3) / ((d4 - d3) + (d5 - d6)))]
+    done = np.zeros(len(a), bool)
+    for cond, (vv, ww) in zip(cases, vals):
+        pick = cond & ~done
+        v = np.where(pick, vv, v)
+        w = np.where(pick, ww, w)
+        done |= cond
+    q = a + v[:, None] * ab + w[:, None] * ac
+    return np.linalg.norm(p - q, axis=1)
+
+
+def test_distance_field_bounds_the_surface_distance(mesh, finfet):
+    """Every voxel's stored bound is at most the brute-force distance from points
+    of that voxel to the nearest face, and the bound is not vacuous (it lets
+    most interior points skip the face search for an 18 A flight)."""
+    dims, h, unit = list(mesh.field_dims), mesh.field_h, mesh.field_unit
+    assert np.prod(dims) <= 4 << 20 and unit == 0.25
+    field = np.ctypeslib.as_array(mesh.field, shape=(int(np.prod(dims)),))
+    _, _, g, _, _, _ = finfet.views()
+    lo, hi = np.array(g.bmin[:]), np.array(g.bmax[:])
+    faces = rt.arr(mesh.faces, 3 * mesh.n_faces, np.int32).reshape(-1, 3)
+    v = rt.arr(mesh.vertices, 3 * mesh.n_vertices).reshape(-1, 3)
+    a, b, c = v[faces[:, 0]], v[faces[:, 1]], v[faces[:, 2]]
+    rng = np.random.default_rng(3)
+    pts = lo + (hi - lo) * rng.random((1500, 3))
+    # points within a few angstroms of faces as well
+    f = rng.integers(0, len(faces), 500)
+    r = rng.random((500, 2))
+    s = np.sqrt(r[:, :1])
+    near = (1 - s) * a[f] + s * (1 - r[:, 1:]) * b[f] + s * r[:, 1:] * c[f] + rng.normal(size=(500, 3)) * 3.0
+    pts = np.clip(np.concatenate([pts, near]), lo, hi)
+    idx = np.minimum(((pts - lo) / h).astype(int), np.array(dims) - 1)
+    bound = field[(idx[:, 0] * dims[1] + idx[:, 1]) * dims[2] + idx[:, 2]] * unit
+    dist = np.array([point_triangle_distance(p, a, b, c).min() for p in pts])
+    assert (bound <= dist + 1e-9).all(), np.max(bound - dist)
+    inside = O.contains_mesh(finfet, pts[:1500])
+    assert np.mean(bound[:1500][inside] > 18.0 + 0.01) > 0.6
