@@ -125,6 +125,9 @@ typedef struct pmc_mesh {
     int32_t field_dims[3];
     double field_h, field_unit;
     const uint8_t* field;
+    /* Per voxel with a positive bound: 1 inside the surface, 0 outside
+     * (contains_point of one voxel centre per connected region). */
+    const uint8_t* field_inside;
 } pmc_mesh;
 
 /* Geometry and cell grid; replaces SimulationDomain::trace_boundary_intersection

@@ -610,6 +610,34 @@ DistanceField build_distance_field(const SurfaceMesh& m, const Vec3& lo, const V
                 }
         }
     }
+    // Side of each face-connected region of positive-bound voxels.
+    f.inside.assign(f.bound.size(), 0);
+    std::vector<uint8_t> seen(f.bound.size(), 0);
+    std::vector<size_t> queue;
+    for (size_t start = 0; start < f.bound.size(); ++start) {
+        if (f.bound[start] == 0 || seen[start]) continue;
+        const int si = static_cast<int>(start / (static_cast<size_t>(ny) * nz));
+        const int sj = static_cast<int>((start / nz) % ny), sk = static_cast<int>(start % nz);
+        const Vec3 centre{lo[0] + (si + 0.5) * f.h, lo[1] + (sj + 0.5) * f.h, lo[2] + (sk + 0.5) * f.h};
+        const uint8_t side = m.contains_point(centre) ? 1 : 0;
+        queue.assign(1, start);
+        seen[start] = 1;
+        while (!queue.empty()) {
+            const size_t v = queue.back();
+            queue.pop_back();
+            f.inside[v] = side;
+            const int i = static_cast<int>(v / (static_cast<size_t>(ny) * nz));
+            const int j = static_cast<int>((v / nz) % ny), k = static_cast<int>(v % nz);
+            const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+            for (const auto& n : nb) {
+                if (n[0] < 0 || n[1] < 0 || n[2] < 0 || n[0] >= nx || n[1] >= ny || n[2] >= nz) continue;
+                const size_t w = (static_cast<size_t>(n[0]) * ny + n[1]) * nz + n[2];
+                if (f.bound[w] == 0 || seen[w]) continue;
+                seen[w] = 1;
+                queue.push_back(w);
+            }
+        }
+    }
     return f;
 }
 

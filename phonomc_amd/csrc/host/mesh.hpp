@@ -105,6 +105,10 @@ struct DistanceField {
     std::array<int, 3> dims{1, 1, 1};
     double h = 0, unit = 0;
     std::vector<uint8_t> bound;
+    // Voxels with a positive bound do not touch the surface, so each connected
+    // region of them lies on one side: 1 inside, 0 outside (by contains_point of
+    // one centre per region).
+    std::vector<uint8_t> inside;
 };
 DistanceField build_distance_field(const SurfaceMesh& mesh, const Vec3& lo, const Vec3& hi, long max_voxels,
                                    double unit);

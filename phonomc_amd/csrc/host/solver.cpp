@@ -222,6 +222,7 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         mv.field_h = a.field.h;
         mv.field_unit = a.field.unit;
         mv.field = a.field.bound.data();
+        mv.field_inside = a.field.inside.data();
         g.kind = 1;
         g.mesh = &p->mesh_view;
     }

@@ -273,12 +273,15 @@ int upload_mesh(pmc_ctx* ctx, const pmc_geometry* g) {
     v.face_nodes = ctx->upload(fn.data(), fn.size());
     v.n_face_nodes = M.n_face_nodes;
     v.field = nullptr;
+    v.field_inside = nullptr;
     if (M.field) {
         const size_t nv = static_cast<size_t>(M.field_dims[0]) * M.field_dims[1] * M.field_dims[2];
         if (M.field_dims[0] <= 0 || M.field_dims[1] <= 0 || M.field_dims[2] <= 0 || nv > (size_t(1) << 30) ||
             !(M.field_h > 0.0) || !(M.field_unit > 0.0))
             return fail(ctx, PMC_ERR_ARGUMENT, "malformed distance field");
         v.field = ctx->upload(reinterpret_cast<const unsigned char*>(M.field), nv);
+        v.field_inside =
+            M.field_inside ? ctx->upload(reinterpret_cast<const unsigned char*>(M.field_inside), nv) : nullptr;
         for (int a = 0; a < 3; ++a) v.field_dims[a] = M.field_dims[a];
         // Rounded so the device bound never exceeds the host's: a shorter unit
         // and a longer voxel edge.

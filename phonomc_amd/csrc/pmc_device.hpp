@@ -212,6 +212,7 @@ struct View {
     const BvhDev* face_nodes;
     int n_face_nodes;
     const unsigned char* field; // distance lower bounds (pmc_mesh field), or null
+    const unsigned char* field_inside; // side of each voxel with a positive bound
     int field_dims[3];
     float field_inv_h, field_unit;
     int event_split;            // k_events blocks per transport block on a mesh

@@ -216,17 +216,22 @@ __device__ int mesh_trace(const View& V, const double p[3], const double v[3], d
     return bf;
 }
 
-// Lower bound of the distance from p to every face, from the distance field
-// (0 without one).  FP32 voxel lookup; callers keep a margin of 2 mesh_tol.
-__device__ __forceinline__ float mesh_clearance(const View& V, const float p[3]) {
-    if (!V.field) return 0.0f;
+// Distance-field voxel of p (FP32 lookup, clamped to the grid).
+__device__ __forceinline__ size_t field_voxel(const View& V, const float p[3]) {
     int k[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const int i = static_cast<int>((p[a] - V.fbmin[a]) * V.field_inv_h);
         k[a] = i < 0 ? 0 : (i >= V.field_dims[a] ? V.field_dims[a] - 1 : i);
     }
-    return V.field[(static_cast<size_t>(k[0]) * V.field_dims[1] + k[1]) * V.field_dims[2] + k[2]] * V.field_unit;
+    return (static_cast<size_t>(k[0]) * V.field_dims[1] + k[1]) * V.field_dims[2] + k[2];
+}
+
+// Lower bound of the distance from p to every face, from the distance field
+// (0 without one).  Callers keep a margin of 2 mesh_tol for the FP32 lookup.
+__device__ __forceinline__ float mesh_clearance(const View& V, const float p[3]) {
+    if (!V.field) return 0.0f;
+    return V.field[field_voxel(V, p)] * V.field_unit;
 }
 
 // Whether a flight of `t` ps at FP32 speed `speed` from p stays clear of every
@@ -2402,6 +2407,15 @@ __global__ void __launch_bounds__(256) k_containment(View V) {
             return;
         }
     if (V.geom_kind == 1) {
+        // A voxel clear of the surface answers for all its points; the full test
+        // runs near the surface only.
+        if (V.field_inside) {
+            const size_t vox = field_voxel(V, p);
+            if (V.field[vox] * V.field_unit > static_cast<float>(2.0 * V.mesh_tol)) {
+                if (!V.field_inside[vox]) raise_error(V, PMC_ERR_ESCAPED, i, p, -1);
+                return;
+            }
+        }
         const double pd[3] = {p[0], p[1], p[2]};
         if (!mesh_contains(V, pd)) raise_error(V, PMC_ERR_ESCAPED, i, p, -1);
     }

@@ -55,7 +55,7 @@ class Mesh(C.Structure):
                 ("simplex_volume", C.POINTER(C.c_double)), ("n_simplex_nodes", C.c_int32),
                 ("simplex_nodes", C.POINTER(BvhNode)), ("simplex_order", C.POINTER(C.c_int32)),
                 ("field_dims", C.c_int32 * 3), ("field_h", C.c_double), ("field_unit", C.c_double),
-                ("field", C.POINTER(C.c_uint8))]
+                ("field", C.POINTER(C.c_uint8)), ("field_inside", C.POINTER(C.c_uint8))]
 
 
 class Geometry(C.Structure):

@@ -225,3 +225,22 @@ def test_distance_field_bounds_the_surface_distance(mesh, finfet):
     assert (bound <= dist + 1e-9).all(), np.max(bound - dist)
     inside = O.contains_mesh(finfet, pts[:1500])
     assert np.mean(bound[:1500][inside] > 18.0 + 0.01) > 0.6
+
+
+def test_distance_field_sides_match_containment(mesh, finfet):
+    """Voxels clear of the surface carry the side of the surface they lie on:
+    the flag equals the oracle's contains_point at random points of them."""
+    dims, h = list(mesh.field_dims), mesh.field_h
+    n = int(np.prod(dims))
+    bound = np.ctypeslib.as_array(mesh.field, shape=(n,))
+    inside = np.ctypeslib.as_array(mesh.field_inside, shape=(n,))
+    _, _, g, _, _, _ = finfet.views()
+    lo = np.array(g.bmin[:])
+    rng = np.random.default_rng(5)
+    clear = np.flatnonzero(bound > 0)
+    pick = rng.choice(clear, 3000, replace=False)
+    ijk = np.stack([pick // (dims[1] * dims[2]), (pick // dims[2]) % dims[1], pick % dims[2]], axis=1)
+    pts = lo + (ijk + rng.random((len(pick), 3))) * h
+    want = O.contains_mesh(finfet, pts)
+    assert np.array_equal(inside[pick].astype(bool), want)
+    assert 0.05 < want.mean() < 0.95  # both sides are present (the bounding box holds outside space)
