@@ -330,6 +330,28 @@ int pmc_set_timing(pmc_ctx* ctx, int32_t enabled);
 void pmc_destroy(pmc_ctx* ctx);
 const char* pmc_status_string(int32_t code);
 
+/* Batched stepping of independent instances.  No counterpart in the reference,
+ * which runs one MonteCarloSolver per process (src/main.cpp:86-102); each step
+ * of the batch is the reference's run_timestep (src/MonteCarloSolver.cpp:348-398)
+ * applied to every instance.  A batch launches each kernel of the step once
+ * over all instances (per-instance block ranges) and replays the step as a
+ * captured hipGraph.  The contexts keep their own tables, carriers, ledgers
+ * and errors and stay readable with pmc_read_*; results equal those of
+ * stepping each context alone.  Every context must be initialized, at the
+ * same step, with the same geometry kind and collision model, without a heat
+ * source; tuning changes after pmc_batch_create are not seen by the batch.
+ * While a batch exists, step its contexts only through it. */
+typedef struct pmc_batch pmc_batch;
+int pmc_batch_create(pmc_ctx* const* contexts, int32_t n, pmc_batch** out);
+/* Advances every instance by nsteps.  On a failure the failing context is
+ * latched with its message (pmc_last_error) and the batch returns its code;
+ * pmc_batch_failed names the first failed instance. */
+int pmc_batch_step(pmc_batch* batch, int32_t nsteps);
+int pmc_batch_failed(const pmc_batch* batch);
+/* 1 (default): replay the captured step graph; 0: launch kernel by kernel. */
+int pmc_batch_set_graph(pmc_batch* batch, int32_t enabled);
+void pmc_batch_destroy(pmc_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

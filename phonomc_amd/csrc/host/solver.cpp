@@ -7,10 +7,49 @@
 #include <fstream>
 #include <iomanip>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
 namespace pmc {
+
+namespace {
+// Material tables depend only on the material, the lookup spacing and the
+// stationary-mode policy; the temperature lookup table alone takes about half a
+// second.  Problems built in one process (a batch of instances) share them.
+struct MaterialEntry {
+    RawMaterial raw;
+    ModeTables modes;
+};
+
+std::shared_ptr<const MaterialEntry> cached_material(const std::string& spec, const std::string& input_directory,
+                                                     double lookup_dt, bool include_stationary) {
+    namespace fs = std::filesystem;
+    std::string key = spec + '\n' + std::to_string(lookup_dt) + '\n' + (include_stationary ? "1" : "0");
+    if (spec.rfind("synthetic:", 0) != 0) { // a file: its path and modification time
+        fs::path path(spec);
+        if (path.is_relative()) path = fs::path(input_directory) / path;
+        std::error_code ec;
+        const auto stamp = fs::last_write_time(fs::is_directory(path, ec) ? path / "material.pmcmat" : path, ec);
+        key += '\n' + fs::absolute(path, ec).string() + '\n' +
+               std::to_string(ec ? 0 : static_cast<long long>(stamp.time_since_epoch().count()));
+    }
+    static std::mutex lock;
+    static std::map<std::string, std::shared_ptr<const MaterialEntry>> cache;
+    {
+        std::lock_guard<std::mutex> g(lock);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    auto entry = std::make_shared<MaterialEntry>();
+    entry->raw = load_material(spec, input_directory);
+    entry->modes = build_mode_tables(entry->raw, lookup_dt, include_stationary);
+    std::lock_guard<std::mutex> g(lock);
+    if (cache.size() >= 16) cache.clear(); // bounded: a process rarely uses more materials
+    return cache.emplace(key, std::move(entry)).first->second;
+}
+} // namespace
 
 std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions& opt) {
     if (c.collision == CollisionModel::Callaway && c.callaway_nonlinear)
@@ -33,10 +72,11 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
 
     auto p = std::make_unique<Problem>();
     p->config = c;
-    p->raw = load_material(c.material_folder, c.input_directory);
     // Stationary modes join the active bank under conservative transport
     // (src/material/MaterialFactory.cpp:51-55).
-    p->modes = build_mode_tables(p->raw, c.temperature_lookup_dt, linearized);
+    const auto material = cached_material(c.material_folder, c.input_directory, c.temperature_lookup_dt, linearized);
+    p->raw = material->raw;
+    p->modes = material->modes;
     p->linearized = linearized;
     p->domain = build_domain(c);
     const auto& m = p->modes;

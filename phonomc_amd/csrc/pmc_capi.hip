@@ -240,11 +240,6 @@ int upload_mesh(pmc_ctx* ctx, const pmc_geometry* g) {
     const pmc_mesh& M = *g->mesh;
     if (M.n_faces <= 0 || M.n_face_nodes <= 0 || !M.face_nodes || !M.face_order)
         return fail(ctx, PMC_ERR_ARGUMENT, "mesh without faces or face hierarchy");
-    // FP32 positions carry about 2^-24 of the largest coordinate; eight of those
-    // quanta are taken as on the surface.
-    double reach = 0.0;
-    for (int a = 0; a < 3; ++a) reach = std::max({reach, std::abs(g->bmin[a]), std::abs(g->bmax[a])});
-    v.mesh_tol = std::max(1e-7, 8.0 * std::ldexp(reach, -24));
     const int nf = M.n_faces;
     std::vector<double4> plane(nf);
     std::vector<double> tri(9 * static_cast<size_t>(nf)), box(6 * static_cast<size_t>(nf));
@@ -453,6 +448,11 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
         return fail(ctx, PMC_ERR_UNSUPPORTED, "geometry must be the analytic box or a triangle mesh");
     View& v = ctx->view;
     v.geom_kind = g->kind;
+    // FP32 positions carry about 2^-24 of the largest coordinate; eight of those
+    // quanta are taken as on the surface (box walls or mesh faces).
+    double reach = 0.0;
+    for (int a = 0; a < 3; ++a) reach = std::max({reach, std::abs(g->bmin[a]), std::abs(g->bmax[a])});
+    v.mesh_tol = std::max(1e-7, 8.0 * std::ldexp(reach, -24));
     for (int a = 0; a < 3; ++a) {
         v.bmin[a] = g->bmin[a];
         v.bmax[a] = g->bmax[a];
@@ -991,6 +991,207 @@ void pmc_destroy(pmc_ctx* ctx) {
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- batches
+struct pmc_batch {
+    std::vector<pmc_ctx*> ctx;
+    hipStream_t stream = nullptr;
+    View* views[2] = {};                    // device copies of the instances' views, by carrier buffer
+    int* offs[pmc_launch::kBatchKinds] = {}; // device prefix offsets of block counts
+    int total[pmc_launch::kBatchKinds] = {};
+    int* errors = nullptr;                  // device: error word per instance
+    bool mesh = false, callaway = false;
+    bool use_graph = true;
+    hipGraphExec_t exec[2] = {};
+    int failed = -1;
+    std::vector<void*> allocations;
+};
+
+namespace {
+template <class T>
+T* batch_upload(pmc_batch* b, const T* host, size_t n) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    b->allocations.push_back(p);
+    if (hipMemcpy(p, host, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return static_cast<T*>(p);
+}
+
+// The step's kernels after the optional containment check, for the instances'
+// carrier buffer `cur` (the sort writes the other one).
+void batch_sequence(pmc_batch* b, int cur) {
+    using namespace pmc_launch;
+    const int n = static_cast<int>(b->ctx.size());
+    auto run = [&](int kind, int buf) { batch_launch(kind, b->mesh, b->views[buf], b->offs[kind], n, b->total[kind], b->stream); };
+    run(kBatchBegin, cur);
+    run(kBatchTransport, cur);
+    run(kBatchEvents, cur);
+    run(kBatchInject, cur);
+    run(kBatchRefresh1, cur);
+    run(kBatchSort, cur);
+    run(kBatchCellPrep, 1 - cur);
+    if (b->callaway) {
+        run(kBatchCallaway, 1 - cur);
+    } else {
+        run(kBatchMoments, 1 - cur);
+        run(kBatchNewton, 1 - cur);
+        run(kBatchRelax, 1 - cur);
+    }
+    run(kBatchRefresh2, 1 - cur);
+}
+
+// Reads every instance's error word; latches the first failed context.
+int batch_check(pmc_batch* b) {
+    const int n = static_cast<int>(b->ctx.size());
+    std::vector<int> codes(n, 0);
+    pmc_launch::batch_errors(b->views[0], n, b->errors, b->stream);
+    hipError_t e = hipMemcpyAsync(codes.data(), b->errors, n * sizeof(int), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        b->failed = 0;
+        for (pmc_ctx* c : b->ctx) c->latched = true;
+        return fail(b->ctx[0], PMC_ERR_DEVICE, std::string("batch: ") + hipGetErrorString(e));
+    }
+    for (int i = 0; i < n; ++i)
+        if (codes[i]) {
+            b->failed = i;
+            return collect_error(b->ctx[i]);
+        }
+    return PMC_OK;
+}
+} // namespace
+
+extern "C" {
+
+int pmc_batch_create(pmc_ctx* const* contexts, int32_t n, pmc_batch** out) {
+    if (!contexts || n <= 0 || !out) return PMC_ERR_ARGUMENT;
+    *out = nullptr;
+    pmc_ctx* c0 = contexts[0];
+    if (!c0) return PMC_ERR_ARGUMENT;
+    for (int i = 0; i < n; ++i) {
+        pmc_ctx* c = contexts[i];
+        if (!c) return PMC_ERR_ARGUMENT;
+        if (c->latched) return PMC_ERR_LATCHED;
+        if (!c->initialized) return fail(c, PMC_ERR_ARGUMENT, "batch: carriers not initialized");
+        if (c->view.hs_on) return fail(c, PMC_ERR_UNSUPPORTED, "batch: instances with a heat source are not batched");
+        if (c->view.geom_kind != c0->view.geom_kind || c->view.callaway != c0->view.callaway)
+            return fail(c, PMC_ERR_ARGUMENT, "batch: instances differ in geometry kind or collision model");
+        if (c->cur != c0->cur || c->host_step != c0->host_step)
+            return fail(c, PMC_ERR_ARGUMENT, "batch: instances are at different steps");
+        for (int j = 0; j < i; ++j)
+            if (contexts[j] == c) return fail(c, PMC_ERR_ARGUMENT, "batch: a context appears twice");
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, PMC_ERR_DEVICE, "batch: stream sync");
+    }
+    auto* b = new pmc_batch();
+    b->ctx.assign(contexts, contexts + n);
+    b->mesh = c0->view.geom_kind == 1;
+    b->callaway = c0->view.callaway != 0;
+    auto bail = [&](const char* what) {
+        pmc_batch_destroy(b);
+        return fail(c0, PMC_ERR_DEVICE, std::string("batch: ") + what);
+    };
+    if (hipStreamCreate(&b->stream) != hipSuccess) return bail("stream");
+    for (int cur = 0; cur < 2; ++cur) {
+        std::vector<View> host(n);
+        for (int i = 0; i < n; ++i) {
+            const int keep = b->ctx[i]->cur;
+            b->ctx[i]->cur = cur;
+            host[i] = view(b->ctx[i]);
+            b->ctx[i]->cur = keep;
+            view(b->ctx[i]);
+        }
+        if (!(b->views[cur] = batch_upload(b, host.data(), host.size()))) return bail("views");
+    }
+    for (int k = 0; k < pmc_launch::kBatchKinds; ++k) {
+        std::vector<int> offs(n + 1, 0);
+        // Each instance gets exactly its single-launch grid.  The kernels that map
+        // blocks to XCDs have counts that are multiples of 8, so their ranges
+        // start at multiples of 8 and keep the mapping.
+        for (int i = 0; i < n; ++i) offs[i + 1] = offs[i] + pmc_launch::batch_blocks(b->ctx[i]->view, k);
+        b->total[k] = offs[n];
+        if (!(b->offs[k] = batch_upload(b, offs.data(), offs.size()))) return bail("offsets");
+    }
+    std::vector<int> zero(n, 0);
+    if (!(b->errors = batch_upload(b, zero.data(), zero.size()))) return bail("error words");
+    *out = b;
+    return PMC_OK;
+}
+
+int pmc_batch_set_graph(pmc_batch* b, int32_t enabled) {
+    if (!b) return PMC_ERR_ARGUMENT;
+    b->use_graph = enabled != 0;
+    return PMC_OK;
+}
+
+int pmc_batch_failed(const pmc_batch* b) { return b ? b->failed : -1; }
+
+int pmc_batch_step(pmc_batch* b, int32_t nsteps) {
+    if (!b) return PMC_ERR_ARGUMENT;
+    if (b->failed >= 0) return PMC_ERR_LATCHED;
+    if (nsteps < 0) return fail(b->ctx[0], PMC_ERR_ARGUMENT, "negative step count");
+    const int n = static_cast<int>(b->ctx.size());
+    for (int s = 0; s < nsteps; ++s) {
+        pmc_ctx* c0 = b->ctx[0];
+        const int cur = c0->cur;
+        if (c0->host_step % 100 == 0)
+            pmc_launch::batch_launch(pmc_launch::kBatchContainment, b->mesh, b->views[cur],
+                                     b->offs[pmc_launch::kBatchContainment], n,
+                                     b->total[pmc_launch::kBatchContainment], b->stream);
+        if (b->use_graph) {
+            if (!b->exec[cur]) { // capture this buffer parity's step once
+                hipGraph_t g = nullptr;
+                hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    batch_sequence(b, cur);
+                    e = hipStreamEndCapture(b->stream, &g);
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[cur], g, nullptr, nullptr, 0);
+                if (g) (void)hipGraphDestroy(g);
+                if (e != hipSuccess) {
+                    b->failed = 0;
+                    for (pmc_ctx* c : b->ctx) c->latched = true;
+                    return fail(c0, PMC_ERR_DEVICE, std::string("batch graph: ") + hipGetErrorString(e));
+                }
+            }
+            if (hipGraphLaunch(b->exec[cur], b->stream) != hipSuccess) {
+                b->failed = 0;
+                for (pmc_ctx* c : b->ctx) c->latched = true;
+                return fail(c0, PMC_ERR_DEVICE, "batch graph launch failed");
+            }
+        } else {
+            batch_sequence(b, cur);
+        }
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            b->failed = 0;
+            for (pmc_ctx* c : b->ctx) c->latched = true;
+            return fail(c0, PMC_ERR_DEVICE, std::string("batch launch: ") + hipGetErrorString(e));
+        }
+        for (pmc_ctx* c : b->ctx) {
+            c->cur = 1 - c->cur;
+            view(c);
+            c->host_step++;
+            c->elapsed_ps += c->view.dt;
+        }
+        if (c0->host_step % c0->error_interval == 0 && s != nsteps - 1) {
+            const int rc = batch_check(b);
+            if (rc) return rc;
+        }
+    }
+    return batch_check(b);
+}
+
+void pmc_batch_destroy(pmc_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto& x : b->exec)
+        if (x) (void)hipGraphExecDestroy(x);
+    for (void* p : b->allocations) (void)hipFree(p);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
 }
 
 } // extern "C"

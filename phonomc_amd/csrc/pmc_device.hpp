@@ -203,7 +203,7 @@ struct View {
     const double* cell_vol;
     // triangle mesh (geom_kind 1); faces and star tetrahedra in BVH leaf order
     int geom_kind;
-    double mesh_tol;          // FP32 positions within this distance of the surface are on it (A)
+    double mesh_tol;          // FP32 positions within this distance of a wall or face are on it (A)
     int nfaces;
     const double4* face_plane; // outward normal, k: n.x + k = 0
     const double* face_tri;    // 9 per face
@@ -286,6 +286,17 @@ struct View {
     const double4* cw_pack; // 2 per mode: (g_N, g_R, B0, B1), (B2, B3, hw, 0); B = sqrt(C) (1, q / w)
     double cw_cbg;          // heat-capacity density at T_bg, eV/(K A^3)
 };
+
+// Transport blocks for a slot capacity (a multiple of 8, one per XCD round).
+__host__ __device__ inline int wave_blocks_for(int capacity) {
+    const int b = (capacity + kSlotsPerBlock - 1) / kSlotsPerBlock;
+    return (b + 7) & ~7;
+}
+
+// Event blocks: one per transport block, event_split per transport block on a mesh.
+__host__ __device__ inline int event_rows_for(const View& v) {
+    return wave_blocks_for(v.capacity) * (v.geom_kind == 1 ? v.event_split : 1);
+}
 
 // Linearized Callaway kernel: one workgroup of kCwThreads per cell, up to
 // kCwPer carriers per thread held in registers.

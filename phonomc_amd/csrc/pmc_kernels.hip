@@ -430,15 +430,37 @@ __device__ __forceinline__ void raise_advance_error(const View& V, int rc, long 
     else raise_error(V, -rc, slot, p, -1);
 }
 
-// Logical block index grouping consecutive carrier chunks on one XCD (blocks are
-// dispatched round-robin over the 8 XCDs of an MI355X).
-__device__ __forceinline__ int xcd_block() {
-    const int g = gridDim.x, b = blockIdx.x;
-    if ((g & 7) != 0) return b;
-    return (b & 7) * (g >> 3) + (b >> 3);
+// A block's place in its instance's launch: G.b of G.n blocks.  A
+// single-instance launch is the hardware grid; a batched launch covers many
+// instances, each with a range of blocks that starts at a multiple of 8.
+struct Grid {
+    int b, n;
+};
+
+__device__ __forceinline__ Grid launch_grid() { return {static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x)}; }
+
+// Batched launch: the instance owning this block, from the prefix offsets of
+// the instances' block counts (an instance with no blocks owns none).
+__device__ __forceinline__ int batch_instance(const int* __restrict__ offs, int n, Grid& G) {
+    const int b = static_cast<int>(blockIdx.x);
+    int lo = 0, hi = n - 1; // last instance whose range starts at or before b
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (offs[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    G = {b - offs[lo], offs[lo + 1] - offs[lo]};
+    return lo;
 }
 
-__device__ __forceinline__ int carrier_slot() { return xcd_block() * blockDim.x + threadIdx.x; }
+// Logical block index grouping consecutive carrier chunks on one XCD (blocks are
+// dispatched round-robin over the 8 XCDs of an MI355X).
+__device__ __forceinline__ int xcd_block(const Grid G) {
+    if ((G.n & 7) != 0) return G.b;
+    return (G.b & 7) * (G.n >> 3) + (G.b >> 3);
+}
+
+__device__ __forceinline__ int carrier_slot(const Grid G) { return xcd_block(G) * blockDim.x + threadIdx.x; }
 
 // ---------------------------------------------------------------- boundary events
 // RoughBoundarySampler::sample (src/solver/RoughBoundarySampler.cpp:459-533)
@@ -674,8 +696,8 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
 // when some lane's cell changes.  Rare scalar events (absorption, injection,
 // rough reflections) go to LDS and leave as one partial row per block.
 
-__device__ __forceinline__ int wave_global() {
-    return xcd_block() * (blockDim.x >> 6) + (threadIdx.x >> 6);
+__device__ __forceinline__ int wave_global(const Grid G) {
+    return xcd_block(G) * (blockDim.x >> 6) + (threadIdx.x >> 6);
 }
 
 template <int K, int STRIDE>
@@ -824,7 +846,7 @@ struct BlockHash {
 // (one-to-one refill), or floor(expected) plus one with probability
 // frac(expected) (ReservoirSampler::sample_flux_emission_count,
 // src/solver/ReservoirSampler.cpp:117-125; Philox stream 4, counter (reservoir, step)).
-__global__ void k_begin(View V) {
+__device__ __forceinline__ void begin_body(const View& V, const Grid G) {
     DevState* st = V.st;
     const int t = threadIdx.x;
     if (t < kMaxReservoirs) {
@@ -858,17 +880,25 @@ __global__ void k_begin(View V) {
     }
 }
 
+__global__ void k_begin(View V) { begin_body(V, launch_grid()); }
+
+__global__ void kb_begin(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    begin_body(views[i], G);
+}
+
 // Free flight of every carrier that reaches no wall within the step: the first
 // iteration of advance() when it ends without an event, bit for bit.  Carriers
 // that do reach a wall are listed per block for k_events and left untouched.
 // Counts carriers per new cell for the cell sort.
 template <bool kMesh>
-__global__ void __launch_bounds__(256) k_transport(View V) {
+__device__ __forceinline__ void transport_body(const View& V, const Grid G) {
     __shared__ int ev_n;
     if (threadIdx.x == 0) ev_n = 0;
     PMC_BLOCK_HASH(H);
     const int n = V.st->n_slots_start;
-    const int blk = xcd_block();
+    const int blk = xcd_block(G);
     const int base = (blk * kWavesPerBlock + (threadIdx.x >> 6)) * 64 * kRowsPerWave;
     const int lane = __lane_id();
     int* queue = V.events + static_cast<size_t>(blk) * kSlotsPerBlock;
@@ -957,11 +987,21 @@ __global__ void __launch_bounds__(256) k_transport(View V) {
     if (threadIdx.x == 0) V.block_events[blk] = ev_n;
 }
 
+template <bool kMesh>
+__global__ void __launch_bounds__(256) k_transport(View V) { transport_body<kMesh>(V, launch_grid()); }
+
+template <bool kMesh>
+__global__ void __launch_bounds__(256) kb_transport(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    transport_body<kMesh>(views[i], G);
+}
+
 // Carriers with wall events this step: the whole event loop of
 // TransportOperator::advance_particle (src/solver/TransportOperator.cpp:405-466)
 // for the slots k_transport listed.  One block per transport block.
 template <bool kMesh>
-__global__ void __launch_bounds__(256) k_events(View V) {
+__device__ __forceinline__ void events_body(const View& V, const Grid G) {
     // On a mesh, event_split blocks share one transport block's list when there
     // are few transport blocks: the face searches are latency-bound and need
     // the extra waves.
@@ -970,7 +1010,7 @@ __global__ void __launch_bounds__(256) k_events(View V) {
     BlockPartials P{sp};
     P.zero();
     PMC_BLOCK_HASH(H);
-    const int row = xcd_block();
+    const int row = xcd_block(G);
     const int blk = row / kSplit, part = row % kSplit;
     const int cnt = V.block_events[blk];
     const int* queue = V.events + static_cast<size_t>(blk) * kSlotsPerBlock;
@@ -1011,11 +1051,23 @@ __global__ void __launch_bounds__(256) k_events(View V) {
     P.write(V, row);
 }
 
+template <bool kMesh>
+__global__ void __launch_bounds__(256) k_events(View V) { events_body<kMesh>(V, launch_grid()); }
+
+template <bool kMesh>
+__global__ void __launch_bounds__(256) kb_events(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    events_body<kMesh>(views[i], G);
+}
+
 // Reservoir injection (ReservoirInjector::inject, src/solver/ReservoirInjector.cpp:62-132)
 // followed by the remaining flight of each injected carrier
 // (TransportOperator::advance_injected, src/solver/TransportOperator.cpp:134-164).
 template <bool kMesh>
-__global__ void __launch_bounds__(256) k_inject(View V, int row0) {
+__device__ __forceinline__ void inject_body(const View& V, const Grid G) {
+    const int row0 = event_rows_for(V); // partial rows after the event blocks
+
     __shared__ double sp[kPartialCols];
     BlockPartials P{sp};
     P.zero();
@@ -1024,7 +1076,7 @@ __global__ void __launch_bounds__(256) k_inject(View V, int row0) {
     const int total = st->emit_total;
     const int base_slot = st->n_slots_start;
     const uint32_t step = static_cast<uint32_t>(st->step);
-    for (int k0 = blockIdx.x * blockDim.x; k0 < total; k0 += gridDim.x * blockDim.x) {
+    for (int k0 = G.b * blockDim.x; k0 < total; k0 += G.n * blockDim.x) {
         const int k = k0 + threadIdx.x;
         const bool active = k < total;
         Tally T{};
@@ -1115,7 +1167,17 @@ __global__ void __launch_bounds__(256) k_inject(View V, int row0) {
         H.count(alive, cell, V.cell_count);
     }
     H.flush_counts(V.cell_count);
-    P.write(V, row0 + blockIdx.x);
+    P.write(V, row0 + G.b);
+}
+
+template <bool kMesh>
+__global__ void __launch_bounds__(256) k_inject(View V) { inject_body<kMesh>(V, launch_grid()); }
+
+template <bool kMesh>
+__global__ void __launch_bounds__(256) kb_inject(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    inject_body<kMesh>(views[i], G);
 }
 
 __device__ double lut_inverse(const View& V, double e) {
@@ -1180,7 +1242,7 @@ __device__ double block_sum(double x, double* red) {
 //            src/solver/BoundaryLedger.cpp:86-115) + counts -> offsets
 //   phase 2: cell energies after relaxation -> temperatures, energy ledger closure
 //   phase 3: initial cell energies -> temperatures and thermal energy
-__global__ void __launch_bounds__(1024) k_refresh(View V, int phase) {
+__device__ __forceinline__ void refresh_body(const View& V, const Grid G, int phase) {
     DevState* st = V.st;
     __shared__ double red[1024];
     __shared__ int scan[1024];
@@ -1300,18 +1362,26 @@ __global__ void __launch_bounds__(1024) k_refresh(View V, int phase) {
     }
 }
 
+__global__ void __launch_bounds__(1024) k_refresh(View V, int phase) { refresh_body(V, launch_grid(), phase); }
+
+__global__ void __launch_bounds__(1024) kb_refresh(const View* __restrict__ views, const int* __restrict__ offs, int n, int phase) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    refresh_body(views[i], G, phase);
+}
+
 // Counting-sort scatter of the live carriers into cell order (destination
 // buffer), dropping absorbed slots: ParticleStorage::compact_alive
 // (src/solver/ParticleStorage.cpp:75-105) and the cell buckets of
 // CellParticleIndex in one pass.  Each block ranks its carriers per cell in
 // LDS, reserves one destination range per (block, cell) from the global cell
 // cursor, then streams the carriers out.
-__global__ void __launch_bounds__(256) k_sort(View V) {
+__device__ __forceinline__ void sort_body(const View& V, const Grid G) {
     __shared__ double esum[kHashSlots];
     for (int t = threadIdx.x; t < kHashSlots; t += blockDim.x) esum[t] = 0.0;
     PMC_BLOCK_HASH(H);
     const int n = V.st->n_slots;
-    const int base = (xcd_block() * kWavesPerBlock + (threadIdx.x >> 6)) * 64 * kSortRows;
+    const int base = (xcd_block(G) * kWavesPerBlock + (threadIdx.x >> 6)) * 64 * kSortRows;
     const int lane = __lane_id();
     int mw[kSortRows], cl[kSortRows];
     double oc[kSortRows];
@@ -1396,6 +1466,14 @@ __global__ void __launch_bounds__(256) k_sort(View V) {
         V.o_cell[dst] = cl[r];
         V.o_uid[dst] = id[r];
     }
+}
+
+__global__ void __launch_bounds__(256) k_sort(View V) { sort_body(V, launch_grid()); }
+
+__global__ void __launch_bounds__(256) kb_sort(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    sort_body(views[i], G);
 }
 
 // Sums v[k] over the block of W waves; every thread receives the same totals
@@ -1483,16 +1561,24 @@ __device__ __forceinline__ CellRta cell_rta(const View& V, int c) {
 }
 
 // Per-cell constants of the collision step, once per cell after the sort.
-__global__ void k_cell_prep(View V) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cell_prep_body(const View& V, const Grid G) {
+    const int c = G.b * blockDim.x + threadIdx.x;
     if (c < V.ncells) V.crta[c] = cell_rta(V, c);
+}
+
+__global__ void k_cell_prep(View V) { cell_prep_body(V, launch_grid()); }
+
+__global__ void kb_cell_prep(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    cell_prep_body(views[i], G);
 }
 
 // Pseudo-temperature sums of one chunk at the cell temperature (terms of
 // rta_accumulate), written as the chunk's partial row.
-__global__ void __launch_bounds__(kChunkThreads) k_moments(View V) {
+__device__ __forceinline__ void moments_body(const View& V, const Grid G) {
     __shared__ double lds[kAccCols * kChunkWaves];
-    const int b = blockIdx.x;
+    const int b = G.b;
     if (b >= V.chunk_start[V.ncells]) return;
     const int c = V.chunk_cell[b];
     const int begin = V.cell_start[c] + (b - V.chunk_start[c]) * kChunk;
@@ -1525,6 +1611,14 @@ __global__ void __launch_bounds__(kChunkThreads) k_moments(View V) {
 #pragma unroll
         for (int k = 0; k < kAccCols; ++k) out[k] = acc[k];
     }
+}
+
+__global__ void __launch_bounds__(kChunkThreads) k_moments(View V) { moments_body(V, launch_grid()); }
+
+__global__ void __launch_bounds__(kChunkThreads) kb_moments(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    moments_body(views[i], G);
 }
 
 // Exact pseudo-temperature sums of one cell over its carriers (the evaluate
@@ -1583,8 +1677,8 @@ __device__ __forceinline__ bool series_sums(const double (&acc)[kAccCols], doubl
 // sums; f(high) is summed exactly only when its upper bound is not negative;
 // f(T) and f'(T) come from the series where trusted, otherwise from the cell's
 // carriers.
-__global__ void __launch_bounds__(256) k_newton(View V) {
-    const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+__device__ __forceinline__ void newton_body(const View& V, const Grid G) {
+    const int c = G.b * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (c >= V.ncells) return;
     const int begin = V.cell_start[c], end = V.cell_start[c + 1];
     if (end <= begin) return;
@@ -1650,12 +1744,20 @@ __global__ void __launch_bounds__(256) k_newton(View V) {
     }
 }
 
+__global__ void __launch_bounds__(256) k_newton(View V) { newton_body(V, launch_grid()); }
+
+__global__ void __launch_bounds__(256) kb_newton(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    newton_body(views[i], G);
+}
+
 // Relaxation n <- max(0, n + a (n_BE(T_R) - n)) and residual
 // (src/solver/RtaCollisionOperator.cpp:167-183) of one chunk, with the
 // deviational energy after relaxation for the following refresh.
-__global__ void __launch_bounds__(kChunkThreads) k_relax(View V) {
+__device__ __forceinline__ void relax_body(const View& V, const Grid G) {
     __shared__ double lds[2 * kChunkWaves];
-    const int b = blockIdx.x;
+    const int b = G.b;
     if (b >= V.chunk_start[V.ncells]) return;
     const int c = V.chunk_cell[b];
     const int begin = V.cell_start[c] + (b - V.chunk_start[c]) * kChunk;
@@ -1700,6 +1802,14 @@ __global__ void __launch_bounds__(kChunkThreads) k_relax(View V) {
         V.chunk_out[2 * static_cast<size_t>(b)] = r[0];
         V.chunk_out[2 * static_cast<size_t>(b) + 1] = r[1];
     }
+}
+
+__global__ void __launch_bounds__(kChunkThreads) k_relax(View V) { relax_body(V, launch_grid()); }
+
+__global__ void __launch_bounds__(kChunkThreads) kb_relax(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    relax_body(views[i], G);
 }
 
 // ---------------------------------------------------------------- Callaway
@@ -1821,10 +1931,10 @@ __device__ void cw_projector(const double (&G)[4][4], double (&P)[4][4]) {
         }
 }
 
-__global__ void __launch_bounds__(kCwThreads) k_callaway(View V) {
+__device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
     __shared__ double lds[18 * kCwWaves + 18];
     __shared__ double Psh[17]; // P_N row-major and 1/G_R
-    const int c = blockIdx.x;
+    const int c = G.b;
     const int begin = V.cell_start[c], end = V.cell_start[c + 1];
     const int n = end - begin;
     const int k0 = V.chunk_start[c], k1 = V.chunk_start[c + 1];
@@ -1993,6 +2103,14 @@ __global__ void __launch_bounds__(kCwThreads) k_callaway(View V) {
         V.chunk_out[2 * static_cast<size_t>(k0)] = out[0];
         V.chunk_out[2 * static_cast<size_t>(k0) + 1] = out[1];
     }
+}
+
+__global__ void __launch_bounds__(kCwThreads) k_callaway(View V) { callaway_body(V, launch_grid()); }
+
+__global__ void __launch_bounds__(kCwThreads) kb_callaway(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    callaway_body(views[i], G);
 }
 
 // ---------------------------------------------------------------- heat source
@@ -2360,7 +2478,7 @@ __global__ void k_zero_flux(View V) {
 // Heat-flux tally (GridStatistics::update_heat_flux, src/solver/GridStatistics.cpp:269-347).
 __global__ void __launch_bounds__(256) k_flux(View V) {
     const int n = V.st->n_slots;
-    const int base = wave_global() * 64 * kRowsPerWave;
+    const int base = wave_global(launch_grid()) * 64 * kRowsPerWave;
     const int lane = __lane_id();
     SegAcc<3, 3> acc;
     acc.reset();
@@ -2393,14 +2511,16 @@ __global__ void k_flux_finish(View V) {
 
 // Domain membership check (TransportOperator::validate_containment,
 // src/solver/TransportOperator.cpp:364-402).
-__global__ void __launch_bounds__(256) k_containment(View V) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void containment_body(const View& V, const Grid G) {
+    const int i = G.b * blockDim.x + threadIdx.x;
     const int n = V.st->n_slots;
     if (i >= n || V.mode[i] < 0) return;
     const float p[3] = {V.px[i], V.py[i], V.pz[i]};
     double scale = 1.0;
     for (int a = 0; a < 3; ++a) scale = fmax(scale, fabs(V.bmax[a] - V.bmin[a]));
-    const double tol = 1e-10 * scale;
+    // The reference's 1e-10 of the extent, widened to the FP32 position quantum:
+    // carriers are clamped to the FP32-rounded walls, which can lie that far out.
+    const double tol = fmax(1e-10 * scale, V.mesh_tol);
     for (int a = 0; a < 3; ++a)
         if (!isfinite(p[a]) || p[a] < V.bmin[a] - tol || p[a] > V.bmax[a] + tol) {
             raise_error(V, PMC_ERR_ESCAPED, i, p, -1);
@@ -2419,6 +2539,14 @@ __global__ void __launch_bounds__(256) k_containment(View V) {
         const double pd[3] = {p[0], p[1], p[2]};
         if (!mesh_contains(V, pd)) raise_error(V, PMC_ERR_ESCAPED, i, p, -1);
     }
+}
+
+__global__ void __launch_bounds__(256) k_containment(View V) { containment_body(V, launch_grid()); }
+
+__global__ void __launch_bounds__(256) kb_containment(const View* __restrict__ views, const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    containment_body(views[i], G);
 }
 
 // ---------------------------------------------------------------- initialization
@@ -2531,21 +2659,24 @@ __global__ void __launch_bounds__(256) k_count_all(View V) {
 
 } // namespace pmc_dev
 
+// Error word of every instance of a batch, for one read by the host.
+__global__ void kb_errors(const pmc_dev::View* __restrict__ views, int n, int* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = views[i].st->error_code;
+}
+
 // ---------------------------------------------------------------- launchers
 namespace pmc_launch {
 using namespace pmc_dev;
 
-int wave_blocks(int capacity) {
-    const int b = (capacity + kSlotsPerBlock - 1) / kSlotsPerBlock;
-    return (b + 7) & ~7;
-}
+int wave_blocks(int capacity) { return wave_blocks_for(capacity); }
 
 int event_split(int capacity) {
     const int b = wave_blocks(capacity);
     return std::max(1, std::min(8, 2048 / b));
 }
 
-int event_rows(const View& v) { return wave_blocks(v.capacity) * (v.geom_kind == 1 ? v.event_split : 1); }
+int event_rows(const View& v) { return event_rows_for(v); }
 
 int partial_rows(const View& v) { return event_rows(v) + kInjectBlocks; }
 
@@ -2563,8 +2694,8 @@ void events(const View& v, hipStream_t s) {
     else hipLaunchKernelGGL(k_events<false>, dim3(event_rows(v)), kWaveBlock, 0, s, v);
 }
 void inject(const View& v, hipStream_t s) {
-    if (v.geom_kind == 1) hipLaunchKernelGGL(k_inject<true>, dim3(kInjectBlocks), dim3(256), 0, s, v, event_rows(v));
-    else hipLaunchKernelGGL(k_inject<false>, dim3(kInjectBlocks), dim3(256), 0, s, v, event_rows(v));
+    if (v.geom_kind == 1) hipLaunchKernelGGL(k_inject<true>, dim3(kInjectBlocks), dim3(256), 0, s, v);
+    else hipLaunchKernelGGL(k_inject<false>, dim3(kInjectBlocks), dim3(256), 0, s, v);
 }
 void refresh(const View& v, hipStream_t s, int phase) { hipLaunchKernelGGL(k_refresh, dim3(1), dim3(1024), 0, s, v, phase); }
 void sort(const View& v, hipStream_t s) {
@@ -2600,6 +2731,63 @@ void flux(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_flux, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
     hipLaunchKernelGGL(k_flux_finish, dim3(cell_blocks(v.ncells)), dim3(256), 0, s, v);
 }
+// ---- batched launches: one grid over many instances (per-instance block ranges)
+int batch_blocks(const View& v, int kind) {
+    switch (kind) {
+    case kBatchBegin:
+    case kBatchRefresh1:
+    case kBatchRefresh2: return 1;
+    case kBatchContainment: return slot_blocks(v.capacity);
+    case kBatchTransport: return wave_blocks(v.capacity);
+    case kBatchEvents: return event_rows(v);
+    case kBatchInject: return kInjectBlocks;
+    case kBatchSort: {
+        const int b = (v.capacity + 64 * kSortRows * kWavesPerBlock - 1) / (64 * kSortRows * kWavesPerBlock);
+        return (b + 7) & ~7;
+    }
+    case kBatchCellPrep: return cell_blocks(v.ncells);
+    case kBatchMoments:
+    case kBatchRelax: return v.max_chunks;
+    case kBatchNewton: return (v.ncells + 3) / 4;
+    case kBatchCallaway: return v.ncells;
+    default: return 0;
+    }
+}
+
+void batch_launch(int kind, bool mesh, const View* views, const int* offs, int n, int total, hipStream_t s) {
+    if (total <= 0) return;
+    const dim3 g(total);
+    switch (kind) {
+    case kBatchBegin: hipLaunchKernelGGL(kb_begin, g, dim3(64), 0, s, views, offs, n); break;
+    case kBatchContainment: hipLaunchKernelGGL(kb_containment, g, dim3(256), 0, s, views, offs, n); break;
+    case kBatchTransport:
+        if (mesh) hipLaunchKernelGGL(kb_transport<true>, g, kWaveBlock, 0, s, views, offs, n);
+        else hipLaunchKernelGGL(kb_transport<false>, g, kWaveBlock, 0, s, views, offs, n);
+        break;
+    case kBatchEvents:
+        if (mesh) hipLaunchKernelGGL(kb_events<true>, g, kWaveBlock, 0, s, views, offs, n);
+        else hipLaunchKernelGGL(kb_events<false>, g, kWaveBlock, 0, s, views, offs, n);
+        break;
+    case kBatchInject:
+        if (mesh) hipLaunchKernelGGL(kb_inject<true>, g, dim3(256), 0, s, views, offs, n);
+        else hipLaunchKernelGGL(kb_inject<false>, g, dim3(256), 0, s, views, offs, n);
+        break;
+    case kBatchRefresh1: hipLaunchKernelGGL(kb_refresh, g, dim3(1024), 0, s, views, offs, n, 1); break;
+    case kBatchRefresh2: hipLaunchKernelGGL(kb_refresh, g, dim3(1024), 0, s, views, offs, n, 2); break;
+    case kBatchSort: hipLaunchKernelGGL(kb_sort, g, kWaveBlock, 0, s, views, offs, n); break;
+    case kBatchCellPrep: hipLaunchKernelGGL(kb_cell_prep, g, dim3(256), 0, s, views, offs, n); break;
+    case kBatchMoments: hipLaunchKernelGGL(kb_moments, g, dim3(kChunkThreads), 0, s, views, offs, n); break;
+    case kBatchNewton: hipLaunchKernelGGL(kb_newton, g, dim3(256), 0, s, views, offs, n); break;
+    case kBatchRelax: hipLaunchKernelGGL(kb_relax, g, dim3(kChunkThreads), 0, s, views, offs, n); break;
+    case kBatchCallaway: hipLaunchKernelGGL(kb_callaway, g, dim3(kCwThreads), 0, s, views, offs, n); break;
+    default: break;
+    }
+}
+
+void batch_errors(const View* views, int n, int* out, hipStream_t s) {
+    hipLaunchKernelGGL(kb_errors, dim3((n + 255) / 256), dim3(256), 0, s, views, n, out);
+}
+
 void containment(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_containment, dim3(slot_blocks(v.capacity)), dim3(256), 0, s, v);
 }

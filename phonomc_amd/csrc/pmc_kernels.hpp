@@ -4,6 +4,26 @@
 #include "pmc_device.hpp"
 
 namespace pmc_launch {
+// Kernel classes of a batched step, in launch order.
+enum BatchKind {
+    kBatchBegin,
+    kBatchContainment,
+    kBatchTransport,
+    kBatchEvents,
+    kBatchInject,
+    kBatchRefresh1,
+    kBatchSort,
+    kBatchCellPrep,
+    kBatchMoments,
+    kBatchNewton,
+    kBatchRelax,
+    kBatchCallaway,
+    kBatchRefresh2,
+    kBatchKinds
+};
+int batch_blocks(const pmc_dev::View& v, int kind);
+void batch_launch(int kind, bool mesh, const pmc_dev::View* views, const int* offs, int n, int total, hipStream_t s);
+void batch_errors(const pmc_dev::View* views, int n, int* out, hipStream_t s);
 int wave_blocks(int capacity);
 int event_split(int capacity);
 int partial_rows(const pmc_dev::View& v);

@@ -206,6 +206,11 @@ def lib():
         "pmc_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pmc_destroy": (None, [P]),
         "pmc_status_string": (C.c_char_p, [C.c_int32]),
+        "pmc_batch_create": (C.c_int, [C.POINTER(P), C.c_int32, C.POINTER(P)]),
+        "pmc_batch_step": (C.c_int, [P, C.c_int32]),
+        "pmc_batch_failed": (C.c_int, [P]),
+        "pmc_batch_set_graph": (C.c_int, [P, C.c_int32]),
+        "pmc_batch_destroy": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -430,6 +435,48 @@ class Context:
     def close(self):
         if getattr(self, "h", None):
             lib().pmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Independent contexts stepped together (pmc_batch): one launch per kernel of
+    the step over all instances, replayed as a captured hipGraph.  Each context
+    stays readable (cells, ledger, carriers) and must be stepped only through the
+    batch while it exists."""
+
+    def __init__(self, contexts, graph=True):
+        self.contexts = list(contexts)
+        handles = (C.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = C.c_void_p()
+        rc = lib().pmc_batch_create(handles, len(self.contexts), C.byref(h))
+        if rc != PMC_OK:
+            for c in self.contexts:
+                e = Error()
+                lib().pmc_last_error(c.h, C.byref(e))
+                if e.message:
+                    raise RuntimeError(f"pmc error {rc}: {e.message.decode()}")
+            raise RuntimeError(f"pmc error {rc}: {STATUS.get(rc, rc)}")
+        self.h = h
+        lib().pmc_batch_set_graph(h, 1 if graph else 0)
+
+    def step(self, n=1):
+        rc = lib().pmc_batch_step(self.h, n)
+        if rc != PMC_OK:
+            i = lib().pmc_batch_failed(self.h)
+            c = self.contexts[max(i, 0)]
+            e = Error()
+            lib().pmc_last_error(c.h, C.byref(e))
+            raise RuntimeError(f"pmc error {rc} in instance {i}: {e.message.decode() or STATUS.get(rc, rc)}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pmc_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

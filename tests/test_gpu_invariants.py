@@ -102,3 +102,28 @@ def test_finfet_mesh_run_keeps_carriers_inside_and_closes_energy():
     fin_top = (centers[:, 2] > 1300.0)
     substrate = centers[:, 2] < 300.0
     assert t[fin_top].mean() > t[substrate].mean() + 0.5
+
+
+def test_walls_rounded_by_fp32_pass_the_containment_check():
+    """A 419.9279728879643 nm film: its upper wall, 4199.27972888 A, rounds up to
+    4199.27978516 in FP32, and carriers clamped to that wall lie 5.6e-5 A outside
+    the FP64 box.  Carriers seeded exactly there pass the containment check of
+    steps 0 and 100 (the tolerance is the FP32 position quantum, not the
+    reference's 1e-10 of the extent, which is for FP64 positions)."""
+    text = rt.config_text("c1_cross_plane.toml", **{"simulation.particle_count": 20000,
+                                                    "geometry.sizes": "[100, 100, 419.9279728879643]"})
+    p = rt.Problem(text)
+    s, m, geo, i, rough, res = p.views()
+    top = np.float32(geo.bmax[2])
+    assert float(top) > geo.bmax[2] + 1e-5
+    rng = np.random.default_rng(0)
+    n = 20000
+    pos = rng.random((n, 3)) * np.array([1000.0, 1000.0, geo.bmax[2]])
+    pos = pos.astype(np.float32)
+    pos[:50, 2] = top
+    modes = rng.integers(0, m.n_active, n).astype(np.int32)
+    occ = rt.arr(m.neq_background, m.n_active)[modes]
+    g = rt.Context(p, initialize=False)
+    g.seed_carriers(pos, modes, occ)
+    g.step(101)
+    assert g.ledger()["step"] == 101
