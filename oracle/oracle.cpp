@@ -558,7 +558,7 @@ double orc_rta(const pmc_mode_table* m, int ncells, const double* cell_T, int64_
 // with eligible carriers and zeroes it.
 int orc_source_deposit(const pmc_mode_table* m, int ncells, double* pending, const double* cell_T, double t_bg,
                        const int32_t* elig, int64_t n, const int32_t* active, const int32_t* cell, double* occ,
-                       double* energy_out, int64_t* updates_out) {
+                       const double* capacity, double* energy_out, int64_t* updates_out) {
     *energy_out = 0.0;
     *updates_out = 0;
     double pending_total = 0.0;
@@ -577,47 +577,62 @@ int orc_source_deposit(const pmc_mode_table* m, int ncells, double* pending, con
         const double target = pending[c] / m->energy_weight;
         if (!std::isfinite(target)) return PMC_ERR_INVALID_STATE;
         const double tb = std::isfinite(cell_T[c]) && cell_T[c] >= 0.0 ? cell_T[c] : t_bg;
-        auto energy = [&](double t, double* deriv) {
-            double e = 0.0, slope = 0.0;
-            for (int64_t i : idx) {
-                const double w = m->omega[active[i]];
-                const double hw = kHbar * std::max(0.0, w);
-                const double n0 = bose(tb, w), nn = bose(t, w);
-                e += hw * (nn - n0);
-                if (deriv && t > 0.0) slope += hw * (hw * nn * (nn + 1.0) / (kBoltz * t * t));
-            }
-            if (deriv) *deriv = slope;
-            return e;
-        };
-        double low = std::max(0.0, tb);
-        double high = std::max(low + 1.0, 1.05 * low + 1.0);
-        double he = energy(high, nullptr);
-        for (int x = 0; he < target && x < 60; ++x) {
-            high = 2.0 * high + 1.0;
-            he = energy(high, nullptr);
-        }
-        if (!(he >= target) || !std::isfinite(he)) return PMC_ERR_INVALID_STATE;
-        double t = 0.5 * (low + high);
-        const double tol = 1e-12 * std::max(1e-6, target);
-        for (int iter = 0; iter < 60; ++iter) {
-            double d = 0.0;
-            const double r = energy(t, &d) - target;
-            if (std::abs(r) <= tol) break;
-            if (r > 0.0) high = t;
-            else low = t;
-            double cand = 0.5 * (low + high);
-            if (d > 0.0 && std::isfinite(d)) {
-                const double nt = t - r / d;
-                if (nt > low && nt < high && std::isfinite(nt)) cand = nt;
-            }
-            t = cand;
-        }
         std::vector<double> inc(idx.size());
+        if (capacity) {
+            // Linearized transport: target * C_i / sum C / hw_i with the mode heat
+            // capacities at T_bg (src/solver/PhononSourceOperator.cpp:221-236).
+            long double total = 0;
+            for (int64_t i : idx) total += capacity[active[i]];
+            if (!(total > 0)) return PMC_ERR_INVALID_STATE;
+            for (size_t k = 0; k < idx.size(); ++k) {
+                const int a = active[idx[k]];
+                inc[k] = static_cast<double>(target * capacity[a] / total) / (kHbar * m->omega[a]);
+            }
+        } else {
+            auto energy = [&](double t, double* deriv) {
+                double e = 0.0, slope = 0.0;
+                for (int64_t i : idx) {
+                    const double w = m->omega[active[i]];
+                    const double hw = kHbar * std::max(0.0, w);
+                    const double n0 = bose(tb, w), nn = bose(t, w);
+                    e += hw * (nn - n0);
+                    if (deriv && t > 0.0) slope += hw * (hw * nn * (nn + 1.0) / (kBoltz * t * t));
+                }
+                if (deriv) *deriv = slope;
+                return e;
+            };
+            double low = std::max(0.0, tb);
+            double high = std::max(low + 1.0, 1.05 * low + 1.0);
+            double he = energy(high, nullptr);
+            for (int x = 0; he < target && x < 60; ++x) {
+                high = 2.0 * high + 1.0;
+                he = energy(high, nullptr);
+            }
+            if (!(he >= target) || !std::isfinite(he)) return PMC_ERR_INVALID_STATE;
+            double t = 0.5 * (low + high);
+            const double tol = 1e-12 * std::max(1e-6, target);
+            for (int iter = 0; iter < 60; ++iter) {
+                double d = 0.0;
+                const double r = energy(t, &d) - target;
+                if (std::abs(r) <= tol) break;
+                if (r > 0.0) high = t;
+                else low = t;
+                double cand = 0.5 * (low + high);
+                if (d > 0.0 && std::isfinite(d)) {
+                    const double nt = t - r / d;
+                    if (nt > low && nt < high && std::isfinite(nt)) cand = nt;
+                }
+                t = cand;
+            }
+            for (size_t k = 0; k < idx.size(); ++k) {
+                const double w = m->omega[active[idx[k]]];
+                inc[k] = bose(t, w) - bose(tb, w);
+            }
+        }
         double applied = 0.0, largest = -1.0;
         size_t at = 0;
         for (size_t k = 0; k < idx.size(); ++k) {
             const double w = m->omega[active[idx[k]]];
-            inc[k] = bose(t, w) - bose(tb, w);
             const double e = kHbar * w * inc[k];
             applied += e;
             if (e > largest) {
@@ -1092,7 +1107,7 @@ int apply_source(orc_sim& s, double factor, double* energy_out, int64_t* updates
     std::vector<int32_t> act(s.mode.begin(), s.mode.end());
     return orc_source_deposit(&s.m, nc, s.hs_pending.data(), s.cell_T.data(), s.setup.background_temperature,
                               s.hs_elig.data(), static_cast<int64_t>(act.size()), act.data(), s.cell.data(),
-                              s.occ.data(), energy_out, updates_out);
+                              s.occ.data(), s.callaway ? s.cw_cap.data() : nullptr, energy_out, updates_out);
 }
 } // namespace
 

@@ -80,10 +80,12 @@ int orc_step(orc_sim* sim, int32_t nsteps);
 int orc_cells(orc_sim* sim, double* temperature, double* energy_density);
 int orc_heat_flux(orc_sim* sim, double* flux);
 int orc_ledger(orc_sim* sim, pmc_ledger* ledger);
-/* Thermal-spectrum deposit of the pending per-cell budgets (src/solver/PhononSourceOperator.cpp:120-341). */
+/* Thermal-spectrum deposit of the pending per-cell budgets (src/solver/PhononSourceOperator.cpp:120-341):
+ * a pseudo-temperature per cell, or, with capacity (per active mode heat capacity at T_bg, linearized
+ * transport), increments proportional to the mode heat capacity. */
 int orc_source_deposit(const pmc_mode_table* m, int ncells, double* pending, const double* cell_T, double t_bg,
                        const int32_t* elig, int64_t n, const int32_t* active, const int32_t* cell, double* occ,
-                       double* energy, int64_t* updates);
+                       const double* capacity, double* energy, int64_t* updates);
 /* Prescribed heat source, thermal spectrum (src/solver/PhononSourceOperator.cpp:120-341). */
 int orc_set_heat_source(orc_sim* sim, const pmc_heat_source* source);
 /* Linearized Callaway transport (before initialization): linearized initial

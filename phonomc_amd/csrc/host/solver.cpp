@@ -59,8 +59,6 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     const bool linearized = c.collision == CollisionModel::Callaway;
     if (linearized && (!std::isfinite(c.background_temperature) || c.background_temperature <= 0))
         throw std::runtime_error("Callaway requires a positive fixed background_temperature.");
-    if (linearized && c.heat_source_enabled)
-        throw std::runtime_error("heat_source with scattering.model = 'callaway' is not supported by this build.");
     if (c.heat_source_enabled && c.has_non_thermal_spectrum)
         throw std::runtime_error("heat_source.spectrum other than 'thermal' is not supported by this build.");
     if (c.has_gradient_drive) throw std::runtime_error("driving.temperature_gradient is not supported by this build.");

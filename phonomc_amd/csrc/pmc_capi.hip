@@ -153,6 +153,8 @@ int collect_error(pmc_ctx* ctx) {
             std::snprintf(msg, sizeof msg, "Could not bracket the occupation-source pseudo-temperature.");
         else if (st.error_facet == -6)
             std::snprintf(msg, sizeof msg, "Heat source would produce an invalid occupation; cell budget retained.");
+        else if (st.error_facet == -12)
+            std::snprintf(msg, sizeof msg, "Source sample has no thermal capacity.");
         else if (st.error_facet == -9)
             std::snprintf(msg, sizeof msg, "Callaway exponential work limit exceeded; reduce time_step.");
         else if (st.error_facet == -10)

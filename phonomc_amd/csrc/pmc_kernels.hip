@@ -2152,7 +2152,18 @@ __global__ void __launch_bounds__(kChunkThreads) k_source_sums(View V, double fa
     double acc[kSrcCols];
 #pragma unroll
     for (int k = 0; k < kSrcCols; ++k) acc[k] = 0.0;
-    if (pend > 0.0) {
+    if (pend > 0.0 && V.callaway) {
+        // Linearized transport: the cell's heat capacity at T_bg over its eligible
+        // carriers (PhononSourceOperator::apply, src/solver/PhononSourceOperator.cpp:221-236).
+        for (int i = begin + static_cast<int>(threadIdx.x); i < end; i += kChunkThreads) {
+            const int m = V.mode[i] & kModeMask;
+            double w, hw;
+            if (!source_eligible(V, m, w, hw)) continue;
+            acc[kSrcCount] += 1.0;
+            acc[kSrcM1] += V.cw_cap[m];
+        }
+        block_reduce<kSrcCols, kChunkWaves>(acc, lds);
+    } else if (pend > 0.0) {
         const double tb = source_base_T(V, c, half);
         const double inv = tb > 0.0 ? 1.0 / tb : 0.0;
         for (int i = begin + static_cast<int>(threadIdx.x); i < end; i += kChunkThreads) {
@@ -2248,6 +2259,21 @@ __global__ void __launch_bounds__(256) k_source_solve(View V, double factor, int
         raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -4);
         return;
     }
+    if (V.callaway) { // increments target * C_i / sum C / hw_i: one scale per cell
+        if (!(acc[kSrcM1] > 0.0)) {
+            raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -12);
+            return;
+        }
+        if (lead) {
+            cellv[0] = tb;
+            cellv[1] = target / acc[kSrcM1];
+            cellv[2] = target;
+            cellv[3] = 1.0;
+            V.hs_pending[c] = 0.0;
+            atomicAdd(&V.st->step_source_e, pend);
+        }
+        return;
+    }
     const double tol = 1e-12 * fmax(1e-6, target);
     long long exact = 0;
     bool done = false;
@@ -2333,7 +2359,8 @@ __global__ void __launch_bounds__(256) k_source_solve(View V, double factor, int
     }
 }
 
-// Per chunk: increments n_BE(T_s) - n_BE(T_b) of the eligible carriers (kept in
+// Per chunk: increments n_BE(T_s) - n_BE(T_b) of the eligible carriers, or
+// scale * C_i / hw_i under linearized transport (kept in
 // the scratch array), their applied energy and the largest-energy carrier
 // (:296-311; ties go to the smallest carrier id).
 __global__ void __launch_bounds__(kChunkThreads) k_source_increments(View V) {
@@ -2354,7 +2381,7 @@ __global__ void __launch_bounds__(kChunkThreads) k_source_increments(View V) {
         double w, hw;
         double inc = 0.0;
         if (source_eligible(V, m, w, hw)) {
-            inc = bose(ts, w) - bose(tb, w);
+            inc = V.callaway ? ts * V.cw_cap[m] / hw : bose(ts, w) - bose(tb, w);
             const double e = hw * inc;
             applied += e;
             const unsigned id = V.uid[i];

@@ -4,6 +4,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     make -f oracle/ref.mk REF=/root/reference && python tests/golden/make_golden.py [--runs] [--source-runs]
     python tests/golden/make_golden.py --callaway [--callaway-runs]   (Callaway fixtures only)
     python tests/golden/make_golden.py --mesh [--mesh-runs]           (FinFET mesh fixtures only)
+    python tests/golden/make_golden.py --callaway-source [--runs]     (Callaway + heat source only)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -27,7 +28,8 @@ from phonomc_amd import runtime as rt  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml",
-             "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml"}
+             "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml",
+             "c1cs": "c1cs_callaway_source.toml"}
 
 
 def write_input(text, folder, name):
@@ -41,11 +43,12 @@ def write_input(text, folder, name):
     return path
 
 
-def problem_and_reference(key, **overrides):
+def problem_and_reference(key, include_stationary=False, **overrides):
     text = rt.config_text(WORKLOADS[key], **overrides)
     problem = rt.Problem(text, input_directory=rt.CONFIGS, mode_sampling=1)
     path = write_input(text, tempfile.mkdtemp(), f"{key}.toml")
-    return problem, R.RefMaterial(problem.material()), R.RefDomain(path), path
+    rm = R.RefMaterial(problem.material(), include_stationary=include_stationary)
+    return problem, rm, R.RefDomain(path), path
 
 
 def material_vectors(problem, rm, rng):
@@ -281,6 +284,15 @@ def main():
             np.savez_compressed(os.path.join(OUT, "run_finfet.npz"),
                                 **whole_run("finfet", 100000, 400, list(range(1000, 1016)), 10))
         print("mesh fixtures written to", OUT)
+        return
+    if "--callaway-source" in sys.argv:
+        rng = np.random.default_rng(20261206)
+        p, rm, rd, _ = problem_and_reference("c1cs", include_stationary=True)
+        np.savez_compressed(os.path.join(OUT, "source_c1cs.npz"), **source_vectors(p, rm, rd, rng, 0.5))
+        if "--runs" in sys.argv:
+            np.savez_compressed(os.path.join(OUT, "run_c1cs.npz"),
+                                **whole_run("c1cs", 20000, 400, list(range(1000, 1016)), 10))
+        print("Callaway source fixtures written to", OUT)
         return
     if "--callaway" in sys.argv:
         rng = np.random.default_rng(20261204)

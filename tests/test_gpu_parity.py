@@ -75,6 +75,7 @@ def test_initial_population_from_cell_field():
                                                    ("c1s_source.toml", 20000, 1, False),
                                                    ("c1c_callaway.toml", 20000, 1, True),
                                                    ("c2c_callaway.toml", 60000, 1, True),
+                                                   ("c1cs_callaway_source.toml", 20000, 1, True),
                                                    ("finfet.toml", 20000, 1, True),
                                                    ("finfet.toml", 20000, 0, False)])
 def test_steps_match_oracle(cfg, n, sampling, series):

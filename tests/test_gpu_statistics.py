@@ -1,6 +1,7 @@
 """Whole-run statistics of the device step against the reference solver.
 
-tests/golden/run_c1.npz, run_c2.npz, run_c1s.npz, run_c1c.npz, run_c2c.npz and run_finfet.npz hold 16 runs of the
+tests/golden/run_c1.npz, run_c2.npz, run_c1s.npz, run_c1c.npz, run_c2c.npz, run_c1cs.npz and run_finfet.npz
+hold 16 runs of the
 reference PhonoMC build per workload (seeds 1000..1015, written by tests/golden/make_golden.py):
 second-half averages of every cell temperature and of kappa_int / kappa_eff as
 the reference writes them to convergence.txt.  The same TOML inputs and seeds
@@ -24,7 +25,8 @@ from phonomc_amd import runtime as rt
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml",
-             "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml"}
+             "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml",
+             "c1cs": "c1cs_callaway_source.toml"}
 
 
 def second_half(folder, ncell, steps):
@@ -67,11 +69,12 @@ def z_score(a, b):
         return np.where(se > 0, diff / np.where(se > 0, se, 1.0), np.where(diff == 0, 0.0, np.inf))
 
 
-@pytest.mark.parametrize("key", ["c1", "c2", "c1s", "c1c", "c2c", "finfet"])
+@pytest.mark.parametrize("key", ["c1", "c2", "c1s", "c1c", "c2c", "c1cs", "finfet"])
 def test_whole_run_statistics_match_reference(key):
     """c1s adds the Gaussian heat source (kappa is not estimated with a source);
     c1c and c2c run linearized Callaway scattering (c2c with strict-elastic rough walls);
-    finfet runs the 4080-triangle FinFET mesh with a heat source in the fin."""
+    c1cs adds the heat source to c1c (deposits by mode heat capacity); finfet runs the
+    4080-triangle FinFET mesh with a heat source in the fin."""
     golden = np.load(os.path.join(HERE, "golden", f"run_{key}.npz"))
     temps, keff, kint = gpu_runs(key, golden)
     zt = z_score(temps, golden["temperature"])
