@@ -242,6 +242,8 @@ def main():
     ap.add_argument("--no-warm-start", action="store_true", help="skip the warm-start leg")
     ap.add_argument("--workload", default=WORKLOAD, help="bundled TOML input (phonomc_amd/configs) to time")
     ap.add_argument("--particles", type=int, default=0, help="override simulation.particle_count")
+    ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
+                    help="after warm-up, run pmc_autotune with STEPS steps per candidate and keep its choice")
     ap.add_argument("--surrogate-only", type=int, metavar="EPOCHS", default=0,
                     help="only train the operator for EPOCHS epochs and print its GEMM roofline (for profiling)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -272,6 +274,11 @@ def main():
     n0 = ctx.ledger()["live_carriers"]
     log(f"[bench] {args.workload}: {n0} carriers initialized; warmup {args.warmup} steps")
     ctx.step(args.warmup)
+    tuned = None
+    if args.autotune:
+        chosen, report = ctx.autotune(args.autotune)
+        tuned = {"chosen": chosen, "kernel_ms": report}
+        log(f"[bench] autotune: {chosen} {report}")
     led0 = ctx.ledger()  # synchronizes the device
     t0 = time.perf_counter()
     ctx.step(args.steps)  # returns after the device finished the last step
@@ -350,6 +357,8 @@ def main():
     if baseline:
         out["speedup_vs_cpu"] = value / baseline["value"]
     out["warm_start"] = warm
+    if tuned is not None:
+        out["autotune"] = tuned
     print(json.dumps(out), flush=True)
     return 0
 

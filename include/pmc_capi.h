@@ -330,6 +330,25 @@ int pmc_set_timing(pmc_ctx* ctx, int32_t enabled);
 void pmc_destroy(pmc_ctx* ctx);
 const char* pmc_status_string(int32_t code);
 
+/* Launch parameters of the step that do not change its results: how many
+ * event blocks share one transport block's list of carriers with wall events,
+ * and the workgroup size of the per-cell Callaway kernel.  No counterpart in the
+ * reference (OpenMP loops, src/solver/TransportOperator.cpp:113-132 and
+ * src/solver/CarrierCollisionOperator.cpp:38-142). */
+typedef struct pmc_launch_config {
+    int32_t event_split;       /* 1, 2, 4 or 8 */
+    int32_t callaway_threads;  /* 128, 256 or 512 (a cell holds at most 16 carriers per thread) */
+} pmc_launch_config;
+int pmc_get_launch_config(pmc_ctx* ctx, pmc_launch_config* out);
+int pmc_set_launch_config(pmc_ctx* ctx, const pmc_launch_config* config);
+/* Autotuner: times `steps` steps per candidate of each parameter with HIP events
+ * on the context's stream and keeps the fastest (Callaway workgroup sizes only
+ * where the fullest cell fits with a 1.5x margin).  The steps are real steps of
+ * the simulation; the parameters change no result beyond FP64 summation order.
+ * report_ms[k] receives the mean kernel time of candidate k in the order
+ * event_split 1, 2, 4, 8, then callaway_threads 128, 256, 512 (0 when not tried). */
+int pmc_autotune(pmc_ctx* ctx, int32_t steps, pmc_launch_config* chosen, double* report_ms, int32_t report_len);
+
 /* Batched stepping of independent instances.  No counterpart in the reference,
  * which runs one MonteCarloSolver per process (src/main.cpp:86-102); each step
  * of the batch is the reference's run_timestep (src/MonteCarloSolver.cpp:348-398)

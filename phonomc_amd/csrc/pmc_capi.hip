@@ -170,7 +170,7 @@ int collect_error(pmc_ctx* ctx) {
             std::snprintf(msg, sizeof msg,
                           "Cell %lld holds more carriers than the Callaway kernel takes per cell (%d); use more "
                           "cells or fewer carriers.",
-                          i, pmc_dev::kCwThreads * pmc_dev::kCwPer);
+                          i, ctx->view.cw_threads * pmc_dev::kCwPer);
         else
             std::snprintf(msg, sizeof msg, "Rough facet without a reflection table reached by particle %lld.", i);
         break;
@@ -513,6 +513,8 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
     const size_t nc = static_cast<size_t>(g->n_cells);
     v.cell_e = ctx->alloc<double>(nc);
     v.cell_tally = ctx->alloc<double>(pmc_dev::kTallyStride * nc);
+    // Default event split: more event blocks at small carrier counts on a mesh,
+    // where the face searches are latency-bound (pmc_autotune can revise it).
     v.event_split = g->kind == 1 ? pmc_launch::event_split(static_cast<int>(v.capacity)) : 1;
     v.partial_rows = pmc_launch::partial_rows(v);
     v.partials = ctx->alloc<double>(static_cast<size_t>(v.partial_rows) * pmc_dev::kPartialCols);
@@ -708,6 +710,7 @@ int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* t) {
     v.cw_cbg = t->heat_capacity_density;
     v.e_bg = t->background_energy_density;
     v.callaway = 1;
+    v.cw_threads = pmc_dev::kCwThreads;
     return check_hip(ctx, hipGetLastError(), "Callaway table");
 }
 
@@ -993,6 +996,91 @@ void pmc_destroy(pmc_ctx* ctx) {
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------- launch configuration
+extern "C" {
+
+int pmc_get_launch_config(pmc_ctx* ctx, pmc_launch_config* out) {
+    if (!ctx || !out) return PMC_ERR_ARGUMENT;
+    out->event_split = ctx->view.event_split;
+    out->callaway_threads = ctx->view.callaway ? ctx->view.cw_threads : pmc_dev::kCwThreads;
+    return PMC_OK;
+}
+
+int pmc_set_launch_config(pmc_ctx* ctx, const pmc_launch_config* c) {
+    if (!ctx || !c) return PMC_ERR_ARGUMENT;
+    if (!ctx->has_geometry) return fail(ctx, PMC_ERR_ARGUMENT, "geometry must be uploaded before the launch configuration");
+    const int s = c->event_split, t = c->callaway_threads;
+    if (!(s == 1 || s == 2 || s == 4 || s == 8) || s > pmc_dev::kMaxEventSplit)
+        return fail(ctx, PMC_ERR_ARGUMENT, "event_split must be 1, 2, 4 or 8");
+    if (!(t == 128 || t == 256 || t == pmc_dev::kCwThreads))
+        return fail(ctx, PMC_ERR_ARGUMENT, "callaway_threads must be 128, 256 or 512");
+    ctx->view.event_split = s;
+    ctx->view.cw_threads = t;
+    return PMC_OK;
+}
+
+int pmc_autotune(pmc_ctx* ctx, int32_t steps, pmc_launch_config* chosen, double* report_ms, int32_t report_len) {
+    if (!ctx || steps <= 0) return PMC_ERR_ARGUMENT;
+    if (ctx->latched) return PMC_ERR_LATCHED;
+    if (!ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "carriers not initialized");
+    for (int k = 0; k < report_len; ++k) report_ms[k] = 0.0;
+    const bool timing = ctx->timing;
+    ctx->timing = true;
+    auto trial = [&](int cls, double& ms) {
+        pmc_reset_kernel_times(ctx);
+        const int rc = pmc_step(ctx, steps);
+        ms = ctx->kernel_launches[cls] ? ctx->kernel_ms[cls] / static_cast<double>(ctx->kernel_launches[cls]) : 0.0;
+        return rc;
+    };
+    int rc = PMC_OK;
+    // Event blocks per transport block.
+    const int splits[4] = {1, 2, 4, 8};
+    int best_split = ctx->view.event_split;
+    double best = -1.0;
+    for (int k = 0; k < 4 && rc == PMC_OK; ++k) {
+        ctx->view.event_split = splits[k];
+        double ms = 0.0;
+        rc = trial(PMC_KERNEL_EVENTS, ms);
+        if (k < report_len && report_ms) report_ms[k] = ms;
+        if (rc == PMC_OK && (best < 0.0 || ms < best)) {
+            best = ms;
+            best_split = splits[k];
+        }
+    }
+    ctx->view.event_split = best_split;
+    // Callaway workgroup size, among those the fullest cell fits with a margin.
+    if (rc == PMC_OK && ctx->view.callaway) {
+        const View& v = view(ctx);
+        std::vector<int> start(static_cast<size_t>(v.ncells) + 1);
+        ctx->note(hipStreamSynchronize(ctx->stream));
+        ctx->note(hipMemcpy(start.data(), v.cell_start, start.size() * sizeof(int), hipMemcpyDeviceToHost));
+        int fullest = 0;
+        for (int c = 0; c < v.ncells; ++c) fullest = std::max(fullest, start[c + 1] - start[c]);
+        const int sizes[3] = {128, 256, pmc_dev::kCwThreads};
+        int best_t = ctx->view.cw_threads;
+        best = -1.0;
+        for (int k = 0; k < 3 && rc == PMC_OK; ++k) {
+            if (sizes[k] != pmc_dev::kCwThreads && 3 * fullest > 2 * sizes[k] * pmc_dev::kCwPer)
+                continue; // the fullest cell needs 1.5x headroom
+            ctx->view.cw_threads = sizes[k];
+            double ms = 0.0;
+            rc = trial(PMC_KERNEL_CALLAWAY, ms);
+            if (4 + k < report_len && report_ms) report_ms[4 + k] = ms;
+            if (rc == PMC_OK && (best < 0.0 || ms < best)) {
+                best = ms;
+                best_t = sizes[k];
+            }
+        }
+        ctx->view.cw_threads = best_t;
+    }
+    pmc_reset_kernel_times(ctx);
+    ctx->timing = timing;
+    if (chosen) pmc_get_launch_config(ctx, chosen);
+    return rc;
 }
 
 } // extern "C"

@@ -215,7 +215,7 @@ struct View {
     const unsigned char* field_inside; // side of each voxel with a positive bound
     int field_dims[3];
     float field_inv_h, field_unit;
-    int event_split;            // k_events blocks per transport block on a mesh
+    int event_split;            // k_events blocks per transport block
     int nsimplices;
     const double* simplex_v;   // 12 per tetrahedron: apex, face vertices
     const double* simplex_inv; // 9 per tetrahedron
@@ -231,7 +231,7 @@ struct View {
     // cells
     double* cell_tally;   // kTallyStride per cell
     double* partials;     // kPartialCols per block row
-    int partial_rows;     // event blocks + injection blocks
+    int partial_rows;     // allocated: transport blocks x kMaxEventSplit + injection blocks
     double* cell_e;       // energy density of the latest refresh
     double* cell_T;
     double* cell_flux;    // 3 per cell
@@ -285,6 +285,7 @@ struct View {
     const double* cw_gr;    // resistive rate 1/ps
     const double4* cw_pack; // 2 per mode: (g_N, g_R, B0, B1), (B2, B3, hw, 0); B = sqrt(C) (1, q / w)
     double cw_cbg;          // heat-capacity density at T_bg, eV/(K A^3)
+    int cw_threads;         // k_callaway workgroup size: 128, 256 or kCwThreads
 };
 
 // Transport blocks for a slot capacity (a multiple of 8, one per XCD round).
@@ -293,10 +294,9 @@ __host__ __device__ inline int wave_blocks_for(int capacity) {
     return (b + 7) & ~7;
 }
 
-// Event blocks: one per transport block, event_split per transport block on a mesh.
-__host__ __device__ inline int event_rows_for(const View& v) {
-    return wave_blocks_for(v.capacity) * (v.geom_kind == 1 ? v.event_split : 1);
-}
+// Event blocks: event_split per transport block (1 to kMaxEventSplit).
+constexpr int kMaxEventSplit = 8;
+__host__ __device__ inline int event_rows_for(const View& v) { return wave_blocks_for(v.capacity) * v.event_split; }
 
 // Linearized Callaway kernel: one workgroup of kCwThreads per cell, up to
 // kCwPer carriers per thread held in registers.

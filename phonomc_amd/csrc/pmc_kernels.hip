@@ -1002,10 +1002,9 @@ __global__ void __launch_bounds__(256) kb_transport(const View* __restrict__ vie
 // for the slots k_transport listed.  One block per transport block.
 template <bool kMesh>
 __device__ __forceinline__ void events_body(const View& V, const Grid G) {
-    // On a mesh, event_split blocks share one transport block's list when there
-    // are few transport blocks: the face searches are latency-bound and need
-    // the extra waves.
-    const int kSplit = kMesh ? V.event_split : 1;
+    // event_split blocks share one transport block's list (more waves for
+    // latency-bound event work when there are few transport blocks).
+    const int kSplit = V.event_split;
     __shared__ double sp[kPartialCols];
     BlockPartials P{sp};
     P.zero();
@@ -1251,7 +1250,8 @@ __device__ __forceinline__ void refresh_body(const View& V, const Grid G, int ph
         for (int c = 0; c < kPartialCols; ++c) {
             if (c == kPAbsorbedRes + V.nres) break;
             double x = 0.0;
-            for (int r = threadIdx.x; r < V.partial_rows; r += blockDim.x) x += V.partials[r * kPartialCols + c];
+            const int rows = event_rows_for(V) + kInjectBlocks; // rows written this step
+            for (int r = threadIdx.x; r < rows; r += blockDim.x) x += V.partials[r * kPartialCols + c];
             const double s = block_sum(x, red);
             if (threadIdx.x == 0) {
                 switch (c) {
@@ -1856,6 +1856,7 @@ __device__ __forceinline__ void block_reduce_wide(double (&v)[K], double* lds) {
     __syncthreads();
 }
 
+template <int W>
 __device__ __forceinline__ void block_max2(double& a, double& b, double* lds) {
     const int wave = threadIdx.x >> 6;
 #pragma unroll
@@ -1865,14 +1866,14 @@ __device__ __forceinline__ void block_max2(double& a, double& b, double* lds) {
     }
     if (__lane_id() == 0) {
         lds[wave] = a;
-        lds[kCwWaves + wave] = b;
+        lds[W + wave] = b;
     }
     __syncthreads();
     a = 0.0;
     b = 0.0;
-    for (int w = 0; w < kCwWaves; ++w) {
+    for (int w = 0; w < W; ++w) {
         a = fmax(a, lds[w]);
-        b = fmax(b, lds[kCwWaves + w]);
+        b = fmax(b, lds[W + w]);
     }
     __syncthreads();
 }
@@ -1931,19 +1932,22 @@ __device__ void cw_projector(const double (&G)[4][4], double (&P)[4][4]) {
         }
 }
 
+// T threads per workgroup (one cell), up to kCwPer carriers per thread.
+template <int T>
 __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
-    __shared__ double lds[18 * kCwWaves + 18];
+    constexpr int kW = T / 64;
+    __shared__ double lds[18 * kW + 18];
     __shared__ double Psh[17]; // P_N row-major and 1/G_R
     const int c = G.b;
     const int begin = V.cell_start[c], end = V.cell_start[c + 1];
     const int n = end - begin;
     const int k0 = V.chunk_start[c], k1 = V.chunk_start[c + 1];
-    for (int k = k0 + static_cast<int>(threadIdx.x); k < k1; k += kCwThreads) {
+    for (int k = k0 + static_cast<int>(threadIdx.x); k < k1; k += T) {
         V.chunk_out[2 * static_cast<size_t>(k)] = 0.0;
         V.chunk_out[2 * static_cast<size_t>(k) + 1] = 0.0;
     }
     if (n <= 0) return;
-    if (n > kCwThreads * kCwPer) {
+    if (n > T * kCwPer) {
         if (threadIdx.x == 0) raise_error(V, PMC_ERR_UNSUPPORTED, c, nullptr, -8);
         return;
     }
@@ -1957,7 +1961,7 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
     double rmax = 0.0, xmax = 0.0;
 #pragma unroll
     for (int r = 0; r < kCwPer; ++r) {
-        const int j = t + r * kCwThreads;
+        const int j = t + r * T;
         md[r] = -1;
         x[r] = 0.0;
         if (j < n) {
@@ -1984,8 +1988,8 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
             xmax = fmax(xmax, fabs(xi));
         }
     }
-    block_reduce_wide<17, kCwWaves>(acc, lds);
-    block_max2(rmax, xmax, lds);
+    block_reduce_wide<17, kW>(acc, lds);
+    block_max2<kW>(rmax, xmax, lds);
     if (t == 0) {
         double G[4][4], P[4][4];
         int q = 0;
@@ -2052,7 +2056,7 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
                     red[5] += tn * tn;
                     red[6] += s[r] * s[r];
                 }
-                block_reduce_wide<7, kCwWaves>(red, lds);
+                block_reduce_wide<7, kW>(red, lds);
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
                     mN[a] = red[a];
@@ -2086,7 +2090,7 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
 #pragma unroll
     for (int r = 0; r < kCwPer; ++r) {
         if (md[r] < 0) continue;
-        const int i = begin + t + r * kCwThreads;
+        const int i = begin + t + r * T;
         const int m = md[r];
         const double4 p0 = V.cw_pack[2 * m], p1 = V.cw_pack[2 * m + 1];
         const double2 mt = V.mtab[m];
@@ -2098,19 +2102,20 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
         out[0] += (kHbar * mt.x * (next - mt.y)) * V.w;
         out[1] += (p1.z * next - p1.z * old) * V.w;
     }
-    block_reduce_wide<2, kCwWaves>(out, lds);
+    block_reduce_wide<2, kW>(out, lds);
     if (t == 0) {
         V.chunk_out[2 * static_cast<size_t>(k0)] = out[0];
         V.chunk_out[2 * static_cast<size_t>(k0) + 1] = out[1];
     }
 }
 
-__global__ void __launch_bounds__(kCwThreads) k_callaway(View V) { callaway_body(V, launch_grid()); }
+template <int T>
+__global__ void __launch_bounds__(T) k_callaway(View V) { callaway_body<T>(V, launch_grid()); }
 
 __global__ void __launch_bounds__(kCwThreads) kb_callaway(const View* __restrict__ views, const int* __restrict__ offs, int n) {
     Grid G;
     const int i = batch_instance(offs, n, G);
-    callaway_body(views[i], G);
+    callaway_body<kCwThreads>(views[i], G);
 }
 
 // ---------------------------------------------------------------- heat source
@@ -2705,7 +2710,8 @@ int event_split(int capacity) {
 
 int event_rows(const View& v) { return event_rows_for(v); }
 
-int partial_rows(const View& v) { return event_rows(v) + kInjectBlocks; }
+// Allocated rows: enough for the largest event split a context may select.
+int partial_rows(const View& v) { return wave_blocks(v.capacity) * kMaxEventSplit + kInjectBlocks; }
 
 static int slot_blocks(int capacity) { return (capacity + 255) / 256; }
 static int cell_blocks(int ncells) { return (ncells + 255) / 256; }
@@ -2742,7 +2748,11 @@ void relax(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_relax, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v);
 }
 void callaway(const View& v, hipStream_t s) {
-    hipLaunchKernelGGL(k_callaway, dim3(v.ncells), dim3(kCwThreads), 0, s, v);
+    switch (v.cw_threads) {
+    case 128: hipLaunchKernelGGL(k_callaway<128>, dim3(v.ncells), dim3(128), 0, s, v); break;
+    case 256: hipLaunchKernelGGL(k_callaway<256>, dim3(v.ncells), dim3(256), 0, s, v); break;
+    default: hipLaunchKernelGGL(k_callaway<kCwThreads>, dim3(v.ncells), dim3(kCwThreads), 0, s, v); break;
+    }
 }
 void source(const View& v, hipStream_t s, double factor, int half) {
     hipLaunchKernelGGL(k_source_sums, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v, factor, half);

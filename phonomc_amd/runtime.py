@@ -94,6 +94,10 @@ class CallawayTable(C.Structure):
                 ("heat_capacity_density", C.c_double), ("background_energy_density", C.c_double)]
 
 
+class LaunchConfig(C.Structure):
+    _fields_ = [("event_split", C.c_int32), ("callaway_threads", C.c_int32)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("series_on", C.c_int32), ("series_radius", C.c_double), ("reserved", C.c_int32 * 6)]
 
@@ -206,6 +210,9 @@ def lib():
         "pmc_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pmc_destroy": (None, [P]),
         "pmc_status_string": (C.c_char_p, [C.c_int32]),
+        "pmc_get_launch_config": (C.c_int, [P, C.POINTER(LaunchConfig)]),
+        "pmc_set_launch_config": (C.c_int, [P, C.POINTER(LaunchConfig)]),
+        "pmc_autotune": (C.c_int, [P, C.c_int32, C.POINTER(LaunchConfig), C.POINTER(C.c_double), C.c_int32]),
         "pmc_batch_create": (C.c_int, [C.POINTER(P), C.c_int32, C.POINTER(P)]),
         "pmc_batch_step": (C.c_int, [P, C.c_int32]),
         "pmc_batch_failed": (C.c_int, [P]),
@@ -370,6 +377,26 @@ class Context:
 
     def step(self, n=1):
         self._check(lib().pmc_step(self.h, n))
+
+    def launch_config(self):
+        c = LaunchConfig()
+        self._check(lib().pmc_get_launch_config(self.h, C.byref(c)))
+        return {"event_split": c.event_split, "callaway_threads": c.callaway_threads}
+
+    def set_launch_config(self, event_split, callaway_threads=512):
+        c = LaunchConfig(event_split, callaway_threads)
+        self._check(lib().pmc_set_launch_config(self.h, C.byref(c)))
+
+    def autotune(self, steps=4):
+        """pmc_autotune: times `steps` real steps per candidate and keeps the fastest.
+        Returns the chosen configuration and the mean kernel time (ms) per candidate."""
+        c = LaunchConfig()
+        rep = (C.c_double * 7)()
+        self._check(lib().pmc_autotune(self.h, steps, C.byref(c), rep, 7))
+        names = ["event_split=1", "event_split=2", "event_split=4", "event_split=8",
+                 "callaway_threads=128", "callaway_threads=256", "callaway_threads=512"]
+        return ({"event_split": c.event_split, "callaway_threads": c.callaway_threads},
+                {k: v for k, v in zip(names, list(rep)) if v > 0})
 
     def cells(self):
         t, e = np.zeros(self.n_cells), np.zeros(self.n_cells)

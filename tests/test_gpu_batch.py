@@ -117,3 +117,30 @@ def test_batch_rejects_mixed_or_sourced_instances():
     other.step(1)
     with pytest.raises(RuntimeError, match="different steps"):
         rt.Batch([film, other])
+
+
+@pytest.mark.parametrize("cfg,n", [("c2c_callaway.toml", 100000), ("finfet.toml", 200000),
+                                   ("c2_inplane_film.toml", 200000)])
+def test_autotuned_launch_changes_no_result(cfg, n):
+    """pmc_autotune times each candidate on real steps and keeps the fastest; a
+    context tuned that way ends where an untuned one with the same seed ends after
+    the same number of steps."""
+    text = rt.config_text(cfg, **{"simulation.particle_count": n})
+    p = rt.Problem(text, input_directory=rt.CONFIGS)
+    a, b = rt.Context(p), rt.Context(p)
+    chosen, report = a.autotune(steps=2)
+    assert chosen["event_split"] in (1, 2, 4, 8)
+    assert chosen["callaway_threads"] in (128, 256, 512)
+    assert len([k for k in report if k.startswith("event_split")]) == 4
+    if p.callaway() is not None:
+        assert "callaway_threads=512" in report
+    steps = a.ledger()["step"]
+    assert steps >= 8
+    b.step(steps)
+    assert_same(a, b)
+    print(cfg, chosen, {k: round(v, 4) for k, v in report.items()})
+    # The configuration can also be set by hand.
+    b.set_launch_config(2, 512)
+    assert b.launch_config() == {"event_split": 2, "callaway_threads": 512}
+    with pytest.raises(RuntimeError, match="event_split"):
+        b.set_launch_config(3, 512)
