@@ -180,6 +180,29 @@ typedef struct pmc_reservoir {
     const double* triangles;       /* [n_faces*9] */
 } pmc_reservoir;
 
+/* Layered materials along one axis with diffuse-mismatch (DMM) interfaces;
+ * replaces LayeredMaterialLayout (src/solver/LayeredMaterialLayout.cpp:11-115),
+ * InterfaceModeSampler (src/solver/InterfaceModeSampler.cpp:11-110) and the DMM
+ * branch of InterfaceInteraction::interact (src/solver/InterfaceInteraction.cpp:47-153).
+ * The materials' active modes form one global bank (pmc_mode_table indexes it);
+ * every material shares the energy weight and carriers are denser where the
+ * phase-space density is (src/MonteCarloSolver.cpp:81-100). */
+typedef struct pmc_layers {
+    int32_t count;                 /* materials, in layer order along the axis */
+    int32_t axis;                  /* 0 x, 1 y, 2 z */
+    const double* planes;          /* [count-1] interface positions (A), increasing */
+    const int32_t* mode_begin;     /* [count+1] global active-mode range of each material */
+    const int32_t* cell_material;  /* [n_cells] */
+    const double* background_energy_density; /* [count] eV/A^3 at T_bg */
+    const int32_t* lut_begin;      /* [count+1] rows of each material in pmc_mode_table.lut_t / lut_e */
+    const double* spatial_volume;  /* [count] A^3 of space per carrier */
+    double band_width;             /* DMM frequency band, rad/ps */
+    const int32_t* bank_begin;     /* [2*count+1] banks by (material, direction: 0 toward -axis, 1 toward +axis) */
+    const int32_t* bank_mode;      /* global active modes, each bank sorted by frequency */
+    const double* bank_omega;      /* [bank_begin[2*count]] */
+    const double* bank_prefix;     /* bank b: n_b+1 cumulative |v_axis| / normalization from bank_begin[b] + b */
+} pmc_layers;
+
 /* Linearized Callaway scattering about the fixed reference T_bg; replaces
  * CarrierCollisionOperator::apply (src/solver/CarrierCollisionOperator.cpp:38-142)
  * with CallawayOperator (src/material/CallawayOperator.cpp:62-201) on the
@@ -225,6 +248,8 @@ typedef struct pmc_ledger {
     double total_source_energy_ev;
     int64_t source_updates;        /* occupation updates by the heat source */
     int64_t source_exact_passes;   /* exact carrier sums of the source pseudo-temperature solve */
+    int64_t interface_events;      /* material-interface crossings handled (layered materials) */
+    int64_t interface_transmitted; /* of which transmitted */
 } pmc_ledger;
 
 typedef struct pmc_error {
@@ -348,6 +373,10 @@ int pmc_set_launch_config(pmc_ctx* ctx, const pmc_launch_config* config);
  * report_ms[k] receives the mean kernel time of candidate k in the order
  * event_split 1, 2, 4, 8, then callaway_threads 128, 256, 512 (0 when not tried). */
 int pmc_autotune(pmc_ctx* ctx, int32_t steps, pmc_launch_config* chosen, double* report_ms, int32_t report_len);
+
+/* Uploads layered materials (after pmc_upload_boundary_tables, before the
+ * population is created).  RTA only; no rough facets and no heat source. */
+int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* layers);
 
 /* Batched stepping of independent instances.  No counterpart in the reference,
  * which runs one MonteCarloSolver per process (src/main.cpp:86-102); each step

@@ -57,6 +57,10 @@ int pmcs_problem_heat_source(const pmcs_problem* problem, pmc_heat_source* out);
 /* Linearized Callaway view (valid while the problem lives): 1 if the input selects
  * scattering.model = 'callaway' (linearized), 0 if not. */
 int pmcs_problem_callaway(const pmcs_problem* problem, pmc_callaway_table* out);
+/* Layered materials: fills *out and returns 1 (0 for one material). */
+int pmcs_problem_layers(const pmcs_problem* problem, pmc_layers* out);
+/* Raw table of layer `layer` (materials.folders order). */
+int pmcs_problem_layer_material(const pmcs_problem* problem, int32_t layer, pmcs_material_view* out);
 /* Writes the procedural FinFET surface (a substrate box with one fin centred on
  * its top, lattice faces as triangles; sizes and lattice step in nm) as a binary
  * STL; returns the triangle count, or -1 with err filled. */

@@ -865,6 +865,11 @@ struct orc_sim {
     std::vector<double> hs_rate, hs_pending;
     std::vector<int> hs_elig;
     double elapsed = 0.0;
+    // layered materials (DMM interfaces)
+    int layers = 1, layer_axis = 0;
+    double band = 0.0;
+    std::vector<double> planes, e_bg_mat, accept, bank_omega, bank_prefix;
+    std::vector<int> mode_begin, cell_mat, lut_begin, bank_begin, bank_mode;
     // linearized Callaway transport
     bool callaway = false;
     pmc_callaway_table cw{};
@@ -955,11 +960,84 @@ struct StepTally {
     double absorbed_e = 0;
     long long absorbed = 0;
     long long rough_events = 0, rough_spec = 0, rough_diff = 0;
+    long long iface_events = 0, iface_transmitted = 0;
 };
 
 // TransportOperator::advance_particle (src/solver/TransportOperator.cpp:405-466) with
 // process_boundary_collision (:287-362) and interact_boundary
 // (src/solver/BoundaryInteraction.cpp:53-162), non-conservative policy.
+int layer_of_mode(const orc_sim& s, int mode) {
+    int k = 0;
+    while (k + 1 < s.layers && mode >= s.mode_begin[k + 1]) ++k;
+    return k;
+}
+
+// InterfaceModeSampler::sample_band (src/solver/InterfaceModeSampler.cpp:72-109)
+// with the uniform u for the flux draw.
+int sample_band(const orc_sim& s, int material, int direction, double w, double u, double& flux) {
+    const int b = 2 * material + direction;
+    const int b0 = s.bank_begin[b], n = s.bank_begin[b + 1] - b0;
+    const double* om = s.bank_omega.data() + b0;
+    const double* pre = s.bank_prefix.data() + b0 + b;
+    const double bin = std::floor(w / s.band);
+    const double lo = bin * s.band, hi = (bin + 1) * s.band;
+    const int begin = static_cast<int>(std::lower_bound(om, om + n, lo) - om);
+    const int end = static_cast<int>(std::lower_bound(om, om + n, hi) - om);
+    flux = end > begin ? pre[end] - pre[begin] : 0.0;
+    if (!(flux > 0.0)) return -1;
+    const double target = pre[begin] + u * flux;
+    int pos = static_cast<int>(std::upper_bound(pre + begin + 1, pre + end + 1, target) - pre);
+    pos = std::clamp(pos, begin + 1, end) - 1;
+    return s.bank_mode[b0 + pos];
+}
+
+// InterfaceInteraction::interact, DMM branch (src/solver/InterfaceInteraction.cpp:47-153),
+// on three uniforms: reflection band draw, transmission band draw, transmission
+// test.  Returns 1 transmitted, 0 reflected into a new mode, -1 mirrored (same
+// mode, velocity reversed along the axis).
+int dmm_core(const orc_sim& s, int a, const V3& vel, double occ, int dest, const double u[3], int& out, double& nocc,
+             V3& nvel) {
+    const int ax = s.layer_axis;
+    const int mat = layer_of_mode(s, a);
+    const double w = s.m.omega[a];
+    const int inc = vel[ax] > 0.0 ? 1 : 0;
+    double fr = 0.0, ft = 0.0;
+    const int mr = sample_band(s, mat, 1 - inc, w, u[0], fr);
+    const int mt = sample_band(s, dest, inc, w, u[1], ft);
+    const double tr = ft > 0 ? ft / std::max(1e-300, fr + ft) : 0.0;
+    const bool transmit = mt >= 0 && u[2] < tr;
+    const int sel = transmit ? mt : mr;
+    out = a;
+    nocc = occ;
+    nvel = vel;
+    if (sel >= 0) {
+        const double W = s.m.energy_weight;
+        const double dev = occ - s.m.neq_background[a];
+        const double n = s.m.neq_background[sel] + (W * w * dev) / std::max(1e-300, W * s.m.omega[sel]);
+        if (std::isfinite(n) && n >= 0.0) {
+            out = sel;
+            nocc = n;
+            nvel = table_velocity(s, sel);
+            return transmit ? 1 : 0;
+        }
+    }
+    nvel[ax] = -nvel[ax];
+    return -1;
+}
+
+void interface_dmm(orc_sim& s, int64_t i, int dest, Uniforms& U, StepTally& tally) {
+    const double u[3] = {U.next(), U.next(), U.next()};
+    int out = 0;
+    double nocc = 0.0;
+    V3 nvel{};
+    const int action = dmm_core(s, s.mode[i], s.vel[i], s.occ[i], dest, u, out, nocc, nvel);
+    tally.iface_events++;
+    if (action == 1) tally.iface_transmitted++;
+    s.mode[i] = out;
+    s.occ[i] = nocc;
+    s.vel[i] = nvel;
+}
+
 int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
     const double dt = s.setup.time_step_ps;
     const double push = s.setup.push_distance_a;
@@ -970,10 +1048,44 @@ int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
     U.k1 = s.k1;
     for (int ev = 0; remaining > 1e-14 && ev < 64; ++ev) {
         const double t_wall = s.ctime[i] * dt;
-        if (!std::isfinite(t_wall) || t_wall > remaining) {
+        // LayeredMaterialLayout::next_crossing (src/solver/LayeredMaterialLayout.cpp:33-60).
+        double t_cross = std::numeric_limits<double>::infinity();
+        int dest = -1;
+        if (s.layers > 1) {
+            const int mat = layer_of_mode(s, s.mode[i]);
+            const double va = s.vel[i][s.layer_axis];
+            double plane = 0.0;
+            if (std::abs(va) > 1e-18) {
+                if (va > 0.0 && mat + 1 < s.layers) {
+                    plane = s.planes[mat];
+                    dest = mat + 1;
+                } else if (va < 0.0 && mat > 0) {
+                    plane = s.planes[mat - 1];
+                    dest = mat - 1;
+                }
+            }
+            if (dest >= 0) {
+                const double t = (plane - s.pos[i][s.layer_axis]) / va;
+                if (std::isfinite(t) && t > 1e-14) t_cross = t;
+                else dest = -1;
+            }
+        }
+        const double t_event = std::min(t_wall, t_cross);
+        if (!std::isfinite(t_event) || t_event > remaining) {
             s.pos[i] = add(s.pos[i], mul(s.vel[i], remaining));
             if (std::isfinite(s.ctime[i])) s.ctime[i] -= remaining / dt;
             return 0;
+        }
+        // The outer wall wins exact ties (TransportOperator.cpp:429-447).
+        if (t_cross < t_wall) {
+            remaining -= std::max(0.0, t_cross);
+            s.pos[i] = add(s.pos[i], mul(s.vel[i], t_cross));
+            U.ev = static_cast<uint32_t>(ev);
+            U.used = 0;
+            interface_dmm(s, i, dest, U, tally);
+            s.pos[i] = add(s.pos[i], mul(s.vel[i], push / std::max(norm(s.vel[i]), 1e-12)));
+            if (!update_cache(s, i)) return -PMC_ERR_BOUNDARY;
+            continue;
         }
         remaining -= std::max(0.0, t_wall);
         const int facet = s.cfacet[i];
@@ -1061,6 +1173,19 @@ void refresh(orc_sim& s) {
     std::vector<int32_t> act(s.mode.begin(), s.mode.end());
     orc_refresh(&s.m, nc, s.g.cell_volume, static_cast<int64_t>(s.mode.size()), act.data(), s.cell.data(),
                 s.occ.data(), s.cell_e.data(), s.cell_T.data());
+    if (s.layers > 1) { // each cell with its material's background and lookup table
+        std::vector<double> e(nc, 0.0);
+        for (size_t i = 0; i < s.mode.size(); ++i) {
+            const int a = s.mode[i];
+            e[s.cell[i]] += kHbar * s.m.omega[a] * (s.occ[i] - s.m.neq_background[a]) * s.m.energy_weight;
+        }
+        for (int c = 0; c < nc; ++c) {
+            const int k = s.cell_mat[c];
+            s.cell_e[c] = e[c] / s.g.cell_volume[c] + s.e_bg_mat[k];
+            s.cell_T[c] = interp(s.m.lut_e + s.lut_begin[k], s.m.lut_t + s.lut_begin[k],
+                                 s.lut_begin[k + 1] - s.lut_begin[k], s.cell_e[c]);
+        }
+    }
     if (s.callaway) // T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225)
         for (int c = 0; c < nc; ++c)
             s.cell_T[c] = s.setup.background_temperature +
@@ -1192,7 +1317,34 @@ int orc_initialize(orc_sim* s, const pmc_init* init) {
             set_error(s, "sample_volume_points_naive: insufficient accepted samples.");
             return PMC_ERR_BOUNDARY;
         }
-        const int m = std::min(static_cast<int>(u_mode * na), na - 1);
+        int layer = 0;
+        if (s->layers > 1) {
+            // Density-weighted rejection (src/solver/Initialization.cpp:49-66), the
+            // device's Philox words: (carrier, trial + 1, 0 and 1).
+            bool accepted = false;
+            for (uint32_t trial = 0; trial < 64 && !accepted; ++trial) {
+                uint32_t ta[4], tb[4];
+                orc_philox(static_cast<uint32_t>(i), 1 + trial, 0, 3u, s->k0, s->k1, ta);
+                orc_philox(static_cast<uint32_t>(i), 1 + trial, 1, 3u, s->k0, s->k1, tb);
+                const double q[3] = {orc_u53(ta[0], ta[1]), orc_u53(ta[2], ta[3]), orc_u53(tb[0], tb[1])};
+                for (int k = 0; k < 3; ++k) p[k] = s->g.bmin[k] + (s->g.bmax[k] - s->g.bmin[k]) * q[k];
+                const int ax = s->layer_axis;
+                const float pa = std::min(std::max(static_cast<float>(p[ax]), static_cast<float>(s->g.bmin[ax])),
+                                          static_cast<float>(s->g.bmax[ax]));
+                layer = 0;
+                while (layer + 1 < s->layers && pa >= static_cast<float>(s->planes[layer])) ++layer;
+                accepted = orc_u53(tb[2], tb[3]) < s->accept[layer];
+            }
+            if (!accepted) {
+                set_error(s, "sample_volume_points_naive: insufficient accepted samples.");
+                return PMC_ERR_BOUNDARY;
+            }
+        }
+        int m = std::min(static_cast<int>(u_mode * na), na - 1);
+        if (s->layers > 1) {
+            const int b0 = s->mode_begin[layer], nm = s->mode_begin[layer + 1] - b0;
+            m = b0 + std::min(static_cast<int>(u_mode * nm), nm - 1);
+        }
         double T = init->uniform_temperature;
         if (init->temperature_mode == 1) {
             const V3 ax{init->hot_point[0] - init->cold_point[0], init->hot_point[1] - init->cold_point[1],
@@ -1247,6 +1399,37 @@ int orc_set_callaway(orc_sim* s, const pmc_callaway_table* t) {
     s->cw.resistive_rate = s->cw_gr.data();
     s->callaway = true;
     return PMC_OK;
+}
+
+int orc_set_layers(orc_sim* s, const pmc_layers* L) {
+    if (!s || !L || L->count < 2) return PMC_ERR_ARGUMENT;
+    const int n = L->count;
+    s->layers = n;
+    s->layer_axis = L->axis;
+    s->band = L->band_width;
+    s->planes.assign(L->planes, L->planes + (n - 1));
+    s->mode_begin.assign(L->mode_begin, L->mode_begin + n + 1);
+    s->cell_mat.assign(L->cell_material, L->cell_material + s->g.n_cells);
+    s->e_bg_mat.assign(L->background_energy_density, L->background_energy_density + n);
+    s->lut_begin.assign(L->lut_begin, L->lut_begin + n + 1);
+    s->bank_begin.assign(L->bank_begin, L->bank_begin + 2 * n + 1);
+    const int nb = L->bank_begin[2 * n];
+    s->bank_mode.assign(L->bank_mode, L->bank_mode + nb);
+    s->bank_omega.assign(L->bank_omega, L->bank_omega + nb);
+    s->bank_prefix.assign(L->bank_prefix, L->bank_prefix + nb + 2 * n);
+    double vmin = L->spatial_volume[0];
+    for (int k = 0; k < n; ++k) vmin = std::min(vmin, L->spatial_volume[k]);
+    s->accept.resize(n);
+    for (int k = 0; k < n; ++k) s->accept[k] = vmin / L->spatial_volume[k];
+    return PMC_OK;
+}
+
+int orc_interface_dmm(orc_sim* s, int active, const double vel[3], double occ, int dest, const double u[3],
+                      int* out_active, double* out_occ, double out_vel[3]) {
+    V3 nvel{};
+    const int action = dmm_core(*s, active, {vel[0], vel[1], vel[2]}, occ, dest, u, *out_active, *out_occ, nvel);
+    for (int k = 0; k < 3; ++k) out_vel[k] = nvel[k];
+    return action;
 }
 
 int orc_set_heat_source(orc_sim* s, const pmc_heat_source* h) {
@@ -1313,6 +1496,10 @@ int orc_step(orc_sim* s, int32_t nsteps) {
                 total.rough_events += local.rough_events;
                 total.rough_spec += local.rough_spec;
                 total.rough_diff += local.rough_diff;
+            total.iface_events += local.iface_events;
+            total.iface_transmitted += local.iface_transmitted;
+                total.iface_events += local.iface_events;
+                total.iface_transmitted += local.iface_transmitted;
                 for (int r = 0; r < nres; ++r) total.absorbed_by_res[r] += local.absorbed_by_res[r];
             }
         }
@@ -1404,6 +1591,8 @@ int orc_step(orc_sim* s, int32_t nsteps) {
             total.rough_events += local.rough_events;
             total.rough_spec += local.rough_spec;
             total.rough_diff += local.rough_diff;
+            total.iface_events += local.iface_events;
+            total.iface_transmitted += local.iface_transmitted;
             for (int r = 0; r < nres; ++r) total.absorbed_by_res[r] += local.absorbed_by_res[r];
         }
         compact(*s);
@@ -1459,6 +1648,8 @@ int orc_step(orc_sim* s, int32_t nsteps) {
         L.rough_events += total.rough_events;
         L.rough_specular += total.rough_spec;
         L.rough_diffuse += total.rough_diff;
+        L.interface_events += total.iface_events;
+        L.interface_transmitted += total.iface_transmitted;
         L.live_carriers = static_cast<int64_t>(s->mode.size());
         L.slots = L.live_carriers;
         s->previous_absorbed = total.absorbed_by_res;

@@ -88,6 +88,15 @@ int orc_source_deposit(const pmc_mode_table* m, int ncells, double* pending, con
                        const double* capacity, double* energy, int64_t* updates);
 /* Prescribed heat source, thermal spectrum (src/solver/PhononSourceOperator.cpp:120-341). */
 int orc_set_heat_source(orc_sim* sim, const pmc_heat_source* source);
+/* Layered materials with DMM interfaces (src/solver/LayeredMaterialLayout.cpp,
+ * src/solver/InterfaceInteraction.cpp); before orc_initialize. */
+int orc_set_layers(orc_sim* sim, const pmc_layers* layers);
+/* One DMM interaction of a carrier of global mode `active` (material by mode range)
+ * with velocity vel and occupation occ toward material dest, on the uniforms u[3].
+ * Returns the action (1 transmit, 0 reflect into a new mode, -1 mirror) and the
+ * outgoing mode, occupation and velocity. */
+int orc_interface_dmm(orc_sim* sim, int active, const double vel[3], double occ, int dest, const double u[3],
+                      int* out_active, double* out_occ, double out_vel[3]);
 /* Linearized Callaway transport (before initialization): linearized initial
  * occupations, linear cell temperature, conservative reflections and Callaway
  * collisions; the reservoir occupations and refill policy come from the setup

@@ -61,6 +61,9 @@ def lib():
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_heat_source.argtypes = [P, C.POINTER(rt.HeatSource)]
         L.orc_set_callaway.argtypes = [P, C.POINTER(rt.CallawayTable)]
+        L.orc_set_layers.argtypes = [P, C.POINTER(rt.Layers)]
+        L.orc_interface_dmm.argtypes = [P, C.c_int, C.POINTER(d), d, C.c_int, C.POINTER(d), C.POINTER(C.c_int),
+                                        C.POINTER(d), C.POINTER(d)]
         L.orc_callaway_apply.restype = d
         L.orc_callaway_apply.argtypes = [C.POINTER(rt.ModeTable), C.POINTER(rt.CallawayTable), C.c_int, C.c_int64,
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(d), d, d,
@@ -116,10 +119,22 @@ class Sim:
         hs = problem.heat_source()
         if hs is not None:
             self._check(lib().orc_set_heat_source(h, C.byref(hs)))
+        lv = problem.layers()
+        if lv is not None:
+            self._check(lib().orc_set_layers(h, C.byref(lv)))
 
     def _check(self, rc):
         if rc:
             raise RuntimeError(f"oracle error {rc}: {lib().orc_last_error(self.h).decode()}")
+
+    def interface_dmm(self, active, vel, occ, dest, u):
+        """orc_interface_dmm: (action, out mode, out occupation, out velocity)."""
+        v = np.ascontiguousarray(vel, dtype=np.float64)
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        out_a, out_o, out_v = C.c_int(), C.c_double(), np.zeros(3)
+        act = lib().orc_interface_dmm(self.h, int(active), _dp(v), float(occ), int(dest), _dp(uu), C.byref(out_a),
+                                      C.byref(out_o), _dp(out_v))
+        return act, out_a.value, out_o.value, out_v
 
     def initialize(self, init=None):
         self._check(lib().orc_initialize(self.h, C.byref(init if init is not None else self.init)))

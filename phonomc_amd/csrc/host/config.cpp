@@ -352,7 +352,37 @@ SimConfig build(const std::map<std::string, std::string>& kv, const std::string&
     if (auto v = get("boundary.periodic_pair")) c.periodic_pairs = regions(*v, "periodic_pair");
     if (auto v = get("io.material_folder")) c.material_folder = unquote(*v);
     if (auto v = get("io.output_folder")) c.output_folder = unquote(*v);
-    if (get("materials.folders")) c.has_layered_materials = true;
+    {
+        // src/config/ConfigParsing.cpp:600-656.
+        const bool other = get("materials.interface_axis") || get("materials.interface_positions") ||
+                           get("materials.interface_model") || get("materials.interface_frequency_bin_thz") ||
+                           get("materials.interface_amm_fraction") || get("materials.interface_parallel_bin_inv_a") ||
+                           get("materials.mass_densities_kg_m3");
+        const auto folders = get("materials.folders");
+        if (!folders && other)
+            throw std::runtime_error("materials.folders is required when any other [materials] key is present.");
+        if (folders) {
+            c.has_layered_materials = true;
+            for (const auto& f : strings(*folders)) c.material_folders.push_back(unquote(trim(f)));
+            if (c.material_folders.empty()) throw std::runtime_error("materials.folders must not be empty.");
+        }
+        if (auto v = get("materials.interface_axis")) {
+            const auto a = lower(unquote(*v));
+            if (a != "x" && a != "y" && a != "z") throw std::runtime_error("materials.interface_axis must be x, y, or z.");
+            c.interface_axis = a == "x" ? 0 : a == "y" ? 1 : 2;
+        }
+        if (auto v = get("materials.interface_positions"))
+            for (double x : numbers(*v)) c.interface_positions.push_back(10.0 * x);
+        if (auto v = get("materials.interface_model")) {
+            const auto m = lower(unquote(*v));
+            if (m != "dmm" && m != "amm" && m != "mmm")
+                throw std::runtime_error("materials.interface_model must be dmm, amm or mmm.");
+            c.interface_model = m;
+        }
+        if (auto v = get("materials.interface_frequency_bin_thz")) c.interface_bin_thz = to_double(*v);
+        if (c.interface_model == "mmm" && !get("materials.interface_amm_fraction"))
+            throw std::runtime_error("MMM requires explicit materials.interface_amm_fraction.");
+    }
     if (auto v = get("mode_excitation.enabled")) c.has_mode_excitation = to_bool(*v);
 
     // Heat source (src/config/ConfigParsing.cpp:658-766).

@@ -66,6 +66,12 @@ struct SimConfig {
     // Keys parsed for compatibility whose features this build rejects at solver set-up.
     bool has_gradient_drive = false;
     bool has_layered_materials = false;
+    // [materials] (src/config/ConfigParsing.cpp:600-656): layered materials along one axis.
+    std::vector<std::string> material_folders;
+    int interface_axis = 0;
+    std::vector<double> interface_positions; // Angstrom
+    std::string interface_model = "dmm";
+    double interface_bin_thz = 0.1;
     bool has_convergence_stop = false;
     bool has_mode_excitation = false;
     bool has_non_thermal_spectrum = false;

@@ -131,6 +131,21 @@ int pmcs_problem_material(const pmcs_problem* pr, pmcs_material_view* out) {
     return PMC_OK;
 }
 
+int pmcs_problem_layers(const pmcs_problem* pr, pmc_layers* out) {
+    if (!pr) return -1;
+    const auto& p = *(pr->operator->());
+    if (out) *out = p.layers_view;
+    return p.layer_modes.size() > 1 ? 1 : 0;
+}
+
+int pmcs_problem_layer_material(const pmcs_problem* pr, int32_t layer, pmcs_material_view* out) {
+    if (!pr || !out) return PMC_ERR_ARGUMENT;
+    const auto& p = *(pr->operator->());
+    if (layer < 0 || layer >= static_cast<int>(p.layer_raw.size())) return PMC_ERR_ARGUMENT;
+    fill_material(*p.layer_raw[layer], p.config.temperature_lookup_dt, out);
+    return PMC_OK;
+}
+
 int pmcs_problem_scalars(const pmcs_problem* pr, double* out) {
     if (!pr || !out) return PMC_ERR_ARGUMENT;
     const auto& p = *(pr->operator->());

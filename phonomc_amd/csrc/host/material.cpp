@@ -514,7 +514,17 @@ RawMaterial load_material(const std::string& spec, const std::string& input_dire
             if (opt != "callaway") throw std::runtime_error("Unknown synthetic material option: " + opt);
             s.callaway_split = true;
         }
-        if (name.rfind("si", 0) != 0) throw std::runtime_error("Unknown synthetic material: " + name);
+        if (name.rfind("ge", 0) == 0) {
+            // Ge-like: larger cell, slower sound, lower optical branches.
+            s.cell_volume_a3 = 5.658 * 5.658 * 5.658 / 4.0;
+            s.ta_velocity = 35.0;
+            s.la_velocity = 53.0;
+            s.optical_thz[0] = 9.0;
+            s.optical_thz[1] = 8.6;
+            s.optical_thz[2] = 8.2;
+        } else if (name.rfind("si", 0) != 0) {
+            throw std::runtime_error("Unknown synthetic material: " + name);
+        }
         s.mesh = std::stoi(name.substr(2));
         if (s.mesh < 2 || s.mesh > 64) throw std::runtime_error("Synthetic mesh must be in [2, 64]");
         return make_synthetic_material(s);

@@ -9,6 +9,7 @@
 #include <limits>
 #include <map>
 #include <mutex>
+#include <numeric>
 #include <stdexcept>
 #include <string>
 
@@ -49,6 +50,174 @@ std::shared_ptr<const MaterialEntry> cached_material(const std::string& spec, co
     if (cache.size() >= 16) cache.clear(); // bounded: a process rarely uses more materials
     return cache.emplace(key, std::move(entry)).first->second;
 }
+
+// Layered materials (src/solver/LayeredMaterialLayout.cpp:11-115,
+// src/MonteCarloSolver.cpp:81-100, src/solver/InterfaceModeSampler.cpp:11-44):
+// one global active-mode bank over the materials in layer order, cell
+// materials, the common energy weight, per-material background energies and
+// lookup tables, and the DMM banks.
+void build_layers(Problem& p) {
+    const SimConfig& c = p.config;
+    const Domain& d = p.domain;
+    auto& L = p.layers;
+    L.count = static_cast<int>(c.material_folders.size());
+    const int axis = c.interface_axis;
+    if (!d.is_box) throw std::runtime_error("Layered materials on a mesh model are not supported by this build.");
+    if (c.interface_positions.size() != static_cast<size_t>(L.count - 1))
+        throw std::invalid_argument("Invalid layered material count, axis or interface count.");
+    L.planes = c.interface_positions;
+    for (size_t i = 0; i < L.planes.size(); ++i)
+        if (!std::isfinite(L.planes[i]) || (i && L.planes[i] <= L.planes[i - 1]))
+            throw std::invalid_argument("Material interface positions must be finite and strictly increasing.");
+    auto material_at = [&](const Vec3& x) {
+        return static_cast<int>(std::upper_bound(L.planes.begin(), L.planes.end(), x[axis]) - L.planes.begin());
+    };
+    // LayeredMaterialLayout::validated_cell_materials (:62-113).
+    const double lo = d.bmin[axis], hi = d.bmax[axis];
+    const double width = (hi - lo) / static_cast<double>(c.grid[axis]);
+    const double tolerance = std::max(1e-10, 1e-10 * std::abs(hi - lo));
+    for (double plane : L.planes) {
+        if (!std::isfinite(plane) || !(plane > lo) || !(plane < hi))
+            throw std::runtime_error("Each material interface must lie strictly inside the loaded geometry bounds.");
+        const double cc = (plane - lo) / width;
+        if (std::abs(cc - std::round(cc)) > tolerance / width)
+            throw std::runtime_error("Each materials.interface_positions value must coincide with a grid-cell boundary.");
+    }
+    L.cell_material.resize(d.cell_count());
+    std::vector<bool> occupied(L.count, false);
+    for (int cell = 0; cell < d.cell_count(); ++cell) occupied[L.cell_material[cell] = material_at(d.cell_center[cell])] = true;
+    if (std::find(occupied.begin(), occupied.end(), false) != occupied.end())
+        throw std::runtime_error("Every material layer must contain at least one active geometry cell.");
+    for (const auto& f : d.facets) {
+        if (f.condition == 'P' && std::abs(f.normal[axis]) > 0.99)
+            throw std::runtime_error("Periodic boundaries normal to materials.interface_axis are not supported, because "
+                                     "they would join different end materials.");
+        if (f.condition == 'T' && std::abs(f.normal[axis]) < 0.99)
+            throw std::runtime_error("In the layered-material model, thermal-reservoir facets must be normal to "
+                                     "materials.interface_axis so each reservoir touches one material.");
+        if (f.condition == 'R')
+            throw std::runtime_error("Layered materials with rough or specular ('R') facets are not supported by this build.");
+    }
+    // Materials: equal temperature grids, one global bank.
+    std::vector<std::shared_ptr<const MaterialEntry>> mats;
+    for (const auto& f : c.material_folders)
+        mats.push_back(cached_material(f, c.input_directory, c.temperature_lookup_dt, false));
+    for (const auto& e : mats)
+        if (e->modes.temps != mats[0]->modes.temps)
+            throw std::runtime_error("Layered materials need equal lifetime temperature grids in this build.");
+    std::vector<double> density(L.count);
+    for (int k = 0; k < L.count; ++k) {
+        const auto& t = mats[k]->modes;
+        density[k] = t.n_active / t.energy_normalization;
+        if (!(density[k] > 0.0) || !std::isfinite(density[k])) throw std::runtime_error("Invalid material phase-space density.");
+    }
+    double states = 0.0;
+    for (int cell = 0; cell < d.cell_count(); ++cell) states += d.cell_volume[cell] * density[L.cell_material[cell]];
+    const double common = states / std::llround(c.particle_count);
+    p.energy_weight = common;
+    L.spatial_volume.resize(L.count);
+    for (int k = 0; k < L.count; ++k) L.spatial_volume[k] = common / density[k];
+    p.particle_volume = L.spatial_volume[0];
+    p.particle_density = 1.0 / L.spatial_volume[0];
+    const auto& temps = mats[0]->modes.temps;
+    const int nt = static_cast<int>(temps.size());
+    p.omega_active.clear();
+    p.neq_background.clear();
+    p.velocity_active.clear();
+    L.mode_begin.assign(1, 0);
+    L.lut_begin.assign(1, 0);
+    std::vector<std::vector<double>> gamma_cols;
+    for (int k = 0; k < L.count; ++k) {
+        const auto& t = mats[k]->modes;
+        for (int a = 0; a < t.n_active; ++a) {
+            const int f = t.active_flat[a];
+            p.omega_active.push_back(t.omega_flat[f]);
+            p.neq_background.push_back(t.bose(c.background_temperature, f));
+            for (int q = 0; q < 3; ++q) p.velocity_active.push_back(t.velocity_flat[f][q]);
+        }
+        L.mode_begin.push_back(L.mode_begin.back() + t.n_active);
+        L.background_energy_density.push_back(t.energy_density_from_temperature(c.background_temperature));
+        L.lut_t.insert(L.lut_t.end(), t.lut_t.begin(), t.lut_t.end());
+        L.lut_e.insert(L.lut_e.end(), t.lut_e.begin(), t.lut_e.end());
+        L.lut_begin.push_back(static_cast<int32_t>(L.lut_t.size()));
+    }
+    const int total = L.mode_begin.back();
+    L.active_flat.resize(total);
+    std::iota(L.active_flat.begin(), L.active_flat.end(), 0);
+    L.gamma_total.assign(static_cast<size_t>(nt) * total, 0.0);
+    for (int k = 0; k < L.count; ++k) {
+        const auto& t = mats[k]->modes;
+        for (int a = 0; a < t.n_active; ++a)
+            for (int r = 0; r < nt; ++r)
+                L.gamma_total[static_cast<size_t>(r) * total + L.mode_begin[k] + a] =
+                    t.gamma_total[static_cast<size_t>(r) * t.n_flat + t.active_flat[a]];
+    }
+    p.background_energy_density = L.background_energy_density[0];
+    // DMM banks: per material and direction along the axis, modes sorted by
+    // frequency with the cumulative |v_axis| / normalization (InterfaceModeSampler::configure).
+    L.bank_begin.assign(1, 0);
+    for (int k = 0; k < L.count; ++k) {
+        const auto& t = mats[k]->modes;
+        for (int dir = 0; dir < 2; ++dir) {
+            std::vector<std::pair<double, int>> ordered;
+            for (int a = 0; a < t.n_active; ++a) {
+                const double comp = t.velocity_flat[t.active_flat[a]][axis];
+                if ((dir == 0 && comp < 0.0) || (dir == 1 && comp > 0.0)) ordered.push_back({t.omega_flat[t.active_flat[a]], a});
+            }
+            std::sort(ordered.begin(), ordered.end());
+            double acc = 0.0;
+            L.bank_prefix.push_back(0.0);
+            for (const auto& o : ordered) {
+                L.bank_omega.push_back(o.first);
+                L.bank_mode.push_back(L.mode_begin[k] + o.second);
+                acc += std::abs(t.velocity_flat[t.active_flat[o.second]][axis]) / t.energy_normalization;
+                L.bank_prefix.push_back(acc);
+            }
+            L.bank_begin.push_back(static_cast<int32_t>(L.bank_mode.size()));
+        }
+    }
+    p.layer_modes.clear();
+    p.layer_raw.clear();
+    for (const auto& e : mats) {
+        p.layer_modes.push_back(&e->modes);
+        p.layer_raw.push_back(&e->raw);
+    }
+    p.layer_materials.assign(mats.begin(), mats.end());
+}
+
+// Reservoirs of a layered box: each reservoir facet touches one material
+// (validated), so its table is the material's, with that material's carrier
+// density, placed in the global bank.
+void layered_reservoirs(Problem& p) {
+    const auto& L = p.layers;
+    const SimConfig& c = p.config;
+    const Domain& d = p.domain;
+    const int total = L.mode_begin.back();
+    p.facet_to_reservoir.assign(d.facets.size(), -1);
+    p.reservoirs.clear();
+    for (int k = 0; k < L.count; ++k) {
+        std::vector<int> f2r;
+        auto tables = build_reservoir_tables(d, *p.layer_modes[k], 1.0 / L.spatial_volume[k], c.time_step, f2r);
+        for (auto& r : tables) {
+            const Vec3& centre = d.facets[r.facet].centroid;
+            const int mat = static_cast<int>(std::upper_bound(L.planes.begin(), L.planes.end(),
+                                                              centre[c.interface_axis]) - L.planes.begin());
+            if (mat != k) continue;
+            const int b = L.mode_begin[k], n = L.mode_begin[k + 1] - b;
+            std::vector<double> cdf(total, 0.0), w(total, 0.0), occ(total, 0.0);
+            for (int j = 0; j < total; ++j) cdf[j] = j < b ? 0.0 : (j >= b + n ? 1.0 : r.cdf[j - b]);
+            for (int a = 0; a < n; ++a) {
+                w[b + a] = r.cdf[a] - (a ? r.cdf[a - 1] : 0.0);
+                occ[b + a] = r.occupation[a];
+            }
+            r.cdf = std::move(cdf);
+            r.occupation = std::move(occ);
+            build_alias(w, r.alias_prob, r.alias_index);
+            p.facet_to_reservoir[r.facet] = static_cast<int>(p.reservoirs.size());
+            p.reservoirs.push_back(std::move(r));
+        }
+    }
+}
 } // namespace
 
 std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions& opt) {
@@ -62,17 +231,26 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     if (c.heat_source_enabled && c.has_non_thermal_spectrum)
         throw std::runtime_error("heat_source.spectrum other than 'thermal' is not supported by this build.");
     if (c.has_gradient_drive) throw std::runtime_error("driving.temperature_gradient is not supported by this build.");
-    if (c.has_layered_materials) throw std::runtime_error("materials.folders (layered materials) is not supported by this build.");
+    const bool layered = c.material_folders.size() > 1;
+    if (layered) {
+        if (linearized) throw std::runtime_error("Layered materials with scattering.model = 'callaway' are not supported by this build.");
+        if (c.heat_source_enabled) throw std::runtime_error("Layered materials with a heat source are not supported by this build.");
+        if (c.interface_model != "dmm")
+            throw std::runtime_error("materials.interface_model = '" + c.interface_model + "' is not supported by this build; use dmm.");
+        if (!std::isfinite(c.interface_bin_thz) || c.interface_bin_thz <= 0)
+            throw std::runtime_error("Interface frequency bin must be finite and positive.");
+    }
     if (c.has_convergence_stop) throw std::runtime_error("convergence.enabled requires gradient driving.");
     if (c.has_mode_excitation) throw std::runtime_error("mode_excitation is not supported by this build.");
     if (c.conservative_boundaries) throw std::runtime_error("simulation.conservative_boundaries is not supported by this build.");
-    if (c.material_folder.empty()) throw std::runtime_error("io.material_folder is required.");
+    if (c.material_folder.empty() && c.material_folders.empty()) throw std::runtime_error("io.material_folder is required.");
 
     auto p = std::make_unique<Problem>();
     p->config = c;
     // Stationary modes join the active bank under conservative transport
     // (src/material/MaterialFactory.cpp:51-55).
-    const auto material = cached_material(c.material_folder, c.input_directory, c.temperature_lookup_dt, linearized);
+    const auto material = cached_material(c.material_folders.empty() ? c.material_folder : c.material_folders.front(),
+                                          c.input_directory, c.temperature_lookup_dt, linearized);
     p->raw = material->raw;
     p->modes = material->modes;
     p->linearized = linearized;
@@ -118,6 +296,7 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         p->omega_active[a] = m.omega_flat[f];
         for (int k = 0; k < 3; ++k) p->velocity_active[3 * a + k] = m.velocity_flat[f][k];
     }
+    if (layered) build_layers(*p);
     // Rough tables on every 'R' facet, or on every reflecting facet with strict
     // elastic tables at T_bg under conservative transport
     // (src/solver/RoughBoundaryInitialization.cpp:9-25).
@@ -128,7 +307,8 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         enabled[f] = linearized ? reflecting : fc.condition == 'R';
     }
     p->rough = build_rough_tables(d, m, enabled, linearized ? c.background_temperature : 0.0, p->facet_to_rough);
-    p->reservoirs = build_reservoir_tables(d, m, p->particle_density, c.time_step, p->facet_to_reservoir);
+    if (layered) layered_reservoirs(*p);
+    else p->reservoirs = build_reservoir_tables(d, m, p->particle_density, c.time_step, p->facet_to_reservoir);
     if (linearized) {
         // Linearized reservoir occupations (src/solver/ReservoirInjector.cpp:104-109).
         for (auto& r : p->reservoirs)
@@ -175,6 +355,12 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     mv.n_active = m.n_active;
     mv.n_flat = m.n_flat;
     mv.active_flat = m.active_flat.data();
+    if (layered) { // the global bank: flat index = global active index
+        const auto& L = p->layers;
+        mv.n_active = static_cast<int>(p->omega_active.size());
+        mv.n_flat = mv.n_active;
+        mv.active_flat = L.active_flat.data();
+    }
     mv.omega = p->omega_active.data();
     mv.velocity = p->velocity_active.data();
     mv.neq_background = p->neq_background.data();
@@ -187,6 +373,27 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     mv.lut_n = static_cast<int>(m.lut_t.size());
     mv.lut_t = m.lut_t.data();
     mv.lut_e = m.lut_e.data();
+    if (layered) {
+        auto& L = p->layers;
+        mv.gamma_total = L.gamma_total.data();
+        mv.lut_n = static_cast<int>(L.lut_t.size());
+        mv.lut_t = L.lut_t.data();
+        mv.lut_e = L.lut_e.data();
+        auto& lv = p->layers_view;
+        lv.count = L.count;
+        lv.axis = c.interface_axis;
+        lv.planes = L.planes.data();
+        lv.mode_begin = L.mode_begin.data();
+        lv.cell_material = L.cell_material.data();
+        lv.background_energy_density = L.background_energy_density.data();
+        lv.lut_begin = L.lut_begin.data();
+        lv.spatial_volume = L.spatial_volume.data();
+        lv.band_width = 2.0 * M_PI * c.interface_bin_thz;
+        lv.bank_begin = L.bank_begin.data();
+        lv.bank_mode = L.bank_mode.data();
+        lv.bank_omega = L.bank_omega.data();
+        lv.bank_prefix = L.bank_prefix.data();
+    }
     auto& g = p->geometry_view;
     g.kind = 0;
     for (int k = 0; k < 3; ++k) {
@@ -366,6 +573,7 @@ MonteCarloSolver::MonteCarloSolver(std::unique_ptr<Problem> problem, std::string
     throw_if_failed(pmc_upload_boundary_tables(ctx_, static_cast<int>(p.rough_views.size()), p.rough_views.data(),
                                                static_cast<int>(p.reservoir_views.size()), p.reservoir_views.data()));
     if (p.linearized) throw_if_failed(pmc_upload_callaway(ctx_, &p.callaway_view));
+    if (p.layer_modes.size() > 1) throw_if_failed(pmc_upload_layers(ctx_, &p.layers_view));
     if (p.heat.enabled) throw_if_failed(pmc_upload_heat_source(ctx_, &p.heat_view));
     throw_if_failed(pmc_initialize_carriers(ctx_, &p.init));
     if (!folder_.empty()) {

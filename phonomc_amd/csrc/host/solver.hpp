@@ -43,6 +43,18 @@ struct Problem {
     std::vector<double> callaway_wavevector; // 3 per active mode
     double heat_capacity_density = 0;
     pmc_callaway_table callaway_view{};
+    // Layered materials behind pmc_layers (materials.folders with more than one entry).
+    struct LayerArrays {
+        int count = 1;
+        std::vector<double> planes, background_energy_density, spatial_volume;
+        std::vector<int32_t> mode_begin, cell_material, lut_begin, bank_begin, bank_mode;
+        std::vector<double> bank_omega, bank_prefix, lut_t, lut_e, gamma_total;
+        std::vector<int32_t> active_flat;
+    } layers;
+    pmc_layers layers_view{};
+    std::vector<const ModeTables*> layer_modes;            // per material (layered problems)
+    std::vector<const RawMaterial*> layer_raw;
+    std::vector<std::shared_ptr<const void>> layer_materials; // keeps them alive
     // Mesh model arrays behind pmc_mesh (geometry kind 1).
     struct MeshArrays {
         std::vector<double> vertices, face_normal, face_k, face_bounds;

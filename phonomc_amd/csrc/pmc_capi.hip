@@ -897,6 +897,8 @@ int pmc_read_ledger(pmc_ctx* ctx, pmc_ledger* out) {
     out->thermal_energy_ev = st.thermal_e;
     out->carrier_steps = st.carrier_steps;
     out->rough_events = st.rough_events;
+    out->interface_events = st.iface_events;
+    out->interface_transmitted = st.iface_transmitted;
     out->rough_specular = st.rough_specular;
     out->rough_diffuse = st.rough_diffuse;
     out->newton_passes = st.newton_passes;
@@ -999,6 +1001,62 @@ void pmc_destroy(pmc_ctx* ctx) {
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------- layered materials
+extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
+    if (!ctx || !L) return PMC_ERR_ARGUMENT;
+    View& v = ctx->view;
+    if (!ctx->has_material || !ctx->has_geometry) return fail(ctx, PMC_ERR_ARGUMENT, "layers need material and geometry");
+    if (ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "layers must be uploaded before the population is created");
+    if (v.callaway || v.hs_on) return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run RTA without a heat source");
+    if (v.geom_kind != 0) return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials need the box geometry");
+    const int n = L->count;
+    if (n < 2 || n > pmc_dev::kMaxLayers || L->axis < 0 || L->axis > 2 || !L->planes || !L->mode_begin ||
+        !L->cell_material || !L->background_energy_density || !L->lut_begin || !L->spatial_volume || !L->bank_begin ||
+        !L->bank_mode || !L->bank_omega || !L->bank_prefix || !(L->band_width > 0.0))
+        return fail(ctx, PMC_ERR_ARGUMENT, "invalid layered-material tables");
+    if (L->mode_begin[0] != 0 || L->mode_begin[n] != v.na || L->lut_begin[0] != 0 || L->lut_begin[n] != v.lut_n ||
+        L->bank_begin[0] != 0)
+        return fail(ctx, PMC_ERR_ARGUMENT, "layer mode or table ranges do not cover the tables");
+    double vmin = L->spatial_volume[0];
+    for (int k = 0; k < n; ++k) {
+        if (L->mode_begin[k + 1] <= L->mode_begin[k] || L->lut_begin[k + 1] < L->lut_begin[k] + 2 ||
+            !(L->spatial_volume[k] > 0.0))
+            return fail(ctx, PMC_ERR_ARGUMENT, "empty material in the layered tables");
+        vmin = std::min(vmin, L->spatial_volume[k]);
+    }
+    for (int k = 0; k + 1 < n; ++k)
+        if (!(L->planes[k] > v.bmin[L->axis] && L->planes[k] < v.bmax[L->axis]) ||
+            (k && !(L->planes[k] > L->planes[k - 1])))
+            return fail(ctx, PMC_ERR_ARGUMENT, "material interfaces must be increasing and inside the box");
+    for (int b = 0; b < 2 * n; ++b)
+        if (L->bank_begin[b + 1] < L->bank_begin[b]) return fail(ctx, PMC_ERR_ARGUMENT, "malformed interface banks");
+    const int nb = L->bank_begin[2 * n];
+    for (int j = 0; j < nb; ++j)
+        if (L->bank_mode[j] < 0 || L->bank_mode[j] >= v.na)
+            return fail(ctx, PMC_ERR_ARGUMENT, "interface bank mode out of range");
+    std::vector<int> cm(L->cell_material, L->cell_material + v.ncells);
+    for (int c : cm)
+        if (c < 0 || c >= n) return fail(ctx, PMC_ERR_ARGUMENT, "cell material out of range");
+    v.layer_axis = L->axis;
+    for (int k = 0; k + 1 < n; ++k) v.layer_plane[k] = static_cast<float>(L->planes[k]);
+    for (int k = 0; k <= n; ++k) {
+        v.mode_begin[k] = L->mode_begin[k];
+        v.lut_begin[k] = L->lut_begin[k];
+    }
+    for (int k = 0; k < n; ++k) {
+        v.e_bg_mat[k] = L->background_energy_density[k];
+        v.layer_accept[k] = vmin / L->spatial_volume[k];
+    }
+    for (int b = 0; b <= 2 * n; ++b) v.bank_begin[b] = L->bank_begin[b];
+    v.band = L->band_width;
+    v.cell_mat = ctx->upload(cm.data(), cm.size());
+    v.bank_mode = ctx->upload(L->bank_mode, static_cast<size_t>(nb));
+    v.bank_omega = ctx->upload(L->bank_omega, static_cast<size_t>(nb));
+    v.bank_prefix = ctx->upload(L->bank_prefix, static_cast<size_t>(nb) + 2 * n);
+    v.layers = n;
+    return check_hip(ctx, hipGetLastError(), "layers");
+}
 
 // ---------------------------------------------------------------- launch configuration
 extern "C" {

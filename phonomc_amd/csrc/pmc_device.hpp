@@ -30,6 +30,7 @@ constexpr int kUnroll = 4;
 constexpr int kWavesPerBlock = 4;
 constexpr int kSlotsPerBlock = 64 * kRowsPerWave * kWavesPerBlock;
 constexpr int kInjectBlocks = 64;
+constexpr int kMaxLayers = 8;     // materials of a layered domain
 
 // Collision kernels work on chunks of up to kChunk consecutive carriers of one
 // cell in the cell-sorted buffer (one block each) and write per-chunk partial
@@ -65,8 +66,10 @@ enum Partial : int {
     kPRoughEvents = 4,
     kPRoughSpecular = 5,
     kPRoughDiffuse = 6,
-    kPAbsorbedRes = 8, // + reservoir index
-    kPartialCols = 8 + kMaxReservoirs
+    kPIfaceEvents = 7,
+    kPIfaceTransmitted = 8,
+    kPAbsorbedRes = 9, // + reservoir index
+    kPartialCols = 9 + kMaxReservoirs
 };
 // Heat-source sums per chunk: Taylor terms of sum hw n_BE(T) about the base
 // temperature (no constant term: f(T_b) = 0 by construction), eligible count,
@@ -147,6 +150,7 @@ struct DevState {
     double thermal_e;
     long long carrier_steps;
     long long rough_events, rough_specular, rough_diffuse;
+    long long iface_events, iface_transmitted;
     double step_source_e, total_source_e;   // heat-source energy deposited (physical eV)
     long long source_updates;               // occupation updates by the heat source
     long long source_passes;                // exact carrier sums of the source pseudo-temperature solve
@@ -224,6 +228,21 @@ struct View {
     const int* simplex_pos;    // original index -> BVH order
     const BvhDev* simplex_nodes;
     int n_simplex_nodes;
+    // layered materials (layers > 1): planes along layer_axis, material of each
+    // cell and of each global mode range, DMM banks by (material, direction)
+    int layers;
+    int layer_axis;
+    float layer_plane[kMaxLayers - 1];
+    int mode_begin[kMaxLayers + 1];
+    const int* cell_mat;
+    double e_bg_mat[kMaxLayers];
+    int lut_begin[kMaxLayers + 1];
+    double layer_accept[kMaxLayers]; // initial positions: smallest spatial volume / this material's
+    double band;                     // DMM frequency band, rad/ps
+    int bank_begin[2 * kMaxLayers + 1];
+    const int* bank_mode;
+    const double* bank_omega;
+    const double* bank_prefix;
     // boundary tables
     const RoughDev* rough;
     const ResDev* res;

@@ -340,6 +340,56 @@ __device__ bool mesh_contains(const View& V, const double p[3]) {
     return false;
 }
 
+// Material of a global active mode (layered domains).
+__device__ __forceinline__ int mode_material(const View& V, int m) {
+    int k = 0;
+    while (k + 1 < V.layers && m >= V.mode_begin[k + 1]) ++k;
+    return k;
+}
+
+// InterfaceModeSampler::sample_band (src/solver/InterfaceModeSampler.cpp:72-109):
+// in bank (material, direction), the modes whose frequency lies in the band
+// [floor(w / bw) bw, (floor(w / bw) + 1) bw), drawn by their cumulative flux with
+// uniform u.  Returns the global mode or -1; flux = the band's total flux.
+__device__ int sample_band(const View& V, int material, int direction, double w, double u, double& flux) {
+    const int b = 2 * material + direction;
+    const int b0 = V.bank_begin[b], n = V.bank_begin[b + 1] - b0;
+    const double* om = V.bank_omega + b0;
+    const double* pre = V.bank_prefix + b0 + b; // n + 1 entries
+    const double bin = floor(w / V.band);
+    const double lo = bin * V.band, hi = (bin + 1.0) * V.band;
+    int begin = 0, end = n;
+    { // lower_bound(lo)
+        int l = 0, h = n;
+        while (l < h) {
+            const int mid = (l + h) >> 1;
+            if (om[mid] < lo) l = mid + 1;
+            else h = mid;
+        }
+        begin = l;
+    }
+    { // lower_bound(hi)
+        int l = begin, h = n;
+        while (l < h) {
+            const int mid = (l + h) >> 1;
+            if (om[mid] < hi) l = mid + 1;
+            else h = mid;
+        }
+        end = l;
+    }
+    flux = end > begin ? pre[end] - pre[begin] : 0.0;
+    if (!(flux > 0.0)) return -1;
+    const double target = pre[begin] + u * flux;
+    int l = begin + 1, h = end + 1; // upper_bound over pre[begin+1 .. end]
+    while (l < h) {
+        const int mid = (l + h) >> 1;
+        if (pre[mid] <= target) l = mid + 1;
+        else h = mid;
+    }
+    int pos = l < begin + 1 ? begin + 1 : (l > end ? end : l);
+    return V.bank_mode[b0 + pos - 1];
+}
+
 __device__ __forceinline__ double relax_weight(const View& V, const CellRta& cr, int m) {
     // PhononMaterial::mode_relaxation_weight (src/material/Thermodynamics.cpp:95-106)
     // with mode_gamma's interpolation (:108-133); row and weight are per cell.
@@ -551,6 +601,7 @@ struct Tally {
     bool absorbed;
     int rough_kind;    // last rough event kind seen (for counters)
     int rough_events, rough_spec, rough_diff;
+    int iface_events, iface_transmitted;
 };
 
 // Free flight for `remaining` ps with up to 64 wall events
@@ -609,6 +660,71 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
                         tbest = t;
                         plane = 2 * a + side;
                     }
+                }
+            }
+            // Layered materials: the next interface plane along the axis
+            // (LayeredMaterialLayout::next_crossing, src/solver/LayeredMaterialLayout.cpp:33-60);
+            // the outer wall wins exact ties.
+            if (V.layers > 1) {
+                const int ax = V.layer_axis, mat = mode_material(V, mw & kModeMask);
+                const float va = vv[ax];
+                int dest = -1;
+                float at = 0.0f;
+                if (va > 0.0f && mat + 1 < V.layers) {
+                    at = V.layer_plane[mat];
+                    dest = mat + 1;
+                } else if (va < 0.0f && mat > 0) {
+                    at = V.layer_plane[mat - 1];
+                    dest = mat - 1;
+                }
+                const float ti = dest >= 0 ? (at - p[ax]) / va : INFINITY;
+                if (ti > 1e-14f && ti < tbest && !(ti > remaining)) {
+                    remaining -= ti;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        float x = fmaf(vv[a], ti, p[a]);
+                        p[a] = fminf(fmaxf(x, V.fbmin[a]), V.fbmax[a]);
+                    }
+                    p[ax] = at;
+                    if (ev == 0 || U.uid != uid) {
+                        U.uid = uid;
+                        U.step = step;
+                        U.k0 = V.key0;
+                        U.k1 = V.key1;
+                    }
+                    U.ev = static_cast<uint32_t>(ev);
+                    U.used = 0;
+                    const int m = mw & kModeMask;
+                    const double w = V.omega[m];
+                    const int inc = va > 0.0f ? 1 : 0;
+                    const double u1 = U.next(), u2 = U.next(), u3 = U.next();
+                    double fr, ft;
+                    // DMM (InterfaceInteraction::interact, src/solver/InterfaceInteraction.cpp:86-101,
+                    // 111-149): reflection band in this material, transmission band in the next.
+                    const int mr = sample_band(V, mat, 1 - inc, w, u1, fr);
+                    const int mt = sample_band(V, dest, inc, w, u2, ft);
+                    const double tr = ft > 0.0 ? ft / fmax(1e-300, fr + ft) : 0.0;
+                    const bool transmit = mt >= 0 && u3 < tr;
+                    const int sel = transmit ? mt : mr;
+                    T.iface_events++;
+                    double nocc = 0.0;
+                    // Energy-preserving remap with the (common) energy weight, :111-122.
+                    if (sel >= 0) nocc = V.neq[sel] + (V.w * w * (occ - V.neq[m])) / fmax(1e-300, V.w * V.omega[sel]);
+                    if (sel < 0 || !isfinite(nocc) || nocc < 0.0) {
+                        mw ^= (1 << ax) << kFlipShift; // mirror along the axis, same mode
+                        mode_changed = true;
+                        p[ax] = va > 0.0f ? nextafterf(at, -INFINITY) : nextafterf(at, INFINITY);
+                        continue;
+                    }
+                    if (transmit) T.iface_transmitted++;
+                    mw = sel;
+                    occ = nocc;
+                    mode_changed = true;
+                    occ_changed = true;
+                    // Place the carrier strictly on its new material's side.
+                    const bool up = transmit ? inc == 1 : inc == 0;
+                    p[ax] = up ? nextafterf(at, INFINITY) : nextafterf(at, -INFINITY);
+                    continue;
                 }
             }
             if (plane < 0 || tbest > remaining) {
@@ -750,6 +866,10 @@ __device__ __forceinline__ void record_events(BlockPartials& P, const Tally& T) 
         P.add(kPRoughEvents, T.rough_events);
         if (T.rough_spec) P.add(kPRoughSpecular, T.rough_spec);
         if (T.rough_diff) P.add(kPRoughDiffuse, T.rough_diff);
+    }
+    if (T.iface_events) {
+        P.add(kPIfaceEvents, T.iface_events);
+        if (T.iface_transmitted) P.add(kPIfaceTransmitted, T.iface_transmitted);
     }
 }
 
@@ -955,6 +1075,14 @@ __device__ __forceinline__ void transport_body(const View& V, const Grid G) {
                             }
                         }
                         event = plane >= 0 && !(tbest > dt);
+                        if (!event && V.layers > 1) { // a material interface within the step
+                            const int ax = V.layer_axis, mat = mode_material(V, mw[j] & kModeMask);
+                            const float va = vv[ax];
+                            float ti = INFINITY;
+                            if (va > 0.0f && mat + 1 < V.layers) ti = (V.layer_plane[mat] - p[ax]) / va;
+                            else if (va < 0.0f && mat > 0) ti = (V.layer_plane[mat - 1] - p[ax]) / va;
+                            event = ti > 1e-14f && !(ti > dt);
+                        }
                     }
                     if (!event) {
 #pragma unroll
@@ -1179,28 +1307,40 @@ __global__ void __launch_bounds__(256) kb_inject(const View* __restrict__ views,
     inject_body<kMesh>(views[i], G);
 }
 
-__device__ double lut_inverse(const View& V, double e) {
-    // PhononMaterial::temperature_from_energy_density: clamped linear interpolation.
-    const int n = V.lut_n;
-    if (e <= V.lut_e[0]) return V.lut_t[0];
-    if (e >= V.lut_e[n - 1]) return V.lut_t[n - 1];
+__device__ double lut_inverse(const View& V, double e, int base, int n) {
+    // PhononMaterial::temperature_from_energy_density: clamped linear interpolation
+    // over rows base .. base+n-1 (one material's table).
+    const double* te = V.lut_e + base;
+    const double* tt = V.lut_t + base;
+    if (e <= te[0]) return tt[0];
+    if (e >= te[n - 1]) return tt[n - 1];
     int lo = 0, hi = n; // upper_bound
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (V.lut_e[mid] <= e) lo = mid + 1;
+        if (te[mid] <= e) lo = mid + 1;
         else hi = mid;
     }
     const int h = lo, l = lo - 1;
-    const double x0 = V.lut_e[l], x1 = V.lut_e[h], y0 = V.lut_t[l], y1 = V.lut_t[h];
+    const double x0 = te[l], x1 = te[h], y0 = tt[l], y1 = tt[h];
     if (fabs(x1 - x0) <= 1e-18) return y0;
     const double t = (e - x0) / (x1 - x0);
     return y0 * (1.0 - t) + y1 * t;
 }
 
-// Cell temperature of an energy density: the lookup table, or under
-// linearized transport T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225).
-__device__ __forceinline__ double cell_temperature(const View& V, double e) {
-    return V.callaway ? V.t_bg + (e - V.e_bg) / V.cw_cbg : lut_inverse(V, e);
+// Background energy density of cell c (its material's, in a layered domain).
+__device__ __forceinline__ double cell_background(const View& V, int c) {
+    return V.layers > 1 ? V.e_bg_mat[V.cell_mat[c]] : V.e_bg;
+}
+
+// Temperature of cell c at energy density e: its material's lookup table, or
+// under linearized transport T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225).
+__device__ __forceinline__ double cell_temperature(const View& V, double e, int c) {
+    if (V.callaway) return V.t_bg + (e - V.e_bg) / V.cw_cbg;
+    if (V.layers > 1) {
+        const int m = V.cell_mat[c];
+        return lut_inverse(V, e, V.lut_begin[m], V.lut_begin[m + 1] - V.lut_begin[m]);
+    }
+    return lut_inverse(V, e, 0, V.lut_n);
 }
 
 __device__ void gamma_row(const View& V, double temp, int& lo, int& hi, double& t) {
@@ -1262,6 +1402,8 @@ __device__ __forceinline__ void refresh_body(const View& V, const Grid G, int ph
                 case kPRoughEvents: st->rough_events += static_cast<long long>(s); break;
                 case kPRoughSpecular: st->rough_specular += static_cast<long long>(s); break;
                 case kPRoughDiffuse: st->rough_diffuse += static_cast<long long>(s); break;
+                case kPIfaceEvents: st->iface_events += static_cast<long long>(s); break;
+                case kPIfaceTransmitted: st->iface_transmitted += static_cast<long long>(s); break;
                 default:
                     if (c >= kPAbsorbedRes && c < kPAbsorbedRes + kMaxReservoirs)
                         st->absorbed_now[c - kPAbsorbedRes] = static_cast<int>(s);
@@ -1332,9 +1474,9 @@ __device__ __forceinline__ void refresh_body(const View& V, const Grid G, int ph
                 if (V.hs_on) esum += V.chunk_src[k];
             }
         }
-        const double e = esum / V.cell_vol[c] + V.e_bg;
+        const double e = esum / V.cell_vol[c] + cell_background(V, c);
         V.cell_e[c] = e;
-        V.cell_T[c] = cell_temperature(V, e);
+        V.cell_T[c] = cell_temperature(V, e, c);
         local += (e - V.zp) * V.cell_vol[c];
     }
     const double energy = block_sum(local, red);
@@ -1549,8 +1691,9 @@ __device__ __forceinline__ double rta_accumulate(const View& V, const CellRta& c
 // deviational energy tallied by the sort (GridStatistics::refresh_temperatures)
 // and the rate-table row of the lifetime temperature.
 __device__ __forceinline__ CellRta cell_rta(const View& V, int c) {
-    const double e = V.cell_tally[static_cast<size_t>(kTallyStride) * c + kTallyEnergy] / V.cell_vol[c] + V.e_bg;
-    const double T = cell_temperature(V, e);
+    const double e = V.cell_tally[static_cast<size_t>(kTallyStride) * c + kTallyEnergy] / V.cell_vol[c] +
+                     cell_background(V, c);
+    const double T = cell_temperature(V, e, c);
     CellRta cr{};
     cr.t0 = T;
     cr.high = fmax(1000.0, 2.0 * fmax(1.0, T));
@@ -2644,13 +2787,45 @@ __global__ void __launch_bounds__(256) k_init(View V, int count, int tmode, doub
         V.cell[i] = 0;
         return;
     }
+    int layer = 0;
+    if (V.layers > 1) {
+        // Layered materials: carriers denser where the phase-space density is,
+        // by rejection with the ratio of spatial volumes (src/solver/Initialization.cpp:49-66),
+        // up to 64 trials on Philox words (carrier, trial + 1, 0 and 1).
+        bool accepted = false;
+        for (uint32_t trial = 0; trial < 64 && !accepted; ++trial) {
+            const uint4 ta = philox(static_cast<uint32_t>(i), 1 + trial, 0, kStreamInit, V.key0, V.key1);
+            const uint4 tb = philox(static_cast<uint32_t>(i), 1 + trial, 1, kStreamInit, V.key0, V.key1);
+            const double q[3] = {u53(ta.x, ta.y), u53(ta.z, ta.w), u53(tb.x, tb.y)};
+            for (int k = 0; k < 3; ++k) p[k] = V.bmin[k] + (V.bmax[k] - V.bmin[k]) * q[k];
+            const float pa = fminf(fmaxf(static_cast<float>(p[V.layer_axis]), V.fbmin[V.layer_axis]),
+                                   V.fbmax[V.layer_axis]);
+            layer = 0;
+            while (layer + 1 < V.layers && pa >= V.layer_plane[layer]) ++layer;
+            accepted = u53(tb.z, tb.w) < V.layer_accept[layer];
+        }
+        if (!accepted) {
+            raise_error(V, PMC_ERR_BOUNDARY, i, nullptr, -11);
+            V.mode[i] = -1;
+            V.uid[i] = static_cast<unsigned>(i);
+            V.cell[i] = 0;
+            return;
+        }
+    }
     float fp[3];
     for (int k = 0; k < 3; ++k) {
         fp[k] = static_cast<float>(p[k]);
         fp[k] = fminf(fmaxf(fp[k], V.fbmin[k]), V.fbmax[k]);
     }
-    int m = static_cast<int>(u[3] * V.na);
-    m = m < V.na ? m : V.na - 1;
+    int m;
+    if (V.layers > 1) { // uniform over the carrier's material (src/solver/Initialization.cpp:140-144)
+        const int b0 = V.mode_begin[layer], nm = V.mode_begin[layer + 1] - b0;
+        const int k = static_cast<int>(u[3] * nm);
+        m = b0 + (k < nm ? k : nm - 1);
+    } else {
+        m = static_cast<int>(u[3] * V.na);
+        m = m < V.na ? m : V.na - 1;
+    }
     double T = t0;
     const int cell = grid_index(V, fp[0], fp[1], fp[2]);
     if (tmode == 1) {

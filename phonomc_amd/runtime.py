@@ -94,6 +94,15 @@ class CallawayTable(C.Structure):
                 ("heat_capacity_density", C.c_double), ("background_energy_density", C.c_double)]
 
 
+class Layers(C.Structure):
+    _fields_ = [("count", C.c_int32), ("axis", C.c_int32), ("planes", C.POINTER(C.c_double)),
+                ("mode_begin", C.POINTER(C.c_int32)), ("cell_material", C.POINTER(C.c_int32)),
+                ("background_energy_density", C.POINTER(C.c_double)), ("lut_begin", C.POINTER(C.c_int32)),
+                ("spatial_volume", C.POINTER(C.c_double)), ("band_width", C.c_double),
+                ("bank_begin", C.POINTER(C.c_int32)), ("bank_mode", C.POINTER(C.c_int32)),
+                ("bank_omega", C.POINTER(C.c_double)), ("bank_prefix", C.POINTER(C.c_double))]
+
+
 class LaunchConfig(C.Structure):
     _fields_ = [("event_split", C.c_int32), ("callaway_threads", C.c_int32)]
 
@@ -114,7 +123,8 @@ class Ledger(C.Structure):
                 ("rough_specular", C.c_int64), ("rough_diffuse", C.c_int64), ("newton_passes", C.c_int64),
                 ("newton_evals", C.c_int64), ("step_source_energy_ev", C.c_double),
                 ("total_source_energy_ev", C.c_double), ("source_updates", C.c_int64),
-                ("source_exact_passes", C.c_int64)]
+                ("source_exact_passes", C.c_int64), ("interface_events", C.c_int64),
+                ("interface_transmitted", C.c_int64)]
 
 
 class Error(C.Structure):
@@ -210,6 +220,9 @@ def lib():
         "pmc_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pmc_destroy": (None, [P]),
         "pmc_status_string": (C.c_char_p, [C.c_int32]),
+        "pmcs_problem_layers": (C.c_int, [P, C.POINTER(Layers)]),
+        "pmcs_problem_layer_material": (C.c_int, [P, C.c_int32, C.POINTER(MaterialView)]),
+        "pmc_upload_layers": (C.c_int, [P, C.POINTER(Layers)]),
         "pmc_get_launch_config": (C.c_int, [P, C.POINTER(LaunchConfig)]),
         "pmc_set_launch_config": (C.c_int, [P, C.POINTER(LaunchConfig)]),
         "pmc_autotune": (C.c_int, [P, C.c_int32, C.POINTER(LaunchConfig), C.POINTER(C.c_double), C.c_int32]),
@@ -291,6 +304,17 @@ class Problem:
         lib().pmcs_problem_material(self.h, C.byref(mv))
         return mv
 
+    def layers(self):
+        """The layered-materials view, or None for one material."""
+        lv = Layers()
+        return lv if lib().pmcs_problem_layers(self.h, C.byref(lv)) == 1 else None
+
+    def layer_material(self, k):
+        mv = MaterialView()
+        if lib().pmcs_problem_layer_material(self.h, k, C.byref(mv)) != 0:
+            raise IndexError(k)
+        return mv
+
     def scalars(self):
         out = (C.c_double * 10)()
         lib().pmcs_problem_scalars(self.h, out)
@@ -346,6 +370,9 @@ class Context:
         cw = problem.callaway()
         if cw is not None:
             self._check(L.pmc_upload_callaway(h, C.byref(cw)))
+        lv = problem.layers()
+        if lv is not None:
+            self._check(L.pmc_upload_layers(h, C.byref(lv)))
         hs = problem.heat_source()
         if hs is not None:
             self._check(L.pmc_upload_heat_source(h, C.byref(hs)))

@@ -154,10 +154,15 @@ def test_unsupported_features_fail_loudly():
     # Callaway on a table without an N/U split: the reference's callaway_data message.
     with pytest.raises(RuntimeError, match="Callaway requires gamma_N and gamma_U"):
         rt.Problem(rt.config_text("c1_cross_plane.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))
-    # Callaway with a heat source is outside this build.
-    with pytest.raises(RuntimeError, match="heat_source with scattering.model = 'callaway'"):
-        rt.Problem(rt.config_text("c1s_source.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]")
-                   .replace("synthetic:si16", "synthetic:si15:callaway"))
+    # Layered materials take the relaxation-time model, DMM interfaces and no heat source.
+    with pytest.raises(RuntimeError, match="Layered materials with a heat source"):
+        rt.Problem(rt.config_text("c1l_layered.toml", **{"heat_source.enabled": "true",
+                                                          "heat_source.power_density": "1e18",
+                                                          "heat_source.profile": '"gaussian"',
+                                                          "heat_source.center": "[50, 50, 50]",
+                                                          "heat_source.sigma": "[10, 10, 10]"}))
+    with pytest.raises(RuntimeError, match="interface_model = 'amm' is not supported"):
+        rt.Problem(rt.config_text("c1l_layered.toml", **{"materials.interface_model": '"amm"'}))
 
 
 def test_heat_source_rates_match_reference(golden):
