@@ -1496,8 +1496,6 @@ int orc_step(orc_sim* s, int32_t nsteps) {
                 total.rough_events += local.rough_events;
                 total.rough_spec += local.rough_spec;
                 total.rough_diff += local.rough_diff;
-            total.iface_events += local.iface_events;
-            total.iface_transmitted += local.iface_transmitted;
                 total.iface_events += local.iface_events;
                 total.iface_transmitted += local.iface_transmitted;
                 for (int r = 0; r < nres; ++r) total.absorbed_by_res[r] += local.absorbed_by_res[r];

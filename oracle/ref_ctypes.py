@@ -60,27 +60,33 @@ def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
+def material_arrays(view, lookup_dt=0.1):
+    """The ref_material_arrays struct of a material view (pointers into the view's arrays)."""
+    m = MaterialArrays()
+    m.volume_a3 = view.volume_a3
+    for k in range(9):
+        m.reciprocal[k] = view.reciprocal[k]
+    m.qpoints, m.branches = view.qpoints, view.branches
+    for k in range(3):
+        m.mesh[k] = view.mesh[k]
+    m.frequency_thz = C.cast(view.frequency_thz, C.POINTER(C.c_double))
+    m.qpoint_fractions = C.cast(view.qpoint_fractions, C.POINTER(C.c_double))
+    m.group_velocity = C.cast(view.group_velocity, C.POINTER(C.c_double))
+    m.ntemps = view.ntemps
+    m.temperatures = C.cast(view.temperatures, C.POINTER(C.c_double))
+    m.gamma = C.cast(view.gamma, C.POINTER(C.c_double))
+    m.gamma_normal = C.cast(view.gamma_normal, C.POINTER(C.c_double)) if view.gamma_normal else None
+    m.gamma_umklapp = C.cast(view.gamma_umklapp, C.POINTER(C.c_double)) if view.gamma_umklapp else None
+    m.gamma_isotope = C.cast(view.gamma_isotope, C.POINTER(C.c_double)) if view.gamma_isotope else None
+    m.temperature_lookup_dt = lookup_dt
+    return m
+
+
 class RefMaterial:
     def __init__(self, view, lookup_dt=0.1, include_stationary=False):
         """view: a pmcs_material_view (phonomc_amd.runtime.MaterialView) or compatible struct.
         include_stationary: the constructor flag of conservative transport (Callaway)."""
-        m = MaterialArrays()
-        m.volume_a3 = view.volume_a3
-        for k in range(9):
-            m.reciprocal[k] = view.reciprocal[k]
-        m.qpoints, m.branches = view.qpoints, view.branches
-        for k in range(3):
-            m.mesh[k] = view.mesh[k]
-        m.frequency_thz = C.cast(view.frequency_thz, C.POINTER(C.c_double))
-        m.qpoint_fractions = C.cast(view.qpoint_fractions, C.POINTER(C.c_double))
-        m.group_velocity = C.cast(view.group_velocity, C.POINTER(C.c_double))
-        m.ntemps = view.ntemps
-        m.temperatures = C.cast(view.temperatures, C.POINTER(C.c_double))
-        m.gamma = C.cast(view.gamma, C.POINTER(C.c_double))
-        m.gamma_normal = C.cast(view.gamma_normal, C.POINTER(C.c_double)) if view.gamma_normal else None
-        m.gamma_umklapp = C.cast(view.gamma_umklapp, C.POINTER(C.c_double)) if view.gamma_umklapp else None
-        m.gamma_isotope = C.cast(view.gamma_isotope, C.POINTER(C.c_double)) if view.gamma_isotope else None
-        m.temperature_lookup_dt = lookup_dt
+        m = material_arrays(view, lookup_dt)
         self.arrays = m
         err = C.create_string_buffer(512)
         h = C.c_void_p()
@@ -316,6 +322,48 @@ def run(toml_path, material_view, steps, threads=0, lookup_dt=0.1):
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return secs.value, ps.value
+
+
+def run_materials(toml_path, material_views, steps, threads=0, lookup_dt=0.1):
+    """ref_run_materials: (seconds, particle_steps, interface events, interface transmissions,
+    run summary text)."""
+    arr = (MaterialArrays * len(material_views))(*[material_arrays(v, lookup_dt) for v in material_views])
+    secs, ps = C.c_double(), C.c_longlong()
+    iface = (C.c_longlong * 2)()
+    err = C.create_string_buffer(1024)
+    summary = C.create_string_buffer(1 << 16)
+    rc = lib().ref_run_materials(toml_path.encode(), arr, len(material_views), steps, threads, C.byref(secs),
+                                 C.byref(ps), iface, summary, 1 << 16, err, 1024)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return secs.value, ps.value, int(iface[0]), int(iface[1]), summary.value.decode()
+
+
+def interface_dmm(material_views, axis, bin_thz, t_ref, weights, material, active, velocity, occupation,
+                  next_material, seeds, lookup_dt=0.1):
+    """ref_interface_dmm on n carriers: dict of action, material, active, occupation,
+    velocity (n x 3) and the consumed uniforms (n x 3)."""
+    arr = (MaterialArrays * len(material_views))(*[material_arrays(v, lookup_dt) for v in material_views])
+    n = len(material)
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    mat, act, vel, occ, nxt = i32(material), i32(active), f64(velocity).reshape(-1), f64(occupation), i32(next_material)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+    w = f64(weights)
+    out = dict(action=np.zeros(n, np.int32), material=np.zeros(n, np.int32), active=np.zeros(n, np.int32),
+               occupation=np.zeros(n), velocity=np.zeros(3 * n), uniforms=np.zeros(3 * n))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    err = C.create_string_buffer(1024)
+    rc = lib().ref_interface_dmm(arr, len(material_views), int(axis), C.c_double(bin_thz), C.c_double(t_ref), _dp(w),
+                                 n, ip(mat), ip(act), _dp(vel), _dp(occ), ip(nxt),
+                                 sd.ctypes.data_as(C.POINTER(C.c_uint64)), ip(out["action"]), ip(out["material"]),
+                                 ip(out["active"]), _dp(out["occupation"]), _dp(out["velocity"]), _dp(out["uniforms"]),
+                                 err, 1024)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    out["velocity"] = out["velocity"].reshape(n, 3)
+    out["uniforms"] = out["uniforms"].reshape(n, 3)
+    return out
 
 
 def callaway_data(material, temperature):

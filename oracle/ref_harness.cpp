@@ -19,6 +19,8 @@
 #include "solver/RoughBoundarySampler.h"
 #include "solver/RtaCollisionOperator.h"
 #include "solver/CarrierCollisionOperator.h"
+#include "solver/InterfaceInteraction.h"
+#include "solver/InterfaceModeSampler.h"
 
 #include <chrono>
 #include <cstring>
@@ -26,6 +28,7 @@
 #include <functional>
 #include <memory>
 #include <random>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -476,6 +479,14 @@ int ref_source_apply(void* domain, void* material, double integrated_time_ps, co
 
 int ref_run(const char* toml_path, const ref_material_arrays* arrays, int steps, int threads,
             double* seconds, long long* particle_steps, char* err, int errlen) {
+    long long interface[2];
+    return ref_run_materials(toml_path, arrays, 1, steps, threads, seconds, particle_steps, interface, nullptr, 0, err,
+                             errlen);
+}
+
+int ref_run_materials(const char* toml_path, const ref_material_arrays* arrays, int n_materials, int steps,
+                      int threads, double* seconds, long long* particle_steps, long long* interface, char* summary,
+                      int summary_len, char* err, int errlen) {
     return guarded(err, errlen, [&] {
 #ifdef PHONOMC_USE_OPENMP
         if (threads > 0) omp_set_num_threads(threads);
@@ -485,10 +496,14 @@ int ref_run(const char* toml_path, const ref_material_arrays* arrays, int steps,
         SimulationConfig config = load_simulation_config(toml_path);
         SimulationDomain domain(config);
         // Constructor flags of MaterialFactory (src/material/MaterialFactory.cpp:51-55).
-        PhononMaterial material(to_material_data(*arrays), config.temperature_lookup_dt,
-                                uses_conservative_transport(config) ||
-                                    config.heat_source_spectrum == HeatSourceSpectrum::Gaussian);
-        std::vector<std::reference_wrapper<const PhononMaterial>> materials{std::cref(material)};
+        std::vector<std::unique_ptr<PhononMaterial>> storage;
+        std::vector<std::reference_wrapper<const PhononMaterial>> materials;
+        for (int k = 0; k < n_materials; ++k) {
+            storage.push_back(std::make_unique<PhononMaterial>(
+                to_material_data(arrays[k]), config.temperature_lookup_dt,
+                uses_conservative_transport(config) || config.heat_source_spectrum == HeatSourceSpectrum::Gaussian));
+            materials.push_back(std::cref(*storage.back()));
+        }
         MonteCarloSolver solver(config, domain, materials);
         long long carried = 0;
         const auto begin = std::chrono::steady_clock::now();
@@ -499,6 +514,76 @@ int ref_run(const char* toml_path, const ref_material_arrays* arrays, int steps,
         const auto end = std::chrono::steady_clock::now();
         *seconds = std::chrono::duration<double>(end - begin).count();
         *particle_steps = carried;
+        interface[0] = solver.interface_event_count();
+        interface[1] = solver.interface_transmitted_count();
+        if (summary && summary_len > 0) {
+            std::ostringstream text;
+            solver.append_profile_summary(text);
+            const std::string t = text.str();
+            const size_t k = std::min(t.size(), static_cast<size_t>(summary_len - 1));
+            std::memcpy(summary, t.data(), k);
+            summary[k] = '\0';
+        }
+    });
+}
+
+int ref_interface_dmm(const ref_material_arrays* arrays, int n_materials, int axis, double frequency_bin_thz,
+                      double reference_temperature, const double* energy_weights, int n, const int* material,
+                      const int* active, const double* velocity, const double* occupation,
+                      const int* next_material, const uint64_t* seeds, int* action, int* out_material,
+                      int* out_active, double* out_occupation, double* out_velocity, double* uniforms, char* err,
+                      int errlen) {
+    return guarded(err, errlen, [&] {
+        std::vector<std::unique_ptr<PhononMaterial>> storage;
+        std::vector<const PhononMaterial*> mats;
+        for (int k = 0; k < n_materials; ++k) {
+            storage.push_back(std::make_unique<PhononMaterial>(to_material_data(arrays[k]),
+                                                               arrays[k].temperature_lookup_dt));
+            mats.push_back(storage.back().get());
+        }
+        phonomc::InterfaceInteractionConfig cfg;
+        cfg.axis = axis;
+        cfg.enabled = true;
+        cfg.linearized = false;
+        cfg.model = InterfaceModel::DiffuseMismatch;
+        cfg.frequency_bin_thz = frequency_bin_thz;
+        cfg.reference_temperature = reference_temperature;
+        phonomc::InterfaceInteraction interaction;
+        interaction.configure(cfg, mats, std::vector<double>(energy_weights, energy_weights + n_materials));
+        // A second sampler on the same tables tells which draws an event makes
+        // (a band draw only when the band carries flux, the choice only when
+        // the transmission band does).
+        phonomc::InterfaceModeSampler bands;
+        bands.configure(mats, axis);
+        const double band_width = 2.0 * std::acos(-1.0) * frequency_bin_thz;
+        for (int i = 0; i < n; ++i) {
+            const PhononMaterial& src = *mats.at(static_cast<size_t>(material[i]));
+            phonomc::BoundaryState in;
+            in.material = material[i];
+            in.mode = src.active_mode_at(active[i]);
+            for (int a = 0; a < 3; ++a) in.velocity[static_cast<size_t>(a)] = velocity[3 * i + a];
+            in.omega = src.mode_angular_frequency(in.mode);
+            in.occupation = occupation[i];
+            const int inc = in.velocity[static_cast<size_t>(axis)] > 0.0 ? 1 : 0;
+            double fr = 0.0, ft = 0.0;
+            std::mt19937_64 scratch(0);
+            bands.sample_band(material[i], 1 - inc, in.omega, band_width, scratch, fr);
+            bands.sample_band(next_material[i], inc, in.omega, band_width, scratch, ft);
+            std::mt19937_64 rng(seeds[i]);
+            std::mt19937_64 probe = rng;
+            double* u = uniforms + 3 * i;
+            u[0] = fr > 0.0 ? canonical(probe) : 0.5;
+            u[1] = ft > 0.0 ? canonical(probe) : 0.5;
+            u[2] = ft > 0.0 ? canonical(probe) : 0.5;
+            const auto r = interaction.interact(in, next_material[i], rng);
+            if (r.action == phonomc::BoundaryAction::Transmit) action[i] = 1;
+            else if (r.state.material == in.material && r.state.mode == in.mode) action[i] = -1;
+            else action[i] = 0;
+            out_material[i] = r.state.material;
+            out_active[i] = mats.at(static_cast<size_t>(r.state.material))->active_index_for_mode(r.state.mode);
+            out_occupation[i] = r.state.occupation;
+            for (int a = 0; a < 3; ++a) out_velocity[3 * i + a] = r.state.velocity[static_cast<size_t>(a)];
+        }
     });
 }
 } // extern "C"

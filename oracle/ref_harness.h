@@ -132,6 +132,30 @@ int ref_rta_apply(void* material, int ncells, const double* cell_temperature, in
 int ref_run(const char* toml_path, const ref_material_arrays* arrays, int steps, int threads,
             double* seconds, long long* particle_steps, char* err, int errlen);
 
+/* The same over several in-memory materials (materials.folders, in order);
+ * interface[2] = MonteCarloSolver::interface_event_count and
+ * interface_transmitted_count after the last step; summary (if not null) =
+ * MonteCarloSolver::append_profile_summary's text, truncated to summary_len - 1. */
+int ref_run_materials(const char* toml_path, const ref_material_arrays* arrays, int n_materials, int steps,
+                      int threads, double* seconds, long long* particle_steps, long long* interface, char* summary,
+                      int summary_len, char* err, int errlen);
+
+/* InterfaceInteraction::interact (DMM) on n carriers: material[i], active
+ * index active[i] of that material, velocity[3i..], occupation[i], toward
+ * next_material[i], with std::mt19937_64(seeds[i]).  Interface settings as
+ * MonteCarloSolver::initialize_interface_mode_banks sets them (axis, bin,
+ * reference temperature, per-material energy weights).  Outputs: action (1
+ * transmitted, 0 reflected into a new mode, -1 axis mirror of the same mode),
+ * material, active index, occupation and velocity after the event, and the
+ * uniforms on [0, 1) the event consumed in the order (reflection band,
+ * transmission band, transmit choice); 0.5 where the event drew none. */
+int ref_interface_dmm(const ref_material_arrays* arrays, int n_materials, int axis, double frequency_bin_thz,
+                      double reference_temperature, const double* energy_weights, int n, const int* material,
+                      const int* active, const double* velocity, const double* occupation,
+                      const int* next_material, const uint64_t* seeds, int* action, int* out_material,
+                      int* out_active, double* out_occupation, double* out_velocity, double* uniforms, char* err,
+                      int errlen);
+
 #ifdef __cplusplus
 }
 #endif

@@ -380,8 +380,13 @@ SimConfig build(const std::map<std::string, std::string>& kv, const std::string&
             c.interface_model = m;
         }
         if (auto v = get("materials.interface_frequency_bin_thz")) c.interface_bin_thz = to_double(*v);
+        if (auto v = get("materials.interface_amm_fraction")) c.interface_amm_fraction = to_double(*v);
+        if (auto v = get("materials.interface_parallel_bin_inv_a")) c.interface_parallel_bin = to_double(*v);
+        if (auto v = get("materials.mass_densities_kg_m3")) c.mass_densities = numbers(*v);
         if (c.interface_model == "mmm" && !get("materials.interface_amm_fraction"))
             throw std::runtime_error("MMM requires explicit materials.interface_amm_fraction.");
+        if (c.interface_model == "amm" && get("materials.interface_amm_fraction") && c.interface_amm_fraction != 1)
+            throw std::runtime_error("AMM explicit fraction must equal 1; use MMM for intermediate weights.");
     }
     if (auto v = get("mode_excitation.enabled")) c.has_mode_excitation = to_bool(*v);
 
@@ -518,6 +523,50 @@ static void validate_heat_source(const SimConfig& c) {
     }
 }
 
+// validate_material_config (src/config/ConfigValidation.cpp:109-193), same messages and order.
+static void validate_materials(const SimConfig& c) {
+    if (!std::isfinite(c.interface_bin_thz) || c.interface_bin_thz <= 0)
+        throw std::runtime_error("materials.interface_frequency_bin_thz must be finite and positive.");
+    if (!std::isfinite(c.interface_amm_fraction) || c.interface_amm_fraction < 0 || c.interface_amm_fraction > 1)
+        throw std::runtime_error("materials.interface_amm_fraction must be in [0,1].");
+    if (!std::isfinite(c.interface_parallel_bin) || c.interface_parallel_bin <= 0)
+        throw std::runtime_error("materials.interface_parallel_bin_inv_a must be finite and positive.");
+    if (c.interface_model == "dmm" && c.interface_amm_fraction != 0)
+        throw std::runtime_error("DMM cannot have nonzero AMM fraction.");
+    if (c.interface_model == "amm" || (c.interface_model == "mmm" && c.interface_amm_fraction > 0)) {
+        if (c.material_folders.size() < 2 || c.mass_densities.size() != c.material_folders.size())
+            throw std::runtime_error("AMM/MMM requires one mass density per material layer.");
+        for (double rho : c.mass_densities)
+            if (!std::isfinite(rho) || rho <= 0)
+                throw std::runtime_error("Material mass densities must be finite and positive.");
+    }
+    if (c.material_folders.empty()) return;
+    if (c.material_folders.size() < 2)
+        throw std::runtime_error("materials.folders must contain at least two folders; use io.material_folder for a "
+                                 "single material.");
+    if (c.interface_positions.size() + 1 != c.material_folders.size())
+        throw std::runtime_error("materials.interface_positions must contain exactly folders.size()-1 values.");
+    for (size_t k = 1; k < c.interface_positions.size(); ++k)
+        if (!(c.interface_positions[k - 1] < c.interface_positions[k]))
+            throw std::runtime_error("materials.interface_positions must be strictly increasing.");
+    const double extent = c.sizes.size() == 3 ? c.sizes[static_cast<size_t>(c.interface_axis)] : 0.0;
+    const int cells = c.interface_axis == 0 ? c.grid[0] : c.interface_axis == 1 ? c.grid[1] : c.grid[2];
+    const double width = extent / static_cast<double>(cells);
+    const double tol = std::max(1e-10, 1e-10 * extent);
+    for (double x : c.interface_positions) {
+        if (!std::isfinite(x)) throw std::runtime_error("materials.interface_positions must be finite.");
+        if (c.model != "box") continue; // mesh bounds are checked after loading
+        if (!(x > 0.0) || !(x < extent))
+            throw std::runtime_error("materials.interface_positions must lie strictly inside the domain along "
+                                     "interface_axis.");
+        const double g = x / width;
+        if (std::abs(g - std::round(g)) > tol / width)
+            throw std::runtime_error("materials.interface_positions must coincide with grid-cell boundaries.");
+    }
+    for (const auto& f : c.material_folders)
+        if (trim(f).empty()) throw std::runtime_error("materials.folders entries must not be empty.");
+}
+
 void validate_config(const SimConfig& c) {
     // Subset of src/config/ConfigValidation.cpp:457-502 and :475-501 with the same messages.
     if (!(c.temperature_lookup_dt > 0.0) || !std::isfinite(c.temperature_lookup_dt))
@@ -553,6 +602,8 @@ void validate_config(const SimConfig& c) {
             throw std::runtime_error("boundary.periodic_pair must contain every periodic boundary region exactly "
                                      "once, in pairs.");
     }
+    if (c.heat_source_enabled) validate_heat_source(c);
+    validate_materials(c);
     if (!std::isfinite(c.background_temperature) || c.background_temperature < 0.0)
         throw std::runtime_error("background_temperature must be a finite non-negative value.");
     if (!std::isfinite(c.lifetime_temperature) || c.lifetime_temperature < 0.0)
@@ -573,7 +624,6 @@ void validate_config(const SimConfig& c) {
              return !std::isfinite(v) || !(v > 0.0);
          })))
         throw std::runtime_error("geometry.sizes for box must contain three positive values.");
-    if (c.heat_source_enabled) validate_heat_source(c);
 }
 
 SimConfig parse_toml_text(const std::string& text, const std::string& input_directory) {

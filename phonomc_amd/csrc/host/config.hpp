@@ -72,6 +72,9 @@ struct SimConfig {
     std::vector<double> interface_positions; // Angstrom
     std::string interface_model = "dmm";
     double interface_bin_thz = 0.1;
+    double interface_amm_fraction = 0.0;
+    double interface_parallel_bin = 0.05; // 1/Angstrom
+    std::vector<double> mass_densities;   // kg/m^3, one per layer (AMM/MMM)
     bool has_convergence_stop = false;
     bool has_mode_excitation = false;
     bool has_non_thermal_spectrum = false;

@@ -747,6 +747,27 @@ void MonteCarloSolver::append_profile_summary(std::ostream& out) const {
     out << "diffuse_selected = " << l.rough_diffuse << '\n';
     out << "specular_ratio = " << static_cast<double>(l.rough_specular) / den << '\n';
     out << "diffuse_ratio = " << static_cast<double>(l.rough_diffuse) / den << '\n';
+    // [material_interface_diagnostics] (src/solver/ResultWriter.cpp:240-289), DMM keys.
+    const SimConfig& c = problem_->config;
+    const auto& L = problem_->layers;
+    const int materials = std::max<int>(1, static_cast<int>(c.material_folders.size()));
+    const double iden = std::max(1.0, static_cast<double>(l.interface_events));
+    out << "\n[material_interface_diagnostics]\n";
+    out << "material_count = " << materials << '\n';
+    out << "model = " << c.interface_model << '\n';
+    if (materials > 1) {
+        out << "frequency_bin_thz = " << c.interface_bin_thz << '\n';
+        out << "amm_fraction = 0\n";
+        out << "carrier_sampling = equal_physical_state_weight\n";
+        out << "fixed_flux_reservoirs = true\n";
+        for (size_t k = 0; k < L.spatial_volume.size(); ++k)
+            out << "material_" << k << "_carrier_volume_a3 = " << L.spatial_volume[k] << '\n';
+    }
+    out << "events_total = " << l.interface_events << '\n';
+    out << "transmitted_total = " << l.interface_transmitted << '\n';
+    out << "reflected_total = " << (l.interface_events - l.interface_transmitted) << '\n';
+    out << "transmission_ratio = " << static_cast<double>(l.interface_transmitted) / iden << '\n';
+    out << "reflection_ratio = " << static_cast<double>(l.interface_events - l.interface_transmitted) / iden << '\n';
     out << "\n[gpu]\n";
     out << "device_steps = " << l.step << '\n';
     out << "carrier_steps = " << l.carrier_steps << '\n';

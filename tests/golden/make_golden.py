@@ -5,6 +5,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --callaway [--callaway-runs]   (Callaway fixtures only)
     python tests/golden/make_golden.py --mesh [--mesh-runs]           (FinFET mesh fixtures only)
     python tests/golden/make_golden.py --callaway-source [--runs]     (Callaway + heat source only)
+    python tests/golden/make_golden.py --layered [--runs]             (layered Si/Ge film only)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -29,7 +30,7 @@ from phonomc_amd import runtime as rt  # noqa: E402
 OUT = os.path.dirname(os.path.abspath(__file__))
 WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml",
              "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml",
-             "c1cs": "c1cs_callaway_source.toml"}
+             "c1cs": "c1cs_callaway_source.toml", "c1l": "c1l_layered.toml"}
 
 
 def write_input(text, folder, name):
@@ -250,9 +251,73 @@ def mesh_vectors(problem, rd, rng):
     return out
 
 
+def carrier_volumes(key, n=20000):
+    """material_k_carrier_volume_a3 of the reference's run summary (one step)."""
+    tmp = tempfile.mkdtemp()
+    text = rt.config_text(WORKLOADS[key], **{"simulation.particle_count": n, "simulation.iterations": 1,
+                                             "io.output_folder": f'"{tmp}/out"'})
+    path = write_input(text, tmp, "run.toml")
+    problem = rt.Problem(text, input_directory=rt.CONFIGS)
+    lv = problem.layers()
+    views = [problem.layer_material(k) for k in range(lv.count)]
+    summary = R.run_materials(path, views, 1, threads=8)[4]
+    found = dict(re.findall(r"^material_(\d+)_carrier_volume_a3 = (\S+)$", summary, re.M))
+    return np.array([float(found[str(k)]) for k in range(lv.count)])
+
+
+def layered_vectors(problem, rng, volumes, bin_thz, t_ref, n=4000):
+    """Layered film: per-material energy weights the reference's way
+    (BackgroundCache, include/solver/BackgroundCache.h:13-19, with the carrier
+    volumes of the reference's run summary) and InterfaceInteraction::interact
+    (DMM) on carriers of both materials heading into the interface, some with a
+    mirrored (axis-flipped) velocity, some with occupations whose remap turns
+    negative."""
+    s, m, g, i, rough, res = problem.views()
+    lv = problem.layers()
+    mb = rt.arr(lv.mode_begin, lv.count + 1, np.int32)
+    views = [problem.layer_material(k) for k in range(lv.count)]
+    refs = [R.RefMaterial(v) for v in views]
+    counts = np.array([lib_count(r) for r in refs])
+    norms = np.array([r.normalization() for r in refs])
+    weights = counts * volumes / norms
+    na = m.n_active
+    vel = rt.arr(m.velocity, 3 * na).reshape(na, 3)
+    neq = rt.arr(m.neq_background, na)
+    ax = lv.axis
+    mat = rng.integers(0, lv.count, n).astype(np.int32)
+    flipped = rng.random(n) < 0.2
+    act, v, occ = np.zeros(n, np.int32), np.zeros((n, 3)), np.zeros(n)
+    for k in range(n):
+        up = mat[k] == 0  # material 0 lies below the interface
+        while True:
+            a = int(rng.integers(0, mb[mat[k] + 1] - mb[mat[k]]))
+            va = vel[mb[mat[k]] + a, ax] * (-1.0 if flipped[k] else 1.0)
+            if (up and va > 0) or (not up and va < 0):
+                break
+        act[k] = a
+        v[k] = vel[mb[mat[k]] + a]
+        if flipped[k]:
+            v[k, ax] = -v[k, ax]
+        occ[k] = neq[mb[mat[k]] + a] * (1.0 + 0.5 * rng.normal()) - (5.0 if rng.random() < 0.05 else 0.0)
+    nxt = (1 - mat).astype(np.int32)
+    seeds = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+    r = R.interface_dmm(views, ax, bin_thz, t_ref, weights, mat, act, v, occ, nxt, seeds)
+    for x in refs:
+        x.close()
+    return dict(bin_thz=bin_thz, t_ref=t_ref, carrier_volumes=volumes, energy_weights=weights, active_counts=counts, normalizations=norms,
+                material=mat, active=act, velocity=v, occupation=occ, next_material=nxt, uniforms=r["uniforms"],
+                out_action=r["action"], out_material=r["material"], out_active=r["active"],
+                out_occupation=r["occupation"], out_velocity=r["velocity"])
+
+
+def lib_count(refmat):
+    return R.lib().ref_material_active_count(refmat.h)
+
+
 def whole_run(key, n, steps, seeds, interval):
-    """Reference runs at reduced size: final cell temperatures and kappa per seed."""
-    temps, keff, kint = [], [], []
+    """Reference runs at reduced size: final cell temperatures and kappa per seed
+    (and, for layered inputs, the interface event and transmission totals)."""
+    temps, keff, kint, iface = [], [], [], []
     for seed in seeds:
         tmp = tempfile.mkdtemp()
         text = rt.config_text(WORKLOADS[key], **{"simulation.particle_count": n, "simulation.iterations": steps,
@@ -261,7 +326,12 @@ def whole_run(key, n, steps, seeds, interval):
                                                  "io.output_folder": f'"{tmp}/out"'})
         path = write_input(text, tmp, "run.toml")
         problem = rt.Problem(text, input_directory=rt.CONFIGS)
-        R.run(path, problem.material(), steps, threads=8)
+        lv = problem.layers()
+        if lv is None:
+            R.run(path, problem.material(), steps, threads=8)
+        else:
+            out = R.run_materials(path, [problem.layer_material(k) for k in range(lv.count)], steps, threads=8)
+            iface.append(out[2:4])
         rows = [line.split() for line in open(os.path.join(tmp, "out", "convergence.txt")) if not line.startswith("#")]
         ncell = int(problem.scalars()["cells"])
         # average the rows of the second half of the run
@@ -270,11 +340,30 @@ def whole_run(key, n, steps, seeds, interval):
         temps.append(tt.mean(axis=0))
         kint.append(np.mean([float(r[2 + ncell + 1]) for r in half]))
         keff.append(np.mean([float(r[2 + ncell + 2]) for r in half]))
-    return dict(temperature=np.array(temps), kappa_eff=np.array(keff), kappa_int=np.array(kint),
-                n=n, steps=steps, seeds=np.array(seeds), interval=interval)
+    out = dict(temperature=np.array(temps), kappa_eff=np.array(keff), kappa_int=np.array(kint),
+               n=n, steps=steps, seeds=np.array(seeds), interval=interval)
+    if iface:
+        out["interface"] = np.array(iface)
+    return out
 
 
 def main():
+    if "--layered" in sys.argv:
+        rng = np.random.default_rng(20261207)
+        text = rt.config_text(WORKLOADS["c1l"])
+        problem = rt.Problem(text, input_directory=rt.CONFIGS, mode_sampling=1)
+        np.savez_compressed(os.path.join(OUT, "interface_c1l.npz"),
+                            **layered_vectors(problem, rng, carrier_volumes("c1l"), 0.1, 300.0))
+        ge = rt.config_text(WORKLOADS["c1"], **{"io.material_folder": '"synthetic:ge16"'})
+        pg = rt.Problem(ge, mode_sampling=1)
+        rg = R.RefMaterial(pg.material())
+        np.savez_compressed(os.path.join(OUT, "material_ge16.npz"), **material_vectors(pg, rg, rng))
+        rg.close()
+        if "--runs" in sys.argv:
+            np.savez_compressed(os.path.join(OUT, "run_c1l.npz"),
+                                **whole_run("c1l", 20000, 400, list(range(1000, 1016)), 10))
+        print("layered fixtures written to", OUT)
+        return
     if "--mesh" in sys.argv:
         rng = np.random.default_rng(20261205)
         p, rm, rd, _ = problem_and_reference("finfet")

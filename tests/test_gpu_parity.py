@@ -31,7 +31,8 @@ def run_pair(cfg, n, steps, sampling, series=True):
 
 
 @pytest.mark.parametrize("cfg,n", [("c1_cross_plane.toml", 20000), ("c2_inplane_film.toml", 60000),
-                                   ("c1c_callaway.toml", 20000), ("finfet.toml", 20000)])
+                                   ("c1c_callaway.toml", 20000), ("finfet.toml", 20000),
+                                   ("c1l_layered.toml", 20000)])
 def test_initial_population_identical(cfg, n):
     p, g, o = run_pair(cfg, n, 0, 1)
     a, b = by_uid(g.carriers()), o.carriers()
@@ -77,7 +78,9 @@ def test_initial_population_from_cell_field():
                                                    ("c2c_callaway.toml", 60000, 1, True),
                                                    ("c1cs_callaway_source.toml", 20000, 1, True),
                                                    ("finfet.toml", 20000, 1, True),
-                                                   ("finfet.toml", 20000, 0, False)])
+                                                   ("finfet.toml", 20000, 0, False),
+                                                   ("c1l_layered.toml", 20000, 1, True),
+                                                   ("c1l_layered.toml", 20000, 1, False)])
 def test_steps_match_oracle(cfg, n, sampling, series):
     """series=False: every pseudo-temperature evaluation is an exact carrier sum,
     the reference's own procedure; series=True: the per-cell Taylor series.
@@ -87,7 +90,9 @@ def test_steps_match_oracle(cfg, n, sampling, series):
     The callaway inputs run linearized Callaway collisions (the device's Gram-space
     FP64 evaluation against the oracle's long-double restatement of the
     reference), fixed-flux reservoirs and, in c2c, strict-elastic discrete-mode
-    reflections with signed occupations."""
+    reflections with signed occupations.  c1l_layered puts a Ge-like layer on a
+    Si-like one: per-material energy lookups, diffuse-mismatch interface events
+    (band search, transmit/reflect choice and occupation remap)."""
     p, g, o = run_pair(cfg, n, 3, sampling, series)
     scale = max(p.scalars()["volume"] ** (1.0 / 3.0), 1.0)
     for step in range(3):
@@ -105,11 +110,13 @@ def test_steps_match_oracle(cfg, n, sampling, series):
         np.testing.assert_allclose(eg, eo, rtol=1e-12)
         lg, lo = g.ledger(), o.ledger()
         for k in ["step_absorbed", "step_injected", "total_absorbed", "total_injected", "rough_events",
-                  "rough_specular", "rough_diffuse", "live_carriers", "source_updates"]:
+                  "rough_specular", "rough_diffuse", "live_carriers", "source_updates", "interface_events",
+                  "interface_transmitted"]:
             assert lg[k] == lo[k], k
         for k in ["step_absorbed_energy_ev", "step_injected_energy_ev", "thermal_energy_ev", "step_source_energy_ev"]:
             assert lg[k] == pytest.approx(lo[k], rel=1e-11, abs=1e-9), k
     led = g.ledger()
+    assert (led["interface_events"] > led["interface_transmitted"] > 0) == (p.layers() is not None)
     assert (led["newton_evals"] > 0) == (p.callaway() is None)
     if not series:  # every evaluation was an exact carrier sum
         assert led["newton_passes"] >= led["newton_evals"]

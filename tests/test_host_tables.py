@@ -162,7 +162,8 @@ def test_unsupported_features_fail_loudly():
                                                           "heat_source.center": "[50, 50, 50]",
                                                           "heat_source.sigma": "[10, 10, 10]"}))
     with pytest.raises(RuntimeError, match="interface_model = 'amm' is not supported"):
-        rt.Problem(rt.config_text("c1l_layered.toml", **{"materials.interface_model": '"amm"'}))
+        rt.Problem(rt.config_text("c1l_layered.toml", **{"materials.interface_model": '"amm"',
+                                                          "materials.mass_densities_kg_m3": "[2329, 5323]"}))
 
 
 def test_heat_source_rates_match_reference(golden):
