@@ -518,6 +518,7 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
     v.event_split = g->kind == 1 ? pmc_launch::event_split(static_cast<int>(v.capacity)) : 1;
     v.partial_rows = pmc_launch::partial_rows(v);
     v.partials = ctx->alloc<double>(static_cast<size_t>(v.partial_rows) * pmc_dev::kPartialCols);
+    v.pcols = pmc_dev::kPAbsorbedRes;
     v.cell_T = ctx->alloc<double>(nc);
     v.cell_flux = ctx->alloc<double>(3 * nc);
     v.cell_count = ctx->alloc<int>(nc);
@@ -593,6 +594,7 @@ int pmc_upload_boundary_tables(pmc_ctx* ctx, int32_t n_rough, const pmc_rough_ta
     }
     v.res = ctx->upload(dd.data(), dd.size());
     v.nres = n_res;
+    v.pcols = pmc_dev::kPAbsorbedRes + n_res;
     ctx->n_reservoirs = n_res;
     ctx->expected_injection = expected;
     // BoundaryLedger::configure: the first step refills the rounded expected counts.
@@ -1225,6 +1227,7 @@ int pmc_batch_create(pmc_ctx* const* contexts, int32_t n, pmc_batch** out) {
         if (c->latched) return PMC_ERR_LATCHED;
         if (!c->initialized) return fail(c, PMC_ERR_ARGUMENT, "batch: carriers not initialized");
         if (c->view.hs_on) return fail(c, PMC_ERR_UNSUPPORTED, "batch: instances with a heat source are not batched");
+        if (c->view.layers > 1) return fail(c, PMC_ERR_UNSUPPORTED, "batch: layered instances are not batched");
         if (c->view.geom_kind != c0->view.geom_kind || c->view.callaway != c0->view.callaway)
             return fail(c, PMC_ERR_ARGUMENT, "batch: instances differ in geometry kind or collision model");
         if (c->cur != c0->cur || c->host_step != c0->host_step)

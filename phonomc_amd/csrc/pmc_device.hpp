@@ -31,6 +31,9 @@ constexpr int kWavesPerBlock = 4;
 constexpr int kSlotsPerBlock = 64 * kRowsPerWave * kWavesPerBlock;
 constexpr int kInjectBlocks = 64;
 constexpr int kMaxLayers = 8;     // materials of a layered domain
+// Flight-kernel variants: axis-aligned box, triangle mesh, box with material
+// interfaces.  Each compiles only its own boundary code.
+constexpr int kGeomBox = 0, kGeomMesh = 1, kGeomLayered = 2;
 
 // Collision kernels work on chunks of up to kChunk consecutive carriers of one
 // cell in the cell-sorted buffer (one block each) and write per-chunk partial
@@ -249,7 +252,8 @@ struct View {
     int nres;
     // cells
     double* cell_tally;   // kTallyStride per cell
-    double* partials;     // kPartialCols per block row
+    double* partials;     // block rows of pcols values
+    int pcols;            // columns written per row: kPAbsorbedRes + nres (<= kPartialCols)
     int partial_rows;     // allocated: transport blocks x kMaxEventSplit + injection blocks
     double* cell_e;       // energy density of the latest refresh
     double* cell_T;

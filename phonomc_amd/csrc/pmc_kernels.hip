@@ -613,7 +613,7 @@ struct Tally {
 // chosen by velocity sign, mesh faces must be left through), so no push
 // distance is needed; positions are clamped into the bounding box.
 // Returns 0 alive, 1 absorbed, or a negative error.
-template <bool kMesh>
+template <int kGeom>
 __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned uid, float remaining, Tally& T,
                        bool& mode_changed, bool& occ_changed) {
     const uint32_t step = static_cast<uint32_t>(V.st->step);
@@ -623,7 +623,7 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
         const float4 v = carrier_velocity(V, mw);
         const float vv[3] = {v.x, v.y, v.z};
         int facet, ha;
-        if constexpr (kMesh) {
+        if constexpr ((kGeom == kGeomMesh)) {
             // Triangle mesh: FP64 trace from the FP32 position.
             const double* vt = V.veld + 3 * static_cast<size_t>(mw & kModeMask);
             const double vd[3] = {(mw & (1 << kFlipShift)) ? -vt[0] : vt[0], (mw & (2 << kFlipShift)) ? -vt[1] : vt[1],
@@ -665,7 +665,7 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             // Layered materials: the next interface plane along the axis
             // (LayeredMaterialLayout::next_crossing, src/solver/LayeredMaterialLayout.cpp:33-60);
             // the outer wall wins exact ties.
-            if (V.layers > 1) {
+            if (kGeom == kGeomLayered) {
                 const int ax = V.layer_axis, mat = mode_material(V, mw & kModeMask);
                 const float va = vv[ax];
                 int dest = -1;
@@ -852,7 +852,7 @@ struct BlockPartials {
     __device__ void add(int col, double x) { atomicAdd(&s[col], x); }
     __device__ void write(const View& V, int row) {
         __syncthreads();
-        for (int c = threadIdx.x; c < kPartialCols; c += blockDim.x) V.partials[row * kPartialCols + c] = s[c];
+        for (int c = threadIdx.x; c < V.pcols; c += blockDim.x) V.partials[row * V.pcols + c] = s[c];
     }
 };
 
@@ -1012,7 +1012,7 @@ __global__ void kb_begin(const View* __restrict__ views, const int* __restrict__
 // iteration of advance() when it ends without an event, bit for bit.  Carriers
 // that do reach a wall are listed per block for k_events and left untouched.
 // Counts carriers per new cell for the cell sort.
-template <bool kMesh>
+template <int kGeom>
 __device__ __forceinline__ void transport_body(const View& V, const Grid G) {
     __shared__ int ev_n;
     if (threadIdx.x == 0) ev_n = 0;
@@ -1054,7 +1054,7 @@ __device__ __forceinline__ void transport_body(const View& V, const Grid G) {
                 if (moves) {
                     const float4 v = carrier_velocity(V, mw[j]);
                     const float vv[3] = {v.x, v.y, v.z};
-                    if constexpr (kMesh) {
+                    if constexpr ((kGeom == kGeomMesh)) {
                         // Carriers the distance field cannot clear go to k_events,
                         // which searches the faces with whole waves of such carriers.
                         event = v.w != 0.0f && !mesh_clear(V, p, v.w, dt);
@@ -1075,7 +1075,7 @@ __device__ __forceinline__ void transport_body(const View& V, const Grid G) {
                             }
                         }
                         event = plane >= 0 && !(tbest > dt);
-                        if (!event && V.layers > 1) { // a material interface within the step
+                        if (kGeom == kGeomLayered && !event) { // a material interface within the step
                             const int ax = V.layer_axis, mat = mode_material(V, mw[j] & kModeMask);
                             const float va = vv[ax];
                             float ti = INFINITY;
@@ -1115,20 +1115,20 @@ __device__ __forceinline__ void transport_body(const View& V, const Grid G) {
     if (threadIdx.x == 0) V.block_events[blk] = ev_n;
 }
 
-template <bool kMesh>
-__global__ void __launch_bounds__(256) k_transport(View V) { transport_body<kMesh>(V, launch_grid()); }
+template <int kGeom>
+__global__ void __launch_bounds__(256) k_transport(View V) { transport_body<kGeom>(V, launch_grid()); }
 
-template <bool kMesh>
+template <int kGeom>
 __global__ void __launch_bounds__(256) kb_transport(const View* __restrict__ views, const int* __restrict__ offs, int n) {
     Grid G;
     const int i = batch_instance(offs, n, G);
-    transport_body<kMesh>(views[i], G);
+    transport_body<kGeom>(views[i], G);
 }
 
 // Carriers with wall events this step: the whole event loop of
 // TransportOperator::advance_particle (src/solver/TransportOperator.cpp:405-466)
 // for the slots k_transport listed.  One block per transport block.
-template <bool kMesh>
+template <int kGeom>
 __device__ __forceinline__ void events_body(const View& V, const Grid G) {
     // event_split blocks share one transport block's list (more waves for
     // latency-bound event work when there are few transport blocks).
@@ -1153,7 +1153,7 @@ __device__ __forceinline__ void events_body(const View& V, const Grid G) {
             float p[3] = {V.px[i], V.py[i], V.pz[i]};
             double occ = V.occ[i];
             bool mc = false, oc = false;
-            const int rc = advance<kMesh>(V, p, mw, occ, V.uid[i], V.dtf, T, mc, oc);
+            const int rc = advance<kGeom>(V, p, mw, occ, V.uid[i], V.dtf, T, mc, oc);
             if (rc < 0) {
                 raise_advance_error(V, rc, i, p);
                 V.mode[i] = -1; // keeps the cell counts equal to the live slots
@@ -1178,20 +1178,20 @@ __device__ __forceinline__ void events_body(const View& V, const Grid G) {
     P.write(V, row);
 }
 
-template <bool kMesh>
-__global__ void __launch_bounds__(256) k_events(View V) { events_body<kMesh>(V, launch_grid()); }
+template <int kGeom>
+__global__ void __launch_bounds__(256) k_events(View V) { events_body<kGeom>(V, launch_grid()); }
 
-template <bool kMesh>
+template <int kGeom>
 __global__ void __launch_bounds__(256) kb_events(const View* __restrict__ views, const int* __restrict__ offs, int n) {
     Grid G;
     const int i = batch_instance(offs, n, G);
-    events_body<kMesh>(views[i], G);
+    events_body<kGeom>(views[i], G);
 }
 
 // Reservoir injection (ReservoirInjector::inject, src/solver/ReservoirInjector.cpp:62-132)
 // followed by the remaining flight of each injected carrier
 // (TransportOperator::advance_injected, src/solver/TransportOperator.cpp:134-164).
-template <bool kMesh>
+template <int kGeom>
 __device__ __forceinline__ void inject_body(const View& V, const Grid G) {
     const int row0 = event_rows_for(V); // partial rows after the event blocks
 
@@ -1272,7 +1272,7 @@ __device__ __forceinline__ void inject_body(const View& V, const Grid G) {
             const double remain = V.dt - t_emit;
             int rc = 0;
             bool mc = false, oc = false;
-            if (remain > 1e-14) rc = advance<kMesh>(V, p, mw, occ, uid, static_cast<float>(remain), T, mc, oc);
+            if (remain > 1e-14) rc = advance<kGeom>(V, p, mw, occ, uid, static_cast<float>(remain), T, mc, oc);
             V.uid[slot] = uid;
             if (rc < 0) {
                 raise_advance_error(V, rc, slot, p);
@@ -1297,14 +1297,25 @@ __device__ __forceinline__ void inject_body(const View& V, const Grid G) {
     P.write(V, row0 + G.b);
 }
 
-template <bool kMesh>
-__global__ void __launch_bounds__(256) k_inject(View V) { inject_body<kMesh>(V, launch_grid()); }
+template <int kGeom>
+__global__ void __launch_bounds__(256) k_inject(View V) {
+    // The View is this kernel's only argument, at offset 0 of the kernarg
+    // segment.  Reading it there keeps the compiler from copying all of it to
+    // per-lane scratch first, which it does here when handed the parameter.
+#if defined(__HIP_DEVICE_COMPILE__)
+    (void)V;
+    const View& KV = *(const View*)__builtin_amdgcn_kernarg_segment_ptr();
+#else
+    const View& KV = V;
+#endif
+    inject_body<kGeom>(KV, launch_grid());
+}
 
-template <bool kMesh>
+template <int kGeom>
 __global__ void __launch_bounds__(256) kb_inject(const View* __restrict__ views, const int* __restrict__ offs, int n) {
     Grid G;
     const int i = batch_instance(offs, n, G);
-    inject_body<kMesh>(views[i], G);
+    inject_body<kGeom>(views[i], G);
 }
 
 __device__ double lut_inverse(const View& V, double e, int base, int n) {
@@ -1387,29 +1398,47 @@ __device__ __forceinline__ void refresh_body(const View& V, const Grid G, int ph
     __shared__ int scan[1024];
     __shared__ int scan2[1024];
     if (phase == 1) {
+        // Every column of the partial rows written this step in one pass: each
+        // thread sums whole rows, then every column is reduced across the block.
+        const int ncols = V.pcols; // rows are packed to the columns in use
+        const int rows = event_rows_for(V) + kInjectBlocks; // rows written this step
+        double acc[kPartialCols];
+#pragma unroll
+        for (int c = 0; c < kPartialCols; ++c) acc[c] = 0.0;
+        for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+            const double* row = V.partials + static_cast<size_t>(r) * ncols;
+#pragma unroll
+            for (int c = 0; c < kPartialCols; ++c)
+                if (c < ncols) acc[c] += row[c];
+        }
+        static_assert(16 * kPartialCols <= 1024, "per-wave column sums must fit the reduction buffer");
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+#pragma unroll
         for (int c = 0; c < kPartialCols; ++c) {
-            if (c == kPAbsorbedRes + V.nres) break;
-            double x = 0.0;
-            const int rows = event_rows_for(V) + kInjectBlocks; // rows written this step
-            for (int r = threadIdx.x; r < rows; r += blockDim.x) x += V.partials[r * kPartialCols + c];
-            const double s = block_sum(x, red);
-            if (threadIdx.x == 0) {
-                switch (c) {
-                case kPAbsorbedE: st->step_absorbed_e = s; break;
-                case kPInjectedE: st->step_injected_e = s; break;
-                case kPAbsorbedN: st->step_absorbed = static_cast<long long>(s); break;
-                case kPInjectedN: st->step_injected = static_cast<long long>(s); break;
-                case kPRoughEvents: st->rough_events += static_cast<long long>(s); break;
-                case kPRoughSpecular: st->rough_specular += static_cast<long long>(s); break;
-                case kPRoughDiffuse: st->rough_diffuse += static_cast<long long>(s); break;
-                case kPIfaceEvents: st->iface_events += static_cast<long long>(s); break;
-                case kPIfaceTransmitted: st->iface_transmitted += static_cast<long long>(s); break;
-                default:
-                    if (c >= kPAbsorbedRes && c < kPAbsorbedRes + kMaxReservoirs)
-                        st->absorbed_now[c - kPAbsorbedRes] = static_cast<int>(s);
-                }
+            const double w = wave_sum(acc[c]);
+            if (lane == 0) red[wave * kPartialCols + c] = w;
+        }
+        __syncthreads();
+        if (static_cast<int>(threadIdx.x) < ncols) {
+            const int c = threadIdx.x;
+            double s = 0.0;
+            for (int w = 0; w < waves; ++w) s += red[w * kPartialCols + c];
+            switch (c) {
+            case kPAbsorbedE: st->step_absorbed_e = s; break;
+            case kPInjectedE: st->step_injected_e = s; break;
+            case kPAbsorbedN: st->step_absorbed = static_cast<long long>(s); break;
+            case kPInjectedN: st->step_injected = static_cast<long long>(s); break;
+            case kPRoughEvents: st->rough_events += static_cast<long long>(s); break;
+            case kPRoughSpecular: st->rough_specular += static_cast<long long>(s); break;
+            case kPRoughDiffuse: st->rough_diffuse += static_cast<long long>(s); break;
+            case kPIfaceEvents: st->iface_events += static_cast<long long>(s); break;
+            case kPIfaceTransmitted: st->iface_transmitted += static_cast<long long>(s); break;
+            default:
+                if (c >= kPAbsorbedRes && c < kPAbsorbedRes + kMaxReservoirs)
+                    st->absorbed_now[c - kPAbsorbedRes] = static_cast<int>(s);
             }
         }
+        __syncthreads();
     }
     const int nc = V.ncells;
     const int per = (nc + static_cast<int>(blockDim.x) - 1) / static_cast<int>(blockDim.x);
@@ -2894,16 +2923,19 @@ static const dim3 kWaveBlock(64 * kWavesPerBlock);
 
 void begin(const View& v, hipStream_t s) { hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, s, v); }
 void transport(const View& v, hipStream_t s) {
-    if (v.geom_kind == 1) hipLaunchKernelGGL(k_transport<true>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
-    else hipLaunchKernelGGL(k_transport<false>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+    if (v.geom_kind == 1) hipLaunchKernelGGL(k_transport<kGeomMesh>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+    else if (v.layers > 1) hipLaunchKernelGGL(k_transport<kGeomLayered>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
+    else hipLaunchKernelGGL(k_transport<kGeomBox>, dim3(wave_blocks(v.capacity)), kWaveBlock, 0, s, v);
 }
 void events(const View& v, hipStream_t s) {
-    if (v.geom_kind == 1) hipLaunchKernelGGL(k_events<true>, dim3(event_rows(v)), kWaveBlock, 0, s, v);
-    else hipLaunchKernelGGL(k_events<false>, dim3(event_rows(v)), kWaveBlock, 0, s, v);
+    if (v.geom_kind == 1) hipLaunchKernelGGL(k_events<kGeomMesh>, dim3(event_rows(v)), kWaveBlock, 0, s, v);
+    else if (v.layers > 1) hipLaunchKernelGGL(k_events<kGeomLayered>, dim3(event_rows(v)), kWaveBlock, 0, s, v);
+    else hipLaunchKernelGGL(k_events<kGeomBox>, dim3(event_rows(v)), kWaveBlock, 0, s, v);
 }
 void inject(const View& v, hipStream_t s) {
-    if (v.geom_kind == 1) hipLaunchKernelGGL(k_inject<true>, dim3(kInjectBlocks), dim3(256), 0, s, v);
-    else hipLaunchKernelGGL(k_inject<false>, dim3(kInjectBlocks), dim3(256), 0, s, v);
+    if (v.geom_kind == 1) hipLaunchKernelGGL(k_inject<kGeomMesh>, dim3(kInjectBlocks), dim3(256), 0, s, v);
+    else if (v.layers > 1) hipLaunchKernelGGL(k_inject<kGeomLayered>, dim3(kInjectBlocks), dim3(256), 0, s, v);
+    else hipLaunchKernelGGL(k_inject<kGeomBox>, dim3(kInjectBlocks), dim3(256), 0, s, v);
 }
 void refresh(const View& v, hipStream_t s, int phase) { hipLaunchKernelGGL(k_refresh, dim3(1), dim3(1024), 0, s, v, phase); }
 void sort(const View& v, hipStream_t s) {
@@ -2973,16 +3005,16 @@ void batch_launch(int kind, bool mesh, const View* views, const int* offs, int n
     case kBatchBegin: hipLaunchKernelGGL(kb_begin, g, dim3(64), 0, s, views, offs, n); break;
     case kBatchContainment: hipLaunchKernelGGL(kb_containment, g, dim3(256), 0, s, views, offs, n); break;
     case kBatchTransport:
-        if (mesh) hipLaunchKernelGGL(kb_transport<true>, g, kWaveBlock, 0, s, views, offs, n);
-        else hipLaunchKernelGGL(kb_transport<false>, g, kWaveBlock, 0, s, views, offs, n);
+        if (mesh) hipLaunchKernelGGL(kb_transport<kGeomMesh>, g, kWaveBlock, 0, s, views, offs, n);
+        else hipLaunchKernelGGL(kb_transport<kGeomBox>, g, kWaveBlock, 0, s, views, offs, n);
         break;
     case kBatchEvents:
-        if (mesh) hipLaunchKernelGGL(kb_events<true>, g, kWaveBlock, 0, s, views, offs, n);
-        else hipLaunchKernelGGL(kb_events<false>, g, kWaveBlock, 0, s, views, offs, n);
+        if (mesh) hipLaunchKernelGGL(kb_events<kGeomMesh>, g, kWaveBlock, 0, s, views, offs, n);
+        else hipLaunchKernelGGL(kb_events<kGeomBox>, g, kWaveBlock, 0, s, views, offs, n);
         break;
     case kBatchInject:
-        if (mesh) hipLaunchKernelGGL(kb_inject<true>, g, dim3(256), 0, s, views, offs, n);
-        else hipLaunchKernelGGL(kb_inject<false>, g, dim3(256), 0, s, views, offs, n);
+        if (mesh) hipLaunchKernelGGL(kb_inject<kGeomMesh>, g, dim3(256), 0, s, views, offs, n);
+        else hipLaunchKernelGGL(kb_inject<kGeomBox>, g, dim3(256), 0, s, views, offs, n);
         break;
     case kBatchRefresh1: hipLaunchKernelGGL(kb_refresh, g, dim3(1024), 0, s, views, offs, n, 1); break;
     case kBatchRefresh2: hipLaunchKernelGGL(kb_refresh, g, dim3(1024), 0, s, views, offs, n, 2); break;
