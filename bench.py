@@ -94,7 +94,12 @@ def cpu_baseline(problem_text, steps, workload=WORKLOAD, carriers=1e7):
         saved = os.dup(1)
         os.dup2(2, 1)
         try:
-            secs, ps = ref_ctypes.run(path, problem.material(), steps, threads=cores)
+            lv = problem.layers()
+            if lv is None:
+                secs, ps = ref_ctypes.run(path, problem.material(), steps, threads=cores)
+            else:  # layered materials: every layer's table, in order
+                secs, ps = ref_ctypes.run_materials(path, [problem.layer_material(k) for k in range(lv.count)],
+                                                    steps, threads=cores)[:2]
         finally:
             sys.stdout.flush()
             os.dup2(saved, 1)
