@@ -1056,6 +1056,24 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     v.bank_mode = ctx->upload(L->bank_mode, static_cast<size_t>(nb));
     v.bank_omega = ctx->upload(L->bank_omega, static_cast<size_t>(nb));
     v.bank_prefix = ctx->upload(L->bank_prefix, static_cast<size_t>(nb) + 2 * n);
+    // Band start positions of every bank: the two lower_bound searches of a DMM
+    // band draw become two loads.  Same products j * band as the device's
+    // floor(w / band) * band, so the same band boundaries.
+    double wmax = 0.0;
+    for (int i = 0; i < nb; ++i) wmax = std::max(wmax, L->bank_omega[i]);
+    const double bands = std::floor(wmax / L->band_width) + 1.0;
+    if (!(bands <= 1 << 20)) return fail(ctx, PMC_ERR_ARGUMENT, "interface frequency band too narrow for the band index");
+    const int nbands = static_cast<int>(bands);
+    std::vector<int32_t> starts(static_cast<size_t>(2 * n) * (nbands + 1));
+    for (int b = 0; b < 2 * n; ++b) {
+        const double* om = L->bank_omega + L->bank_begin[b];
+        const int cnt = L->bank_begin[b + 1] - L->bank_begin[b];
+        for (int j = 0; j <= nbands; ++j)
+            starts[static_cast<size_t>(b) * (nbands + 1) + j] =
+                static_cast<int32_t>(std::lower_bound(om, om + cnt, static_cast<double>(j) * L->band_width) - om);
+    }
+    v.band_start = ctx->upload(starts.data(), starts.size());
+    v.band_count = nbands;
     v.layers = n;
     return check_hip(ctx, hipGetLastError(), "layers");
 }

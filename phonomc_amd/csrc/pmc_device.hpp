@@ -246,6 +246,11 @@ struct View {
     const int* bank_mode;
     const double* bank_omega;
     const double* bank_prefix;
+    // Per bank, band_count + 1 positions: lower_bound of j * band in the bank's
+    // sorted frequencies, j = 0 .. band_count (every frequency is below
+    // band_count * band), so a band's modes are [start[j], start[j + 1]).
+    const int* band_start;
+    int band_count;
     // boundary tables
     const RoughDev* rough;
     const ResDev* res;

@@ -350,33 +350,21 @@ __device__ __forceinline__ int mode_material(const View& V, int m) {
 // InterfaceModeSampler::sample_band (src/solver/InterfaceModeSampler.cpp:72-109):
 // in bank (material, direction), the modes whose frequency lies in the band
 // [floor(w / bw) bw, (floor(w / bw) + 1) bw), drawn by their cumulative flux with
-// uniform u.  Returns the global mode or -1; flux = the band's total flux.
+// uniform u.  Returns the global mode or -1; flux = the band's total flux.  The
+// band's bounds come from the band index built at upload (the reference's two
+// lower_bound searches, done once per band).
 __device__ int sample_band(const View& V, int material, int direction, double w, double u, double& flux) {
     const int b = 2 * material + direction;
-    const int b0 = V.bank_begin[b], n = V.bank_begin[b + 1] - b0;
-    const double* om = V.bank_omega + b0;
-    const double* pre = V.bank_prefix + b0 + b; // n + 1 entries
+    const int b0 = V.bank_begin[b];
+    const double* pre = V.bank_prefix + b0 + b; // bank size + 1 entries
     const double bin = floor(w / V.band);
-    const double lo = bin * V.band, hi = (bin + 1.0) * V.band;
-    int begin = 0, end = n;
-    { // lower_bound(lo)
-        int l = 0, h = n;
-        while (l < h) {
-            const int mid = (l + h) >> 1;
-            if (om[mid] < lo) l = mid + 1;
-            else h = mid;
-        }
-        begin = l;
+    if (!(bin < static_cast<double>(V.band_count))) { // above every frequency of every bank
+        flux = 0.0;
+        return -1;
     }
-    { // lower_bound(hi)
-        int l = begin, h = n;
-        while (l < h) {
-            const int mid = (l + h) >> 1;
-            if (om[mid] < hi) l = mid + 1;
-            else h = mid;
-        }
-        end = l;
-    }
+    // [lower_bound(bin * band), lower_bound((bin + 1) * band)) from the band index.
+    const int* bs = V.band_start + b * (V.band_count + 1) + static_cast<int>(bin);
+    const int begin = bs[0], end = bs[1];
     flux = end > begin ? pre[end] - pre[begin] : 0.0;
     if (!(flux > 0.0)) return -1;
     const double target = pre[begin] + u * flux;
