@@ -1481,9 +1481,35 @@ __device__ __forceinline__ void refresh_body(const View& V, const Grid G, int ph
         return;
     }
     double local = 0.0, residual = 0.0;
+    // Chunk partials of the relaxation.  With at most four cells per wave, one
+    // wave per cell with lanes striding the cell's chunks, so a few cells of many
+    // chunks do not serialize on one thread (the sums wait in cell_e for the
+    // per-cell pass); with more cells, each thread sums its own cells' chunks.
+    const int waves = static_cast<int>(blockDim.x) >> 6;
+    const bool wave_cells = phase == 2 && nc <= 4 * waves;
+    if (wave_cells) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int c = wave; c < nc; c += waves) {
+            double es = 0.0, rs = 0.0;
+            for (int k = V.chunk_start[c] + lane; k < V.chunk_start[c + 1]; k += 64) {
+                es += V.chunk_out[2 * static_cast<size_t>(k)];
+                rs += V.chunk_out[2 * static_cast<size_t>(k) + 1];
+                if (V.hs_on) es += V.chunk_src[k];
+            }
+            es = wave_sum(es);
+            rs = wave_sum(rs);
+            if (lane == 0) {
+                V.cell_e[c] = es;
+                residual += rs;
+            }
+        }
+        __syncthreads();
+    }
     for (int c = c0; c < c1; ++c) {
         double esum = V.cell_tally[static_cast<size_t>(kTallyStride) * c + kTallyEnergy];
-        if (phase == 2) { // chunk partials of the relaxation, in chunk order
+        if (wave_cells) {
+            esum = V.cell_e[c];
+        } else if (phase == 2) { // in chunk order
             esum = 0.0;
             for (int k = V.chunk_start[c]; k < V.chunk_start[c + 1]; ++k) {
                 esum += V.chunk_out[2 * static_cast<size_t>(k)];
