@@ -47,7 +47,9 @@ typedef struct pmc_setup {
     double lifetime_temperature;
     int32_t refill_policy;         /* 0 one-to-one (previous absorbed count), 1 fixed incoming flux */
     int32_t mode_sampling;         /* 0 alias table, 1 inverse CDF (reference index order) */
-    int32_t temperature_inversion; /* 0 lookup table (reference), 1 Newton on the Bose-Einstein sum */
+    int32_t temperature_inversion; /* 0 lookup table (reference), 1 per-cell Newton on the exact
+                                    * Bose-Einstein crystal energy (single-material RTA transport;
+                                    * pmc_upload_callaway, pmc_upload_layers and batches reject it) */
     int32_t reserved[8];
 } pmc_setup;
 
@@ -70,6 +72,11 @@ typedef struct pmc_mode_table {
     int32_t lut_n;
     const double* lut_t;           /* [lut_n] K */
     const double* lut_e;           /* [lut_n] eV/A^3, strictly increasing */
+    /* max(1, Nq*V_uc) (A^3): the crystal energy density is sum over active modes
+     * of hw*n_BE(T) / energy_normalization + zero_point_density
+     * (PhononMaterial::crystal_energy_density, src/material/Thermodynamics.cpp:151-163).
+     * Read by the Newton temperature inversion only. */
+    double energy_normalization;
 } pmc_mode_table;
 
 /* One surface facet; replaces SimulationDomain facet queries
@@ -307,10 +314,14 @@ int pmc_read_cells(pmc_ctx* ctx, double* temperature, double* energy_density);
 /* Heat-flux tally (src/solver/GridStatistics.cpp:269-347), W/m^2, [n_cells*3]. */
 int pmc_read_heat_flux(pmc_ctx* ctx, double* flux);
 int pmc_read_ledger(pmc_ctx* ctx, pmc_ledger* ledger);
-/* Carrier state for tests and checkpoints: slots in [0, *n) (dead slots have mode -1). */
+/* Carrier state for tests and checkpoints: slots in [0, *n) (dead slots have mode -1).
+ * On input *n is the capacity (slots) of the arrays given; with every array NULL
+ * it is ignored.  On output *n is the number of slots in use; PMC_ERR_ARGUMENT
+ * when that exceeds the capacity (nothing is copied). */
 int pmc_read_carriers(pmc_ctx* ctx, int64_t* n, float* position, int32_t* mode, double* occupation,
                       int32_t* cell);
-/* Philox stream ids of slots [0, n) (initial carriers 0..N-1, injected ones numbered on). */
+/* Philox stream ids of slots [0, n) (initial carriers 0..N-1, injected ones numbered on);
+ * n may not exceed the slots in use. */
 int pmc_read_uids(pmc_ctx* ctx, int64_t n, uint32_t* uid);
 int pmc_last_error(pmc_ctx* ctx, pmc_error* error);
 

@@ -39,6 +39,9 @@ def lib():
                                         C.POINTER(d)]
         L.orc_rough_sample.argtypes = [C.POINTER(rt.ModeTable), C.POINTER(rt.RoughTable), C.c_int, d, C.POINTER(d),
                                        C.POINTER(C.c_int), C.POINTER(d), C.POINTER(C.c_int)]
+        L.orc_rough_sample_signed.argtypes = [C.POINTER(rt.ModeTable), C.POINTER(rt.RoughTable), C.c_int, d,
+                                              C.POINTER(d), C.POINTER(C.c_int), C.POINTER(d), C.POINTER(C.c_int),
+                                              C.c_int]
         L.orc_reservoir_mode.argtypes = [C.POINTER(rt.Reservoir), C.c_int, d, C.c_int]
         L.orc_surface_point.argtypes = [C.POINTER(rt.Reservoir), d, d, d, C.POINTER(d)]
         L.orc_refresh.argtypes = [C.POINTER(rt.ModeTable), C.c_int, C.POINTER(d), C.c_int64, C.POINTER(C.c_int32),
@@ -192,6 +195,18 @@ def rough_sample(problem, table_index, in_active, in_occ, uniforms):
     u = np.ascontiguousarray(uniforms, dtype=np.float64)
     used = lib().orc_rough_sample(C.byref(m), C.byref(r), int(in_active), float(in_occ), _dp(u), C.byref(out_a),
                                   C.byref(out_o), C.byref(kind))
+    return out_a.value, out_o.value, kind.value, used
+
+
+def rough_sample_signed(problem, table, in_active, in_occ, uniforms, signed_weights=1):
+    """orc_rough_sample_signed on a table index of the problem or on a RoughTable
+    structure (tests pass edited copies)."""
+    _, m, _, _, rough, _ = problem.views()
+    r = rough[table] if isinstance(table, int) else table
+    out_a, out_o, kind = C.c_int(), C.c_double(), C.c_int()
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    used = lib().orc_rough_sample_signed(C.byref(m), C.byref(r), int(in_active), float(in_occ), _dp(u),
+                                         C.byref(out_a), C.byref(out_o), C.byref(kind), int(signed_weights))
     return out_a.value, out_o.value, kind.value, used
 
 

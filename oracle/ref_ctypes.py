@@ -216,6 +216,21 @@ class RefDomain:
             raise RuntimeError(err.value.decode())
         return out_a, out_o, u.reshape(n, 2)
 
+    def rough_sample_strict(self, material, facet, t_bg, in_active, in_occ, seeds):
+        """Strict-elastic table of conservative transport, signed weights."""
+        in_active = np.ascontiguousarray(in_active, dtype=np.int32)
+        in_occ = np.ascontiguousarray(in_occ, dtype=np.float64)
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        n = len(in_active)
+        out_a, out_o, u = np.zeros(n, dtype=np.int32), np.zeros(n), np.zeros(2 * n)
+        err = C.create_string_buffer(512)
+        rc = lib().ref_rough_sample_strict(self.h, material.h, facet, C.c_double(t_bg), n, _ip(in_active),
+                                           _dp(in_occ), seeds.ctypes.data_as(C.POINTER(C.c_uint64)), _ip(out_a),
+                                           _dp(out_o), _dp(u), err, 512)
+        if rc != 0:
+            raise RuntimeError(err.value.decode())
+        return out_a, out_o, u.reshape(n, 2)
+
     def interact(self, material, facet, tref, weight, in_active, in_vel, in_occ, seeds):
         in_active = np.ascontiguousarray(in_active, dtype=np.int32)
         in_vel = np.ascontiguousarray(in_vel, dtype=np.float64)

@@ -104,6 +104,22 @@ phonomc::RoughBoundarySampler make_rough(const SimulationDomain& g, const Phonon
     return s;
 }
 
+// Rough sampler configured the way MonteCarloSolver::initialize_rough_boundary_scattering
+// does under conservative transport (src/solver/RoughBoundaryInitialization.cpp:9-25):
+// every reflecting facet (not a reservoir, not a paired periodic face) gets a
+// strict-elastic table built at the equilibrium temperature.
+phonomc::RoughBoundarySampler make_rough_strict(const SimulationDomain& g, const PhononMaterial& m, double t_bg) {
+    std::vector<phonomc::RoughFacetSpec> facets;
+    for (int f = 0; f < g.mesh().facet_count(); ++f) {
+        const char c = g.facet_boundary_condition(f);
+        const bool reflecting = c != 'T' && !(c == 'P' && g.has_periodic_pair(f));
+        facets.push_back({reflecting, g.mesh().facet_normals()[f], std::max(0.0, g.roughness_for_facet(f, 0.0))});
+    }
+    phonomc::RoughBoundarySampler s;
+    s.configure(facets, {&m}, nullptr, true, t_bg);
+    return s;
+}
+
 double canonical(std::mt19937_64& rng) {
     return std::uniform_real_distribution<double>(0.0, 1.0)(rng);
 }
@@ -313,6 +329,29 @@ int ref_rough_sample(void* domain, void* material, int facet, double reference_t
             uniforms[2 * i + 1] = canonical(probe);
             const auto r = sampler.sample(m, table, m.active_mode_at(in_active[i]), in_occupation[i],
                                           reference_temperature, rng, false);
+            out_active[i] = m.active_index_for_mode(r.mode);
+            out_occupation[i] = r.occupation;
+        }
+    });
+}
+
+int ref_rough_sample_strict(void* domain, void* material, int facet, double t_bg, int n, const int* in_active,
+                            const double* in_occupation, const uint64_t* seeds, int* out_active,
+                            double* out_occupation, double* uniforms, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& g = *dom(domain).domain;
+        const auto& m = mat(material);
+        const auto sampler = make_rough_strict(g, m, t_bg);
+        const int table = sampler.table_index(0, facet);
+        if (table < 0) throw std::invalid_argument("facet has no strict-elastic table");
+        for (int i = 0; i < n; ++i) {
+            std::mt19937_64 rng(seeds[i]);
+            std::mt19937_64 probe = rng;
+            uniforms[2 * i] = canonical(probe);
+            uniforms[2 * i + 1] = canonical(probe);
+            // Signed weights, as interact_boundary passes under conservative
+            // transport (src/solver/BoundaryInteraction.cpp:90-113).
+            const auto r = sampler.sample(m, table, m.active_mode_at(in_active[i]), in_occupation[i], t_bg, rng, true);
             out_active[i] = m.active_index_for_mode(r.mode);
             out_occupation[i] = r.occupation;
         }

@@ -92,6 +92,13 @@ int ref_rough_sample(void* domain, void* material, int facet, double reference_t
                      int n, const int* in_active, const double* in_occupation,
                      const uint64_t* seeds, int* out_active, double* out_occupation,
                      double* uniforms, char* err, int errlen);
+/* RoughBoundarySampler as configured under conservative (linearized Callaway)
+ * transport (src/solver/RoughBoundaryInitialization.cpp:9-25): strict-elastic
+ * tables at t_bg on every reflecting facet, sampled with signed weights
+ * (src/solver/BoundaryInteraction.cpp:90-113).  Same outputs as ref_rough_sample. */
+int ref_rough_sample_strict(void* domain, void* material, int facet, double t_bg, int n, const int* in_active,
+                            const double* in_occupation, const uint64_t* seeds, int* out_active,
+                            double* out_occupation, double* uniforms, char* err, int errlen);
 /* interact_boundary (src/solver/BoundaryInteraction.cpp:53-162) for one facet. */
 int ref_interact_boundary(void* domain, void* material, int facet, double reference_temperature,
                           double energy_weight, int n, const int* in_active,

@@ -373,6 +373,7 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     mv.lut_n = static_cast<int>(m.lut_t.size());
     mv.lut_t = m.lut_t.data();
     mv.lut_e = m.lut_e.data();
+    mv.energy_normalization = m.energy_normalization;
     if (layered) {
         auto& L = p->layers;
         mv.gamma_total = L.gamma_total.data();

@@ -39,6 +39,7 @@ struct pmc_ctx {
     View view{};
     bool has_material = false, has_geometry = false, has_tables = false, initialized = false;
     bool latched = false;
+    pmc_batch* owner = nullptr; // batch that steps this context (pmc_batch_create .. pmc_batch_destroy)
     pmc_error error{};
     int64_t host_step = 0;
     double elapsed_ps = 0.0;   // accumulated like the reference's elapsed_time_
@@ -340,7 +341,8 @@ int pmc_create(const pmc_setup* setup, pmc_ctx** out) {
     if (!setup || !out) return PMC_ERR_ARGUMENT;
     *out = nullptr;
     if (!(setup->time_step_ps > 0) || !std::isfinite(setup->time_step_ps) || setup->capacity <= 0 ||
-        setup->capacity > (1ll << 30) || !(setup->push_distance_a >= 0))
+        setup->capacity > (1ll << 30) || !(setup->push_distance_a >= 0) ||
+        (setup->temperature_inversion != 0 && setup->temperature_inversion != 1))
         return PMC_ERR_ARGUMENT;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return PMC_ERR_DEVICE;
@@ -433,6 +435,9 @@ int pmc_upload_material(pmc_ctx* ctx, const pmc_mode_table* t) {
     v.w = t->energy_weight;
     v.e_bg = t->background_energy_density;
     v.zp = t->zero_point_density;
+    v.e_norm = t->energy_normalization;
+    if (v.inversion == 1 && !(t->energy_normalization >= 1.0 && std::isfinite(t->energy_normalization)))
+        return fail(ctx, PMC_ERR_ARGUMENT, "Newton temperature inversion needs the energy normalization (>= 1)");
     v.lut_n = t->lut_n;
     v.lut_t = ctx->upload(t->lut_t, t->lut_n);
     v.lut_e = ctx->upload(t->lut_e, t->lut_n);
@@ -520,6 +525,7 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
     v.partials = ctx->alloc<double>(static_cast<size_t>(v.partial_rows) * pmc_dev::kPartialCols);
     v.pcols = pmc_dev::kPAbsorbedRes;
     v.cell_T = ctx->alloc<double>(nc);
+    v.cell_tx = ctx->alloc<double>(nc);
     v.cell_flux = ctx->alloc<double>(3 * nc);
     v.cell_count = ctx->alloc<int>(nc);
     v.cell_start = ctx->alloc<int>(nc + 1);
@@ -534,7 +540,7 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
     const size_t blocks = static_cast<size_t>(pmc_launch::wave_blocks(v.capacity));
     v.events = ctx->alloc<int>(blocks * pmc_dev::kSlotsPerBlock);
     v.block_events = ctx->alloc<int>(blocks);
-    if (!v.facets || !v.voxel || !v.cell_vol || !v.cell_e || !v.cell_T || !v.cell_tally || !v.partials ||
+    if (!v.facets || !v.voxel || !v.cell_vol || !v.cell_e || !v.cell_T || !v.cell_tx || !v.cell_tally || !v.partials ||
         !v.cell_flux || !v.cell_count || !v.cell_start || !v.cell_cursor || !v.crta || !v.events || !v.block_events ||
         !v.chunk_start || !v.chunk_cell || !v.chunk_acc || !v.chunk_out || !v.aw)
         return fail(ctx, PMC_ERR_DEVICE, "geometry upload failed");
@@ -660,6 +666,7 @@ static int finish_population(pmc_ctx* ctx, int64_t count) {
     ctx->cur = 1 - ctx->cur;
     View& w = view(ctx);
     pmc_launch::refresh(w, ctx->stream, 3);
+    if (w.inversion == 1) pmc_launch::invert(w, ctx->stream, 1);
     int rc = check_hip(ctx, hipGetLastError(), "initial refresh");
     if (rc) return rc;
     ctx->initialized = true;
@@ -671,6 +678,10 @@ int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* t) {
     if (!t || !t->capacity || !t->wavevector || !t->normal_rate || !t->resistive_rate)
         return fail(ctx, PMC_ERR_ARGUMENT, "Callaway table arrays missing");
     if (!ctx->has_material) return fail(ctx, PMC_ERR_ARGUMENT, "material must be uploaded before the Callaway table");
+    if (ctx->view.inversion == 1)
+        return fail(ctx, PMC_ERR_UNSUPPORTED,
+                    "Newton temperature inversion applies to RTA transport; linearized Callaway uses "
+                    "T_bg + (e - E_bg) / C_bg");
     if (ctx->initialized)
         return fail(ctx, PMC_ERR_ARGUMENT, "the Callaway table must be uploaded before the population is created");
     if (!(t->heat_capacity_density > 0.0) || !std::isfinite(t->heat_capacity_density) ||
@@ -810,6 +821,7 @@ extern "C" {
 int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
     if (!ctx) return PMC_ERR_ARGUMENT;
     if (ctx->latched) return PMC_ERR_LATCHED; // the first failure's message stays in ctx->error
+    if (ctx->owner) return fail(ctx, PMC_ERR_ARGUMENT, "context belongs to a batch: step it with pmc_batch_step");
     if (!ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "carriers not initialized");
     if (nsteps < 0) return fail(ctx, PMC_ERR_ARGUMENT, "negative step count");
     for (int s = 0; s < nsteps; ++s) {
@@ -830,7 +842,10 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
         T.run(PMC_KERNEL_SORT, [&] { pmc_launch::sort(v, st); });
         ctx->cur = 1 - ctx->cur;
         View& w = view(ctx);
-        T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::cell_prep(w, st); });
+        T.run(PMC_KERNEL_REFRESH, [&] {
+            if (w.inversion == 1) pmc_launch::invert(w, st, 0);
+            pmc_launch::cell_prep(w, st);
+        });
         if (w.callaway) {
             T.run(PMC_KERNEL_CALLAWAY, [&] { pmc_launch::callaway(w, st); });
         } else {
@@ -839,7 +854,10 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
             T.run(PMC_KERNEL_RELAX, [&] { pmc_launch::relax(w, st); });
         }
         if (w.hs_on) T.run(PMC_KERNEL_SOURCE, [&] { pmc_launch::source(w, st, source_time_factor(ctx->hs, mid, end), 1); });
-        T.run(PMC_KERNEL_REFRESH, [&] { pmc_launch::refresh(w, st, 2); });
+        T.run(PMC_KERNEL_REFRESH, [&] {
+            pmc_launch::refresh(w, st, 2);
+            if (w.inversion == 1) pmc_launch::invert(w, st, 1);
+        });
         int rc = check_hip(ctx, hipGetLastError(), "step launch");
         if (rc) {
             ctx->latched = true;
@@ -919,6 +937,11 @@ int pmc_read_carriers(pmc_ctx* ctx, int64_t* n, float* position, int32_t* mode, 
     ctx->note(hipStreamSynchronize(ctx->stream));
     ctx->note(hipMemcpy(&st, v.st, sizeof(DevState), hipMemcpyDeviceToHost));
     const int64_t count = st.n_slots;
+    // *n is the caller's capacity (slots) whenever it asks for data.
+    if ((position || mode || occupation || cell) && (*n < 0 || count > *n)) {
+        *n = count;
+        return fail(ctx, PMC_ERR_ARGUMENT, "carrier buffers hold fewer slots than are in use");
+    }
     if (position) {
         std::vector<float> x(count), y(count), z(count);
         ctx->note(hipMemcpy(x.data(), v.px, count * sizeof(float), hipMemcpyDeviceToHost));
@@ -941,6 +964,9 @@ int pmc_read_uids(pmc_ctx* ctx, int64_t n, uint32_t* uid) {
     if (!ctx || !uid) return PMC_ERR_ARGUMENT;
     View& v = view(ctx);
     ctx->note(hipStreamSynchronize(ctx->stream));
+    DevState st{};
+    ctx->note(hipMemcpy(&st, v.st, sizeof(DevState), hipMemcpyDeviceToHost));
+    if (n < 0 || n > st.n_slots) return fail(ctx, PMC_ERR_ARGUMENT, "uid count outside the slots in use");
     ctx->note(hipMemcpy(uid, v.uid, n * sizeof(unsigned), hipMemcpyDeviceToHost));
     return check_hip(ctx, hipGetLastError(), "read uids");
 }
@@ -953,6 +979,7 @@ int pmc_last_error(pmc_ctx* ctx, pmc_error* error) {
 
 int pmc_set_tuning(pmc_ctx* ctx, const pmc_tuning* t) {
     if (!ctx || !t) return PMC_ERR_ARGUMENT;
+    if (ctx->owner) return fail(ctx, PMC_ERR_ARGUMENT, "context belongs to a batch: tuning is fixed until the batch is destroyed");
     if (!(t->series_radius >= 0.0) || !(t->series_radius <= 0.5))
         return fail(ctx, PMC_ERR_ARGUMENT, "series_radius must lie in [0, 0.5]");
     ctx->view.series_on = t->series_on ? 1 : 0;
@@ -1011,6 +1038,8 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     if (!ctx->has_material || !ctx->has_geometry) return fail(ctx, PMC_ERR_ARGUMENT, "layers need material and geometry");
     if (ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "layers must be uploaded before the population is created");
     if (v.callaway || v.hs_on) return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run RTA without a heat source");
+    if (v.inversion != 0)
+        return fail(ctx, PMC_ERR_UNSUPPORTED, "Newton temperature inversion covers one material; layers use their tables");
     if (v.geom_kind != 0) return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials need the box geometry");
     const int n = L->count;
     if (n < 2 || n > pmc_dev::kMaxLayers || L->axis < 0 || L->axis > 2 || !L->planes || !L->mode_begin ||
@@ -1040,6 +1069,22 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     std::vector<int> cm(L->cell_material, L->cell_material + v.ncells);
     for (int c : cm)
         if (c < 0 || c >= n) return fail(ctx, PMC_ERR_ARGUMENT, "cell material out of range");
+    // Band start positions of every bank: the two lower_bound searches of a DMM
+    // band draw become two loads.  Same products j * band as the device's
+    // floor(w / band) * band, so the same band boundaries.  Narrower bands than
+    // the index holds (over 2^20 bands) keep the two searches on the device.
+    double wmax = 0.0;
+    for (int i = 0; i < nb; ++i) wmax = std::max(wmax, L->bank_omega[i]);
+    const double bands = std::floor(wmax / L->band_width) + 1.0;
+    const int nbands = bands <= static_cast<double>(1 << 20) ? static_cast<int>(bands) : 0;
+    std::vector<int32_t> starts(static_cast<size_t>(2 * n) * (nbands + 1));
+    for (int b = 0; b < 2 * n && nbands; ++b) {
+        const double* om = L->bank_omega + L->bank_begin[b];
+        const int cnt = L->bank_begin[b + 1] - L->bank_begin[b];
+        for (int j = 0; j <= nbands; ++j)
+            starts[static_cast<size_t>(b) * (nbands + 1) + j] =
+                static_cast<int32_t>(std::lower_bound(om, om + cnt, static_cast<double>(j) * L->band_width) - om);
+    }
     v.layer_axis = L->axis;
     for (int k = 0; k + 1 < n; ++k) v.layer_plane[k] = static_cast<float>(L->planes[k]);
     for (int k = 0; k <= n; ++k) {
@@ -1056,23 +1101,7 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     v.bank_mode = ctx->upload(L->bank_mode, static_cast<size_t>(nb));
     v.bank_omega = ctx->upload(L->bank_omega, static_cast<size_t>(nb));
     v.bank_prefix = ctx->upload(L->bank_prefix, static_cast<size_t>(nb) + 2 * n);
-    // Band start positions of every bank: the two lower_bound searches of a DMM
-    // band draw become two loads.  Same products j * band as the device's
-    // floor(w / band) * band, so the same band boundaries.
-    double wmax = 0.0;
-    for (int i = 0; i < nb; ++i) wmax = std::max(wmax, L->bank_omega[i]);
-    const double bands = std::floor(wmax / L->band_width) + 1.0;
-    if (!(bands <= 1 << 20)) return fail(ctx, PMC_ERR_ARGUMENT, "interface frequency band too narrow for the band index");
-    const int nbands = static_cast<int>(bands);
-    std::vector<int32_t> starts(static_cast<size_t>(2 * n) * (nbands + 1));
-    for (int b = 0; b < 2 * n; ++b) {
-        const double* om = L->bank_omega + L->bank_begin[b];
-        const int cnt = L->bank_begin[b + 1] - L->bank_begin[b];
-        for (int j = 0; j <= nbands; ++j)
-            starts[static_cast<size_t>(b) * (nbands + 1) + j] =
-                static_cast<int32_t>(std::lower_bound(om, om + cnt, static_cast<double>(j) * L->band_width) - om);
-    }
-    v.band_start = ctx->upload(starts.data(), starts.size());
+    v.band_start = nbands ? ctx->upload(starts.data(), starts.size()) : nullptr;
     v.band_count = nbands;
     v.layers = n;
     return check_hip(ctx, hipGetLastError(), "layers");
@@ -1090,6 +1119,7 @@ int pmc_get_launch_config(pmc_ctx* ctx, pmc_launch_config* out) {
 
 int pmc_set_launch_config(pmc_ctx* ctx, const pmc_launch_config* c) {
     if (!ctx || !c) return PMC_ERR_ARGUMENT;
+    if (ctx->owner) return fail(ctx, PMC_ERR_ARGUMENT, "context belongs to a batch: its launch split is fixed");
     if (!ctx->has_geometry) return fail(ctx, PMC_ERR_ARGUMENT, "geometry must be uploaded before the launch configuration");
     const int s = c->event_split, t = c->callaway_threads;
     if (!(s == 1 || s == 2 || s == 4 || s == 8) || s > pmc_dev::kMaxEventSplit)
@@ -1102,10 +1132,12 @@ int pmc_set_launch_config(pmc_ctx* ctx, const pmc_launch_config* c) {
 }
 
 int pmc_autotune(pmc_ctx* ctx, int32_t steps, pmc_launch_config* chosen, double* report_ms, int32_t report_len) {
-    if (!ctx || steps <= 0) return PMC_ERR_ARGUMENT;
+    if (!ctx || steps <= 0 || report_len < 0) return PMC_ERR_ARGUMENT;
     if (ctx->latched) return PMC_ERR_LATCHED;
+    if (ctx->owner) return fail(ctx, PMC_ERR_ARGUMENT, "context belongs to a batch: tune it before pmc_batch_create");
     if (!ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "carriers not initialized");
-    for (int k = 0; k < report_len; ++k) report_ms[k] = 0.0;
+    if (report_ms)
+        for (int k = 0; k < report_len; ++k) report_ms[k] = 0.0;
     const bool timing = ctx->timing;
     ctx->timing = true;
     auto trial = [&](int cls, double& ms) {
@@ -1246,6 +1278,9 @@ int pmc_batch_create(pmc_ctx* const* contexts, int32_t n, pmc_batch** out) {
         if (!c->initialized) return fail(c, PMC_ERR_ARGUMENT, "batch: carriers not initialized");
         if (c->view.hs_on) return fail(c, PMC_ERR_UNSUPPORTED, "batch: instances with a heat source are not batched");
         if (c->view.layers > 1) return fail(c, PMC_ERR_UNSUPPORTED, "batch: layered instances are not batched");
+        if (c->view.inversion != 0)
+            return fail(c, PMC_ERR_UNSUPPORTED, "batch: instances with Newton temperature inversion are not batched");
+        if (c->owner) return fail(c, PMC_ERR_ARGUMENT, "batch: a context already belongs to a batch");
         if (c->view.geom_kind != c0->view.geom_kind || c->view.callaway != c0->view.callaway)
             return fail(c, PMC_ERR_ARGUMENT, "batch: instances differ in geometry kind or collision model");
         if (c->cur != c0->cur || c->host_step != c0->host_step)
@@ -1285,6 +1320,7 @@ int pmc_batch_create(pmc_ctx* const* contexts, int32_t n, pmc_batch** out) {
     }
     std::vector<int> zero(n, 0);
     if (!(b->errors = batch_upload(b, zero.data(), zero.size()))) return bail("error words");
+    for (pmc_ctx* c : b->ctx) c->owner = b;
     *out = b;
     return PMC_OK;
 }
@@ -1302,6 +1338,9 @@ int pmc_batch_step(pmc_batch* b, int32_t nsteps) {
     if (b->failed >= 0) return PMC_ERR_LATCHED;
     if (nsteps < 0) return fail(b->ctx[0], PMC_ERR_ARGUMENT, "negative step count");
     const int n = static_cast<int>(b->ctx.size());
+    for (pmc_ctx* c : b->ctx)
+        if (c->cur != b->ctx[0]->cur || c->host_step != b->ctx[0]->host_step || c->owner != b)
+            return fail(c, PMC_ERR_INVALID_STATE, "batch: an instance left the batch's step");
     for (int s = 0; s < nsteps; ++s) {
         pmc_ctx* c0 = b->ctx[0];
         const int cur = c0->cur;
@@ -1356,6 +1395,8 @@ int pmc_batch_step(pmc_batch* b, int32_t nsteps) {
 void pmc_batch_destroy(pmc_batch* b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (pmc_ctx* c : b->ctx)
+        if (c && c->owner == b) c->owner = nullptr;
     for (auto& x : b->exec)
         if (x) (void)hipGraphExecDestroy(x);
     for (void* p : b->allocations) (void)hipFree(p);

@@ -194,6 +194,7 @@ struct View {
     const double* temps;
     const double2* gamma2;  // [ntemps][na]: (gamma(T_r), gamma(T_r+1)) of each active mode
     double w, e_bg, zp;
+    double e_norm;          // max(1, Nq*Vuc): crystal energy density = sum hw n / e_norm + zp
     int lut_n;
     const double* lut_t;
     const double* lut_e;
@@ -262,6 +263,7 @@ struct View {
     int partial_rows;     // allocated: transport blocks x kMaxEventSplit + injection blocks
     double* cell_e;       // energy density of the latest refresh
     double* cell_T;
+    double* cell_tx;      // Newton temperature of the post-transport energy (inversion 1)
     double* cell_flux;    // 3 per cell
     int* cell_count;      // live carriers per cell after transport and injection
     int* cell_start;      // [ncells+1] first slot of each cell in the sorted buffer

@@ -358,13 +358,35 @@ __device__ int sample_band(const View& V, int material, int direction, double w,
     const int b0 = V.bank_begin[b];
     const double* pre = V.bank_prefix + b0 + b; // bank size + 1 entries
     const double bin = floor(w / V.band);
-    if (!(bin < static_cast<double>(V.band_count))) { // above every frequency of every bank
-        flux = 0.0;
-        return -1;
+    int begin, end;
+    if (V.band_start) {
+        if (!(bin < static_cast<double>(V.band_count))) { // above every frequency of every bank
+            flux = 0.0;
+            return -1;
+        }
+        // [lower_bound(bin * band), lower_bound((bin + 1) * band)) from the band index.
+        const int* bs = V.band_start + b * (V.band_count + 1) + static_cast<int>(bin);
+        begin = bs[0];
+        end = bs[1];
+    } else { // bands too narrow for the index: the two searches
+        const double* om = V.bank_omega + b0;
+        const int cnt = V.bank_begin[b + 1] - b0;
+        const double lo_w = bin * V.band, hi_w = (bin + 1.0) * V.band;
+        int l = 0, h = cnt;
+        while (l < h) {
+            const int mid = (l + h) >> 1;
+            if (om[mid] < lo_w) l = mid + 1;
+            else h = mid;
+        }
+        begin = l;
+        h = cnt;
+        while (l < h) {
+            const int mid = (l + h) >> 1;
+            if (om[mid] < hi_w) l = mid + 1;
+            else h = mid;
+        }
+        end = l;
     }
-    // [lower_bound(bin * band), lower_bound((bin + 1) * band)) from the band index.
-    const int* bs = V.band_start + b * (V.band_count + 1) + static_cast<int>(bin);
-    const int begin = bs[0], end = bs[1];
     flux = end > begin ? pre[end] - pre[begin] : 0.0;
     if (!(flux > 0.0)) return -1;
     const double target = pre[begin] + u * flux;
@@ -1736,7 +1758,7 @@ __device__ __forceinline__ double rta_accumulate(const View& V, const CellRta& c
 __device__ __forceinline__ CellRta cell_rta(const View& V, int c) {
     const double e = V.cell_tally[static_cast<size_t>(kTallyStride) * c + kTallyEnergy] / V.cell_vol[c] +
                      cell_background(V, c);
-    const double T = cell_temperature(V, e, c);
+    const double T = V.inversion == 1 ? V.cell_tx[c] : cell_temperature(V, e, c);
     CellRta cr{};
     cr.t0 = T;
     cr.high = fmax(1000.0, 2.0 * fmax(1.0, T));
@@ -1758,6 +1780,54 @@ __global__ void kb_cell_prep(const View* __restrict__ views, const int* __restri
     Grid G;
     const int i = batch_instance(offs, n, G);
     cell_prep_body(views[i], G);
+}
+
+// Per-cell Newton inversion of the exact crystal energy (temperature_inversion 1):
+// solves sum_a hw_a n_BE(T, w_a) / e_norm + zp = e over the active modes, the sum
+// PhononMaterial::crystal_energy_density evaluates (src/material/Thermodynamics.cpp:151-163)
+// and from which the reference tabulates its lookup table (:178-224).  One block
+// per cell, started at the table's temperature, with dn/dT = n (n + 1) x / T.
+// source 0: the post-transport energy of the tally (into cell_tx, read by the
+// per-cell collision constants); source 1: the refreshed energy (cell_e, into cell_T).
+__global__ void __launch_bounds__(256) k_invert(View V, int source) {
+    __shared__ double red[2 * 4];
+    const int c = blockIdx.x;
+    if (c >= V.ncells) return;
+    const double e = source == 0 ? V.cell_tally[static_cast<size_t>(kTallyStride) * c + kTallyEnergy] / V.cell_vol[c] +
+                                       V.e_bg
+                                 : V.cell_e[c];
+    double T = lut_inverse(V, e, 0, V.lut_n);
+    const double target = (e - V.zp) * V.e_norm; // sum hw n_BE(T) the cell holds
+    if (target > 0.0 && T > 0.0) {
+        for (int it = 0; it < 32; ++it) {
+            double v[2] = {0.0, 0.0};
+            const double inv_t = 1.0 / T;
+            for (int a = threadIdx.x; a < V.na; a += blockDim.x) {
+                const double w = V.omega[a];
+                if (!(w > 0.0)) continue;
+                const double x = kHbar * w / (kBoltz * T);
+                if (x > 700.0) continue;
+                const double n = 1.0 / fmax(exp(x) - 1.0, 1e-12);
+                const double hw = kHbar * w;
+                v[0] += hw * n;
+                v[1] += hw * n * (n + 1.0) * x * inv_t;
+            }
+            block_reduce<2, 4>(v, red);
+            const double f = v[0] - target;
+            if (!(v[1] > 0.0)) break;
+            double next = T - f / v[1];
+            if (!(next > 0.0)) next = 0.5 * T;
+            const double step = next - T;
+            T = next;
+            if (fabs(step) <= 1e-14 * T) break;
+        }
+    } else {
+        T = 0.0;
+    }
+    if (threadIdx.x == 0) {
+        if (source == 0) V.cell_tx[c] = T;
+        else V.cell_T[c] = T;
+    }
 }
 
 // Pseudo-temperature sums of one chunk at the cell temperature (terms of
@@ -2955,6 +3025,9 @@ void refresh(const View& v, hipStream_t s, int phase) { hipLaunchKernelGGL(k_ref
 void sort(const View& v, hipStream_t s) {
     const int b = (v.capacity + 64 * kSortRows * kWavesPerBlock - 1) / (64 * kSortRows * kWavesPerBlock);
     hipLaunchKernelGGL(k_sort, dim3((b + 7) & ~7), kWaveBlock, 0, s, v);
+}
+void invert(const View& v, hipStream_t s, int source) {
+    hipLaunchKernelGGL(k_invert, dim3(v.ncells), dim3(256), 0, s, v, source);
 }
 void cell_prep(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_cell_prep, dim3(cell_blocks(v.ncells)), dim3(256), 0, s, v);

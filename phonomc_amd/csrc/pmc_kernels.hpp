@@ -34,6 +34,7 @@ void refresh(const pmc_dev::View& v, hipStream_t s, int phase);
 void events(const pmc_dev::View& v, hipStream_t s);
 void sort(const pmc_dev::View& v, hipStream_t s);
 void cell_prep(const pmc_dev::View& v, hipStream_t s);
+void invert(const pmc_dev::View& v, hipStream_t s, int source);
 void moments(const pmc_dev::View& v, hipStream_t s);
 void newton(const pmc_dev::View& v, hipStream_t s);
 void relax(const pmc_dev::View& v, hipStream_t s);

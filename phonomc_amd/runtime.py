@@ -32,7 +32,8 @@ class ModeTable(C.Structure):
                 ("neq_background", C.POINTER(C.c_double)), ("n_temps", C.c_int32), ("temps", C.POINTER(C.c_double)),
                 ("gamma_total", C.POINTER(C.c_double)), ("energy_weight", C.c_double),
                 ("background_energy_density", C.c_double), ("zero_point_density", C.c_double),
-                ("lut_n", C.c_int32), ("lut_t", C.POINTER(C.c_double)), ("lut_e", C.POINTER(C.c_double))]
+                ("lut_n", C.c_int32), ("lut_t", C.POINTER(C.c_double)), ("lut_e", C.POINTER(C.c_double)),
+                ("energy_normalization", C.c_double)]
 
 
 class Facet(C.Structure):
@@ -346,11 +347,16 @@ class Problem:
 class Context:
     """Device context (pmc_ctx) built from a Problem's views."""
 
-    def __init__(self, problem, seed=None, mode_sampling=None, capacity=None, initialize=True, initial_field=None):
+    def __init__(self, problem, seed=None, mode_sampling=None, capacity=None, initialize=True, initial_field=None,
+                 inversion=None):
         """initial_field: per-cell initial temperature [n_cells] (K), the warm start
-        (pmc_init.temperature_mode = 2); otherwise the input's own initial temperature."""
+        (pmc_init.temperature_mode = 2); otherwise the input's own initial temperature.
+        inversion: 1 for the per-cell Newton inversion of the exact crystal energy
+        instead of the lookup table (pmc_setup.temperature_inversion)."""
         L = lib()
         s, m, g, i, rough, res = problem.views()
+        if inversion is not None:
+            s.temperature_inversion = inversion
         if seed is not None:
             s.seed = seed
         if mode_sampling is not None:

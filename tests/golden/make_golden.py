@@ -6,6 +6,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --mesh [--mesh-runs]           (FinFET mesh fixtures only)
     python tests/golden/make_golden.py --callaway-source [--runs]     (Callaway + heat source only)
     python tests/golden/make_golden.py --layered [--runs]             (layered Si/Ge film only)
+    python tests/golden/make_golden.py --strict [--runs]              (strict-elastic walls of Callaway inputs)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -30,7 +31,8 @@ from phonomc_amd import runtime as rt  # noqa: E402
 OUT = os.path.dirname(os.path.abspath(__file__))
 WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml",
              "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml",
-             "c1cs": "c1cs_callaway_source.toml", "c1l": "c1l_layered.toml"}
+             "c1cs": "c1cs_callaway_source.toml", "c1l": "c1l_layered.toml",
+             "finfet_cw": "finfet_callaway.toml"}
 
 
 def write_input(text, folder, name):
@@ -113,6 +115,38 @@ def rough_vectors(problem, rm, rd, rng, facets):
         action, oa2, ov, oo2, absorbed, shift = rd.interact(rm, facet, 300.0, 1.0, ins, vel, occ, seeds)
         out[f"f{facet}_action"] = action
         out[f"f{facet}_ivel"] = ov
+    return out
+
+
+def strict_vectors(problem, rm, rd, rng, n_per_facet):
+    """Strict-elastic reflections of conservative transport on every reflecting
+    facet: RoughBoundarySampler built with require_discrete_reflection and the
+    background temperature, sampled with signed weights.  Nine in ten inputs are
+    modes incident on the facet with signed linearized occupations (some
+    negative); the rest are any active mode."""
+    s, m, g, i, rough, res = problem.views()
+    na = m.n_active
+    vel = rt.arr(m.velocity, 3 * na).reshape(na, 3)
+    neq = rt.arr(m.neq_background, na)
+    t_bg = s.background_temperature
+    facets = [g.facets[f] for f in range(g.n_facets)]
+    out = {"facets": np.array([f for f in range(g.n_facets) if facets[f].rough_table >= 0], dtype=np.int32),
+           "t_bg": t_bg}
+    for f in out["facets"]:
+        normal = np.array(facets[f].normal[:3])
+        incident = np.nonzero(vel @ normal > 0.0)[0]
+        ins = rng.choice(incident, n_per_facet).astype(np.int32)
+        anyk = rng.random(n_per_facet) < 0.1
+        ins[anyk] = rng.integers(0, na, int(anyk.sum()))
+        occ = neq[ins] + (neq[ins] + 0.05) * rng.normal(0.0, 0.3, n_per_facet)
+        occ[rng.random(n_per_facet) < 0.05] *= -1.0
+        seeds = rng.integers(1, 2 ** 62, n_per_facet, dtype=np.uint64)
+        oa, oo, u = rd.rough_sample_strict(rm, int(f), t_bg, ins, occ, seeds)
+        out[f"f{f}_in"] = ins
+        out[f"f{f}_occ"] = occ
+        out[f"f{f}_u"] = u
+        out[f"f{f}_out"] = oa
+        out[f"f{f}_out_occ"] = oo
     return out
 
 
@@ -314,7 +348,7 @@ def lib_count(refmat):
     return R.lib().ref_material_active_count(refmat.h)
 
 
-def whole_run(key, n, steps, seeds, interval):
+def whole_run(key, n, steps, seeds, interval, **overrides):
     """Reference runs at reduced size: final cell temperatures and kappa per seed
     (and, for layered inputs, the interface event and transmission totals)."""
     temps, keff, kint, iface = [], [], [], []
@@ -323,7 +357,7 @@ def whole_run(key, n, steps, seeds, interval):
         text = rt.config_text(WORKLOADS[key], **{"simulation.particle_count": n, "simulation.iterations": steps,
                                                  "simulation.random_seed": seed,
                                                  "simulation.convergence_write_interval": interval,
-                                                 "io.output_folder": f'"{tmp}/out"'})
+                                                 "io.output_folder": f'"{tmp}/out"', **overrides})
         path = write_input(text, tmp, "run.toml")
         problem = rt.Problem(text, input_directory=rt.CONFIGS)
         lv = problem.layers()
@@ -373,6 +407,25 @@ def main():
             np.savez_compressed(os.path.join(OUT, "run_finfet.npz"),
                                 **whole_run("finfet", 100000, 400, list(range(1000, 1016)), 10))
         print("mesh fixtures written to", OUT)
+        return
+    if "--strict" in sys.argv:
+        rng = np.random.default_rng(20261208)
+        # c2c as configured (1 nm: mostly diffuse), c2c with 0.05 nm walls (mostly
+        # specular partners) and the config-3 FinFET (10 nm, every reflecting facet).
+        smooth = {"boundary.boundary_values": "[302, 298, 0.0, 0.0, 0.05, 0.05]"}
+        for key, name, n, over in (("c2c", "rough_c2c.npz", 4000, {}),
+                                   ("c2c", "rough_c2c_smooth.npz", 4000, smooth),
+                                   ("finfet_cw", "rough_finfet_cw.npz", 1500, {})):
+            p, rm, rd, _ = problem_and_reference(key, include_stationary=True, **over)
+            np.savez_compressed(os.path.join(OUT, name), **strict_vectors(p, rm, rd, rng, n))
+            rm.close()
+        if "--runs" in sys.argv:
+            # config 3 physics (Callaway N/U + isotope on the FinFET mesh, diffuse
+            # walls, 1e20 W/m^3 source) on the 20 x 10 x 15 grid of the RTA FinFET runs
+            np.savez_compressed(os.path.join(OUT, "run_finfet_cw.npz"),
+                                **whole_run("finfet_cw", 100000, 400, list(range(1000, 1016)), 10,
+                                            **{"simulation.grid_xyz": "[20, 10, 15]"}))
+        print("strict-elastic fixtures written to", OUT)
         return
     if "--callaway-source" in sys.argv:
         rng = np.random.default_rng(20261206)

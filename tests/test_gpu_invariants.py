@@ -127,3 +127,33 @@ def test_walls_rounded_by_fp32_pass_the_containment_check():
     g.seed_carriers(pos, modes, occ)
     g.step(101)
     assert g.ledger()["step"] == 101
+
+
+def test_config3_finfet_callaway_full_size():
+    """Config 3 at its full size: the FinFET mesh with linearized Callaway N/U +
+    isotope scattering, diffuse (10 nm) strict-elastic walls, the fixed-flux
+    bottom reservoir and the 1e20 W/m^3 Gaussian source, 5e7 carriers on the
+    64 x 32 x 32 grid.  Over 101 steps the device containment check (steps 0 and
+    100) finds no escaped carrier, every step deposits the source's power
+    (rate x dt to 1e-12) and closes the energy balance (1e-12 of the thermal
+    energy), reflections take the strict-elastic tables, and the fin top heats."""
+    p = rt.Problem(rt.config_text("finfet_callaway.toml"), input_directory=rt.CONFIGS)
+    hs = p.heat_source()
+    total_rate = np.ctypeslib.as_array(hs.rate, shape=(hs.n_cells,)).sum()
+    dt = 0.2
+    g = rt.Context(p)
+    start = g.ledger()
+    assert abs(start["live_carriers"] - 50_000_000) < 100
+    for _ in range(101):
+        g.step(1)
+        led = g.ledger()
+        assert abs(led["step_balance_residual_ev"]) <= 1e-12 * abs(led["thermal_energy_ev"]) + 1e-9
+        assert led["step_source_energy_ev"] == pytest.approx(total_rate * dt, rel=1e-12)
+    assert led["step"] == 101 and led["rough_events"] > 0 and led["total_injected"] > 0
+    assert abs(led["live_carriers"] - 50_000_000) < 0.01 * 50_000_000
+    t, _ = g.cells()
+    centers, _ = p.cells()
+    fin_top = centers[:, 2] > 1300.0
+    substrate = centers[:, 2] < 300.0
+    assert np.isfinite(t).all()
+    assert t[fin_top].mean() > t[substrate].mean()

@@ -1,7 +1,7 @@
 """Whole-run statistics of the device step against the reference solver.
 
-tests/golden/run_c1.npz, run_c2.npz, run_c1s.npz, run_c1c.npz, run_c2c.npz, run_c1cs.npz, run_finfet.npz
-and run_c1l.npz hold 16 runs of the
+tests/golden/run_c1.npz, run_c2.npz, run_c1s.npz, run_c1c.npz, run_c2c.npz, run_c1cs.npz, run_finfet.npz,
+run_c1l.npz and run_finfet_cw.npz hold 16 runs of the
 reference PhonoMC build per workload (seeds 1000..1015, written by tests/golden/make_golden.py):
 second-half averages of every cell temperature and of kappa_int / kappa_eff as
 the reference writes them to convergence.txt.  The same TOML inputs and seeds
@@ -29,7 +29,10 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml",
              "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml",
-             "c1cs": "c1cs_callaway_source.toml", "c1l": "c1l_layered.toml"}
+             "c1cs": "c1cs_callaway_source.toml", "c1l": "c1l_layered.toml",
+             "finfet_cw": "finfet_callaway.toml"}
+# Inputs the reference runs were made with beyond the bundled file (tests/golden/make_golden.py).
+EXTRA = {"finfet_cw": {"simulation.grid_xyz": "[20, 10, 15]"}}
 
 
 def second_half(folder, ncell, steps):
@@ -58,7 +61,7 @@ def gpu_runs(key, golden):
         text = rt.config_text(WORKLOADS[key], **{"simulation.particle_count": n, "simulation.iterations": steps,
                                                  "simulation.random_seed": int(seed),
                                                  "simulation.convergence_write_interval": interval,
-                                                 "io.output_folder": f'"{tmp}/out"'})
+                                                 "io.output_folder": f'"{tmp}/out"', **EXTRA.get(key, {})})
         solver = rt.Solver(rt.Problem(text, input_directory=rt.CONFIGS), os.path.join(tmp, "out"))
         solver.run(steps)
         ncell = solver.n_cells
@@ -83,13 +86,15 @@ def z_score(a, b):
         return np.where(se > 0, diff / np.where(se > 0, se, 1.0), np.where(diff == 0, 0.0, np.inf))
 
 
-@pytest.mark.parametrize("key", ["c1", "c2", "c1s", "c1c", "c2c", "c1cs", "finfet", "c1l"])
+@pytest.mark.parametrize("key", ["c1", "c2", "c1s", "c1c", "c2c", "c1cs", "finfet", "c1l", "finfet_cw"])
 def test_whole_run_statistics_match_reference(key):
     """c1s adds the Gaussian heat source (kappa is not estimated with a source);
     c1c and c2c run linearized Callaway scattering (c2c with strict-elastic rough walls);
     c1cs adds the heat source to c1c (deposits by mode heat capacity); finfet runs the
     4080-triangle FinFET mesh with a heat source in the fin; c1l is the Si/Ge
-    layered film with a diffuse-mismatch interface."""
+    layered film with a diffuse-mismatch interface; finfet_cw is config 3's
+    physics (Callaway N/U + isotope on the FinFET, strict-elastic diffuse walls,
+    1e20 W/m^3 source, fixed-flux reservoir) on a 20 x 10 x 15 grid."""
     golden = np.load(os.path.join(HERE, "golden", f"run_{key}.npz"))
     temps, keff, kint, iface = gpu_runs(key, golden)
     zt = z_score(temps, golden["temperature"])
