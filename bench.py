@@ -28,12 +28,26 @@ dense BF16 MFMA peak.
 """
 import argparse
 import json
+import math
 import os
 import re
 import shutil
+import subprocess
 import sys
 import tempfile
 import time
+
+# The CPUs this process may use, read before any OpenMP library loads: with a
+# binding set, the OpenMP runtime pins the initial thread to one place and the
+# affinity mask seen afterwards is that single place.
+try:
+    HOST_AT_START = set(os.sched_getaffinity(0))
+except AttributeError:
+    HOST_AT_START = None
+# OpenMP reads its binding at start-up: threads of the CPU baseline bind one per
+# physical core, packed (set before any OpenMP library loads).
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -65,20 +79,50 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def host_cores():
+def cpu_topology():
+    """Host CPUs this process may use: the affinity set, the physical cores among
+    them (lscpu -p=CPU,Core,Socket) and the cgroup CPU quota when one is set.
+    The baseline runs one OpenMP thread per physical core, capped by the quota."""
+    if HOST_AT_START is not None:  # taken before OpenMP pinned this thread
+        allowed = HOST_AT_START
+    else:
+        allowed = set(range(os.cpu_count() or 1))
+    physical = None
     try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+        out = subprocess.run(["lscpu", "-p=CPU,Core,Socket"], capture_output=True, text=True, check=True).stdout
+        cores = set()
+        for line in out.splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            cpu, core, sock = line.split(",")[:3]
+            if int(cpu) in allowed:
+                cores.add((sock, core))
+        physical = len(cores) or None
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        pass
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    threads = physical or len(allowed)
+    if quota:
+        threads = min(threads, max(1, math.floor(quota)))
+    return {"threads": max(1, threads), "physical_cores": physical, "affinity_cpus": len(allowed),
+            "cgroup_cpus": quota, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
 
 
-def cpu_baseline(problem_text, steps, workload=WORKLOAD, carriers=1e7):
-    """Times the reference solver (oracle/_ref) on the full workload for `steps` steps."""
+def cpu_baseline(problem_text, steps, workload=WORKLOAD, carriers=1e7, warmup=50):
+    """Times the reference solver (oracle/_ref) on the full workload: `warmup`
+    untimed steps after initialization, then `steps` timed steps."""
     from oracle import ref_ctypes
     from phonomc_amd import runtime as rt
 
-    cores = host_cores()
+    topo = cpu_topology()
+    cores = topo["threads"]
     tmp = tempfile.mkdtemp(prefix="pmc_ref_")
     text = re.sub(r'output_folder = "[^"]*"', f'output_folder = "{tmp}/out"', problem_text)
     path = os.path.join(tmp, "workload.toml")
@@ -96,10 +140,10 @@ def cpu_baseline(problem_text, steps, workload=WORKLOAD, carriers=1e7):
         try:
             lv = problem.layers()
             if lv is None:
-                secs, ps = ref_ctypes.run(path, problem.material(), steps, threads=cores)
+                secs, ps = ref_ctypes.run(path, problem.material(), steps, threads=cores, warmup=warmup)
             else:  # layered materials: every layer's table, in order
                 secs, ps = ref_ctypes.run_materials(path, [problem.layer_material(k) for k in range(lv.count)],
-                                                    steps, threads=cores)[:2]
+                                                    steps, threads=cores, warmup=warmup)[:2]
         finally:
             sys.stdout.flush()
             os.dup2(saved, 1)
@@ -109,6 +153,7 @@ def cpu_baseline(problem_text, steps, workload=WORKLOAD, carriers=1e7):
         from oracle import oracle_ctypes as O
         sim = O.Sim(problem, threads=cores)
         sim.initialize()
+        sim.step(warmup)
         t0 = time.perf_counter()
         before = sim.ledger()["carrier_steps"]
         sim.step(steps)
@@ -116,8 +161,10 @@ def cpu_baseline(problem_text, steps, workload=WORKLOAD, carriers=1e7):
         ps = sim.ledger()["carrier_steps"] - before
         kind = "port"
     return {"value": ps / secs, "unit": "particle-steps/s", "cores": cores, "kind": kind,
-            "sample": f"{workload} at {carriers:.3g} carriers, {steps} steps of run_timestep after initialization, "
-                      f"OpenMP threads={cores}", "seconds": secs}
+            "sample": f"{workload} at {carriers:.3g} carriers: {warmup} warm-up then {steps} timed steps of "
+                      f"run_timestep after initialization, OpenMP threads={cores} (one per physical core, "
+                      f"OMP_PROC_BIND={topo['omp_proc_bind']}, OMP_PLACES={topo['omp_places']})",
+            "seconds": secs, "warmup_steps": warmup, "timed_steps": steps, "host": topo}
 
 
 def copy_probe(mib):
@@ -242,7 +289,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--timing-steps", type=int, default=40)
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-steps", type=int, default=200, help="timed steps of the CPU baseline")
+    ap.add_argument("--cpu-warmup", type=int, default=50, help="untimed steps of the CPU baseline first")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the warm-start leg")
     ap.add_argument("--workload", default=WORKLOAD, help="bundled TOML input (phonomc_amd/configs) to time")
@@ -324,8 +372,8 @@ def main():
 
     baseline = None
     if not args.no_cpu_baseline:
-        log(f"[bench] CPU baseline: reference solver, {args.cpu_steps} steps")
-        baseline = cpu_baseline(text, args.cpu_steps, args.workload, n0)
+        log(f"[bench] CPU baseline: reference solver, {args.cpu_warmup} + {args.cpu_steps} steps")
+        baseline = cpu_baseline(text, args.cpu_steps, args.workload, n0, warmup=args.cpu_warmup)
         log(f"[bench] CPU baseline {baseline['value']:.4e} particle-steps/s on {baseline['cores']} threads")
 
     ctx.close()

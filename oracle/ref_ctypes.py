@@ -328,18 +328,19 @@ def source_apply(domain, material, integrated_time, cell_temperature, active, ce
     return occ, dep.value, upd.value
 
 
-def run(toml_path, material_view, steps, threads=0, lookup_dt=0.1):
+def run(toml_path, material_view, steps, threads=0, lookup_dt=0.1, warmup=0):
     mat = RefMaterial(material_view, lookup_dt)  # keeps the arrays struct alive
     secs, ps = C.c_double(), C.c_longlong()
     err = C.create_string_buffer(1024)
-    rc = lib().ref_run(toml_path.encode(), C.byref(mat.arrays), steps, threads, C.byref(secs), C.byref(ps), err, 1024)
+    rc = lib().ref_run(toml_path.encode(), C.byref(mat.arrays), warmup, steps, threads, C.byref(secs), C.byref(ps),
+                       err, 1024)
     mat.close()
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return secs.value, ps.value
 
 
-def run_materials(toml_path, material_views, steps, threads=0, lookup_dt=0.1):
+def run_materials(toml_path, material_views, steps, threads=0, lookup_dt=0.1, warmup=0):
     """ref_run_materials: (seconds, particle_steps, interface events, interface transmissions,
     run summary text)."""
     arr = (MaterialArrays * len(material_views))(*[material_arrays(v, lookup_dt) for v in material_views])
@@ -347,7 +348,7 @@ def run_materials(toml_path, material_views, steps, threads=0, lookup_dt=0.1):
     iface = (C.c_longlong * 2)()
     err = C.create_string_buffer(1024)
     summary = C.create_string_buffer(1 << 16)
-    rc = lib().ref_run_materials(toml_path.encode(), arr, len(material_views), steps, threads, C.byref(secs),
+    rc = lib().ref_run_materials(toml_path.encode(), arr, len(material_views), warmup, steps, threads, C.byref(secs),
                                  C.byref(ps), iface, summary, 1 << 16, err, 1024)
     if rc != 0:
         raise RuntimeError(err.value.decode())

@@ -516,16 +516,16 @@ int ref_source_apply(void* domain, void* material, double integrated_time_ps, co
     });
 }
 
-int ref_run(const char* toml_path, const ref_material_arrays* arrays, int steps, int threads,
+int ref_run(const char* toml_path, const ref_material_arrays* arrays, int warmup, int steps, int threads,
             double* seconds, long long* particle_steps, char* err, int errlen) {
     long long interface[2];
-    return ref_run_materials(toml_path, arrays, 1, steps, threads, seconds, particle_steps, interface, nullptr, 0, err,
-                             errlen);
+    return ref_run_materials(toml_path, arrays, 1, warmup, steps, threads, seconds, particle_steps, interface, nullptr,
+                             0, err, errlen);
 }
 
-int ref_run_materials(const char* toml_path, const ref_material_arrays* arrays, int n_materials, int steps,
-                      int threads, double* seconds, long long* particle_steps, long long* interface, char* summary,
-                      int summary_len, char* err, int errlen) {
+int ref_run_materials(const char* toml_path, const ref_material_arrays* arrays, int n_materials, int warmup,
+                      int steps, int threads, double* seconds, long long* particle_steps, long long* interface,
+                      char* summary, int summary_len, char* err, int errlen) {
     return guarded(err, errlen, [&] {
 #ifdef PHONOMC_USE_OPENMP
         if (threads > 0) omp_set_num_threads(threads);
@@ -544,6 +544,7 @@ int ref_run_materials(const char* toml_path, const ref_material_arrays* arrays, 
             materials.push_back(std::cref(*storage.back()));
         }
         MonteCarloSolver solver(config, domain, materials);
+        for (int s = 0; s < warmup; ++s) solver.run_timestep();
         long long carried = 0;
         const auto begin = std::chrono::steady_clock::now();
         for (int s = 0; s < steps; ++s) {

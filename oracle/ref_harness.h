@@ -133,19 +133,20 @@ int ref_rta_apply(void* material, int ncells, const double* cell_temperature, in
                   double* occupation, double dt, double particle_volume, int local_lifetime,
                   double lifetime_temperature, double* residual_ev, char* err, int errlen);
 
-/* Whole reference run: TOML input, in-memory material, `steps` calls of
- * MonteCarloSolver::run_timestep.  The reference writes its usual result folder
- * (the TOML's io.output_folder).  seconds = wall time of the step loop only. */
-int ref_run(const char* toml_path, const ref_material_arrays* arrays, int steps, int threads,
+/* Whole reference run: TOML input, in-memory material, `warmup` untimed then
+ * `steps` timed calls of MonteCarloSolver::run_timestep.  The reference writes
+ * its usual result folder (the TOML's io.output_folder).  seconds = wall time
+ * of the timed step loop only; particle_steps counts its live carriers. */
+int ref_run(const char* toml_path, const ref_material_arrays* arrays, int warmup, int steps, int threads,
             double* seconds, long long* particle_steps, char* err, int errlen);
 
 /* The same over several in-memory materials (materials.folders, in order);
  * interface[2] = MonteCarloSolver::interface_event_count and
  * interface_transmitted_count after the last step; summary (if not null) =
  * MonteCarloSolver::append_profile_summary's text, truncated to summary_len - 1. */
-int ref_run_materials(const char* toml_path, const ref_material_arrays* arrays, int n_materials, int steps,
-                      int threads, double* seconds, long long* particle_steps, long long* interface, char* summary,
-                      int summary_len, char* err, int errlen);
+int ref_run_materials(const char* toml_path, const ref_material_arrays* arrays, int n_materials, int warmup,
+                      int steps, int threads, double* seconds, long long* particle_steps, long long* interface,
+                      char* summary, int summary_len, char* err, int errlen);
 
 /* InterfaceInteraction::interact (DMM) on n carriers: material[i], active
  * index active[i] of that material, velocity[3i..], occupation[i], toward

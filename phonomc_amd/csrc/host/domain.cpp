@@ -6,6 +6,8 @@
 #include <filesystem>
 #include <fstream>
 #include <limits>
+#include <map>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 
@@ -160,10 +162,59 @@ void build_mesh_domain(Domain& d, const SimConfig& c) {
 }
 } // namespace
 
+namespace {
+// Mesh surfaces and their voxel grids, reused across problems in one process:
+// parameter sweeps (batches of one geometry with different sources, reservoir
+// temperatures or seeds) would otherwise re-read the surface and re-test every
+// voxel centre for each instance.  Keyed by the resolved file (path, size,
+// modification time) and the coplanar-merge flag, the grid also by its sizes.
+struct GridCells {
+    Vec3 inv_cell{};
+    std::vector<int> voxel_to_cell;
+    std::vector<Vec3> cell_center;
+};
+std::mutex cache_lock;
+std::map<std::string, Domain> surface_cache;
+std::map<std::string, GridCells> grid_cache;
+constexpr size_t kCacheEntries = 8;
+
+std::string surface_key(const SimConfig& c) {
+    namespace fs = std::filesystem;
+    fs::path path(c.model);
+    if (!path.is_absolute()) {
+        const fs::path p1 = fs::current_path() / path, p2 = fs::path(c.input_directory) / path;
+        path = fs::exists(p1) ? p1 : p2;
+    }
+    std::error_code ec;
+    const auto size = fs::file_size(path, ec);
+    const auto stamp = fs::last_write_time(path, ec).time_since_epoch().count();
+    std::ostringstream k;
+    k << fs::absolute(path.lexically_normal()).string() << '|' << size << '|' << stamp << '|'
+      << c.merge_coplanar_facets;
+    return k.str();
+}
+} // namespace
+
 Domain build_domain(const SimConfig& c) {
     Domain d;
+    std::string key;
     if (c.model != "box") {
-        build_mesh_domain(d, c);
+        key = surface_key(c);
+        bool hit = false;
+        {
+            std::lock_guard<std::mutex> hold(cache_lock);
+            auto it = surface_cache.find(key);
+            if (it != surface_cache.end()) {
+                d = it->second;
+                hit = true;
+            }
+        }
+        if (!hit) {
+            build_mesh_domain(d, c);
+            std::lock_guard<std::mutex> hold(cache_lock);
+            if (surface_cache.size() >= kCacheEntries) surface_cache.erase(surface_cache.begin());
+            surface_cache[key] = d;
+        }
     } else {
         if (c.sizes.size() != 3 ||
             std::any_of(c.sizes.begin(), c.sizes.end(), [](double s) { return !std::isfinite(s) || s <= 0; }))
@@ -261,6 +312,19 @@ Domain build_domain(const SimConfig& c) {
     const int nx = c.grid[0], ny = c.grid[1], nz = c.grid[2];
     if (nx <= 0 || ny <= 0 || nz <= 0) throw std::runtime_error("Grid sizes must be positive.");
     d.grid = {nx, ny, nz};
+    const std::string grid_key = key.empty() ? std::string() : key + "|" + std::to_string(nx) + "x" +
+                                                                 std::to_string(ny) + "x" + std::to_string(nz);
+    if (!grid_key.empty()) {
+        std::lock_guard<std::mutex> hold(cache_lock);
+        auto it = grid_cache.find(grid_key);
+        if (it != grid_cache.end()) {
+            d.inv_cell = it->second.inv_cell;
+            d.voxel_to_cell = it->second.voxel_to_cell;
+            d.cell_center = it->second.cell_center;
+            d.cell_volume.assign(d.cell_center.size(), d.volume / static_cast<double>(d.cell_center.size()));
+            return d;
+        }
+    }
     const Vec3 ext = sub(d.bmax, d.bmin);
     for (int a = 0; a < 3; ++a) d.inv_cell[a] = ext[a] > 0.0 ? static_cast<double>(d.grid[a]) / ext[a] : 0.0;
     d.voxel_to_cell.assign(static_cast<size_t>(nx) * ny * nz, -1);
@@ -296,6 +360,11 @@ Domain build_domain(const SimConfig& c) {
         }
     }
     d.cell_volume.assign(d.cell_center.size(), d.volume / static_cast<double>(d.cell_center.size()));
+    if (!grid_key.empty()) {
+        std::lock_guard<std::mutex> hold(cache_lock);
+        if (grid_cache.size() >= kCacheEntries) grid_cache.erase(grid_cache.begin());
+        grid_cache[grid_key] = GridCells{d.inv_cell, d.voxel_to_cell, d.cell_center};
+    }
     return d;
 }
 

@@ -629,25 +629,15 @@ int pmc_upload_heat_source(pmc_ctx* ctx, const pmc_heat_source* h) {
     ctx->hs = *h;
     ctx->hs.rate = nullptr;
     ctx->hs.mode_eligible = nullptr;
+    v.hs_square = h->square;
+    v.hs_start = h->start_ps;
+    v.hs_end = h->end_ps;
+    v.hs_period = h->period_ps;
+    v.hs_on_duration = h->on_duration_ps;
+    v.hs_duty = h->duty_cycle;
+    v.hs_amplitude = h->amplitude;
     v.hs_on = 1;
     return check_hip(ctx, hipGetLastError(), "heat source upload");
-}
-
-// Integrated drive time over [t0, t1) (MonteCarloSolver::local_heat_source_integrated_time_factor,
-// src/solver/HeatSourceDriving.cpp:14-46).
-static double source_time_factor(const pmc_heat_source& h, double t0, double t1) {
-    const double b = std::max(t0, h.start_ps);
-    const double e = h.end_ps >= 0.0 ? std::min(t1, h.end_ps) : t1;
-    if (e <= b) return 0.0;
-    if (!h.square) return h.amplitude * (e - b);
-    const double on = h.on_duration_ps < 0.0 ? h.duty_cycle * h.period_ps : h.on_duration_ps;
-    if (on <= 0.0) return 0.0;
-    auto on_time = [&](double rel) {
-        if (!(rel > 0.0)) return 0.0;
-        const double cycles = std::floor(rel / h.period_ps);
-        return cycles * on + std::min(rel - cycles * h.period_ps, on);
-    };
-    return h.amplitude * std::max(0.0, on_time(e - h.start_ps) - on_time(b - h.start_ps));
 }
 
 static int finish_population(pmc_ctx* ctx, int64_t count) {
@@ -828,13 +818,12 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
         View& v = view(ctx);
         hipStream_t st = ctx->stream;
         StepTimer T{ctx};
-        const double mid = ctx->elapsed_ps + 0.5 * v.dt, end = ctx->elapsed_ps + v.dt;
         T.run(PMC_KERNEL_BEGIN, [&] {
             pmc_launch::begin(v, st);
             if (ctx->host_step % 100 == 0) pmc_launch::containment(v, st);
         });
         if (v.hs_on)
-            T.run(PMC_KERNEL_SOURCE, [&] { pmc_launch::source(v, st, source_time_factor(ctx->hs, ctx->elapsed_ps, mid), 0); });
+            T.run(PMC_KERNEL_SOURCE, [&] { pmc_launch::source(v, st, 0); });
         T.run(PMC_KERNEL_TRANSPORT, [&] { pmc_launch::transport(v, st); });
         T.run(PMC_KERNEL_EVENTS, [&] { pmc_launch::events(v, st); });
         T.run(PMC_KERNEL_INJECT, [&] { pmc_launch::inject(v, st); });
@@ -853,7 +842,7 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
             T.run(PMC_KERNEL_NEWTON, [&] { pmc_launch::newton(w, st); });
             T.run(PMC_KERNEL_RELAX, [&] { pmc_launch::relax(w, st); });
         }
-        if (w.hs_on) T.run(PMC_KERNEL_SOURCE, [&] { pmc_launch::source(w, st, source_time_factor(ctx->hs, mid, end), 1); });
+        if (w.hs_on) T.run(PMC_KERNEL_SOURCE, [&] { pmc_launch::source(w, st, 1); });
         T.run(PMC_KERNEL_REFRESH, [&] {
             pmc_launch::refresh(w, st, 2);
             if (w.inversion == 1) pmc_launch::invert(w, st, 1);
@@ -1203,7 +1192,7 @@ struct pmc_batch {
     int* offs[pmc_launch::kBatchKinds] = {}; // device prefix offsets of block counts
     int total[pmc_launch::kBatchKinds] = {};
     int* errors = nullptr;                  // device: error word per instance
-    bool mesh = false, callaway = false;
+    bool mesh = false, callaway = false, source = false;
     bool use_graph = true;
     hipGraphExec_t exec[2] = {};
     int failed = -1;
@@ -1227,6 +1216,12 @@ void batch_sequence(pmc_batch* b, int cur) {
     const int n = static_cast<int>(b->ctx.size());
     auto run = [&](int kind, int buf) { batch_launch(kind, b->mesh, b->views[buf], b->offs[kind], n, b->total[kind], b->stream); };
     run(kBatchBegin, cur);
+    if (b->source) { // first half-step deposit (src/MonteCarloSolver.cpp:366-369)
+        run(kBatchSourceSums0, cur);
+        run(kBatchSourceSolve0, cur);
+        run(kBatchSourceIncrements, cur);
+        run(kBatchSourceApply0, cur);
+    }
     run(kBatchTransport, cur);
     run(kBatchEvents, cur);
     run(kBatchInject, cur);
@@ -1239,6 +1234,12 @@ void batch_sequence(pmc_batch* b, int cur) {
         run(kBatchMoments, 1 - cur);
         run(kBatchNewton, 1 - cur);
         run(kBatchRelax, 1 - cur);
+    }
+    if (b->source) { // second half-step deposit (:377-379)
+        run(kBatchSourceSums1, 1 - cur);
+        run(kBatchSourceSolve1, 1 - cur);
+        run(kBatchSourceIncrements, 1 - cur);
+        run(kBatchSourceApply1, 1 - cur);
     }
     run(kBatchRefresh2, 1 - cur);
 }
@@ -1276,7 +1277,6 @@ int pmc_batch_create(pmc_ctx* const* contexts, int32_t n, pmc_batch** out) {
         if (!c) return PMC_ERR_ARGUMENT;
         if (c->latched) return PMC_ERR_LATCHED;
         if (!c->initialized) return fail(c, PMC_ERR_ARGUMENT, "batch: carriers not initialized");
-        if (c->view.hs_on) return fail(c, PMC_ERR_UNSUPPORTED, "batch: instances with a heat source are not batched");
         if (c->view.layers > 1) return fail(c, PMC_ERR_UNSUPPORTED, "batch: layered instances are not batched");
         if (c->view.inversion != 0)
             return fail(c, PMC_ERR_UNSUPPORTED, "batch: instances with Newton temperature inversion are not batched");
@@ -1293,6 +1293,7 @@ int pmc_batch_create(pmc_ctx* const* contexts, int32_t n, pmc_batch** out) {
     b->ctx.assign(contexts, contexts + n);
     b->mesh = c0->view.geom_kind == 1;
     b->callaway = c0->view.callaway != 0;
+    for (pmc_ctx* c : b->ctx) b->source = b->source || c->view.hs_on;
     auto bail = [&](const char* what) {
         pmc_batch_destroy(b);
         return fail(c0, PMC_ERR_DEVICE, std::string("batch: ") + what);

@@ -141,6 +141,7 @@ struct DevState {
     int n_live;
     int n_slots_start;    // slots before injection this step
     int step;             // completed steps
+    double elapsed;       // simulated time (ps), summed dt by dt like the reference's elapsed_time_
     unsigned next_uid;
     int emit_total;
     int emit_count[kMaxReservoirs];     // emissions this step per reservoir
@@ -284,6 +285,9 @@ struct View {
     double* hs_acc;              // kSrcCols per chunk
     double* hs_cell;             // kSrcCellCols per cell: T_b, T_s, target, deposit flag
     double* chunk_src;           // energy change of the second half-step deposit per chunk (tally units)
+    // time profile of the source (pmc_heat_source): constant or square wave
+    int hs_square;
+    double hs_start, hs_end, hs_period, hs_on_duration, hs_duty, hs_amplitude;
     int* events;          // per transport block: slots with a wall event this step
     int* block_events;    // per transport block: number of such slots
     DevState* st;

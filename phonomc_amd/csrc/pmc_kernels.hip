@@ -1564,6 +1564,7 @@ __device__ __forceinline__ void refresh_body(const View& V, const Grid G, int ph
             st->total_balance_e += balance;
             st->thermal_e = energy;
             st->step += 1;
+            st->elapsed += V.dt;
         }
         st->n_slots = st->n_live;
     }
@@ -2134,9 +2135,7 @@ __device__ __forceinline__ void block_max2(double& a, double& b, double* lds) {
     __syncthreads();
 }
 
-// Coefficients of the kept orthonormal directions (modified Gram-Schmidt with
-// two passes in the inner product <c, d> = c^T G d, kept directions in basis
-// order) summed into P = sum c c^T.
+// Inner product <a, b> = a^T G b of basis coefficient vectors.
 __device__ __forceinline__ double cw_inner(const double (&G)[4][4], const double (&a)[4], const double (&b)[4]) {
     double s = 0.0;
 #pragma unroll
@@ -2147,45 +2146,6 @@ __device__ __forceinline__ double cw_inner(const double (&G)[4][4], const double
         s += a[i] * gb;
     }
     return s;
-}
-
-__device__ void cw_projector(const double (&G)[4][4], double (&P)[4][4]) {
-    double Q[4][4];
-    bool keep[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        keep[k] = false;
-        double c[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) Q[k][i] = 0.0;
-        if (!(G[k][k] > 0.0)) continue; // inactive mechanism or absent momentum component
-        c[k] = 1.0 / sqrt(G[k][k]);
-#pragma unroll
-        for (int pass = 0; pass < 2; ++pass)
-#pragma unroll
-            for (int q = 0; q < k; ++q) {
-                if (!keep[q]) continue;
-                const double proj = cw_inner(G, Q[q], c);
-#pragma unroll
-                for (int i = 0; i < 4; ++i) c[i] -= proj * Q[q][i];
-            }
-        const double len2 = cw_inner(G, c, c);
-        if (!(len2 >= 1e-14)) continue; // dependent direction
-        const double inv = 1.0 / sqrt(len2);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) Q[k][i] = c[i] * inv;
-        keep[k] = true;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            double v = 0.0;
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (keep[k]) v += Q[k][i] * Q[k][j];
-            P[i][j] = v;
-        }
 }
 
 // T threads per workgroup (one cell), up to kCwPer carriers per thread.
@@ -2246,19 +2206,59 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
     }
     block_reduce_wide<17, kW>(acc, lds);
     block_max2<kW>(rmax, xmax, lds);
-    if (t == 0) {
-        double G[4][4], P[4][4];
+    {
+        // Normal-gain directions (CallawayOperator.cpp:23-57): modified
+        // Gram-Schmidt, two passes, in coefficient space of the basis
+        // sqrt(g_N) B with the Gram matrix G; every thread holds G and runs it.
+        // Projections use G; the length that decides whether a direction is
+        // dependent (the reference drops lengths below 1e-12 of the normalized
+        // vector) is summed explicitly over the cell's carriers, since in G it
+        // would be a difference of O(1) terms resolved only to about 1e-8.
+        double Gm[4][4], Q[4][4];
+        bool keep[4];
         int q = 0;
         for (int a = 0; a < 4; ++a)
             for (int d = a; d < 4; ++d) {
-                G[a][d] = acc[q];
-                G[d][a] = acc[q];
+                Gm[a][d] = acc[q];
+                Gm[d][a] = acc[q];
                 ++q;
             }
-        cw_projector(G, P);
-        for (int a = 0; a < 4; ++a)
-            for (int d = 0; d < 4; ++d) Psh[4 * a + d] = P[a][d];
-        Psh[16] = acc[10] > 0.0 ? 1.0 / acc[10] : 0.0;
+        for (int k = 0; k < 4; ++k) {
+            keep[k] = false;
+            for (int i = 0; i < 4; ++i) Q[k][i] = 0.0;
+            if (!(Gm[k][k] > 0.0)) continue; // inactive mechanism or absent momentum component
+            double cf[4] = {0.0, 0.0, 0.0, 0.0};
+            cf[k] = 1.0 / sqrt(Gm[k][k]);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int j = 0; j < k; ++j) {
+                    if (!keep[j]) continue;
+                    const double proj = cw_inner(Gm, Q[j], cf);
+                    for (int i = 0; i < 4; ++i) cf[i] -= proj * Q[j][i];
+                }
+            double len2[1] = {0.0};
+#pragma unroll
+            for (int r = 0; r < kCwPer; ++r) {
+                if (md[r] < 0) continue;
+                const double4 p0 = V.cw_pack[2 * md[r]], p1 = V.cw_pack[2 * md[r] + 1];
+                const double v = cf[0] * p0.z + cf[1] * p0.w + cf[2] * p1.x + cf[3] * p1.y;
+                len2[0] += p0.x * v * v;
+            }
+            block_reduce_wide<1, kW>(len2, lds);
+            if (!(len2[0] >= 1e-24)) continue; // rank-deficient cell geometry
+            const double inv = 1.0 / sqrt(len2[0]);
+            for (int i = 0; i < 4; ++i) Q[k][i] = cf[i] * inv;
+            keep[k] = true;
+        }
+        if (t == 0) {
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 4; ++d) {
+                    double v = 0.0;
+                    for (int k = 0; k < 4; ++k)
+                        if (keep[k]) v += Q[k][a] * Q[k][d];
+                    Psh[4 * a + d] = v;
+                }
+            Psh[16] = acc[10] > 0.0 ? 1.0 / acc[10] : 0.0;
+        }
     }
     __syncthreads();
     const double gr_inv = Psh[16];
@@ -2381,6 +2381,27 @@ __global__ void __launch_bounds__(kCwThreads) kb_callaway(const View* __restrict
 // (base temperature: the previous refresh), half = 1 after relaxation (base:
 // the refresh after transport).
 
+// Integrated drive time of the half step (MonteCarloSolver::
+// local_heat_source_integrated_time_factor, src/solver/HeatSourceDriving.cpp:14-46)
+// from the device's elapsed time: half 0 covers [t, t + dt/2), half 1
+// [t + dt/2, t + dt), with t summed step by step as the reference sums it.
+__device__ double source_factor(const View& V, int half) {
+    const double t0 = V.st->elapsed, mid = t0 + 0.5 * V.dt, t1 = t0 + V.dt;
+    const double a = half == 0 ? t0 : mid, z = half == 0 ? mid : t1;
+    const double b = fmax(a, V.hs_start);
+    const double e = V.hs_end >= 0.0 ? fmin(z, V.hs_end) : z;
+    if (e <= b) return 0.0;
+    if (!V.hs_square) return V.hs_amplitude * (e - b);
+    const double on = V.hs_on_duration < 0.0 ? V.hs_duty * V.hs_period : V.hs_on_duration;
+    if (on <= 0.0) return 0.0;
+    auto on_time = [&](double rel) {
+        if (!(rel > 0.0)) return 0.0;
+        const double cycles = floor(rel / V.hs_period);
+        return cycles * on + fmin(rel - cycles * V.hs_period, on);
+    };
+    return V.hs_amplitude * fmax(0.0, on_time(e - V.hs_start) - on_time(b - V.hs_start));
+}
+
 __device__ __forceinline__ double source_base_T(const View& V, int c, int half) {
     const double t = half == 0 ? V.cell_T[c] : V.crta[c].t0;
     return (isfinite(t) && t >= 0.0) ? t : V.t_bg;
@@ -2403,10 +2424,11 @@ __device__ __forceinline__ int chunk_range(const View& V, int b, int& begin, int
 
 // Per chunk: Taylor terms of sum hw n_BE(T) about the base temperature T_b of
 // the eligible carriers of cells with pending energy.
-__global__ void __launch_bounds__(kChunkThreads) k_source_sums(View V, double factor, int half) {
+__device__ __forceinline__ void source_sums_body(const View& V, const Grid G, int half) {
     __shared__ double lds[kSrcCols * kChunkWaves];
-    const int b = blockIdx.x;
+    const int b = G.b;
     if (b >= V.chunk_start[V.ncells]) return;
+    const double factor = source_factor(V, half);
     int begin, end;
     const int c = chunk_range(V, b, begin, end);
     const double pend = V.hs_pending[c] + V.hs_rate[c] * factor;
@@ -2485,8 +2507,9 @@ __device__ void source_exact(const View& V, int begin, int end, double tb, doubl
 // 1e-12 * max(1e-6, target).  Newton from T_b on the series when it is trusted,
 // else the reference's bracket + safeguarded Newton on exact sums (:259-293).
 // Cells without eligible carriers keep their budget (:162-164).
-__global__ void __launch_bounds__(256) k_source_solve(View V, double factor, int half) {
-    const int c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+__device__ __forceinline__ void source_solve_body(const View& V, const Grid G, int half) {
+    const int c = G.b * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const double factor = source_factor(V, half);
     if (c >= V.ncells) return;
     const bool lead = __lane_id() == 0;
     double* cellv = V.hs_cell + static_cast<size_t>(kSrcCellCols) * c;
@@ -2624,9 +2647,9 @@ __global__ void __launch_bounds__(256) k_source_solve(View V, double factor, int
 // scale * C_i / hw_i under linearized transport (kept in
 // the scratch array), their applied energy and the largest-energy carrier
 // (:296-311; ties go to the smallest carrier id).
-__global__ void __launch_bounds__(kChunkThreads) k_source_increments(View V) {
+__device__ __forceinline__ void source_increments_body(const View& V, const Grid G) {
     __shared__ double lds[3 * kChunkWaves];
-    const int b = blockIdx.x;
+    const int b = G.b;
     if (b >= V.chunk_start[V.ncells]) return;
     int begin, end;
     const int c = chunk_range(V, b, begin, end);
@@ -2700,9 +2723,9 @@ __global__ void __launch_bounds__(kChunkThreads) k_source_increments(View V) {
 // Per chunk: roundoff correction on the cell's largest-energy carrier so the
 // deposit equals the target exactly (:312-318), validation (:319-326) and the
 // occupation update; records the energy change for the following refresh.
-__global__ void __launch_bounds__(kChunkThreads) k_source_apply(View V, int half) {
+__device__ __forceinline__ void source_apply_body(const View& V, const Grid G, int half) {
     __shared__ double lds[2 * kChunkWaves];
-    const int b = blockIdx.x;
+    const int b = G.b;
     if (b >= V.chunk_start[V.ncells]) return;
     int begin, end;
     const int c = chunk_range(V, b, begin, end);
@@ -2752,6 +2775,41 @@ __global__ void __launch_bounds__(kChunkThreads) k_source_apply(View V, int half
         atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->source_updates),
                   static_cast<unsigned long long>(r[1]));
     }
+}
+
+__global__ void __launch_bounds__(kChunkThreads) k_source_sums(View V, int half) {
+    source_sums_body(V, launch_grid(), half);
+}
+__global__ void __launch_bounds__(256) k_source_solve(View V, int half) { source_solve_body(V, launch_grid(), half); }
+__global__ void __launch_bounds__(kChunkThreads) k_source_increments(View V) {
+    source_increments_body(V, launch_grid());
+}
+__global__ void __launch_bounds__(kChunkThreads) k_source_apply(View V, int half) {
+    source_apply_body(V, launch_grid(), half);
+}
+__global__ void __launch_bounds__(kChunkThreads) kb_source_sums(const View* __restrict__ views,
+                                                                const int* __restrict__ offs, int n, int half) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    source_sums_body(views[i], G, half);
+}
+__global__ void __launch_bounds__(256) kb_source_solve(const View* __restrict__ views, const int* __restrict__ offs,
+                                                       int n, int half) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    source_solve_body(views[i], G, half);
+}
+__global__ void __launch_bounds__(kChunkThreads) kb_source_increments(const View* __restrict__ views,
+                                                                      const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    source_increments_body(views[i], G);
+}
+__global__ void __launch_bounds__(kChunkThreads) kb_source_apply(const View* __restrict__ views,
+                                                                 const int* __restrict__ offs, int n, int half) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    source_apply_body(views[i], G, half);
 }
 
 __global__ void k_zero_tally(View V) {
@@ -3048,9 +3106,9 @@ void callaway(const View& v, hipStream_t s) {
     default: hipLaunchKernelGGL(k_callaway<kCwThreads>, dim3(v.ncells), dim3(kCwThreads), 0, s, v); break;
     }
 }
-void source(const View& v, hipStream_t s, double factor, int half) {
-    hipLaunchKernelGGL(k_source_sums, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v, factor, half);
-    hipLaunchKernelGGL(k_source_solve, dim3((v.ncells + 3) / 4), dim3(256), 0, s, v, factor, half);
+void source(const View& v, hipStream_t s, int half) {
+    hipLaunchKernelGGL(k_source_sums, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v, half);
+    hipLaunchKernelGGL(k_source_solve, dim3((v.ncells + 3) / 4), dim3(256), 0, s, v, half);
     hipLaunchKernelGGL(k_source_increments, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v);
     hipLaunchKernelGGL(k_source_apply, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v, half);
 }
@@ -3081,6 +3139,13 @@ int batch_blocks(const View& v, int kind) {
     case kBatchRelax: return v.max_chunks;
     case kBatchNewton: return (v.ncells + 3) / 4;
     case kBatchCallaway: return v.ncells;
+    case kBatchSourceSums0:
+    case kBatchSourceSums1:
+    case kBatchSourceIncrements:
+    case kBatchSourceApply0:
+    case kBatchSourceApply1: return v.hs_on ? v.max_chunks : 0; // instances without a source get no blocks
+    case kBatchSourceSolve0:
+    case kBatchSourceSolve1: return v.hs_on ? (v.ncells + 3) / 4 : 0;
     default: return 0;
     }
 }
@@ -3111,6 +3176,21 @@ void batch_launch(int kind, bool mesh, const View* views, const int* offs, int n
     case kBatchNewton: hipLaunchKernelGGL(kb_newton, g, dim3(256), 0, s, views, offs, n); break;
     case kBatchRelax: hipLaunchKernelGGL(kb_relax, g, dim3(kChunkThreads), 0, s, views, offs, n); break;
     case kBatchCallaway: hipLaunchKernelGGL(kb_callaway, g, dim3(kCwThreads), 0, s, views, offs, n); break;
+    case kBatchSourceSums0:
+    case kBatchSourceSums1:
+        hipLaunchKernelGGL(kb_source_sums, g, dim3(kChunkThreads), 0, s, views, offs, n, kind == kBatchSourceSums1);
+        break;
+    case kBatchSourceSolve0:
+    case kBatchSourceSolve1:
+        hipLaunchKernelGGL(kb_source_solve, g, dim3(256), 0, s, views, offs, n, kind == kBatchSourceSolve1);
+        break;
+    case kBatchSourceIncrements:
+        hipLaunchKernelGGL(kb_source_increments, g, dim3(kChunkThreads), 0, s, views, offs, n);
+        break;
+    case kBatchSourceApply0:
+    case kBatchSourceApply1:
+        hipLaunchKernelGGL(kb_source_apply, g, dim3(kChunkThreads), 0, s, views, offs, n, kind == kBatchSourceApply1);
+        break;
     default: break;
     }
 }

@@ -19,6 +19,13 @@ enum BatchKind {
     kBatchRelax,
     kBatchCallaway,
     kBatchRefresh2,
+    kBatchSourceSums0,      // heat-source half step before transport
+    kBatchSourceSolve0,
+    kBatchSourceIncrements,
+    kBatchSourceApply0,
+    kBatchSourceSums1,      // heat-source half step after collisions
+    kBatchSourceSolve1,
+    kBatchSourceApply1,
     kBatchKinds
 };
 int batch_blocks(const pmc_dev::View& v, int kind);
@@ -39,7 +46,7 @@ void moments(const pmc_dev::View& v, hipStream_t s);
 void newton(const pmc_dev::View& v, hipStream_t s);
 void relax(const pmc_dev::View& v, hipStream_t s);
 void callaway(const pmc_dev::View& v, hipStream_t s);
-void source(const pmc_dev::View& v, hipStream_t s, double factor, int half);
+void source(const pmc_dev::View& v, hipStream_t s, int half);
 void clear_cells(const pmc_dev::View& v, hipStream_t s);
 void flux(const pmc_dev::View& v, hipStream_t s);
 void containment(const pmc_dev::View& v, hipStream_t s);
