@@ -64,6 +64,24 @@ int pmc_nn_gemm(int32_t m, int32_t n, int32_t k, int32_t batch, float alpha, con
                 int64_t stride_a, int32_t ta, const float* b, int32_t ldb, int64_t stride_b, int32_t tb,
                 float beta, float* c, int32_t ldc, int64_t stride_c);
 
+/* Fixed capacity (floats) of the split-k buffer of pmc_nn_gemm_noalloc: 16 MiB
+ * per GPU.  The launch plan (the one pmc_nn_gemm uses: at most about 512
+ * workgroups of two 64-deep stages) needs at most about 3.2 M floats. */
+#define PMC_NN_GEMM_SCRATCH_FLOATS (4u << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the product of pmc_nn_gemm, same kernels and launch plan,
+ * on device arrays a, b and c, enqueued on `stream` (a hipStream_t; NULL is the
+ * null stream) without waiting.  It calls no allocator.  Split-k partial
+ * products go to a buffer of PMC_NN_GEMM_SCRATCH_FLOATS floats that the
+ * library holds on each GPU until the process ends; a product whose plan needs
+ * more returns PMC_ERR_ARGUMENT.  Calls that overlap on one GPU (other
+ * streams, other threads) race on that buffer. */
+int pmc_nn_gemm_noalloc(int32_t m, int32_t n, int32_t k, int32_t batch, float alpha, const float* a, int32_t lda,
+                        int64_t stride_a, int32_t ta, const float* b, int32_t ldb, int64_t stride_b, int32_t tb,
+                        float beta, float* c, int32_t ldc, int64_t stride_c, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

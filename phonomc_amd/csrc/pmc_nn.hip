@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -295,33 +296,58 @@ struct pmc_nn {
 
 namespace {
 
+// Launch plan of one product.  Split k when the tile grid leaves most of the
+// 256 CUs idle: every stage of a workgroup is one exposed global-memory round
+// trip at these sizes, so aim at about 512 workgroups of at most two stages
+// (128 k) each.  ws_floats: split-k partial products the plan needs.
+struct GemmPlan {
+    int splits = 1, kchunk = 1;
+    size_t ws_floats = 0;
+};
+
+GemmPlan plan_gemm(int m, int n, int k, int batch) {
+    GemmPlan p;
+    const long long tiles = static_cast<long long>((n + kTN - 1) / kTN) * ((m + kTM - 1) / kTM) * batch;
+    p.kchunk = std::max(k, 1);
+    if (tiles < 256 && k > 2 * kTK) {
+        p.splits = static_cast<int>(std::min<long long>((k + 2 * kTK - 1) / (2 * kTK), (512 + tiles - 1) / tiles));
+        if (p.splits > 1) {
+            p.kchunk = ((k + p.splits - 1) / p.splits + kTK - 1) / kTK * kTK;
+            p.splits = (k + p.kchunk - 1) / p.kchunk;
+        }
+    }
+    if (p.splits > 1) p.ws_floats = static_cast<size_t>(p.splits) * batch * m * n;
+    return p;
+}
+
+// Enqueues the product on `stream` with the plan's partial products in ws.
+void launch_gemm(const GemmPlan& p, int m, int n, int k, int batch, float alpha, const float* a, int lda, long long sa,
+                 int ta, const float* b, int ldb, long long sb, int tb, float beta, float* c, int ldc, long long sc,
+                 float* ws, hipStream_t stream) {
+    GemmArgs g{m, n, k, a, lda, sa, ta, b, ldb, sb, tb, c, ldc, sc, alpha, beta, p.splits, p.kchunk, ws};
+    dim3 grid((n + kTN - 1) / kTN, (m + kTM - 1) / kTM, static_cast<unsigned>(batch) * p.splits);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, stream, g);
+    if (p.splits > 1) {
+        const long long total = static_cast<long long>(batch) * m * n;
+        const unsigned nb = static_cast<unsigned>(std::min<long long>((total + 255) / 256, 2048));
+        hipLaunchKernelGGL(k_splitk_reduce, dim3(nb), dim3(256), 0, stream, ws, p.splits, m, n, total, c, ldc, sc,
+                           beta);
+    }
+}
+
 void gemm(pmc_nn* nn, int m, int n, int k, int batch, float alpha, const float* a, int lda, long long sa, int ta,
           const float* b, int ldb, long long sb, int tb, float beta, float* c, int ldc, long long sc) {
     if (m <= 0 || n <= 0 || batch <= 0) return;
-    // Split k when the tile grid leaves most of the 256 CUs idle: every stage of
-    // a workgroup is one exposed global-memory round trip at these sizes, so aim
-    // at about 512 workgroups of at most two stages (128 k) each.
-    const long long tiles = static_cast<long long>((n + kTN - 1) / kTN) * ((m + kTM - 1) / kTM) * batch;
-    int splits = 1, kchunk = std::max(k, 1);
-    if (tiles < 256 && k > 2 * kTK) {
-        splits = static_cast<int>(std::min<long long>((k + 2 * kTK - 1) / (2 * kTK), (512 + tiles - 1) / tiles));
-        if (splits > 1) {
-            kchunk = ((k + splits - 1) / splits + kTK - 1) / kTK * kTK;
-            splits = (k + kchunk - 1) / kchunk;
-        }
-    }
+    const GemmPlan plan = plan_gemm(m, n, k, batch);
     float* ws = nullptr;
-    if (splits > 1) {
-        const size_t need = static_cast<size_t>(splits) * batch * m * n;
-        if (need > nn->ws_cap) {
-            nn->ws = nn->alloc<float>(need);
-            nn->ws_cap = nn->ws ? need : 0;
+    if (plan.splits > 1) {
+        if (plan.ws_floats > nn->ws_cap) {
+            nn->ws = nn->alloc<float>(plan.ws_floats);
+            nn->ws_cap = nn->ws ? plan.ws_floats : 0;
         }
         ws = nn->ws;
         if (!ws) return;
     }
-    GemmArgs g{m, n, k, a, lda, sa, ta, b, ldb, sb, tb, c, ldc, sc, alpha, beta, splits, kchunk, ws};
-    dim3 grid((n + kTN - 1) / kTN, (m + kTM - 1) / kTM, static_cast<unsigned>(batch) * splits);
     nn->gemm_flops += 2.0 * m * n * static_cast<double>(k) * batch;
     ++nn->gemm_launches;
     const bool timed = nn->timing && nn->ev[0] != nullptr;
@@ -329,13 +355,7 @@ void gemm(pmc_nn* nn, int m, int n, int k, int batch, float alpha, const float* 
         if (nn->ev_used == pmc_nn::kTimed) nn->harvest();
         nn->note(hipEventRecord(nn->ev[2 * nn->ev_used], nn->stream));
     }
-    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, nn->stream, g);
-    if (splits > 1) {
-        const long long total = static_cast<long long>(batch) * m * n;
-        const unsigned nb = static_cast<unsigned>(std::min<long long>((total + 255) / 256, 2048));
-        hipLaunchKernelGGL(k_splitk_reduce, dim3(nb), dim3(256), 0, nn->stream, ws, splits, m, n, total, c, ldc, sc,
-                           beta);
-    }
+    launch_gemm(plan, m, n, k, batch, alpha, a, lda, sa, ta, b, ldb, sb, tb, beta, c, ldc, sc, ws, nn->stream);
     if (timed) nn->note(hipEventRecord(nn->ev[2 * nn->ev_used++ + 1], nn->stream));
 }
 
@@ -698,3 +718,47 @@ int pmc_nn_gemm(int32_t m, int32_t n, int32_t k, int32_t batch, float alpha, con
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------- GEMM without allocation
+// Split-k partial products of pmc_nn_gemm_noalloc: one copy per GPU, held for
+// the life of the process.  Every partial a plan reads is written first by
+// k_gemm of the same call, so nothing needs clearing between calls.
+namespace {
+static __device__ __align__(256) float scratch[PMC_NN_GEMM_SCRATCH_FLOATS];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_lock;
+float* scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's scratch, looked up once per device with that
+// device current (the host-side symbol is only a stand-in for it).
+hipError_t device_scratch(float** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> hold(scratch_lock);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = scratch_ptr[dev];
+    return hipSuccess;
+}
+} // namespace
+
+extern "C" int pmc_nn_gemm_noalloc(int32_t m, int32_t n, int32_t k, int32_t batch, float alpha, const float* a,
+                                   int32_t lda, int64_t stride_a, int32_t ta, const float* b, int32_t ldb,
+                                   int64_t stride_b, int32_t tb, float beta, float* c, int32_t ldc, int64_t stride_c,
+                                   void* stream) {
+    if (m <= 0 || n <= 0 || k <= 0 || batch <= 0 || !a || !b || !c) return PMC_ERR_ARGUMENT;
+    const GemmPlan plan = plan_gemm(m, n, k, batch);
+    if (plan.ws_floats > PMC_NN_GEMM_SCRATCH_FLOATS) return PMC_ERR_ARGUMENT; // partials exceed the fixed buffer
+    float* ws = nullptr;
+    if (plan.splits > 1 && device_scratch(&ws) != hipSuccess) return PMC_ERR_DEVICE;
+    launch_gemm(plan, m, n, k, batch, alpha, a, lda, stride_a, ta, b, ldb, stride_b, tb, beta, c, ldc, stride_c, ws,
+                static_cast<hipStream_t>(stream));
+    return hipGetLastError() == hipSuccess ? PMC_OK : PMC_ERR_DEVICE;
+}
