@@ -171,7 +171,10 @@ def copy_probe(mib):
     """One device-to-device copy of `mib` MiB (hipMemcpy blit kernel): the
     calibration for HBM byte counters (FETCH_SIZE / WRITE_SIZE per dispatch)."""
     import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
+    from phonomc_amd import runtime as rt
+    # the HIP runtime the library is linked against (a bare "libamdhip64.so"
+    # can name torch's bundled copy instead)
+    hip = C.CDLL(os.path.join(rt.BUILD, "libpmc_device.so"))
     n = mib << 20
     a, b = C.c_void_p(), C.c_void_p()
     if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):

@@ -223,8 +223,8 @@ typedef struct pmc_layers {
  * (src/solver/BoundaryInteraction.cpp:90-113).  The caller supplies linearized
  * reservoir occupations in pmc_reservoir.occupation, strict-elastic rough tables
  * and pmc_setup.refill_policy = 1 (fixed incoming flux).  All arrays by active
- * index, evaluated at T_bg.  Call after pmc_upload_material and before the
- * population is created. */
+ * index, evaluated at T_bg.  Call after pmc_upload_material and
+ * pmc_upload_geometry and before the population is created. */
 typedef struct pmc_callaway_table {
     const double* capacity;        /* [n_active] mode heat capacity, eV/K */
     const double* wavevector;      /* [n_active*3] Cartesian q, 1/Angstrom */
@@ -373,7 +373,7 @@ const char* pmc_status_string(int32_t code);
  * src/solver/CarrierCollisionOperator.cpp:38-142). */
 typedef struct pmc_launch_config {
     int32_t event_split;       /* 1, 2, 4 or 8 */
-    int32_t callaway_threads;  /* 128, 256 or 512 (a cell holds at most 16 carriers per thread) */
+    int32_t callaway_threads;  /* 256, 384 or 1024 (a cell holds at most 8 carriers per thread) */
 } pmc_launch_config;
 int pmc_get_launch_config(pmc_ctx* ctx, pmc_launch_config* out);
 int pmc_set_launch_config(pmc_ctx* ctx, const pmc_launch_config* config);
@@ -382,7 +382,7 @@ int pmc_set_launch_config(pmc_ctx* ctx, const pmc_launch_config* config);
  * where the fullest cell fits with a 1.5x margin).  The steps are real steps of
  * the simulation; the parameters change no result beyond FP64 summation order.
  * report_ms[k] receives the mean kernel time of candidate k in the order
- * event_split 1, 2, 4, 8, then callaway_threads 128, 256, 512 (0 when not tried). */
+ * event_split 1, 2, 4, 8, then callaway_threads 256, 384, 1024 (0 when not tried). */
 int pmc_autotune(pmc_ctx* ctx, int32_t steps, pmc_launch_config* chosen, double* report_ms, int32_t report_len);
 
 /* Uploads layered materials (after pmc_upload_boundary_tables, before the

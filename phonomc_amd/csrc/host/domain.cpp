@@ -211,6 +211,7 @@ Domain build_domain(const SimConfig& c) {
         }
         if (!hit) {
             build_mesh_domain(d, c);
+            d.surface_key = key;
             std::lock_guard<std::mutex> hold(cache_lock);
             if (surface_cache.size() >= kCacheEntries) surface_cache.erase(surface_cache.begin());
             surface_cache[key] = d;

@@ -25,6 +25,7 @@ struct Facet {
 
 struct Domain {
     bool is_box = true;
+    std::string surface_key;               // identity of a mesh model's surface (file, size, time, merge flag)
     SurfaceMesh mesh;                      // surface of a mesh model (empty for the box)
     Vec3 bmin{}, bmax{};
     double volume = 0;

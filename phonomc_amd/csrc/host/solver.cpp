@@ -415,6 +415,18 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         const SurfaceMesh& sm = d.mesh;
         auto& a = p->mesh;
         const int nf = static_cast<int>(sm.faces.size());
+        static std::mutex mesh_lock;
+        static std::map<std::string, Problem::MeshArrays> mesh_cache; // per surface, reused across problems
+        bool cached = false;
+        {
+            std::lock_guard<std::mutex> hold(mesh_lock);
+            auto it = mesh_cache.find(d.surface_key);
+            if (!d.surface_key.empty() && it != mesh_cache.end()) {
+                a = it->second;
+                cached = true;
+            }
+        }
+        if (!cached) {
         std::vector<std::array<Vec3, 2>> boxes(nf);
         for (const auto& v : sm.vertices) a.vertices.insert(a.vertices.end(), v.begin(), v.end());
         for (int f = 0; f < nf; ++f) {
@@ -442,6 +454,15 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
                              0.5 * (s.bounds[0][2] + s.bounds[1][2])});
         }
         a.simplex_bvh = build_bvh(sboxes, scent, 4);
+        // 4M voxels of 1-byte bounds in quarter-angstrom units (0 to 63.75 A).
+        a.field = build_distance_field(sm, d.bmin, d.bmax, 4L << 20, 0.25);
+        if (!d.surface_key.empty()) {
+            std::lock_guard<std::mutex> hold(mesh_lock);
+            if (mesh_cache.size() >= 8) mesh_cache.erase(mesh_cache.begin());
+            mesh_cache[d.surface_key] = a;
+        }
+        }
+        const auto& star = sm.simplices();
         auto& mv = p->mesh_view;
         mv.n_vertices = static_cast<int>(sm.vertices.size());
         mv.vertices = a.vertices.data();
@@ -462,8 +483,6 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         mv.n_simplex_nodes = static_cast<int>(a.simplex_bvh.nodes.size());
         mv.simplex_nodes = a.simplex_bvh.nodes.data();
         mv.simplex_order = a.simplex_bvh.order.data();
-        // 4M voxels of 1-byte bounds in quarter-angstrom units (0 to 63.75 A).
-        a.field = build_distance_field(sm, d.bmin, d.bmax, 4L << 20, 0.25);
         for (int k = 0; k < 3; ++k) mv.field_dims[k] = a.field.dims[k];
         mv.field_h = a.field.h;
         mv.field_unit = a.field.unit;

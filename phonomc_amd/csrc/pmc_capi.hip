@@ -368,6 +368,7 @@ int pmc_create(const pmc_setup* setup, pmc_ctx** out) {
     v.series_radius = 0.01;
     v.t_bg = setup->background_temperature;
     v.refill_fixed = setup->refill_policy == 1 ? 1 : 0;
+    v.cw_threads = pmc_dev::kCwThreads;
     for (auto& b : ctx->buf) {
         const size_t n = static_cast<size_t>(setup->capacity);
         b.px = ctx->alloc<float>(n);
@@ -667,7 +668,8 @@ int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* t) {
     if (!ctx) return PMC_ERR_ARGUMENT;
     if (!t || !t->capacity || !t->wavevector || !t->normal_rate || !t->resistive_rate)
         return fail(ctx, PMC_ERR_ARGUMENT, "Callaway table arrays missing");
-    if (!ctx->has_material) return fail(ctx, PMC_ERR_ARGUMENT, "material must be uploaded before the Callaway table");
+    if (!ctx->has_material || !ctx->has_geometry)
+        return fail(ctx, PMC_ERR_ARGUMENT, "material and geometry must be uploaded before the Callaway table");
     if (ctx->view.inversion == 1)
         return fail(ctx, PMC_ERR_UNSUPPORTED,
                     "Newton temperature inversion applies to RTA transport; linearized Callaway uses "
@@ -710,6 +712,9 @@ int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* t) {
     v.cw_gn = ctx->upload(t->normal_rate, na);
     v.cw_gr = ctx->upload(t->resistive_rate, na);
     v.cw_pack = reinterpret_cast<const double4*>(ctx->upload(pack.data(), pack.size()));
+    v.cw_dir = ctx->alloc<double>(4 * static_cast<size_t>(v.capacity));
+    v.cw_cell = ctx->alloc<double>(pmc_dev::kCwRec * static_cast<size_t>(v.ncells));
+    if (!v.cw_dir || !v.cw_cell) return fail(ctx, PMC_ERR_DEVICE, "Callaway direction buffer");
     v.cw_cbg = t->heat_capacity_density;
     v.e_bg = t->background_energy_density;
     v.callaway = 1;
@@ -1102,7 +1107,7 @@ extern "C" {
 int pmc_get_launch_config(pmc_ctx* ctx, pmc_launch_config* out) {
     if (!ctx || !out) return PMC_ERR_ARGUMENT;
     out->event_split = ctx->view.event_split;
-    out->callaway_threads = ctx->view.callaway ? ctx->view.cw_threads : pmc_dev::kCwThreads;
+    out->callaway_threads = ctx->view.cw_threads;
     return PMC_OK;
 }
 
@@ -1113,8 +1118,8 @@ int pmc_set_launch_config(pmc_ctx* ctx, const pmc_launch_config* c) {
     const int s = c->event_split, t = c->callaway_threads;
     if (!(s == 1 || s == 2 || s == 4 || s == 8) || s > pmc_dev::kMaxEventSplit)
         return fail(ctx, PMC_ERR_ARGUMENT, "event_split must be 1, 2, 4 or 8");
-    if (!(t == 128 || t == 256 || t == pmc_dev::kCwThreads))
-        return fail(ctx, PMC_ERR_ARGUMENT, "callaway_threads must be 128, 256 or 512");
+    if (!(t == 256 || t == 384 || t == pmc_dev::kCwThreads))
+        return fail(ctx, PMC_ERR_ARGUMENT, "callaway_threads must be 256, 384 or 1024");
     ctx->view.event_split = s;
     ctx->view.cw_threads = t;
     return PMC_OK;
@@ -1159,7 +1164,7 @@ int pmc_autotune(pmc_ctx* ctx, int32_t steps, pmc_launch_config* chosen, double*
         ctx->note(hipMemcpy(start.data(), v.cell_start, start.size() * sizeof(int), hipMemcpyDeviceToHost));
         int fullest = 0;
         for (int c = 0; c < v.ncells; ++c) fullest = std::max(fullest, start[c + 1] - start[c]);
-        const int sizes[3] = {128, 256, pmc_dev::kCwThreads};
+        const int sizes[3] = {256, 384, pmc_dev::kCwThreads};
         int best_t = ctx->view.cw_threads;
         best = -1.0;
         for (int k = 0; k < 3 && rc == PMC_OK; ++k) {

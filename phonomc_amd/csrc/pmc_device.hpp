@@ -317,9 +317,11 @@ struct View {
     const double* cw_q;     // Cartesian wavevector, 3 per mode, 1/A
     const double* cw_gn;    // normal rate 1/ps
     const double* cw_gr;    // resistive rate 1/ps
+    double* cw_dir;         // 4 per slot: explicit normal-gain directions of cells that need them
+    double* cw_cell;        // kCwRec per cell: projector, rates, moments of the Callaway setup
     const double4* cw_pack; // 2 per mode: (g_N, g_R, B0, B1), (B2, B3, hw, 0); B = sqrt(C) (1, q / w)
     double cw_cbg;          // heat-capacity density at T_bg, eV/(K A^3)
-    int cw_threads;         // k_callaway workgroup size: 128, 256 or kCwThreads
+    int cw_threads;         // k_callaway workgroup size: 256, 384 or kCwThreads
 };
 
 // Transport blocks for a slot capacity (a multiple of 8, one per XCD round).
@@ -332,9 +334,10 @@ __host__ __device__ inline int wave_blocks_for(int capacity) {
 constexpr int kMaxEventSplit = 8;
 __host__ __device__ inline int event_rows_for(const View& v) { return wave_blocks_for(v.capacity) * v.event_split; }
 
-// Linearized Callaway kernel: one workgroup of kCwThreads per cell, up to
-// kCwPer carriers per thread held in registers.
-constexpr int kCwThreads = 512, kCwWaves = kCwThreads / 64, kCwPer = 16;
+// Linearized Callaway kernel: one workgroup of 256, 384 or kCwThreads per cell,
+// up to kCwPer carriers per thread held in registers.
+constexpr int kCwThreads = 1024, kCwWaves = kCwThreads / 64, kCwPer = 8;
+constexpr int kCwRec = 32; // doubles of the per-cell Callaway record
 
 
 } // namespace pmc_dev

@@ -2148,33 +2148,201 @@ __device__ __forceinline__ double cw_inner(const double (&G)[4][4], const double
     return s;
 }
 
-// T threads per workgroup (one cell), up to kCwPer carriers per thread.
+// Linearized Callaway collisions in two kernels with one workgroup of T threads
+// per cell (up to kCwPer carriers per thread).  k_callaway_setup sums the
+// cell's Gram matrices and moments and builds the normal-gain projector into
+// a per-cell record; k_callaway runs the Taylor steps with the carriers' x in
+// registers, the record read as uniform (scalar) data.
+//
+// Record columns (kCwRec doubles per cell).
+constexpr int kCwP = 0;           // 16: projector in basis-moment space (row-major)
+constexpr int kCwGrInv = 16;      // 1 / sum g_R B0^2 (resistive gain)
+constexpr int kCwRmax = 17, kCwXmax = 18;
+constexpr int kCwMoments = 19;    // 4 normal moments of x (basis or explicit directions)
+constexpr int kCwMomR = 23, kCwXX = 24;
+constexpr int kCwExplicit = 25;   // 1: gain weights from the explicit directions in cw_dir
+constexpr int kCwEvolve = 26;     // 1: the cell evolves (nonzero x, rates and dt)
+
 template <int T>
-__device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
+__device__ __forceinline__ void callaway_setup_body(const View& V, const Grid G) {
     constexpr int kW = T / 64;
     __shared__ double lds[18 * kW + 18];
-    __shared__ double Psh[17]; // P_N row-major and 1/G_R
     const int c = G.b;
     const int begin = V.cell_start[c], end = V.cell_start[c + 1];
     const int n = end - begin;
     const int k0 = V.chunk_start[c], k1 = V.chunk_start[c + 1];
+    double* rec = V.cw_cell + static_cast<size_t>(kCwRec) * c;
     for (int k = k0 + static_cast<int>(threadIdx.x); k < k1; k += T) {
         V.chunk_out[2 * static_cast<size_t>(k)] = 0.0;
         V.chunk_out[2 * static_cast<size_t>(k) + 1] = 0.0;
     }
-    if (n <= 0) return;
-    if (n > T * kCwPer) {
-        if (threadIdx.x == 0) raise_error(V, PMC_ERR_UNSUPPORTED, c, nullptr, -8);
+    if (n <= 0 || n > T * kCwPer) {
+        if (threadIdx.x == 0) {
+            rec[kCwEvolve] = 0.0;
+            if (n > 0) raise_error(V, PMC_ERR_UNSUPPORTED, c, nullptr, -8);
+        }
         return;
     }
     const int t = threadIdx.x;
-    int md[kCwPer];
-    double x[kCwPer], s[kCwPer];
-    // Setup: x = e / sqrt(C), Gram matrices, largest rate and |x|, moments of x.
+    // x = e / sqrt(C); Gram matrices, largest rate and |x|, moments of x.
     double acc[17];
 #pragma unroll
     for (int k = 0; k < 17; ++k) acc[k] = 0.0;
     double rmax = 0.0, xmax = 0.0;
+    for (int j = t; j < n; j += T) {
+        const int m = V.mode[begin + j] & kModeMask;
+        const double4 p0 = V.cw_pack[2 * m], p1 = V.cw_pack[2 * m + 1];
+        const double e = p1.z * (V.occ[begin + j] - V.mtab[m].y);
+        if (!isfinite(e)) raise_error(V, PMC_ERR_INVALID_STATE, begin + j, nullptr, -1);
+        const double xi = e / p0.z;
+        const double b[4] = {p0.z, p0.w, p1.x, p1.y};
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int d = a; d < 4; ++d) acc[q++] += p0.x * b[a] * b[d];
+        acc[10] += p0.y * b[0] * b[0];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) acc[11 + a] += p0.x * b[a] * xi;
+        acc[15] += p0.y * b[0] * xi;
+        acc[16] += xi * xi;
+        rmax = fmax(rmax, p0.x + p0.y);
+        xmax = fmax(xmax, fabs(xi));
+    }
+    block_reduce_wide<17, kW>(acc, lds);
+    block_max2<kW>(rmax, xmax, lds);
+    // Normal-gain directions (CallawayOperator.cpp:23-57): modified Gram-Schmidt,
+    // two passes, of the basis sqrt(g_N) B, each normalized first; a direction
+    // whose length after projection is below 1e-12 is dropped.  Every thread
+    // runs it in coefficient space with the Gram matrix G.  That is exact enough
+    // when every direction keeps a length of at least 1e-3 (squared 1e-6); a
+    // cell with a shorter one (dependent or nearly dependent carrier momenta, as
+    // in cells of a few carriers) redoes it on explicit per-carrier vectors,
+    // like the reference, kept in cw_dir for the Taylor steps.
+    double Gm[4][4], Q[4][4];
+    bool keep[4];
+    bool explicit_dirs = false;
+    {
+        int q = 0;
+        for (int a = 0; a < 4; ++a)
+            for (int d = a; d < 4; ++d) {
+                Gm[a][d] = acc[q];
+                Gm[d][a] = acc[q];
+                ++q;
+            }
+    }
+    for (int k = 0; k < 4; ++k) {
+        keep[k] = false;
+        for (int i = 0; i < 4; ++i) Q[k][i] = 0.0;
+        if (!(Gm[k][k] > 0.0)) continue; // inactive mechanism or absent momentum component
+        double cf[4] = {0.0, 0.0, 0.0, 0.0};
+        cf[k] = 1.0 / sqrt(Gm[k][k]);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int j = 0; j < k; ++j) {
+                if (!keep[j]) continue;
+                const double proj = cw_inner(Gm, Q[j], cf);
+                for (int i = 0; i < 4; ++i) cf[i] -= proj * Q[j][i];
+            }
+        const double len2 = cw_inner(Gm, cf, cf);
+        if (!(len2 >= 1e-6)) {
+            explicit_dirs = true;
+            break;
+        }
+        const double inv = 1.0 / sqrt(len2);
+        for (int i = 0; i < 4; ++i) Q[k][i] = cf[i] * inv;
+        keep[k] = true;
+    }
+    double mom[4] = {acc[11], acc[12], acc[13], acc[14]};
+    int nkeep = 0;
+    if (explicit_dirs) {
+        // Column nkeep of the cell's cw_dir rows is the candidate; kept
+        // directions stay in columns 0 .. nkeep-1, unused columns end at zero.
+        for (int k = 0; k < 4; ++k) {
+            if (!(Gm[k][k] > 0.0)) continue;
+            const double inv0 = 1.0 / sqrt(Gm[k][k]);
+            for (int j = t; j < n; j += T) {
+                const int m = V.mode[begin + j] & kModeMask;
+                const double4 p0 = V.cw_pack[2 * m], p1 = V.cw_pack[2 * m + 1];
+                const double b = k == 0 ? p0.z : k == 1 ? p0.w : k == 2 ? p1.x : p1.y;
+                V.cw_dir[4 * static_cast<size_t>(begin + j) + nkeep] = sqrt(p0.x) * b * inv0;
+            }
+            for (int pass = 0; pass < 2; ++pass)
+                for (int i = 0; i < nkeep; ++i) {
+                    double proj[1] = {0.0};
+                    for (int j = t; j < n; j += T) {
+                        const double* row = V.cw_dir + 4 * static_cast<size_t>(begin + j);
+                        proj[0] += row[i] * row[nkeep];
+                    }
+                    block_reduce_wide<1, kW>(proj, lds);
+                    for (int j = t; j < n; j += T) {
+                        double* row = V.cw_dir + 4 * static_cast<size_t>(begin + j);
+                        row[nkeep] -= proj[0] * row[i];
+                    }
+                }
+            double len2[1] = {0.0};
+            for (int j = t; j < n; j += T) {
+                const double v = V.cw_dir[4 * static_cast<size_t>(begin + j) + nkeep];
+                len2[0] += v * v;
+            }
+            block_reduce_wide<1, kW>(len2, lds);
+            if (!(len2[0] >= 1e-24)) continue; // rank-deficient cell geometry
+            const double inv = 1.0 / sqrt(len2[0]);
+            for (int j = t; j < n; j += T) V.cw_dir[4 * static_cast<size_t>(begin + j) + nkeep] *= inv;
+            ++nkeep;
+        }
+        // Moments of x on the explicit directions.
+        for (int i = 0; i < 4; ++i) mom[i] = 0.0;
+        for (int j = t; j < n; j += T) {
+            double* row = V.cw_dir + 4 * static_cast<size_t>(begin + j);
+            for (int i = nkeep; i < 4; ++i) row[i] = 0.0;
+            const int m = V.mode[begin + j] & kModeMask;
+            const double4 p0 = V.cw_pack[2 * m], p1 = V.cw_pack[2 * m + 1];
+            const double xi = p1.z * (V.occ[begin + j] - V.mtab[m].y) / p0.z;
+            const double sg = sqrt(p0.x);
+            for (int i = 0; i < 4; ++i) mom[i] += sg * row[i] * xi;
+        }
+        block_reduce_wide<4, kW>(mom, lds);
+    }
+    if (t == 0) {
+        for (int a = 0; a < 4; ++a)
+            for (int d = 0; d < 4; ++d) {
+                double v = 0.0;
+                if (explicit_dirs) {
+                    v = (a == d && a < nkeep) ? 1.0 : 0.0;
+                } else {
+                    for (int k = 0; k < 4; ++k)
+                        if (keep[k]) v += Q[k][a] * Q[k][d];
+                }
+                rec[kCwP + 4 * a + d] = v;
+            }
+        rec[kCwGrInv] = acc[10] > 0.0 ? 1.0 / acc[10] : 0.0;
+        rec[kCwRmax] = rmax;
+        rec[kCwXmax] = xmax;
+        for (int a = 0; a < 4; ++a) rec[kCwMoments + a] = mom[a];
+        rec[kCwMomR] = acc[15];
+        rec[kCwXX] = acc[16];
+        rec[kCwExplicit] = explicit_dirs ? 1.0 : 0.0;
+        rec[kCwEvolve] = (xmax > 0.0 && V.dt > 0.0 && rmax > 0.0) ? 1.0 : 0.0;
+    }
+}
+
+template <int T>
+__device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
+    constexpr int kW = T / 64;
+    __shared__ double lds[18 * kW + 18];
+    const int c = G.b;
+    const int begin = V.cell_start[c], end = V.cell_start[c + 1];
+    const int n = end - begin;
+    if (n <= 0 || n > T * kCwPer) return;
+    const int k0 = V.chunk_start[c];
+    const double* rec = V.cw_cell + static_cast<size_t>(kCwRec) * c;
+    const int t = threadIdx.x;
+    const bool evolve = rec[kCwEvolve] != 0.0;
+    const bool explicit_dirs = rec[kCwExplicit] != 0.0;
+    int md[kCwPer];
+    double x[kCwPer], s[kCwPer];
+    const double xmax = rec[kCwXmax];
+    const double inv = evolve ? 1.0 / xmax : 0.0;
 #pragma unroll
     for (int r = 0; r < kCwPer; ++r) {
         const int j = t + r * T;
@@ -2183,89 +2351,11 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
         if (j < n) {
             const int m = V.mode[begin + j] & kModeMask;
             md[r] = m;
-            const double4 p0 = V.cw_pack[2 * m], p1 = V.cw_pack[2 * m + 1];
-            const double occ = V.occ[begin + j];
-            const double e = p1.z * (occ - V.mtab[m].y);
-            if (!isfinite(e)) raise_error(V, PMC_ERR_INVALID_STATE, begin + j, nullptr, -1);
-            const double xi = e / p0.z;
-            x[r] = xi;
-            const double b[4] = {p0.z, p0.w, p1.x, p1.y};
-            int q = 0;
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int d = a; d < 4; ++d) acc[q++] += p0.x * b[a] * b[d];
-            acc[10] += p0.y * b[0] * b[0];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) acc[11 + a] += p0.x * b[a] * xi;
-            acc[15] += p0.y * b[0] * xi;
-            acc[16] += xi * xi;
-            rmax = fmax(rmax, p0.x + p0.y);
-            xmax = fmax(xmax, fabs(xi));
+            x[r] = V.cw_pack[2 * m + 1].z * (V.occ[begin + j] - V.mtab[m].y) / V.cw_pack[2 * m].z * inv;
         }
     }
-    block_reduce_wide<17, kW>(acc, lds);
-    block_max2<kW>(rmax, xmax, lds);
-    {
-        // Normal-gain directions (CallawayOperator.cpp:23-57): modified
-        // Gram-Schmidt, two passes, in coefficient space of the basis
-        // sqrt(g_N) B with the Gram matrix G; every thread holds G and runs it.
-        // Projections use G; the length that decides whether a direction is
-        // dependent (the reference drops lengths below 1e-12 of the normalized
-        // vector) is summed explicitly over the cell's carriers, since in G it
-        // would be a difference of O(1) terms resolved only to about 1e-8.
-        double Gm[4][4], Q[4][4];
-        bool keep[4];
-        int q = 0;
-        for (int a = 0; a < 4; ++a)
-            for (int d = a; d < 4; ++d) {
-                Gm[a][d] = acc[q];
-                Gm[d][a] = acc[q];
-                ++q;
-            }
-        for (int k = 0; k < 4; ++k) {
-            keep[k] = false;
-            for (int i = 0; i < 4; ++i) Q[k][i] = 0.0;
-            if (!(Gm[k][k] > 0.0)) continue; // inactive mechanism or absent momentum component
-            double cf[4] = {0.0, 0.0, 0.0, 0.0};
-            cf[k] = 1.0 / sqrt(Gm[k][k]);
-            for (int pass = 0; pass < 2; ++pass)
-                for (int j = 0; j < k; ++j) {
-                    if (!keep[j]) continue;
-                    const double proj = cw_inner(Gm, Q[j], cf);
-                    for (int i = 0; i < 4; ++i) cf[i] -= proj * Q[j][i];
-                }
-            double len2[1] = {0.0};
-#pragma unroll
-            for (int r = 0; r < kCwPer; ++r) {
-                if (md[r] < 0) continue;
-                const double4 p0 = V.cw_pack[2 * md[r]], p1 = V.cw_pack[2 * md[r] + 1];
-                const double v = cf[0] * p0.z + cf[1] * p0.w + cf[2] * p1.x + cf[3] * p1.y;
-                len2[0] += p0.x * v * v;
-            }
-            block_reduce_wide<1, kW>(len2, lds);
-            if (!(len2[0] >= 1e-24)) continue; // rank-deficient cell geometry
-            const double inv = 1.0 / sqrt(len2[0]);
-            for (int i = 0; i < 4; ++i) Q[k][i] = cf[i] * inv;
-            keep[k] = true;
-        }
-        if (t == 0) {
-            for (int a = 0; a < 4; ++a)
-                for (int d = 0; d < 4; ++d) {
-                    double v = 0.0;
-                    for (int k = 0; k < 4; ++k)
-                        if (keep[k]) v += Q[k][a] * Q[k][d];
-                    Psh[4 * a + d] = v;
-                }
-            Psh[16] = acc[10] > 0.0 ? 1.0 / acc[10] : 0.0;
-        }
-    }
-    __syncthreads();
-    const double gr_inv = Psh[16];
-    const double dt = V.dt;
-    const bool evolve = xmax > 0.0 && dt > 0.0 && rmax > 0.0;
-    double scale = 1.0;
     if (evolve) {
+        const double dt = V.dt, rmax = rec[kCwRmax];
         const double steps_d = fmax(1.0, ceil(dt * rmax / 4.0));
         if (!(steps_d <= 100000.0) || steps_d * n > 200000000.0) {
             if (t == 0) raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -9);
@@ -2273,13 +2363,10 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
         }
         const int steps = static_cast<int>(steps_d);
         const double h = dt / steps, scaled_norm = h * rmax, tol = 2e-14 / steps;
-        scale = xmax;
-        const double inv = 1.0 / xmax;
-        double mN[4], mR = acc[15] * inv, xx = acc[16] * inv * inv;
+        const double gr_inv = rec[kCwGrInv];
+        double mN[4], mR = rec[kCwMomR] * inv, xx = rec[kCwXX] * inv * inv;
 #pragma unroll
-        for (int a = 0; a < 4; ++a) mN[a] = acc[11 + a] * inv;
-#pragma unroll
-        for (int r = 0; r < kCwPer; ++r) x[r] *= inv;
+        for (int a = 0; a < 4; ++a) mN[a] = rec[kCwMoments + a] * inv;
         for (int st = 0; st < steps; ++st) {
             const double reference = sqrt(xx);
 #pragma unroll
@@ -2291,23 +2378,40 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
                 double a4[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    a4[i] = Psh[4 * i] * mN[0] + Psh[4 * i + 1] * mN[1] + Psh[4 * i + 2] * mN[2] + Psh[4 * i + 3] * mN[3];
+                    a4[i] = rec[kCwP + 4 * i] * mN[0] + rec[kCwP + 4 * i + 1] * mN[1] + rec[kCwP + 4 * i + 2] * mN[2] +
+                            rec[kCwP + 4 * i + 3] * mN[3];
                 const double rr = mR * gr_inv, hk = h / k;
                 double red[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int r = 0; r < kCwPer; ++r) {
                     if (md[r] < 0) continue;
-                    const double4 p0 = V.cw_pack[2 * md[r]], p1 = V.cw_pack[2 * md[r] + 1];
-                    const double act = -(p0.x + p0.y) * x[r] +
-                                       p0.x * (p0.z * a4[0] + p0.w * a4[1] + p1.x * a4[2] + p1.y * a4[3]) +
+                    const double4 p0 = V.cw_pack[2 * md[r]];
+                    const double2 p1 = *reinterpret_cast<const double2*>(V.cw_pack + 2 * md[r] + 1);
+                    // Gain weights of the carrier: g_N B (basis moments through P),
+                    // or sqrt(g_N) times its explicit directions.
+                    double w0, w1, w2, w3;
+                    if (explicit_dirs) {
+                        const double sg = sqrt(p0.x);
+                        const double* row = V.cw_dir + 4 * static_cast<size_t>(begin + t + r * T);
+                        w0 = sg * row[0];
+                        w1 = sg * row[1];
+                        w2 = sg * row[2];
+                        w3 = sg * row[3];
+                    } else {
+                        w0 = p0.x * p0.z;
+                        w1 = p0.x * p0.w;
+                        w2 = p0.x * p1.x;
+                        w3 = p0.x * p1.y;
+                    }
+                    const double act = -(p0.x + p0.y) * x[r] + (w0 * a4[0] + w1 * a4[1] + w2 * a4[2] + w3 * a4[3]) +
                                        p0.y * p0.z * rr;
                     const double tn = act * hk;
                     x[r] = tn;
                     s[r] += tn;
-                    red[0] += p0.x * p0.z * tn;
-                    red[1] += p0.x * p0.w * tn;
-                    red[2] += p0.x * p1.x * tn;
-                    red[3] += p0.x * p1.y * tn;
+                    red[0] += w0 * tn;
+                    red[1] += w1 * tn;
+                    red[2] += w2 * tn;
+                    red[3] += w3 * tn;
                     red[4] += p0.y * p0.z * tn;
                     red[5] += tn * tn;
                     red[6] += s[r] * s[r];
@@ -2348,15 +2452,15 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
         if (md[r] < 0) continue;
         const int i = begin + t + r * T;
         const int m = md[r];
-        const double4 p0 = V.cw_pack[2 * m], p1 = V.cw_pack[2 * m + 1];
+        const double b0 = V.cw_pack[2 * m].z, hw = V.cw_pack[2 * m + 1].z;
         const double2 mt = V.mtab[m];
         const double old = V.occ[i];
-        const double evolved = evolve ? x[r] * scale * p0.z : p1.z * (old - mt.y);
-        const double next = mt.y + evolved / p1.z;
+        const double evolved = evolve ? x[r] * xmax * b0 : hw * (old - mt.y);
+        const double next = mt.y + evolved / hw;
         if (!isfinite(next)) raise_error(V, PMC_ERR_INVALID_STATE, i, nullptr, -1);
         V.occ[i] = next;
         out[0] += (kHbar * mt.x * (next - mt.y)) * V.w;
-        out[1] += (p1.z * next - p1.z * old) * V.w;
+        out[1] += (hw * next - hw * old) * V.w;
     }
     block_reduce_wide<2, kW>(out, lds);
     if (t == 0) {
@@ -2366,7 +2470,16 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
 }
 
 template <int T>
+__global__ void __launch_bounds__(T) k_callaway_setup(View V) { callaway_setup_body<T>(V, launch_grid()); }
+template <int T>
 __global__ void __launch_bounds__(T) k_callaway(View V) { callaway_body<T>(V, launch_grid()); }
+
+__global__ void __launch_bounds__(kCwThreads) kb_callaway_setup(const View* __restrict__ views,
+                                                                const int* __restrict__ offs, int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    callaway_setup_body<kCwThreads>(views[i], G);
+}
 
 __global__ void __launch_bounds__(kCwThreads) kb_callaway(const View* __restrict__ views, const int* __restrict__ offs, int n) {
     Grid G;
@@ -3101,9 +3214,18 @@ void relax(const View& v, hipStream_t s) {
 }
 void callaway(const View& v, hipStream_t s) {
     switch (v.cw_threads) {
-    case 128: hipLaunchKernelGGL(k_callaway<128>, dim3(v.ncells), dim3(128), 0, s, v); break;
-    case 256: hipLaunchKernelGGL(k_callaway<256>, dim3(v.ncells), dim3(256), 0, s, v); break;
-    default: hipLaunchKernelGGL(k_callaway<kCwThreads>, dim3(v.ncells), dim3(kCwThreads), 0, s, v); break;
+    case 256:
+        hipLaunchKernelGGL(k_callaway_setup<256>, dim3(v.ncells), dim3(256), 0, s, v);
+        hipLaunchKernelGGL(k_callaway<256>, dim3(v.ncells), dim3(256), 0, s, v);
+        break;
+    case 384:
+        hipLaunchKernelGGL(k_callaway_setup<384>, dim3(v.ncells), dim3(384), 0, s, v);
+        hipLaunchKernelGGL(k_callaway<384>, dim3(v.ncells), dim3(384), 0, s, v);
+        break;
+    default:
+        hipLaunchKernelGGL(k_callaway_setup<kCwThreads>, dim3(v.ncells), dim3(kCwThreads), 0, s, v);
+        hipLaunchKernelGGL(k_callaway<kCwThreads>, dim3(v.ncells), dim3(kCwThreads), 0, s, v);
+        break;
     }
 }
 void source(const View& v, hipStream_t s, int half) {
@@ -3175,7 +3297,10 @@ void batch_launch(int kind, bool mesh, const View* views, const int* offs, int n
     case kBatchMoments: hipLaunchKernelGGL(kb_moments, g, dim3(kChunkThreads), 0, s, views, offs, n); break;
     case kBatchNewton: hipLaunchKernelGGL(kb_newton, g, dim3(256), 0, s, views, offs, n); break;
     case kBatchRelax: hipLaunchKernelGGL(kb_relax, g, dim3(kChunkThreads), 0, s, views, offs, n); break;
-    case kBatchCallaway: hipLaunchKernelGGL(kb_callaway, g, dim3(kCwThreads), 0, s, views, offs, n); break;
+    case kBatchCallaway:
+        hipLaunchKernelGGL(kb_callaway_setup, g, dim3(kCwThreads), 0, s, views, offs, n);
+        hipLaunchKernelGGL(kb_callaway, g, dim3(kCwThreads), 0, s, views, offs, n);
+        break;
     case kBatchSourceSums0:
     case kBatchSourceSums1:
         hipLaunchKernelGGL(kb_source_sums, g, dim3(kChunkThreads), 0, s, views, offs, n, kind == kBatchSourceSums1);

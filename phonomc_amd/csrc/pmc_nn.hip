@@ -2,6 +2,7 @@
 // a BF16 matrix-core GEMM with FP32 accumulation, elementwise kernels, and the
 // forward / backward / Adam passes composed from them on one HIP stream.
 #include "pmc_surrogate.h"
+#include "pmc_fno.h"
 #include "pmc_capi.h"
 
 #include <hip/hip_runtime.h>
@@ -16,7 +17,7 @@
 
 namespace pmc_nn_dev {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // FP32 -> BF16, round to nearest even (inputs are finite).
@@ -45,13 +46,15 @@ struct GemmArgs {
     float* ws;   // splits > 1: partial products [batch][splits][m][n]
 };
 
-constexpr int kTM = 64, kTN = 64, kTK = 64, kPad = 4;
+constexpr int kTM = 64, kTN = 64, kTK = 64, kPad = 8;
 
 // C = alpha op(A) op(B) + beta C for one 64x64 tile per workgroup.  Four
-// waves each own a 32x32 quadrant as 2x2 MFMA 16x16x16 tiles.  Operands are
-// staged through LDS as BF16 with k contiguous, 64 k per stage (16 loads in
-// flight per operand and thread):
-//   A fragment of lane l: rows l%16, k = 4(l/16)..+3;  B: columns l%16, same k;
+// waves each own a 32x32 quadrant as 2x2 MFMA tiles of gfx950's
+// v_mfma_f32_16x16x32_bf16.  Operands are staged through LDS as BF16 with k
+// contiguous (rows padded to 72 elements, so each lane's 8-element fragment is
+// one aligned 16-byte read), 64 k per stage (16 loads in flight per operand and
+// thread):
+//   A fragment of lane l: row l%16, k = 8(l/16)..+7;  B: column l%16, same k;
 //   result lane l holds rows 4(l/16)..+3 of column l%16.
 // With splits > 1 each workgroup sums one k range and writes alpha * partial
 // to the workspace; k_splitk_reduce adds the splits in order.
@@ -96,19 +99,19 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
             Bs[ni][ki] = to_bf16(v);
         }
         __syncthreads();
-        const int kq = 4 * (lane >> 4), r16 = lane & 15;
+        const int kq = 8 * (lane >> 4), r16 = lane & 15;
 #pragma unroll
-        for (int ks = 0; ks < kTK; ks += 16) {
-            bf16x4 af[2], bfr[2];
+        for (int ks = 0; ks < kTK; ks += 32) {
+            bf16x8 af[2], bfr[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x4*>(&As[wm + 16 * i + r16][ks + kq]);
+            for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x8*>(&As[wm + 16 * i + r16][ks + kq]);
 #pragma unroll
-            for (int j = 0; j < 2; ++j) bfr[j] = *reinterpret_cast<const bf16x4*>(&Bs[wn + 16 * j + r16][ks + kq]);
+            for (int j = 0; j < 2; ++j) bfr[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + 16 * j + r16][ks + kq]);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[i], bfr[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
@@ -762,3 +765,530 @@ extern "C" int pmc_nn_gemm_noalloc(int32_t m, int32_t n, int32_t k, int32_t batc
                 static_cast<hipStream_t>(stream));
     return hipGetLastError() == hipSuccess ? PMC_OK : PMC_ERR_DEVICE;
 }
+
+// ---------------------------------------------------------------- 3-D spectral operator
+// include/pmc_fno.h.  Every product goes through the same BF16 MFMA GEMM; the
+// separable truncated Fourier transform is three axis contractions per
+// direction (cos/sin matrices built on the host), the per-mode complex mixing
+// one GEMM batched over modes.  Intermediate layouts (channels last, pair index
+// p/q/r/s = real/imaginary part):
+//   h [B][X][Y][Z][W] -> T1 [B][X][Y][P][Kz][W] -> T2 [B][X][Q][Ky][Kz][W]
+//   -> F [B][R][Kx][Ky][Kz][W] = [B][R][M][W] -> Fm [M][B][R][W] (mixing)
+//   -> G [B][S][M][W] -> U2 [B][X][Q][Ky][Kz][W] -> U1 [B][X][Y][P][Kz][W] -> [B][X][Y][Z][W].
+// Each transform matrix orders its rows so that the pair index lands next to
+// the axis the following contraction runs over.
+namespace pmc_nn_dev {
+
+// src [b][2][M][W] <-> dst [M][b][2][W] (dir 0 forward, 1 back).
+__global__ void k_fno_perm(const float* __restrict__ src, float* __restrict__ dst, int b, int M, int W, int dir) {
+    const long long n = static_cast<long long>(b) * 2 * M * W;
+    for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * blockDim.x) {
+        // i indexes the [b][2][M][W] layout.
+        const int c = static_cast<int>(i % W);
+        const long long rest = i / W;
+        const int m = static_cast<int>(rest % M);
+        const long long br = rest / M;
+        const int r = static_cast<int>(br % 2);
+        const int bb = static_cast<int>(br / 2);
+        const long long j = ((static_cast<long long>(m) * b + bb) * 2 + r) * W + c;
+        if (dir == 0) dst[j] = src[i];
+        else dst[i] = src[j];
+    }
+}
+
+// Complex weights R [M][2][W][W] -> real block maps B [M][2W][2W] acting on
+// (re, im) rows: [[Wr, Wi], [-Wi, Wr]].
+__global__ void k_fno_expand(const float* __restrict__ R, float* __restrict__ B, int M, int W) {
+    const long long n = static_cast<long long>(M) * 4 * W * W;
+    const int W2 = 2 * W;
+    for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const int col = static_cast<int>(i % W2);
+        const int row = static_cast<int>((i / W2) % W2);
+        const long long m = i / (static_cast<long long>(W2) * W2);
+        const int r = row / W, c = row % W, s = col / W, cp = col % W;
+        const float* Rm = R + m * 2LL * W * W;
+        const float wr = Rm[c * W + cp], wi = Rm[W * W + c * W + cp];
+        B[i] = (r == s) ? wr : (r == 0 ? wi : -wi);
+    }
+}
+
+// Gradient of the block maps folded back onto the complex weights.
+__global__ void k_fno_fold(const float* __restrict__ gB, float* __restrict__ gR, int M, int W) {
+    const long long n = static_cast<long long>(M) * W * W;
+    const int W2 = 2 * W;
+    for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const int cp = static_cast<int>(i % W), c = static_cast<int>((i / W) % W);
+        const long long m = i / (static_cast<long long>(W) * W);
+        const float* g = gB + m * W2 * W2;
+        const float g00 = g[c * W2 + cp], g11 = g[(W + c) * W2 + W + cp];
+        const float g01 = g[c * W2 + W + cp], g10 = g[(W + c) * W2 + cp];
+        gR[m * 2LL * W * W + c * W + cp] = g00 + g11;
+        gR[m * 2LL * W * W + W * W + c * W + cp] = g01 - g10;
+    }
+}
+
+// Weighted squared error: dy = 2 w (y - t) / wsum; sse += w (y - t)^2.
+__global__ void k_wmse(const float* y, const float* t, const float* w, float* dy, long long n, float wsum, float* sse) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    float e = 0.f;
+    if (i < n) {
+        const float r = y[i] - t[i];
+        dy[i] = 2.f * w[i] * r / wsum;
+        e = w[i] * r * r;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(sse, e);
+}
+
+// v[0 .. n) = 0 (a fill kernel rather than a memset node).
+__global__ void k_zero(float* v, long long n) {
+    for (long long i = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * blockDim.x)
+        v[i] = 0.f;
+}
+
+// Sum of a device array into out[0] (one block).
+__global__ void k_sum(const float* v, long long n, float* out) {
+    __shared__ float part[256];
+    float s = 0.f;
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) s += v[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if (static_cast<int>(threadIdx.x) < k) part[threadIdx.x] += part[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = part[0];
+}
+
+} // namespace pmc_nn_dev
+
+struct pmc_fno {
+    pmc_nn eng; // stream, allocations, GEMM workspace and accounting
+    pmc_fno_shape sh{};
+    int X = 0, Y = 0, Z = 0, S = 0, W = 0, H = 0, L = 0, Cin = 0, m = 0, M = 0;
+    int64_t nparam = 0;
+    int64_t oP = 0, obp = 0, oQ1 = 0, ob1 = 0, oQ2 = 0, ob2 = 0;
+    std::vector<int64_t> oR, oW, ob;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    float *Dz = nullptr, *Dy = nullptr, *Dx = nullptr, *Ez = nullptr, *Ey = nullptr, *Ex = nullptr;
+    float *x = nullptr, *y = nullptr, *t = nullptr, *w = nullptr, *dy = nullptr, *sse = nullptr, *wsum = nullptr;
+    std::vector<float*> h, pre, Fm, Bl;
+    float *T1 = nullptr, *T2 = nullptr, *F = nullptr, *Gm = nullptr, *G = nullptr, *dB = nullptr;
+    float *preq = nullptr, *aq = nullptr, *daq = nullptr, *dh = nullptr, *dh2 = nullptr, *dpre = nullptr;
+    float *ds_in = nullptr, *ds_t = nullptr, *ds_w = nullptr;
+    const float* ds_src[3] = {nullptr, nullptr, nullptr};
+    int* idx = nullptr;
+    int ds_n = 0;
+    int64_t adam_t = 0;
+};
+
+namespace {
+using pmc_nn_dev::k_fno_perm;
+
+unsigned grid_for(long long n) { return static_cast<unsigned>(std::min<long long>((n + 255) / 256, 8192)); }
+
+// K(hin) into out ([b][S][W]); keeps the mode coefficients of the layer in Fm.
+void fno_spectral(pmc_fno* f, int b, const float* hin, float* out, float* Fm, const float* Bl) {
+    pmc_nn* E = &f->eng;
+    const int X = f->X, Y = f->Y, Z = f->Z, W = f->W, m = f->m, M = f->M;
+    const int N = 2 * m * m * W; // (ky, kz, c) row length of T2 / F / G / U2 per (x or kx, pair)
+    const long long bxy = static_cast<long long>(b) * X * Y;
+    gemm(E, 2 * m, W, Z, static_cast<int>(bxy), 1.f, f->Dz, Z, 0, 0, hin, W, static_cast<long long>(Z) * W, 0, 0.f,
+         f->T1, W, 2LL * m * W);
+    gemm(E, 4 * m, m * W, 2 * Y, b * X, 1.f, f->Dy, 2 * Y, 0, 0, f->T1, m * W, 2LL * Y * m * W, 0, 0.f, f->T2, m * W,
+         4LL * m * m * W);
+    gemm(E, 4 * m, N, 2 * X, b, 1.f, f->Dx, 2 * X, 0, 0, f->T2, N, 2LL * X * N, 0, 0.f, f->F, N, 4LL * m * N);
+    hipLaunchKernelGGL(k_fno_perm, dim3(grid_for(2LL * b * M * W)), dim3(256), 0, E->stream, f->F, Fm, b, M, W, 0);
+    gemm(E, b, 2 * W, 2 * W, M, 1.f, Fm, 2 * W, 2LL * b * W, 0, Bl, 2 * W, 4LL * W * W, 0, 0.f, f->Gm, 2 * W,
+         2LL * b * W);
+    hipLaunchKernelGGL(k_fno_perm, dim3(grid_for(2LL * b * M * W)), dim3(256), 0, E->stream, f->Gm, f->G, b, M, W, 1);
+    gemm(E, 2 * X, N, 4 * m, b, 1.f, f->Ex, 4 * m, 0, 0, f->G, N, 4LL * m * N, 0, 0.f, f->T2, N, 2LL * X * N);
+    gemm(E, 2 * Y, m * W, 4 * m, b * X, 1.f, f->Ey, 4 * m, 0, 0, f->T2, m * W, 4LL * m * m * W, 0, 0.f, f->T1, m * W,
+         2LL * Y * m * W);
+    gemm(E, Z, W, 2 * m, static_cast<int>(bxy), 1.f, f->Ez, 2 * m, 0, 0, f->T1, W, 2LL * m * W, 0, 0.f, out, W,
+         static_cast<long long>(Z) * W);
+}
+
+// Adjoint of fno_spectral for the output gradient dout: dh += K^T dout and the
+// gradient of the layer's complex weights into gR (Fm is overwritten).
+void fno_spectral_back(pmc_fno* f, int b, const float* dout, float* Fm, const float* Bl, float* gR, float* dh) {
+    pmc_nn* E = &f->eng;
+    const int X = f->X, Y = f->Y, Z = f->Z, W = f->W, m = f->m, M = f->M;
+    const int N = 2 * m * m * W;
+    const long long bxy = static_cast<long long>(b) * X * Y;
+    gemm(E, 2 * m, W, Z, static_cast<int>(bxy), 1.f, f->Ez, 2 * m, 0, 1, dout, W, static_cast<long long>(Z) * W, 0, 0.f,
+         f->T1, W, 2LL * m * W);
+    gemm(E, 4 * m, m * W, 2 * Y, b * X, 1.f, f->Ey, 4 * m, 0, 1, f->T1, m * W, 2LL * Y * m * W, 0, 0.f, f->T2, m * W,
+         4LL * m * m * W);
+    gemm(E, 4 * m, N, 2 * X, b, 1.f, f->Ex, 4 * m, 0, 1, f->T2, N, 2LL * X * N, 0, 0.f, f->G, N, 4LL * m * N);
+    hipLaunchKernelGGL(k_fno_perm, dim3(grid_for(2LL * b * M * W)), dim3(256), 0, E->stream, f->G, f->Gm, b, M, W, 0);
+    gemm(E, 2 * W, 2 * W, b, M, 1.f, Fm, 2 * W, 2LL * b * W, 1, f->Gm, 2 * W, 2LL * b * W, 0, 0.f, f->dB, 2 * W,
+         4LL * W * W);
+    hipLaunchKernelGGL(pmc_nn_dev::k_fno_fold, dim3(grid_for(static_cast<long long>(M) * W * W)), dim3(256), 0,
+                       E->stream, f->dB, gR, M, W);
+    gemm(E, b, 2 * W, 2 * W, M, 1.f, f->Gm, 2 * W, 2LL * b * W, 0, Bl, 2 * W, 4LL * W * W, 1, 0.f, Fm, 2 * W,
+         2LL * b * W);
+    hipLaunchKernelGGL(k_fno_perm, dim3(grid_for(2LL * b * M * W)), dim3(256), 0, E->stream, Fm, f->F, b, M, W, 1);
+    gemm(E, 2 * X, N, 4 * m, b, 1.f, f->Dx, 2 * X, 0, 1, f->F, N, 4LL * m * N, 0, 0.f, f->T2, N, 2LL * X * N);
+    gemm(E, 2 * Y, m * W, 4 * m, b * X, 1.f, f->Dy, 2 * Y, 0, 1, f->T2, m * W, 4LL * m * m * W, 0, 0.f, f->T1, m * W,
+         2LL * Y * m * W);
+    gemm(E, Z, W, 2 * m, static_cast<int>(bxy), 1.f, f->Dz, Z, 0, 1, f->T1, W, 2LL * m * W, 0, 1.f, dh, W,
+         static_cast<long long>(Z) * W);
+}
+
+void fno_forward(pmc_fno* f, int b) {
+    pmc_nn* E = &f->eng;
+    const int S = f->S, W = f->W, H = f->H, L = f->L;
+    const long long R = static_cast<long long>(b) * S;
+    float* P = f->params;
+    gemm(E, static_cast<int>(R), W, f->Cin, 1, 1.f, f->x, f->Cin, 0, 0, P + f->oP, W, 0, 0, 0.f, f->h[0], W, 0);
+    hipLaunchKernelGGL(k_bias_act, dim3(blocks(R * W)), dim3(256), 0, E->stream, f->h[0], P + f->obp, f->h[0], R, W, 0);
+    for (int l = 0; l < L; ++l) {
+        hipLaunchKernelGGL(pmc_nn_dev::k_fno_expand, dim3(grid_for(4LL * f->M * W * W)), dim3(256), 0, E->stream,
+                           P + f->oR[l], f->Bl[l], f->M, W);
+        fno_spectral(f, b, f->h[l], f->pre[l], f->Fm[l], f->Bl[l]);
+        gemm(E, static_cast<int>(R), W, W, 1, 1.f, f->h[l], W, 0, 0, P + f->oW[l], W, 0, 0, 1.f, f->pre[l], W, 0);
+        hipLaunchKernelGGL(k_bias_act, dim3(blocks(R * W)), dim3(256), 0, E->stream, f->pre[l], P + f->ob[l],
+                           f->h[l + 1], R, W, l < L - 1 ? 1 : 0);
+    }
+    gemm(E, static_cast<int>(R), H, W, 1, 1.f, f->h[L], W, 0, 0, P + f->oQ1, H, 0, 0, 0.f, f->preq, H, 0);
+    hipLaunchKernelGGL(k_bias_act, dim3(blocks(R * H)), dim3(256), 0, E->stream, f->preq, P + f->ob1, f->aq, R, H, 1);
+    gemm(E, static_cast<int>(R), 1, H, 1, 1.f, f->aq, H, 0, 0, P + f->oQ2, 1, 0, 0, 0.f, f->y, 1, 0);
+    hipLaunchKernelGGL(k_bias_act, dim3(blocks(R)), dim3(256), 0, E->stream, f->y, P + f->ob2, f->y, R, 1, 0);
+}
+
+// Loss and gradient of `b` samples (x, t, w on the device): grads overwritten.
+void fno_backward(pmc_fno* f, int b, float wsum) {
+    pmc_nn* E = &f->eng;
+    const int S = f->S, W = f->W, H = f->H, L = f->L;
+    const long long R = static_cast<long long>(b) * S;
+    float* P = f->params;
+    float* Gd = f->grads;
+    hipLaunchKernelGGL(k_zero, dim3(grid_for(f->nparam)), dim3(256), 0, E->stream, Gd, static_cast<long long>(f->nparam));
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, E->stream, f->sse, 1LL);
+    hipLaunchKernelGGL(pmc_nn_dev::k_wmse, dim3(blocks(R)), dim3(256), 0, E->stream, f->y, f->t, f->w, f->dy, R, wsum,
+                       f->sse);
+    gemm(E, H, 1, static_cast<int>(R), 1, 1.f, f->aq, H, 0, 1, f->dy, 1, 0, 0, 0.f, Gd + f->oQ2, 1, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, E->stream, f->dy, R, 1, Gd + f->ob2);
+    gemm(E, static_cast<int>(R), H, 1, 1, 1.f, f->dy, 1, 0, 0, P + f->oQ2, 1, 0, 1, 0.f, f->daq, H, 0);
+    hipLaunchKernelGGL(k_gelu_back, dim3(blocks(R * H)), dim3(256), 0, E->stream, f->daq, f->preq, R * H);
+    gemm(E, W, H, static_cast<int>(R), 1, 1.f, f->h[L], W, 0, 1, f->daq, H, 0, 0, 0.f, Gd + f->oQ1, H, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, E->stream, f->daq, R, H, Gd + f->ob1);
+    gemm(E, static_cast<int>(R), W, H, 1, 1.f, f->daq, H, 0, 0, P + f->oQ1, H, 0, 1, 0.f, f->dh, W, 0);
+    for (int l = L - 1; l >= 0; --l) {
+        float* dpre = f->dpre;
+        E->note(hipMemcpyAsync(dpre, f->dh, R * W * sizeof(float), hipMemcpyDeviceToDevice, E->stream));
+        if (l < L - 1)
+            hipLaunchKernelGGL(k_gelu_back, dim3(blocks(R * W)), dim3(256), 0, E->stream, dpre, f->pre[l], R * W);
+        hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, E->stream, dpre, R, W, Gd + f->ob[l]);
+        gemm(E, W, W, static_cast<int>(R), 1, 1.f, f->h[l], W, 0, 1, dpre, W, 0, 0, 0.f, Gd + f->oW[l], W, 0);
+        gemm(E, static_cast<int>(R), W, W, 1, 1.f, dpre, W, 0, 0, P + f->oW[l], W, 0, 1, 0.f, f->dh2, W, 0);
+        fno_spectral_back(f, b, dpre, f->Fm[l], f->Bl[l], Gd + f->oR[l], f->dh2);
+        std::swap(f->dh, f->dh2);
+    }
+    gemm(E, f->Cin, W, static_cast<int>(R), 1, 1.f, f->x, f->Cin, 0, 1, f->dh, W, 0, 0, 0.f, Gd + f->oP, W, 0);
+    hipLaunchKernelGGL(k_colsum, dim3(64), dim3(256), 0, E->stream, f->dh, R, W, Gd + f->obp);
+}
+
+int fno_finish(pmc_fno* f) { return finish(&f->eng, "fno"); }
+
+// Transform matrices (row-major float, 1/sqrt(n) per axis both ways).
+void fno_matrices(int n, int m, std::vector<float>& fwd_pair, std::vector<float>& inv_pair) {
+    // Complex axis of length n with kept frequencies k_i = i (i < m) or i - 2m:
+    // forward [(q, k) = 2*2m][(y, p) = 2n], inverse [(y, p)][(q, k)].
+    const double two_pi = 6.283185307179586476925286766559, sc = 1.0 / std::sqrt(static_cast<double>(n));
+    const int K = 2 * m;
+    fwd_pair.assign(static_cast<size_t>(2 * K) * 2 * n, 0.f);
+    inv_pair.assign(static_cast<size_t>(2 * n) * 2 * K, 0.f);
+    for (int ki = 0; ki < K; ++ki) {
+        const int k = ki < m ? ki : ki - K;
+        for (int y = 0; y < n; ++y) {
+            const double ph = two_pi * static_cast<double>(k) * y / n;
+            const float c = static_cast<float>(std::cos(ph) * sc), s = static_cast<float>(std::sin(ph) * sc);
+            auto F = [&](int q, int p) -> float& { return fwd_pair[static_cast<size_t>(q * K + ki) * 2 * n + 2 * y + p]; };
+            auto I = [&](int p, int q) -> float& { return inv_pair[static_cast<size_t>(2 * y + p) * 2 * K + q * K + ki]; };
+            // (a + ib) e^{-i ph}: re = a c + b s, im = b c - a s
+            F(0, 0) = c;
+            F(0, 1) = s;
+            F(1, 0) = -s;
+            F(1, 1) = c;
+            // (a + ib) e^{+i ph}: re = a c - b s, im = a s + b c
+            I(0, 0) = c;
+            I(0, 1) = -s;
+            I(1, 0) = s;
+            I(1, 1) = c;
+        }
+    }
+}
+} // namespace
+
+extern "C" {
+
+int pmc_fno_create(const pmc_fno_shape* sh, pmc_fno** out) {
+    if (!sh || !out || sh->nx <= 0 || sh->ny <= 0 || sh->nz <= 0 || sh->in_channels <= 0 || sh->width <= 0 ||
+        sh->width > 128 || sh->layers <= 0 || sh->modes <= 0 || 2 * sh->modes > sh->nx || 2 * sh->modes > sh->ny ||
+        sh->modes > sh->nz / 2 || sh->hidden <= 0 || sh->hidden > 256 || sh->max_batch <= 0)
+        return PMC_ERR_ARGUMENT;
+    auto* f = new pmc_fno();
+    f->sh = *sh;
+    f->X = sh->nx;
+    f->Y = sh->ny;
+    f->Z = sh->nz;
+    f->S = sh->nx * sh->ny * sh->nz;
+    f->W = sh->width;
+    f->H = sh->hidden;
+    f->L = sh->layers;
+    f->Cin = sh->in_channels;
+    f->m = sh->modes;
+    f->M = 4 * f->m * f->m * f->m;
+    pmc_nn* E = &f->eng;
+    if (hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete f;
+        return PMC_ERR_DEVICE;
+    }
+    const int W = f->W, H = f->H, M = f->M, m = f->m, X = f->X, Y = f->Y, Z = f->Z;
+    int64_t o = 0;
+    f->oP = o; o += static_cast<int64_t>(f->Cin) * W;
+    f->obp = o; o += W;
+    for (int l = 0; l < f->L; ++l) {
+        f->oR.push_back(o); o += static_cast<int64_t>(M) * 2 * W * W;
+        f->oW.push_back(o); o += static_cast<int64_t>(W) * W;
+        f->ob.push_back(o); o += W;
+    }
+    f->oQ1 = o; o += static_cast<int64_t>(W) * H;
+    f->ob1 = o; o += H;
+    f->oQ2 = o; o += H;
+    f->ob2 = o; o += 1;
+    f->nparam = o;
+    // Initialization: uniform(-a, a) with a = sqrt(3 / fan_in) (spectral
+    // weights: fan-in of the 2W real inputs of a mode); zero biases.
+    std::vector<float> p(o, 0.f);
+    uint64_t seed = sh->seed;
+    auto fill = [&](int64_t off, int64_t n, double fan_in) {
+        const double a = std::sqrt(3.0 / fan_in);
+        for (int64_t i = 0; i < n; ++i) p[off + i] = static_cast<float>((2.0 * uniform(seed) - 1.0) * a);
+    };
+    fill(f->oP, static_cast<int64_t>(f->Cin) * W, f->Cin);
+    for (int l = 0; l < f->L; ++l) {
+        fill(f->oR[l], static_cast<int64_t>(M) * 2 * W * W, 2.0 * W);
+        fill(f->oW[l], static_cast<int64_t>(W) * W, W);
+    }
+    fill(f->oQ1, static_cast<int64_t>(W) * H, W);
+    fill(f->oQ2, H, H);
+    // Transform matrices.  z is the real axis: forward [(p, kz)][z], inverse
+    // [z][(p, kz)] with the factor 2 of the omitted negative kz (kz > 0).
+    std::vector<float> dz(static_cast<size_t>(2 * m) * Z), ez(static_cast<size_t>(Z) * 2 * m);
+    {
+        const double two_pi = 6.283185307179586476925286766559, sc = 1.0 / std::sqrt(static_cast<double>(Z));
+        for (int k = 0; k < m; ++k)
+            for (int z = 0; z < Z; ++z) {
+                const double ph = two_pi * k * z / Z;
+                const double c = std::cos(ph) * sc, s = std::sin(ph) * sc, w2 = k == 0 ? 1.0 : 2.0;
+                dz[static_cast<size_t>(k) * Z + z] = static_cast<float>(c);
+                dz[static_cast<size_t>(m + k) * Z + z] = static_cast<float>(-s);
+                ez[static_cast<size_t>(z) * 2 * m + k] = static_cast<float>(w2 * c);
+                ez[static_cast<size_t>(z) * 2 * m + m + k] = static_cast<float>(-w2 * s);
+            }
+    }
+    std::vector<float> dy, ey, dx, ex;
+    fno_matrices(Y, m, dy, ey);
+    fno_matrices(X, m, dx, ex);
+    const size_t B = sh->max_batch;
+    const size_t R = B * f->S;
+    f->params = E->alloc<float>(o);
+    f->grads = E->alloc<float>(o);
+    f->adam_m = E->alloc<float>(o);
+    f->adam_v = E->alloc<float>(o);
+    f->Dz = E->alloc<float>(dz.size());
+    f->Ez = E->alloc<float>(ez.size());
+    f->Dy = E->alloc<float>(dy.size());
+    f->Ey = E->alloc<float>(ey.size());
+    f->Dx = E->alloc<float>(dx.size());
+    f->Ex = E->alloc<float>(ex.size());
+    f->x = E->alloc<float>(R * f->Cin);
+    f->y = E->alloc<float>(R);
+    f->t = E->alloc<float>(R);
+    f->w = E->alloc<float>(R);
+    f->dy = E->alloc<float>(R);
+    f->sse = E->alloc<float>(1);
+    f->wsum = E->alloc<float>(1);
+    for (int l = 0; l <= f->L; ++l) f->h.push_back(E->alloc<float>(R * W));
+    const size_t modes_rows = B * 2 * static_cast<size_t>(M) * W;
+    for (int l = 0; l < f->L; ++l) {
+        f->pre.push_back(E->alloc<float>(R * W));
+        f->Fm.push_back(E->alloc<float>(modes_rows));
+        f->Bl.push_back(E->alloc<float>(static_cast<size_t>(M) * 4 * W * W));
+    }
+    f->T1 = E->alloc<float>(B * X * Y * 2 * static_cast<size_t>(m) * W);
+    f->T2 = E->alloc<float>(B * 2 * X * 2 * static_cast<size_t>(m) * m * W);
+    f->F = E->alloc<float>(modes_rows);
+    f->Gm = E->alloc<float>(modes_rows);
+    f->G = E->alloc<float>(modes_rows);
+    f->dB = E->alloc<float>(static_cast<size_t>(M) * 4 * W * W);
+    f->preq = E->alloc<float>(R * H);
+    f->aq = E->alloc<float>(R * H);
+    f->daq = E->alloc<float>(R * H);
+    f->dh = E->alloc<float>(R * W);
+    f->dh2 = E->alloc<float>(R * W);
+    f->dpre = E->alloc<float>(R * W);
+    f->idx = E->alloc<int>(B);
+    if (E->err != hipSuccess) {
+        pmc_fno_destroy(f);
+        return PMC_ERR_DEVICE;
+    }
+    E->note(hipMemcpy(f->params, p.data(), o * sizeof(float), hipMemcpyHostToDevice));
+    E->note(hipMemset(f->adam_m, 0, o * sizeof(float)));
+    E->note(hipMemset(f->adam_v, 0, o * sizeof(float)));
+    E->note(hipMemcpy(f->Dz, dz.data(), dz.size() * sizeof(float), hipMemcpyHostToDevice));
+    E->note(hipMemcpy(f->Ez, ez.data(), ez.size() * sizeof(float), hipMemcpyHostToDevice));
+    E->note(hipMemcpy(f->Dy, dy.data(), dy.size() * sizeof(float), hipMemcpyHostToDevice));
+    E->note(hipMemcpy(f->Ey, ey.data(), ey.size() * sizeof(float), hipMemcpyHostToDevice));
+    E->note(hipMemcpy(f->Dx, dx.data(), dx.size() * sizeof(float), hipMemcpyHostToDevice));
+    E->note(hipMemcpy(f->Ex, ex.data(), ex.size() * sizeof(float), hipMemcpyHostToDevice));
+    if (E->err != hipSuccess) {
+        pmc_fno_destroy(f);
+        return PMC_ERR_DEVICE;
+    }
+    *out = f;
+    return PMC_OK;
+}
+
+int64_t pmc_fno_param_count(const pmc_fno* f) { return f ? f->nparam : 0; }
+
+int pmc_fno_get_params(pmc_fno* f, float* params) {
+    if (!f || !params) return PMC_ERR_ARGUMENT;
+    f->eng.note(hipMemcpyAsync(params, f->params, f->nparam * sizeof(float), hipMemcpyDeviceToHost, f->eng.stream));
+    return fno_finish(f);
+}
+
+int pmc_fno_set_params(pmc_fno* f, const float* params) {
+    if (!f || !params) return PMC_ERR_ARGUMENT;
+    f->eng.note(hipMemcpyAsync(f->params, params, f->nparam * sizeof(float), hipMemcpyHostToDevice, f->eng.stream));
+    return fno_finish(f);
+}
+
+int pmc_fno_predict(pmc_fno* f, int32_t batch, const float* in, float* out) {
+    if (!f || !in || !out || batch <= 0) return PMC_ERR_ARGUMENT;
+    const int S = f->S;
+    pmc_nn* E = &f->eng;
+    for (int b0 = 0; b0 < batch; b0 += f->sh.max_batch) {
+        const int b = std::min(f->sh.max_batch, batch - b0);
+        E->note(hipMemcpyAsync(f->x, in + static_cast<size_t>(b0) * S * f->Cin,
+                               static_cast<size_t>(b) * S * f->Cin * sizeof(float), hipMemcpyHostToDevice, E->stream));
+        fno_forward(f, b);
+        E->note(hipMemcpyAsync(out + static_cast<size_t>(b0) * S, f->y, static_cast<size_t>(b) * S * sizeof(float),
+                               hipMemcpyDeviceToHost, E->stream));
+    }
+    return fno_finish(f);
+}
+
+int pmc_fno_loss_grad(pmc_fno* f, int32_t batch, const float* in, const float* target, const float* weight,
+                      float* loss, float* grad) {
+    if (!f || !in || !target || !weight || !loss || batch <= 0 || batch > f->sh.max_batch) return PMC_ERR_ARGUMENT;
+    const size_t R = static_cast<size_t>(batch) * f->S;
+    double ws = 0.0;
+    for (size_t i = 0; i < R; ++i) ws += weight[i];
+    if (!(ws > 0.0)) return PMC_ERR_ARGUMENT;
+    pmc_nn* E = &f->eng;
+    E->note(hipMemcpyAsync(f->x, in, R * f->Cin * sizeof(float), hipMemcpyHostToDevice, E->stream));
+    E->note(hipMemcpyAsync(f->t, target, R * sizeof(float), hipMemcpyHostToDevice, E->stream));
+    E->note(hipMemcpyAsync(f->w, weight, R * sizeof(float), hipMemcpyHostToDevice, E->stream));
+    fno_forward(f, batch);
+    fno_backward(f, batch, static_cast<float>(ws));
+    float sse = 0.f;
+    E->note(hipMemcpyAsync(&sse, f->sse, sizeof(float), hipMemcpyDeviceToHost, E->stream));
+    if (grad) E->note(hipMemcpyAsync(grad, f->grads, f->nparam * sizeof(float), hipMemcpyDeviceToHost, E->stream));
+    const int rc = fno_finish(f);
+    *loss = static_cast<float>(sse / ws);
+    return rc;
+}
+
+int pmc_fno_train(pmc_fno* f, int32_t n, const float* in, const float* target, const float* weight, int32_t epochs,
+                  int32_t batch, float lr, uint64_t seed, float* losses) {
+    if (!f || !in || !target || !weight || n <= 0 || epochs < 0 || batch <= 0 || batch > f->sh.max_batch ||
+        batch > n)
+        return PMC_ERR_ARGUMENT;
+    pmc_nn* E = &f->eng;
+    const int S = f->S;
+    const size_t in_row = static_cast<size_t>(S) * f->Cin;
+    const bool upload = f->ds_n != n || f->ds_src[0] != in || f->ds_src[1] != target || f->ds_src[2] != weight;
+    if (f->ds_n != n) {
+        f->ds_in = E->alloc<float>(static_cast<size_t>(n) * in_row);
+        f->ds_t = E->alloc<float>(static_cast<size_t>(n) * S);
+        f->ds_w = E->alloc<float>(static_cast<size_t>(n) * S);
+        f->ds_n = n;
+        if (E->err != hipSuccess) return PMC_ERR_DEVICE;
+    }
+    if (upload) {
+        E->note(hipMemcpyAsync(f->ds_in, in, static_cast<size_t>(n) * in_row * sizeof(float), hipMemcpyHostToDevice,
+                               E->stream));
+        E->note(hipMemcpyAsync(f->ds_t, target, static_cast<size_t>(n) * S * sizeof(float), hipMemcpyHostToDevice,
+                               E->stream));
+        E->note(hipMemcpyAsync(f->ds_w, weight, static_cast<size_t>(n) * S * sizeof(float), hipMemcpyHostToDevice,
+                               E->stream));
+        f->ds_src[0] = in;
+        f->ds_src[1] = target;
+        f->ds_src[2] = weight;
+    }
+    // Weight sum of each sample (host), for the batch normalization.
+    std::vector<double> wrow(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < S; ++k) wrow[i] += weight[static_cast<size_t>(i) * S + k];
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    uint64_t s = seed;
+    const float b1 = 0.9f, b2 = 0.999f;
+    for (int e = 0; e < epochs; ++e) {
+        for (int i = n - 1; i > 0; --i) std::swap(order[i], order[mix(s) % static_cast<uint64_t>(i + 1)]);
+        double total = 0.0;
+        int nb = 0;
+        for (int b0 = 0; b0 + batch <= n; b0 += batch, ++nb) {
+            double ws = 0.0;
+            for (int j = 0; j < batch; ++j) ws += wrow[order[b0 + j]];
+            E->note(hipMemcpyAsync(f->idx, order.data() + b0, batch * sizeof(int), hipMemcpyHostToDevice, E->stream));
+            hipLaunchKernelGGL(k_gather, dim3(blocks(static_cast<long long>(batch) * in_row)), dim3(256), 0, E->stream,
+                               f->ds_in, f->idx, f->x, batch, static_cast<long long>(in_row));
+            hipLaunchKernelGGL(k_gather, dim3(blocks(static_cast<long long>(batch) * S)), dim3(256), 0, E->stream,
+                               f->ds_t, f->idx, f->t, batch, static_cast<long long>(S));
+            hipLaunchKernelGGL(k_gather, dim3(blocks(static_cast<long long>(batch) * S)), dim3(256), 0, E->stream,
+                               f->ds_w, f->idx, f->w, batch, static_cast<long long>(S));
+            fno_forward(f, batch);
+            fno_backward(f, batch, static_cast<float>(ws));
+            ++f->adam_t;
+            const float c1 = 1.f - std::pow(b1, static_cast<float>(f->adam_t));
+            const float c2 = 1.f - std::pow(b2, static_cast<float>(f->adam_t));
+            hipLaunchKernelGGL(k_adam, dim3(blocks(f->nparam)), dim3(256), 0, E->stream, f->params, f->grads,
+                               f->adam_m, f->adam_v, static_cast<long long>(f->nparam), lr, b1, b2, c1, c2);
+            float sse = 0.f;
+            E->note(hipMemcpyAsync(&sse, f->sse, sizeof(float), hipMemcpyDeviceToHost, E->stream));
+            E->note(hipStreamSynchronize(E->stream));
+            total += sse / ws;
+        }
+        if (losses) losses[e] = static_cast<float>(total / std::max(1, nb));
+    }
+    return fno_finish(f);
+}
+
+int pmc_fno_set_timing(pmc_fno* f, int32_t on) { return f ? pmc_nn_set_timing(&f->eng, on) : PMC_ERR_ARGUMENT; }
+
+int pmc_fno_gemm_stats(pmc_fno* f, double* flops, int64_t* launches, double* ms, int32_t reset) {
+    return f ? pmc_nn_gemm_stats(&f->eng, flops, launches, ms, reset) : PMC_ERR_ARGUMENT;
+}
+
+void pmc_fno_destroy(pmc_fno* f) {
+    if (!f) return;
+    pmc_nn* E = &f->eng;
+    if (E->stream) (void)hipStreamSynchronize(E->stream);
+    for (auto& e : E->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (void* p : E->allocs) (void)hipFree(p);
+    if (E->stream) (void)hipStreamDestroy(E->stream);
+    delete f;
+}
+
+} // extern "C"

@@ -416,7 +416,7 @@ class Context:
         self._check(lib().pmc_get_launch_config(self.h, C.byref(c)))
         return {"event_split": c.event_split, "callaway_threads": c.callaway_threads}
 
-    def set_launch_config(self, event_split, callaway_threads=512):
+    def set_launch_config(self, event_split, callaway_threads=1024):
         c = LaunchConfig(event_split, callaway_threads)
         self._check(lib().pmc_set_launch_config(self.h, C.byref(c)))
 
@@ -427,7 +427,7 @@ class Context:
         rep = (C.c_double * 7)()
         self._check(lib().pmc_autotune(self.h, steps, C.byref(c), rep, 7))
         names = ["event_split=1", "event_split=2", "event_split=4", "event_split=8",
-                 "callaway_threads=128", "callaway_threads=256", "callaway_threads=512"]
+                 "callaway_threads=256", "callaway_threads=384", "callaway_threads=1024"]
         return ({"event_split": c.event_split, "callaway_threads": c.callaway_threads},
                 {k: v for k, v in zip(names, list(rep)) if v > 0})
 

@@ -249,17 +249,17 @@ def test_autotuned_launch_changes_no_result(cfg, n):
     a, b = rt.Context(p), rt.Context(p)
     chosen, report = a.autotune(steps=2)
     assert chosen["event_split"] in (1, 2, 4, 8)
-    assert chosen["callaway_threads"] in (128, 256, 512)
+    assert chosen["callaway_threads"] in (256, 384, 1024)
     assert len([k for k in report if k.startswith("event_split")]) == 4
     if p.callaway() is not None:
-        assert "callaway_threads=512" in report
+        assert "callaway_threads=1024" in report
     steps = a.ledger()["step"]
     assert steps >= 8
     b.step(steps)
     assert_same(a, b)
     print(cfg, chosen, {k: round(v, 4) for k, v in report.items()})
     # The configuration can also be set by hand.
-    b.set_launch_config(2, 512)
-    assert b.launch_config() == {"event_split": 2, "callaway_threads": 512}
+    b.set_launch_config(2, 384)
+    assert b.launch_config() == {"event_split": 2, "callaway_threads": 384}
     with pytest.raises(RuntimeError, match="event_split"):
         b.set_launch_config(3, 512)

@@ -143,3 +143,29 @@ def test_steps_match_oracle(cfg, n, sampling, series):
     qg, qo = g.heat_flux(), o.heat_flux()
     np.testing.assert_allclose(qg, qo, rtol=1e-5, atol=1e-6 * np.abs(qo).max())
     print(f"{cfg} n={n}: largest position error {worst:.2e} of the extent")
+
+
+@pytest.mark.parametrize("t_bg", [250.0, 400.0])
+def test_callaway_cells_of_few_carriers_match_oracle(t_bg):
+    """Config-3 physics at 2e5 carriers on the full 64 x 32 x 32 grid (about 8
+    carriers per cell, many cells with dependent carrier momenta) at other
+    background temperatures, with the source moved down the fin: cells whose
+    normal-gain directions are (nearly) dependent take the explicit per-carrier
+    Gram-Schmidt, the others the Gram-matrix form; both against the oracle's
+    long-double restatement of the reference, three steps."""
+    over = {"simulation.background_temperature": f"{t_bg}", "simulation.lifetime_temperature": f"{t_bg}",
+            "simulation.initial_temperature": f"{t_bg}", "boundary.boundary_values": f"[{t_bg}, 10.0]",
+            "heat_source.center": "[100, 50, 100]", "heat_source.sigma": "[10, 10, 10]",
+            "heat_source.power_density": "1e19"}
+    p, g, o = run_pair("finfet_callaway.toml", 200000, 3, 1, True, over)
+    for step in range(3):
+        g.step(1)
+        o.step(1)
+        a, b = by_uid(g.carriers()), o.carriers()
+        assert np.array_equal(a["uid"], b["uid"]), f"step {step}: live carrier sets differ"
+        assert np.array_equal(a["mode"] & MASK, b["mode"]), f"step {step}: modes differ"
+        assert np.array_equal(a["cell"], b["cell"]), f"step {step}: cells differ"
+        np.testing.assert_allclose(a["occupation"], b["occupation"], rtol=1e-12, atol=1e-300)
+        tg, _ = g.cells()
+        to, _ = o.cells()
+        np.testing.assert_allclose(tg, to, rtol=0, atol=1e-9)

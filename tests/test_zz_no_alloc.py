@@ -15,7 +15,11 @@ pytestmark = pytest.mark.gpu
 
 
 def _hip():
-    h = C.CDLL("libamdhip64.so")
+    # HIP entry points as the library resolves them: a process that imports
+    # torch after loading the library holds a second HIP runtime (torch's
+    # bundled copy) whose null stream and device synchronization know nothing
+    # of the library's launches, and a bare "libamdhip64.so" may name it.
+    h = C.CDLL(rt.BUILD + "/libpmc_device.so")
     h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
     h.hipFree.argtypes = [C.c_void_p]
@@ -86,9 +90,11 @@ def test_noalloc_gemm_matches_allocating_gemm_twice():
     for (m, n, k, batch, ta, tb) in [(96, 80, 2048, 3, 0, 1), (40, 24, 640, 2, 1, 0)]:
         a, b, c = _case(rng, m, n, k, batch, ta, tb)
         host, dev = _run_both(hip, L, a, b, c, ta, tb, 0.75, 0.5)
-        np.testing.assert_array_equal(dev, host)  # same kernels, same plan
         ref = _reference(a, b, c, ta, tb, 0.75, 0.5)
+        print(f"{(m, n, k, batch, ta, tb)}: |allocating - numpy| {np.abs(host - ref).max():.3e}, "
+              f"|no-alloc - numpy| {np.abs(dev - ref).max():.3e}", flush=True)
         np.testing.assert_allclose(dev, ref, rtol=0, atol=1e-5 * np.sqrt(k))
+        np.testing.assert_array_equal(dev, host)  # same kernels, same plan
 
 
 def test_noalloc_gemm_rejects_bad_arguments():
