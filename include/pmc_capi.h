@@ -257,6 +257,13 @@ typedef struct pmc_ledger {
     int64_t source_exact_passes;   /* exact carrier sums of the source pseudo-temperature solve */
     int64_t interface_events;      /* material-interface crossings handled (layered materials) */
     int64_t interface_transmitted; /* of which transmitted */
+    /* Heat-source ledger (PrescribedHeatSource::accrue / deposit_cell,
+     * src/solver/PrescribedHeatSource.cpp:232-275): updates this step, the
+     * integrated time factor (prescribed energy of a cell = its rate times
+     * this) and the energy still pending in cells without eligible carriers. */
+    int64_t step_source_updates;
+    double source_time_ps;
+    double source_pending_ev;
 } pmc_ledger;
 
 typedef struct pmc_error {
@@ -313,6 +320,10 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps);
 int pmc_read_cells(pmc_ctx* ctx, double* temperature, double* energy_density);
 /* Heat-flux tally (src/solver/GridStatistics.cpp:269-347), W/m^2, [n_cells*3]. */
 int pmc_read_heat_flux(pmc_ctx* ctx, double* flux);
+/* Heat-source energy still pending per cell (eV, [n_cells]): the budget of a
+ * cell without eligible carriers (PrescribedHeatSource::cell_pending_ev,
+ * src/solver/PrescribedHeatSource.cpp:232-262).  Zeros without a source. */
+int pmc_read_source_pending(pmc_ctx* ctx, double* pending);
 int pmc_read_ledger(pmc_ctx* ctx, pmc_ledger* ledger);
 /* Carrier state for tests and checkpoints: slots in [0, *n) (dead slots have mode -1).
  * On input *n is the capacity (slots) of the arrays given; with every array NULL

@@ -76,6 +76,11 @@ double pmcs_problem_energy_from_temperature(const pmcs_problem* problem, double 
 int pmcs_problem_facets(const pmcs_problem* problem, int32_t* n, double* normals, double* centroids, double* areas,
                         char* conditions, double* shifts, double* values, int32_t* face_counts);
 int pmcs_problem_cells(const pmcs_problem* problem, double* centers, double* volumes);
+/* summary.txt ([input], [geometry], [grid], [boundary], [files]) and
+ * grid_centers.csv of the problem in `folder`, as the solver writes them at
+ * start-up (SimulationDomain::write_domain_summary, src/geometry/DomainOutput.cpp:12-241);
+ * host only. */
+int pmcs_problem_write_summary(const pmcs_problem* problem, const char* folder, char* err, int32_t errlen);
 
 /* Builds a solver (takes ownership of the problem); output_folder may be NULL for no files. */
 int pmcs_solver_create(pmcs_problem* problem, const char* output_folder, pmcs_solver** out, char* err, int32_t errlen);

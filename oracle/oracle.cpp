@@ -863,6 +863,7 @@ struct orc_sim {
     bool hs_on = false;
     pmc_heat_source hs{};
     std::vector<double> hs_rate, hs_pending;
+    double source_time = 0.0;
     std::vector<int> hs_elig;
     double elapsed = 0.0;
     // layered materials (DMM interfaces)
@@ -1224,6 +1225,7 @@ double source_time_factor(const pmc_heat_source& h, double t0, double t1) {
 int apply_source(orc_sim& s, double factor, double* energy_out, int64_t* updates_out) {
     *energy_out = 0.0;
     *updates_out = 0;
+    s.source_time += factor; // every cell's prescribed energy is its rate times this sum
     double pending_total = 0.0;
     for (double p : s.hs_pending) pending_total += p;
     if (factor <= 0.0 && pending_total <= 0.0) return 0;
@@ -1630,6 +1632,10 @@ int orc_step(orc_sim* s, int32_t nsteps) {
         L.step_source_energy_ev = source_e;
         L.total_source_energy_ev += source_e;
         L.source_updates += source_n;
+        L.step_source_updates = source_n;
+        L.source_time_ps = s->source_time;
+        L.source_pending_ev = 0.0;
+        for (double p : s->hs_pending) L.source_pending_ev += p;
         L.step_absorbed = total.absorbed;
         L.step_injected = emissions;
         L.step_absorbed_energy_ev = total.absorbed_e;

@@ -328,6 +328,15 @@ def source_apply(domain, material, integrated_time, cell_temperature, active, ce
     return occ, dep.value, upd.value
 
 
+def domain_summary(toml_path, folder):
+    """ref_domain_summary: the reference's summary.txt and grid_centers.csv in `folder`."""
+    L = lib()
+    L.ref_domain_summary.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
+    err = C.create_string_buffer(512)
+    if L.ref_domain_summary(toml_path.encode(), folder.encode(), err, 512) != 0:
+        raise RuntimeError(err.value.decode())
+
+
 def run(toml_path, material_view, steps, threads=0, lookup_dt=0.1, warmup=0):
     mat = RefMaterial(material_view, lookup_dt)  # keeps the arrays struct alive
     secs, ps = C.c_double(), C.c_longlong()

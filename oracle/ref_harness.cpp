@@ -230,6 +230,14 @@ int ref_domain_create(const char* toml_path, void** out, char* err, int errlen) 
     });
 }
 
+int ref_domain_summary(const char* toml_path, const char* folder, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        SimulationConfig config = load_simulation_config(toml_path);
+        config.output_folder = folder;
+        SimulationDomain domain(config); // the constructor writes the summary (src/SimulationDomain.cpp:86)
+    });
+}
+
 void ref_domain_destroy(void* domain) { delete static_cast<Domain*>(domain); }
 
 int ref_domain_facet_count(void* domain) { return dom(domain).domain->mesh().facet_count(); }

@@ -62,6 +62,9 @@ double ref_energy_density_normalization(void* material);
 
 /* SimulationConfig + SimulationDomain built from a TOML file by the reference parser. */
 int ref_domain_create(const char* toml_path, void** out, char* err, int errlen);
+/* SimulationDomain::write_domain_summary (src/geometry/DomainOutput.cpp:12-241) of the
+ * input with its output folder replaced by `folder`: summary.txt and grid_centers.csv. */
+int ref_domain_summary(const char* toml_path, const char* folder, char* err, int errlen);
 void ref_domain_destroy(void* domain);
 int ref_domain_facet_count(void* domain);
 int ref_domain_grid_count(void* domain);

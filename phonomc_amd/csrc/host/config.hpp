@@ -16,7 +16,7 @@ enum class HeatProfile { Uniform, Gaussian };
 struct SimConfig {
     std::string input_directory;
     std::string model = "box";
-    std::vector<double> sizes{2000.0, 2000.0, 2000.0}; // Angstrom
+    std::vector<double> sizes{200.0, 200.0, 200.0}; // Angstrom (include/SimulationConfig.h:46)
     bool merge_coplanar_facets = true;
     double particle_count = 1e4;
     double time_step = 1.0; // ps

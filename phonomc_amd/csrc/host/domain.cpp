@@ -5,6 +5,7 @@
 #include <deque>
 #include <filesystem>
 #include <fstream>
+#include <iomanip>
 #include <limits>
 #include <map>
 #include <mutex>
@@ -369,7 +370,7 @@ Domain build_domain(const SimConfig& c) {
     return d;
 }
 
-void write_domain_summary(const Domain& d, const SimConfig& c, const std::string& folder) {
+void write_domain_summary(const Domain& d, const SimConfig& c, const std::string& folder, int n_simplices) {
     namespace fs = std::filesystem;
     fs::create_directories(folder);
     std::ofstream out(fs::path(folder) / "summary.txt");
@@ -392,7 +393,15 @@ void write_domain_summary(const Domain& d, const SimConfig& c, const std::string
     out << "temperature_lookup_dt = " << c.temperature_lookup_dt << '\n';
     out << "background_temperature = " << c.background_temperature << '\n';
     out << "callaway_formulation = " << (c.callaway_nonlinear ? "nonlinear" : "linearized") << '\n';
-    out << "conservative_boundaries = " << (c.conservative_boundaries ? "true" : "false") << '\n';
+    // Linearized Callaway transport is conservative transport (uses_conservative_transport,
+    // include/SimulationConfig.h:153-156).
+    const bool conservative = c.collision == CollisionModel::Callaway || c.callaway_nonlinear || c.conservative_boundaries;
+    out << "conservative_boundaries = " << (conservative ? "true" : "false") << '\n';
+    if (conservative) {
+        out << "collision_model = " << (c.collision == CollisionModel::Callaway ? "callaway" : "rta") << '\n';
+        out << "collision_reference_temperature = " << c.background_temperature << '\n';
+        out << "reservoir_generation = fixed_incoming_flux\n";
+    }
     out << "lifetime_temperature = ";
     if (c.lifetime_local) out << "local";
     else out << c.lifetime_temperature;
@@ -414,14 +423,74 @@ void write_domain_summary(const Domain& d, const SimConfig& c, const std::string
     }
     out << '\n';
     out << "boundary_values = " << join(c.boundary_values) << '\n';
+    // Selector tokens as the reference keeps them: the mode, then each box's six
+    // numbers at 17 significant digits (ConfigParsing.cpp flatten_boxes).
+    auto tokens = [](const std::vector<std::array<double, 6>>& boxes) {
+        if (boxes.empty()) return std::string("[]");
+        std::ostringstream o;
+        o << "[relative_box";
+        for (const auto& b : boxes)
+            for (double x : b) {
+                std::ostringstream t;
+                t << std::setprecision(17) << x;
+                o << ", " << t.str();
+            }
+        o << "]";
+        return o.str();
+    };
+    auto strings = [](const std::vector<std::string>& v) {
+        std::ostringstream o;
+        o << "[";
+        for (size_t i = 0; i < v.size(); ++i) o << (i ? ", " : "") << v[i];
+        o << "]";
+        return o.str();
+    };
+    auto scaled = [&](const std::vector<double>& v, double f) {
+        std::vector<double> w(v);
+        for (double& x : w) x *= f;
+        return join(w);
+    };
     out << "boundary_position_selector_mode = " << (c.boundary_boxes.empty() ? "none" : "region") << '\n';
     out << "boundary_position_selector_count = " << c.boundary_boxes.size() << '\n';
+    out << "boundary_position_tokens = " << tokens(c.boundary_boxes) << '\n';
     out << "periodic_pair_region_count = " << c.periodic_pairs.size() << '\n';
+    out << "periodic_pair_tokens = " << tokens(c.periodic_pairs) << '\n';
     out << "material_folder = " << c.material_folder << '\n';
+    out << "material_folders = " << strings(c.material_folders) << '\n';
+    out << "material_interface_axis = " << "xyz"[c.interface_axis] << '\n';
+    out << "material_interface_positions_nm = " << scaled(c.interface_positions, 0.1) << '\n';
+    out << "material_interface_model = "
+        << (c.interface_model == "amm" ? "acoustic_mismatch" : c.interface_model == "mmm" ? "mixed_mismatch"
+                                                                                           : "diffuse_mismatch")
+        << '\n';
+    out << "material_interface_amm_fraction = " << (c.interface_model == "amm" ? 1.0 : c.interface_amm_fraction) << '\n';
+    out << "material_interface_parallel_bin_inv_a = " << c.interface_parallel_bin << '\n';
     out << "output_folder = " << folder << '\n';
     out << "heat_source_enabled = " << (c.heat_source_enabled ? "true" : "false") << '\n';
     out << "heat_source_profile = " << (c.heat_profile == HeatProfile::Gaussian ? "gaussian" : "uniform") << '\n';
+    out << "heat_source_min_nm = " << scaled(c.heat_min, 0.1) << '\n';
+    out << "heat_source_max_nm = " << scaled(c.heat_max, 0.1) << '\n';
+    out << "heat_source_center_nm = " << scaled(c.heat_center, 0.1) << '\n';
+    out << "heat_source_centers_nm = " << scaled(c.heat_centers, 0.1) << '\n';
+    out << "heat_source_sigma_nm = " << scaled(c.heat_sigma, 0.1) << '\n';
+    out << "heat_source_half_width_nm = " << scaled(c.heat_half_width, 0.1) << '\n';
+    out << "heat_source_min_angstrom = " << join(c.heat_min) << '\n';
+    out << "heat_source_max_angstrom = " << join(c.heat_max) << '\n';
+    out << "heat_source_center_angstrom = " << join(c.heat_center) << '\n';
+    out << "heat_source_centers_angstrom = " << join(c.heat_centers) << '\n';
+    out << "heat_source_sigma_angstrom = " << join(c.heat_sigma) << '\n';
+    out << "heat_source_half_width_angstrom = " << join(c.heat_half_width) << '\n';
     out << "heat_source_power_density = " << c.heat_power_density << '\n';
+    out << "heat_source_power_densities = " << join(c.heat_power_densities) << '\n';
+    out << "heat_source_frequency_min_THz = " << c.heat_frequency_min << '\n';
+    out << "heat_source_frequency_max_THz = " << c.heat_frequency_max << '\n';
+    out << "heat_source_time_profile = " << (c.heat_time_square ? "square" : "constant") << '\n';
+    // [mode_excitation] is parsed and rejected at set-up: a summary is only
+    // written for inputs without it.
+    out << "mode_excitation_enabled = false\n";
+    out << "mode_excitation_frequency_min_THz = -1\n";
+    out << "mode_excitation_frequency_max_THz = -1\n";
+    out << "mode_excitation_occupation_multiplier = 1\n";
     out << '\n';
     out << "[geometry]\n";
     out << "volume = " << d.volume << '\n';
@@ -429,7 +498,7 @@ void write_domain_summary(const Domain& d, const SimConfig& c, const std::string
     out << "bounds_max = [" << d.bmax[0] << ", " << d.bmax[1] << ", " << d.bmax[2] << "]\n";
     out << "n_faces = " << d.faces.size() << '\n';
     out << "n_facets = " << d.facets.size() << '\n';
-    out << "n_simplices = 0\n\n";
+    out << "n_simplices = " << n_simplices << "\n\n";
     out << "[grid]\n";
     out << "count = " << d.cell_count() << '\n';
     out << "lookup = voxel_to_active_grid_o1\n\n";

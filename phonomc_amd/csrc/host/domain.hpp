@@ -51,6 +51,7 @@ struct Domain {
 Domain build_domain(const SimConfig& config);
 
 // summary.txt geometry sections and grid_centers.csv (src/geometry/DomainOutput.cpp:12-241).
-void write_domain_summary(const Domain& d, const SimConfig& c, const std::string& folder);
+// n_simplices: star tetrahedra of a mesh model (0 for a box).
+void write_domain_summary(const Domain& d, const SimConfig& c, const std::string& folder, int n_simplices = 0);
 
 } // namespace pmc

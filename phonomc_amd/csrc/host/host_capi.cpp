@@ -2,6 +2,7 @@
 #include "pmc_solver.h"
 #include "solver.hpp"
 
+#include <cstdio>
 #include <cstring>
 #include <exception>
 #include <fstream>
@@ -191,6 +192,18 @@ int pmcs_problem_facets(const pmcs_problem* pr, int32_t* n, double* normals, dou
         if (conditions) conditions[f] = d.facets[f].condition;
         if (values) values[f] = d.facets[f].value;
         if (face_counts) face_counts[f] = static_cast<int32_t>(d.facets[f].faces.size());
+    }
+    return PMC_OK;
+}
+
+int pmcs_problem_write_summary(const pmcs_problem* pr, const char* folder, char* err, int32_t errlen) {
+    if (!pr || !folder) return PMC_ERR_ARGUMENT;
+    try {
+        pmc::write_domain_summary((*pr)->domain, (*pr)->config, folder,
+                                  static_cast<int>((*pr)->mesh.simplex_volume.size()));
+    } catch (const std::exception& e) {
+        if (err && errlen > 0) std::snprintf(err, static_cast<size_t>(errlen), "%s", e.what());
+        return PMC_ERR_ARGUMENT;
     }
     return PMC_OK;
 }

@@ -597,7 +597,7 @@ MonteCarloSolver::MonteCarloSolver(std::unique_ptr<Problem> problem, std::string
     if (p.heat.enabled) throw_if_failed(pmc_upload_heat_source(ctx_, &p.heat_view));
     throw_if_failed(pmc_initialize_carriers(ctx_, &p.init));
     if (!folder_.empty()) {
-        write_domain_summary(p.domain, p.config, folder_);
+        write_domain_summary(p.domain, p.config, folder_, static_cast<int>(p.mesh.simplex_volume.size()));
         write_convergence_header();
     }
     if (log_) {
@@ -732,11 +732,14 @@ void MonteCarloSolver::append_convergence_row() {
     out << step_ << " " << time;
     for (double t : cell_T_) out << " " << t;
     out << std::setprecision(12);
+    const double prescribed = source_prescribed_ev(l);
     out << " " << mean_flux_ << " " << kappa_int_ << " " << kappa_eff_ << " " << l.live_carriers << " "
         << l.step_absorbed << " " << l.step_injected << " " << 0 << " " << (l.step_injected - l.step_absorbed) << " "
-        << l.step_absorbed_energy_ev << " " << l.step_injected_energy_ev << " " << 0 << " " << 0.0 << " "
-        << l.step_lifetime_residual_ev << " " << l.step_balance_residual_ev << " " << l.thermal_energy_ev << " "
-        << 0.0 << " " << 0.0 << " " << 0.0 << '\n';
+        << l.step_absorbed_energy_ev << " " << l.step_injected_energy_ev << " " << l.step_source_updates << " "
+        << l.step_source_energy_ev << " " << l.step_lifetime_residual_ev << " " << l.step_balance_residual_ev << " "
+        << l.thermal_energy_ev << " " << prescribed << " " << l.source_pending_ev << " "
+        << (prescribed - l.total_source_energy_ev - l.source_pending_ev) << '\n';
+    if (problem_->heat.enabled) write_source_ledger(l);
     if (problem_->config.write_cell_heat_flux) {
         std::ofstream f(std::filesystem::path(folder_) / "grid_heat_flux.csv", std::ios::app);
         f << std::setprecision(17);
@@ -746,9 +749,81 @@ void MonteCarloSolver::append_convergence_row() {
     }
 }
 
+// Prescribed energy so far: every cell accrues its rate times the integrated
+// time factor (PrescribedHeatSource::accrue, src/solver/PrescribedHeatSource.cpp:232-252).
+double MonteCarloSolver::source_prescribed_ev(const pmc_ledger& l) const {
+    if (!problem_->heat.enabled) return 0.0;
+    double rate = 0.0;
+    for (double r : problem_->heat.rates) rate += r;
+    return rate * l.source_time_ps;
+}
+
+void MonteCarloSolver::write_source_ledger(const pmc_ledger& l) const {
+    // heat_source_ledger.txt (ResultWriter::format_source_ledger, src/solver/ResultWriter.cpp:343-354).
+    const auto& p = *problem_;
+    const int nc = p.domain.cell_count();
+    std::vector<double> pending(nc);
+    throw_if_failed(pmc_read_source_pending(ctx_, pending.data()));
+    std::ofstream out(std::filesystem::path(folder_) / "heat_source_ledger.txt", std::ios::trunc);
+    if (!out) throw std::runtime_error("Cannot open output file: " + folder_ + "/heat_source_ledger.txt");
+    out << std::setprecision(17) << "# prescribed_lattice_power; not an electron-phonon calculation\n"
+        << "# spectrum=thermal; pending energy is not yet represented by phonons\n"
+        << "# cell x_A y_A z_A material base_power_W prescribed_eV deposited_eV pending_eV\n";
+    const auto& cm = p.layers.cell_material;
+    for (int c = 0; c < nc; ++c) {
+        const auto& x = p.domain.cell_center[c];
+        const double rate = p.heat.rates[c];
+        const double prescribed = rate * l.source_time_ps;
+        out << c << " " << x[0] << " " << x[1] << " " << x[2] << " " << (cm.empty() ? 0 : cm[c]) << " "
+            << rate / 6.241509074e6 << " " << prescribed << " " << (prescribed - pending[c]) << " " << pending[c]
+            << '\n';
+    }
+}
+
+void MonteCarloSolver::append_source_summary(std::ostream& out, const pmc_ledger& l) const {
+    // [local_heat_source_occupation_injection] (ResultWriter::format_source_summary,
+    // src/solver/ResultWriter.cpp:121-170); this build deposits the thermal spectrum only.
+    const SimConfig& c = problem_->config;
+    const HeatSource& h = problem_->heat;
+    const double prescribed = source_prescribed_ev(l);
+    out << "\n[local_heat_source_occupation_injection]\n";
+    out << "enabled = " << (h.enabled ? "true" : "false") << '\n';
+    out << "profile = " << (c.heat_profile == HeatProfile::Gaussian ? "gaussian" : "uniform") << '\n';
+    if (c.heat_total_power) out << "specified_total_power_w = " << *c.heat_total_power << '\n';
+    else out << "power_density_wm3 = " << c.heat_power_density << '\n';
+    out << "model = prescribed_lattice_power\n";
+    out << "electron_transport = false\n";
+    out << "empty_cell_policy = defer_locally\n";
+    out << "spectrum = thermal\n";
+    out << "power_input = " << (c.heat_total_power ? "total_power" : "power_density") << '\n';
+    out << "base_total_power_w = " << (h.enabled ? h.base_power_w() : 0.0) << '\n';
+    out << "frequency_min_thz = " << c.heat_frequency_min << '\n';
+    out << "frequency_max_thz = " << c.heat_frequency_max << '\n';
+    out << "branches_zero_based = [";
+    for (size_t i = 0; i < c.heat_branches.size(); ++i) out << (i ? ", " : "") << c.heat_branches[i];
+    out << "]\nbranch_weights_per_sampled_mode = []\n";
+    out << "prescribed_energy_total_ev = " << prescribed << '\n';
+    out << "pending_energy_total_ev = " << l.source_pending_ev << '\n';
+    out << "deposition_complete = " << (l.source_pending_ev == 0.0 ? "true" : "false") << '\n';
+    out << "source_ledger_residual_ev = " << (prescribed - l.total_source_energy_ev - l.source_pending_ev) << '\n';
+    out << "time_profile = " << (c.heat_time_square ? "square" : "constant") << '\n';
+    out << "time_start_ps = " << c.heat_time_start << '\n';
+    out << "time_end_ps = " << c.heat_time_end << '\n';
+    out << "period_ps = " << c.heat_period << '\n';
+    out << "on_duration_ps = " << c.heat_on_duration << '\n';
+    out << "duty_cycle = " << c.heat_duty_cycle << '\n';
+    out << "amplitude = " << c.heat_amplitude << '\n';
+    out << "occupation_updates_total = " << l.source_updates << '\n';
+    out << "injected_energy_total_ev = " << l.total_source_energy_ev << '\n';
+    out << "occupation_updates_last_step = " << l.step_source_updates << '\n';
+    out << "injected_energy_last_step_ev = " << l.step_source_energy_ev << '\n';
+    if (h.enabled && !folder_.empty()) write_source_ledger(l);
+}
+
 void MonteCarloSolver::append_profile_summary(std::ostream& out) const {
     // [performance] / [rough_boundary_diagnostics] keys of
     // ResultWriter::format_run_summary (src/solver/ResultWriter.cpp:172-341) plus a [gpu] section.
+    const auto& p = *problem_;
     const pmc_ledger l = ledger();
     out << std::setprecision(17);
     out << "\n[performance]\n";
@@ -774,7 +849,10 @@ void MonteCarloSolver::append_profile_summary(std::ostream& out) const {
     const double iden = std::max(1.0, static_cast<double>(l.interface_events));
     out << "\n[material_interface_diagnostics]\n";
     out << "material_count = " << materials << '\n';
-    out << "model = " << c.interface_model << '\n';
+    out << "model = "
+        << (c.interface_model == "amm" ? "acoustic_mismatch" : c.interface_model == "mmm" ? "mixed_mismatch"
+                                                                                           : "diffuse_mismatch")
+        << '\n';
     if (materials > 1) {
         out << "frequency_bin_thz = " << c.interface_bin_thz << '\n';
         out << "amm_fraction = 0\n";
@@ -788,6 +866,30 @@ void MonteCarloSolver::append_profile_summary(std::ostream& out) const {
     out << "reflected_total = " << (l.interface_events - l.interface_transmitted) << '\n';
     out << "transmission_ratio = " << static_cast<double>(l.interface_transmitted) / iden << '\n';
     out << "reflection_ratio = " << static_cast<double>(l.interface_events - l.interface_transmitted) / iden << '\n';
+    // src/solver/ResultWriter.cpp:290-310: strict transport never loses a carrier,
+    // so no recovery happens; the device checks containment every 100 steps.
+    out << "\n[escaped_particle_recovery]\n";
+    out << "check_interval = 100\n";
+    out << "policy = strict\n";
+    out << "recovery_events = 0\n";
+    out << "recovered_particles_total = 0\n";
+    out << "\n[reservoir_balance_diagnostics]\n";
+    out << "reservoir_gen = " << (p.linearized ? "fixed_incoming_flux" : "one_to_one") << '\n';
+    out << "absorbed_particles_total = " << l.total_absorbed << '\n';
+    out << "injected_particles_total = " << l.total_injected << '\n';
+    out << "recovered_particles_total = 0\n";
+    out << "net_particles_total = " << (l.total_injected - l.total_absorbed) << '\n';
+    out << "absorbed_deviational_energy_total_ev = " << l.total_absorbed_energy_ev << '\n';
+    out << "injected_deviational_energy_total_ev = " << l.total_injected_energy_ev << '\n';
+    out << "\n[energy_weighting]\n";
+    out << "initial_particle_count = " << p.init.count << '\n';
+    out << "current_particle_count = " << l.live_carriers << '\n';
+    out << "particle_spatial_weight_a3 = " << p.particle_volume << '\n';
+    out << "background_reference_temperature_k = " << c.background_temperature << '\n';
+    out << "total_thermal_energy_ev = " << l.thermal_energy_ev << '\n';
+    out << "lifetime_energy_residual_total_ev = " << l.total_lifetime_residual_ev << '\n';
+    out << "energy_balance_residual_total_ev = " << l.total_balance_residual_ev << '\n';
+    append_source_summary(out, l);
     out << "\n[gpu]\n";
     out << "device_steps = " << l.step << '\n';
     out << "carrier_steps = " << l.carrier_steps << '\n';

@@ -110,6 +110,9 @@ private:
     void record_observables();
     void write_convergence_header();
     void append_convergence_row();
+    double source_prescribed_ev(const pmc_ledger& l) const;
+    void write_source_ledger(const pmc_ledger& l) const;
+    void append_source_summary(std::ostream& out, const pmc_ledger& l) const;
 
     std::unique_ptr<Problem> problem_;
     std::string folder_;

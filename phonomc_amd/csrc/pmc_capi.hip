@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -714,7 +715,8 @@ int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* t) {
     v.cw_pack = reinterpret_cast<const double4*>(ctx->upload(pack.data(), pack.size()));
     v.cw_dir = ctx->alloc<double>(4 * static_cast<size_t>(v.capacity));
     v.cw_cell = ctx->alloc<double>(pmc_dev::kCwRec * static_cast<size_t>(v.ncells));
-    if (!v.cw_dir || !v.cw_cell) return fail(ctx, PMC_ERR_DEVICE, "Callaway direction buffer");
+    v.cw_big = ctx->alloc<int>(v.ncells);
+    if (!v.cw_dir || !v.cw_cell || !v.cw_big) return fail(ctx, PMC_ERR_DEVICE, "Callaway direction buffer");
     v.cw_cbg = t->heat_capacity_density;
     v.e_bg = t->background_energy_density;
     v.callaway = 1;
@@ -888,6 +890,15 @@ int pmc_read_heat_flux(pmc_ctx* ctx, double* flux) {
     return check_hip(ctx, hipGetLastError(), "heat flux");
 }
 
+int pmc_read_source_pending(pmc_ctx* ctx, double* pending) {
+    if (!ctx || !pending || !ctx->has_geometry) return PMC_ERR_ARGUMENT;
+    const View& v = ctx->view;
+    ctx->note(hipStreamSynchronize(ctx->stream));
+    if (v.hs_on) ctx->note(hipMemcpy(pending, v.hs_pending, v.ncells * sizeof(double), hipMemcpyDeviceToHost));
+    else std::fill(pending, pending + v.ncells, 0.0);
+    return check_hip(ctx, hipGetLastError(), "read source pending");
+}
+
 int pmc_read_ledger(pmc_ctx* ctx, pmc_ledger* out) {
     if (!ctx || !out) return PMC_ERR_ARGUMENT;
     DevState st{};
@@ -921,6 +932,15 @@ int pmc_read_ledger(pmc_ctx* ctx, pmc_ledger* out) {
     out->total_source_energy_ev = st.total_source_e;
     out->source_updates = st.source_updates;
     out->source_exact_passes = st.source_passes;
+    out->step_source_updates = st.step_source_updates;
+    out->source_time_ps = st.source_time;
+    out->source_pending_ev = 0.0;
+    if (ctx->view.hs_on) { // PrescribedHeatSource::pending_energy_ev: the cells summed in order
+        std::vector<double> pending(ctx->view.ncells);
+        ctx->note(hipMemcpy(pending.data(), ctx->view.hs_pending, pending.size() * sizeof(double),
+                            hipMemcpyDeviceToHost));
+        for (double e : pending) out->source_pending_ev += e;
+    }
     return check_hip(ctx, hipGetLastError(), "read ledger");
 }
 
@@ -1194,8 +1214,16 @@ struct pmc_batch {
     std::vector<pmc_ctx*> ctx;
     hipStream_t stream = nullptr;
     View* views[2] = {};                    // device copies of the instances' views, by carrier buffer
-    int* offs[pmc_launch::kBatchKinds] = {}; // device prefix offsets of block counts
-    int total[pmc_launch::kBatchKinds] = {};
+    // Per kernel kind, the launches of one batched step: the device prefix
+    // offsets of the instances' block counts, shifted to the launch's first
+    // block, and the launch's block count.  A kind of more than kMaxBlocks
+    // blocks in all is split (a launch's work-items must stay below 2^32).
+    struct Piece {
+        int* offs;
+        int blocks;
+    };
+    static constexpr int kMaxBlocks = 1 << 21;
+    std::vector<Piece> pieces[pmc_launch::kBatchKinds];
     int* errors = nullptr;                  // device: error word per instance
     bool mesh = false, callaway = false, source = false;
     bool use_graph = true;
@@ -1216,10 +1244,15 @@ T* batch_upload(pmc_batch* b, const T* host, size_t n) {
 
 // The step's kernels after the optional containment check, for the instances'
 // carrier buffer `cur` (the sort writes the other one).
+void batch_kind(pmc_batch* b, int kind, int buf) {
+    const int n = static_cast<int>(b->ctx.size());
+    for (const pmc_batch::Piece& p : b->pieces[kind])
+        pmc_launch::batch_launch(kind, b->mesh, b->views[buf], p.offs, n, p.blocks, b->stream);
+}
+
 void batch_sequence(pmc_batch* b, int cur) {
     using namespace pmc_launch;
-    const int n = static_cast<int>(b->ctx.size());
-    auto run = [&](int kind, int buf) { batch_launch(kind, b->mesh, b->views[buf], b->offs[kind], n, b->total[kind], b->stream); };
+    auto run = [&](int kind, int buf) { batch_kind(b, kind, buf); };
     run(kBatchBegin, cur);
     if (b->source) { // first half-step deposit (src/MonteCarloSolver.cpp:366-369)
         run(kBatchSourceSums0, cur);
@@ -1234,6 +1267,7 @@ void batch_sequence(pmc_batch* b, int cur) {
     run(kBatchSort, cur);
     run(kBatchCellPrep, 1 - cur);
     if (b->callaway) {
+        run(kBatchCallawayWave, 1 - cur);
         run(kBatchCallaway, 1 - cur);
     } else {
         run(kBatchMoments, 1 - cur);
@@ -1320,9 +1354,19 @@ int pmc_batch_create(pmc_ctx* const* contexts, int32_t n, pmc_batch** out) {
         // Each instance gets exactly its single-launch grid.  The kernels that map
         // blocks to XCDs have counts that are multiples of 8, so their ranges
         // start at multiples of 8 and keep the mapping.
-        for (int i = 0; i < n; ++i) offs[i + 1] = offs[i] + pmc_launch::batch_blocks(b->ctx[i]->view, k);
-        b->total[k] = offs[n];
-        if (!(b->offs[k] = batch_upload(b, offs.data(), offs.size()))) return bail("offsets");
+        long long total = 0;
+        for (int i = 0; i < n; ++i) {
+            total += pmc_launch::batch_blocks(b->ctx[i]->view, k);
+            if (total > INT_MAX / 2) return bail("too many workgroups");
+            offs[i + 1] = static_cast<int>(total);
+        }
+        for (int base = 0; base < offs[n]; base += pmc_batch::kMaxBlocks) {
+            std::vector<int> shifted(n + 1);
+            for (int i = 0; i <= n; ++i) shifted[i] = offs[i] - base;
+            int* d = batch_upload(b, shifted.data(), shifted.size());
+            if (!d) return bail("offsets");
+            b->pieces[k].push_back({d, std::min(pmc_batch::kMaxBlocks, offs[n] - base)});
+        }
     }
     std::vector<int> zero(n, 0);
     if (!(b->errors = batch_upload(b, zero.data(), zero.size()))) return bail("error words");
@@ -1350,10 +1394,7 @@ int pmc_batch_step(pmc_batch* b, int32_t nsteps) {
     for (int s = 0; s < nsteps; ++s) {
         pmc_ctx* c0 = b->ctx[0];
         const int cur = c0->cur;
-        if (c0->host_step % 100 == 0)
-            pmc_launch::batch_launch(pmc_launch::kBatchContainment, b->mesh, b->views[cur],
-                                     b->offs[pmc_launch::kBatchContainment], n,
-                                     b->total[pmc_launch::kBatchContainment], b->stream);
+        if (c0->host_step % 100 == 0) batch_kind(b, pmc_launch::kBatchContainment, cur);
         if (b->use_graph) {
             if (!b->exec[cur]) { // capture this buffer parity's step once
                 hipGraph_t g = nullptr;

@@ -157,9 +157,12 @@ struct DevState {
     long long iface_events, iface_transmitted;
     double step_source_e, total_source_e;   // heat-source energy deposited (physical eV)
     long long source_updates;               // occupation updates by the heat source
+    long long step_source_updates;          // of which this step
+    double source_time;                     // integrated source time factor (ps): prescribed = total rate * this
     long long source_passes;                // exact carrier sums of the source pseudo-temperature solve
     long long newton_passes;  // exact carrier sums of the pseudo-temperature solve
     long long newton_evals;   // all evaluations (series or exact)
+    int cw_big;               // length of the large-cell Callaway list this step
     int error_code;
     long long error_slot;
     float error_pos[3];
@@ -322,6 +325,7 @@ struct View {
     const double4* cw_pack; // 2 per mode: (g_N, g_R, B0, B1), (B2, B3, hw, 0); B = sqrt(C) (1, q / w)
     double cw_cbg;          // heat-capacity density at T_bg, eV/(K A^3)
     int cw_threads;         // k_callaway workgroup size: 256, 384 or kCwThreads
+    int* cw_big;            // cells of more than kCwWaveMax carriers this step (count in DevState)
 };
 
 // Transport blocks for a slot capacity (a multiple of 8, one per XCD round).
@@ -334,9 +338,15 @@ __host__ __device__ inline int wave_blocks_for(int capacity) {
 constexpr int kMaxEventSplit = 8;
 __host__ __device__ inline int event_rows_for(const View& v) { return wave_blocks_for(v.capacity) * v.event_split; }
 
-// Linearized Callaway kernel: one workgroup of 256, 384 or kCwThreads per cell,
-// up to kCwPer carriers per thread held in registers.
+// Linearized Callaway kernels.  A cell of at most kCwWaveMax carriers is one
+// wavefront's work (kCwTinyPer or kCwWavePer carriers per lane in registers,
+// reductions by cross-lane shuffles); a larger cell goes on a per-step list to
+// one workgroup of 256, 384 or kCwThreads threads, up to kCwPer carriers per
+// thread.
 constexpr int kCwThreads = 1024, kCwWaves = kCwThreads / 64, kCwPer = 8;
+constexpr int kCwTinyPer = 2, kCwWavePer = 8;
+constexpr int kCwTinyMax = 64 * kCwTinyPer, kCwWaveMax = 64 * kCwWavePer;
+constexpr int kCwBigBlocks = 2048; // workgroups that walk a context's list of large cells
 constexpr int kCwRec = 32; // doubles of the per-cell Callaway record
 
 

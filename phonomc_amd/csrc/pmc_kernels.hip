@@ -1001,6 +1001,7 @@ __device__ __forceinline__ void begin_body(const View& V, const Grid G) {
         st->n_slots_start = st->n_slots;
         st->carrier_steps += st->n_live;
         st->step_source_e = 0.0;
+        st->step_source_updates = 0;
     }
     __syncthreads();
     if (t == 0) {
@@ -1773,6 +1774,7 @@ __device__ __forceinline__ CellRta cell_rta(const View& V, int c) {
 __device__ __forceinline__ void cell_prep_body(const View& V, const Grid G) {
     const int c = G.b * blockDim.x + threadIdx.x;
     if (c < V.ncells) V.crta[c] = cell_rta(V, c);
+    if (c == 0 && V.callaway) V.st->cw_big = 0; // the Callaway kernels that follow list large cells anew
 }
 
 __global__ void k_cell_prep(View V) { cell_prep_body(V, launch_grid()); }
@@ -2148,11 +2150,12 @@ __device__ __forceinline__ double cw_inner(const double (&G)[4][4], const double
     return s;
 }
 
-// Linearized Callaway collisions in two kernels with one workgroup of T threads
-// per cell (up to kCwPer carriers per thread).  k_callaway_setup sums the
-// cell's Gram matrices and moments and builds the normal-gain projector into
-// a per-cell record; k_callaway runs the Taylor steps with the carriers' x in
-// registers, the record read as uniform (scalar) data.
+// Linearized Callaway collisions of a large cell (more than kCwWaveMax
+// carriers, listed by the one-wave kernels below) in two kernels with one
+// workgroup of T threads per cell (up to kCwPer carriers per thread).
+// k_callaway_setup sums the cell's Gram matrices and moments and builds the
+// normal-gain projector into a per-cell record; k_callaway runs the Taylor
+// steps with the carriers' x in registers, the record read as uniform data.
 //
 // Record columns (kCwRec doubles per cell).
 constexpr int kCwP = 0;           // 16: projector in basis-moment space (row-major)
@@ -2164,10 +2167,9 @@ constexpr int kCwExplicit = 25;   // 1: gain weights from the explicit direction
 constexpr int kCwEvolve = 26;     // 1: the cell evolves (nonzero x, rates and dt)
 
 template <int T>
-__device__ __forceinline__ void callaway_setup_body(const View& V, const Grid G) {
+__device__ __forceinline__ void callaway_setup_body(const View& V, const int c) {
     constexpr int kW = T / 64;
     __shared__ double lds[18 * kW + 18];
-    const int c = G.b;
     const int begin = V.cell_start[c], end = V.cell_start[c + 1];
     const int n = end - begin;
     const int k0 = V.chunk_start[c], k1 = V.chunk_start[c + 1];
@@ -2327,10 +2329,9 @@ __device__ __forceinline__ void callaway_setup_body(const View& V, const Grid G)
 }
 
 template <int T>
-__device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
+__device__ __forceinline__ void callaway_body(const View& V, const int c) {
     constexpr int kW = T / 64;
     __shared__ double lds[18 * kW + 18];
-    const int c = G.b;
     const int begin = V.cell_start[c], end = V.cell_start[c + 1];
     const int n = end - begin;
     if (n <= 0 || n > T * kCwPer) return;
@@ -2469,22 +2470,381 @@ __device__ __forceinline__ void callaway_body(const View& V, const Grid G) {
     }
 }
 
+// The workgroups of a context take the listed large cells in turn.
 template <int T>
-__global__ void __launch_bounds__(T) k_callaway_setup(View V) { callaway_setup_body<T>(V, launch_grid()); }
+__device__ __forceinline__ void callaway_big_setup(const View& V, const Grid G) {
+    const int count = min(V.st->cw_big, V.ncells);
+    for (int i = G.b; i < count; i += G.n) callaway_setup_body<T>(V, V.cw_big[i]);
+}
 template <int T>
-__global__ void __launch_bounds__(T) k_callaway(View V) { callaway_body<T>(V, launch_grid()); }
+__device__ __forceinline__ void callaway_big(const View& V, const Grid G) {
+    const int count = min(V.st->cw_big, V.ncells);
+    for (int i = G.b; i < count; i += G.n) callaway_body<T>(V, V.cw_big[i]);
+}
+
+template <int T>
+__global__ void __launch_bounds__(T) k_callaway_setup(View V) { callaway_big_setup<T>(V, launch_grid()); }
+template <int T>
+__global__ void __launch_bounds__(T) k_callaway(View V) { callaway_big<T>(V, launch_grid()); }
 
 __global__ void __launch_bounds__(kCwThreads) kb_callaway_setup(const View* __restrict__ views,
                                                                 const int* __restrict__ offs, int n) {
     Grid G;
     const int i = batch_instance(offs, n, G);
-    callaway_setup_body<kCwThreads>(views[i], G);
+    callaway_big_setup<kCwThreads>(views[i], G);
 }
 
 __global__ void __launch_bounds__(kCwThreads) kb_callaway(const View* __restrict__ views, const int* __restrict__ offs, int n) {
     Grid G;
     const int i = batch_instance(offs, n, G);
-    callaway_body<kCwThreads>(views[i], G);
+    callaway_big<kCwThreads>(views[i], G);
+}
+
+// ---- one wavefront per cell
+// A double from the lane that DPP control `kCtrl` names (both 32-bit halves).
+template <int kCtrl>
+__device__ __forceinline__ double dpp_double(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E;          // quad_perm [1,0,3,2], [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141, kDppMirror = 0x140; // within 8 and 16 lanes
+
+// Sum over the wavefront as one wave-uniform value: in each row of 16 lanes,
+// partner exchanges (xor 1, xor 2, half-row mirror, row mirror) leave every
+// lane with the same row sum (each stage adds two equal partials in either
+// order), then the four row sums are read out of lanes 0, 16, 32 and 48.
+template <int K>
+__device__ __forceinline__ void wave_sum(double (&v)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppXor1>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppXor2>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppHalfMirror>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppMirror>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        v[k] = ((read_lane(v[k], 0) + read_lane(v[k], 16)) + read_lane(v[k], 32)) + read_lane(v[k], 48);
+}
+__device__ __forceinline__ double wave_max(double a) {
+    a = fmax(a, dpp_double<kDppXor1>(a));
+    a = fmax(a, dpp_double<kDppXor2>(a));
+    a = fmax(a, dpp_double<kDppHalfMirror>(a));
+    a = fmax(a, dpp_double<kDppMirror>(a));
+    return fmax(fmax(read_lane(a, 0), read_lane(a, 16)), fmax(read_lane(a, 32), read_lane(a, 48)));
+}
+
+// The whole collision of a cell of at most 64 P carriers (the setup and the
+// Taylor steps of the large-cell kernels, same arithmetic per carrier) on one
+// wavefront: lane l holds carriers l, l + 64, ... with their gain weights.
+// Cells of lo < n <= 64 P carriers are this class's (lo < 0 takes empty cells
+// too); with `list`, a larger cell is appended to the large-cell list.
+template <int P>
+__device__ __forceinline__ void callaway_wave_body(const View& V, const int c, const int lo, const bool list) {
+    const int lane = __lane_id();
+    const int begin = V.cell_start[c], end = V.cell_start[c + 1];
+    const int n = end - begin;
+    if (n > 64 * P) {
+        if (list && lane == 0) {
+            const int slot = atomicAdd(&V.st->cw_big, 1);
+            if (slot < V.ncells) V.cw_big[slot] = c;
+        }
+        return;
+    }
+    if (n <= lo) return;
+    const int k0 = V.chunk_start[c], k1 = V.chunk_start[c + 1];
+    for (int k = k0 + lane; k < k1; k += 64) {
+        V.chunk_out[2 * static_cast<size_t>(k)] = 0.0;
+        V.chunk_out[2 * static_cast<size_t>(k) + 1] = 0.0;
+    }
+    if (n <= 0) return;
+    // Setup: x = e / sqrt(C), Gram matrices, largest rate and |x|, moments.
+    int md[P];
+    double x[P], dg[P], cr[P], w[P][4];
+    double acc[17];
+#pragma unroll
+    for (int k = 0; k < 17; ++k) acc[k] = 0.0;
+    double rmax = 0.0, xmax = 0.0;
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        const int j = lane + 64 * r;
+        md[r] = -1;
+        x[r] = dg[r] = cr[r] = 0.0;
+        w[r][0] = w[r][1] = w[r][2] = w[r][3] = 0.0;
+        if (j >= n) continue;
+        const int m = V.mode[begin + j] & kModeMask;
+        md[r] = m;
+        const double4 p0 = V.cw_pack[2 * m], p1 = V.cw_pack[2 * m + 1];
+        const double e = p1.z * (V.occ[begin + j] - V.mtab[m].y);
+        if (!isfinite(e)) raise_error(V, PMC_ERR_INVALID_STATE, begin + j, nullptr, -1);
+        const double xi = e / p0.z;
+        x[r] = xi;
+        dg[r] = p0.x + p0.y;
+        cr[r] = p0.y * p0.z;
+        const double b[4] = {p0.z, p0.w, p1.x, p1.y};
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int d = a; d < 4; ++d) acc[q++] += p0.x * b[a] * b[d];
+        acc[10] += p0.y * b[0] * b[0];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            w[r][a] = p0.x * b[a];
+            acc[11 + a] += w[r][a] * xi;
+        }
+        acc[15] += p0.y * b[0] * xi;
+        acc[16] += xi * xi;
+        rmax = fmax(rmax, p0.x + p0.y);
+        xmax = fmax(xmax, fabs(xi));
+    }
+    wave_sum<17>(acc);
+    rmax = wave_max(rmax);
+    xmax = wave_max(xmax);
+    // Normal-gain directions, as in callaway_setup_body: Gram space when every
+    // direction keeps a length of at least 1e-3, else explicit vectors.
+    double Gm[4][4], Q[4][4];
+    bool keep[4];
+    bool explicit_dirs = false;
+    {
+        int q = 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int d = a; d < 4; ++d) {
+                Gm[a][d] = acc[q];
+                Gm[d][a] = acc[q];
+                ++q;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        keep[k] = false;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Q[k][i] = 0.0;
+    }
+    for (int k = 0; k < 4; ++k) {
+        if (!(Gm[k][k] > 0.0)) continue;
+        double cf[4] = {0.0, 0.0, 0.0, 0.0};
+        cf[k] = 1.0 / sqrt(Gm[k][k]);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int j = 0; j < k; ++j) {
+                if (!keep[j]) continue;
+                const double proj = cw_inner(Gm, Q[j], cf);
+                for (int i = 0; i < 4; ++i) cf[i] -= proj * Q[j][i];
+            }
+        const double len2 = cw_inner(Gm, cf, cf);
+        if (!(len2 >= 1e-6)) {
+            explicit_dirs = true;
+            break;
+        }
+        const double inv = 1.0 / sqrt(len2);
+        for (int i = 0; i < 4; ++i) Q[k][i] = cf[i] * inv;
+        keep[k] = true;
+    }
+    double mom[4] = {acc[11], acc[12], acc[13], acc[14]};
+    double Pm[4][4];
+    if (explicit_dirs) {
+        // Directions in w (column q of carrier r), built column by column.
+        int nkeep = 0;
+#pragma unroll
+        for (int r = 0; r < P; ++r) w[r][0] = w[r][1] = w[r][2] = w[r][3] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!(Gm[k][k] > 0.0)) continue;
+            const double inv0 = 1.0 / sqrt(Gm[k][k]);
+            double cand[P];
+#pragma unroll
+            for (int r = 0; r < P; ++r) {
+                cand[r] = 0.0;
+                if (md[r] < 0) continue;
+                const double4 p0 = V.cw_pack[2 * md[r]], p1 = V.cw_pack[2 * md[r] + 1];
+                const double b = k == 0 ? p0.z : k == 1 ? p0.w : k == 2 ? p1.x : p1.y;
+                cand[r] = sqrt(p0.x) * b * inv0;
+            }
+            for (int pass = 0; pass < 2; ++pass)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (i >= nkeep) break;
+                    double proj[1] = {0.0};
+#pragma unroll
+                    for (int r = 0; r < P; ++r) proj[0] += w[r][i] * cand[r];
+                    wave_sum<1>(proj);
+#pragma unroll
+                    for (int r = 0; r < P; ++r) cand[r] -= proj[0] * w[r][i];
+                }
+            double len2[1] = {0.0};
+#pragma unroll
+            for (int r = 0; r < P; ++r) len2[0] += cand[r] * cand[r];
+            wave_sum<1>(len2);
+            if (!(len2[0] >= 1e-24)) continue; // rank-deficient cell geometry
+            const double inv = 1.0 / sqrt(len2[0]);
+#pragma unroll
+            for (int r = 0; r < P; ++r) {
+                const double v = cand[r] * inv;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q == nkeep) w[r][q] = v;
+            }
+            ++nkeep;
+        }
+        // Gain weights sqrt(g_N) times the directions; moments of x on them.
+#pragma unroll
+        for (int i = 0; i < 4; ++i) mom[i] = 0.0;
+#pragma unroll
+        for (int r = 0; r < P; ++r) {
+            if (md[r] < 0) continue;
+            const double sg = sqrt(V.cw_pack[2 * md[r]].x);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                w[r][i] = sg * w[r][i];
+                mom[i] += w[r][i] * x[r];
+            }
+        }
+        wave_sum<4>(mom);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int d = 0; d < 4; ++d) Pm[a][d] = (a == d && a < nkeep) ? 1.0 : 0.0;
+    } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                double v = 0.0;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (keep[k]) v += Q[k][a] * Q[k][d];
+                Pm[a][d] = v;
+            }
+    }
+    const double gr_inv = acc[10] > 0.0 ? 1.0 / acc[10] : 0.0;
+    const bool evolve = xmax > 0.0 && V.dt > 0.0 && rmax > 0.0;
+    if (evolve) {
+        const double inv = 1.0 / xmax;
+#pragma unroll
+        for (int r = 0; r < P; ++r) x[r] = x[r] * inv;
+        const double dt = V.dt;
+        const double steps_d = fmax(1.0, ceil(dt * rmax / 4.0));
+        if (!(steps_d <= 100000.0) || steps_d * n > 200000000.0) {
+            if (lane == 0) raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -9);
+            return;
+        }
+        const int steps = static_cast<int>(steps_d);
+        const double h = dt / steps, scaled_norm = h * rmax, tol = 2e-14 / steps;
+        double mN[4], mR = acc[15] * inv, xx = acc[16] * inv * inv;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) mN[a] = mom[a] * inv;
+        double sacc[P];
+        for (int st = 0; st < steps; ++st) {
+            const double reference = sqrt(xx);
+#pragma unroll
+            for (int r = 0; r < P; ++r) sacc[r] = x[r];
+            double sN[4] = {mN[0], mN[1], mN[2], mN[3]}, sR = mR;
+            bool converged = false;
+            double ss = xx;
+            for (int k = 1; k <= 40; ++k) {
+                double a4[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    a4[i] = Pm[i][0] * mN[0] + Pm[i][1] * mN[1] + Pm[i][2] * mN[2] + Pm[i][3] * mN[3];
+                const double rr = mR * gr_inv, hk = h / k;
+                double red[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int r = 0; r < P; ++r) {
+                    if (md[r] < 0) continue;
+                    const double act = -dg[r] * x[r] +
+                                       (w[r][0] * a4[0] + w[r][1] * a4[1] + w[r][2] * a4[2] + w[r][3] * a4[3]) +
+                                       cr[r] * rr;
+                    const double tn = act * hk;
+                    x[r] = tn;
+                    sacc[r] += tn;
+                    red[0] += w[r][0] * tn;
+                    red[1] += w[r][1] * tn;
+                    red[2] += w[r][2] * tn;
+                    red[3] += w[r][3] * tn;
+                    red[4] += cr[r] * tn;
+                    red[5] += tn * tn;
+                    red[6] += sacc[r] * sacc[r];
+                }
+                wave_sum<7>(red);
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    mN[a] = red[a];
+                    sN[a] += red[a];
+                }
+                mR = red[4];
+                sR += red[4];
+                ss = red[6];
+                if (scaled_norm < k + 2) {
+                    const double tail = sqrt(red[5]) * scaled_norm / (k + 1) / (1.0 - scaled_norm / (k + 2));
+                    if (tail <= tol * reference) {
+                        converged = true;
+                        break;
+                    }
+                }
+            }
+            if (!converged) {
+                if (lane == 0) raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -10);
+                return;
+            }
+#pragma unroll
+            for (int r = 0; r < P; ++r) x[r] = sacc[r];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) mN[a] = sN[a];
+            mR = sR;
+            xx = ss;
+        }
+    }
+    // Occupations from the evolved energies: n = n_eq + (x |x|max sqrt(C)) / hw.
+    double out[2] = {0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        if (md[r] < 0) continue;
+        const int i = begin + lane + 64 * r;
+        const int m = md[r];
+        const double b0 = V.cw_pack[2 * m].z, hw = V.cw_pack[2 * m + 1].z;
+        const double2 mt = V.mtab[m];
+        const double old = V.occ[i];
+        const double evolved = evolve ? x[r] * xmax * b0 : hw * (old - mt.y);
+        const double next = mt.y + evolved / hw;
+        if (!isfinite(next)) raise_error(V, PMC_ERR_INVALID_STATE, i, nullptr, -1);
+        V.occ[i] = next;
+        out[0] += (kHbar * mt.x * (next - mt.y)) * V.w;
+        out[1] += (hw * next - hw * old) * V.w;
+    }
+    wave_sum<2>(out);
+    if (lane == 0) {
+        V.chunk_out[2 * static_cast<size_t>(k0)] = out[0];
+        V.chunk_out[2 * static_cast<size_t>(k0) + 1] = out[1];
+    }
+}
+
+// Four cells per 256-thread workgroup, one per wavefront.  Class 0: cells of
+// up to kCwTinyMax carriers (empty ones included); class 1: up to kCwWaveMax,
+// listing the larger cells.
+template <int kClass>
+__device__ __forceinline__ void callaway_wave_cells(const View& V, const Grid G) {
+    const int c = G.b * 4 + static_cast<int>(threadIdx.x >> 6);
+    if (c >= V.ncells) return;
+    if (kClass == 0) callaway_wave_body<kCwTinyPer>(V, c, -1, false);
+    else callaway_wave_body<kCwWavePer>(V, c, kCwTinyMax, true);
+}
+template <int kClass>
+__global__ void __launch_bounds__(256) k_callaway_wave(View V) { callaway_wave_cells<kClass>(V, launch_grid()); }
+template <int kClass>
+__global__ void __launch_bounds__(256) kb_callaway_wave(const View* __restrict__ views, const int* __restrict__ offs,
+                                                         int n) {
+    Grid G;
+    const int i = batch_instance(offs, n, G);
+    callaway_wave_cells<kClass>(views[i], G);
 }
 
 // ---------------------------------------------------------------- heat source
@@ -2625,6 +2985,9 @@ __device__ __forceinline__ void source_solve_body(const View& V, const Grid G, i
     const double factor = source_factor(V, half);
     if (c >= V.ncells) return;
     const bool lead = __lane_id() == 0;
+    // Integrated source time: every cell accrues rate * factor (the prescribed
+    // energy is the total rate times this sum).
+    if (c == 0 && lead) V.st->source_time += factor;
     double* cellv = V.hs_cell + static_cast<size_t>(kSrcCellCols) * c;
     const double pend = V.hs_pending[c] + V.hs_rate[c] * factor;
     if (!(pend > 0.0)) {
@@ -2886,6 +3249,8 @@ __device__ __forceinline__ void source_apply_body(const View& V, const Grid G, i
     if (threadIdx.x == 0) {
         if (half == 1) V.chunk_src[b] = r[0];
         atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->source_updates),
+                  static_cast<unsigned long long>(r[1]));
+        atomicAdd(reinterpret_cast<unsigned long long*>(&V.st->step_source_updates),
                   static_cast<unsigned long long>(r[1]));
     }
 }
@@ -3213,18 +3578,21 @@ void relax(const View& v, hipStream_t s) {
     hipLaunchKernelGGL(k_relax, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v);
 }
 void callaway(const View& v, hipStream_t s) {
+    const dim3 wg((v.ncells + 3) / 4), big(std::min(v.ncells, kCwBigBlocks));
+    hipLaunchKernelGGL(k_callaway_wave<0>, wg, dim3(256), 0, s, v);
+    hipLaunchKernelGGL(k_callaway_wave<1>, wg, dim3(256), 0, s, v);
     switch (v.cw_threads) {
     case 256:
-        hipLaunchKernelGGL(k_callaway_setup<256>, dim3(v.ncells), dim3(256), 0, s, v);
-        hipLaunchKernelGGL(k_callaway<256>, dim3(v.ncells), dim3(256), 0, s, v);
+        hipLaunchKernelGGL(k_callaway_setup<256>, big, dim3(256), 0, s, v);
+        hipLaunchKernelGGL(k_callaway<256>, big, dim3(256), 0, s, v);
         break;
     case 384:
-        hipLaunchKernelGGL(k_callaway_setup<384>, dim3(v.ncells), dim3(384), 0, s, v);
-        hipLaunchKernelGGL(k_callaway<384>, dim3(v.ncells), dim3(384), 0, s, v);
+        hipLaunchKernelGGL(k_callaway_setup<384>, big, dim3(384), 0, s, v);
+        hipLaunchKernelGGL(k_callaway<384>, big, dim3(384), 0, s, v);
         break;
     default:
-        hipLaunchKernelGGL(k_callaway_setup<kCwThreads>, dim3(v.ncells), dim3(kCwThreads), 0, s, v);
-        hipLaunchKernelGGL(k_callaway<kCwThreads>, dim3(v.ncells), dim3(kCwThreads), 0, s, v);
+        hipLaunchKernelGGL(k_callaway_setup<kCwThreads>, big, dim3(kCwThreads), 0, s, v);
+        hipLaunchKernelGGL(k_callaway<kCwThreads>, big, dim3(kCwThreads), 0, s, v);
         break;
     }
 }
@@ -3260,7 +3628,13 @@ int batch_blocks(const View& v, int kind) {
     case kBatchMoments:
     case kBatchRelax: return v.max_chunks;
     case kBatchNewton: return (v.ncells + 3) / 4;
-    case kBatchCallaway: return v.ncells;
+    case kBatchCallawayWave: return (v.ncells + 3) / 4;
+    // Workgroups walking the large-cell list: as many as cells when the
+    // population can fill cells beyond one wavefront's reach, else a few.
+    case kBatchCallaway:
+        return static_cast<long long>(v.capacity) > static_cast<long long>(kCwTinyMax) * v.ncells
+                   ? std::min(v.ncells, 256)
+                   : std::min(v.ncells, 4);
     case kBatchSourceSums0:
     case kBatchSourceSums1:
     case kBatchSourceIncrements:
@@ -3297,6 +3671,10 @@ void batch_launch(int kind, bool mesh, const View* views, const int* offs, int n
     case kBatchMoments: hipLaunchKernelGGL(kb_moments, g, dim3(kChunkThreads), 0, s, views, offs, n); break;
     case kBatchNewton: hipLaunchKernelGGL(kb_newton, g, dim3(256), 0, s, views, offs, n); break;
     case kBatchRelax: hipLaunchKernelGGL(kb_relax, g, dim3(kChunkThreads), 0, s, views, offs, n); break;
+    case kBatchCallawayWave:
+        hipLaunchKernelGGL(kb_callaway_wave<0>, g, dim3(256), 0, s, views, offs, n);
+        hipLaunchKernelGGL(kb_callaway_wave<1>, g, dim3(256), 0, s, views, offs, n);
+        break;
     case kBatchCallaway:
         hipLaunchKernelGGL(kb_callaway_setup, g, dim3(kCwThreads), 0, s, views, offs, n);
         hipLaunchKernelGGL(kb_callaway, g, dim3(kCwThreads), 0, s, views, offs, n);

@@ -17,7 +17,8 @@ enum BatchKind {
     kBatchMoments,
     kBatchNewton,
     kBatchRelax,
-    kBatchCallaway,
+    kBatchCallawayWave,     // one wavefront per cell of at most kCwWaveMax carriers
+    kBatchCallaway,         // the listed larger cells
     kBatchRefresh2,
     kBatchSourceSums0,      // heat-source half step before transport
     kBatchSourceSolve0,

@@ -125,7 +125,8 @@ class Ledger(C.Structure):
                 ("newton_evals", C.c_int64), ("step_source_energy_ev", C.c_double),
                 ("total_source_energy_ev", C.c_double), ("source_updates", C.c_int64),
                 ("source_exact_passes", C.c_int64), ("interface_events", C.c_int64),
-                ("interface_transmitted", C.c_int64)]
+                ("interface_transmitted", C.c_int64), ("step_source_updates", C.c_int64),
+                ("source_time_ps", C.c_double), ("source_pending_ev", C.c_double)]
 
 
 class Error(C.Structure):
@@ -181,6 +182,7 @@ def lib():
                                           C.POINTER(C.c_double), C.c_char_p, C.POINTER(C.c_double),
                                           C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
         "pmcs_problem_cells": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "pmcs_problem_write_summary": (C.c_int, [P, C.c_char_p, C.c_char_p, C.c_int32]),
         "pmcs_solver_create": (C.c_int, [P, C.c_char_p, C.POINTER(P), C.c_char_p, C.c_int32]),
         "pmcs_solver_run": (C.c_int, [P, C.c_int32, C.c_char_p, C.c_int32]),
         "pmcs_solver_timestep": (C.c_int32, [P]),
@@ -203,6 +205,7 @@ def lib():
         "pmc_step": (C.c_int, [P, C.c_int32]),
         "pmc_read_cells": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "pmc_read_heat_flux": (C.c_int, [P, C.POINTER(C.c_double)]),
+        "pmc_read_source_pending": (C.c_int, [P, C.POINTER(C.c_double)]),
         "pmc_read_ledger": (C.c_int, [P, C.POINTER(Ledger)]),
         "pmc_read_carriers": (C.c_int, [P, C.POINTER(C.c_int64), C.POINTER(C.c_float), C.POINTER(C.c_int32),
                                         C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
@@ -337,6 +340,12 @@ class Problem:
         return dict(normals=normals.reshape(k, 3), centroids=cents.reshape(k, 3), areas=areas,
                     conditions=cond.raw[:k].decode(), shifts=shifts.reshape(k, 3), values=values, faces=faces)
 
+    def write_summary(self, folder):
+        """summary.txt and grid_centers.csv of the problem in `folder` (host only)."""
+        err = C.create_string_buffer(512)
+        if lib().pmcs_problem_write_summary(self.h, folder.encode(), err, 512) != 0:
+            raise RuntimeError(err.value.decode())
+
     def cells(self):
         n = int(self.scalars()["cells"])
         centers, volumes = np.zeros(3 * n), np.zeros(n)
@@ -440,6 +449,12 @@ class Context:
         q = np.zeros(3 * self.n_cells)
         self._check(lib().pmc_read_heat_flux(self.h, _ptr(q, C.c_double)))
         return q.reshape(-1, 3)
+
+    def source_pending(self):
+        """Heat-source energy pending per cell (eV)."""
+        out = np.zeros(self.n_cells)
+        self._check(lib().pmc_read_source_pending(self.h, _ptr(out, C.c_double)))
+        return out
 
     def ledger(self):
         led = Ledger()

@@ -43,6 +43,7 @@ T_MID, T_HALF = 325.0, 75.0    # reservoir channel: (t_bg - T_MID) / T_HALF
 EV_PS_PER_WATT = 6.241509074e6  # eV/ps per W
 Y_UNIT_W = 1e-6                # the target is the rise per microwatt of source power
 Y_SCALE = 10.0                 # K per microwatt per unit of the operator output
+P_CAP_W = 5e-4                 # largest total source power a case runs at
 
 
 def sample_cases(n, seed):
@@ -57,6 +58,17 @@ def sample_cases(n, seed):
     return out
 
 
+def run_power_density(p):
+    """The power density a case runs at: the sampled one, or less when the
+    Gaussian's total power exceeds P_CAP_W.  Linearized transport is linear in
+    the source (a run at half the power has half the rise, carrier for
+    carrier), so the rise per microwatt does not depend on it; far above the cap
+    the linearized deviations drive occupations negative and the source
+    deposit fails (as the reference's does)."""
+    total = p["power"] * (2.0 * np.pi) ** 1.5 * (p["sigma"] * 1e-9) ** 3
+    return p["power"] * min(1.0, P_CAP_W / total)
+
+
 def case_text(p, carriers=None, seed=None, dt=DT):
     t = f"{p['t_bg']:.6f}"
     o = {"simulation.background_temperature": t, "simulation.lifetime_temperature": t,
@@ -64,7 +76,7 @@ def case_text(p, carriers=None, seed=None, dt=DT):
          "boundary.boundary_values": f"[{t}, 10.0]",
          "heat_source.center": f"[{FIN_XY_NM[0]}, {FIN_XY_NM[1]}, {FIN_BASE_NM + p['height']:.6f}]",
          "heat_source.sigma": f"[{p['sigma']:.6f}, {p['sigma']:.6f}, {p['sigma']:.6f}]",
-         "heat_source.power_density": f"{p['power']:.6e}"}
+         "heat_source.power_density": f"{run_power_density(p):.6e}"}
     if carriers is not None:
         o["simulation.particle_count"] = int(carriers)
     if seed is not None:
@@ -122,12 +134,6 @@ def features(problem, p, geom):
     return f
 
 
-def target(problem, p, geom, cell_t):
-    """Operator target on the grid: (T - t_bg) per microwatt, in units of Y_SCALE."""
-    watts, _ = source_power(problem)
-    return geom.to_grid((cell_t - p["t_bg"]) / (watts / Y_UNIT_W) / Y_SCALE)
-
-
 def field_from_prediction(problem, p, geom, pred):
     """Per-cell initial temperatures from an operator output row."""
     watts, _ = source_power(problem)
@@ -153,20 +159,24 @@ def generate(cases, carriers, burn_in, average, sample=10, seed0=500000, dt=DT, 
     ctx = [rt.Context(p) for p in problems]
     log(f"[gen] contexts initialized at {time.perf_counter() - t0:.1f} s")
     batch = rt.Batch(ctx)
+    last = time.perf_counter()
     done = 0
-    while done < burn_in:
-        k = min(500, burn_in - done)
+    while done < burn_in:  # short launches, a progress line every few seconds
+        k = min(10, burn_in - done)
         batch.step(k)
         done += k
-        log(f"[gen] burn-in {done}/{burn_in} steps at {time.perf_counter() - t0:.1f} s")
+        if time.perf_counter() - last > 5.0 or done == burn_in:
+            last = time.perf_counter()
+            log(f"[gen] burn-in {done}/{burn_in} steps at {last - t0:.1f} s")
     acc = [np.zeros(c.n_cells) for c in ctx]
     nsamp = average // sample
     for s in range(nsamp):
         batch.step(sample)
         for k, c in enumerate(ctx):
             acc[k] += c.cells()[0]
-        if s % 50 == 49:
-            log(f"[gen] averaging {s + 1}/{nsamp} samples at {time.perf_counter() - t0:.1f} s")
+        if time.perf_counter() - last > 5.0 or s == nsamp - 1:
+            last = time.perf_counter()
+            log(f"[gen] averaging {s + 1}/{nsamp} samples at {last - t0:.1f} s")
     batch.close()
     fields = np.array([a / nsamp for a in acc])
     for c in ctx:
@@ -174,47 +184,76 @@ def generate(cases, carriers, burn_in, average, sample=10, seed0=500000, dt=DT, 
     return problems, fields
 
 
-def quantize(rows):
-    """Rows of floats as uint8 with a per-row offset and scale (8 bits of each row's range)."""
-    lo = rows.min(axis=1)
-    hi = rows.max(axis=1)
-    scale = np.where(hi > lo, (hi - lo) / 255.0, 1.0)
-    q = np.clip(np.rint((rows - lo[:, None]) / scale[:, None]), 0, 255).astype(np.uint8)
-    return q, lo, scale
+def quadrant(geom):
+    """Grid indices of the solid voxels of the x < nx/2, y < ny/2 quadrant and of
+    their three mirror images (the fin, its substrate, the source and the
+    reservoir are symmetric under x -> nx-1-x and y -> ny-1-y)."""
+    X, Y, Z = GRID
+    solid = geom.mask.reshape(GRID) > 0
+    if not (np.array_equal(solid, solid[::-1]) and np.array_equal(solid, solid[:, ::-1])):
+        raise ValueError("the cell set is not mirror-symmetric")
+    x, y, z = np.nonzero(solid[: X // 2, : Y // 2])
+    flat = lambda a, b: (a * Y + b) * Z + z  # noqa: E731
+    return np.stack([flat(x, y), flat(X - 1 - x, y), flat(x, Y - 1 - y), flat(X - 1 - x, Y - 1 - y)])
 
 
-def dequantize(q, lo, scale):
-    return lo[:, None] + q.astype(np.float64) * scale[:, None]
+def fold(grid_rows, quad):
+    """Mirror average of full-grid rows [n, S] onto the quadrant [n, q]."""
+    g = np.asarray(grid_rows, dtype=np.float64)
+    return g[:, quad].mean(axis=1)
 
 
-def save_dataset(path, cases, rises, meta):
-    """rises: [n][n_cells] steady T - t_bg (K) in the solver's cell order."""
-    q, lo, scale = quantize(np.asarray(rises))
-    params = np.array([[c[k] for k in PARAM_KEYS] for c in cases])
-    np.savez_compressed(path, params=params, rise_q=q, rise_lo=lo, rise_scale=scale,
-                        **{k: np.asarray(v) for k, v in meta.items()})
+def unfold(quad_rows, quad):
+    """Quadrant rows [n, q] back onto the full grid [n, S] (zero outside the solid)."""
+    q = np.asarray(quad_rows)
+    out = np.zeros((q.shape[0], int(np.prod(GRID))), dtype=np.float32)
+    for k in range(4):
+        out[:, quad[k]] = q
+    return out
 
 
-def load_dataset(path):
+def pack_dataset(raw_paths, out, geometry_problem):
+    """The committed data set from raw generator files (np.savez of params,
+    watts, rise [n, n_cells]): each case's steady rise per microwatt of source
+    power on the grid, mirror-averaged onto the quadrant (which also halves the
+    Monte Carlo noise), stored as 16 bits per value with a per-case offset and
+    scale."""
+    geom = Geometry(geometry_problem)
+    quad = quadrant(geom)
+    params, g = [], []
+    for path in raw_paths:
+        d = np.load(path, allow_pickle=False)
+        per_uw = d["rise"].astype(np.float64) / (d["watts"][:, None] / Y_UNIT_W)
+        g.append(fold(np.stack([geom.to_grid(r) for r in per_uw]), quad))
+        params.append(d["params"])
+    g, params = np.concatenate(g), np.concatenate(params)
+    lo, hi = g.min(axis=1), g.max(axis=1)
+    scale = np.where(hi > lo, (hi - lo) / 65535.0, 1.0)
+    q = np.clip(np.rint((g - lo[:, None]) / scale[:, None]), 0, 65535).astype(np.uint16)
+    np.savez_compressed(out, params=params, g_q=q, g_lo=lo, g_scale=scale, dt=DT, config=CONFIG)
+
+
+def load_dataset(path, geometry_problem):
+    """(cases, rise per microwatt [n, S] on the full grid, geometry)."""
     d = np.load(path, allow_pickle=False)
     cases = [dict(zip(PARAM_KEYS, map(float, row))) for row in d["params"]]
-    return cases, dequantize(d["rise_q"], d["rise_lo"], d["rise_scale"]), d
-
-
-def training_arrays(cases, rises, geometry_problem):
-    """Inputs [n, S, 3], targets [n, S] and weights (the solid mask) [n, S] of a
-    data set; features and power scales from each case's own problem (cheap:
-    the geometry caches are warm)."""
     geom = Geometry(geometry_problem)
+    g = d["g_lo"][:, None] + d["g_q"].astype(np.float64) * d["g_scale"][:, None]
+    return cases, unfold(g, quadrant(geom)), geom
+
+
+def training_arrays(cases, per_uw, geom, problems=None):
+    """Inputs [n, S, 3], targets (rise per microwatt / Y_SCALE) [n, S] and
+    weights (the solid mask) [n, S]."""
     n, S = len(cases), int(np.prod(GRID))
     x = np.zeros((n, S, 3), dtype=np.float32)
-    y = np.zeros((n, S), dtype=np.float32)
-    problems = build_problems(cases, 20000, 1)
+    if problems is None:
+        problems = build_problems(cases, 20000, 1)
     for k, (p, pr) in enumerate(zip(cases, problems)):
         x[k] = features(pr, p, geom)
-        y[k] = target(pr, p, geom, p["t_bg"] + rises[k])
+    y = (np.asarray(per_uw) / Y_SCALE).astype(np.float32)
     w = np.broadcast_to(geom.mask, (n, S)).astype(np.float32)
-    return x, y, w, geom
+    return x, y, w
 
 
 class WarmStart3d:
@@ -240,7 +279,9 @@ class WarmStart3d:
         return self.op.predict(x)
 
     def initial_field(self, problem, p, geom):
-        pred = self.predict(features(problem, p, geom)[None])[0]
+        """Per-cell initial temperatures: the prediction, mirror-averaged."""
+        quad = quadrant(geom)
+        pred = unfold(fold(self.predict(features(problem, p, geom)[None]), quad), quad)[0]
         return field_from_prediction(problem, p, geom, pred)
 
 
@@ -266,59 +307,122 @@ def run_windows(problem, steps, window, sample, initial_field=None, seed=None):
         ctx.close()
 
 
-def time_to_steady(avg, t_ref, tol=0.5):
-    rms = np.sqrt(np.mean((avg - t_ref[None, :]) ** 2, axis=1))
-    hit = np.nonzero(rms <= tol)[0]
-    return (int(hit[0]) if len(hit) else None), rms
+def first_below(err, tol):
+    hit = np.nonzero(np.asarray(err) <= tol)[0]
+    return int(hit[0]) if len(hit) else None
 
 
-def evaluate(model, p, carriers, ref_burn_in, ref_average, run_steps, window, sample=10, tol=0.5, seed=4242,
-             log=None):
-    """Plain against warm-started Monte Carlo on one case at matched statistical
-    error: the same carriers, windows and sampling.  T_ref is the mean of a long
-    plain run after `ref_burn_in` steps; the plain run starts from t_bg, the
-    warm run from the operator's field (prediction time counted)."""
+def reference_fields(cases, carriers, burn_in, average, sample=10, seed0=900000, log=None):
+    """T_ref of each case: the mean of a plain run (its own seed) over `average`
+    steps after `burn_in`, the cases stepped together as one batch."""
+    problems = build_problems(cases, carriers, seed0)
+    ctx = [rt.Context(pr) for pr in problems]
+    batch = rt.Batch(ctx)
+    try:
+        done = 0
+        while done < burn_in:
+            k = min(500, burn_in - done)
+            batch.step(k)
+            done += k
+            if log:
+                log(f"[ref] burn-in {done}/{burn_in}")
+        acc = [np.zeros(c.n_cells) for c in ctx]
+        for _ in range(average // sample):
+            batch.step(sample)
+            for a, c in zip(acc, ctx):
+                a += c.cells()[0]
+        return [a / (average // sample) for a in acc]
+    finally:
+        batch.close()
+        for c in ctx:
+            c.close()
+
+
+def evaluate(model, p, carriers, t_ref, steps, window=20, sample=5, tol=0.5, seed=4242, log=None):
+    """Plain against warm-started Monte Carlo on one case, with common random
+    numbers.  Three contexts share the seed, and therefore every carrier
+    position, mode and scattering draw: the transport is linear in the
+    occupations, so runs that start from different fields differ only by the
+    decaying image of their initial difference (the stationary noise is common
+    to all three).  The yardstick starts at T_ref (a long plain run's mean),
+    the plain run at t_bg, the warm run at the operator's field.  A run is
+    steady at the first window whose average is within `tol` K RMS of the
+    yardstick's same window; the plain and warm runs are timed by the wall
+    clock, each from context creation, the operator's prediction included."""
     problem = problem_of(p, carriers, seed)
     geom = Geometry(problem)
-    ref, _ = run_windows(problem, ref_burn_in + ref_average, window, sample, seed=seed)
-    t_ref = ref[ref_burn_in // window:].mean(axis=0)
-    plain, plain_wall = run_windows(problem, run_steps, window, sample, seed=seed + 1)
+    yard, _ = run_windows(problem, steps, window, sample, initial_field=t_ref, seed=seed)
+    plain, plain_wall = run_windows(problem, steps, window, sample, seed=seed)
     t0 = time.perf_counter()
     field = model.initial_field(problem, p, geom)
     t_pred = time.perf_counter() - t0
-    warm, warm_wall = run_windows(problem, run_steps, window, sample, initial_field=field, seed=seed + 2)
-    k_plain, rms_plain = time_to_steady(plain, t_ref, tol)
-    k_warm, rms_warm = time_to_steady(warm, t_ref, tol)
-    half = len(plain) // 2
-
-    def rms(a):
-        return float(np.sqrt(np.mean((a - t_ref) ** 2)))
-
+    warm, warm_wall = run_windows(problem, steps, window, sample, initial_field=field, seed=seed)
+    err_plain = np.sqrt(np.mean((plain - yard) ** 2, axis=1))
+    err_warm = np.sqrt(np.mean((warm - yard) ** 2, axis=1))
+    k_plain, k_warm = first_below(err_plain, tol), first_below(err_warm, tol)
+    rise = t_ref - p["t_bg"]
     out = {
         "case": p, "carriers": carriers, "window_steps": window, "tol_k": tol,
-        "steady_rms_rise_k": float(np.sqrt(np.mean((t_ref - p["t_bg"]) ** 2))),
-        "peak_rise_k": float((t_ref - p["t_bg"]).max()),
-        "plain_windows": None if k_plain is None else k_plain + 1,
-        "warm_windows": None if k_warm is None else k_warm + 1,
+        "steady_rms_rise_k": float(np.sqrt(np.mean(rise ** 2))), "peak_rise_k": float(rise.max()),
+        "operator_rms_k": float(np.sqrt(np.mean((field - t_ref) ** 2))),
+        "operator_rel_rms": float(np.sqrt(np.mean((field - t_ref) ** 2) / np.mean(rise ** 2))),
+        "plain_steps_to_steady": None if k_plain is None else (k_plain + 1) * window,
+        "warm_steps_to_steady": None if k_warm is None else (k_warm + 1) * window,
         "plain_seconds": None if k_plain is None else float(plain_wall[k_plain]),
         "warm_seconds": None if k_warm is None else float(warm_wall[k_warm] + t_pred),
         "predict_seconds": t_pred,
-        "operator_rms_k": rms(field),
-        "uniform_rms_k": rms(np.full_like(t_ref, p["t_bg"])),
-        "window_noise_rms_k": float(np.sqrt(np.mean((ref[ref_burn_in // window:] - t_ref) ** 2))),
-        "plain_late_rms_k": rms(plain[half:].mean(axis=0)),
-        "warm_late_rms_k": rms(warm[half:].mean(axis=0)),
-        "rms_plain": rms_plain.tolist(), "rms_warm": rms_warm.tolist(),
+        "err_plain_k": err_plain.tolist(), "err_warm_k": err_warm.tolist(),
     }
-    if out["plain_windows"] and out["warm_windows"]:
-        out["speedup_steps"] = out["plain_windows"] / out["warm_windows"]
+    if k_plain is not None and k_warm is not None:
+        out["speedup_steps"] = (k_plain + 1) / (k_warm + 1)
         out["speedup_wall"] = out["plain_seconds"] / out["warm_seconds"]
     if log:
-        log(f"[eval] steady RMS rise {out['steady_rms_rise_k']:.2f} K (peak {out['peak_rise_k']:.1f} K): operator "
-            f"{out['operator_rms_k']:.3f} K RMS from T_ref (t_bg start {out['uniform_rms_k']:.2f} K); 0.5 K RMS "
-            f"after {out['plain_windows']} plain vs {out['warm_windows']} warm windows of {window} steps; "
-            f"second-half RMS {out['plain_late_rms_k']:.3f} K plain, {out['warm_late_rms_k']:.3f} K warm")
+        log(f"[eval] RMS rise {out['steady_rms_rise_k']:.1f} K (peak {out['peak_rise_k']:.0f} K); operator "
+            f"{out['operator_rms_k']:.2f} K RMS ({100 * out['operator_rel_rms']:.2f} %); {tol} K RMS after "
+            f"{out['plain_steps_to_steady']} plain vs {out['warm_steps_to_steady']} warm steps; "
+            f"{out['plain_seconds']} vs {out['warm_seconds']} s")
     return out
+
+
+def evaluate_batch(model, cases, carriers, t_refs, steps, window=20, sample=5, tol=0.5, seed0=700000, log=None):
+    """evaluate() over many cases as one batch (yardstick, plain and warm
+    contexts of every case stepped together): steps to steady per case."""
+    problems = build_problems(cases, carriers, seed0)
+    geom = Geometry(problems[0])
+    fields = [model.initial_field(pr, p, geom) for pr, p in zip(problems, cases)]
+    ctx = []
+    for k, pr in enumerate(problems):
+        ctx += [rt.Context(pr, initial_field=t_refs[k]), rt.Context(pr), rt.Context(pr, initial_field=fields[k])]
+    batch = rt.Batch(ctx)
+    nwin = steps // window
+    err = np.zeros((len(cases), 2, nwin))
+    try:
+        for w in range(nwin):
+            acc = [np.zeros(c.n_cells) for c in ctx]
+            for _ in range(window // sample):
+                batch.step(sample)
+                for a, c in zip(acc, ctx):
+                    a += c.cells()[0]
+            for k in range(len(cases)):
+                y = acc[3 * k]
+                err[k, 0, w] = np.sqrt(np.mean((acc[3 * k + 1] - y) ** 2)) / (window // sample)
+                err[k, 1, w] = np.sqrt(np.mean((acc[3 * k + 2] - y) ** 2)) / (window // sample)
+            if log and (w + 1) % 25 == 0:
+                log(f"[eval] {(w + 1) * window}/{steps} steps")
+    finally:
+        batch.close()
+        for c in ctx:
+            c.close()
+    rows = []
+    for k, p in enumerate(cases):
+        rise = t_refs[k] - p["t_bg"]
+        kp, kw = first_below(err[k, 0], tol), first_below(err[k, 1], tol)
+        rows.append({"case": p, "steady_rms_rise_k": float(np.sqrt(np.mean(rise ** 2))),
+                     "operator_rms_k": float(np.sqrt(np.mean((fields[k] - t_refs[k]) ** 2))),
+                     "operator_rel_rms": float(np.sqrt(np.mean((fields[k] - t_refs[k]) ** 2) / np.mean(rise ** 2))),
+                     "plain_steps_to_steady": None if kp is None else (kp + 1) * window,
+                     "warm_steps_to_steady": None if kw is None else (kw + 1) * window})
+    return rows, err
 
 
 def main(argv=None):
@@ -327,16 +431,19 @@ def main(argv=None):
     ap.add_argument("--cases", type=int, default=768)
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--carriers", type=int, default=250000)
-    ap.add_argument("--burn-in", type=int, default=4000)
+    ap.add_argument("--burn-in", type=int, default=3000)
     ap.add_argument("--average", type=int, default=1000)
+    ap.add_argument("--first", type=int, default=0, help="index of the first case of the sample to run")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
-    cases = sample_cases(a.cases, a.seed)
+    cases = sample_cases(a.first + a.cases, a.seed)[a.first:]
     log = lambda m: print(m, flush=True)  # noqa: E731
-    problems, fields = generate(cases, a.carriers, a.burn_in, a.average, log=log)
+    problems, fields = generate(cases, a.carriers, a.burn_in, a.average, seed0=500000 + a.first, log=log)
     rises = fields - np.array([c["t_bg"] for c in cases])[:, None]
-    save_dataset(a.out, cases, rises, dict(seed=a.seed, carriers=a.carriers, burn_in=a.burn_in,
-                                           average=a.average, dt=DT, config=CONFIG))
+    watts = np.array([source_power(p)[0] for p in problems])
+    np.savez(a.out, params=np.array([[c[k] for k in PARAM_KEYS] for c in cases]), watts=watts,
+             rise=rises.astype(np.float16), seed=a.seed, first=a.first, carriers=a.carriers, burn_in=a.burn_in,
+             average=a.average, dt=DT)
     log(f"[gen] wrote {a.out}: {len(cases)} cases, RMS rise per case {np.sqrt((rises ** 2).mean(axis=1)).min():.3g}"
         f" .. {np.sqrt((rises ** 2).mean(axis=1)).max():.3g} K")
 

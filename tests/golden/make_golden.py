@@ -381,7 +381,45 @@ def whole_run(key, n, steps, seeds, interval, **overrides):
     return out
 
 
+OUTPUT_RUN = dict(key="c1cs", n=20000, steps=200, interval=50, seed=1000)
+# Inputs whose domain summary (summary.txt at start-up, grid_centers.csv) is kept.
+SUMMARY_WORKLOADS = ["c1", "c2", "c1cs", "c1l", "finfet", "finfet_cw"]
+SUMMARY_OVERRIDES = {"finfet_cw": {"simulation.grid_xyz": "[20, 10, 15]"}}
+
+
+def output_files():
+    """One reference run of the heated Callaway film writing its output files:
+    the convergence.txt header and rows, heat_source_ledger.txt and the run
+    summary text (ResultWriter::format_run_summary)."""
+    o = OUTPUT_RUN
+    tmp = tempfile.mkdtemp()
+    text = rt.config_text(WORKLOADS[o["key"]], **{"simulation.particle_count": o["n"],
+                                                  "simulation.iterations": o["steps"],
+                                                  "simulation.random_seed": o["seed"],
+                                                  "simulation.convergence_write_interval": o["interval"],
+                                                  "io.output_folder": f'"{tmp}/out"'})
+    path = write_input(text, tmp, "run.toml")
+    problem = rt.Problem(text, input_directory=rt.CONFIGS)
+    summary = R.run_materials(path, [problem.material()], o["steps"], threads=8)[4]
+    lines = open(os.path.join(tmp, "out", "convergence.txt")).read().splitlines()
+    rows = np.array([[float(x) for x in line.split()] for line in lines[1:]])
+    ledger = open(os.path.join(tmp, "out", "heat_source_ledger.txt")).read()
+    domain = {}
+    for key in SUMMARY_WORKLOADS:
+        t = rt.config_text(WORKLOADS[key], **SUMMARY_OVERRIDES.get(key, {}))
+        d = tempfile.mkdtemp()
+        R.domain_summary(write_input(t, d, "run.toml"), os.path.join(d, "out"))
+        domain["domain_summary_" + key] = np.array(open(os.path.join(d, "out", "summary.txt")).read())
+        domain["grid_centers_" + key] = np.array(open(os.path.join(d, "out", "grid_centers.csv")).read())
+    return dict(convergence_header=np.array(lines[0]), convergence=rows, source_ledger=np.array(ledger),
+                summary=np.array(summary), **domain, **{k: np.array(v) for k, v in o.items()})
+
+
 def main():
+    if "--outputs" in sys.argv:
+        np.savez_compressed(os.path.join(OUT, "outputs_c1cs.npz"), **output_files())
+        print("output-file fixture written to", OUT)
+        return
     if "--layered" in sys.argv:
         rng = np.random.default_rng(20261207)
         text = rt.config_text(WORKLOADS["c1l"])

@@ -263,3 +263,22 @@ def test_autotuned_launch_changes_no_result(cfg, n):
     assert b.launch_config() == {"event_split": 2, "callaway_threads": 384}
     with pytest.raises(RuntimeError, match="event_split"):
         b.set_launch_config(3, 512)
+
+
+def test_batch_of_many_finfets_splits_its_launches():
+    """96 heated FinFET instances (linearized Callaway, dt 1 ps, 2e4 carriers):
+    the heat-source kernels take one workgroup per carrier chunk, about 2.4e4
+    per instance, so their batched grid exceeds one launch's 2^21 workgroups
+    and is split; sampled instances still end where they end stepped alone."""
+    from phonomc_amd import warmstart3d as w3
+    cases = w3.sample_cases(96, 17)
+    problems = [w3.problem_of(p, 20000, 300 + k) for k, p in enumerate(cases)]
+    ca = [rt.Context(p) for p in problems]
+    batch = rt.Batch(ca)
+    batch.step(6)
+    for k in (0, 47, 95):
+        alone = rt.Context(problems[k])
+        alone.step(6)
+        assert_same(ca[k], alone)
+        alone.close()
+    batch.close()

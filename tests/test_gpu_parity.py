@@ -124,9 +124,10 @@ def test_steps_match_oracle(cfg, n, sampling, series):
         lg, lo = g.ledger(), o.ledger()
         for k in ["step_absorbed", "step_injected", "total_absorbed", "total_injected", "rough_events",
                   "rough_specular", "rough_diffuse", "live_carriers", "source_updates", "interface_events",
-                  "interface_transmitted"]:
+                  "interface_transmitted", "step_source_updates"]:
             assert lg[k] == lo[k], k
-        for k in ["step_absorbed_energy_ev", "step_injected_energy_ev", "thermal_energy_ev", "step_source_energy_ev"]:
+        for k in ["step_absorbed_energy_ev", "step_injected_energy_ev", "thermal_energy_ev", "step_source_energy_ev",
+                  "source_time_ps", "source_pending_ev"]:
             assert lg[k] == pytest.approx(lo[k], rel=1e-11, abs=1e-9), k
     led = g.ledger()
     assert (led["interface_events"] > led["interface_transmitted"] > 0) == (p.layers() is not None)
