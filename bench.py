@@ -17,14 +17,15 @@ cpu_baseline object times the reference PhonoMC solver compiled from its own
 sources (oracle/_ref) on a bounded sample of the same workload on the host
 cores of this machine.
 
-The warm_start object is the second half of the metric: the spectral operator
-is trained on the device (BF16 MFMA) from the committed heated-film data set
-(phonomc_amd/data/heated_film_fields.npz), then plain and warm-started Monte
-Carlo of the hottest held-out case are timed to 0.5 K RMS of a long plain
-run's field (phonomc_amd/warmstart.py: evaluate).  Its roofline object is the
+The warm_start object is the second half of the metric: the 3-D spectral
+operator of the FinFET is trained on the device (BF16 MFMA) from the committed
+data set of batched-solver fields (phonomc_amd/data/finfet_fields.npz), then
+plain and warm-started Monte Carlo of a held-out case are timed, with common
+random numbers, to 0.5 K RMS of a run started at the case's long-run field
+(phonomc_amd/warmstart3d.py: evaluate).  Its roofline object is the
 operator's GEMM rate (algorithmic flops over HIP-event durations of the GEMM
 launches of forward + backward passes at the training batch) against the
-dense BF16 MFMA peak.
+dense BF16 MFMA peak.  --warm-start film runs the heated-film version instead.
 """
 import argparse
 import json
@@ -286,6 +287,48 @@ def warm_start_leg(epochs=None, evaluate=True):
     }
 
 
+FINFET_EVAL = dict(carriers=2000000, ref_burn_in=3000, ref_average=2000, steps=3000, window=20, sample=5)
+
+
+def finfet_warm_start_leg(epochs=48, evaluate=True):
+    """The FinFET warm start: the 3-D operator (64 x 32 x 32 grid, 3 channels, 4
+    spectral layers, width 32, 12 modes per axis, BF16 MFMA) trained on the
+    device from the committed data set of batched-solver fields
+    (phonomc_amd/data/finfet_fields.npz), then one held-out case (the median
+    rise of the held-out set) run plain and warm-started with common random
+    numbers to 0.5 K RMS of a run started at its long-run field
+    (phonomc_amd/warmstart3d.py: evaluate)."""
+    import numpy as np
+
+    from phonomc_amd import warmstart3d as w3
+
+    model, info, (x, y, w) = w3.train(epochs=epochs, log=log)
+    roofline = w3.gemm_roofline(model, x, y, w)
+    log(f"[bench] operator GEMMs: {roofline['achieved']:.2f} TFLOP/s over {roofline['launches_per_pass']:.0f} launches "
+        f"per forward + backward ({roofline['avg_launch_ms'] * 1e3:.1f} us per launch)")
+    out = {"workload": "FinFET (fin 10x40x100 nm on a 200x100x50 nm substrate, 64x32x32 grid, 23 536 cells), "
+                       "linearized Callaway, Gaussian source, bottom reservoir at t_bg, dt 1 ps",
+           "operator": {"grid": [64, 32, 32], "channels": 3, "layers": 4, "width": 32, "modes": 12, **info},
+           "roofline": roofline}
+    if not evaluate:
+        return out
+    cases = w3.load_dataset_cases(w3.DATA_PATH)[-w3.HELD_OUT:]
+    rms = np.sqrt((y[-w3.HELD_OUT:].astype(np.float64) ** 2).sum(axis=1))
+    case = cases[int(np.argsort(rms)[len(rms) // 2])]
+    e = FINFET_EVAL
+    t0 = time.perf_counter()
+    t_ref = w3.reference_fields([case], e["carriers"], e["ref_burn_in"], e["ref_average"], log=log)[0]
+    log(f"[bench] FinFET T_ref in {time.perf_counter() - t0:.1f} s")
+    ev = w3.evaluate(model, case, e["carriers"], t_ref, e["steps"], e["window"], e["sample"], log=log)
+    out.update({k: ev[k] for k in ("case", "carriers", "window_steps", "tol_k", "steady_rms_rise_k", "peak_rise_k",
+                                   "operator_rms_k", "operator_rel_rms", "plain_steps_to_steady",
+                                   "warm_steps_to_steady", "plain_seconds", "warm_seconds", "predict_seconds")})
+    out["speedup_steps"] = ev.get("speedup_steps")
+    out["speedup_wall"] = ev.get("speedup_wall")
+    out["reference"] = {"burn_in_steps": e["ref_burn_in"], "average_steps": e["ref_average"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -296,6 +339,8 @@ def main():
     ap.add_argument("--cpu-warmup", type=int, default=50, help="untimed steps of the CPU baseline first")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-warm-start", action="store_true", help="skip the warm-start leg")
+    ap.add_argument("--warm-start", choices=["finfet", "film"], default="finfet",
+                    help="warm-start leg: the FinFET operator (default) or the heated film")
     ap.add_argument("--workload", default=WORKLOAD, help="bundled TOML input (phonomc_amd/configs) to time")
     ap.add_argument("--particles", type=int, default=0, help="override simulation.particle_count")
     ap.add_argument("--autotune", type=int, default=0, metavar="STEPS",
@@ -319,7 +364,8 @@ def main():
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
     if args.surrogate_only:
-        print(json.dumps(warm_start_leg(epochs=args.surrogate_only, evaluate=False)), flush=True)
+        leg = warm_start_leg if args.warm_start == "film" else finfet_warm_start_leg
+        print(json.dumps(leg(epochs=args.surrogate_only, evaluate=False)), flush=True)
         return 0
 
     from phonomc_amd import runtime as rt
@@ -383,9 +429,9 @@ def main():
     warm = None
     if not args.no_warm_start:
         log("[bench] warm start: training the operator and timing plain vs warm-started runs")
-        warm = warm_start_leg()
-        log(f"[bench] warm start: {warm['plain_steps_to_steady']} vs {warm['warm_steps_to_steady']} steps to 0.5 K "
-            f"RMS ({warm['speedup_wall']:.1f}x wall); GEMM {warm['roofline']['achieved']:.2f} TFLOP/s")
+        warm = warm_start_leg() if args.warm_start == "film" else finfet_warm_start_leg()
+        log(f"[bench] warm start: {warm.get('plain_steps_to_steady')} vs {warm.get('warm_steps_to_steady')} steps to "
+            f"0.5 K RMS ({warm.get('speedup_wall')} x wall); GEMM {warm['roofline']['achieved']:.2f} TFLOP/s")
 
     out = {
         "metric": "particle-steps/s @1 GPU; time-to-steady T (0.5 K RMS) warm-start vs plain",

@@ -7,6 +7,7 @@
 #include <filesystem>
 #include <fstream>
 #include <limits>
+#include <map>
 #include <numeric>
 #include <stdexcept>
 
@@ -145,7 +146,7 @@ void validate_raw_material(const RawMaterial& d) {
         if (c <= 0) throw std::runtime_error("mesh entries must be positive integers");
         full *= static_cast<size_t>(c);
     }
-    if (full != static_cast<size_t>(d.qpoints))
+    if (full * d.bz_refinement != static_cast<size_t>(d.qpoints)) // a completed table holds bz_refinement copies
         throw std::runtime_error("Full Brillouin-zone grid required: qpoint count differs from "
                                  "mesh product. Expand reduced data first.");
     for (double x : d.frequency_thz)
@@ -233,13 +234,6 @@ ModeTables build_mode_tables(const RawMaterial& raw, double lookup_dt, bool incl
     if (!std::isfinite(lookup_dt) || lookup_dt <= 0.0)
         throw std::invalid_argument("Temperature lookup spacing must be finite and positive");
     validate_raw_material(raw);
-    // N/U data: the reference completes the equal-weight first-BZ quadrature at
-    // construction (src/PhononMaterial.cpp:38-40).  A refinement above one
-    // replicates zone-boundary q points, which this build does not do.
-    if (!raw.gamma_normal.empty() && callaway_bz_refinement(raw) != 1)
-        throw std::runtime_error("N/U material needs equal-weight Brillouin-zone completion (q points on the zone "
-                                 "boundary), which this build does not support; use a mesh without zone-boundary "
-                                 "points (an odd mesh).");
     ModeTables t;
     t.include_stationary = include_stationary;
     t.qpoints = raw.qpoints;
@@ -282,19 +276,23 @@ ModeTables build_mode_tables(const RawMaterial& raw, double lookup_dt, bool incl
                                           (v[2] * kg[2]) * (v[2] * kg[2]));
         }
     }
-    // Active modes: |v| > 1e-12 (or omega > 0 with stationary modes), q-major
-    // then branch (src/PhononMaterial.cpp:140-160).
+    // Active modes: |v| > 1e-12 (or omega > 0 with stationary modes), by mesh
+    // point, then branch, then the point's first-BZ copies, so that the copies
+    // of a mode sit together (src/PhononMaterial.cpp:140-160; one copy without
+    // completion).
     t.flat_to_active.assign(nm, -1);
-    for (int q = 0; q < raw.qpoints; ++q)
-        for (int b = 0; b < raw.branches; ++b) {
-            const int fi = q * raw.branches + b;
-            const auto& v = t.velocity_flat[fi];
-            if (include_stationary ? t.omega_flat[fi] > 0.0
-                                   : std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) > 1e-12) {
-                t.flat_to_active[fi] = static_cast<int>(t.active_flat.size());
-                t.active_flat.push_back(fi);
+    const int copies = static_cast<int>(raw.bz_refinement);
+    for (int base = 0; base < raw.qpoints; base += copies)
+        for (int b = 0; b < raw.branches; ++b)
+            for (int copy = 0; copy < copies; ++copy) {
+                const int fi = (base + copy) * raw.branches + b;
+                const auto& v = t.velocity_flat[fi];
+                if (include_stationary ? t.omega_flat[fi] > 0.0
+                                       : std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) > 1e-12) {
+                    t.flat_to_active[fi] = static_cast<int>(t.active_flat.size());
+                    t.active_flat.push_back(fi);
+                }
             }
-        }
     t.n_active = static_cast<int>(t.active_flat.size());
     if (t.n_active == 0) throw std::runtime_error("No active modes found in material data.");
     t.energy_normalization = std::max(1.0, static_cast<double>(raw.qpoints) * raw.volume_a3);
@@ -350,10 +348,12 @@ double length2(const std::array<std::array<double, 3>, 3>& b, const std::array<d
 }
 } // namespace
 
-unsigned callaway_bz_refinement(const RawMaterial& d) {
-    // src/material/BzQuadrature.cpp:11-110: complete Gamma-centred mesh, no
-    // duplicates, first-BZ representatives; refinement = lcm of the numbers of
-    // equally short representatives.
+namespace {
+// src/material/BzQuadrature.cpp:11-110: complete Gamma-centred mesh, no
+// duplicates, first-BZ representatives; refinement = lcm of the numbers of
+// equally short representatives.  reps (optional) receives each q point's
+// representatives in the order of the search (i, j, k ascending).
+unsigned bz_representatives(const RawMaterial& d, std::vector<std::vector<std::array<double, 3>>>* reps) {
     const auto& b = d.reciprocal;
     const double det = det3(b);
     if (!std::isfinite(det) || det == 0) throw std::invalid_argument("Callaway reciprocal lattice is singular.");
@@ -386,21 +386,25 @@ unsigned callaway_bz_refinement(const RawMaterial& d) {
         }
         if (count > 100000) throw std::invalid_argument("Callaway first-BZ validation work limit exceeded.");
         std::vector<double> lengths;
+        std::vector<std::array<double, 3>> shifted;
         double shortest = original;
         for (int i = lo[0]; i <= hi[0]; ++i)
             for (int j = lo[1]; j <= hi[1]; ++j)
                 for (int k = lo[2]; k <= hi[2]; ++k) {
-                    const double r = length2(b, {q[0] - i, q[1] - j, q[2] - k});
+                    const std::array<double, 3> p{q[0] - i, q[1] - j, q[2] - k};
+                    const double r = length2(b, p);
                     shortest = std::min(shortest, r);
                     lengths.push_back(r);
+                    shifted.push_back(p);
                 }
         const double tolerance = 1e-10 * std::max(shortest, 1e-20);
         if (original > shortest + tolerance)
             throw std::invalid_argument("Callaway qpoint is outside the first Brillouin zone; supply first-BZ "
                                         "representatives.");
-        unsigned m = 0;
-        for (double r : lengths)
-            if (std::abs(r - shortest) <= tolerance) ++m;
+        std::vector<std::array<double, 3>> mine;
+        for (size_t c = 0; c < lengths.size(); ++c)
+            if (std::abs(lengths[c] - shortest) <= tolerance) mine.push_back(shifted[c]);
+        const unsigned m = static_cast<unsigned>(mine.size());
         if (m == 0) throw std::runtime_error("Empty Callaway BZ representative set.");
         unsigned g = refinement, h = m;
         while (h) {
@@ -408,12 +412,58 @@ unsigned callaway_bz_refinement(const RawMaterial& d) {
             g = h;
             h = t;
         }
-        refinement = refinement / g * m;
+        const unsigned long long next = static_cast<unsigned long long>(refinement / g) * m;
+        if (next > 96 || next * static_cast<unsigned long long>(d.qpoints) * d.branches >
+                             static_cast<unsigned long long>(std::numeric_limits<int>::max()))
+            throw std::length_error("Callaway equal-weight BZ refinement exceeds supported storage; use a smaller "
+                                    "mesh.");
+        refinement = static_cast<unsigned>(next);
+        if (reps) reps->push_back(std::move(mine));
     }
     std::sort(keys.begin(), keys.end());
     if (std::adjacent_find(keys.begin(), keys.end()) != keys.end())
         throw std::invalid_argument("Callaway input q mesh contains duplicate points modulo reciprocal translations.");
     return refinement;
+}
+} // namespace
+
+unsigned callaway_bz_refinement(const RawMaterial& d) { return bz_representatives(d, nullptr); }
+
+RawMaterial complete_callaway_bz(const RawMaterial& d) {
+    std::vector<std::vector<std::array<double, 3>>> reps;
+    const unsigned refinement = bz_representatives(d, &reps);
+    RawMaterial out = d;
+    out.bz_refinement = refinement;
+    if (refinement == 1) return out;
+    // Source mode of every completed mode: q's representatives in turn, each
+    // repeated refinement / |reps| times, every branch of q per copy.
+    const size_t nb = static_cast<size_t>(d.branches), old_nm = static_cast<size_t>(d.qpoints) * nb;
+    std::vector<size_t> source;
+    source.reserve(old_nm * refinement);
+    out.qpoint_fractions.clear();
+    for (int q = 0; q < d.qpoints; ++q)
+        for (const auto& r : reps[q])
+            for (unsigned copy = 0; copy < refinement / reps[q].size(); ++copy) {
+                out.qpoint_fractions.push_back(r);
+                for (size_t br = 0; br < nb; ++br) source.push_back(static_cast<size_t>(q) * nb + br);
+            }
+    const auto expand = [&](const std::vector<double>& v) {
+        std::vector<double> e;
+        if (v.empty()) return e;
+        e.reserve(v.size() * refinement);
+        for (size_t t = 0; t < v.size() / old_nm; ++t)
+            for (size_t i : source) e.push_back(v[t * old_nm + i]);
+        return e;
+    };
+    out.frequency_thz = expand(d.frequency_thz);
+    out.gamma = expand(d.gamma);
+    out.gamma_normal = expand(d.gamma_normal);
+    out.gamma_umklapp = expand(d.gamma_umklapp);
+    out.gamma_isotope = expand(d.gamma_isotope);
+    out.group_velocity.clear();
+    for (size_t i : source) out.group_velocity.push_back(d.group_velocity[i]);
+    out.qpoints = d.qpoints * static_cast<int>(refinement);
+    return out;
 }
 
 CallawayTable callaway_data(const ModeTables& t, const RawMaterial& raw, double temperature) {
@@ -431,8 +481,51 @@ CallawayTable callaway_data(const ModeTables& t, const RawMaterial& raw, double 
             throw std::invalid_argument("Callaway data inconsistent: gamma_N + gamma_U != gamma. Supply unmodified, "
                                         "same-run linewidths in the same units.");
     }
-    if (callaway_bz_refinement(raw) != 1)
-        throw std::invalid_argument("Incomplete equal-weight Callaway BZ quadrature.");
+    // The completed mesh (src/material/PhononMaterialCollisionData.cpp:52-132):
+    // every mesh point exactly bz_refinement times, every q point a shortest
+    // representative within 1e-5.
+    {
+        std::map<std::array<int, 3>, unsigned> points;
+        for (const auto& q : raw.qpoint_fractions) {
+            std::array<int, 3> key{};
+            for (int a = 0; a < 3; ++a) {
+                const double x = (q[a] - std::floor(q[a])) * raw.mesh[a];
+                const long index = std::lround(x);
+                if (std::abs(x - index) > 2e-6 * raw.mesh[a])
+                    throw std::invalid_argument("Callaway requires a complete Gamma-centred full q mesh.");
+                key[a] = static_cast<int>(index % raw.mesh[a]);
+            }
+            if (++points[key] > raw.bz_refinement)
+                throw std::invalid_argument("Callaway q mesh contains duplicate points modulo reciprocal translations.");
+        }
+        for (const auto& e : points)
+            if (e.second != raw.bz_refinement)
+                throw std::invalid_argument("Incomplete equal-weight Callaway BZ quadrature.");
+        const auto& b = raw.reciprocal;
+        const auto inv = inverse3(b, det3(b));
+        for (const auto& q : raw.qpoint_fractions) {
+            const double k2 = length2(b, q);
+            int lo[3], hi[3];
+            double count = 1;
+            for (int a = 0; a < 3; ++a) {
+                double nrm = 0;
+                for (double x : inv[a]) nrm += x * x;
+                const double extent = std::sqrt(k2 * nrm);
+                if (!std::isfinite(extent) || extent > 100 || std::abs(q[a]) > 100)
+                    throw std::invalid_argument("Callaway first-BZ validation exceeds supported lattice conditioning.");
+                lo[a] = static_cast<int>(std::ceil(q[a] - extent - 1e-10));
+                hi[a] = static_cast<int>(std::floor(q[a] + extent + 1e-10));
+                count *= hi[a] - lo[a] + 1;
+            }
+            if (count > 100000) throw std::invalid_argument("Callaway first-BZ validation work limit exceeded.");
+            for (int i = lo[0]; i <= hi[0]; ++i)
+                for (int j = lo[1]; j <= hi[1]; ++j)
+                    for (int l = lo[2]; l <= hi[2]; ++l)
+                        if (length2(b, {q[0] - i, q[1] - j, q[2] - l}) < k2 - 1e-5 * std::max(k2, 1e-20))
+                            throw std::invalid_argument("Callaway qpoint is outside the first Brillouin zone; supply "
+                                                        "first-BZ representatives.");
+        }
+    }
     size_t hi = static_cast<size_t>(std::lower_bound(t.temps.begin(), t.temps.end(), temperature) - t.temps.begin());
     const size_t lo = hi > 0 ? hi - 1 : hi;
     const double fraction = hi == lo ? 0 : (temperature - t.temps[lo]) / (t.temps[hi] - t.temps[lo]);

@@ -27,6 +27,7 @@ struct RawMaterial {
     std::vector<double> gamma;                       // nT*nq*nb, THz linewidth
     std::vector<double> gamma_normal, gamma_umklapp; // optional, like gamma
     std::vector<double> gamma_isotope;               // optional, nq*nb
+    unsigned bz_refinement = 1;                      // copies per mesh point after first-BZ completion
 };
 
 // Parameters of the seeded synthetic Si-like table.
@@ -94,6 +95,11 @@ CallawayTable callaway_data(const ModeTables& t, const RawMaterial& raw, double 
 // Returns the equal-weight refinement the reference would apply (1 when every
 // q point has a unique shortest representative).
 unsigned callaway_bz_refinement(const RawMaterial& raw);
+// Equal-weight first-Brillouin-zone completion of an N/U table
+// (complete_callaway_bz, src/material/BzQuadrature.cpp:11-161): every q point
+// becomes bz_refinement points, its equally short representatives repeated
+// evenly, with every per-mode array expanded alike.
+RawMaterial complete_callaway_bz(const RawMaterial& raw);
 
 // Loads a material named by io.material_folder: "synthetic:si16[:callaway]" or a
 // directory holding material.pmcmat (the raw binary layout written by save_raw_material).

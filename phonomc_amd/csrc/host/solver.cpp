@@ -20,7 +20,8 @@ namespace {
 // stationary-mode policy; the temperature lookup table alone takes about half a
 // second.  Problems built in one process (a batch of instances) share them.
 struct MaterialEntry {
-    RawMaterial raw;
+    RawMaterial raw;       // as loaded (the material view)
+    RawMaterial completed; // N/U tables: equal-weight first-BZ completion, which the mode tables use
     ModeTables modes;
 };
 
@@ -45,7 +46,9 @@ std::shared_ptr<const MaterialEntry> cached_material(const std::string& spec, co
     }
     auto entry = std::make_shared<MaterialEntry>();
     entry->raw = load_material(spec, input_directory);
-    entry->modes = build_mode_tables(entry->raw, lookup_dt, include_stationary);
+    // PhononMaterial completes an N/U table at construction (src/PhononMaterial.cpp:41-43).
+    entry->completed = entry->raw.gamma_normal.empty() ? entry->raw : complete_callaway_bz(entry->raw);
+    entry->modes = build_mode_tables(entry->completed, lookup_dt, include_stationary);
     std::lock_guard<std::mutex> g(lock);
     if (cache.size() >= 16) cache.clear(); // bounded: a process rarely uses more materials
     return cache.emplace(key, std::move(entry)).first->second;
@@ -275,7 +278,7 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         if (!(hc > 0.0) || !std::isfinite(hc))
             throw std::invalid_argument("Linearized temperature requires positive heat capacity.");
         p->heat_capacity_density = hc;
-        p->callaway = callaway_data(m, p->raw, c.background_temperature);
+        p->callaway = callaway_data(m, material->completed, c.background_temperature);
         p->callaway_wavevector.resize(3 * static_cast<size_t>(m.n_active));
         for (int a = 0; a < m.n_active; ++a)
             for (int k = 0; k < 3; ++k) p->callaway_wavevector[3 * a + k] = p->callaway.wavevector[a][k];

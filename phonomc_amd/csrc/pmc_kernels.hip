@@ -2085,12 +2085,77 @@ __global__ void __launch_bounds__(kChunkThreads) kb_relax(const View* __restrict
 // G_N = sum_i g_N,i B_i B_i^T and P_N = sum_k c_k c_k^T from modified
 // Gram-Schmidt (two passes) of the normalized basis in G_N's inner product: the
 // reference's orthogonalization of the n-vectors carried out in the 4-dim
-// coefficient space.  A direction is skipped when its basis vector vanishes or
-// its squared length after projection is below 1e-14 (lengths from a Gram matrix
-// resolve to about 1e-8; the reference skips below 1e-12 on explicit vectors).
-// exp(dt L) is the reference's Taylor scheme: ceil(dt max rate / 4) substeps,
-// terms until the geometric tail bound is at most 2e-14/steps of |x|.  Every
-// Taylor term is one pass over the registers with one block reduction.
+// coefficient space, while every direction keeps a squared length of at least
+// 1e-6; otherwise on explicit per-carrier vectors, skipping a direction below
+// the reference's 1e-12 (callaway_setup_body).  exp(dt L) is the reference's
+// Taylor scheme: ceil(dt max rate / 4) substeps, terms until the geometric tail
+// bound is at most 2e-14/steps of |x|.  Every Taylor term is one pass over the
+// registers with one block sum.
+// A double from the lane that DPP control `kCtrl` names (both 32-bit halves).
+template <int kCtrl>
+__device__ __forceinline__ double dpp_double(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E;          // quad_perm [1,0,3,2], [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141, kDppMirror = 0x140; // within 8 and 16 lanes
+
+// Sum over the wavefront as one wave-uniform value: in each row of 16 lanes,
+// partner exchanges (xor 1, xor 2, half-row mirror, row mirror) leave every
+// lane with the same row sum (each stage adds two equal partials in either
+// order), then the four row sums are read out of lanes 0, 16, 32 and 48.
+template <int K>
+__device__ __forceinline__ void wave_sum(double (&v)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppXor1>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppXor2>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppHalfMirror>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppMirror>(v[k]);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        v[k] = ((read_lane(v[k], 0) + read_lane(v[k], 16)) + read_lane(v[k], 32)) + read_lane(v[k], 48);
+}
+__device__ __forceinline__ double wave_max(double a) {
+    a = fmax(a, dpp_double<kDppXor1>(a));
+    a = fmax(a, dpp_double<kDppXor2>(a));
+    a = fmax(a, dpp_double<kDppHalfMirror>(a));
+    a = fmax(a, dpp_double<kDppMirror>(a));
+    return fmax(fmax(read_lane(a, 0), read_lane(a, 16)), fmax(read_lane(a, 32), read_lane(a, 48)));
+}
+
+// Block sums of K values with one barrier: DPP wave sums, lane 0 of each wave
+// stores its K sums in one of two LDS buffers (alternated by `parity`, so a
+// wave never overwrites a buffer another wave may still read), and every
+// thread adds the W rows in wave order, which leaves the same totals in every
+// thread.  Consecutive calls on one lds must alternate parity; buffer p starts
+// at lds + p * stride (stride >= K W for every K used), so lds holds 2 stride
+// doubles.
+template <int K, int W>
+__device__ __forceinline__ void block_sum_sync1(double (&v)[K], double* lds, int parity, int stride) {
+    const int wave = threadIdx.x >> 6;
+    wave_sum<K>(v);
+    double* buf = lds + parity * stride;
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) buf[wave * K + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = buf[k];
+        for (int w = 1; w < W; ++w) s += buf[w * K + k];
+        v[k] = s;
+    }
+}
+
 // Block sums of K values for many waves: wave sums to LDS, K threads add the
 // W wave rows (in wave order), every thread reads the K totals.  lds holds
 // K * W + K doubles.
@@ -2340,6 +2405,7 @@ __device__ __forceinline__ void callaway_body(const View& V, const int c) {
     const int t = threadIdx.x;
     const bool evolve = rec[kCwEvolve] != 0.0;
     const bool explicit_dirs = rec[kCwExplicit] != 0.0;
+    int par = 0; // LDS buffer of the next block sum
     int md[kCwPer];
     double x[kCwPer], s[kCwPer];
     const double xmax = rec[kCwXmax];
@@ -2417,7 +2483,8 @@ __device__ __forceinline__ void callaway_body(const View& V, const int c) {
                     red[5] += tn * tn;
                     red[6] += s[r] * s[r];
                 }
-                block_reduce_wide<7, kW>(red, lds);
+                block_sum_sync1<7, kW>(red, lds, par, 7 * kW);
+                par ^= 1;
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
                     mN[a] = red[a];
@@ -2463,11 +2530,12 @@ __device__ __forceinline__ void callaway_body(const View& V, const int c) {
         out[0] += (kHbar * mt.x * (next - mt.y)) * V.w;
         out[1] += (hw * next - hw * old) * V.w;
     }
-    block_reduce_wide<2, kW>(out, lds);
+    block_sum_sync1<2, kW>(out, lds, par, 7 * kW);
     if (t == 0) {
         V.chunk_out[2 * static_cast<size_t>(k0)] = out[0];
         V.chunk_out[2 * static_cast<size_t>(k0) + 1] = out[1];
     }
+    __syncthreads(); // the workgroup's next cell reuses the buffers from parity 0
 }
 
 // The workgroups of a context take the listed large cells in turn.
@@ -2501,46 +2569,6 @@ __global__ void __launch_bounds__(kCwThreads) kb_callaway(const View* __restrict
 }
 
 // ---- one wavefront per cell
-// A double from the lane that DPP control `kCtrl` names (both 32-bit halves).
-template <int kCtrl>
-__device__ __forceinline__ double dpp_double(double v) {
-    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), kCtrl, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), kCtrl, 0xF, 0xF, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double read_lane(double v, int lane) {
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
-                            __builtin_amdgcn_readlane(__double2loint(v), lane));
-}
-constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E;          // quad_perm [1,0,3,2], [2,3,0,1]
-constexpr int kDppHalfMirror = 0x141, kDppMirror = 0x140; // within 8 and 16 lanes
-
-// Sum over the wavefront as one wave-uniform value: in each row of 16 lanes,
-// partner exchanges (xor 1, xor 2, half-row mirror, row mirror) leave every
-// lane with the same row sum (each stage adds two equal partials in either
-// order), then the four row sums are read out of lanes 0, 16, 32 and 48.
-template <int K>
-__device__ __forceinline__ void wave_sum(double (&v)[K]) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppXor1>(v[k]);
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppXor2>(v[k]);
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppHalfMirror>(v[k]);
-#pragma unroll
-    for (int k = 0; k < K; ++k) v[k] += dpp_double<kDppMirror>(v[k]);
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-        v[k] = ((read_lane(v[k], 0) + read_lane(v[k], 16)) + read_lane(v[k], 32)) + read_lane(v[k], 48);
-}
-__device__ __forceinline__ double wave_max(double a) {
-    a = fmax(a, dpp_double<kDppXor1>(a));
-    a = fmax(a, dpp_double<kDppXor2>(a));
-    a = fmax(a, dpp_double<kDppHalfMirror>(a));
-    a = fmax(a, dpp_double<kDppMirror>(a));
-    return fmax(fmax(read_lane(a, 0), read_lane(a, 16)), fmax(read_lane(a, 32), read_lane(a, 48)));
-}
-
 // The whole collision of a cell of at most 64 P carriers (the setup and the
 // Taylor steps of the large-cell kernels, same arithmetic per carrier) on one
 // wavefront: lane l holds carriers l, l + 64, ... with their gain weights.

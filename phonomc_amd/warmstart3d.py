@@ -26,11 +26,15 @@ Time to steady is measured as for the heated film (phonomc_amd.warmstart):
 window averages against a long plain run's field, 0.5 K RMS over the cells.
 """
 import concurrent.futures
+import os
 import time
 
 import numpy as np
 
 from . import runtime as rt
+
+DATA_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "finfet_fields.npz")
+HELD_OUT = 64                  # the last cases of the data set are held out of training
 
 CONFIG = "finfet_callaway.toml"
 GRID = (64, 32, 32)
@@ -233,6 +237,11 @@ def pack_dataset(raw_paths, out, geometry_problem):
     np.savez_compressed(out, params=params, g_q=q, g_lo=lo, g_scale=scale, dt=DT, config=CONFIG)
 
 
+def load_dataset_cases(path):
+    d = np.load(path, allow_pickle=False)
+    return [dict(zip(PARAM_KEYS, map(float, row))) for row in d["params"]]
+
+
 def load_dataset(path, geometry_problem):
     """(cases, rise per microwatt [n, S] on the full grid, geometry)."""
     d = np.load(path, allow_pickle=False)
@@ -270,9 +279,10 @@ class WarmStart3d:
         losses = []
         per = max(1, epochs // stages)
         for k in range(stages):
+            t0 = time.perf_counter()
             losses.append(self.op.train(x, y, w, per, batch=batch, lr=lr * 0.5 ** k, seed=seed + k))
             if log:
-                log(f"[fno] stage {k + 1}/{stages}: loss {losses[-1][-1]:.4e}")
+                log(f"[fno] stage {k + 1}/{stages}: loss {losses[-1][-1]:.4e} ({time.perf_counter() - t0:.1f} s)")
         return np.concatenate(losses)
 
     def predict(self, x):
@@ -283,6 +293,57 @@ class WarmStart3d:
         quad = quadrant(geom)
         pred = unfold(fold(self.predict(features(problem, p, geom)[None]), quad), quad)[0]
         return field_from_prediction(problem, p, geom, pred)
+
+
+def relative_weights(y, mask):
+    """Per-point loss weights [n, S]: the solid mask over each case's mean
+    square target, so that every case counts by its relative error."""
+    ms = (y.astype(np.float64) ** 2 * mask).sum(axis=1) / mask.sum(axis=1)
+    return (mask / np.maximum(ms, 1e-30)[:, None]).astype(np.float32)
+
+
+def train(epochs=48, lr=1e-3, stages=4, path=DATA_PATH, log=None):
+    """The operator trained on the data set but its last HELD_OUT cases.
+    Returns (model, info) with the held-out relative RMS error of the rise
+    over the solid cells, the training time and the loss history."""
+    cases_all = load_dataset_cases(path)
+    geometry_problem = problem_of(cases_all[0], 20000, 1)
+    cases, per_uw, geom = load_dataset(path, geometry_problem)
+    x, y, mask = training_arrays(cases, per_uw, geom)
+    w = relative_weights(y, mask)
+    n = len(cases) - HELD_OUT
+    model = WarmStart3d()
+    t0 = time.perf_counter()
+    losses = model.fit(x[:n], y[:n], w[:n], epochs=epochs, lr=lr, stages=stages, log=log)
+    seconds = time.perf_counter() - t0
+    quad = quadrant(geom)
+    pred = np.concatenate([unfold(fold(model.predict(x[k:k + 4]), quad), quad) for k in range(n, len(cases), 4)])
+    err = np.sqrt(((pred - y[n:]) ** 2 * mask[n:]).sum(axis=1) / (y[n:] ** 2 * mask[n:]).sum(axis=1))
+    info = {"train_cases": n, "held_out_cases": len(cases) - n, "epochs": int(len(losses)),
+            "train_seconds": seconds, "final_loss": float(losses[-1]),
+            "held_out_relative_rms": float(err.mean()), "held_out_relative_rms_max": float(err.max())}
+    if log:
+        log(f"[fno] trained on {n} cases in {seconds:.1f} s; held-out relative RMS {err.mean():.4f} "
+            f"(max {err.max():.4f})")
+    return model, info, (x, y, w)
+
+
+def gemm_roofline(model, x, y, w, reps=10, peak_tflops=2500.0):
+    """Roofline object of the operator's GEMMs: forward + backward passes at
+    the training batch, algorithmic flops (2 m n k per product) over the
+    HIP-event durations of the GEMM launches."""
+    op = model.op
+    b = op.shape.max_batch
+    op.set_timing(True)
+    op.gemm_stats(reset=True)
+    for _ in range(reps):
+        op.loss_grad(x[:b], y[:b], w[:b])
+    flops, launches, ms = op.gemm_stats(reset=True)
+    op.set_timing(False)
+    tflops = flops / (ms / 1e3) / 1e12
+    return {"bound": "mfma", "achieved": tflops, "peak": peak_tflops, "unit": "TFLOP/s", "frac": tflops / peak_tflops,
+            "traffic": None, "kernel": "k_gemm", "flops_per_pass": flops / reps, "launches_per_pass": launches / reps,
+            "gemm_ms_per_pass": ms / reps, "avg_launch_ms": ms / max(1, launches)}
 
 
 def run_windows(problem, steps, window, sample, initial_field=None, seed=None):

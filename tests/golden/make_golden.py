@@ -479,6 +479,10 @@ def main():
         text = rt.config_text(WORKLOADS["c1c"])
         np.savez_compressed(os.path.join(OUT, "callaway_c1c.npz"),
                             **callaway_vectors(rt.Problem(text, mode_sampling=1), rng))
+        # the even 16^3 table, completed to first-BZ copies by both sides
+        text16 = rt.config_text(WORKLOADS["c1c"], **{"io.material_folder": '"synthetic:si16:callaway"'})
+        np.savez_compressed(os.path.join(OUT, "callaway_c1c16.npz"),
+                            **callaway_vectors(rt.Problem(text16, mode_sampling=1), np.random.default_rng(20261209)))
         if "--callaway-runs" in sys.argv:
             np.savez_compressed(os.path.join(OUT, "run_c1c.npz"),
                                 **whole_run("c1c", 20000, 400, list(range(1000, 1016)), 10))

@@ -3,7 +3,10 @@
 - The host's CallawayData (capacities, wavevectors, N and resistive rates at
   T_bg over the active bank with stationary modes) equals the reference's
   PhononMaterial::callaway_data bit for bit (SHA-256 of every array,
-  tests/golden/callaway_c1c.npz).
+  tests/golden/callaway_c1c.npz), on the odd 15^3 table and on the even 16^3
+  table, whose zone-boundary q points the equal-weight first-BZ completion
+  turns into eight copies each (complete_callaway_bz,
+  src/material/BzQuadrature.cpp:11-161; tests/golden/callaway_c1c16.npz).
 - The oracle's CarrierCollisionOperator restatement reproduces the reference's
   occupations on three random carrier sets (dt 1, 3 and 12 ps; the last takes
   two Taylor substeps).
@@ -26,9 +29,23 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 HBAR = 6.582119569e-4
 
 
+TABLES = {"si15": ("synthetic:si15:callaway", "callaway_c1c.npz"),
+          "si16": ("synthetic:si16:callaway", "callaway_c1c16.npz")}
+
+
+def problem_of(table):
+    spec = TABLES[table][0]
+    return rt.Problem(rt.config_text("c1c_callaway.toml", **{"io.material_folder": f'"{spec}"'}), mode_sampling=1)
+
+
 @pytest.fixture(scope="module")
 def c1c():
-    return rt.Problem(rt.config_text("c1c_callaway.toml"), mode_sampling=1)
+    return problem_of("si15")
+
+
+@pytest.fixture(scope="module", params=sorted(TABLES))
+def table(request):
+    return problem_of(request.param), np.load(os.path.join(HERE, "golden", TABLES[request.param][1]))
 
 
 @pytest.fixture(scope="module")
@@ -45,21 +62,39 @@ def host_table(problem):
             "normal_rate": rt.arr(cw.normal_rate, na), "resistive_rate": rt.arr(cw.resistive_rate, na)}
 
 
-def test_callaway_table_matches_reference(c1c, golden):
-    t = host_table(c1c)
+def test_callaway_table_matches_reference(table):
+    problem, golden = table
+    t = host_table(problem)
     assert len(t["omega"]) == int(golden["active_count"])
     for k, v in t.items():
         assert hashlib.sha256(np.ascontiguousarray(v).tobytes()).hexdigest() == str(golden[f"digest_{k}"]), k
         np.testing.assert_array_equal(v[golden["rows"]], golden[f"rows_{k}"])
-    # Stationary modes (optical branches at Gamma) are active under Callaway.
-    _, m, _, _, _, _ = c1c.views()
+    # Stationary modes are active under Callaway: the optical branches at Gamma
+    # (15^3), and on the even mesh also the zone-boundary modes of the sine
+    # dispersion, each in all of its first-BZ copies.
+    _, m, _, _, _, _ = problem.views()
     v = rt.arr(m.velocity, 3 * m.n_active).reshape(-1, 3)
-    assert np.sum(np.linalg.norm(v, axis=1) <= 1e-12) == 3
+    copies = m.n_flat // (int(problem.material().qpoints) * int(problem.material().branches))
+    stationary = int(np.sum(np.linalg.norm(v, axis=1) <= 1e-12))
+    assert stationary == 3 if copies == 1 else (stationary > 3 * copies and stationary % copies == 0)
+
+
+def test_even_mesh_completion_counts():
+    """16^3 mesh on a cubic reciprocal lattice: q points with one, two or three
+    components at the zone boundary (1/2) have 2, 4 or 8 shortest
+    representatives, so every point is kept 8 times (lcm), the representatives
+    of each in equal numbers."""
+    p = problem_of("si16")
+    _, m, _, _, _, _ = p.views()
+    raw = p.material()
+    assert m.n_flat == 8 * raw.qpoints * raw.branches
+    assert m.n_active == 8 * (raw.qpoints * raw.branches) - 3 * 8  # acoustic Gamma modes stay inactive
 
 
 @pytest.mark.parametrize("tag", ["a", "b", "c"])
-def test_callaway_collisions_match_reference(c1c, golden, tag):
-    occ, residual = O.callaway_apply(c1c, golden[f"{tag}_active"], golden[f"{tag}_cell"], golden[f"{tag}_occ"],
+def test_callaway_collisions_match_reference(table, tag):
+    problem, golden = table
+    occ, residual = O.callaway_apply(problem, golden[f"{tag}_active"], golden[f"{tag}_cell"], golden[f"{tag}_occ"],
                                      float(golden[f"{tag}_dt"]))
     np.testing.assert_allclose(occ, golden[f"{tag}_occ_after"], rtol=1e-14, atol=0)
     assert residual == pytest.approx(float(golden[f"{tag}_residual"]), abs=1e-9)
@@ -159,7 +194,5 @@ def test_callaway_inputs_rejected_with_messages():
     base = rt.config_text("c1c_callaway.toml")
     with pytest.raises(RuntimeError, match="nonlinear"):
         rt.Problem(base.replace('model = "callaway"', 'model = "callaway"\ncallaway_formulation = "nonlinear"'))
-    with pytest.raises(RuntimeError, match="zone"):
-        rt.Problem(base.replace("synthetic:si15:callaway", "synthetic:si16:callaway"))
     with pytest.raises(RuntimeError, match="gamma_N and gamma_U"):
         rt.Problem(base.replace("synthetic:si15:callaway", "synthetic:si15"))
