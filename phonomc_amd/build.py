@@ -53,6 +53,35 @@ def build_host():
           "-Wl,-rpath,$ORIGIN", "-o", os.path.join(OUT, "pmc")])
 
 
+HDF5_PREFIX = os.environ.get("PMC_HDF5_PREFIX", "/opt/conda")
+
+
+def build_importer():
+    """pmc_import_material (phono3py HDF5 folder -> raw material table), where the
+    HDF5 C++ library is installed."""
+    inc, lib = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "H5Cpp.h")):
+        print(f"(no HDF5 C++ library under {HDF5_PREFIX}: pmc_import_material not built)", flush=True)
+        return
+    # Only the HDF5 libraries come from the prefix (a directory of links to them
+    # on the tool's run path): the C++ runtime stays the system's.
+    links = os.path.join(OUT, "hdf5")
+    os.makedirs(links, exist_ok=True)
+    objs = []
+    for name in ("libhdf5_cpp.so", "libhdf5.so"):
+        real = os.path.realpath(os.path.join(lib, name))
+        soname = [f for f in os.listdir(lib) if f.startswith(name + ".") and f.count(".") == 2]
+        for f in soname + [name]:
+            dst = os.path.join(links, f)
+            if os.path.lexists(dst):
+                os.remove(dst)
+            os.symlink(real, dst)
+        objs.append(os.path.join(links, name))
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-I", os.path.join(CSRC, "host"), "-I", inc,
+          os.path.join(CSRC, "host", "hdf5_import.cpp"), os.path.join(CSRC, "host", "material.cpp"), *objs,
+          "-Wl,-rpath,$ORIGIN/hdf5", "-o", os.path.join(OUT, "pmc_import_material")])
+
+
 def build_oracle():
     _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-j8"])
 
@@ -67,6 +96,7 @@ def main():
     os.makedirs(OUT, exist_ok=True)
     build_device()
     build_host()
+    build_importer()
     build_oracle()
     old = os.getcwd()
     os.chdir(ROOT)

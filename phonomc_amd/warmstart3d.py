@@ -495,10 +495,35 @@ def main(argv=None):
     ap.add_argument("--burn-in", type=int, default=3000)
     ap.add_argument("--average", type=int, default=1000)
     ap.add_argument("--first", type=int, default=0, help="index of the first case of the sample to run")
+    ap.add_argument("--evaluate", type=int, default=0, metavar="N",
+                    help="instead: train on the committed data set and evaluate N fresh held-out cases")
+    ap.add_argument("--epochs", type=int, default=48)
+    ap.add_argument("--steps", type=int, default=3000, help="evaluation: steps of the plain and warm runs")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
-    cases = sample_cases(a.first + a.cases, a.seed)[a.first:]
     log = lambda m: print(m, flush=True)  # noqa: E731
+    if a.evaluate:
+        import json
+        model, info, _ = train(epochs=a.epochs, log=log)
+        cases = sample_cases(a.evaluate, a.seed + 1000)  # parameter sets outside the data set
+        t0 = time.perf_counter()
+        t_refs = reference_fields(cases, a.carriers, a.burn_in, a.average, log=log)
+        log(f"[eval] reference fields in {time.perf_counter() - t0:.1f} s")
+        rows, err = evaluate_batch(model, cases, a.carriers, t_refs, a.steps, log=log)
+        sp = [r["plain_steps_to_steady"] / r["warm_steps_to_steady"] for r in rows
+              if r["plain_steps_to_steady"] and r["warm_steps_to_steady"]]
+        summary = {"operator": info, "carriers": a.carriers, "steps": a.steps, "reference_burn_in": a.burn_in,
+                   "reference_average": a.average, "cases": rows,
+                   "speedup_steps_median": float(np.median(sp)) if sp else None,
+                   "speedup_steps_geomean": float(np.exp(np.mean(np.log(sp)))) if sp else None,
+                   "reached": len(sp), "evaluated": len(rows)}
+        with open(a.out, "w") as f:
+            json.dump(summary, f)
+        np.save(a.out.replace(".json", "_err.npy"), err.astype(np.float32))
+        log(f"[eval] {len(sp)}/{len(rows)} cases reached 0.5 K RMS in both runs; step speed-up median "
+            f"{summary['speedup_steps_median']}, geometric mean {summary['speedup_steps_geomean']}")
+        return
+    cases = sample_cases(a.first + a.cases, a.seed)[a.first:]
     problems, fields = generate(cases, a.carriers, a.burn_in, a.average, seed0=500000 + a.first, log=log)
     rises = fields - np.array([c["t_bg"] for c in cases])[:, None]
     watts = np.array([source_power(p)[0] for p in problems])
