@@ -429,7 +429,11 @@ def main():
     warm = None
     if not args.no_warm_start:
         log("[bench] warm start: training the operator and timing plain vs warm-started runs")
-        warm = warm_start_leg() if args.warm_start == "film" else finfet_warm_start_leg()
+        from phonomc_amd import warmstart3d as w3
+        film = args.warm_start == "film" or not os.path.exists(w3.DATA_PATH)
+        if film and args.warm_start != "film":
+            log(f"[bench] {w3.DATA_PATH} absent: the heated-film warm start runs instead")
+        warm = warm_start_leg() if film else finfet_warm_start_leg()
         log(f"[bench] warm start: {warm.get('plain_steps_to_steady')} vs {warm.get('warm_steps_to_steady')} steps to "
             f"0.5 K RMS ({warm.get('speedup_wall')} x wall); GEMM {warm['roofline']['achieved']:.2f} TFLOP/s")
 
