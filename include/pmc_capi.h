@@ -50,7 +50,12 @@ typedef struct pmc_setup {
     int32_t temperature_inversion; /* 0 lookup table (reference), 1 per-cell Newton on the exact
                                     * Bose-Einstein crystal energy (single-material RTA transport;
                                     * pmc_upload_callaway, pmc_upload_layers and batches reject it) */
-    int32_t reserved[8];
+    int32_t conservative_walls;    /* 1: conservative transport with Bose occupations
+                                    * (simulation.conservative_boundaries = true, include/SimulationConfig.h:153-156):
+                                    * every reflecting facet reflects through its strict-elastic table with
+                                    * signed occupations (src/solver/BoundaryInteraction.cpp:90-113).
+                                    * pmc_upload_callaway and pmc_upload_callaway_nonlinear imply it. */
+    int32_t reserved[7];
 } pmc_setup;
 
 /* Material and background tables by active-mode index; replaces PhononMaterial
@@ -234,6 +239,31 @@ typedef struct pmc_callaway_table {
     double background_energy_density; /* E_bg: exact crystal energy density at T_bg */
 } pmc_callaway_table;
 
+/* Nonlinear Callaway scattering (scattering.callaway_formulation = 'nonlinear');
+ * replaces NonlinearCallawayCollision::apply
+ * (src/solver/NonlinearCallawayCollision.cpp:287-344) and nonlinear_callaway_evolve
+ * (:251-285): per cell, N(dt/2) R(dt) N(dt/2) exponential BGK substeps toward the
+ * displaced Bose target whose a-weighted energy (and, for N, crystal momentum)
+ * moments match the carriers' (channel, :25-248), rates from
+ * PhononMaterial::mode_callaway_rates (src/material/PhononMaterialCollisionData.cpp:167-196)
+ * at the cell temperature (lifetime_local) or pmc_setup.lifetime_temperature.
+ * Uploading it switches the context to the reference's nonlinear conservative
+ * transport (include/SimulationConfig.h:148-156): Bose occupations, lookup-table
+ * cell temperatures, discrete-mode reflection with the strict-elastic tables on
+ * every reflecting facet.  The caller supplies Bose reservoir occupations,
+ * strict-elastic rough tables at T_bg and pmc_setup.refill_policy = 1.  Tables
+ * by (temperature sample, active index); raw linewidths, the 4 pi factor is the
+ * device's.  Call after pmc_upload_material and pmc_upload_geometry and before
+ * the population is created. */
+typedef struct pmc_callaway_rates {
+    int32_t n_temps;
+    const double* temps;           /* [n_temps] K, increasing */
+    const double* gamma_normal;    /* [n_temps*n_active] gamma_N */
+    const double* gamma_umklapp;   /* [n_temps*n_active] gamma_U */
+    const double* gamma_extra;     /* [n_temps*n_active] isotope/impurity/defect linewidths */
+    const double* wavevector;      /* [n_active*3] Cartesian q, 1/Angstrom */
+} pmc_callaway_rates;
+
 /* Per-step and cumulative ledgers; replaces BoundaryLedger/EnergyLedger
  * (src/solver/BoundaryLedger.cpp:30-141, src/solver/EnergyLedger.cpp:58-90). */
 typedef struct pmc_ledger {
@@ -306,6 +336,8 @@ int pmc_upload_boundary_tables(pmc_ctx* ctx, int32_t n_rough, const pmc_rough_ta
                                int32_t n_reservoirs, const pmc_reservoir* reservoirs);
 /* Enables the heat source (after material and geometry). */
 int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* table);
+/* Nonlinear Callaway (pmc_callaway_rates); single contexts only, batches reject it. */
+int pmc_upload_callaway_nonlinear(pmc_ctx* ctx, const pmc_callaway_rates* rates);
 int pmc_upload_heat_source(pmc_ctx* ctx, const pmc_heat_source* source);
 /* Samples the initial population on the device (Initialization.cpp:48-113) and
  * performs the initial temperature refresh (Observables.cpp:12-21). */

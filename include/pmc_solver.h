@@ -57,6 +57,9 @@ int pmcs_problem_heat_source(const pmcs_problem* problem, pmc_heat_source* out);
 /* Linearized Callaway view (valid while the problem lives): 1 if the input selects
  * scattering.model = 'callaway' (linearized), 0 if not. */
 int pmcs_problem_callaway(const pmcs_problem* problem, pmc_callaway_table* out);
+/* Nonlinear Callaway rate tables (valid while the problem lives): 1 if the input
+ * selects scattering.callaway_formulation = 'nonlinear', 0 if not. */
+int pmcs_problem_callaway_nonlinear(const pmcs_problem* problem, pmc_callaway_rates* out);
 /* Layered materials: fills *out and returns 1 (0 for one material). */
 int pmcs_problem_layers(const pmcs_problem* problem, pmc_layers* out);
 /* Raw table of layer `layer` (materials.folders order). */

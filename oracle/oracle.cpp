@@ -836,6 +836,261 @@ double orc_callaway_apply(const pmc_mode_table* m, const pmc_callaway_table* t, 
 
 } // extern "C"
 
+// ---------------------------------------------------------------- nonlinear Callaway
+// nonlinear_callaway_evolve (src/solver/NonlinearCallawayCollision.cpp:251-285):
+// three exponential BGK substeps N(dt/2) R(dt) N(dt/2), each relaxing toward the
+// displaced Bose target n_BE(phi), phi = sum_j lambda_j b_j, whose a-weighted
+// moments equal the carriers' (channel, :25-248).  Restated in long double, as
+// the reference computes it.
+namespace {
+using LD = long double;
+
+LD nl_bose(LD x) { return x > 700 ? 0 : 1 / std::expm1(x); } // :19-21
+
+// One substep on the occupations n of one cell.  rate: the channel's rates,
+// normal: energy plus crystal-momentum constraints (else energy only).
+int nl_channel(std::vector<double>& n, const std::vector<double>& omega, const std::vector<V3>& q,
+               const std::vector<double>& rate, double T, double dt, bool normal) {
+    const size_t N = n.size();
+    // weights a = 1 - exp(-rate dt) and the metric a n_eq (1 + n_eq) at the cell
+    // temperature (:34-43); phi0 = hw / kT is the undisplaced equilibrium.
+    std::vector<LD> a(N), metric(N), phi0(N);
+    LD activity = 0;
+    for (size_t i = 0; i < N; ++i) {
+        a[i] = -std::expm1(-rate[i] * dt);
+        activity += a[i];
+        phi0[i] = kHbar * omega[i] / (kBoltz * T);
+        const LD e = nl_bose(phi0[i]);
+        metric[i] = a[i] * e * (1 + e);
+    }
+    if (activity == 0) return 0;
+    // Constraint directions (energy phi0, then q_x, q_y, q_z for N), each
+    // normalized, orthogonalized twice against the kept ones in the metric,
+    // dropped when shorter than 1e-12 (:46-87).
+    std::vector<std::vector<LD>> B;
+    std::array<LD, 4> lam{};
+    const int ndir = normal ? 4 : 1;
+    for (int j = 0; j < ndir; ++j) {
+        std::vector<LD> v(N);
+        LD start = 0;
+        for (size_t i = 0; i < N; ++i) {
+            v[i] = j == 0 ? phi0[i] : static_cast<LD>(q[i][j - 1]);
+            start += metric[i] * v[i] * v[i];
+        }
+        if (start == 0) continue;
+        start = std::sqrt(start);
+        for (auto& x : v) x /= start;
+        for (int pass = 0; pass < 2; ++pass)
+            for (const auto& b : B) {
+                LD d = 0;
+                for (size_t i = 0; i < N; ++i) d += metric[i] * v[i] * b[i];
+                for (size_t i = 0; i < N; ++i) v[i] -= d * b[i];
+            }
+        LD len = 0;
+        for (size_t i = 0; i < N; ++i) len += metric[i] * v[i] * v[i];
+        len = std::sqrt(len);
+        if (len < 1e-12L) continue;
+        for (auto& x : v) x /= len;
+        if (j == 0) lam[0] = start * len;
+        B.push_back(std::move(v));
+    }
+    const int dim = static_cast<int>(B.size());
+    if (dim == 0) return 1; // "no representable energy constraint"
+    // Gradient scale per direction (:91-97).
+    std::array<LD, 4> scale{};
+    for (int j = 0; j < dim; ++j)
+        for (size_t i = 0; i < N; ++i)
+            scale[j] += std::abs(a[i] * B[j][i]) * std::max(static_cast<LD>(n[i]), nl_bose(phi0[i]));
+    // Convex dual objective sum a (-log(1 - e^-phi) + n phi), its gradient
+    // sum a b (n - n_BE(phi)) and Hessian sum a b b' n_BE (1 + n_BE) (:98-131);
+    // infinite where some weighted phi is not positive and finite.
+    std::array<LD, 4> g{};
+    std::array<std::array<LD, 4>, 4> H{};
+    std::vector<LD> target(N);
+    auto objective = [&](const std::array<LD, 4>& x, bool hessian) -> LD {
+        g = {};
+        if (hessian) H = {};
+        LD f = 0;
+        for (size_t i = 0; i < N; ++i) {
+            if (a[i] == 0) continue;
+            LD phi = 0;
+            for (int j = 0; j < dim; ++j) phi += x[j] * B[j][i];
+            if (!(phi > 0) || !std::isfinite(phi)) return std::numeric_limits<LD>::infinity();
+            const LD e = nl_bose(phi);
+            if (hessian) target[i] = e;
+            f += a[i] * (-std::log(-std::expm1(-phi)) + n[i] * phi);
+            for (int j = 0; j < dim; ++j) {
+                g[j] += a[i] * B[j][i] * (n[i] - e);
+                if (hessian)
+                    for (int k = 0; k < dim; ++k) H[j][k] += a[i] * B[j][i] * B[k][i] * e * (1 + e);
+            }
+        }
+        return f;
+    };
+    auto gradient_error = [&]() {
+        LD err = 0;
+        for (int j = 0; j < dim; ++j) err = std::max(err, std::abs(g[j]) / std::max(1e-30L, scale[j]));
+        return err;
+    };
+    // Damped Newton: 70 iterations at most, converged when every scaled
+    // gradient component is below 2e-13 (:132-228).
+    bool converged = false;
+    for (int it = 0; it < 70 && !converged; ++it) {
+        const LD f0 = objective(lam, true);
+        const LD err = gradient_error();
+        if (err < 2e-13L) {
+            converged = true;
+            break;
+        }
+        // H d = -g by Gaussian elimination with partial pivoting.
+        std::array<LD, 4> d{};
+        for (int j = 0; j < dim; ++j) d[j] = -g[j];
+        auto M = H;
+        for (int col = 0; col < dim; ++col) {
+            int piv = col;
+            for (int r = col + 1; r < dim; ++r)
+                if (std::abs(M[r][col]) > std::abs(M[piv][col])) piv = r;
+            if (!(std::abs(M[piv][col]) > 1e-30L)) return 2; // "Singular ... moment Jacobian"
+            std::swap(M[col], M[piv]);
+            std::swap(d[col], d[piv]);
+            for (int r = col + 1; r < dim; ++r) {
+                const LD f = M[r][col] / M[col][col];
+                for (int k = col; k < dim; ++k) M[r][k] -= f * M[col][k];
+                d[r] -= f * d[col];
+            }
+        }
+        for (int r = dim - 1; r >= 0; --r) {
+            for (int k = r + 1; k < dim; ++k) d[r] -= M[r][k] * d[k];
+            d[r] /= M[r][r];
+        }
+        LD slope = 0;
+        for (int j = 0; j < dim; ++j) slope += g[j] * d[j];
+        // Backtracking: accept sufficient decrease or a 10 % smaller gradient error.
+        bool accepted = false;
+        for (LD t = 1; t > 1e-14L; t *= .5L) {
+            std::array<LD, 4> trial = lam;
+            for (int j = 0; j < dim; ++j) trial[j] += t * d[j];
+            const LD f1 = objective(trial, false);
+            const LD e1 = gradient_error();
+            if (std::isfinite(f1) && (f1 <= f0 + 1e-4L * t * slope || e1 < err * .9L)) {
+                lam = trial;
+                accepted = true;
+                break;
+            }
+        }
+        if (!accepted) return 3; // "moment line search failed"
+    }
+    if (!converged) return 4; // "moment solve did not converge"
+    for (size_t i = 0; i < N; ++i)
+        if (a[i] > 0) {
+            n[i] = static_cast<double>((1 - a[i]) * n[i] + a[i] * target[i]);
+            if (!std::isfinite(n[i]) || n[i] < 0) return 5; // "occupation is invalid"
+        }
+    return 0;
+}
+} // namespace
+
+extern "C" {
+
+int orc_nl_callaway_evolve(int n, const double* omega, const double* wavevector, const double* normal_rate,
+                           const double* resistive_rate, const double* occupation, double T, double dt, double* out) {
+    // Argument checks of :254-275.
+    if (n <= 0 || !std::isfinite(T) || T <= 0 || !std::isfinite(dt) || dt < 0) return 6;
+    std::vector<double> w(omega, omega + n), gn(normal_rate, normal_rate + n), gr(resistive_rate, resistive_rate + n);
+    std::vector<double> occ(occupation, occupation + n);
+    std::vector<V3> q(n);
+    for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(occ[i]) || occ[i] < 0 || !std::isfinite(w[i]) || w[i] <= 0 || !std::isfinite(gn[i]) ||
+            gn[i] < 0 || !std::isfinite(gr[i]) || gr[i] < 0)
+            return 6;
+        for (int k = 0; k < 3; ++k) {
+            q[i][k] = wavevector[3 * i + k];
+            if (!std::isfinite(q[i][k])) return 6;
+        }
+    }
+    if (dt > 0) {
+        int rc = nl_channel(occ, w, q, gn, T, dt * .5, true);
+        if (!rc) rc = nl_channel(occ, w, q, gr, T, dt, false);
+        if (!rc) rc = nl_channel(occ, w, q, gn, T, dt * .5, true);
+        if (rc) return rc;
+    }
+    std::copy(occ.begin(), occ.end(), out);
+    return 0;
+}
+
+int orc_nl_callaway_rates(const pmc_mode_table* m, const pmc_callaway_rates* r, double T, int active, double* normal,
+                          double* resistive) {
+    // PhononMaterial::mode_callaway_rates (src/material/PhononMaterialCollisionData.cpp:167-196):
+    // 4 pi times the linear interpolation between the bracketing samples (no clamping).
+    const int nt = r->n_temps;
+    if (!std::isfinite(T) || T <= 0 || nt <= 0 || T < r->temps[0] || T > r->temps[nt - 1]) return 7;
+    const int hi = static_cast<int>(std::lower_bound(r->temps, r->temps + nt, T) - r->temps);
+    const int lo = hi ? hi - 1 : hi;
+    const double f = lo == hi ? 0 : (T - r->temps[lo]) / (r->temps[hi] - r->temps[lo]);
+    const size_t na = static_cast<size_t>(m->n_active), a = static_cast<size_t>(active);
+    auto rate = [&](const double* t) { return 4 * kPi * ((1 - f) * t[lo * na + a] + f * t[hi * na + a]); };
+    *normal = rate(r->gamma_normal);
+    *resistive = rate(r->gamma_umklapp) + rate(r->gamma_extra);
+    return 0;
+}
+
+double orc_nl_callaway_apply(const pmc_mode_table* m, const pmc_callaway_rates* r, int ncells,
+                             const double* cell_temperature, int lifetime_local, double lifetime_temperature,
+                             int64_t n, const int32_t* active, const int32_t* cell, double* occ, double dt,
+                             double energy_weight, int* status) {
+    // NonlinearCallawayCollision::apply (src/solver/NonlinearCallawayCollision.cpp:287-344):
+    // each cell's carriers in index order, rates at the cell temperature or the
+    // fixed lifetime temperature, new occupations committed together, residual
+    // sum hw (n' - n) times the energy weight.
+    std::vector<std::vector<int64_t>> members(ncells);
+    for (int64_t i = 0; i < n; ++i) members[cell[i]].push_back(i);
+    std::vector<double> next(occ, occ + n);
+    std::vector<LD> residual(ncells, 0);
+    std::vector<int> err(ncells, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < ncells; ++c) {
+        const auto& idx = members[c];
+        if (idx.empty()) continue;
+        const double T = cell_temperature[c], rT = lifetime_local ? T : lifetime_temperature;
+        const size_t k = idx.size();
+        std::vector<double> w(k), q(3 * k), gn(k), gr(k), n0(k), out(k);
+        for (size_t j = 0; j < k; ++j) {
+            const int a = active[idx[j]];
+            if (orc_nl_callaway_rates(m, r, rT, a, &gn[j], &gr[j])) {
+                err[c] = 7;
+                break;
+            }
+            w[j] = m->omega[a];
+            for (int d = 0; d < 3; ++d) q[3 * j + d] = r->wavevector[3 * a + d];
+            n0[j] = occ[idx[j]];
+        }
+        if (err[c]) continue;
+        const int rc = orc_nl_callaway_evolve(static_cast<int>(k), w.data(), q.data(), gn.data(), gr.data(),
+                                              n0.data(), T, dt, out.data());
+        if (rc) {
+            err[c] = rc;
+            continue;
+        }
+        for (size_t j = 0; j < k; ++j) {
+            next[idx[j]] = out[j];
+            residual[c] += kHbar * w[j] * (static_cast<LD>(out[j]) - n0[j]);
+        }
+        residual[c] *= energy_weight;
+    }
+    *status = 0;
+    for (int e : err)
+        if (e) {
+            *status = e;
+            return 0.0;
+        }
+    LD sum = 0;
+    for (auto x : residual) sum += x;
+    std::copy(next.begin(), next.end(), occ);
+    return static_cast<double>(sum);
+}
+
+} // extern "C"
+
 // ---------------------------------------------------------------- whole-step simulator
 struct orc_sim {
     pmc_setup setup;
@@ -871,10 +1126,17 @@ struct orc_sim {
     double band = 0.0;
     std::vector<double> planes, e_bg_mat, accept, bank_omega, bank_prefix;
     std::vector<int> mode_begin, cell_mat, lut_begin, bank_begin, bank_mode;
-    // linearized Callaway transport
+    // Transport policy (include/SimulationConfig.h:148-156): conservative walls
+    // (strict-elastic tables on every reflecting facet, signed occupations) and
+    // linearized occupations; the collision operator is RTA, linearized
+    // Callaway (callaway) or nonlinear Callaway (nl_callaway).
+    bool conservative = false, linearized = false;
     bool callaway = false;
     pmc_callaway_table cw{};
     std::vector<double> cw_cap, cw_q, cw_gn, cw_gr;
+    bool nl_callaway = false;
+    pmc_callaway_rates nl{};
+    std::vector<double> nl_t, nl_gn, nl_gu, nl_ge, nl_q;
 };
 
 namespace {
@@ -1104,7 +1366,7 @@ int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
         }
         if (F.condition == 'P' && F.periodic_pair >= 0) {
             s.pos[i] = add(s.pos[i], V3{F.shift[0], F.shift[1], F.shift[2]});
-        } else if (F.rough_table >= 0 && (F.condition == 'R' || s.callaway)) {
+        } else if (F.rough_table >= 0 && (F.condition == 'R' || s.conservative)) {
             // Conservative transport reflects every reflecting facet through its
             // table with signed occupations and needs an inward outgoing mode
             // (src/solver/BoundaryInteraction.cpp:90-113).
@@ -1115,12 +1377,12 @@ int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
             int out, kind;
             double o;
             orc_rough_sample_signed(&s.m, &s.rough[F.rough_table], a, s.occ[i], u, &out, &o, &kind,
-                                    s.callaway ? 1 : 0);
+                                    s.conservative ? 1 : 0);
             tally.rough_events++;
             if (kind == 1) tally.rough_spec++;
             if (kind == 2) tally.rough_diff++;
             V3 v = table_velocity(s, out);
-            if (s.callaway) {
+            if (s.conservative) {
                 if (!std::isfinite(o) || !(s.m.omega[out] > 0.0) || !(dot(v, n) < 0.0)) return -kOrcModeMapping;
             } else if (dot(v, n) > 0.0) {
                 v = sub(v, mul(n, 2.0 * dot(v, n)));
@@ -1128,7 +1390,7 @@ int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
             s.mode[i] = out;
             s.vel[i] = v;
             s.occ[i] = o;
-        } else if (s.callaway) {
+        } else if (s.conservative) {
             return -kOrcModeMapping;
         } else if (F.condition == 'R') {
             return -PMC_ERR_UNSUPPORTED;
@@ -1187,7 +1449,7 @@ void refresh(orc_sim& s) {
                                  s.lut_begin[k + 1] - s.lut_begin[k], s.cell_e[c]);
         }
     }
-    if (s.callaway) // T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225)
+    if (s.linearized) // T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225)
         for (int c = 0; c < nc; ++c)
             s.cell_T[c] = s.setup.background_temperature +
                           (s.cell_e[c] - s.cw.background_energy_density) / s.cw.heat_capacity_density;
@@ -1234,7 +1496,7 @@ int apply_source(orc_sim& s, double factor, double* energy_out, int64_t* updates
     std::vector<int32_t> act(s.mode.begin(), s.mode.end());
     return orc_source_deposit(&s.m, nc, s.hs_pending.data(), s.cell_T.data(), s.setup.background_temperature,
                               s.hs_elig.data(), static_cast<int64_t>(act.size()), act.data(), s.cell.data(),
-                              s.occ.data(), s.callaway ? s.cw_cap.data() : nullptr, energy_out, updates_out);
+                              s.occ.data(), s.linearized ? s.cw_cap.data() : nullptr, energy_out, updates_out);
 }
 } // namespace
 
@@ -1260,6 +1522,7 @@ int orc_create(const pmc_setup* setup, const pmc_mode_table* modes, const pmc_ge
     s->g.facets = s->facets.data();
     s->rough.assign(rough, rough + n_rough);
     s->res.assign(res, res + n_res);
+    s->conservative = setup->conservative_walls != 0;
     s->k0 = static_cast<uint32_t>(setup->seed & 0xffffffffu);
     s->k1 = static_cast<uint32_t>(setup->seed >> 32) ^ (setup->instance * 0x9E3779B9u);
     s->cell_e.assign(geometry->n_cells, 0.0);
@@ -1362,7 +1625,7 @@ int orc_initialize(orc_sim* s, const pmc_init* init) {
         if (init->temperature_mode == 2) T = init->cell_temperature[orc_grid_index(&s->g, pf.data())];
         s->pos[i] = pf;
         s->mode[i] = m;
-        if (s->callaway) { // PhononMaterial::linearized_bose_occupation (Thermodynamics.cpp:27-37)
+        if (s->linearized) { // PhononMaterial::linearized_bose_occupation (Thermodynamics.cpp:27-37)
             const double hw = kHbar * s->m.omega[m];
             s->occ[i] = hw > 0.0 ? s->m.neq_background[m] +
                                        s->cw_cap[m] * (T - s->setup.background_temperature) / hw
@@ -1400,6 +1663,29 @@ int orc_set_callaway(orc_sim* s, const pmc_callaway_table* t) {
     s->cw.normal_rate = s->cw_gn.data();
     s->cw.resistive_rate = s->cw_gr.data();
     s->callaway = true;
+    s->conservative = true;
+    s->linearized = true;
+    return PMC_OK;
+}
+
+int orc_set_callaway_nonlinear(orc_sim* s, const pmc_callaway_rates* r) {
+    if (!s || !r || r->n_temps <= 0 || !r->temps || !r->gamma_normal || !r->gamma_umklapp || !r->gamma_extra ||
+        !r->wavevector)
+        return PMC_ERR_ARGUMENT;
+    const size_t na = static_cast<size_t>(s->m.n_active), nt = static_cast<size_t>(r->n_temps);
+    s->nl_t.assign(r->temps, r->temps + nt);
+    s->nl_gn.assign(r->gamma_normal, r->gamma_normal + nt * na);
+    s->nl_gu.assign(r->gamma_umklapp, r->gamma_umklapp + nt * na);
+    s->nl_ge.assign(r->gamma_extra, r->gamma_extra + nt * na);
+    s->nl_q.assign(r->wavevector, r->wavevector + 3 * na);
+    s->nl = *r;
+    s->nl.temps = s->nl_t.data();
+    s->nl.gamma_normal = s->nl_gn.data();
+    s->nl.gamma_umklapp = s->nl_gu.data();
+    s->nl.gamma_extra = s->nl_ge.data();
+    s->nl.wavevector = s->nl_q.data();
+    s->nl_callaway = true;
+    s->conservative = true; // nonlinear occupations, conservative walls (include/SimulationConfig.h:153-156)
     return PMC_OK;
 }
 
@@ -1607,6 +1893,16 @@ int orc_step(orc_sim* s, int32_t nsteps) {
                                           s->cell.data(), s->occ.data(), dt, s->m.energy_weight, &status);
             if (status) {
                 set_error(s, "Callaway collision failure");
+                return PMC_ERR_INVALID_STATE;
+            }
+        } else if (s->nl_callaway) {
+            int status = 0;
+            residual = orc_nl_callaway_apply(&s->m, &s->nl, s->g.n_cells, s->cell_T.data(), s->setup.lifetime_local,
+                                             s->setup.lifetime_temperature, static_cast<int64_t>(act.size()),
+                                             act.data(), s->cell.data(), s->occ.data(), dt, s->m.energy_weight,
+                                             &status);
+            if (status) {
+                set_error(s, "nonlinear Callaway collision failure " + std::to_string(status));
                 return PMC_ERR_INVALID_STATE;
             }
         } else {

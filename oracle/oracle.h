@@ -102,6 +102,27 @@ int orc_interface_dmm(orc_sim* sim, int active, const double vel[3], double occ,
  * collisions; the reservoir occupations and refill policy come from the setup
  * and reservoir tables. */
 int orc_set_callaway(orc_sim* sim, const pmc_callaway_table* table);
+/* Nonlinear Callaway transport (before initialization): Bose occupations,
+ * lookup-table cell temperatures, conservative reflections and the nonlinear
+ * N/R substeps. */
+int orc_set_callaway_nonlinear(orc_sim* sim, const pmc_callaway_rates* rates);
+/* nonlinear_callaway_evolve (src/solver/NonlinearCallawayCollision.cpp:251-285) on
+ * one cell of n carriers (rates in 1/ps, wavevector [n*3]); returns 0, or the
+ * reason it failed: 1 no representable energy constraint, 2 singular moment
+ * Jacobian, 3 line search failed, 4 no convergence, 5 invalid occupation,
+ * 6 invalid state or mode. */
+int orc_nl_callaway_evolve(int n, const double* omega, const double* wavevector, const double* normal_rate,
+                           const double* resistive_rate, const double* occupation, double T, double dt, double* out);
+/* PhononMaterial::mode_callaway_rates at T for one active mode; 7 when T is
+ * outside the table. */
+int orc_nl_callaway_rates(const pmc_mode_table* m, const pmc_callaway_rates* r, double T, int active, double* normal,
+                          double* resistive);
+/* NonlinearCallawayCollision::apply over all cells; returns the residual (eV),
+ * *status as orc_nl_callaway_evolve or 7. */
+double orc_nl_callaway_apply(const pmc_mode_table* m, const pmc_callaway_rates* r, int ncells,
+                             const double* cell_temperature, int lifetime_local, double lifetime_temperature,
+                             int64_t n, const int32_t* active, const int32_t* cell, double* occ, double dt,
+                             double energy_weight, int* status);
 /* Live carriers sorted by uid. */
 int64_t orc_carrier_count(orc_sim* sim);
 int orc_carriers(orc_sim* sim, double* position, int32_t* mode, double* occupation, int32_t* cell, uint32_t* uid);

@@ -64,6 +64,14 @@ def lib():
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_heat_source.argtypes = [P, C.POINTER(rt.HeatSource)]
         L.orc_set_callaway.argtypes = [P, C.POINTER(rt.CallawayTable)]
+        L.orc_set_callaway_nonlinear.argtypes = [P, C.POINTER(rt.CallawayRates)]
+        L.orc_nl_callaway_evolve.argtypes = [C.c_int] + [C.POINTER(d)] * 5 + [d, d, C.POINTER(d)]
+        L.orc_nl_callaway_rates.argtypes = [C.POINTER(rt.ModeTable), C.POINTER(rt.CallawayRates), d, C.c_int,
+                                            C.POINTER(d), C.POINTER(d)]
+        L.orc_nl_callaway_apply.restype = d
+        L.orc_nl_callaway_apply.argtypes = [C.POINTER(rt.ModeTable), C.POINTER(rt.CallawayRates), C.c_int,
+                                            C.POINTER(d), C.c_int, d, C.c_int64, C.POINTER(C.c_int32),
+                                            C.POINTER(C.c_int32), C.POINTER(d), d, d, C.POINTER(C.c_int)]
         L.orc_set_layers.argtypes = [P, C.POINTER(rt.Layers)]
         L.orc_interface_dmm.argtypes = [P, C.c_int, C.POINTER(d), d, C.c_int, C.POINTER(d), C.POINTER(C.c_int),
                                         C.POINTER(d), C.POINTER(d)]
@@ -119,6 +127,9 @@ class Sim:
         cw = problem.callaway()
         if cw is not None:
             self._check(lib().orc_set_callaway(h, C.byref(cw)))
+        nl = problem.callaway_nonlinear()
+        if nl is not None:
+            self._check(lib().orc_set_callaway_nonlinear(h, C.byref(nl)))
         hs = problem.heat_source()
         if hs is not None:
             self._check(lib().orc_set_heat_source(h, C.byref(hs)))
@@ -335,3 +346,40 @@ def callaway_apply(problem, active, cell, occupation, dt):
     if status.value:
         raise RuntimeError(f"orc_callaway_apply failed: {status.value}")
     return occ, res
+
+
+def nl_callaway_evolve(omega, wavevector, normal_rate, resistive_rate, occupation, temperature, dt):
+    """orc_nl_callaway_evolve on one cell: (status, new occupations)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (omega, np.reshape(wavevector, -1), normal_rate,
+                                                                resistive_rate, occupation)]
+    out = np.zeros(len(arrs[0]))
+    rc = lib().orc_nl_callaway_evolve(len(out), *[_dp(a) for a in arrs], temperature, dt, _dp(out))
+    return rc, out
+
+
+def nl_callaway_rates(problem, temperature, active):
+    """orc_nl_callaway_rates of active modes: (status, normal, resistive)."""
+    _, m, _, _, _, _ = problem.views()
+    nl = problem.callaway_nonlinear()
+    rn, rr = np.zeros(len(active)), np.zeros(len(active))
+    status = 0
+    for k, a in enumerate(active):
+        x, y = C.c_double(), C.c_double()
+        status = status or lib().orc_nl_callaway_rates(C.byref(m), C.byref(nl), temperature, int(a), C.byref(x),
+                                                       C.byref(y))
+        rn[k], rr[k] = x.value, y.value
+    return status, rn, rr
+
+
+def nl_callaway_apply(problem, cell_temperature, active, cell, occupation, dt, local, lifetime_temperature):
+    """Nonlinear Callaway collisions (orc_nl_callaway_apply): (status, new occupations, residual eV)."""
+    _, m, g, _, _, _ = problem.views()
+    nl = problem.callaway_nonlinear()
+    ct = np.ascontiguousarray(cell_temperature, dtype=np.float64)
+    a = np.ascontiguousarray(active, dtype=np.int32)
+    c = np.ascontiguousarray(cell, dtype=np.int32)
+    occ = np.array(occupation, dtype=np.float64)
+    status = C.c_int()
+    res = lib().orc_nl_callaway_apply(C.byref(m), C.byref(nl), g.n_cells, _dp(ct), int(local), lifetime_temperature,
+                                      len(a), _ip(a), _ip(c), _dp(occ), dt, m.energy_weight, C.byref(status))
+    return status.value, occ, res

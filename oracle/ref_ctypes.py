@@ -417,3 +417,47 @@ def callaway_apply(material, t_bg, ncells, active, cell, occupation, dt, particl
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return occ, res.value
+
+
+def nl_callaway_evolve(omega, wavevector, normal_rate, resistive_rate, occupation, temperature, dt):
+    """nonlinear_callaway_evolve on one cell: the new occupations, or RuntimeError
+    with the reference's message."""
+    n = len(omega)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (omega, np.reshape(wavevector, -1), normal_rate,
+                                                                resistive_rate, occupation)]
+    out = np.zeros(n)
+    err = C.create_string_buffer(512)
+    rc = lib().ref_nl_callaway_evolve(n, *[_dp(a) for a in arrs], C.c_double(temperature), C.c_double(dt), _dp(out),
+                                      err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return out
+
+
+def nl_callaway_rates(material, temperature, active):
+    """PhononMaterial::mode_callaway_rates(T) of active modes: (normal, resistive) 1/ps."""
+    active = np.ascontiguousarray(active, dtype=np.int32)
+    rn, rr = np.zeros(len(active)), np.zeros(len(active))
+    err = C.create_string_buffer(512)
+    rc = lib().ref_nl_callaway_rates(material.h, C.c_double(temperature), len(active), _ip(active), _dp(rn), _dp(rr),
+                                     err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return rn, rr
+
+
+def nl_callaway_apply(material, t_bg, cell_temperature, local, lifetime_temperature, active, cell, occupation, dt,
+                      particle_volume):
+    """NonlinearCallawayCollision::configure(T_bg) + apply: (new occupations, residual eV)."""
+    active = np.ascontiguousarray(active, dtype=np.int32)
+    cell = np.ascontiguousarray(cell, dtype=np.int32)
+    ct = np.ascontiguousarray(cell_temperature, dtype=np.float64)
+    occ = np.array(occupation, dtype=np.float64)
+    res = C.c_double()
+    err = C.create_string_buffer(512)
+    rc = lib().ref_nl_callaway_apply(material.h, C.c_double(t_bg), len(ct), _dp(ct), int(local),
+                                     C.c_double(lifetime_temperature), len(active), _ip(active), _ip(cell), _dp(occ),
+                                     C.c_double(dt), C.c_double(particle_volume), C.byref(res), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return occ, res.value

@@ -19,6 +19,7 @@
 #include "solver/RoughBoundarySampler.h"
 #include "solver/RtaCollisionOperator.h"
 #include "solver/CarrierCollisionOperator.h"
+#include "solver/NonlinearCallawayCollision.h"
 #include "solver/InterfaceInteraction.h"
 #include "solver/InterfaceModeSampler.h"
 
@@ -176,6 +177,62 @@ int ref_callaway_apply(void* material, double t_bg, int ncells, int n, const int
         phonomc::CarrierCollisionOperator op;
         op.configure({&m}, t_bg);
         *residual_ev = op.apply(p, {&m}, {index.counts(), materials, temps, &index}, dt, particle_volume);
+        for (int i = 0; i < n; ++i) occupation[i] = p.occupation[i];
+    });
+}
+
+int ref_nl_callaway_evolve(int n, const double* omega, const double* wavevector, const double* normal_rate,
+                           const double* resistive_rate, const double* occupation, double temperature, double dt,
+                           double* out, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        phonomc::CallawayData d;
+        d.omega.assign(omega, omega + n);
+        d.normal_rate.assign(normal_rate, normal_rate + n);
+        d.resistive_rate.assign(resistive_rate, resistive_rate + n);
+        d.wavevector.resize(static_cast<size_t>(n));
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) d.wavevector[static_cast<size_t>(i)][static_cast<size_t>(k)] = wavevector[3 * i + k];
+        const auto r = phonomc::nonlinear_callaway_evolve(d, std::vector<double>(occupation, occupation + n),
+                                                          temperature, dt);
+        std::copy(r.begin(), r.end(), out);
+    });
+}
+
+int ref_nl_callaway_rates(void* material, double temperature, int n, const int* active, double* normal_rate,
+                          double* resistive_rate, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& m = mat(material);
+        for (int i = 0; i < n; ++i) {
+            const auto r = m.mode_callaway_rates(temperature, m.active_mode_at(active[i]));
+            normal_rate[i] = r[0];
+            resistive_rate[i] = r[1];
+        }
+    });
+}
+
+int ref_nl_callaway_apply(void* material, double t_bg, int ncells, const double* cell_temperature, int local,
+                          double lifetime_temperature, int n, const int* active, const int* cell, double* occupation,
+                          double dt, double particle_volume, double* residual_ev, char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        const auto& m = mat(material);
+        phonomc::ParticleStorage p;
+        p.reset(n);
+        for (int i = 0; i < n; ++i) {
+            p.modes[i] = m.active_mode_at(active[i]);
+            p.grid_ids[i] = cell[i];
+            p.occupation[i] = occupation[i];
+            p.temperatures[i] = cell_temperature[cell[i]];
+            p.material_ids[i] = 0;
+        }
+        p.material_spatial_volumes_a3 = {particle_volume};
+        phonomc::CellParticleIndex index;
+        index.ensure(p, ncells);
+        std::vector<int> materials(ncells, 0);
+        std::vector<double> temps(cell_temperature, cell_temperature + ncells);
+        phonomc::NonlinearCallawayCollision op;
+        op.configure({&m}, t_bg);
+        *residual_ev = op.apply(p, {&m}, {index.counts(), materials, temps, &index},
+                                {dt, particle_volume, local != 0, lifetime_temperature});
         for (int i = 0; i < n; ++i) occupation[i] = p.occupation[i];
     });
 }

@@ -48,6 +48,17 @@ int ref_callaway_data(void* material, double temperature, double* omega, double*
 int ref_callaway_apply(void* material, double t_bg, int ncells, int n, const int* active, const int* cell,
                        double* occupation, double dt, double particle_volume, double* residual_ev, char* err,
                        int errlen);
+/* nonlinear_callaway_evolve (src/solver/NonlinearCallawayCollision.cpp:251-285) on one cell. */
+int ref_nl_callaway_evolve(int n, const double* omega, const double* wavevector, const double* normal_rate,
+                           const double* resistive_rate, const double* occupation, double temperature, double dt,
+                           double* out, char* err, int errlen);
+/* PhononMaterial::mode_callaway_rates (src/material/PhononMaterialCollisionData.cpp:167-196). */
+int ref_nl_callaway_rates(void* material, double temperature, int n, const int* active, double* normal_rate,
+                          double* resistive_rate, char* err, int errlen);
+/* NonlinearCallawayCollision::configure(T_bg) + apply (:251-344) on a carrier set. */
+int ref_nl_callaway_apply(void* material, double t_bg, int ncells, const double* cell_temperature, int local,
+                          double lifetime_temperature, int n, const int* active, const int* cell, double* occupation,
+                          double dt, double particle_volume, double* residual_ev, char* err, int errlen);
 void ref_material_destroy(void* material);
 int ref_material_active_count(void* material);
 /* Active-mode list in the reference's sampling order (src/PhononMaterial.cpp:136-156). */

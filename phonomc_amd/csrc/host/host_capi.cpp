@@ -126,6 +126,13 @@ int pmcs_problem_callaway(const pmcs_problem* pr, pmc_callaway_table* out) {
     return p.linearized ? 1 : 0;
 }
 
+int pmcs_problem_callaway_nonlinear(const pmcs_problem* pr, pmc_callaway_rates* out) {
+    if (!pr) return -1;
+    const auto& p = *(pr->operator->());
+    if (out) *out = p.nl_view;
+    return p.nonlinear ? 1 : 0;
+}
+
 int pmcs_problem_material(const pmcs_problem* pr, pmcs_material_view* out) {
     if (!pr || !out) return PMC_ERR_ARGUMENT;
     fill_material((*pr)->raw, (*pr)->config.temperature_lookup_dt, out);

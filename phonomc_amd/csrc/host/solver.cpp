@@ -224,19 +224,25 @@ void layered_reservoirs(Problem& p) {
 } // namespace
 
 std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions& opt) {
-    if (c.collision == CollisionModel::Callaway && c.callaway_nonlinear)
-        throw std::runtime_error("scattering.callaway_formulation = 'nonlinear' is not supported by this build.");
-    // Linearized Callaway: fixed-reference, conservative transport
-    // (include/SimulationConfig.h:148-156).
-    const bool linearized = c.collision == CollisionModel::Callaway;
-    if (linearized && (!std::isfinite(c.background_temperature) || c.background_temperature <= 0))
+    // Transport policies (include/SimulationConfig.h:148-156): linearized Callaway
+    // is linearized, conservative transport; nonlinear Callaway and
+    // simulation.conservative_boundaries are conservative transport with Bose
+    // occupations.
+    const bool nonlinear = c.collision == CollisionModel::Callaway && c.callaway_nonlinear;
+    const bool linearized = c.collision == CollisionModel::Callaway && !nonlinear;
+    const bool conservative = linearized || nonlinear || c.conservative_boundaries;
+    if (c.collision == CollisionModel::Callaway &&
+        (!std::isfinite(c.background_temperature) || c.background_temperature <= 0))
         throw std::runtime_error("Callaway requires a positive fixed background_temperature.");
     if (c.heat_source_enabled && c.has_non_thermal_spectrum)
         throw std::runtime_error("heat_source.spectrum other than 'thermal' is not supported by this build.");
     if (c.has_gradient_drive) throw std::runtime_error("driving.temperature_gradient is not supported by this build.");
     const bool layered = c.material_folders.size() > 1;
     if (layered) {
-        if (linearized) throw std::runtime_error("Layered materials with scattering.model = 'callaway' are not supported by this build.");
+        if (c.collision == CollisionModel::Callaway)
+            throw std::runtime_error("Layered materials with scattering.model = 'callaway' are not supported by this build.");
+        if (conservative)
+            throw std::runtime_error("Layered materials with simulation.conservative_boundaries are not supported by this build.");
         if (c.heat_source_enabled) throw std::runtime_error("Layered materials with a heat source are not supported by this build.");
         if (c.interface_model != "dmm")
             throw std::runtime_error("materials.interface_model = '" + c.interface_model + "' is not supported by this build; use dmm.");
@@ -245,7 +251,6 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     }
     if (c.has_convergence_stop) throw std::runtime_error("convergence.enabled requires gradient driving.");
     if (c.has_mode_excitation) throw std::runtime_error("mode_excitation is not supported by this build.");
-    if (c.conservative_boundaries) throw std::runtime_error("simulation.conservative_boundaries is not supported by this build.");
     if (c.material_folder.empty() && c.material_folders.empty()) throw std::runtime_error("io.material_folder is required.");
 
     auto p = std::make_unique<Problem>();
@@ -253,10 +258,12 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     // Stationary modes join the active bank under conservative transport
     // (src/material/MaterialFactory.cpp:51-55).
     const auto material = cached_material(c.material_folders.empty() ? c.material_folder : c.material_folders.front(),
-                                          c.input_directory, c.temperature_lookup_dt, linearized);
+                                          c.input_directory, c.temperature_lookup_dt, conservative);
     p->raw = material->raw;
     p->modes = material->modes;
     p->linearized = linearized;
+    p->conservative = conservative;
+    p->nonlinear = nonlinear;
     p->domain = build_domain(c);
     const auto& m = p->modes;
     const auto& d = p->domain;
@@ -290,6 +297,36 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         cv.heat_capacity_density = p->heat_capacity_density;
         cv.background_energy_density = p->background_energy_density;
     }
+    if (nonlinear) {
+        // NonlinearCallawayCollision::configure validates callaway_data(T_bg)
+        // (src/solver/NonlinearCallawayCollision.cpp:287-301); apply then reads
+        // mode_callaway_rates at the rate temperature and the Cartesian q
+        // (src/material/PhononMaterialCollisionData.cpp:167-196).
+        const CallawayTable checked = callaway_data(m, material->completed, c.background_temperature);
+        auto& r = p->nl_rates;
+        const size_t na = static_cast<size_t>(m.n_active), nt = m.temps.size(), nf = static_cast<size_t>(m.n_flat);
+        r.temps = m.temps;
+        r.gamma_normal.resize(nt * na);
+        r.gamma_umklapp.resize(nt * na);
+        r.gamma_extra.resize(nt * na);
+        for (size_t t = 0; t < nt; ++t)
+            for (size_t a = 0; a < na; ++a) {
+                const size_t f = t * nf + static_cast<size_t>(m.active_flat[a]);
+                r.gamma_normal[t * na + a] = m.gamma_normal[f];
+                r.gamma_umklapp[t * na + a] = m.gamma_umklapp[f];
+                r.gamma_extra[t * na + a] = m.gamma_extra[f];
+            }
+        r.wavevector.resize(3 * na);
+        for (size_t a = 0; a < na; ++a)
+            for (int k = 0; k < 3; ++k) r.wavevector[3 * a + k] = checked.wavevector[a][k];
+        auto& v = p->nl_view;
+        v.n_temps = static_cast<int32_t>(nt);
+        v.temps = r.temps.data();
+        v.gamma_normal = r.gamma_normal.data();
+        v.gamma_umklapp = r.gamma_umklapp.data();
+        v.gamma_extra = r.gamma_extra.data();
+        v.wavevector = r.wavevector.data();
+    }
     p->neq_background.resize(m.n_active);
     p->omega_active.resize(m.n_active);
     p->velocity_active.resize(3 * static_cast<size_t>(m.n_active));
@@ -307,9 +344,9 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     for (size_t f = 0; f < d.facets.size(); ++f) {
         const auto& fc = d.facets[f];
         const bool reflecting = fc.condition != 'T' && !(fc.condition == 'P' && fc.periodic_pair >= 0);
-        enabled[f] = linearized ? reflecting : fc.condition == 'R';
+        enabled[f] = conservative ? reflecting : fc.condition == 'R';
     }
-    p->rough = build_rough_tables(d, m, enabled, linearized ? c.background_temperature : 0.0, p->facet_to_rough);
+    p->rough = build_rough_tables(d, m, enabled, conservative ? c.background_temperature : 0.0, p->facet_to_rough);
     if (layered) layered_reservoirs(*p);
     else p->reservoirs = build_reservoir_tables(d, m, p->particle_density, c.time_step, p->facet_to_reservoir);
     if (linearized) {
@@ -539,9 +576,10 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     s.lifetime_temperature = c.lifetime_temperature;
     // Fixed incoming flux under conservative transport, else one-to-one refill
     // (src/solver/ReservoirInitialization.cpp:30-33).
-    s.refill_policy = linearized ? 1 : 0;
+    s.refill_policy = conservative ? 1 : 0;
     s.mode_sampling = opt.mode_sampling;
     s.temperature_inversion = 0;
+    s.conservative_walls = conservative ? 1 : 0;
 
     // Initial temperatures (src/solver/Initialization.cpp:181-265).
     auto& in = p->init;
@@ -596,6 +634,7 @@ MonteCarloSolver::MonteCarloSolver(std::unique_ptr<Problem> problem, std::string
     throw_if_failed(pmc_upload_boundary_tables(ctx_, static_cast<int>(p.rough_views.size()), p.rough_views.data(),
                                                static_cast<int>(p.reservoir_views.size()), p.reservoir_views.data()));
     if (p.linearized) throw_if_failed(pmc_upload_callaway(ctx_, &p.callaway_view));
+    if (p.nonlinear) throw_if_failed(pmc_upload_callaway_nonlinear(ctx_, &p.nl_view));
     if (p.layer_modes.size() > 1) throw_if_failed(pmc_upload_layers(ctx_, &p.layers_view));
     if (p.heat.enabled) throw_if_failed(pmc_upload_heat_source(ctx_, &p.heat_view));
     throw_if_failed(pmc_initialize_carriers(ctx_, &p.init));
@@ -877,7 +916,7 @@ void MonteCarloSolver::append_profile_summary(std::ostream& out) const {
     out << "recovery_events = 0\n";
     out << "recovered_particles_total = 0\n";
     out << "\n[reservoir_balance_diagnostics]\n";
-    out << "reservoir_gen = " << (p.linearized ? "fixed_incoming_flux" : "one_to_one") << '\n';
+    out << "reservoir_gen = " << (p.conservative ? "fixed_incoming_flux" : "one_to_one") << '\n';
     out << "absorbed_particles_total = " << l.total_absorbed << '\n';
     out << "injected_particles_total = " << l.total_injected << '\n';
     out << "recovered_particles_total = 0\n";

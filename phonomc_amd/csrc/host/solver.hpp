@@ -43,6 +43,16 @@ struct Problem {
     std::vector<double> callaway_wavevector; // 3 per active mode
     double heat_capacity_density = 0;
     pmc_callaway_table callaway_view{};
+    // Conservative transport (strict-elastic walls, fixed incoming flux,
+    // stationary modes in the bank): linearized Callaway, nonlinear Callaway or
+    // simulation.conservative_boundaries.
+    bool conservative = false;
+    // Nonlinear Callaway (scattering.callaway_formulation = 'nonlinear').
+    bool nonlinear = false;
+    struct NonlinearRates {
+        std::vector<double> temps, gamma_normal, gamma_umklapp, gamma_extra, wavevector; // by (T sample, active)
+    } nl_rates;
+    pmc_callaway_rates nl_view{};
     // Layered materials behind pmc_layers (materials.folders with more than one entry).
     struct LayerArrays {
         int count = 1;

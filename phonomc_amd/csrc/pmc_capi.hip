@@ -3,6 +3,7 @@
 #include "pmc_capi.h"
 #include "pmc_device.hpp"
 #include "pmc_kernels.hpp"
+#include "pmc_nlcallaway.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -161,6 +162,25 @@ int collect_error(pmc_ctx* ctx) {
             std::snprintf(msg, sizeof msg, "Callaway exponential work limit exceeded; reduce time_step.");
         else if (st.error_facet == -10)
             std::snprintf(msg, sizeof msg, "Callaway exponential did not converge.");
+        else if (st.error_facet == -20 - pmc_nl::kNoEnergyConstraint)
+            std::snprintf(msg, sizeof msg, "Nonlinear Callaway has no representable energy constraint");
+        else if (st.error_facet == -20 - pmc_nl::kSingularJacobian)
+            std::snprintf(msg, sizeof msg, "Singular nonlinear Callaway moment Jacobian");
+        else if (st.error_facet == -20 - pmc_nl::kLineSearch)
+            std::snprintf(msg, sizeof msg, "Nonlinear Callaway moment line search failed");
+        else if (st.error_facet == -20 - pmc_nl::kNoConvergence)
+            std::snprintf(msg, sizeof msg, "Nonlinear Callaway moment solve did not converge");
+        else if (st.error_facet == -20 - pmc_nl::kInvalidOccupation)
+            std::snprintf(msg, sizeof msg, "Nonlinear Callaway occupation is invalid");
+        else if (st.error_facet == -20 - pmc_nl::kTemperatureRange) {
+            // the rate temperature: the fixed lifetime temperature or cell i's
+            double t = ctx->view.lifetime_T;
+            if (ctx->view.lifetime_local && i >= 0 && i < ctx->view.ncells) {
+                pmc_dev::CellRta cr{};
+                if (hipMemcpy(&cr, ctx->view.crta + i, sizeof cr, hipMemcpyDeviceToHost) == hipSuccess) t = cr.t0;
+            }
+            std::snprintf(msg, sizeof msg, "Nonlinear Callaway temperature %f K is outside the supplied N/U table", t);
+        }
         else
             std::snprintf(msg, sizeof msg, "Invalid particle in grid energy statistics.");
         break;
@@ -343,7 +363,8 @@ int pmc_create(const pmc_setup* setup, pmc_ctx** out) {
     *out = nullptr;
     if (!(setup->time_step_ps > 0) || !std::isfinite(setup->time_step_ps) || setup->capacity <= 0 ||
         setup->capacity > (1ll << 30) || !(setup->push_distance_a >= 0) ||
-        (setup->temperature_inversion != 0 && setup->temperature_inversion != 1))
+        (setup->temperature_inversion != 0 && setup->temperature_inversion != 1) ||
+        (setup->conservative_walls != 0 && setup->conservative_walls != 1))
         return PMC_ERR_ARGUMENT;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return PMC_ERR_DEVICE;
@@ -369,6 +390,7 @@ int pmc_create(const pmc_setup* setup, pmc_ctx** out) {
     v.series_radius = 0.01;
     v.t_bg = setup->background_temperature;
     v.refill_fixed = setup->refill_policy == 1 ? 1 : 0;
+    v.conservative = setup->conservative_walls != 0 ? 1 : 0;
     v.cw_threads = pmc_dev::kCwThreads;
     for (auto& b : ctx->buf) {
         const size_t n = static_cast<size_t>(setup->capacity);
@@ -677,6 +699,7 @@ int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* t) {
                     "T_bg + (e - E_bg) / C_bg");
     if (ctx->initialized)
         return fail(ctx, PMC_ERR_ARGUMENT, "the Callaway table must be uploaded before the population is created");
+    if (ctx->view.nl_callaway) return fail(ctx, PMC_ERR_ARGUMENT, "the context already runs nonlinear Callaway");
     if (!(t->heat_capacity_density > 0.0) || !std::isfinite(t->heat_capacity_density) ||
         !std::isfinite(t->background_energy_density))
         return fail(ctx, PMC_ERR_ARGUMENT, "Linearized temperature requires positive heat capacity.");
@@ -720,8 +743,50 @@ int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* t) {
     v.cw_cbg = t->heat_capacity_density;
     v.e_bg = t->background_energy_density;
     v.callaway = 1;
+    v.conservative = 1;
+    v.linearized = 1;
     v.cw_threads = pmc_dev::kCwThreads;
     return check_hip(ctx, hipGetLastError(), "Callaway table");
+}
+
+int pmc_upload_callaway_nonlinear(pmc_ctx* ctx, const pmc_callaway_rates* r) {
+    if (!ctx) return PMC_ERR_ARGUMENT;
+    if (!r || r->n_temps <= 0 || !r->temps || !r->gamma_normal || !r->gamma_umklapp || !r->gamma_extra ||
+        !r->wavevector)
+        return fail(ctx, PMC_ERR_ARGUMENT, "nonlinear Callaway rate tables missing");
+    if (!ctx->has_material || !ctx->has_geometry)
+        return fail(ctx, PMC_ERR_ARGUMENT, "material and geometry must be uploaded before the Callaway tables");
+    if (ctx->initialized)
+        return fail(ctx, PMC_ERR_ARGUMENT, "the Callaway tables must be uploaded before the population is created");
+    if (ctx->view.callaway) return fail(ctx, PMC_ERR_ARGUMENT, "the context already runs linearized Callaway");
+    if (ctx->view.inversion == 1)
+        return fail(ctx, PMC_ERR_UNSUPPORTED, "Newton temperature inversion applies to RTA transport");
+    if (ctx->view.layers > 1)
+        return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run RTA without a heat source");
+    View& v = ctx->view;
+    const size_t na = static_cast<size_t>(v.na), nt = static_cast<size_t>(r->n_temps);
+    for (size_t k = 0; k < nt; ++k)
+        if (!std::isfinite(r->temps[k]) || (k && !(r->temps[k] > r->temps[k - 1])))
+            return fail(ctx, PMC_ERR_ARGUMENT, "rate temperatures must be finite and increasing");
+    for (size_t k = 0; k < nt * na; ++k)
+        if (!std::isfinite(r->gamma_normal[k]) || r->gamma_normal[k] < 0 || !std::isfinite(r->gamma_umklapp[k]) ||
+            r->gamma_umklapp[k] < 0 || !std::isfinite(r->gamma_extra[k]) || r->gamma_extra[k] < 0)
+            return fail(ctx, PMC_ERR_ARGUMENT, "Invalid nonlinear Callaway mode");
+    for (size_t k = 0; k < 3 * na; ++k)
+        if (!std::isfinite(r->wavevector[k])) return fail(ctx, PMC_ERR_ARGUMENT, "Invalid nonlinear Callaway wavevector");
+    v.nl_nt = r->n_temps;
+    v.nl_temps = ctx->upload(r->temps, nt);
+    v.nl_gn = ctx->upload(r->gamma_normal, nt * na);
+    v.nl_gu = ctx->upload(r->gamma_umklapp, nt * na);
+    v.nl_ge = ctx->upload(r->gamma_extra, nt * na);
+    v.nl_q = ctx->upload(r->wavevector, 3 * na);
+    v.nl_scratch = ctx->alloc<double>(7 * static_cast<size_t>(v.capacity));
+    if (!v.nl_temps || !v.nl_gn || !v.nl_gu || !v.nl_ge || !v.nl_q || !v.nl_scratch)
+        return fail(ctx, PMC_ERR_DEVICE, "nonlinear Callaway tables");
+    v.nl_callaway = 1;
+    v.conservative = 1;
+    v.linearized = 0;
+    return check_hip(ctx, hipGetLastError(), "nonlinear Callaway tables");
 }
 
 int pmc_initialize_carriers(pmc_ctx* ctx, const pmc_init* init) {
@@ -844,6 +909,8 @@ int pmc_step(pmc_ctx* ctx, int32_t nsteps) {
         });
         if (w.callaway) {
             T.run(PMC_KERNEL_CALLAWAY, [&] { pmc_launch::callaway(w, st); });
+        } else if (w.nl_callaway) {
+            T.run(PMC_KERNEL_CALLAWAY, [&] { pmc_launch::nlcallaway(w, st); });
         } else {
             T.run(PMC_KERNEL_MOMENTS, [&] { pmc_launch::moments(w, st); });
             T.run(PMC_KERNEL_NEWTON, [&] { pmc_launch::newton(w, st); });
@@ -1051,7 +1118,8 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     View& v = ctx->view;
     if (!ctx->has_material || !ctx->has_geometry) return fail(ctx, PMC_ERR_ARGUMENT, "layers need material and geometry");
     if (ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "layers must be uploaded before the population is created");
-    if (v.callaway || v.hs_on) return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run RTA without a heat source");
+    if (v.callaway || v.nl_callaway || v.conservative || v.hs_on)
+        return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run RTA without a heat source or conservative walls");
     if (v.inversion != 0)
         return fail(ctx, PMC_ERR_UNSUPPORTED, "Newton temperature inversion covers one material; layers use their tables");
     if (v.geom_kind != 0) return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials need the box geometry");
@@ -1319,6 +1387,8 @@ int pmc_batch_create(pmc_ctx* const* contexts, int32_t n, pmc_batch** out) {
         if (c->view.layers > 1) return fail(c, PMC_ERR_UNSUPPORTED, "batch: layered instances are not batched");
         if (c->view.inversion != 0)
             return fail(c, PMC_ERR_UNSUPPORTED, "batch: instances with Newton temperature inversion are not batched");
+        if (c->view.nl_callaway)
+            return fail(c, PMC_ERR_UNSUPPORTED, "batch: nonlinear Callaway instances are not batched");
         if (c->owner) return fail(c, PMC_ERR_ARGUMENT, "batch: a context already belongs to a batch");
         if (c->view.geom_kind != c0->view.geom_kind || c->view.callaway != c0->view.callaway)
             return fail(c, PMC_ERR_ARGUMENT, "batch: instances differ in geometry kind or collision model");

@@ -314,7 +314,14 @@ struct View {
     int series_on;        // evaluate pseudo-temperature sums from the per-cell series
     double series_radius; // largest |T0/T - 1| at which the series is tried
     int refill_fixed;     // reservoirs emit floor(expected) + Bernoulli(frac) carriers per step
-    // linearized Callaway transport (fixed reference T_bg)
+    // Transport policy (include/SimulationConfig.h:148-156): discrete-mode
+    // reflection through the strict-elastic tables on every reflecting facet
+    // (conservative), occupations and cell temperatures linearized about T_bg
+    // (linearized).  Linearized Callaway sets both; nonlinear Callaway and
+    // conservative RTA set conservative only.
+    int conservative;
+    int linearized;
+    // linearized Callaway collisions (fixed reference T_bg)
     int callaway;
     const double* cw_cap;   // mode heat capacity at T_bg, eV/K
     const double* cw_q;     // Cartesian wavevector, 3 per mode, 1/A
@@ -326,6 +333,15 @@ struct View {
     double cw_cbg;          // heat-capacity density at T_bg, eV/(K A^3)
     int cw_threads;         // k_callaway workgroup size: 256, 384 or kCwThreads
     int* cw_big;            // cells of more than kCwWaveMax carriers this step (count in DevState)
+    // nonlinear Callaway collisions (rates at the cell or lifetime temperature)
+    int nl_callaway;
+    int nl_nt;              // rate temperature samples
+    const double* nl_temps;
+    const double* nl_gn;    // [nl_nt * na] gamma_N
+    const double* nl_gu;    // [nl_nt * na] gamma_U
+    const double* nl_ge;    // [nl_nt * na] isotope/impurity/defect
+    const double* nl_q;     // Cartesian wavevector, 3 per mode, 1/A
+    double* nl_scratch;     // 7 per slot (structure of arrays): a, metric, n0, 4 directions
 };
 
 // Transport blocks for a slot capacity (a multiple of 8, one per XCD round).

@@ -6,6 +6,7 @@
 // FP64 atomic per wave per tallied quantity.
 #include "pmc_device.hpp"
 #include "pmc_kernels.hpp"
+#include "pmc_nlcallaway.hpp"
 
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -773,7 +774,7 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
         }
         // Conservative transport reflects through the table on every reflecting
         // facet; otherwise only rough facets have one.
-        if (F.rough >= 0 && (F.cond == 'R' || V.callaway)) {
+        if (F.rough >= 0 && (F.cond == 'R' || V.conservative)) {
             if (ev == 0 || U.uid != uid) {
                 U.uid = uid;
                 U.step = step;
@@ -784,7 +785,7 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             U.used = 0;
             int out_ai, kind;
             double out_occ;
-            rough_sample(V, V.rough[F.rough], m, occ, U, out_ai, out_occ, kind, V.callaway != 0);
+            rough_sample(V, V.rough[F.rough], m, occ, U, out_ai, out_occ, kind, V.conservative != 0);
             T.rough_events++;
             if (kind == 1) T.rough_spec++;
             else if (kind == 2) T.rough_diff++;
@@ -794,7 +795,7 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             int nw = out_ai;
             const double* vo = V.veld + 3 * static_cast<size_t>(out_ai);
             const double vn = vo[0] * F.n[0] + vo[1] * F.n[1] + vo[2] * F.n[2];
-            if (V.callaway) {
+            if (V.conservative) {
                 const double wo = V.omega[out_ai];
                 if (!isfinite(out_occ) || !(wo > 0.0) || !(vn < 0.0)) return -kErrModeMapping;
             } else if (vn > 0.0) {
@@ -807,7 +808,7 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             continue;
         }
         if (F.cond == 'R') return -PMC_ERR_UNSUPPORTED; // table-less rough facet
-        if (V.callaway) return -kErrModeMapping;    // reflecting facet without a table
+        if (V.conservative) return -kErrModeMapping; // reflecting facet without a table
         // Specular mirror (periodic facet without a partner).
         mw ^= (1 << ha) << kFlipShift;
         mode_changed = true;
@@ -1357,7 +1358,7 @@ __device__ __forceinline__ double cell_background(const View& V, int c) {
 // Temperature of cell c at energy density e: its material's lookup table, or
 // under linearized transport T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225).
 __device__ __forceinline__ double cell_temperature(const View& V, double e, int c) {
-    if (V.callaway) return V.t_bg + (e - V.e_bg) / V.cw_cbg;
+    if (V.linearized) return V.t_bg + (e - V.e_bg) / V.cw_cbg;
     if (V.layers > 1) {
         const int m = V.cell_mat[c];
         return lut_inverse(V, e, V.lut_begin[m], V.lut_begin[m + 1] - V.lut_begin[m]);
@@ -2875,6 +2876,78 @@ __global__ void __launch_bounds__(256) kb_callaway_wave(const View* __restrict__
     callaway_wave_cells<kClass>(views[i], G);
 }
 
+// ---------------------------------------------------------------- nonlinear Callaway
+// NonlinearCallawayCollision::apply (src/solver/NonlinearCallawayCollision.cpp:287-344):
+// one wavefront per cell walks the cell's sorted slots in every pass of
+// pmc_nl::cell (the N(dt/2) R(dt) N(dt/2) moment-matched BGK substeps), with
+// wave sums by cross-lane DPP exchanges and per-slot values in the scratch
+// arrays.  Rates at the post-transport cell temperature (lifetime_local) or at
+// the fixed lifetime temperature.  The cell's energy after the collision and its
+// lifetime residual go to its first chunk, as the other collision kernels write
+// them for the energy refresh.
+struct WaveMapReduce {
+    template <int K, class F>
+    __device__ __forceinline__ void map_reduce(int begin, int end, double (&acc)[K], F&& f) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = 0.0;
+        for (int i = begin + static_cast<int>(__lane_id()); i < end; i += 64) f(i, acc);
+        wave_sum<K>(acc);
+    }
+};
+
+__device__ __forceinline__ void nlcallaway_cell(const View& V, const int c) {
+    const int lane = __lane_id();
+    const int begin = V.cell_start[c], end = V.cell_start[c + 1];
+    const int k0 = V.chunk_start[c], k1 = V.chunk_start[c + 1];
+    for (int k = k0 + lane; k < k1; k += 64) {
+        V.chunk_out[2 * static_cast<size_t>(k)] = 0.0;
+        V.chunk_out[2 * static_cast<size_t>(k) + 1] = 0.0;
+    }
+    if (end <= begin) return;
+    const int64_t cap = V.capacity;
+    pmc_nl::Data D;
+    D.mode = V.mode;
+    D.mode_mask = kModeMask;
+    D.occ = V.occ;
+    D.omega = &V.mtab[0].x;
+    D.neq_bg = &V.mtab[0].y;
+    D.omega_stride = 2;
+    D.q = V.nl_q;
+    D.n_active = V.na;
+    D.n_temps = V.nl_nt;
+    D.temps = V.nl_temps;
+    D.gn = V.nl_gn;
+    D.gu = V.nl_gu;
+    D.ge = V.nl_ge;
+    D.a = V.nl_scratch;
+    D.metric = V.nl_scratch + cap;
+    D.n0 = V.nl_scratch + 2 * cap;
+    D.b = V.nl_scratch + 3 * cap;
+    D.bstride = cap;
+    D.energy_weight = V.w;
+    const double T = V.crta[c].t0;
+    const double rate_T = V.lifetime_local ? T : V.lifetime_T;
+    double out[2];
+    WaveMapReduce x;
+    const int rc = pmc_nl::cell(x, D, begin, end, T, rate_T, V.dt, out);
+    if (rc) {
+        // facet -20 - rc names the reason; the slot field carries the cell
+        if (lane == 0) raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -20 - rc);
+        return;
+    }
+    if (lane == 0) {
+        V.chunk_out[2 * static_cast<size_t>(k0)] = out[0];
+        V.chunk_out[2 * static_cast<size_t>(k0) + 1] = out[1];
+    }
+}
+
+// Four cells per 256-thread workgroup, one per wavefront.
+__global__ void __launch_bounds__(256) k_nlcallaway(View V) {
+    const int c = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    if (c >= V.ncells) return;
+    nlcallaway_cell(V, c);
+}
+
 // ---------------------------------------------------------------- heat source
 // Prescribed lattice heat source, thermal spectrum: the deposit of
 // PhononSourceOperator::apply (src/solver/PhononSourceOperator.cpp:120-341) as
@@ -2936,7 +3009,7 @@ __device__ __forceinline__ void source_sums_body(const View& V, const Grid G, in
     double acc[kSrcCols];
 #pragma unroll
     for (int k = 0; k < kSrcCols; ++k) acc[k] = 0.0;
-    if (pend > 0.0 && V.callaway) {
+    if (pend > 0.0 && V.linearized) {
         // Linearized transport: the cell's heat capacity at T_bg over its eligible
         // carriers (PhononSourceOperator::apply, src/solver/PhononSourceOperator.cpp:221-236).
         for (int i = begin + static_cast<int>(threadIdx.x); i < end; i += kChunkThreads) {
@@ -3047,7 +3120,7 @@ __device__ __forceinline__ void source_solve_body(const View& V, const Grid G, i
         raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -4);
         return;
     }
-    if (V.callaway) { // increments target * C_i / sum C / hw_i: one scale per cell
+    if (V.linearized) { // increments target * C_i / sum C / hw_i: one scale per cell
         if (!(acc[kSrcM1] > 0.0)) {
             raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -12);
             return;
@@ -3169,7 +3242,7 @@ __device__ __forceinline__ void source_increments_body(const View& V, const Grid
         double w, hw;
         double inc = 0.0;
         if (source_eligible(V, m, w, hw)) {
-            inc = V.callaway ? ts * V.cw_cap[m] / hw : bose(ts, w) - bose(tb, w);
+            inc = V.linearized ? ts * V.cw_cap[m] / hw : bose(ts, w) - bose(tb, w);
             const double e = hw * inc;
             applied += e;
             const unsigned id = V.uid[i];
@@ -3521,7 +3594,7 @@ __global__ void __launch_bounds__(256) k_init(View V, int count, int tmode, doub
     V.mode[i] = m;
     // Linearized transport: n_eq(T_bg) + C(T_bg) (T - T_bg) / hw
     // (PhononMaterial::linearized_bose_occupation, src/material/Thermodynamics.cpp:27-37).
-    if (V.callaway) {
+    if (V.linearized) {
         const double hw = kHbar * V.omega[m];
         V.occ[i] = hw > 0.0 ? V.neq[m] + V.cw_cap[m] * (T - V.t_bg) / hw : 0.0;
     } else {
@@ -3623,6 +3696,9 @@ void callaway(const View& v, hipStream_t s) {
         hipLaunchKernelGGL(k_callaway<kCwThreads>, big, dim3(kCwThreads), 0, s, v);
         break;
     }
+}
+void nlcallaway(const View& v, hipStream_t s) {
+    hipLaunchKernelGGL(k_nlcallaway, dim3((v.ncells + 3) / 4), dim3(256), 0, s, v);
 }
 void source(const View& v, hipStream_t s, int half) {
     hipLaunchKernelGGL(k_source_sums, dim3(v.max_chunks), dim3(kChunkThreads), 0, s, v, half);

@@ -47,6 +47,7 @@ void moments(const pmc_dev::View& v, hipStream_t s);
 void newton(const pmc_dev::View& v, hipStream_t s);
 void relax(const pmc_dev::View& v, hipStream_t s);
 void callaway(const pmc_dev::View& v, hipStream_t s);
+void nlcallaway(const pmc_dev::View& v, hipStream_t s);
 void source(const pmc_dev::View& v, hipStream_t s, int half);
 void clear_cells(const pmc_dev::View& v, hipStream_t s);
 void flux(const pmc_dev::View& v, hipStream_t s);

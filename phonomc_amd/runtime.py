@@ -23,7 +23,8 @@ class Setup(C.Structure):
     _fields_ = [("capacity", C.c_int64), ("time_step_ps", C.c_double), ("push_distance_a", C.c_double),
                 ("seed", C.c_uint64), ("instance", C.c_uint32), ("background_temperature", C.c_double),
                 ("lifetime_local", C.c_int32), ("lifetime_temperature", C.c_double), ("refill_policy", C.c_int32),
-                ("mode_sampling", C.c_int32), ("temperature_inversion", C.c_int32), ("reserved", C.c_int32 * 8)]
+                ("mode_sampling", C.c_int32), ("temperature_inversion", C.c_int32),
+                ("conservative_walls", C.c_int32), ("reserved", C.c_int32 * 7)]
 
 
 class ModeTable(C.Structure):
@@ -93,6 +94,12 @@ class CallawayTable(C.Structure):
     _fields_ = [("capacity", C.POINTER(C.c_double)), ("wavevector", C.POINTER(C.c_double)),
                 ("normal_rate", C.POINTER(C.c_double)), ("resistive_rate", C.POINTER(C.c_double)),
                 ("heat_capacity_density", C.c_double), ("background_energy_density", C.c_double)]
+
+
+class CallawayRates(C.Structure):
+    _fields_ = [("n_temps", C.c_int32), ("temps", C.POINTER(C.c_double)), ("gamma_normal", C.POINTER(C.c_double)),
+                ("gamma_umklapp", C.POINTER(C.c_double)), ("gamma_extra", C.POINTER(C.c_double)),
+                ("wavevector", C.POINTER(C.c_double))]
 
 
 class Layers(C.Structure):
@@ -221,6 +228,8 @@ def lib():
         "pmcs_write_finfet_stl": (C.c_int, [C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                             C.POINTER(C.c_double), C.c_char_p, C.c_int]),
         "pmc_upload_callaway": (C.c_int, [P, C.POINTER(CallawayTable)]),
+        "pmc_upload_callaway_nonlinear": (C.c_int, [P, C.POINTER(CallawayRates)]),
+        "pmcs_problem_callaway_nonlinear": (C.c_int, [P, C.POINTER(CallawayRates)]),
         "pmc_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pmc_destroy": (None, [P]),
         "pmc_status_string": (C.c_char_p, [C.c_int32]),
@@ -302,6 +311,11 @@ class Problem:
         """The linearized Callaway view, or None under RTA."""
         t = CallawayTable()
         return t if lib().pmcs_problem_callaway(self.h, C.byref(t)) == 1 else None
+
+    def callaway_nonlinear(self):
+        """The nonlinear Callaway rate tables, or None unless the input selects them."""
+        t = CallawayRates()
+        return t if lib().pmcs_problem_callaway_nonlinear(self.h, C.byref(t)) == 1 else None
 
     def material(self):
         mv = MaterialView()
@@ -385,6 +399,9 @@ class Context:
         cw = problem.callaway()
         if cw is not None:
             self._check(L.pmc_upload_callaway(h, C.byref(cw)))
+        nl = problem.callaway_nonlinear()
+        if nl is not None:
+            self._check(L.pmc_upload_callaway_nonlinear(h, C.byref(nl)))
         lv = problem.layers()
         if lv is not None:
             self._check(L.pmc_upload_layers(h, C.byref(lv)))

@@ -7,6 +7,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --callaway-source [--runs]     (Callaway + heat source only)
     python tests/golden/make_golden.py --layered [--runs]             (layered Si/Ge film only)
     python tests/golden/make_golden.py --strict [--runs]              (strict-elastic walls of Callaway inputs)
+    python tests/golden/make_golden.py --nonlinear [--runs]           (nonlinear Callaway, conservative RTA walls)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -32,7 +33,8 @@ OUT = os.path.dirname(os.path.abspath(__file__))
 WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "c1s_source.toml",
              "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml",
              "c1cs": "c1cs_callaway_source.toml", "c1l": "c1l_layered.toml",
-             "finfet_cw": "finfet_callaway.toml"}
+             "finfet_cw": "finfet_callaway.toml", "c1cn": "c1cn_nonlinear.toml", "c2cn": "c2cn_nonlinear.toml",
+             "c2r": "c2r_conservative.toml"}
 
 
 def write_input(text, folder, name):
@@ -254,6 +256,59 @@ def callaway_vectors(problem, rng):
     return out
 
 
+def nonlinear_vectors(problem, rng):
+    """Nonlinear Callaway of the reference built with stationary modes:
+    mode_callaway_rates at temperatures inside and at the ends of the table,
+    nonlinear_callaway_evolve on single cells of 1 to 2000 carriers at three
+    temperatures and time steps with occupations scattered about (and far from)
+    the Bose value, and NonlinearCallawayCollision::apply on a carrier set with
+    local and fixed rate temperatures."""
+    s, m, g, i, rough, res = problem.views()
+    rm = R.RefMaterial(problem.material(), include_stationary=True)
+    nl = problem.callaway_nonlinear()
+    na, nc = m.n_active, g.n_cells
+    omega = rt.arr(m.omega, na)
+    q = rt.arr(nl.wavevector, 3 * na).reshape(na, 3)
+    temps = rt.arr(nl.temps, nl.n_temps)
+    out = {"active_count": na}
+    # rates
+    rt_t = np.array([temps[1] * 0.5, temps[1], 150.0, 299.99, 300.0, 456.789, temps[-1]])
+    act = rng.integers(0, na, 200).astype(np.int32)
+    out["rates_active"] = act
+    out["rates_t"] = rt_t
+    out["rates_normal"] = np.array([R.nl_callaway_rates(rm, t, act)[0] for t in rt_t])
+    out["rates_resistive"] = np.array([R.nl_callaway_rates(rm, t, act)[1] for t in rt_t])
+    out["rates_outside"] = np.array([temps[-1] + 0.5, -1.0])
+    # single cells
+    k = 0
+    for T in (250.0, 300.0, 400.0):
+        for size in (1, 2, 5, 40, 300, 2000):
+            for dt, spread in ((1.0, 0.05), (4.0, 0.4)):
+                a = rng.integers(0, na, size).astype(np.int32)
+                rn, rr = R.nl_callaway_rates(rm, T, a)
+                neq = 1.0 / np.expm1(6.582119569e-4 * omega[a] / (8.617333262145e-5 * T))
+                occ = np.maximum(neq * (1.0 + spread * rng.standard_normal(size)), 0.0)
+                ref = R.nl_callaway_evolve(omega[a], q[a], rn, rr, occ, T, dt)
+                out.update({f"cell{k}_t": T, f"cell{k}_dt": dt, f"cell{k}_active": a, f"cell{k}_normal": rn,
+                            f"cell{k}_resistive": rr, f"cell{k}_occ": occ, f"cell{k}_after": ref})
+                k += 1
+    out["cells"] = k
+    # apply on a population
+    vp = problem.scalars()["particle_volume"]
+    n = 6000
+    for tag, local, lt in (("a", 1, 300.0), ("b", 0, 310.0)):
+        act = rng.integers(0, na, n).astype(np.int32)
+        cell = rng.integers(0, nc, n).astype(np.int32)
+        ct = 295.0 + 10.0 * rng.random(nc)
+        neq = 1.0 / np.expm1(6.582119569e-4 * omega[act] / (8.617333262145e-5 * ct[cell]))
+        occ = np.maximum(neq * (1.0 + 0.1 * rng.standard_normal(n)), 0.0)
+        new, residual = R.nl_callaway_apply(rm, s.background_temperature, ct, local, lt, act, cell, occ, 1.0, vp)
+        out.update({f"{tag}_local": local, f"{tag}_lifetime_t": lt, f"{tag}_cell_t": ct, f"{tag}_active": act,
+                    f"{tag}_cell": cell, f"{tag}_occ": occ, f"{tag}_after": new, f"{tag}_residual": residual})
+    rm.close()
+    return out
+
+
 def mesh_vectors(problem, rd, rng):
     """Mesh model: facets and grid, and the reference's trace, containment and
     nearest facet on interior rays, rays from surface points and points on,
@@ -445,6 +500,24 @@ def main():
             np.savez_compressed(os.path.join(OUT, "run_finfet.npz"),
                                 **whole_run("finfet", 100000, 400, list(range(1000, 1016)), 10))
         print("mesh fixtures written to", OUT)
+        return
+    if "--nonlinear" in sys.argv:
+        rng = np.random.default_rng(20261210)
+        text = rt.config_text(WORKLOADS["c1cn"])
+        np.savez_compressed(os.path.join(OUT, "callaway_nl_c1cn.npz"),
+                            **nonlinear_vectors(rt.Problem(text, input_directory=rt.CONFIGS, mode_sampling=1), rng))
+        # strict-elastic tables of conservative RTA walls (the c2 film's rough walls)
+        p, rm, rd, _ = problem_and_reference("c2r", include_stationary=True)
+        np.savez_compressed(os.path.join(OUT, "rough_c2r.npz"), **strict_vectors(p, rm, rd, rng, 3000))
+        rm.close()
+        if "--runs" in sys.argv:
+            np.savez_compressed(os.path.join(OUT, "run_c1cn.npz"),
+                                **whole_run("c1cn", 20000, 400, list(range(1000, 1016)), 10))
+            np.savez_compressed(os.path.join(OUT, "run_c2cn.npz"),
+                                **whole_run("c2cn", 100000, 300, list(range(1000, 1016)), 10))
+            np.savez_compressed(os.path.join(OUT, "run_c2r.npz"),
+                                **whole_run("c2r", 100000, 300, list(range(1000, 1016)), 10))
+        print("nonlinear Callaway fixtures written to", OUT)
         return
     if "--strict" in sys.argv:
         rng = np.random.default_rng(20261208)

@@ -192,7 +192,9 @@ def test_callaway_conservation_laws(c1c):
 
 def test_callaway_inputs_rejected_with_messages():
     base = rt.config_text("c1c_callaway.toml")
-    with pytest.raises(RuntimeError, match="nonlinear"):
-        rt.Problem(base.replace('model = "callaway"', 'model = "callaway"\ncallaway_formulation = "nonlinear"'))
+    nonlinear = base.replace('model = "callaway"', 'model = "callaway"\ncallaway_formulation = "nonlinear"')
+    assert rt.Problem(nonlinear).callaway_nonlinear() is not None
+    with pytest.raises(RuntimeError, match="gamma_N and gamma_U"):
+        rt.Problem(nonlinear.replace("synthetic:si15:callaway", "synthetic:si15"))
     with pytest.raises(RuntimeError, match="gamma_N and gamma_U"):
         rt.Problem(base.replace("synthetic:si15:callaway", "synthetic:si15"))

@@ -1,0 +1,55 @@
+"""Whole runs of the oracle against whole runs of the reference, on the CPU.
+
+The nonlinear Callaway film (c1cn: displaced-Bose N/R substeps, strict-elastic
+conservative transport with fixed-flux reservoirs and Bose occupations) is run
+by the oracle for 8 seeds of its own Philox streams and compared with the 16
+reference runs of tests/golden/run_c1cn.npz (2e4 carriers, 400 steps): the
+second-half mean of every cell temperature, sampled every 10 steps as the
+reference writes convergence.txt, by the z-score of test_gpu_statistics
+(max |z| <= 4.5 over the 20 cells, mean z^2 <= 2.5).  This pins the oracle's
+whole step for this transport policy, which the device is then compared with
+carrier by carrier.
+"""
+import concurrent.futures
+import os
+
+import numpy as np
+
+from oracle import oracle_ctypes as O
+from phonomc_amd import runtime as rt
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def oracle_run(cfg, n, steps, interval, seed):
+    p = rt.Problem(rt.config_text(cfg, **{"simulation.particle_count": n, "simulation.random_seed": seed}),
+                   input_directory=rt.CONFIGS)
+    sim = O.Sim(p, threads=1)
+    sim.initialize()
+    acc = []
+    for s in range(1, steps + 1):
+        sim.step(1)
+        if s % interval == 0 and s > steps // 2:
+            acc.append(sim.cells()[0])
+    sim.close()
+    return np.mean(acc, axis=0)
+
+
+def z_score(a, b):
+    se = np.sqrt(a.var(axis=0, ddof=1) / len(a) + b.var(axis=0, ddof=1) / len(b))
+    return (a.mean(axis=0) - b.mean(axis=0)) / se
+
+
+def test_nonlinear_callaway_oracle_runs_match_reference():
+    golden = np.load(os.path.join(HERE, "golden", "run_c1cn.npz"))
+    n, steps, interval = int(golden["n"]), int(golden["steps"]), int(golden["interval"])
+    seeds = [5000 + k for k in range(8)]
+    with concurrent.futures.ThreadPoolExecutor(8) as pool:
+        temps = np.array(list(pool.map(lambda s: oracle_run("c1cn_nonlinear.toml", n, steps, interval, s), seeds)))
+    z = z_score(temps, golden["temperature"])
+    print(f"c1cn oracle: cell T max|z| {np.abs(z).max():.2f}, mean z^2 {np.mean(z ** 2):.2f}")
+    assert np.abs(z).max() <= 4.5
+    assert np.mean(z ** 2) <= 2.5
+    # the profile carries the imposed gradient (301 K -> 299 K)
+    t = temps.mean(axis=0)
+    assert t[0] > 300.3 and t[-1] < 299.7

@@ -9,7 +9,8 @@ so within a window they cancel.)
 The fixtures (tests/golden/rough_c2c.npz with the 1 nm walls of the film,
 rough_c2c_smooth.npz with 0.05 nm walls so that most reflections are specular,
 rough_finfet_cw.npz on all eleven FinFET facets that reflect; written by
-make_golden.py --strict) hold the reference's outputs on the uniforms it drew.
+make_golden.py --strict; rough_c2r.npz, the RTA film with
+simulation.conservative_boundaries, written by make_golden.py --nonlinear) hold the reference's outputs on the uniforms it drew.
 The product's host tables (csrc/host/boundary_tables.cpp) fed to the oracle's
 sampler with the same uniforms must reproduce them bit for bit: outgoing mode,
 occupation and its sign.  Planted mistakes in the tables or in the sampler's
@@ -24,7 +25,9 @@ from phonomc_amd import runtime as rt
 
 SMOOTH = {"boundary.boundary_values": "[302, 298, 0.0, 0.0, 0.05, 0.05]"}
 CASES = [("c2c_callaway.toml", "rough_c2c.npz", {}), ("c2c_callaway.toml", "rough_c2c_smooth.npz", SMOOTH),
-         ("finfet_callaway.toml", "rough_finfet_cw.npz", {})]
+         ("finfet_callaway.toml", "rough_finfet_cw.npz", {}),
+         # conservative RTA walls (simulation.conservative_boundaries, no N/U split)
+         ("c2r_conservative.toml", "rough_c2r.npz", {})]
 _problems = {}
 
 
