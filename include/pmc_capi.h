@@ -213,6 +213,20 @@ typedef struct pmc_layers {
     const int32_t* bank_mode;      /* global active modes, each bank sorted by frequency */
     const double* bank_omega;      /* [bank_begin[2*count]] */
     const double* bank_prefix;     /* bank b: n_b+1 cumulative |v_axis| / normalization from bank_begin[b] + b */
+    /* Acoustic (AMM) and mixed (MMM) mismatch (InterfaceInteraction::interact,
+     * src/solver/InterfaceInteraction.cpp:46-153; AcousticInterfaceSampler,
+     * src/solver/AcousticInterfaceSampler.cpp:27-472).  amm_fraction 0: DMM only
+     * (the AMM arrays may be NULL); 1: AMM only; in between: an event takes AMM
+     * with this probability.  AMM rows by interface f and side s (0: material f
+     * toward +axis, 1: material f+1 toward -axis): block 2f+s starts at
+     * amm_block[2f+s] and holds one entry per active mode of the source material
+     * (local index) in amm_reflected and one more in amm_edge_begin. */
+    double amm_fraction;
+    const int32_t* amm_block;      /* [2*(count-1)+1] */
+    const int32_t* amm_reflected;  /* global active mode of the mirror partner; -1: no incoming row */
+    const int32_t* amm_edge_begin; /* row r's transmitted edges are [amm_edge_begin[r], amm_edge_begin[r+1]) */
+    const int32_t* amm_edge_mode;  /* global active mode transmitted into */
+    const double* amm_edge_prob;   /* its probability */
 } pmc_layers;
 
 /* Linearized Callaway scattering about the fixed reference T_bg; replaces

@@ -62,6 +62,11 @@ int pmcs_problem_callaway(const pmcs_problem* problem, pmc_callaway_table* out);
 int pmcs_problem_callaway_nonlinear(const pmcs_problem* problem, pmc_callaway_rates* out);
 /* Layered materials: fills *out and returns 1 (0 for one material). */
 int pmcs_problem_layers(const pmcs_problem* problem, pmc_layers* out);
+/* Audit of the AMM tables of interface `face` (AcousticInterfaceSampler::Audit,
+ * include/solver/AcousticInterfaceSampler.h:12-17): incident acoustic flux,
+ * transmitted flux and no-overlap flux of sides 0 and 1, geometric overlap flux,
+ * capacity-removed flux, largest capacity ratio.  1 when filled, 0 without AMM. */
+int pmcs_problem_amm_audit(const pmcs_problem* problem, int32_t face, double* out);
 /* Raw table of layer `layer` (materials.folders order). */
 int pmcs_problem_layer_material(const pmcs_problem* problem, int32_t layer, pmcs_material_view* out);
 /* Writes the procedural FinFET surface (a substrate box with one fin centred on

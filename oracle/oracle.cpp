@@ -1126,6 +1126,10 @@ struct orc_sim {
     double band = 0.0;
     std::vector<double> planes, e_bg_mat, accept, bank_omega, bank_prefix;
     std::vector<int> mode_begin, cell_mat, lut_begin, bank_begin, bank_mode;
+    // acoustic / mixed mismatch (AMM rows in global active indices)
+    double amm_fraction = 0.0;
+    std::vector<int> amm_block, amm_refl, amm_ebeg, amm_emode;
+    std::vector<double> amm_eprob;
     // Transport policy (include/SimulationConfig.h:148-156): conservative walls
     // (strict-elastic tables on every reflecting facet, signed occupations) and
     // linearized occupations; the collision operator is RTA, linearized
@@ -1258,18 +1262,13 @@ int sample_band(const orc_sim& s, int material, int direction, double w, double 
 // on three uniforms: reflection band draw, transmission band draw, transmission
 // test.  Returns 1 transmitted, 0 reflected into a new mode, -1 mirrored (same
 // mode, velocity reversed along the axis).
-int dmm_core(const orc_sim& s, int a, const V3& vel, double occ, int dest, const double u[3], int& out, double& nocc,
-             V3& nvel) {
+// The energy-preserving remap into the selected mode, or a mirror along the
+// axis when there is none or its occupation is invalid
+// (src/solver/InterfaceInteraction.cpp:103-152, nonlinear occupations).
+int interface_outcome(const orc_sim& s, int a, const V3& vel, double occ, int sel, bool transmit, int& out,
+                      double& nocc, V3& nvel) {
     const int ax = s.layer_axis;
-    const int mat = layer_of_mode(s, a);
     const double w = s.m.omega[a];
-    const int inc = vel[ax] > 0.0 ? 1 : 0;
-    double fr = 0.0, ft = 0.0;
-    const int mr = sample_band(s, mat, 1 - inc, w, u[0], fr);
-    const int mt = sample_band(s, dest, inc, w, u[1], ft);
-    const double tr = ft > 0 ? ft / std::max(1e-300, fr + ft) : 0.0;
-    const bool transmit = mt >= 0 && u[2] < tr;
-    const int sel = transmit ? mt : mr;
     out = a;
     nocc = occ;
     nvel = vel;
@@ -1288,17 +1287,74 @@ int dmm_core(const orc_sim& s, int a, const V3& vel, double occ, int dest, const
     return -1;
 }
 
-void interface_dmm(orc_sim& s, int64_t i, int dest, Uniforms& U, StepTally& tally) {
-    const double u[3] = {U.next(), U.next(), U.next()};
+// Diffuse mismatch (:86-101): a band draw in the reflection bank of this
+// material (u[0]), one in the transmission bank of the next (u[1]), the choice
+// by the flux ratio (u[2]).
+int dmm_core(const orc_sim& s, int a, const V3& vel, double occ, int dest, const double u[3], int& out, double& nocc,
+             V3& nvel) {
+    const int ax = s.layer_axis;
+    const int mat = layer_of_mode(s, a);
+    const double w = s.m.omega[a];
+    const int inc = vel[ax] > 0.0 ? 1 : 0;
+    double fr = 0.0, ft = 0.0;
+    const int mr = sample_band(s, mat, 1 - inc, w, u[0], fr);
+    const int mt = sample_band(s, dest, inc, w, u[1], ft);
+    const double tr = ft > 0 ? ft / std::max(1e-300, fr + ft) : 0.0;
+    const bool transmit = mt >= 0 && u[2] < tr;
+    return interface_outcome(s, a, vel, occ, transmit ? mt : mr, transmit, out, nocc, nvel);
+}
+
+// InterfaceInteraction::interact (src/solver/InterfaceInteraction.cpp:46-153) on
+// the draws u in the order they are made: the channel (mixed mismatch only),
+// then the AMM row draw (AcousticInterfaceSampler::sample,
+// src/solver/AcousticInterfaceSampler.cpp:457-471) or the three DMM draws.  A
+// tangent carrier mirrors before any AMM draw.  Returns 1 transmitted, 0
+// reflected into a new mode, -1 mirrored, -2 no AMM row for the mode in this
+// direction ("AMM incoming state is tangent or unsupported.").
+int interface_core(const orc_sim& s, int a, const V3& vel, double occ, int dest, const double u[4], int& out,
+                   double& nocc, V3& nvel) {
+    const bool mixed = s.amm_fraction > 0.0 && s.amm_fraction < 1.0;
+    const bool amm = s.amm_fraction >= 1.0 || (mixed && u[0] < s.amm_fraction);
+    const double* r = u + (mixed ? 1 : 0);
+    if (!amm) return dmm_core(s, a, vel, occ, dest, r, out, nocc, nvel);
+    const int ax = s.layer_axis, mat = layer_of_mode(s, a);
+    out = a;
+    nocc = occ;
+    nvel = vel;
+    if (std::abs(vel[ax]) < 1e-7) {
+        nvel[ax] = -nvel[ax];
+        return -1;
+    }
+    const int face = std::min(mat, dest), side = dest > mat ? 0 : 1;
+    const int row = s.amm_block[2 * face + side] + (a - s.mode_begin[mat]);
+    int sel = s.amm_refl[row];
+    if (sel < 0) return -2;
+    bool transmit = false;
+    double x = r[0];
+    for (int e = s.amm_ebeg[row]; e < s.amm_ebeg[row + 1]; ++e) {
+        x -= s.amm_eprob[e];
+        if (x < 0) {
+            sel = s.amm_emode[e];
+            transmit = true;
+            break;
+        }
+    }
+    return interface_outcome(s, a, vel, occ, sel, transmit, out, nocc, nvel);
+}
+
+bool interface_event(orc_sim& s, int64_t i, int dest, Uniforms& U, StepTally& tally) {
+    const double u[4] = {U.next(), U.next(), U.next(), U.next()};
     int out = 0;
     double nocc = 0.0;
     V3 nvel{};
-    const int action = dmm_core(s, s.mode[i], s.vel[i], s.occ[i], dest, u, out, nocc, nvel);
+    const int action = interface_core(s, s.mode[i], s.vel[i], s.occ[i], dest, u, out, nocc, nvel);
+    if (action == -2) return false;
     tally.iface_events++;
     if (action == 1) tally.iface_transmitted++;
     s.mode[i] = out;
     s.occ[i] = nocc;
     s.vel[i] = nvel;
+    return true;
 }
 
 int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
@@ -1345,7 +1401,7 @@ int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
             s.pos[i] = add(s.pos[i], mul(s.vel[i], t_cross));
             U.ev = static_cast<uint32_t>(ev);
             U.used = 0;
-            interface_dmm(s, i, dest, U, tally);
+            if (!interface_event(s, i, dest, U, tally)) return -PMC_ERR_BOUNDARY;
             s.pos[i] = add(s.pos[i], mul(s.vel[i], push / std::max(norm(s.vel[i]), 1e-12)));
             if (!update_cache(s, i)) return -PMC_ERR_BOUNDARY;
             continue;
@@ -1709,7 +1765,25 @@ int orc_set_layers(orc_sim* s, const pmc_layers* L) {
     for (int k = 0; k < n; ++k) vmin = std::min(vmin, L->spatial_volume[k]);
     s->accept.resize(n);
     for (int k = 0; k < n; ++k) s->accept[k] = vmin / L->spatial_volume[k];
+    s->amm_fraction = L->amm_fraction;
+    if (L->amm_fraction > 0.0) {
+        s->amm_block.assign(L->amm_block, L->amm_block + 2 * (n - 1) + 1);
+        const int rows = s->amm_block.back();
+        s->amm_refl.assign(L->amm_reflected, L->amm_reflected + rows);
+        s->amm_ebeg.assign(L->amm_edge_begin, L->amm_edge_begin + rows);
+        const int edges = s->amm_ebeg.back();
+        s->amm_emode.assign(L->amm_edge_mode, L->amm_edge_mode + edges);
+        s->amm_eprob.assign(L->amm_edge_prob, L->amm_edge_prob + edges);
+    }
     return PMC_OK;
+}
+
+int orc_interface(orc_sim* s, int active, const double vel[3], double occ, int dest, const double u[4],
+                  int* out_active, double* out_occ, double out_vel[3]) {
+    V3 nvel{};
+    const int action = interface_core(*s, active, {vel[0], vel[1], vel[2]}, occ, dest, u, *out_active, *out_occ, nvel);
+    for (int k = 0; k < 3; ++k) out_vel[k] = nvel[k];
+    return action;
 }
 
 int orc_interface_dmm(orc_sim* s, int active, const double vel[3], double occ, int dest, const double u[3],

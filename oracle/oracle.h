@@ -95,6 +95,12 @@ int orc_set_layers(orc_sim* sim, const pmc_layers* layers);
  * with velocity vel and occupation occ toward material dest, on the uniforms u[3].
  * Returns the action (1 transmit, 0 reflect into a new mode, -1 mirror) and the
  * outgoing mode, occupation and velocity. */
+/* InterfaceInteraction::interact for any interface model (DMM, AMM, MMM) on the
+ * draws u in order (the channel draw first under MMM, then the AMM row draw or
+ * the three DMM draws); returns 1 transmitted, 0 reflected into a new mode,
+ * -1 mirrored, -2 no AMM row for the mode in this direction. */
+int orc_interface(orc_sim* sim, int active, const double vel[3], double occ, int dest, const double u[4],
+                  int* out_active, double* out_occ, double out_vel[3]);
 int orc_interface_dmm(orc_sim* sim, int active, const double vel[3], double occ, int dest, const double u[3],
                       int* out_active, double* out_occ, double out_vel[3]);
 /* Linearized Callaway transport (before initialization): linearized initial

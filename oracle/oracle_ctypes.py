@@ -73,6 +73,8 @@ def lib():
                                             C.POINTER(d), C.c_int, d, C.c_int64, C.POINTER(C.c_int32),
                                             C.POINTER(C.c_int32), C.POINTER(d), d, d, C.POINTER(C.c_int)]
         L.orc_set_layers.argtypes = [P, C.POINTER(rt.Layers)]
+        L.orc_interface.argtypes = [P, C.c_int, C.POINTER(d), d, C.c_int, C.POINTER(d), C.POINTER(C.c_int),
+                                    C.POINTER(d), C.POINTER(d)]
         L.orc_interface_dmm.argtypes = [P, C.c_int, C.POINTER(d), d, C.c_int, C.POINTER(d), C.POINTER(C.c_int),
                                         C.POINTER(d), C.POINTER(d)]
         L.orc_callaway_apply.restype = d
@@ -148,6 +150,16 @@ class Sim:
         out_a, out_o, out_v = C.c_int(), C.c_double(), np.zeros(3)
         act = lib().orc_interface_dmm(self.h, int(active), _dp(v), float(occ), int(dest), _dp(uu), C.byref(out_a),
                                       C.byref(out_o), _dp(out_v))
+        return act, out_a.value, out_o.value, out_v
+
+    def interface(self, active, vel, occ, dest, u):
+        """orc_interface (any interface model, draws u[0..3] in order): (action, out mode,
+        out occupation, out velocity)."""
+        v = np.ascontiguousarray(vel, dtype=np.float64)
+        uu = np.ascontiguousarray(u, dtype=np.float64)
+        out_a, out_o, out_v = C.c_int(), C.c_double(), np.zeros(3)
+        act = lib().orc_interface(self.h, int(active), _dp(v), float(occ), int(dest), _dp(uu), C.byref(out_a),
+                                  C.byref(out_o), _dp(out_v))
         return act, out_a.value, out_o.value, out_v
 
     def initialize(self, init=None):

@@ -461,3 +461,50 @@ def nl_callaway_apply(material, t_bg, cell_temperature, local, lifetime_temperat
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return occ, res.value
+
+
+def amm_rows(material_views, axis, densities, bin_thz, parallel_bin, face, side, n_source, edge_capacity=2000000):
+    """AcousticInterfaceSampler rows of one interface side: (reflected [n], edge_begin [n+1],
+    edge_active, edge_probability, audit[9])."""
+    arrays = (MaterialArrays * len(material_views))(*[material_arrays(v) for v in material_views])
+    dens = np.ascontiguousarray(densities, dtype=np.float64)
+    refl = np.zeros(n_source, dtype=np.int32)
+    eb = np.zeros(n_source + 1, dtype=np.int32)
+    ea = np.zeros(edge_capacity, dtype=np.int32)
+    ep = np.zeros(edge_capacity)
+    audit = np.zeros(9)
+    err = C.create_string_buffer(512)
+    rc = lib().ref_amm_rows(arrays, len(material_views), int(axis), _dp(dens), C.c_double(bin_thz),
+                            C.c_double(parallel_bin), int(face), int(side), _ip(refl), _ip(eb), _ip(ea), _dp(ep),
+                            int(edge_capacity), _dp(audit), err, 512)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    return refl, eb, ea[:eb[-1]].copy(), ep[:eb[-1]].copy(), audit
+
+
+def interface_model(material_views, axis, model, amm_fraction, densities, parallel_bin, bin_thz, t_ref, weights,
+                    material, active, velocity, occupation, next_material, seeds, lookup_dt=0.1):
+    """ref_interface_model (model 0 DMM, 1 AMM, 2 MMM) on n carriers: dict of action,
+    material, active, occupation, velocity (n x 3) and the draws made (n x 4)."""
+    arr = (MaterialArrays * len(material_views))(*[material_arrays(v, lookup_dt) for v in material_views])
+    n = len(material)
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    mat, act, vel, occ, nxt = i32(material), i32(active), f64(velocity).reshape(-1), f64(occupation), i32(next_material)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+    w, dens = f64(weights), f64(densities)
+    out = dict(action=np.zeros(n, np.int32), material=np.zeros(n, np.int32), active=np.zeros(n, np.int32),
+               occupation=np.zeros(n), velocity=np.zeros(3 * n), uniforms=np.zeros(4 * n))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    err = C.create_string_buffer(1024)
+    rc = lib().ref_interface_model(arr, len(material_views), int(axis), int(model), C.c_double(amm_fraction),
+                                   _dp(dens), C.c_double(parallel_bin), C.c_double(bin_thz), C.c_double(t_ref), _dp(w),
+                                   n, ip(mat), ip(act), _dp(vel), _dp(occ), ip(nxt),
+                                   sd.ctypes.data_as(C.POINTER(C.c_uint64)), ip(out["action"]), ip(out["material"]),
+                                   ip(out["active"]), _dp(out["occupation"]), _dp(out["velocity"]),
+                                   _dp(out["uniforms"]), err, 1024)
+    if rc != 0:
+        raise RuntimeError(err.value.decode())
+    out["velocity"] = out["velocity"].reshape(n, 3)
+    out["uniforms"] = out["uniforms"].reshape(n, 4)
+    return out

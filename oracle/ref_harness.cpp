@@ -22,6 +22,7 @@
 #include "solver/NonlinearCallawayCollision.h"
 #include "solver/InterfaceInteraction.h"
 #include "solver/InterfaceModeSampler.h"
+#include "solver/AcousticInterfaceSampler.h"
 
 #include <chrono>
 #include <cstring>
@@ -628,6 +629,132 @@ int ref_run_materials(const char* toml_path, const ref_material_arrays* arrays, 
             const size_t k = std::min(t.size(), static_cast<size_t>(summary_len - 1));
             std::memcpy(summary, t.data(), k);
             summary[k] = '\0';
+        }
+    });
+}
+
+int ref_amm_rows(const ref_material_arrays* arrays, int n_materials, int axis, const double* densities,
+                 double frequency_bin_thz, double parallel_bin_inv_a, int face, int side, int* reflected,
+                 int* edge_begin, int* edge_active, double* edge_probability, int edge_capacity, double* audit,
+                 char* err, int errlen) {
+    return guarded(err, errlen, [&] {
+        std::vector<std::unique_ptr<PhononMaterial>> storage;
+        std::vector<const PhononMaterial*> mats;
+        for (int k = 0; k < n_materials; ++k) {
+            storage.push_back(std::make_unique<PhononMaterial>(to_material_data(arrays[k]),
+                                                               arrays[k].temperature_lookup_dt));
+            mats.push_back(storage.back().get());
+        }
+        phonomc::AcousticInterfaceSampler amm;
+        amm.configure(mats, axis, std::vector<double>(densities, densities + n_materials), frequency_bin_thz,
+                      parallel_bin_inv_a);
+        const int source = face + side, destination = face + 1 - side;
+        const int n = mats[static_cast<size_t>(source)]->active_mode_count();
+        int e = 0;
+        edge_begin[0] = 0;
+        for (int i = 0; i < n; ++i) {
+            const auto& r = amm.row(source, destination, i);
+            reflected[i] = r.reflected;
+            for (const auto& edge : r.transmitted) {
+                if (e >= edge_capacity) throw std::runtime_error("edge capacity exceeded");
+                edge_active[e] = edge.active;
+                edge_probability[e] = edge.probability;
+                ++e;
+            }
+            edge_begin[i + 1] = e;
+        }
+        const auto& a = amm.audit(face);
+        const double values[9] = {a.incident_acoustic_flux[0], a.incident_acoustic_flux[1], a.transmitted_flux[0],
+                                  a.transmitted_flux[1], a.no_overlap_flux[0], a.no_overlap_flux[1],
+                                  a.geometric_overlap_flux, a.capacity_removed_flux, a.maximum_capacity_ratio};
+        std::copy(values, values + 9, audit);
+    });
+}
+
+int ref_interface_model(const ref_material_arrays* arrays, int n_materials, int axis, int model,
+                        double amm_fraction, const double* densities, double parallel_bin_inv_a,
+                        double frequency_bin_thz, double reference_temperature, const double* energy_weights, int n,
+                        const int* material, const int* active, const double* velocity, const double* occupation,
+                        const int* next_material, const uint64_t* seeds, int* action, int* out_material,
+                        int* out_active, double* out_occupation, double* out_velocity, double* uniforms, char* err,
+                        int errlen) {
+    return guarded(err, errlen, [&] {
+        std::vector<std::unique_ptr<PhononMaterial>> storage;
+        std::vector<const PhononMaterial*> mats;
+        for (int k = 0; k < n_materials; ++k) {
+            storage.push_back(std::make_unique<PhononMaterial>(to_material_data(arrays[k]),
+                                                               arrays[k].temperature_lookup_dt));
+            mats.push_back(storage.back().get());
+        }
+        phonomc::InterfaceInteractionConfig cfg;
+        cfg.axis = axis;
+        cfg.enabled = true;
+        cfg.linearized = false;
+        cfg.model = model == 1 ? InterfaceModel::AcousticMismatch
+                    : model == 2 ? InterfaceModel::MixedMismatch
+                                 : InterfaceModel::DiffuseMismatch;
+        cfg.frequency_bin_thz = frequency_bin_thz;
+        cfg.amm_fraction = amm_fraction;
+        cfg.parallel_bin_inv_a = parallel_bin_inv_a;
+        cfg.mass_densities_kg_m3.assign(densities, densities + n_materials);
+        cfg.reference_temperature = reference_temperature;
+        phonomc::InterfaceInteraction interaction;
+        interaction.configure(cfg, mats, std::vector<double>(energy_weights, energy_weights + n_materials));
+        phonomc::InterfaceModeSampler bands;
+        bands.configure(mats, axis);
+        const double band_width = 2.0 * std::acos(-1.0) * frequency_bin_thz;
+        const double p_amm = model == 1 ? 1.0 : model == 2 ? amm_fraction : 0.0;
+        for (int i = 0; i < n; ++i) {
+            const PhononMaterial& src = *mats.at(static_cast<size_t>(material[i]));
+            phonomc::BoundaryState in;
+            in.material = material[i];
+            in.mode = src.active_mode_at(active[i]);
+            for (int a = 0; a < 3; ++a) in.velocity[static_cast<size_t>(a)] = velocity[3 * i + a];
+            in.omega = src.mode_angular_frequency(in.mode);
+            in.occupation = occupation[i];
+            // The draws the reference makes, in order, from a copy of its engine:
+            // the channel draw under mixed mismatch, then the AMM row draw or the
+            // DMM draws (a band draw only when the band carries flux, the choice
+            // only when the transmission band does); 0.5 marks a draw not made.
+            std::mt19937_64 rng(seeds[i]);
+            std::mt19937_64 probe = rng;
+            double* u = uniforms + 4 * i;
+            for (int k = 0; k < 4; ++k) u[k] = 0.5;
+            int base = 0;
+            bool amm = p_amm >= 1;
+            if (p_amm > 0 && p_amm < 1) {
+                u[0] = canonical(probe);
+                amm = u[0] < p_amm;
+                base = 1;
+            }
+            if (amm) {
+                if (!(std::abs(in.velocity[static_cast<size_t>(axis)]) < 1e-7)) u[base] = canonical(probe);
+            } else {
+                const int inc = in.velocity[static_cast<size_t>(axis)] > 0.0 ? 1 : 0;
+                double fr = 0.0, ft = 0.0;
+                std::mt19937_64 scratch(0);
+                bands.sample_band(material[i], 1 - inc, in.omega, band_width, scratch, fr);
+                bands.sample_band(next_material[i], inc, in.omega, band_width, scratch, ft);
+                u[base] = fr > 0.0 ? canonical(probe) : 0.5;
+                u[base + 1] = ft > 0.0 ? canonical(probe) : 0.5;
+                u[base + 2] = ft > 0.0 ? canonical(probe) : 0.5;
+            }
+            try {
+                const auto r = interaction.interact(in, next_material[i], rng);
+                if (r.action == phonomc::BoundaryAction::Transmit) action[i] = 1;
+                else if (r.state.material == in.material && r.state.mode == in.mode) action[i] = -1;
+                else action[i] = 0;
+                out_material[i] = r.state.material;
+                out_active[i] = mats.at(static_cast<size_t>(r.state.material))->active_index_for_mode(r.state.mode);
+                out_occupation[i] = r.state.occupation;
+                for (int a = 0; a < 3; ++a) out_velocity[3 * i + a] = r.state.velocity[static_cast<size_t>(a)];
+            } catch (const std::exception&) {
+                action[i] = -2;
+                out_material[i] = material[i];
+                out_active[i] = active[i];
+                out_occupation[i] = occupation[i];
+                for (int a = 0; a < 3; ++a) out_velocity[3 * i + a] = velocity[3 * i + a];
+            }
         }
     });
 }

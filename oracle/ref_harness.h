@@ -171,6 +171,24 @@ int ref_run_materials(const char* toml_path, const ref_material_arrays* arrays, 
  * material, active index, occupation and velocity after the event, and the
  * uniforms on [0, 1) the event consumed in the order (reflection band,
  * transmission band, transmit choice); 0.5 where the event drew none. */
+/* AcousticInterfaceSampler::configure + row (src/solver/AcousticInterfaceSampler.cpp:27-455):
+ * the rows of one interface side by the source material's active index (reflected
+ * partner, transmitted edges as CSR) and the interface's audit (incident, transmitted,
+ * no-overlap flux per side, overlap, capacity removed, largest capacity ratio). */
+int ref_amm_rows(const ref_material_arrays* arrays, int n_materials, int axis, const double* densities,
+                 double frequency_bin_thz, double parallel_bin_inv_a, int face, int side, int* reflected,
+                 int* edge_begin, int* edge_active, double* edge_probability, int edge_capacity, double* audit,
+                 char* err, int errlen);
+/* InterfaceInteraction::interact for model 0 DMM, 1 AMM, 2 MMM (fraction amm_fraction)
+ * on n carriers: outcomes as ref_interface_dmm (action -2: the reference threw) and
+ * the draws it made, four per carrier in order (0.5 where none was made). */
+int ref_interface_model(const ref_material_arrays* arrays, int n_materials, int axis, int model,
+                        double amm_fraction, const double* densities, double parallel_bin_inv_a,
+                        double frequency_bin_thz, double reference_temperature, const double* energy_weights, int n,
+                        const int* material, const int* active, const double* velocity, const double* occupation,
+                        const int* next_material, const uint64_t* seeds, int* action, int* out_material,
+                        int* out_active, double* out_occupation, double* out_velocity, double* uniforms, char* err,
+                        int errlen);
 int ref_interface_dmm(const ref_material_arrays* arrays, int n_materials, int axis, double frequency_bin_thz,
                       double reference_temperature, const double* energy_weights, int n, const int* material,
                       const int* active, const double* velocity, const double* occupation,

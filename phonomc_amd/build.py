@@ -19,8 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PMC_OFFLOAD_ARCH", "gfx950")
 
 DEVICE_SOURCES = ["pmc_kernels.hip", "pmc_capi.hip", "pmc_nn.hip"]
-HOST_SOURCES = ["material.cpp", "config.cpp", "mesh.cpp", "domain.cpp", "boundary_tables.cpp", "heat_source.cpp", "solver.cpp",
-                "host_capi.cpp"]
+HOST_SOURCES = ["material.cpp", "config.cpp", "mesh.cpp", "domain.cpp", "boundary_tables.cpp", "heat_source.cpp", "amm.cpp",
+                "solver.cpp", "host_capi.cpp"]
 
 
 def _run(cmd):

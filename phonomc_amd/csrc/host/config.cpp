@@ -374,10 +374,12 @@ SimConfig build(const std::map<std::string, std::string>& kv, const std::string&
         if (auto v = get("materials.interface_positions"))
             for (double x : numbers(*v)) c.interface_positions.push_back(10.0 * x);
         if (auto v = get("materials.interface_model")) {
+            // parse_interface_model (src/config/ConfigParsing.cpp:157-169): long or short names
             const auto m = lower(unquote(*v));
-            if (m != "dmm" && m != "amm" && m != "mmm")
-                throw std::runtime_error("materials.interface_model must be dmm, amm or mmm.");
-            c.interface_model = m;
+            if (m == "dmm" || m == "diffuse_mismatch") c.interface_model = "dmm";
+            else if (m == "amm" || m == "acoustic_mismatch") c.interface_model = "amm";
+            else if (m == "mmm" || m == "mixed_mismatch") c.interface_model = "mmm";
+            else throw std::runtime_error("materials.interface_model must be dmm, amm or mmm.");
         }
         if (auto v = get("materials.interface_frequency_bin_thz")) c.interface_bin_thz = to_double(*v);
         if (auto v = get("materials.interface_amm_fraction")) c.interface_amm_fraction = to_double(*v);

@@ -2,6 +2,7 @@
 #include "pmc_solver.h"
 #include "solver.hpp"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -131,6 +132,18 @@ int pmcs_problem_callaway_nonlinear(const pmcs_problem* pr, pmc_callaway_rates* 
     const auto& p = *(pr->operator->());
     if (out) *out = p.nl_view;
     return p.nonlinear ? 1 : 0;
+}
+
+int pmcs_problem_amm_audit(const pmcs_problem* pr, int32_t face, double* out) {
+    if (!pr || !out) return -1;
+    const auto& audits = (*pr)->layers.amm_audit;
+    if (face < 0 || face >= static_cast<int32_t>(audits.size())) return 0;
+    const pmc::AmmAudit& a = audits[static_cast<size_t>(face)];
+    const double v[9] = {a.incident_acoustic_flux[0], a.incident_acoustic_flux[1], a.transmitted_flux[0],
+                         a.transmitted_flux[1], a.no_overlap_flux[0], a.no_overlap_flux[1],
+                         a.geometric_overlap_flux, a.capacity_removed_flux, a.maximum_capacity_ratio};
+    std::copy(v, v + 9, out);
+    return 1;
 }
 
 int pmcs_problem_material(const pmcs_problem* pr, pmcs_material_view* out) {

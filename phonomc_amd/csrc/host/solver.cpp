@@ -186,6 +186,41 @@ void build_layers(Problem& p) {
         p.layer_raw.push_back(&e->raw);
     }
     p.layer_materials.assign(mats.begin(), mats.end());
+    // Acoustic/mixed mismatch (InterfaceInteraction::configure,
+    // src/solver/InterfaceInteraction.cpp:11-44): the AMM rows, in global
+    // active indices, blocked by (interface, side).
+    L.amm_fraction = c.interface_model == "amm" ? 1.0 : c.interface_model == "mmm" ? c.interface_amm_fraction : 0.0;
+    L.amm_block.assign(1, 0);
+    L.amm_reflected.clear();
+    L.amm_edge_begin.clear();
+    L.amm_edge_mode.clear();
+    L.amm_edge_prob.clear();
+    L.amm_audit.clear();
+    if (L.amm_fraction > 0) {
+        const auto tables = build_amm_tables(p.layer_modes, p.layer_raw, axis, c.mass_densities, c.interface_bin_thz,
+                                             c.interface_parallel_bin);
+        for (size_t f = 0; f < tables.size(); ++f) {
+            const AmmInterface& t = tables[f];
+            for (int side = 0; side < 2; ++side) {
+                const int src = static_cast<int>(f) + side, dst = static_cast<int>(f) + 1 - side;
+                const int n = L.mode_begin[src + 1] - L.mode_begin[src];
+                const int e0 = static_cast<int>(L.amm_edge_mode.size());
+                for (int i = 0; i < n; ++i) {
+                    const int r = t.reflected[side][i];
+                    L.amm_reflected.push_back(r < 0 ? -1 : L.mode_begin[src] + r);
+                    L.amm_edge_begin.push_back(e0 + t.edge_begin[side][i]);
+                }
+                L.amm_reflected.push_back(-1);
+                L.amm_edge_begin.push_back(e0 + t.edge_begin[side][n]);
+                for (size_t e = 0; e < t.edge_mode[side].size(); ++e) {
+                    L.amm_edge_mode.push_back(L.mode_begin[dst] + t.edge_mode[side][e]);
+                    L.amm_edge_prob.push_back(t.edge_prob[side][e]);
+                }
+                L.amm_block.push_back(static_cast<int32_t>(L.amm_reflected.size()));
+            }
+            L.amm_audit.push_back(t.audit);
+        }
+    }
 }
 
 // Reservoirs of a layered box: each reservoir facet touches one material
@@ -244,8 +279,6 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         if (conservative)
             throw std::runtime_error("Layered materials with simulation.conservative_boundaries are not supported by this build.");
         if (c.heat_source_enabled) throw std::runtime_error("Layered materials with a heat source are not supported by this build.");
-        if (c.interface_model != "dmm")
-            throw std::runtime_error("materials.interface_model = '" + c.interface_model + "' is not supported by this build; use dmm.");
         if (!std::isfinite(c.interface_bin_thz) || c.interface_bin_thz <= 0)
             throw std::runtime_error("Interface frequency bin must be finite and positive.");
     }
@@ -434,6 +467,12 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         lv.bank_mode = L.bank_mode.data();
         lv.bank_omega = L.bank_omega.data();
         lv.bank_prefix = L.bank_prefix.data();
+        lv.amm_fraction = L.amm_fraction;
+        lv.amm_block = L.amm_fraction > 0 ? L.amm_block.data() : nullptr;
+        lv.amm_reflected = L.amm_fraction > 0 ? L.amm_reflected.data() : nullptr;
+        lv.amm_edge_begin = L.amm_fraction > 0 ? L.amm_edge_begin.data() : nullptr;
+        lv.amm_edge_mode = L.amm_fraction > 0 && !L.amm_edge_mode.empty() ? L.amm_edge_mode.data() : nullptr;
+        lv.amm_edge_prob = L.amm_fraction > 0 && !L.amm_edge_prob.empty() ? L.amm_edge_prob.data() : nullptr;
     }
     auto& g = p->geometry_view;
     g.kind = 0;
@@ -884,7 +923,7 @@ void MonteCarloSolver::append_profile_summary(std::ostream& out) const {
     out << "diffuse_selected = " << l.rough_diffuse << '\n';
     out << "specular_ratio = " << static_cast<double>(l.rough_specular) / den << '\n';
     out << "diffuse_ratio = " << static_cast<double>(l.rough_diffuse) / den << '\n';
-    // [material_interface_diagnostics] (src/solver/ResultWriter.cpp:240-289), DMM keys.
+    // [material_interface_diagnostics] (src/solver/ResultWriter.cpp:240-289).
     const SimConfig& c = problem_->config;
     const auto& L = problem_->layers;
     const int materials = std::max<int>(1, static_cast<int>(c.material_folders.size()));
@@ -897,7 +936,30 @@ void MonteCarloSolver::append_profile_summary(std::ostream& out) const {
         << '\n';
     if (materials > 1) {
         out << "frequency_bin_thz = " << c.interface_bin_thz << '\n';
-        out << "amm_fraction = 0\n";
+        out << "amm_fraction = " << L.amm_fraction << '\n';
+        if (L.amm_fraction > 0) {
+            out << "amm_approximation = dispersive_scalar_acoustic_no_mode_conversion\n";
+            out << "amm_optical_treatment = specular_reflection\n";
+            out << "amm_kernel = reciprocal_linearized_qcell_overlap_v2\n";
+            out << "amm_bin_role = candidate_search_only\n";
+            out << "amm_channel_flux_units = inverse_Angstrom2_inverse_ps\n";
+            for (size_t i = 0; i < L.amm_audit.size(); ++i) {
+                const AmmAudit& a = L.amm_audit[i];
+                const std::string prefix = "interface_" + std::to_string(i) + "_";
+                out << prefix << "geometric_overlap_flux = " << a.geometric_overlap_flux << '\n';
+                out << prefix << "capacity_removed_flux = " << a.capacity_removed_flux << '\n';
+                out << prefix << "maximum_capacity_ratio = " << a.maximum_capacity_ratio << '\n';
+                for (int side = 0; side < 2; ++side) {
+                    const std::string key = prefix + "side_" + std::to_string(side) + "_";
+                    out << key << "incident_acoustic_flux = " << a.incident_acoustic_flux[side] << '\n';
+                    out << key << "transmitted_flux = " << a.transmitted_flux[side] << '\n';
+                    out << key << "no_overlap_flux = " << a.no_overlap_flux[side] << '\n';
+                }
+            }
+            out << "parallel_wavevector_bin_inv_a = " << c.interface_parallel_bin << '\n';
+            for (size_t m = 0; m < c.mass_densities.size(); ++m)
+                out << "material_" << m << "_mass_density_kg_m3 = " << c.mass_densities[m] << '\n';
+        }
         out << "carrier_sampling = equal_physical_state_weight\n";
         out << "fixed_flux_reservoirs = true\n";
         for (size_t k = 0; k < L.spatial_volume.size(); ++k)

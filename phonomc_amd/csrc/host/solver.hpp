@@ -5,6 +5,7 @@
 // reference cadence and formats (src/solver/Diagnostics.cpp:200-293,
 // src/solver/ResultWriter.cpp:25-119).
 #pragma once
+#include "amm.hpp"
 #include "boundary_tables.hpp"
 #include "heat_source.hpp"
 #include "config.hpp"
@@ -60,6 +61,11 @@ struct Problem {
         std::vector<int32_t> mode_begin, cell_material, lut_begin, bank_begin, bank_mode;
         std::vector<double> bank_omega, bank_prefix, lut_t, lut_e, gamma_total;
         std::vector<int32_t> active_flat;
+        // AMM rows (pmc_layers.amm_*), global active indices; audits per interface
+        double amm_fraction = 0;
+        std::vector<int32_t> amm_block, amm_reflected, amm_edge_begin, amm_edge_mode;
+        std::vector<double> amm_edge_prob;
+        std::vector<AmmAudit> amm_audit;
     } layers;
     pmc_layers layers_view{};
     std::vector<const ModeTables*> layer_modes;            // per material (layered problems)

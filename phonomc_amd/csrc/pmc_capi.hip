@@ -205,6 +205,8 @@ int collect_error(pmc_ctx* ctx) {
                           i);
         else if (st.error_facet == -11)
             std::snprintf(msg, sizeof msg, "sample_volume_points_naive: insufficient accepted samples.");
+        else if (st.error_facet == -13)
+            std::snprintf(msg, sizeof msg, "AMM incoming state is tangent or unsupported.");
         else
             std::snprintf(msg, sizeof msg,
                           "Boundary intersection failed while advancing particles for particle %lld at (%f, %f, %f). "
@@ -1183,8 +1185,44 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     v.bank_mode = ctx->upload(L->bank_mode, static_cast<size_t>(nb));
     v.bank_omega = ctx->upload(L->bank_omega, static_cast<size_t>(nb));
     v.bank_prefix = ctx->upload(L->bank_prefix, static_cast<size_t>(nb) + 2 * n);
+    // AMM rows (all-or-nothing: validated before anything is uploaded).
+    const double fa = L->amm_fraction;
+    if (!(fa >= 0.0 && fa <= 1.0)) return fail(ctx, PMC_ERR_ARGUMENT, "AMM fraction must be in [0,1].");
+    int n_rows = 0, n_edges = 0;
+    if (fa > 0.0) {
+        if (!L->amm_block || !L->amm_reflected || !L->amm_edge_begin)
+            return fail(ctx, PMC_ERR_ARGUMENT, "AMM tables missing");
+        if (L->amm_block[0] != 0) return fail(ctx, PMC_ERR_ARGUMENT, "malformed AMM blocks");
+        for (int f = 0; f + 1 < n; ++f)
+            for (int sd = 0; sd < 2; ++sd) {
+                const int b = 2 * f + sd, src = f + sd;
+                if (L->amm_block[b + 1] - L->amm_block[b] != L->mode_begin[src + 1] - L->mode_begin[src] + 1)
+                    return fail(ctx, PMC_ERR_ARGUMENT, "AMM block sizes do not match the materials");
+            }
+        n_rows = L->amm_block[2 * (n - 1)];
+        n_edges = L->amm_edge_begin[n_rows - 1];
+        if (n_edges > 0 && (!L->amm_edge_mode || !L->amm_edge_prob))
+            return fail(ctx, PMC_ERR_ARGUMENT, "AMM edge arrays missing");
+        for (int r = 0; r < n_rows; ++r) {
+            if (L->amm_reflected[r] < -1 || L->amm_reflected[r] >= v.na || L->amm_edge_begin[r] < 0 ||
+                L->amm_edge_begin[r] > n_edges || (r && L->amm_edge_begin[r] < L->amm_edge_begin[r - 1]))
+                return fail(ctx, PMC_ERR_ARGUMENT, "AMM rows out of range");
+        }
+        for (int e = 0; e < n_edges; ++e)
+            if (L->amm_edge_mode[e] < 0 || L->amm_edge_mode[e] >= v.na || !std::isfinite(L->amm_edge_prob[e]) ||
+                L->amm_edge_prob[e] < 0.0)
+                return fail(ctx, PMC_ERR_ARGUMENT, "AMM edges out of range");
+    }
     v.band_start = nbands ? ctx->upload(starts.data(), starts.size()) : nullptr;
     v.band_count = nbands;
+    v.amm_fraction = fa;
+    if (fa > 0.0) {
+        for (int b = 0; b <= 2 * (n - 1); ++b) v.amm_block[b] = L->amm_block[b];
+        v.amm_refl = ctx->upload(L->amm_reflected, static_cast<size_t>(n_rows));
+        v.amm_ebeg = ctx->upload(L->amm_edge_begin, static_cast<size_t>(n_rows));
+        v.amm_emode = n_edges ? ctx->upload(L->amm_edge_mode, static_cast<size_t>(n_edges)) : nullptr;
+        v.amm_eprob = n_edges ? ctx->upload(L->amm_edge_prob, static_cast<size_t>(n_edges)) : nullptr;
+    }
     v.layers = n;
     return check_hip(ctx, hipGetLastError(), "layers");
 }

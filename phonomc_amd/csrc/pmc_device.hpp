@@ -256,6 +256,16 @@ struct View {
     // band_count * band), so a band's modes are [start[j], start[j + 1]).
     const int* band_start;
     int band_count;
+    // Acoustic (AMM) / mixed (MMM) mismatch: probability of the AMM channel per
+    // event (0 DMM, 1 AMM), and the AMM rows by (interface, side) block:
+    // row amm_block[2f+s] + local source mode, its mirror partner and its
+    // transmitted edges [amm_ebeg[r], amm_ebeg[r + 1]).
+    double amm_fraction;
+    int amm_block[2 * (kMaxLayers - 1) + 1];
+    const int* amm_refl;
+    const int* amm_ebeg;
+    const int* amm_emode;
+    const double* amm_eprob;
     // boundary tables
     const RoughDev* rough;
     const ResDev* res;

@@ -485,9 +485,13 @@ __device__ void raise_error(const View& V, int code, long long slot, const float
 // advance() result for a discrete-mode reflection that found no inward mode
 // (conservative transport); reported as PMC_ERR_BOUNDARY with facet -7.
 constexpr int kErrModeMapping = 64;
+// advance() result for a carrier that meets an AMM interface in a direction its
+// mode has no row for (a mirrored carrier); PMC_ERR_BOUNDARY with facet -13.
+constexpr int kErrAmmRow = 65;
 
 __device__ __forceinline__ void raise_advance_error(const View& V, int rc, long long slot, const float* p) {
     if (rc == -kErrModeMapping) raise_error(V, PMC_ERR_BOUNDARY, slot, p, -7);
+    else if (rc == -kErrAmmRow) raise_error(V, PMC_ERR_BOUNDARY, slot, p, -13);
     else raise_error(V, -rc, slot, p, -1);
 }
 
@@ -708,15 +712,48 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
                     const int m = mw & kModeMask;
                     const double w = V.omega[m];
                     const int inc = va > 0.0f ? 1 : 0;
-                    const double u1 = U.next(), u2 = U.next(), u3 = U.next();
-                    double fr, ft;
-                    // DMM (InterfaceInteraction::interact, src/solver/InterfaceInteraction.cpp:86-101,
-                    // 111-149): reflection band in this material, transmission band in the next.
-                    const int mr = sample_band(V, mat, 1 - inc, w, u1, fr);
-                    const int mt = sample_band(V, dest, inc, w, u2, ft);
-                    const double tr = ft > 0.0 ? ft / fmax(1e-300, fr + ft) : 0.0;
-                    const bool transmit = mt >= 0 && u3 < tr;
-                    const int sel = transmit ? mt : mr;
+                    // InterfaceInteraction::interact (src/solver/InterfaceInteraction.cpp:46-153):
+                    // mixed mismatch draws the channel first; acoustic mismatch walks the
+                    // mode's row of transmitted edges with one draw (AcousticInterfaceSampler::sample,
+                    // src/solver/AcousticInterfaceSampler.cpp:457-471); diffuse mismatch
+                    // draws a reflection band in this material, a transmission band in
+                    // the next and the choice.
+                    const bool mixed = V.amm_fraction > 0.0 && V.amm_fraction < 1.0;
+                    const double um = mixed ? U.next() : 1.0;
+                    const bool amm = V.amm_fraction >= 1.0 || (mixed && um < V.amm_fraction);
+                    bool transmit = false;
+                    int sel = -1;
+                    if (amm) {
+                        const double vax = V.veld[3 * static_cast<size_t>(m) + ax] * ((mw >> (kFlipShift + ax)) & 1 ? -1.0 : 1.0);
+                        if (fabs(vax) < 1e-7) { // tangent: mirror along the axis
+                            T.iface_events++;
+                            mw ^= (1 << ax) << kFlipShift;
+                            mode_changed = true;
+                            p[ax] = va > 0.0f ? nextafterf(at, -INFINITY) : nextafterf(at, INFINITY);
+                            continue;
+                        }
+                        const int face = dest > mat ? mat : dest, side = dest > mat ? 0 : 1;
+                        const int r = V.amm_block[2 * face + side] + (m - V.mode_begin[mat]);
+                        sel = V.amm_refl[r];
+                        if (sel < 0) return -kErrAmmRow;
+                        double x = U.next();
+                        for (int e = V.amm_ebeg[r], e1 = V.amm_ebeg[r + 1]; e < e1; ++e) {
+                            x -= V.amm_eprob[e];
+                            if (x < 0.0) {
+                                sel = V.amm_emode[e];
+                                transmit = true;
+                                break;
+                            }
+                        }
+                    } else {
+                        const double u1 = U.next(), u2 = U.next(), u3 = U.next();
+                        double fr, ft;
+                        const int mr = sample_band(V, mat, 1 - inc, w, u1, fr);
+                        const int mt = sample_band(V, dest, inc, w, u2, ft);
+                        const double tr = ft > 0.0 ? ft / fmax(1e-300, fr + ft) : 0.0;
+                        transmit = mt >= 0 && u3 < tr;
+                        sel = transmit ? mt : mr;
+                    }
                     T.iface_events++;
                     double nocc = 0.0;
                     // Energy-preserving remap with the (common) energy weight, :111-122.

@@ -108,7 +108,10 @@ class Layers(C.Structure):
                 ("background_energy_density", C.POINTER(C.c_double)), ("lut_begin", C.POINTER(C.c_int32)),
                 ("spatial_volume", C.POINTER(C.c_double)), ("band_width", C.c_double),
                 ("bank_begin", C.POINTER(C.c_int32)), ("bank_mode", C.POINTER(C.c_int32)),
-                ("bank_omega", C.POINTER(C.c_double)), ("bank_prefix", C.POINTER(C.c_double))]
+                ("bank_omega", C.POINTER(C.c_double)), ("bank_prefix", C.POINTER(C.c_double)),
+                ("amm_fraction", C.c_double), ("amm_block", C.POINTER(C.c_int32)),
+                ("amm_reflected", C.POINTER(C.c_int32)), ("amm_edge_begin", C.POINTER(C.c_int32)),
+                ("amm_edge_mode", C.POINTER(C.c_int32)), ("amm_edge_prob", C.POINTER(C.c_double))]
 
 
 class LaunchConfig(C.Structure):
@@ -230,6 +233,7 @@ def lib():
         "pmc_upload_callaway": (C.c_int, [P, C.POINTER(CallawayTable)]),
         "pmc_upload_callaway_nonlinear": (C.c_int, [P, C.POINTER(CallawayRates)]),
         "pmcs_problem_callaway_nonlinear": (C.c_int, [P, C.POINTER(CallawayRates)]),
+        "pmcs_problem_amm_audit": (C.c_int, [P, C.c_int32, C.POINTER(C.c_double)]),
         "pmc_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
         "pmc_destroy": (None, [P]),
         "pmc_status_string": (C.c_char_p, [C.c_int32]),
@@ -311,6 +315,11 @@ class Problem:
         """The linearized Callaway view, or None under RTA."""
         t = CallawayTable()
         return t if lib().pmcs_problem_callaway(self.h, C.byref(t)) == 1 else None
+
+    def amm_audit(self, face=0):
+        """The AMM audit of interface `face` (9 values, see pmc_solver.h), or None without AMM."""
+        out = (C.c_double * 9)()
+        return np.array(out[:]) if lib().pmcs_problem_amm_audit(self.h, face, out) == 1 else None
 
     def callaway_nonlinear(self):
         """The nonlinear Callaway rate tables, or None unless the input selects them."""

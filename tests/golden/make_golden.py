@@ -8,6 +8,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --layered [--runs]             (layered Si/Ge film only)
     python tests/golden/make_golden.py --strict [--runs]              (strict-elastic walls of Callaway inputs)
     python tests/golden/make_golden.py --nonlinear [--runs]           (nonlinear Callaway, conservative RTA walls)
+    python tests/golden/make_golden.py --interfaces [--runs]          (acoustic and mixed mismatch interfaces)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -34,7 +35,7 @@ WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "
              "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml",
              "c1cs": "c1cs_callaway_source.toml", "c1l": "c1l_layered.toml",
              "finfet_cw": "finfet_callaway.toml", "c1cn": "c1cn_nonlinear.toml", "c2cn": "c2cn_nonlinear.toml",
-             "c2r": "c2r_conservative.toml"}
+             "c2r": "c2r_conservative.toml", "c1la": "c1la_amm.toml", "c1lm": "c1lm_mmm.toml"}
 
 
 def write_input(text, folder, name):
@@ -399,6 +400,75 @@ def layered_vectors(problem, rng, volumes, bin_thz, t_ref, n=4000):
                 out_occupation=r["occupation"], out_velocity=r["velocity"])
 
 
+def amm_vectors(problem):
+    """AcousticInterfaceSampler rows of both sides of the Si/Ge interface (every
+    row: mirror partner and transmitted edges) and the interface audit."""
+    lv = problem.layers()
+    mb = rt.arr(lv.mode_begin, lv.count + 1, np.int32)
+    views = [problem.layer_material(k) for k in range(lv.count)]
+    c = problem.config if hasattr(problem, "config") else None
+    out = {}
+    for side in (0, 1):
+        n = int(mb[side + 1] - mb[side])
+        refl, eb, ea, ep, audit = R.amm_rows(views, lv.axis, AMM_DENSITIES, 0.1, 0.05, 0, side, n)
+        out.update({f"side{side}_reflected": refl, f"side{side}_edge_begin": eb, f"side{side}_edge_active": ea,
+                    f"side{side}_edge_probability": ep})
+        out["audit"] = audit
+    return out
+
+
+def interface_model_vectors(problem, rng, model, fraction, volumes, bin_thz, t_ref, n=4000):
+    """InterfaceInteraction::interact under AMM (model 1) or MMM (model 2) on
+    carriers of both materials heading into the interface: one in ten with a
+    mirrored (axis-flipped) velocity (AMM has no row for it and the reference
+    throws), a few tangent modes (|v_axis| < 1e-7), some occupations whose remap
+    turns negative."""
+    s, m, g, i, rough, res = problem.views()
+    lv = problem.layers()
+    mb = rt.arr(lv.mode_begin, lv.count + 1, np.int32)
+    views = [problem.layer_material(k) for k in range(lv.count)]
+    refs = [R.RefMaterial(v) for v in views]
+    counts = np.array([lib_count(r) for r in refs])
+    norms = np.array([r.normalization() for r in refs])
+    weights = counts * volumes / norms
+    na = m.n_active
+    vel = rt.arr(m.velocity, 3 * na).reshape(na, 3)
+    neq = rt.arr(m.neq_background, na)
+    ax = lv.axis
+    mat = rng.integers(0, lv.count, n).astype(np.int32)
+    flipped = rng.random(n) < 0.1
+    tangent = rng.random(n) < 0.02
+    act, v, occ = np.zeros(n, np.int32), np.zeros((n, 3)), np.zeros(n)
+    for k in range(n):
+        up = mat[k] == 0
+        while True:
+            a = int(rng.integers(0, mb[mat[k] + 1] - mb[mat[k]]))
+            va = vel[mb[mat[k]] + a, ax] * (-1.0 if flipped[k] else 1.0)
+            if tangent[k]:
+                if abs(va) < 1e-7:
+                    break
+            elif (up and va > 0) or (not up and va < 0):
+                break
+        act[k] = a
+        v[k] = vel[mb[mat[k]] + a]
+        if flipped[k]:
+            v[k, ax] = -v[k, ax]
+        occ[k] = neq[mb[mat[k]] + a] * (1.0 + 0.5 * rng.normal()) - (5.0 if rng.random() < 0.05 else 0.0)
+    nxt = (1 - mat).astype(np.int32)
+    seeds = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+    r = R.interface_model(views, ax, model, fraction, AMM_DENSITIES, 0.05, bin_thz, t_ref, weights, mat, act, v, occ,
+                          nxt, seeds)
+    for x in refs:
+        x.close()
+    return dict(model=model, fraction=fraction, bin_thz=bin_thz, t_ref=t_ref, carrier_volumes=volumes,
+                energy_weights=weights, material=mat, active=act, velocity=v, occupation=occ, next_material=nxt,
+                uniforms=r["uniforms"], out_action=r["action"], out_material=r["material"], out_active=r["active"],
+                out_occupation=r["occupation"], out_velocity=r["velocity"])
+
+
+AMM_DENSITIES = [2329.0, 5323.0]
+
+
 def lib_count(refmat):
     return R.lib().ref_material_active_count(refmat.h)
 
@@ -500,6 +570,23 @@ def main():
             np.savez_compressed(os.path.join(OUT, "run_finfet.npz"),
                                 **whole_run("finfet", 100000, 400, list(range(1000, 1016)), 10))
         print("mesh fixtures written to", OUT)
+        return
+    if "--interfaces" in sys.argv:
+        rng = np.random.default_rng(20261211)
+        pa = rt.Problem(rt.config_text(WORKLOADS["c1la"]), input_directory=rt.CONFIGS, mode_sampling=1)
+        np.savez_compressed(os.path.join(OUT, "amm_c1la.npz"), **amm_vectors(pa))
+        volumes = carrier_volumes("c1la")
+        np.savez_compressed(os.path.join(OUT, "interface_c1la.npz"),
+                            **interface_model_vectors(pa, rng, 1, 1.0, volumes, 0.1, 300.0))
+        pm = rt.Problem(rt.config_text(WORKLOADS["c1lm"]), input_directory=rt.CONFIGS, mode_sampling=1)
+        np.savez_compressed(os.path.join(OUT, "interface_c1lm.npz"),
+                            **interface_model_vectors(pm, rng, 2, 0.5, volumes, 0.1, 300.0))
+        if "--runs" in sys.argv:
+            np.savez_compressed(os.path.join(OUT, "run_c1la.npz"),
+                                **whole_run("c1la", 20000, 400, list(range(1000, 1016)), 10))
+            np.savez_compressed(os.path.join(OUT, "run_c1lm.npz"),
+                                **whole_run("c1lm", 20000, 400, list(range(1000, 1016)), 10))
+        print("interface fixtures written to", OUT)
         return
     if "--nonlinear" in sys.argv:
         rng = np.random.default_rng(20261210)

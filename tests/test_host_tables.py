@@ -161,9 +161,9 @@ def test_unsupported_features_fail_loudly():
                                                           "heat_source.profile": '"gaussian"',
                                                           "heat_source.center": "[50, 50, 50]",
                                                           "heat_source.sigma": "[10, 10, 10]"}))
-    with pytest.raises(RuntimeError, match="interface_model = 'amm' is not supported"):
-        rt.Problem(rt.config_text("c1l_layered.toml", **{"materials.interface_model": '"amm"',
-                                                          "materials.mass_densities_kg_m3": "[2329, 5323]"}))
+    # Layered films run DMM, AMM and MMM interfaces, but not Callaway.
+    with pytest.raises(RuntimeError, match="Layered materials with scattering.model = 'callaway'"):
+        rt.Problem(rt.config_text("c1l_layered.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))
 
 
 def test_heat_source_rates_match_reference(golden):

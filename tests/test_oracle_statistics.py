@@ -1,5 +1,9 @@
 """Whole runs of the oracle against whole runs of the reference, on the CPU.
 
+The acoustic-mismatch Si/Ge film (c1la) is compared the same way with
+tests/golden/run_c1la.npz, cell temperatures and the interface event and
+transmission totals of each run (|z| <= 3).
+
 The nonlinear Callaway film (c1cn: displaced-Bose N/R substeps, strict-elastic
 conservative transport with fixed-flux reservoirs and Bose occupations) is run
 by the oracle for 8 seeds of its own Philox streams and compared with the 16
@@ -21,7 +25,7 @@ from phonomc_amd import runtime as rt
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def oracle_run(cfg, n, steps, interval, seed):
+def oracle_run(cfg, n, steps, interval, seed, counters=False):
     p = rt.Problem(rt.config_text(cfg, **{"simulation.particle_count": n, "simulation.random_seed": seed}),
                    input_directory=rt.CONFIGS)
     sim = O.Sim(p, threads=1)
@@ -31,8 +35,10 @@ def oracle_run(cfg, n, steps, interval, seed):
         sim.step(1)
         if s % interval == 0 and s > steps // 2:
             acc.append(sim.cells()[0])
+    led = sim.ledger()
     sim.close()
-    return np.mean(acc, axis=0)
+    t = np.mean(acc, axis=0)
+    return (t, [led["interface_events"], led["interface_transmitted"]]) if counters else t
 
 
 def z_score(a, b):
@@ -53,3 +59,22 @@ def test_nonlinear_callaway_oracle_runs_match_reference():
     # the profile carries the imposed gradient (301 K -> 299 K)
     t = temps.mean(axis=0)
     assert t[0] > 300.3 and t[-1] < 299.7
+
+
+def test_acoustic_mismatch_oracle_runs_match_reference():
+    golden = np.load(os.path.join(HERE, "golden", "run_c1la.npz"))
+    n, steps, interval = int(golden["n"]), int(golden["steps"]), int(golden["interval"])
+    seeds = [6000 + k for k in range(8)]
+    with concurrent.futures.ThreadPoolExecutor(8) as pool:
+        runs = list(pool.map(lambda s: oracle_run("c1la_amm.toml", n, steps, interval, s, counters=True), seeds))
+    temps = np.array([r[0] for r in runs])
+    iface = np.array([r[1] for r in runs], dtype=float)
+    z = z_score(temps, golden["temperature"])
+    zi = z_score(iface, golden["interface"].astype(float))
+    print(f"c1la oracle: cell T max|z| {np.abs(z).max():.2f}, mean z^2 {np.mean(z ** 2):.2f}; interface events "
+          f"{iface[:, 0].mean():.1f} vs {golden['interface'][:, 0].mean():.1f} (z {zi[0]:+.2f}), transmitted "
+          f"{iface[:, 1].mean():.1f} vs {golden['interface'][:, 1].mean():.1f} (z {zi[1]:+.2f})")
+    assert np.abs(z).max() <= 4.5
+    assert np.mean(z ** 2) <= 2.5
+    assert np.all(np.abs(zi) <= 3.0)
+    assert iface[:, 1].mean() > 0
