@@ -331,6 +331,16 @@ typedef struct pmc_heat_source {
     int32_t square;                /* 0 constant, 1 square wave */
     double start_ps, end_ps;       /* end < 0: no end */
     double period_ps, on_duration_ps, duty_cycle, amplitude;
+    /* Spectrum (PhononSourceOperator, src/solver/PhononSourceOperator.cpp:20-118,
+     * :186-220): 0 thermal (pseudo-temperature, or heat capacities under
+     * linearized transport); 1 weighted: a cell's eligible carriers share its
+     * energy in proportion to mode_weight (the branch weight over the largest);
+     * 2 gaussian: in proportion to exp(mode_weight) with
+     * mode_weight = -((f - f_c) / sigma)^2 / 2, f = omega / 2 pi in THz, normalized
+     * per cell about the cell's largest weight.  mode_weight may be NULL for 0. */
+    int32_t spectrum;
+    const double* mode_weight;     /* [n_active] */
+    double frequency_center_thz, frequency_sigma_thz; /* gaussian */
 } pmc_heat_source;
 
 /* Initial population (src/solver/Initialization.cpp:32-275). */

@@ -559,6 +559,15 @@ double orc_rta(const pmc_mode_table* m, int ncells, const double* cell_T, int64_
 int orc_source_deposit(const pmc_mode_table* m, int ncells, double* pending, const double* cell_T, double t_bg,
                        const int32_t* elig, int64_t n, const int32_t* active, const int32_t* cell, double* occ,
                        const double* capacity, double* energy_out, int64_t* updates_out) {
+    return orc_source_deposit_spectrum(m, ncells, pending, cell_T, t_bg, elig, n, active, cell, occ, capacity, 0,
+                                       nullptr, 0.0, 1.0, energy_out, updates_out);
+}
+
+int orc_source_deposit_spectrum(const pmc_mode_table* m, int ncells, double* pending, const double* cell_T,
+                                double t_bg, const int32_t* elig, int64_t n, const int32_t* active,
+                                const int32_t* cell, double* occ, const double* capacity, int spectrum,
+                                const double* mode_weight, double center_thz, double sigma_thz, double* energy_out,
+                                int64_t* updates_out) {
     *energy_out = 0.0;
     *updates_out = 0;
     double pending_total = 0.0;
@@ -578,7 +587,33 @@ int orc_source_deposit(const pmc_mode_table* m, int ncells, double* pending, con
         if (!std::isfinite(target)) return PMC_ERR_INVALID_STATE;
         const double tb = std::isfinite(cell_T[c]) && cell_T[c] >= 0.0 ? cell_T[c] : t_bg;
         std::vector<double> inc(idx.size());
-        if (capacity) {
+        if (spectrum == 2) {
+            // Gaussian in frequency (src/solver/PhononSourceOperator.cpp:186-206), normalized on
+            // the cell's carriers about their largest log weight, in long double.
+            std::vector<long double> lw(idx.size());
+            long double peak = -std::numeric_limits<long double>::infinity();
+            for (size_t k = 0; k < idx.size(); ++k) {
+                const long double f = m->omega[active[idx[k]]] / 6.283185307179586476925286766559L;
+                const long double z = (f - center_thz) / sigma_thz;
+                lw[k] = -.5L * z * z;
+                peak = std::max(peak, lw[k]);
+            }
+            long double sum = 0;
+            for (auto& w : lw) {
+                w = std::exp(w - peak);
+                sum += w;
+            }
+            for (size_t k = 0; k < idx.size(); ++k)
+                inc[k] = static_cast<double>(target * lw[k] / sum) / (kHbar * m->omega[active[idx[k]]]);
+        } else if (spectrum == 1) {
+            // Branch weights (:207-217): target * (w / sum w) / hw.
+            double sum = 0.0;
+            for (int64_t i : idx) sum += mode_weight[active[i]];
+            for (size_t k = 0; k < idx.size(); ++k) {
+                const int a = active[idx[k]];
+                inc[k] = target * (mode_weight[a] / sum) / (kHbar * m->omega[a]);
+            }
+        } else if (capacity) {
             // Linearized transport: target * C_i / sum C / hw_i with the mode heat
             // capacities at T_bg (src/solver/PhononSourceOperator.cpp:221-236).
             long double total = 0;
@@ -1120,6 +1155,7 @@ struct orc_sim {
     std::vector<double> hs_rate, hs_pending;
     double source_time = 0.0;
     std::vector<int> hs_elig;
+    std::vector<double> hs_weight; // spectrum weights per active mode (weighted spectrum)
     double elapsed = 0.0;
     // layered materials (DMM interfaces)
     int layers = 1, layer_axis = 0;
@@ -1550,9 +1586,11 @@ int apply_source(orc_sim& s, double factor, double* energy_out, int64_t* updates
     const int nc = s.g.n_cells;
     for (int c = 0; c < nc; ++c) s.hs_pending[c] += s.hs_rate[c] * factor;
     std::vector<int32_t> act(s.mode.begin(), s.mode.end());
-    return orc_source_deposit(&s.m, nc, s.hs_pending.data(), s.cell_T.data(), s.setup.background_temperature,
-                              s.hs_elig.data(), static_cast<int64_t>(act.size()), act.data(), s.cell.data(),
-                              s.occ.data(), s.linearized ? s.cw_cap.data() : nullptr, energy_out, updates_out);
+    return orc_source_deposit_spectrum(&s.m, nc, s.hs_pending.data(), s.cell_T.data(), s.setup.background_temperature,
+                                       s.hs_elig.data(), static_cast<int64_t>(act.size()), act.data(), s.cell.data(),
+                                       s.occ.data(), s.linearized ? s.cw_cap.data() : nullptr, s.hs.spectrum,
+                                       s.hs_weight.empty() ? nullptr : s.hs_weight.data(), s.hs.frequency_center_thz,
+                                       s.hs.frequency_sigma_thz, energy_out, updates_out);
 }
 } // namespace
 
@@ -1799,6 +1837,8 @@ int orc_set_heat_source(orc_sim* s, const pmc_heat_source* h) {
     s->hs = *h;
     s->hs_rate.assign(h->rate, h->rate + h->n_cells);
     s->hs_elig.assign(h->mode_eligible, h->mode_eligible + s->m.n_active);
+    if (h->spectrum != 0 && !h->mode_weight) return PMC_ERR_ARGUMENT;
+    if (h->mode_weight) s->hs_weight.assign(h->mode_weight, h->mode_weight + s->m.n_active);
     s->hs_pending.assign(h->n_cells, 0.0);
     s->hs_on = true;
     return PMC_OK;

@@ -87,6 +87,13 @@ int orc_source_deposit(const pmc_mode_table* m, int ncells, double* pending, con
                        const int32_t* elig, int64_t n, const int32_t* active, const int32_t* cell, double* occ,
                        const double* capacity, double* energy, int64_t* updates);
 /* Prescribed heat source, thermal spectrum (src/solver/PhononSourceOperator.cpp:120-341). */
+/* orc_source_deposit with the source spectrum of pmc_heat_source: 0 thermal,
+ * 1 weighted (mode_weight per active mode), 2 gaussian (center, sigma in THz). */
+int orc_source_deposit_spectrum(const pmc_mode_table* m, int ncells, double* pending, const double* cell_T,
+                                double t_bg, const int32_t* elig, int64_t n, const int32_t* active,
+                                const int32_t* cell, double* occ, const double* capacity, int spectrum,
+                                const double* mode_weight, double center_thz, double sigma_thz, double* energy_out,
+                                int64_t* updates_out);
 int orc_set_heat_source(orc_sim* sim, const pmc_heat_source* source);
 /* Layered materials with DMM interfaces (src/solver/LayeredMaterialLayout.cpp,
  * src/solver/InterfaceInteraction.cpp); before orc_initialize. */

@@ -81,6 +81,10 @@ def lib():
         L.orc_callaway_apply.argtypes = [C.POINTER(rt.ModeTable), C.POINTER(rt.CallawayTable), C.c_int, C.c_int64,
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(d), d, d,
                                          C.POINTER(C.c_int)]
+        L.orc_source_deposit_spectrum.argtypes = [C.POINTER(rt.ModeTable), C.c_int, C.POINTER(d), C.POINTER(d), d,
+                                                  C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32),
+                                                  C.POINTER(C.c_int32), C.POINTER(d), C.POINTER(d), C.c_int,
+                                                  C.POINTER(d), d, d, C.POINTER(d), C.POINTER(C.c_int64)]
         L.orc_source_deposit.argtypes = [C.POINTER(rt.ModeTable), C.c_int, C.POINTER(d), C.POINTER(d), d,
                                          C.POINTER(C.c_int32), C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(d), C.POINTER(d), C.POINTER(d), C.POINTER(C.c_int64)]
@@ -313,8 +317,8 @@ def rta(problem, cell_temperature, active, cell, occupation, dt, local, lifetime
 
 
 def source_deposit(problem, pending, cell_temperature, active, cell, occupation):
-    """Thermal-spectrum deposit of per-cell pending budgets (eV); returns occupations, energy, updates, pending.
-    Under linearized Callaway the increments follow the mode heat capacities at T_bg."""
+    """Heat-source deposit of per-cell pending budgets (eV) with the input's spectrum; returns occupations,
+    energy, updates, pending.  Thermal under linearized Callaway: increments by mode heat capacity at T_bg."""
     _, m, g, _, _, _ = problem.views()
     hs = problem.heat_source()
     pend = np.array(pending, dtype=np.float64)
@@ -325,8 +329,10 @@ def source_deposit(problem, pending, cell_temperature, active, cell, occupation)
     e, n = C.c_double(), C.c_int64()
     cw = problem.callaway()
     cap = cw.capacity if cw is not None else None
-    rc = lib().orc_source_deposit(C.byref(m), g.n_cells, _dp(pend), _dp(ct), problem.views()[0].background_temperature,
-                                  hs.mode_eligible, len(a), _ip(a), _ip(c), _dp(o), cap, C.byref(e), C.byref(n))
+    rc = lib().orc_source_deposit_spectrum(C.byref(m), g.n_cells, _dp(pend), _dp(ct),
+                                           problem.views()[0].background_temperature, hs.mode_eligible, len(a), _ip(a),
+                                           _ip(c), _dp(o), cap, hs.spectrum, hs.mode_weight, hs.frequency_center_thz,
+                                           hs.frequency_sigma_thz, C.byref(e), C.byref(n))
     if rc:
         raise RuntimeError(f"orc_source_deposit: {rc}")
     return o, e.value, n.value, pend

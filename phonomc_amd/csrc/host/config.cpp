@@ -404,9 +404,10 @@ SimConfig build(const std::map<std::string, std::string>& kv, const std::string&
     }
     if (auto v = get("heat_source.spectrum")) {
         const auto s = lower(unquote(*v));
-        if (s == "weighted" || s == "gaussian") c.has_non_thermal_spectrum = true;
-        else if (s != "thermal")
-            throw std::runtime_error("heat_source.spectrum must be 'thermal', 'weighted', or 'gaussian'.");
+        if (s == "thermal") c.heat_spectrum = 0;
+        else if (s == "weighted") c.heat_spectrum = 1;
+        else if (s == "gaussian") c.heat_spectrum = 2;
+        else throw std::runtime_error("heat_source.spectrum must be 'thermal', 'weighted', or 'gaussian'.");
     }
     if (auto v = get("heat_source.branches")) {
         for (double b : numbers(*v)) {
@@ -415,8 +416,9 @@ SimConfig build(const std::map<std::string, std::string>& kv, const std::string&
             c.heat_branches.push_back(static_cast<int>(b));
         }
     }
-    if (get("heat_source.branch_weights") || get("heat_source.frequency_center") || get("heat_source.frequency_sigma"))
-        c.has_non_thermal_spectrum = true;
+    if (auto v = get("heat_source.branch_weights")) c.heat_branch_weights = numbers(*v);
+    if (auto v = get("heat_source.frequency_center")) c.heat_frequency_center = to_double(*v);
+    if (auto v = get("heat_source.frequency_sigma")) c.heat_frequency_sigma = to_double(*v);
     if (auto v = get("heat_source.power_density")) c.heat_power_density = to_double(*v);
     if (auto v = get("heat_source.power_densities")) c.heat_power_densities = numbers(*v);
     if (auto v = get("heat_source.frequency_min")) c.heat_frequency_min = to_double(*v);
@@ -482,6 +484,28 @@ static void validate_heat_source(const SimConfig& c) {
     std::sort(branches.begin(), branches.end());
     if ((!branches.empty() && branches.front() < 0) || std::adjacent_find(branches.begin(), branches.end()) != branches.end())
         throw std::runtime_error("heat_source.branches must contain unique nonnegative indices.");
+    // Spectrum parameters (src/config/ConfigValidation.cpp:232-262).
+    if (c.heat_spectrum == 2) {
+        if (!c.heat_frequency_center || !c.heat_frequency_sigma || !std::isfinite(*c.heat_frequency_center) ||
+            *c.heat_frequency_center < 0 || !std::isfinite(*c.heat_frequency_sigma) || *c.heat_frequency_sigma <= 0)
+            throw std::runtime_error("Gaussian spectrum requires finite frequency_center >= 0 and frequency_sigma > 0, in THz.");
+    } else if (c.heat_frequency_center || c.heat_frequency_sigma) {
+        throw std::runtime_error("frequency_center/frequency_sigma require heat_source.spectrum='gaussian'.");
+    }
+    if (c.heat_spectrum != 1) {
+        if (!c.heat_branch_weights.empty())
+            throw std::runtime_error("branch_weights requires heat_source.spectrum='weighted'.");
+    } else {
+        bool positive = false;
+        for (size_t b = 0; b < c.heat_branch_weights.size(); ++b) {
+            const double w = c.heat_branch_weights[b];
+            if (!std::isfinite(w) || w < 0.0)
+                throw std::runtime_error("heat_source.branch_weights must be finite and nonnegative.");
+            if (branches.empty() || std::binary_search(branches.begin(), branches.end(), static_cast<int>(b)))
+                positive = positive || w > 0.0;
+        }
+        if (!positive) throw std::runtime_error("weighted heat_source needs positive weights on selected branches.");
+    }
     for (double q : c.heat_power_densities)
         if (!(q > 0.0) || !std::isfinite(q))
             throw std::runtime_error("heat_source.power_densities values must be finite and positive.");

@@ -59,6 +59,9 @@ struct SimConfig {
     std::optional<double> heat_total_power;   // W
     double heat_frequency_min = -1.0, heat_frequency_max = -1.0; // THz, <0: no cutoff
     std::vector<int> heat_branches;
+    int heat_spectrum = 0;                    // 0 thermal, 1 weighted, 2 gaussian
+    std::vector<double> heat_branch_weights;  // weighted spectrum, per branch
+    std::optional<double> heat_frequency_center, heat_frequency_sigma; // gaussian spectrum, THz
     bool heat_time_square = false;
     double heat_time_start = 0.0, heat_time_end = -1.0; // ps
     double heat_period = 0.0, heat_on_duration = -1.0, heat_duty_cycle = 0.5, heat_amplitude = 1.0;
@@ -77,7 +80,6 @@ struct SimConfig {
     std::vector<double> mass_densities;   // kg/m^3, one per layer (AMM/MMM)
     bool has_convergence_stop = false;
     bool has_mode_excitation = false;
-    bool has_non_thermal_spectrum = false;
 };
 
 SimConfig parse_toml_file(const std::string& path);

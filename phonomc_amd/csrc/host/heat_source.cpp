@@ -100,7 +100,21 @@ HeatSource build_heat_source(const SimConfig& c, const Domain& d, const ModeTabl
         throw std::runtime_error("Total heat-source rate must be finite and representably positive.");
     // Modes the thermal spectrum deposits into (PhononSourceOperator::mode_weight,
     // src/solver/PhononSourceOperator.cpp:105-118): frequency window and branch filter.
+    // PhononSourceOperator::configure (:20-103): branch indices and weights per
+    // material, weights over the largest; the Gaussian's log weights over the
+    // positive-frequency bank.
+    for (int b : c.heat_branches)
+        if (b >= m.branches) throw std::invalid_argument("heat_source.branches index exceeds a material's branch count.");
+    std::vector<double> branch_weight = c.heat_branch_weights;
+    if (!branch_weight.empty()) {
+        const double scale = *std::max_element(branch_weight.begin(), branch_weight.end());
+        for (double& w : branch_weight) w /= scale;
+    }
+    if (c.heat_spectrum == 1 && branch_weight.size() != static_cast<size_t>(m.branches))
+        throw std::invalid_argument("heat_source.branch_weights must match each material's branch count.");
+    hs.spectrum = c.heat_spectrum;
     hs.mode_eligible.assign(m.n_active, 1);
+    hs.mode_weight.assign(m.n_active, 0.0);
     for (int a = 0; a < m.n_active; ++a) {
         const int f = m.active_flat[a];
         const double thz = m.omega_flat[f] / kTwoPi;
@@ -108,7 +122,29 @@ HeatSource build_heat_source(const SimConfig& c, const Domain& d, const ModeTabl
                   !(c.heat_frequency_max >= 0.0 && thz > c.heat_frequency_max);
         if (ok && !c.heat_branches.empty())
             ok = std::find(c.heat_branches.begin(), c.heat_branches.end(), f % m.branches) != c.heat_branches.end();
-        hs.mode_eligible[a] = ok ? 1 : 0;
+        double weight = ok ? 1.0 : 0.0;
+        if (ok && c.heat_spectrum == 1) weight = branch_weight[static_cast<size_t>(f % m.branches)];
+        hs.mode_eligible[a] = weight > 0.0 ? 1 : 0;
+        hs.mode_weight[a] = c.heat_spectrum == 1 ? weight : 0.0;
+    }
+    if (c.heat_spectrum == 2) {
+        size_t positive = 0;
+        for (double w : m.omega_flat)
+            if (w > 0) ++positive;
+        if (positive != static_cast<size_t>(m.n_active))
+            throw std::invalid_argument("Gaussian source requires all positive-frequency modes, including stationary modes.");
+        hs.frequency_center = *c.heat_frequency_center;
+        hs.frequency_sigma = *c.heat_frequency_sigma;
+        bool any = false;
+        for (int a = 0; a < m.n_active; ++a) {
+            if (!hs.mode_eligible[a]) continue;
+            const long double f = m.omega_flat[m.active_flat[a]] / 6.283185307179586476925286766559L;
+            const long double z = (f - hs.frequency_center) / hs.frequency_sigma;
+            hs.mode_weight[a] = static_cast<double>(-0.5L * z * z);
+            any = true;
+        }
+        if (!any)
+            throw std::invalid_argument("Gaussian source branch/frequency filters select no positive-frequency modes.");
     }
     hs.enabled = true;
     hs.selected_cells = selected;

@@ -13,7 +13,13 @@ namespace pmc {
 struct HeatSource {
     bool enabled = false;
     std::vector<double> rates;     // eV/ps per cell
-    std::vector<int> mode_eligible; // per active mode: the thermal spectrum's frequency/branch filter
+    std::vector<int> mode_eligible; // per active mode: the spectrum's frequency/branch filter (and weight > 0)
+    // Spectrum (PhononSourceOperator, src/solver/PhononSourceOperator.cpp:20-118):
+    // 0 thermal, 1 weighted (mode_weight = the branch weight over the largest),
+    // 2 gaussian (mode_weight = -((f - f_c) / sigma)^2 / 2, f = omega / 2 pi in THz).
+    int spectrum = 0;
+    std::vector<double> mode_weight;
+    double frequency_center = 0.0, frequency_sigma = 1.0; // THz, gaussian
     int selected_cells = 0;
     bool square = false;
     double start = 0.0, end = -1.0, period = 0.0, on_duration = -1.0, duty_cycle = 0.5, amplitude = 1.0;

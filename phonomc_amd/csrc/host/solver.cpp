@@ -269,8 +269,6 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     if (c.collision == CollisionModel::Callaway &&
         (!std::isfinite(c.background_temperature) || c.background_temperature <= 0))
         throw std::runtime_error("Callaway requires a positive fixed background_temperature.");
-    if (c.heat_source_enabled && c.has_non_thermal_spectrum)
-        throw std::runtime_error("heat_source.spectrum other than 'thermal' is not supported by this build.");
     if (c.has_gradient_drive) throw std::runtime_error("driving.temperature_gradient is not supported by this build.");
     const bool layered = c.material_folders.size() > 1;
     if (layered) {
@@ -288,10 +286,11 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
 
     auto p = std::make_unique<Problem>();
     p->config = c;
-    // Stationary modes join the active bank under conservative transport
-    // (src/material/MaterialFactory.cpp:51-55).
+    // Stationary modes join the active bank under conservative transport and
+    // for the Gaussian source spectrum (src/material/MaterialFactory.cpp:51-55).
     const auto material = cached_material(c.material_folders.empty() ? c.material_folder : c.material_folders.front(),
-                                          c.input_directory, c.temperature_lookup_dt, conservative);
+                                          c.input_directory, c.temperature_lookup_dt,
+                                          conservative || (c.heat_source_enabled && c.heat_spectrum == 2));
     p->raw = material->raw;
     p->modes = material->modes;
     p->linearized = linearized;
@@ -409,6 +408,10 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         h.on_duration_ps = p->heat.on_duration;
         h.duty_cycle = p->heat.duty_cycle;
         h.amplitude = p->heat.amplitude;
+        h.spectrum = p->heat.spectrum;
+        h.mode_weight = p->heat.mode_weight.data();
+        h.frequency_center_thz = p->heat.frequency_center;
+        h.frequency_sigma_thz = p->heat.frequency_sigma;
     }
 
     // C-ABI views.
@@ -847,8 +850,9 @@ void MonteCarloSolver::write_source_ledger(const pmc_ledger& l) const {
     throw_if_failed(pmc_read_source_pending(ctx_, pending.data()));
     std::ofstream out(std::filesystem::path(folder_) / "heat_source_ledger.txt", std::ios::trunc);
     if (!out) throw std::runtime_error("Cannot open output file: " + folder_ + "/heat_source_ledger.txt");
+    static const char* kSpectra[] = {"thermal", "weighted", "gaussian"};
     out << std::setprecision(17) << "# prescribed_lattice_power; not an electron-phonon calculation\n"
-        << "# spectrum=thermal; pending energy is not yet represented by phonons\n"
+        << "# spectrum=" << kSpectra[problem_->config.heat_spectrum] << "; pending energy is not yet represented by phonons\n"
         << "# cell x_A y_A z_A material base_power_W prescribed_eV deposited_eV pending_eV\n";
     const auto& cm = p.layers.cell_material;
     for (int c = 0; c < nc; ++c) {
@@ -863,7 +867,7 @@ void MonteCarloSolver::write_source_ledger(const pmc_ledger& l) const {
 
 void MonteCarloSolver::append_source_summary(std::ostream& out, const pmc_ledger& l) const {
     // [local_heat_source_occupation_injection] (ResultWriter::format_source_summary,
-    // src/solver/ResultWriter.cpp:121-170); this build deposits the thermal spectrum only.
+    // src/solver/ResultWriter.cpp:121-170).
     const SimConfig& c = problem_->config;
     const HeatSource& h = problem_->heat;
     const double prescribed = source_prescribed_ev(l);
@@ -875,14 +879,23 @@ void MonteCarloSolver::append_source_summary(std::ostream& out, const pmc_ledger
     out << "model = prescribed_lattice_power\n";
     out << "electron_transport = false\n";
     out << "empty_cell_policy = defer_locally\n";
-    out << "spectrum = thermal\n";
+    static const char* kSpectra[] = {"thermal", "weighted", "gaussian"};
+    out << "spectrum = " << kSpectra[c.heat_spectrum] << '\n';
+    if (c.heat_spectrum == 2) {
+        out << "frequency_center_thz = " << c.heat_frequency_center.value_or(0) << '\n';
+        out << "frequency_sigma_thz = " << c.heat_frequency_sigma.value_or(0) << '\n';
+        out << "spectral_normalization = existing_carrier_weights_within_each_cell\n";
+        out << "spectral_weight = exp(-0.5*((f-f_center)/sigma_f)^2); energy_per_mode\n";
+    }
     out << "power_input = " << (c.heat_total_power ? "total_power" : "power_density") << '\n';
     out << "base_total_power_w = " << (h.enabled ? h.base_power_w() : 0.0) << '\n';
     out << "frequency_min_thz = " << c.heat_frequency_min << '\n';
     out << "frequency_max_thz = " << c.heat_frequency_max << '\n';
     out << "branches_zero_based = [";
     for (size_t i = 0; i < c.heat_branches.size(); ++i) out << (i ? ", " : "") << c.heat_branches[i];
-    out << "]\nbranch_weights_per_sampled_mode = []\n";
+    out << "]\nbranch_weights_per_sampled_mode = [";
+    for (size_t i = 0; i < c.heat_branch_weights.size(); ++i) out << (i ? ", " : "") << c.heat_branch_weights[i];
+    out << "]\n";
     out << "prescribed_energy_total_ev = " << prescribed << '\n';
     out << "pending_energy_total_ev = " << l.source_pending_ev << '\n';
     out << "deposition_complete = " << (l.source_pending_ev == 0.0 ? "true" : "false") << '\n';

@@ -640,8 +640,16 @@ int pmc_upload_heat_source(pmc_ctx* ctx, const pmc_heat_source* h) {
     if (!ctx || !h || !ctx->has_material || !ctx->has_geometry || h->n_cells != ctx->view.ncells || !h->rate ||
         !h->mode_eligible)
         return fail(ctx, PMC_ERR_ARGUMENT, "invalid heat source");
+    if (h->spectrum < 0 || h->spectrum > 2 || (h->spectrum != 0 && !h->mode_weight))
+        return fail(ctx, PMC_ERR_ARGUMENT, "heat-source spectrum must be 0, 1 or 2 with mode weights for 1 and 2");
     View& v = ctx->view;
+    if (h->spectrum != 0)
+        for (int a = 0; a < v.na; ++a)
+            if (!std::isfinite(h->mode_weight[a]) || (h->spectrum == 1 && h->mode_weight[a] < 0.0))
+                return fail(ctx, PMC_ERR_ARGUMENT, "invalid heat-source mode weight");
     const size_t nc = static_cast<size_t>(v.ncells);
+    v.hs_spectrum = h->spectrum;
+    v.hs_w = h->spectrum != 0 ? ctx->upload(h->mode_weight, static_cast<size_t>(v.na)) : nullptr;
     v.hs_rate = ctx->upload(h->rate, nc);
     v.hs_elig = ctx->upload(h->mode_eligible, static_cast<size_t>(v.na));
     v.hs_pending = ctx->alloc<double>(nc);
@@ -655,6 +663,7 @@ int pmc_upload_heat_source(pmc_ctx* ctx, const pmc_heat_source* h) {
     ctx->hs = *h;
     ctx->hs.rate = nullptr;
     ctx->hs.mode_eligible = nullptr;
+    ctx->hs.mode_weight = nullptr;
     v.hs_square = h->square;
     v.hs_start = h->start_ps;
     v.hs_end = h->end_ps;

@@ -298,6 +298,9 @@ struct View {
     double* hs_acc;              // kSrcCols per chunk
     double* hs_cell;             // kSrcCellCols per cell: T_b, T_s, target, deposit flag
     double* chunk_src;           // energy change of the second half-step deposit per chunk (tally units)
+    // spectrum: 0 thermal, 1 weighted (hs_w = branch weight), 2 gaussian (hs_w = log weight)
+    int hs_spectrum;
+    const double* hs_w;
     // time profile of the source (pmc_heat_source): constant or square wave
     int hs_square;
     double hs_start, hs_end, hs_period, hs_on_duration, hs_duty, hs_amplitude;

@@ -3033,6 +3033,22 @@ __device__ __forceinline__ int chunk_range(const View& V, int b, int& begin, int
     return c;
 }
 
+// Largest value over the block of W waves (every thread receives it); -inf
+// when every value is.
+template <int W>
+__device__ __forceinline__ double block_max_of(double a, double* lds) {
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a = fmax(a, __shfl_xor(a, off, 64));
+    if (__lane_id() == 0) lds[wave] = a;
+    __syncthreads();
+    a = -INFINITY;
+#pragma unroll
+    for (int w = 0; w < W; ++w) a = fmax(a, lds[w]);
+    __syncthreads();
+    return a;
+}
+
 // Per chunk: Taylor terms of sum hw n_BE(T) about the base temperature T_b of
 // the eligible carriers of cells with pending energy.
 __device__ __forceinline__ void source_sums_body(const View& V, const Grid G, int half) {
@@ -3046,15 +3062,38 @@ __device__ __forceinline__ void source_sums_body(const View& V, const Grid G, in
     double acc[kSrcCols];
 #pragma unroll
     for (int k = 0; k < kSrcCols; ++k) acc[k] = 0.0;
-    if (pend > 0.0 && V.linearized) {
-        // Linearized transport: the cell's heat capacity at T_bg over its eligible
-        // carriers (PhononSourceOperator::apply, src/solver/PhononSourceOperator.cpp:221-236).
+    if (pend > 0.0 && V.hs_spectrum == 2) {
+        // Gaussian spectrum (PhononSourceOperator::apply, src/solver/PhononSourceOperator.cpp:186-206):
+        // the chunk's largest log weight M and sum exp(l - M), combined per cell
+        // about the cell's largest (a lane keeps a running maximum and rescales).
+        double mx = -INFINITY, sx = 0.0;
         for (int i = begin + static_cast<int>(threadIdx.x); i < end; i += kChunkThreads) {
             const int m = V.mode[i] & kModeMask;
             double w, hw;
             if (!source_eligible(V, m, w, hw)) continue;
             acc[kSrcCount] += 1.0;
-            acc[kSrcM1] += V.cw_cap[m];
+            const double l = V.hs_w[m];
+            if (l > mx) {
+                sx = sx * exp(mx - l) + 1.0;
+                mx = l;
+            } else {
+                sx += exp(l - mx);
+            }
+        }
+        const double M = block_max_of<kChunkWaves>(mx, lds);
+        acc[kSrcM1] = sx > 0.0 ? sx * exp(mx - M) : 0.0;
+        block_reduce<kSrcCols, kChunkWaves>(acc, lds);
+        acc[kSrcM1 + 1] = M;
+    } else if (pend > 0.0 && (V.hs_spectrum == 1 || V.linearized)) {
+        // Weighted spectrum: the branch weights of the eligible carriers (:207-217);
+        // thermal spectrum under linearized transport: their heat capacities at
+        // T_bg (:221-236).  Either way one scale per cell.
+        for (int i = begin + static_cast<int>(threadIdx.x); i < end; i += kChunkThreads) {
+            const int m = V.mode[i] & kModeMask;
+            double w, hw;
+            if (!source_eligible(V, m, w, hw)) continue;
+            acc[kSrcCount] += 1.0;
+            acc[kSrcM1] += V.hs_spectrum == 1 ? V.hs_w[m] : V.cw_cap[m];
         }
         block_reduce<kSrcCols, kChunkWaves>(acc, lds);
     } else if (pend > 0.0) {
@@ -3157,7 +3196,30 @@ __device__ __forceinline__ void source_solve_body(const View& V, const Grid G, i
         raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -4);
         return;
     }
-    if (V.linearized) { // increments target * C_i / sum C / hw_i: one scale per cell
+    if (V.hs_spectrum == 2) { // increments target * exp(l_i - P) / S / hw_i about the cell's largest P
+        double mloc = -INFINITY;
+        for (int ch = k0 + __lane_id(); ch < k1; ch += 64) {
+            const double* row = V.hs_acc + static_cast<size_t>(ch) * kSrcCols;
+            if (row[kSrcCount] > 0.0) mloc = fmax(mloc, row[kSrcM1 + 1]);
+        }
+        const double P = wave_max(mloc);
+        double sloc = 0.0;
+        for (int ch = k0 + __lane_id(); ch < k1; ch += 64) {
+            const double* row = V.hs_acc + static_cast<size_t>(ch) * kSrcCols;
+            if (row[kSrcCount] > 0.0) sloc += row[kSrcM1] * exp(row[kSrcM1 + 1] - P);
+        }
+        const double S = wave_sum(sloc);
+        if (lead) {
+            cellv[0] = P;
+            cellv[1] = target / S;
+            cellv[2] = target;
+            cellv[3] = 1.0;
+            V.hs_pending[c] = 0.0;
+            atomicAdd(&V.st->step_source_e, pend);
+        }
+        return;
+    }
+    if (V.hs_spectrum == 1 || V.linearized) { // increments target * w_i / sum w / hw_i: one scale per cell
         if (!(acc[kSrcM1] > 0.0)) {
             raise_error(V, PMC_ERR_INVALID_STATE, c, nullptr, -12);
             return;
@@ -3279,7 +3341,9 @@ __device__ __forceinline__ void source_increments_body(const View& V, const Grid
         double w, hw;
         double inc = 0.0;
         if (source_eligible(V, m, w, hw)) {
-            inc = V.linearized ? ts * V.cw_cap[m] / hw : bose(ts, w) - bose(tb, w);
+            if (V.hs_spectrum == 2) inc = ts * exp(V.hs_w[m] - tb) / hw; // tb holds the cell's largest log weight
+            else if (V.hs_spectrum == 1) inc = ts * V.hs_w[m] / hw;
+            else inc = V.linearized ? ts * V.cw_cap[m] / hw : bose(ts, w) - bose(tb, w);
             const double e = hw * inc;
             applied += e;
             const unsigned id = V.uid[i];

@@ -87,7 +87,9 @@ class Reservoir(C.Structure):
 class HeatSource(C.Structure):
     _fields_ = [("n_cells", C.c_int32), ("rate", C.POINTER(C.c_double)), ("mode_eligible", C.POINTER(C.c_int32)),
                 ("square", C.c_int32), ("start_ps", C.c_double), ("end_ps", C.c_double), ("period_ps", C.c_double),
-                ("on_duration_ps", C.c_double), ("duty_cycle", C.c_double), ("amplitude", C.c_double)]
+                ("on_duration_ps", C.c_double), ("duty_cycle", C.c_double), ("amplitude", C.c_double),
+                ("spectrum", C.c_int32), ("mode_weight", C.POINTER(C.c_double)),
+                ("frequency_center_thz", C.c_double), ("frequency_sigma_thz", C.c_double)]
 
 
 class CallawayTable(C.Structure):

@@ -9,6 +9,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --strict [--runs]              (strict-elastic walls of Callaway inputs)
     python tests/golden/make_golden.py --nonlinear [--runs]           (nonlinear Callaway, conservative RTA walls)
     python tests/golden/make_golden.py --interfaces [--runs]          (acoustic and mixed mismatch interfaces)
+    python tests/golden/make_golden.py --spectra [--runs]             (weighted and Gaussian source spectra)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -35,7 +36,8 @@ WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "
              "c1c": "c1c_callaway.toml", "c2c": "c2c_callaway.toml", "finfet": "finfet.toml",
              "c1cs": "c1cs_callaway_source.toml", "c1l": "c1l_layered.toml",
              "finfet_cw": "finfet_callaway.toml", "c1cn": "c1cn_nonlinear.toml", "c2cn": "c2cn_nonlinear.toml",
-             "c2r": "c2r_conservative.toml", "c1la": "c1la_amm.toml", "c1lm": "c1lm_mmm.toml"}
+             "c2r": "c2r_conservative.toml", "c1la": "c1la_amm.toml", "c1lm": "c1lm_mmm.toml",
+             "c1sw": "c1sw_weighted.toml", "c1sg": "c1sg_gaussian.toml"}
 
 
 def write_input(text, folder, name):
@@ -570,6 +572,24 @@ def main():
             np.savez_compressed(os.path.join(OUT, "run_finfet.npz"),
                                 **whole_run("finfet", 100000, 400, list(range(1000, 1016)), 10))
         print("mesh fixtures written to", OUT)
+        return
+    if "--spectra" in sys.argv:
+        rng = np.random.default_rng(20261212)
+        # weighted branches; Gaussian at 5 THz (1.5 THz); a narrow Gaussian above every
+        # mode (40 THz, 0.05 THz: every weight underflows without the per-cell peak)
+        narrow = {"heat_source.frequency_center": "40.0", "heat_source.frequency_sigma": "0.05"}
+        for key, name, stationary, over in (("c1sw", "source_c1sw.npz", False, {}),
+                                            ("c1sg", "source_c1sg.npz", True, {}),
+                                            ("c1sg", "source_c1sg_narrow.npz", True, narrow)):
+            p, rm, rd, _ = problem_and_reference(key, include_stationary=stationary, **over)
+            np.savez_compressed(os.path.join(OUT, name), **source_vectors(p, rm, rd, rng, 0.5))
+            rm.close()
+        if "--runs" in sys.argv:
+            np.savez_compressed(os.path.join(OUT, "run_c1sw.npz"),
+                                **whole_run("c1sw", 20000, 400, list(range(1000, 1016)), 10))
+            np.savez_compressed(os.path.join(OUT, "run_c1sg.npz"),
+                                **whole_run("c1sg", 20000, 400, list(range(1000, 1016)), 10))
+        print("source spectrum fixtures written to", OUT)
         return
     if "--interfaces" in sys.argv:
         rng = np.random.default_rng(20261211)

@@ -177,8 +177,18 @@ def test_heat_source_rates_match_reference(golden):
 
 
 @pytest.mark.parametrize("bad,message", [
-    ({"heat_source.spectrum": '"weighted"', "heat_source.branch_weights": "[1, 1, 1, 1, 1, 1]"},
-     "spectrum other than 'thermal' is not supported"),
+    ({"heat_source.spectrum": '"weighted"', "heat_source.branch_weights": "[0, 0, 0, 0, 0, 0]"},
+     "weighted heat_source needs positive weights on selected branches."),
+    ({"heat_source.spectrum": '"weighted"', "heat_source.branch_weights": "[1, 1, 1]"},
+     "heat_source.branch_weights must match each material's branch count."),
+    ({"heat_source.branch_weights": "[1, 1, 1, 1, 1, 1]"}, "branch_weights requires heat_source.spectrum='weighted'."),
+    ({"heat_source.spectrum": '"gaussian"', "heat_source.frequency_center": "5.0"},
+     r"Gaussian spectrum requires finite frequency_center >= 0 and frequency_sigma > 0, in THz."),
+    ({"heat_source.frequency_sigma": "1.0"}, "frequency_center/frequency_sigma require heat_source.spectrum='gaussian'."),
+    ({"heat_source.branches": "[7]"}, "heat_source.branches index exceeds a material's branch count."),
+    ({"heat_source.spectrum": '"gaussian"', "heat_source.frequency_center": "5.0", "heat_source.frequency_sigma": "1.0",
+      "heat_source.frequency_min": "40"},
+     "Gaussian source branch/frequency filters select no positive-frequency modes."),
     ({"heat_source.time_profile": '"square"'}, "square heat_source.time_profile requires a finite positive period."),
     ({"heat_source.duty_cycle": "1.5"}, r"heat_source.duty_cycle must be in \[0, 1\]."),
     ({"heat_source.power_density": "-1"}, "enabled heat_source requires positive total_power or power_density."),

@@ -76,17 +76,27 @@ def test_refresh_and_rta_match_reference(problems, golden, key):
     assert res == pytest.approx(float(g["residual"]), rel=1e-9, abs=1e-12)
 
 
-@pytest.mark.parametrize("cfg,fixture", [("c1s_source.toml", "source_c1s.npz"),
-                                         ("c1cs_callaway_source.toml", "source_c1cs.npz")])
-def test_source_deposit_matches_reference(golden, cfg, fixture):
-    """Thermal-spectrum heat-source deposit against PhononSourceOperator::apply:
-    under RTA a bracket + Newton pseudo-temperature per cell, under linearized
-    Callaway increments proportional to the mode heat capacity at T_bg; both with
-    the roundoff correction on the largest carrier."""
+NARROW = {"heat_source.frequency_center": "40.0", "heat_source.frequency_sigma": "0.05"}
+
+
+@pytest.mark.parametrize("cfg,fixture,over", [("c1s_source.toml", "source_c1s.npz", {}),
+                                              ("c1cs_callaway_source.toml", "source_c1cs.npz", {}),
+                                              ("c1sw_weighted.toml", "source_c1sw.npz", {}),
+                                              ("c1sg_gaussian.toml", "source_c1sg.npz", {}),
+                                              ("c1sg_gaussian.toml", "source_c1sg_narrow.npz", NARROW)])
+def test_source_deposit_matches_reference(golden, cfg, fixture, over):
+    """Heat-source deposit against PhononSourceOperator::apply: the thermal
+    spectrum under RTA (a bracket + Newton pseudo-temperature per cell) and under
+    linearized Callaway (increments proportional to the mode heat capacity at
+    T_bg), the weighted spectrum (branch weights over the largest, branch 3 off)
+    and the Gaussian spectrum in frequency (5 THz centre, 1.5 THz width; and a
+    0.05 THz Gaussian at 40 THz, above every mode, whose weights survive only by
+    the per-cell peak normalization); all with the roundoff correction on the
+    largest carrier."""
     from phonomc_amd import runtime as rt
 
     g = golden(fixture)
-    p = rt.Problem(rt.config_text(cfg), mode_sampling=1)
+    p = rt.Problem(rt.config_text(cfg, **over), input_directory=rt.CONFIGS, mode_sampling=1)
     pending = g["rates"] * float(g["factor"])
     occ, energy, updates, left = O.source_deposit(p, pending, g["cell_t"], g["active"], g["cell"], g["occ"])
     np.testing.assert_allclose(occ, g["occ_after"], rtol=1e-12, atol=0)
@@ -121,3 +131,22 @@ def test_planted_mistake_is_caught(problems, golden):
         a, _, _, _ = O.rough_sample(p, 0, ins[k], occ[k], [u[k, 0], 1.0 - u[k, 1], 0.5, 0.5])
         wrong += a != out[k]
     assert wrong > 0
+
+
+def test_source_spectrum_planted_mistake_is_caught(golden):
+    """A Gaussian centre read in rad/ps instead of THz (2 pi times too high)
+    moves the deposit far from the reference's; so does depositing the weighted
+    budget thermally."""
+    from phonomc_amd import runtime as rt
+
+    g = golden("source_c1sg.npz")
+    wrong = rt.Problem(rt.config_text("c1sg_gaussian.toml", **{"heat_source.frequency_center": str(5.0 * 2 * np.pi)}),
+                       input_directory=rt.CONFIGS, mode_sampling=1)
+    occ, *_ = O.source_deposit(wrong, g["rates"] * float(g["factor"]), g["cell_t"], g["active"], g["cell"], g["occ"])
+    assert np.max(np.abs(occ - g["occ_after"]) / np.abs(g["occ_after"])) > 1e-3
+    g = golden("source_c1sw.npz")
+    thermal = rt.Problem(rt.config_text("c1sw_weighted.toml").replace('spectrum = "weighted"', 'spectrum = "thermal"')
+                         .replace("branch_weights = [1.0, 0.5, 2.0, 0.0, 0.3, 0.1]\n", ""),
+                         input_directory=rt.CONFIGS, mode_sampling=1)
+    occ, *_ = O.source_deposit(thermal, g["rates"] * float(g["factor"]), g["cell_t"], g["active"], g["cell"], g["occ"])
+    assert np.max(np.abs(occ - g["occ_after"]) / np.abs(g["occ_after"])) > 1e-3
