@@ -351,6 +351,12 @@ typedef struct pmc_init {
     double cold_point[3], hot_point[3];
     double cold_temperature, hot_temperature;
     const double* cell_temperature;/* [n_cells] for mode 2 (warm start) */
+    /* Initial targeted modal over-occupation (MonteCarloSolver::apply_initial_mode_excitation,
+     * src/solver/Initialization.cpp:146-178): with excite = 1 the initial occupation
+     * of every carrier whose frequency omega / 2 pi lies in [excite_fmin_thz,
+     * excite_fmax_thz] (a negative bound: none) is multiplied by excite_multiplier. */
+    int32_t excite;
+    double excite_fmin_thz, excite_fmax_thz, excite_multiplier;
 } pmc_init;
 
 int pmc_create(const pmc_setup* setup, pmc_ctx** out);

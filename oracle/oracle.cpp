@@ -1727,6 +1727,12 @@ int orc_initialize(orc_sim* s, const pmc_init* init) {
         } else {
             s->occ[i] = bose(T, s->m.omega[m]);
         }
+        if (init->excite) { // apply_initial_mode_excitation (src/solver/Initialization.cpp:146-178)
+            const double f = s->m.omega[m] / 6.283185307179586476925286766559;
+            if (std::isfinite(f) && !(init->excite_fmin_thz >= 0.0 && f < init->excite_fmin_thz) &&
+                !(init->excite_fmax_thz >= 0.0 && f > init->excite_fmax_thz))
+                s->occ[i] *= init->excite_multiplier;
+        }
         s->uid[i] = static_cast<uint32_t>(i);
     }
     return finish_population(s);

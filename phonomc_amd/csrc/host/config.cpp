@@ -391,6 +391,9 @@ SimConfig build(const std::map<std::string, std::string>& kv, const std::string&
             throw std::runtime_error("AMM explicit fraction must equal 1; use MMM for intermediate weights.");
     }
     if (auto v = get("mode_excitation.enabled")) c.has_mode_excitation = to_bool(*v);
+    if (auto v = get("mode_excitation.frequency_min")) c.excite_fmin = to_double(*v);
+    if (auto v = get("mode_excitation.frequency_max")) c.excite_fmax = to_double(*v);
+    if (auto v = get("mode_excitation.occupation_multiplier")) c.excite_multiplier = to_double(*v);
 
     // Heat source (src/config/ConfigParsing.cpp:658-766).
     bool enabled_set = false;
@@ -629,6 +632,17 @@ void validate_config(const SimConfig& c) {
                                      "once, in pairs.");
     }
     if (c.heat_source_enabled) validate_heat_source(c);
+    // validate_mode_excitation_config (src/config/ConfigValidation.cpp:81-107).
+    if (c.has_mode_excitation) {
+        if (!(c.excite_multiplier > 0.0) || !std::isfinite(c.excite_multiplier))
+            throw std::runtime_error("mode_excitation.occupation_multiplier must be finite and positive.");
+        if (!(c.excite_fmin < 0.0) && !(std::isfinite(c.excite_fmin) && c.excite_fmin >= 0.0))
+            throw std::runtime_error("mode_excitation.frequency_min must be finite and non-negative if provided.");
+        if (!(c.excite_fmax < 0.0) && !(std::isfinite(c.excite_fmax) && c.excite_fmax >= 0.0))
+            throw std::runtime_error("mode_excitation.frequency_max must be finite and non-negative if provided.");
+        if (c.excite_fmin >= 0.0 && c.excite_fmax >= 0.0 && c.excite_fmax < c.excite_fmin)
+            throw std::runtime_error("mode_excitation.frequency_max must be no smaller than frequency_min.");
+    }
     validate_materials(c);
     if (!std::isfinite(c.background_temperature) || c.background_temperature < 0.0)
         throw std::runtime_error("background_temperature must be a finite non-negative value.");

@@ -79,7 +79,8 @@ struct SimConfig {
     double interface_parallel_bin = 0.05; // 1/Angstrom
     std::vector<double> mass_densities;   // kg/m^3, one per layer (AMM/MMM)
     bool has_convergence_stop = false;
-    bool has_mode_excitation = false;
+    bool has_mode_excitation = false;        // mode_excitation.enabled
+    double excite_fmin = -1.0, excite_fmax = -1.0, excite_multiplier = 1.0; // THz (<0: no cut), factor
 };
 
 SimConfig parse_toml_file(const std::string& path);

@@ -485,12 +485,10 @@ void write_domain_summary(const Domain& d, const SimConfig& c, const std::string
     out << "heat_source_frequency_min_THz = " << c.heat_frequency_min << '\n';
     out << "heat_source_frequency_max_THz = " << c.heat_frequency_max << '\n';
     out << "heat_source_time_profile = " << (c.heat_time_square ? "square" : "constant") << '\n';
-    // [mode_excitation] is parsed and rejected at set-up: a summary is only
-    // written for inputs without it.
-    out << "mode_excitation_enabled = false\n";
-    out << "mode_excitation_frequency_min_THz = -1\n";
-    out << "mode_excitation_frequency_max_THz = -1\n";
-    out << "mode_excitation_occupation_multiplier = 1\n";
+    out << "mode_excitation_enabled = " << (c.has_mode_excitation ? "true" : "false") << '\n';
+    out << "mode_excitation_frequency_min_THz = " << c.excite_fmin << '\n';
+    out << "mode_excitation_frequency_max_THz = " << c.excite_fmax << '\n';
+    out << "mode_excitation_occupation_multiplier = " << c.excite_multiplier << '\n';
     out << '\n';
     out << "[geometry]\n";
     out << "volume = " << d.volume << '\n';

@@ -281,7 +281,6 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
             throw std::runtime_error("Interface frequency bin must be finite and positive.");
     }
     if (c.has_convergence_stop) throw std::runtime_error("convergence.enabled requires gradient driving.");
-    if (c.has_mode_excitation) throw std::runtime_error("mode_excitation is not supported by this build.");
     if (c.material_folder.empty() && c.material_folders.empty()) throw std::runtime_error("io.material_folder is required.");
 
     auto p = std::make_unique<Problem>();
@@ -626,6 +625,12 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     // Initial temperatures (src/solver/Initialization.cpp:181-265).
     auto& in = p->init;
     in.count = n;
+    // Initial mode excitation (src/solver/Initialization.cpp:146-178): skipped for a
+    // multiplier within 1e-15 of 1.
+    in.excite = c.has_mode_excitation && c.excite_multiplier > 0.0 && std::abs(c.excite_multiplier - 1.0) >= 1e-15;
+    in.excite_fmin_thz = c.excite_fmin;
+    in.excite_fmax_thz = c.excite_fmax;
+    in.excite_multiplier = c.excite_multiplier;
     double tmin = 299.0, tmax = 301.0;
     bool any = false;
     for (const auto& r : p->reservoirs) {

@@ -806,7 +806,13 @@ int pmc_initialize_carriers(pmc_ctx* ctx, const pmc_init* init) {
     if (init->count <= 0 || init->count > ctx->setup.capacity) return fail(ctx, PMC_ERR_ARGUMENT, "invalid carrier count");
     if (init->temperature_mode == 2 && !init->cell_temperature)
         return fail(ctx, PMC_ERR_ARGUMENT, "per-cell initial temperature field missing");
+    if (init->excite && (!(init->excite_multiplier > 0.0) || !std::isfinite(init->excite_multiplier)))
+        return fail(ctx, PMC_ERR_ARGUMENT, "mode_excitation.occupation_multiplier must be finite and positive.");
     View& v = view(ctx);
+    v.ex_on = init->excite ? 1 : 0;
+    v.ex_fmin = init->excite_fmin_thz;
+    v.ex_fmax = init->excite_fmax_thz;
+    v.ex_mult = init->excite_multiplier;
     double* field = nullptr;
     if (init->temperature_mode == 2) field = ctx->upload(init->cell_temperature, v.ncells);
     pmc_launch::init(v, ctx->stream, static_cast<int>(init->count), init->temperature_mode, init->uniform_temperature,

@@ -346,6 +346,9 @@ struct View {
     double cw_cbg;          // heat-capacity density at T_bg, eV/(K A^3)
     int cw_threads;         // k_callaway workgroup size: 256, 384 or kCwThreads
     int* cw_big;            // cells of more than kCwWaveMax carriers this step (count in DevState)
+    // initial mode excitation (pmc_init.excite): occupation factor in a frequency band
+    int ex_on;
+    double ex_fmin, ex_fmax, ex_mult; // THz (< 0: no bound)
     // nonlinear Callaway collisions (rates at the cell or lifetime temperature)
     int nl_callaway;
     int nl_nt;              // rate temperature samples

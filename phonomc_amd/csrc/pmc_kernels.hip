@@ -3701,6 +3701,11 @@ __global__ void __launch_bounds__(256) k_init(View V, int count, int tmode, doub
     } else {
         V.occ[i] = bose(T, V.omega[m]);
     }
+    if (V.ex_on) { // MonteCarloSolver::apply_initial_mode_excitation (src/solver/Initialization.cpp:146-178)
+        const double f = V.omega[m] / 6.283185307179586476925286766559;
+        if (isfinite(f) && !(V.ex_fmin >= 0.0 && f < V.ex_fmin) && !(V.ex_fmax >= 0.0 && f > V.ex_fmax))
+            V.occ[i] *= V.ex_mult;
+    }
     V.cell[i] = cell;
     V.uid[i] = static_cast<unsigned>(i);
 }

@@ -149,7 +149,9 @@ class Error(C.Structure):
 class Init(C.Structure):
     _fields_ = [("count", C.c_int64), ("temperature_mode", C.c_int32), ("uniform_temperature", C.c_double),
                 ("cold_point", C.c_double * 3), ("hot_point", C.c_double * 3), ("cold_temperature", C.c_double),
-                ("hot_temperature", C.c_double), ("cell_temperature", C.POINTER(C.c_double))]
+                ("hot_temperature", C.c_double), ("cell_temperature", C.POINTER(C.c_double)),
+                ("excite", C.c_int32), ("excite_fmin_thz", C.c_double), ("excite_fmax_thz", C.c_double),
+                ("excite_multiplier", C.c_double)]
 
 
 class MaterialView(C.Structure):
