@@ -188,7 +188,13 @@ int collect_error(pmc_ctx* ctx) {
         std::snprintf(msg, sizeof msg, "Carrier slot capacity exhausted while injecting into slot %lld.", i);
         break;
     case PMC_ERR_UNSUPPORTED:
-        if (st.error_facet == -8)
+        if (st.error_facet == -14)
+            std::snprintf(msg, sizeof msg,
+                          "Particle %lld was mirrored on a facet that is not axis-aligned (a rough wall kept its "
+                          "incoming mode because the remapped occupation would be negative, or a periodic facet "
+                          "without a partner); the mirrored direction is not a table velocity.",
+                          i);
+        else if (st.error_facet == -8)
             std::snprintf(msg, sizeof msg,
                           "Cell %lld holds more carriers than the Callaway kernel takes per cell (%d); use more "
                           "cells or fewer carriers.",
@@ -514,7 +520,10 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
         f[i].res = g->facets[i].reservoir;
         f[i].pair = g->facets[i].periodic_pair;
         // Carriers keep a table mode plus per-axis velocity signs, so a mirror
-        // reflection needs an axis-aligned facet.
+        // reflection (a periodic facet without a partner, outside conservative
+        // transport) needs an axis-aligned facet.  A facet with a reflection
+        // table sends carriers into table modes that point into the body, on
+        // any orientation.
         f[i].axis = -1;
         for (int a = 0; a < 3; ++a) {
             const double o1 = f[i].n[(a + 1) % 3], o2 = f[i].n[(a + 2) % 3];
@@ -522,11 +531,11 @@ int pmc_upload_geometry(pmc_ctx* ctx, const pmc_geometry* g) {
                 f[i].axis = a;
         }
         const bool mirrors = f[i].cond == 'R' || (f[i].cond == 'P' && f[i].pair < 0);
-        if (mirrors && f[i].axis < 0) {
-            char msg[160];
+        if (mirrors && f[i].axis < 0 && f[i].rough < 0) {
+            char msg[200];
             std::snprintf(msg, sizeof msg,
-                          "facet %d reflects carriers but is not axis-aligned; the device takes mirror "
-                          "reflections on axis-aligned facets only",
+                          "facet %d reflects carriers by a mirror but is not axis-aligned; the device takes "
+                          "mirror reflections on axis-aligned facets only",
                           i);
             return fail(ctx, PMC_ERR_UNSUPPORTED, msg);
         }

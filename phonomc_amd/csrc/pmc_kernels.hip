@@ -488,10 +488,15 @@ constexpr int kErrModeMapping = 64;
 // advance() result for a carrier that meets an AMM interface in a direction its
 // mode has no row for (a mirrored carrier); PMC_ERR_BOUNDARY with facet -13.
 constexpr int kErrAmmRow = 65;
+// advance() result for a mirror reflection on a facet that is not axis-aligned:
+// the mirrored velocity is no table velocity with sign flips, so the carrier
+// state cannot hold it; PMC_ERR_UNSUPPORTED with facet -14.
+constexpr int kErrObliqueMirror = 66;
 
 __device__ __forceinline__ void raise_advance_error(const View& V, int rc, long long slot, const float* p) {
     if (rc == -kErrModeMapping) raise_error(V, PMC_ERR_BOUNDARY, slot, p, -7);
     else if (rc == -kErrAmmRow) raise_error(V, PMC_ERR_BOUNDARY, slot, p, -13);
+    else if (rc == -kErrObliqueMirror) raise_error(V, PMC_ERR_UNSUPPORTED, slot, p, -14);
     else raise_error(V, -rc, slot, p, -1);
 }
 
@@ -828,7 +833,10 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             else if (kind == 2) T.rough_diff++;
             // New velocity is the table velocity of the outgoing mode, mirrored if it
             // points out of the facet; under conservative transport it must point in
-            // (src/solver/BoundaryInteraction.cpp:90-113).
+            // (src/solver/BoundaryInteraction.cpp:90-113).  The sampler returns modes
+            // that point in, except when it keeps the incoming mode because a remapped
+            // occupation would be negative; on a facet that is not axis-aligned that
+            // mirror is no table direction (ha < 0).
             int nw = out_ai;
             const double* vo = V.veld + 3 * static_cast<size_t>(out_ai);
             const double vn = vo[0] * F.n[0] + vo[1] * F.n[1] + vo[2] * F.n[2];
@@ -836,6 +844,7 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
                 const double wo = V.omega[out_ai];
                 if (!isfinite(out_occ) || !(wo > 0.0) || !(vn < 0.0)) return -kErrModeMapping;
             } else if (vn > 0.0) {
+                if (ha < 0) return -kErrObliqueMirror;
                 nw |= (1 << ha) << kFlipShift;
             }
             if (nw != mw) mode_changed = true;
@@ -847,6 +856,7 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
         if (F.cond == 'R') return -PMC_ERR_UNSUPPORTED; // table-less rough facet
         if (V.conservative) return -kErrModeMapping; // reflecting facet without a table
         // Specular mirror (periodic facet without a partner).
+        if (ha < 0) return -kErrObliqueMirror;
         mw ^= (1 << ha) << kFlipShift;
         mode_changed = true;
     }

@@ -10,6 +10,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --nonlinear [--runs]           (nonlinear Callaway, conservative RTA walls)
     python tests/golden/make_golden.py --interfaces [--runs]          (acoustic and mixed mismatch interfaces)
     python tests/golden/make_golden.py --spectra [--runs]             (weighted and Gaussian source spectra)
+    python tests/golden/make_golden.py --oblique [--runs]             (tapered fin: rough walls not axis-aligned)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -20,6 +21,7 @@ written; nothing of the reference's code is copied.
 import os
 import re
 import shutil
+import struct
 import sys
 import tempfile
 
@@ -37,7 +39,7 @@ WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "
              "c1cs": "c1cs_callaway_source.toml", "c1l": "c1l_layered.toml",
              "finfet_cw": "finfet_callaway.toml", "c1cn": "c1cn_nonlinear.toml", "c2cn": "c2cn_nonlinear.toml",
              "c2r": "c2r_conservative.toml", "c1la": "c1la_amm.toml", "c1lm": "c1lm_mmm.toml",
-             "c1sw": "c1sw_weighted.toml", "c1sg": "c1sg_gaussian.toml"}
+             "c1sw": "c1sw_weighted.toml", "c1sg": "c1sg_gaussian.toml", "taper": "taper_fin.toml"}
 
 
 def write_input(text, folder, name):
@@ -49,6 +51,47 @@ def write_input(text, folder, name):
     with open(path, "w") as f:
         f.write(text)
     return path
+
+
+def write_tapered_fin_stl(path, substrate=(200.0, 100.0, 50.0), base=(80.0, 120.0), top=(95.0, 105.0),
+                          span=(30.0, 70.0), height=100.0):
+    """Binary STL (nm) of a fin with sloped x walls on a substrate: the fin's
+    footprint base[0]..base[1] narrows to top[0]..top[1] over `height`, between
+    span[0] and span[1] in y.  28 triangles, closed and consistently wound."""
+    sx, sy, sz = substrate
+    z1 = sz + height
+    tris = []
+
+    def quad(a, b, c, d, outward):
+        for t in ((a, b, c), (a, c, d)):
+            p = [np.array(v, float) for v in t]
+            if np.cross(p[1] - p[0], p[2] - p[0]) @ np.array(outward, float) < 0:
+                t = (t[0], t[2], t[1])
+            tris.append(t)
+
+    quad((0, 0, 0), (sx, 0, 0), (sx, sy, 0), (0, sy, 0), (0, 0, -1))
+    quad((0, 0, 0), (0, sy, 0), (0, sy, sz), (0, 0, sz), (-1, 0, 0))
+    quad((sx, 0, 0), (sx, sy, 0), (sx, sy, sz), (sx, 0, sz), (1, 0, 0))
+    quad((0, 0, 0), (sx, 0, 0), (sx, 0, sz), (0, 0, sz), (0, -1, 0))
+    quad((0, sy, 0), (sx, sy, 0), (sx, sy, sz), (0, sy, sz), (0, 1, 0))
+    (b0, b1), (t0, t1), (y0, y1) = base, top, span
+    outer = [(0, 0, sz), (sx, 0, sz), (sx, sy, sz), (0, sy, sz)]
+    inner = [(b0, y0, sz), (b1, y0, sz), (b1, y1, sz), (b0, y1, sz)]
+    for k in range(4):  # substrate top around the fin's footprint
+        quad(outer[k], outer[(k + 1) % 4], inner[(k + 1) % 4], inner[k], (0, 0, 1))
+    quad((b0, y0, sz), (b0, y1, sz), (t0, y1, z1), (t0, y0, z1), (-height, 0, t0 - b0))
+    quad((b1, y0, sz), (b1, y1, sz), (t1, y1, z1), (t1, y0, z1), (height, 0, b1 - t1))
+    quad((b0, y0, sz), (b1, y0, sz), (t1, y0, z1), (t0, y0, z1), (0, -1, 0))
+    quad((b0, y1, sz), (b1, y1, sz), (t1, y1, z1), (t0, y1, z1), (0, 1, 0))
+    quad((t0, y0, z1), (t1, y0, z1), (t1, y1, z1), (t0, y1, z1), (0, 0, 1))
+    with open(path, "wb") as f:
+        f.write(b"tapered fin on a substrate (nm)".ljust(80, b" "))
+        f.write(struct.pack("<I", len(tris)))
+        for t in tris:
+            p = [np.array(v, float) for v in t]
+            n = np.cross(p[1] - p[0], p[2] - p[0])
+            f.write(struct.pack("<12fH", *(n / np.linalg.norm(n)), *p[0], *p[1], *p[2], 0))
+    return len(tris)
 
 
 def problem_and_reference(key, include_stationary=False, **overrides):
@@ -543,6 +586,24 @@ def output_files():
 
 
 def main():
+    if "--oblique" in sys.argv:
+        rng = np.random.default_rng(20261213)
+        write_tapered_fin_stl(os.path.join(rt.CONFIGS, "taper_fin.stl"))
+        # tapered fin: the fin's two x walls lean 8.5 degrees off the vertical
+        p, rm, rd, _ = problem_and_reference("taper")
+        np.savez_compressed(os.path.join(OUT, "domain_taper.npz"), **mesh_vectors(p, rd, rng))
+        np.savez_compressed(os.path.join(OUT, "rough_taper.npz"), **rough_vectors(p, rm, rd, rng, [5, 6, 7]))
+        rm.close()
+        # the same walls as strict-elastic tables (simulation.conservative_boundaries)
+        p, rm, rd, _ = problem_and_reference("taper", include_stationary=True,
+                                             **{"simulation.conservative_boundaries": "true"})
+        np.savez_compressed(os.path.join(OUT, "rough_taper_strict.npz"), **strict_vectors(p, rm, rd, rng, 1500))
+        rm.close()
+        if "--runs" in sys.argv:
+            np.savez_compressed(os.path.join(OUT, "run_taper.npz"),
+                                **whole_run("taper", 100000, 400, list(range(1000, 1016)), 10))
+        print("oblique-wall fixtures written to", OUT)
+        return
     if "--outputs" in sys.argv:
         np.savez_compressed(os.path.join(OUT, "outputs_c1cs.npz"), **output_files())
         print("output-file fixture written to", OUT)

@@ -1,5 +1,10 @@
 """Whole runs of the oracle against whole runs of the reference, on the CPU.
 
+The tapered fin (taper_fin.toml: rough fin walls 8.5 degrees off the vertical,
+Gaussian source in the fin, 1e5 carriers, 400 steps) is compared the same way
+with tests/golden/run_taper.npz, cell temperatures only, cells the heat never
+reaches (T_bg in every run of both sets) scoring zero.
+
 The acoustic-mismatch Si/Ge film (c1la) is compared the same way with
 tests/golden/run_c1la.npz, cell temperatures and the interface event and
 transmission totals of each run (|z| <= 3).
@@ -8,8 +13,8 @@ The nonlinear Callaway film (c1cn: displaced-Bose N/R substeps, strict-elastic
 conservative transport with fixed-flux reservoirs and Bose occupations) is run
 by the oracle for 8 seeds of its own Philox streams and compared with the 16
 reference runs of tests/golden/run_c1cn.npz (2e4 carriers, 400 steps): the
-second-half mean of every cell temperature, sampled every 10 steps as the
-reference writes convergence.txt, by the z-score of test_gpu_statistics
+second-half mean of every cell temperature, sampled every 10 steps and printed
+to six significant digits as the reference writes convergence.txt, by the z-score of test_gpu_statistics
 (max |z| <= 4.5 over the 20 cells, mean z^2 <= 2.5).  This pins the oracle's
 whole step for this transport policy, which the device is then compared with
 carrier by carrier.
@@ -34,7 +39,9 @@ def oracle_run(cfg, n, steps, interval, seed, counters=False):
     for s in range(1, steps + 1):
         sim.step(1)
         if s % interval == 0 and s > steps // 2:
-            acc.append(sim.cells()[0])
+            # as convergence.txt prints them: six significant digits (a 1e-3 K quantum
+            # near 300 K), which matters where a cell stays within 1e-3 K of T_bg
+            acc.append([float(f"{t:g}") for t in sim.cells()[0]])
     led = sim.ledger()
     sim.close()
     t = np.mean(acc, axis=0)
@@ -78,3 +85,21 @@ def test_acoustic_mismatch_oracle_runs_match_reference():
     assert np.mean(z ** 2) <= 2.5
     assert np.all(np.abs(zi) <= 3.0)
     assert iface[:, 1].mean() > 0
+
+
+def test_tapered_fin_oracle_runs_match_reference():
+    golden = np.load(os.path.join(HERE, "golden", "run_taper.npz"))
+    n, steps, interval = int(golden["n"]), int(golden["steps"]), int(golden["interval"])
+    seeds = [7000 + k for k in range(8)]
+    with concurrent.futures.ThreadPoolExecutor(8) as pool:
+        temps = np.array(list(pool.map(lambda s: oracle_run("taper_fin.toml", n, steps, interval, s), seeds)))
+    ref = golden["temperature"]
+    spread = (temps.std(axis=0) > 0) | (ref.std(axis=0) > 0)
+    # a cell without spread in either set must hold the same value in both
+    assert np.array_equal(temps[:, ~spread].mean(axis=0), ref[:, ~spread].mean(axis=0))
+    z = z_score(temps[:, spread], ref[:, spread])
+    print(f"taper oracle: cell T max|z| {np.abs(z).max():.2f}, mean z^2 {np.mean(z ** 2):.2f} over {spread.sum()} cells")
+    assert np.abs(z).max() <= 4.5
+    assert np.mean(z ** 2) <= 2.5
+    # the source heats the fin top above T_bg
+    assert temps.mean(axis=0).max() > 300.5
