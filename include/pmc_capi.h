@@ -173,6 +173,12 @@ typedef struct pmc_rough_table {
     const int32_t* roulette_active;
     const double* roulette_cdf;
     int32_t strict_elastic;
+    /* The material's active-mode range for layered materials (the arrays are by
+     * global active index; a row outside the range has no specular partner or
+     * window); mode_count 0: every active mode.  The last-resort draw over any
+     * mode (RoughBoundarySampler::sample_diffuse, :451-455) stays in this range. */
+    int32_t mode_begin;
+    int32_t mode_count;
 } pmc_rough_table;
 
 /* Thermal reservoir on one facet; replaces ReservoirSampler/ReservoirInjector
@@ -227,6 +233,14 @@ typedef struct pmc_layers {
     const int32_t* amm_edge_begin; /* row r's transmitted edges are [amm_edge_begin[r], amm_edge_begin[r+1]) */
     const int32_t* amm_edge_mode;  /* global active mode transmitted into */
     const double* amm_edge_prob;   /* its probability */
+    /* Reflection tables by material (RoughBoundarySampler::table_index,
+     * src/solver/RoughBoundarySampler.cpp:68-74, used at
+     * src/solver/TransportOperator.cpp:302): facet f reflects a carrier of
+     * material k through table facet_rough[k * n_facets + f] of
+     * pmc_upload_boundary_tables (-1: no table).  Those tables hold global active
+     * indices with mode_begin / mode_count set to the material's range.  NULL:
+     * every material uses pmc_facet.rough_table. */
+    const int32_t* facet_rough;
 } pmc_layers;
 
 /* Linearized Callaway scattering about the fixed reference T_bg; replaces

@@ -432,7 +432,9 @@ int orc_rough_sample_signed(const pmc_mode_table* m, const pmc_rough_table* r, i
             const int k = std::min(static_cast<int>(u[used++] * r->n_outgoing), r->n_outgoing - 1);
             o = r->outgoing_active[k];
         } else {
-            o = std::min(static_cast<int>(u[used++] * na), na - 1);
+            // any mode of the table's material (RoughBoundarySampler::sample_diffuse :451-455)
+            const int lo = r->mode_count ? r->mode_begin : 0, cnt = r->mode_count ? r->mode_count : na;
+            o = lo + std::min(static_cast<int>(u[used++] * cnt), cnt - 1);
         }
     }
     *kind = 2;
@@ -1133,6 +1135,7 @@ struct orc_sim {
     pmc_geometry g;
     std::vector<pmc_facet> facets;
     std::vector<pmc_rough_table> rough;
+    std::vector<int32_t> layer_rough; // [layers * facets] table by (material, facet), empty: facets' own
     std::vector<pmc_reservoir> res;
     uint32_t k0 = 0, k1 = 0;
     int step = 0;
@@ -1456,9 +1459,14 @@ int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
             s.alive[i] = 0;
             return 1;
         }
+        // Layered materials reflect through the table of the carrier's material
+        // (RoughBoundarySampler::table_index, src/solver/TransportOperator.cpp:302).
+        const int rtab = s.layer_rough.empty()
+                             ? F.rough_table
+                             : s.layer_rough[static_cast<size_t>(layer_of_mode(s, a)) * s.facets.size() + facet];
         if (F.condition == 'P' && F.periodic_pair >= 0) {
             s.pos[i] = add(s.pos[i], V3{F.shift[0], F.shift[1], F.shift[2]});
-        } else if (F.rough_table >= 0 && (F.condition == 'R' || s.conservative)) {
+        } else if (rtab >= 0 && (F.condition == 'R' || s.conservative)) {
             // Conservative transport reflects every reflecting facet through its
             // table with signed occupations and needs an inward outgoing mode
             // (src/solver/BoundaryInteraction.cpp:90-113).
@@ -1468,7 +1476,7 @@ int advance(orc_sim& s, int64_t i, double remaining, StepTally& tally) {
             for (int k = 0; k < 4; ++k) u[k] = U.next();
             int out, kind;
             double o;
-            orc_rough_sample_signed(&s.m, &s.rough[F.rough_table], a, s.occ[i], u, &out, &o, &kind,
+            orc_rough_sample_signed(&s.m, &s.rough[rtab], a, s.occ[i], u, &out, &o, &kind,
                                     s.conservative ? 1 : 0);
             tally.rough_events++;
             if (kind == 1) tally.rough_spec++;
@@ -1809,6 +1817,10 @@ int orc_set_layers(orc_sim* s, const pmc_layers* L) {
     for (int k = 0; k < n; ++k) vmin = std::min(vmin, L->spatial_volume[k]);
     s->accept.resize(n);
     for (int k = 0; k < n; ++k) s->accept[k] = vmin / L->spatial_volume[k];
+    if (L->facet_rough)
+        s->layer_rough.assign(L->facet_rough, L->facet_rough + static_cast<size_t>(n) * s->facets.size());
+    else
+        s->layer_rough.clear();
     s->amm_fraction = L->amm_fraction;
     if (L->amm_fraction > 0.0) {
         s->amm_block.assign(L->amm_block, L->amm_block + 2 * (n - 1) + 1);

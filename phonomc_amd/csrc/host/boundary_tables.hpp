@@ -14,6 +14,7 @@ namespace pmc {
 struct RoughTable {
     int facet = -1;
     bool strict_elastic = false;
+    int mode_begin = 0, mode_count = 0; // a layered material's global active range (0 count: all modes)
     std::vector<double> specularity;      // per active mode
     std::vector<int> spec_match;          // per active mode, -1 = none
     std::vector<int> diffuse_begin, diffuse_end;

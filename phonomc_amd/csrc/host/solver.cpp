@@ -98,8 +98,6 @@ void build_layers(Problem& p) {
         if (f.condition == 'T' && std::abs(f.normal[axis]) < 0.99)
             throw std::runtime_error("In the layered-material model, thermal-reservoir facets must be normal to "
                                      "materials.interface_axis so each reservoir touches one material.");
-        if (f.condition == 'R')
-            throw std::runtime_error("Layered materials with rough or specular ('R') facets are not supported by this build.");
     }
     // Materials: equal temperature grids, one global bank.
     std::vector<std::shared_ptr<const MaterialEntry>> mats;
@@ -221,6 +219,56 @@ void build_layers(Problem& p) {
             L.amm_audit.push_back(t.audit);
         }
     }
+}
+
+// Reflection tables of a layered box: RoughBoundarySampler builds one table per
+// (material, facet) from that material's own modes (src/solver/RoughBoundarySampler.cpp:77-368,
+// table_index :68-74).  Each material's tables are moved into the global bank:
+// mode indices shifted by the material's first active mode, per-mode rows
+// widened to every active mode (rows of other materials' modes have no
+// partner or window; no carrier of theirs meets this table).
+void layered_rough_tables(Problem& p, const std::vector<bool>& enabled) {
+    auto& L = p.layers;
+    const Domain& d = p.domain;
+    const int nf = static_cast<int>(d.facets.size());
+    const int total = L.mode_begin.back();
+    p.rough.clear();
+    L.facet_rough.assign(static_cast<size_t>(L.count) * nf, -1);
+    for (int k = 0; k < L.count; ++k) {
+        std::vector<int> f2r;
+        auto tables = build_rough_tables(d, *p.layer_modes[k], enabled, 0.0, f2r);
+        const int b = L.mode_begin[k], n = L.mode_begin[k + 1] - b;
+        for (int f = 0; f < nf; ++f)
+            if (f2r[f] >= 0) L.facet_rough[static_cast<size_t>(k) * nf + f] = static_cast<int32_t>(p.rough.size()) + f2r[f];
+        for (auto& t : tables) {
+            RoughTable g;
+            g.facet = t.facet;
+            g.strict_elastic = t.strict_elastic;
+            g.mode_begin = b;
+            g.mode_count = n;
+            g.specularity.assign(total, 0.0);
+            g.spec_match.assign(total, -1);
+            g.diffuse_begin.assign(total, -1);
+            g.diffuse_end.assign(total, -1);
+            for (int a = 0; a < n; ++a) {
+                g.specularity[b + a] = t.specularity[a];
+                g.spec_match[b + a] = t.spec_match[a] < 0 ? -1 : b + t.spec_match[a];
+                g.diffuse_begin[b + a] = t.diffuse_begin[a];
+                g.diffuse_end[b + a] = t.diffuse_end[a];
+            }
+            for (int& x : t.outgoing_active) x += b;
+            for (int& x : t.outgoing_sorted) x += b;
+            for (int& x : t.roulette_active) x += b;
+            g.outgoing_active = std::move(t.outgoing_active);
+            g.outgoing_sorted = std::move(t.outgoing_sorted);
+            g.residual_prefix = std::move(t.residual_prefix);
+            g.roulette_active = std::move(t.roulette_active);
+            g.roulette_cdf = std::move(t.roulette_cdf);
+            p.rough.push_back(std::move(g));
+        }
+    }
+    // the facet's own entry names material 0's table
+    for (int f = 0; f < nf; ++f) p.facet_to_rough[f] = L.facet_rough[f];
 }
 
 // Reservoirs of a layered box: each reservoir facet touches one material
@@ -378,6 +426,7 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         enabled[f] = conservative ? reflecting : fc.condition == 'R';
     }
     p->rough = build_rough_tables(d, m, enabled, conservative ? c.background_temperature : 0.0, p->facet_to_rough);
+    if (layered) layered_rough_tables(*p, enabled);
     if (layered) layered_reservoirs(*p);
     else p->reservoirs = build_reservoir_tables(d, m, p->particle_density, c.time_step, p->facet_to_reservoir);
     if (linearized) {
@@ -475,6 +524,7 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         lv.amm_edge_begin = L.amm_fraction > 0 ? L.amm_edge_begin.data() : nullptr;
         lv.amm_edge_mode = L.amm_fraction > 0 && !L.amm_edge_mode.empty() ? L.amm_edge_mode.data() : nullptr;
         lv.amm_edge_prob = L.amm_fraction > 0 && !L.amm_edge_prob.empty() ? L.amm_edge_prob.data() : nullptr;
+        lv.facet_rough = L.facet_rough.empty() ? nullptr : L.facet_rough.data();
     }
     auto& g = p->geometry_view;
     g.kind = 0;
@@ -586,6 +636,8 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         v.roulette_active = r.roulette_active.data();
         v.roulette_cdf = r.roulette_cdf.data();
         v.strict_elastic = r.strict_elastic ? 1 : 0;
+        v.mode_begin = r.mode_begin;
+        v.mode_count = r.mode_count;
         p->rough_views.push_back(v);
     }
     double expected = 0;
@@ -615,9 +667,10 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     s.background_temperature = c.background_temperature;
     s.lifetime_local = c.lifetime_local ? 1 : 0;
     s.lifetime_temperature = c.lifetime_temperature;
-    // Fixed incoming flux under conservative transport, else one-to-one refill
-    // (src/solver/ReservoirInitialization.cpp:30-33).
-    s.refill_policy = conservative ? 1 : 0;
+    // Fixed incoming flux under conservative transport and for layered materials,
+    // else one-to-one refill (src/solver/ReservoirInitialization.cpp:30-33; the run
+    // summary's reservoir_gen names only the conservative case, Diagnostics.cpp:102-104).
+    s.refill_policy = conservative || layered ? 1 : 0;
     s.mode_sampling = opt.mode_sampling;
     s.temperature_inversion = 0;
     s.conservative_walls = conservative ? 1 : 0;

@@ -66,6 +66,8 @@ struct Problem {
         std::vector<int32_t> amm_block, amm_reflected, amm_edge_begin, amm_edge_mode;
         std::vector<double> amm_edge_prob;
         std::vector<AmmAudit> amm_audit;
+        // Reflection table of facet f for material k at k * n_facets + f (pmc_layers.facet_rough)
+        std::vector<int32_t> facet_rough;
     } layers;
     pmc_layers layers_view{};
     std::vector<const ModeTables*> layer_modes;            // per material (layered problems)

@@ -48,6 +48,7 @@ struct pmc_ctx {
     pmc_heat_source hs{};      // scalars of the time profile (pointers not kept)
     int error_interval = 8;   // steps between reads of the device error word
     int n_reservoirs = 0;
+    std::vector<std::pair<int, int>> rough_range; // (mode_begin, mode_count) of every uploaded reflection table
     double expected_injection = 0;
     bool timing = false;
     hipError_t hip_error = hipSuccess; // first failure of an unchecked-in-place HIP call
@@ -611,7 +612,13 @@ int pmc_upload_boundary_tables(pmc_ctx* ctx, int32_t n_rough, const pmc_rough_ta
         rd[r].roul_active = ctx->upload(t.roulette_active, t.n_roulette);
         rd[r].roul_cdf = ctx->upload(t.roulette_cdf, t.n_roulette);
         rd[r].strict = t.strict_elastic;
+        if (t.mode_count < 0 || t.mode_begin < 0 || t.mode_begin + t.mode_count > na)
+            return fail(ctx, PMC_ERR_ARGUMENT, "reflection table mode range outside the active modes");
+        rd[r].mode_lo = t.mode_count ? t.mode_begin : 0;
+        rd[r].mode_n = t.mode_count ? t.mode_count : na;
     }
+    ctx->rough_range.clear();
+    for (int r = 0; r < n_rough; ++r) ctx->rough_range.push_back({rough[r].mode_begin, rough[r].mode_count});
     v.rough = ctx->upload(rd.data(), rd.size());
     std::vector<ResDev> dd(std::max(1, n_res));
     double expected = 0;
@@ -1193,6 +1200,20 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
             starts[static_cast<size_t>(b) * (nbands + 1) + j] =
                 static_cast<int32_t>(std::lower_bound(om, om + cnt, static_cast<double>(j) * L->band_width) - om);
     }
+    // Reflection tables by material: each must cover exactly its material's modes.
+    std::vector<int32_t> fr;
+    if (L->facet_rough) {
+        fr.assign(L->facet_rough, L->facet_rough + static_cast<size_t>(n) * v.nfacets);
+        for (int k = 0; k < n; ++k)
+            for (int f = 0; f < v.nfacets; ++f) {
+                const int t = fr[static_cast<size_t>(k) * v.nfacets + f];
+                if (t < -1 || t >= static_cast<int>(ctx->rough_range.size()))
+                    return fail(ctx, PMC_ERR_ARGUMENT, "layer reflection table out of range");
+                if (t >= 0 && (ctx->rough_range[t].first != L->mode_begin[k] ||
+                               ctx->rough_range[t].second != L->mode_begin[k + 1] - L->mode_begin[k]))
+                    return fail(ctx, PMC_ERR_ARGUMENT, "layer reflection table does not cover its material's modes");
+            }
+    }
     v.layer_axis = L->axis;
     for (int k = 0; k + 1 < n; ++k) v.layer_plane[k] = static_cast<float>(L->planes[k]);
     for (int k = 0; k <= n; ++k) {
@@ -1206,6 +1227,7 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     for (int b = 0; b <= 2 * n; ++b) v.bank_begin[b] = L->bank_begin[b];
     v.band = L->band_width;
     v.cell_mat = ctx->upload(cm.data(), cm.size());
+    v.layer_rough = fr.empty() ? nullptr : ctx->upload(fr.data(), fr.size());
     v.bank_mode = ctx->upload(L->bank_mode, static_cast<size_t>(nb));
     v.bank_omega = ctx->upload(L->bank_omega, static_cast<size_t>(nb));
     v.bank_prefix = ctx->upload(L->bank_prefix, static_cast<size_t>(nb) + 2 * n);

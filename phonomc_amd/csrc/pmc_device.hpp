@@ -118,6 +118,7 @@ struct RoughDev {
     const int* roul_active;
     const double* roul_cdf;
     int strict;
+    int mode_lo, mode_n; // active modes of the table's material (last-resort draw)
 };
 
 struct ResDev {
@@ -243,6 +244,7 @@ struct View {
     float layer_plane[kMaxLayers - 1];
     int mode_begin[kMaxLayers + 1];
     const int* cell_mat;
+    const int* layer_rough; // [layers * nfacets] reflection table by (material, facet); null: facets' own
     double e_bg_mat[kMaxLayers];
     int lut_begin[kMaxLayers + 1];
     double layer_accept[kMaxLayers]; // initial positions: smallest spatial volume / this material's

@@ -602,8 +602,8 @@ __device__ void rough_sample(const View& V, const RoughDev& R, int in_ai, double
             int k = static_cast<int>(U.next() * R.n_out);
             o = R.out_active[k < R.n_out ? k : R.n_out - 1];
         } else {
-            int k = static_cast<int>(U.next() * V.na);
-            o = k < V.na ? k : V.na - 1;
+            int k = static_cast<int>(U.next() * R.mode_n);
+            o = R.mode_lo + (k < R.mode_n ? k : R.mode_n - 1);
         }
     }
     kind = 2;
@@ -815,8 +815,11 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             continue;
         }
         // Conservative transport reflects through the table on every reflecting
-        // facet; otherwise only rough facets have one.
-        if (F.rough >= 0 && (F.cond == 'R' || V.conservative)) {
+        // facet; otherwise only rough facets have one.  Layered materials keep one
+        // table per (material, facet).
+        int rtab = F.rough;
+        if (kGeom == kGeomLayered && V.layer_rough) rtab = V.layer_rough[mode_material(V, m) * V.nfacets + facet];
+        if (rtab >= 0 && (F.cond == 'R' || V.conservative)) {
             if (ev == 0 || U.uid != uid) {
                 U.uid = uid;
                 U.step = step;
@@ -827,7 +830,7 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
             U.used = 0;
             int out_ai, kind;
             double out_occ;
-            rough_sample(V, V.rough[F.rough], m, occ, U, out_ai, out_occ, kind, V.conservative != 0);
+            rough_sample(V, V.rough[rtab], m, occ, U, out_ai, out_occ, kind, V.conservative != 0);
             T.rough_events++;
             if (kind == 1) T.rough_spec++;
             else if (kind == 2) T.rough_diff++;

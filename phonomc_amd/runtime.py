@@ -73,7 +73,8 @@ class RoughTable(C.Structure):
                 ("n_outgoing", C.c_int32), ("outgoing_active", C.POINTER(C.c_int32)),
                 ("outgoing_sorted", C.POINTER(C.c_int32)), ("residual_prefix", C.POINTER(C.c_double)),
                 ("n_roulette", C.c_int32), ("roulette_active", C.POINTER(C.c_int32)),
-                ("roulette_cdf", C.POINTER(C.c_double)), ("strict_elastic", C.c_int32)]
+                ("roulette_cdf", C.POINTER(C.c_double)), ("strict_elastic", C.c_int32),
+                ("mode_begin", C.c_int32), ("mode_count", C.c_int32)]
 
 
 class Reservoir(C.Structure):
@@ -113,7 +114,8 @@ class Layers(C.Structure):
                 ("bank_omega", C.POINTER(C.c_double)), ("bank_prefix", C.POINTER(C.c_double)),
                 ("amm_fraction", C.c_double), ("amm_block", C.POINTER(C.c_int32)),
                 ("amm_reflected", C.POINTER(C.c_int32)), ("amm_edge_begin", C.POINTER(C.c_int32)),
-                ("amm_edge_mode", C.POINTER(C.c_int32)), ("amm_edge_prob", C.POINTER(C.c_double))]
+                ("amm_edge_mode", C.POINTER(C.c_int32)), ("amm_edge_prob", C.POINTER(C.c_double)),
+                ("facet_rough", C.POINTER(C.c_int32))]
 
 
 class LaunchConfig(C.Structure):

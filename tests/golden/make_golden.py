@@ -11,6 +11,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --interfaces [--runs]          (acoustic and mixed mismatch interfaces)
     python tests/golden/make_golden.py --spectra [--runs]             (weighted and Gaussian source spectra)
     python tests/golden/make_golden.py --oblique [--runs]             (tapered fin: rough walls not axis-aligned)
+    python tests/golden/make_golden.py --layered-walls [--runs]       (layered ribbon with rough side walls)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -39,7 +40,8 @@ WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "
              "c1cs": "c1cs_callaway_source.toml", "c1l": "c1l_layered.toml",
              "finfet_cw": "finfet_callaway.toml", "c1cn": "c1cn_nonlinear.toml", "c2cn": "c2cn_nonlinear.toml",
              "c2r": "c2r_conservative.toml", "c1la": "c1la_amm.toml", "c1lm": "c1lm_mmm.toml",
-             "c1sw": "c1sw_weighted.toml", "c1sg": "c1sg_gaussian.toml", "taper": "taper_fin.toml"}
+             "c1sw": "c1sw_weighted.toml", "c1sg": "c1sg_gaussian.toml", "taper": "taper_fin.toml",
+             "c1lr": "c1lr_rough.toml"}
 
 
 def write_input(text, folder, name):
@@ -585,7 +587,41 @@ def output_files():
                 summary=np.array(summary), **domain, **{k: np.array(v) for k, v in o.items()})
 
 
+def layered_rough_vectors(problem, rd, rng, facets, n=3000):
+    """Reflections on the rough side walls of a layered box, per material: the
+    reference's sampler built for each material alone (its tables are the
+    per-material tables of the layered run, RoughBoundarySampler.cpp:108-368),
+    inputs and outputs in that material's local active indices."""
+    out = {}
+    for k in range(problem.layers().count):
+        rm = R.RefMaterial(problem.layer_material(k))
+        na = lib_count(rm)
+        for facet in facets:
+            ins = rng.integers(0, na, n).astype(np.int32)
+            occ = rng.uniform(0.0, 3.0, n)
+            seeds = rng.integers(1, 2 ** 62, n, dtype=np.uint64)
+            oa, oo, u = rd.rough_sample(rm, facet, 300.0, ins, occ, seeds)
+            out[f"m{k}_f{facet}_in"] = ins
+            out[f"m{k}_f{facet}_occ"] = occ
+            out[f"m{k}_f{facet}_u"] = u
+            out[f"m{k}_f{facet}_out"] = oa
+            out[f"m{k}_f{facet}_out_occ"] = oo
+        rm.close()
+    return out
+
+
 def main():
+    if "--layered-walls" in sys.argv:
+        rng = np.random.default_rng(20261214)
+        text = rt.config_text(WORKLOADS["c1lr"])
+        problem = rt.Problem(text, input_directory=rt.CONFIGS, mode_sampling=1)
+        rd = R.RefDomain(write_input(text, tempfile.mkdtemp(), "c1lr.toml"))
+        np.savez_compressed(os.path.join(OUT, "rough_c1lr.npz"), **layered_rough_vectors(problem, rd, rng, [0, 1]))
+        if "--runs" in sys.argv:
+            np.savez_compressed(os.path.join(OUT, "run_c1lr.npz"),
+                                **whole_run("c1lr", 20000, 400, list(range(1000, 1016)), 10))
+        print("layered rough-wall fixtures written to", OUT)
+        return
     if "--oblique" in sys.argv:
         rng = np.random.default_rng(20261213)
         write_tapered_fin_stl(os.path.join(rt.CONFIGS, "taper_fin.stl"))
