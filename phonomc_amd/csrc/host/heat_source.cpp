@@ -12,7 +12,7 @@ constexpr double kEvPerWattPs = 6.241509074e6;
 constexpr double kTwoPi = 6.283185307179586476925286766559;
 } // namespace
 
-HeatSource build_heat_source(const SimConfig& c, const Domain& d, const ModeTables& m) {
+HeatSource build_heat_source(const SimConfig& c, const Domain& d, const std::vector<const ModeTables*>& materials) {
     HeatSource hs;
     if (!c.heat_source_enabled) return hs;
     const int ncell = d.cell_count();
@@ -103,48 +103,57 @@ HeatSource build_heat_source(const SimConfig& c, const Domain& d, const ModeTabl
     // PhononSourceOperator::configure (:20-103): branch indices and weights per
     // material, weights over the largest; the Gaussian's log weights over the
     // positive-frequency bank.
-    for (int b : c.heat_branches)
-        if (b >= m.branches) throw std::invalid_argument("heat_source.branches index exceeds a material's branch count.");
     std::vector<double> branch_weight = c.heat_branch_weights;
     if (!branch_weight.empty()) {
         const double scale = *std::max_element(branch_weight.begin(), branch_weight.end());
         for (double& w : branch_weight) w /= scale;
     }
-    if (c.heat_spectrum == 1 && branch_weight.size() != static_cast<size_t>(m.branches))
-        throw std::invalid_argument("heat_source.branch_weights must match each material's branch count.");
     hs.spectrum = c.heat_spectrum;
-    hs.mode_eligible.assign(m.n_active, 1);
-    hs.mode_weight.assign(m.n_active, 0.0);
-    for (int a = 0; a < m.n_active; ++a) {
-        const int f = m.active_flat[a];
-        const double thz = m.omega_flat[f] / kTwoPi;
-        bool ok = std::isfinite(thz) && !(c.heat_frequency_min >= 0.0 && thz < c.heat_frequency_min) &&
-                  !(c.heat_frequency_max >= 0.0 && thz > c.heat_frequency_max);
-        if (ok && !c.heat_branches.empty())
-            ok = std::find(c.heat_branches.begin(), c.heat_branches.end(), f % m.branches) != c.heat_branches.end();
-        double weight = ok ? 1.0 : 0.0;
-        if (ok && c.heat_spectrum == 1) weight = branch_weight[static_cast<size_t>(f % m.branches)];
-        hs.mode_eligible[a] = weight > 0.0 ? 1 : 0;
-        hs.mode_weight[a] = c.heat_spectrum == 1 ? weight : 0.0;
-    }
     if (c.heat_spectrum == 2) {
-        size_t positive = 0;
-        for (double w : m.omega_flat)
-            if (w > 0) ++positive;
-        if (positive != static_cast<size_t>(m.n_active))
-            throw std::invalid_argument("Gaussian source requires all positive-frequency modes, including stationary modes.");
         hs.frequency_center = *c.heat_frequency_center;
         hs.frequency_sigma = *c.heat_frequency_sigma;
-        bool any = false;
+    }
+    hs.mode_eligible.clear();
+    hs.mode_weight.clear();
+    // One material after the other, in the order of the global active bank.
+    for (const ModeTables* mp : materials) {
+        const ModeTables& m = *mp;
+        for (int b : c.heat_branches)
+            if (b >= m.branches) throw std::invalid_argument("heat_source.branches index exceeds a material's branch count.");
+        if (c.heat_spectrum == 1 && branch_weight.size() != static_cast<size_t>(m.branches))
+            throw std::invalid_argument("heat_source.branch_weights must match each material's branch count.");
+        const size_t base = hs.mode_eligible.size();
+        hs.mode_eligible.resize(base + m.n_active, 1);
+        hs.mode_weight.resize(base + m.n_active, 0.0);
         for (int a = 0; a < m.n_active; ++a) {
-            if (!hs.mode_eligible[a]) continue;
-            const long double f = m.omega_flat[m.active_flat[a]] / 6.283185307179586476925286766559L;
-            const long double z = (f - hs.frequency_center) / hs.frequency_sigma;
-            hs.mode_weight[a] = static_cast<double>(-0.5L * z * z);
-            any = true;
+            const int f = m.active_flat[a];
+            const double thz = m.omega_flat[f] / kTwoPi;
+            bool ok = std::isfinite(thz) && !(c.heat_frequency_min >= 0.0 && thz < c.heat_frequency_min) &&
+                      !(c.heat_frequency_max >= 0.0 && thz > c.heat_frequency_max);
+            if (ok && !c.heat_branches.empty())
+                ok = std::find(c.heat_branches.begin(), c.heat_branches.end(), f % m.branches) != c.heat_branches.end();
+            double weight = ok ? 1.0 : 0.0;
+            if (ok && c.heat_spectrum == 1) weight = branch_weight[static_cast<size_t>(f % m.branches)];
+            hs.mode_eligible[base + a] = weight > 0.0 ? 1 : 0;
+            hs.mode_weight[base + a] = c.heat_spectrum == 1 ? weight : 0.0;
         }
-        if (!any)
-            throw std::invalid_argument("Gaussian source branch/frequency filters select no positive-frequency modes.");
+        if (c.heat_spectrum == 2) {
+            size_t positive = 0;
+            for (double w : m.omega_flat)
+                if (w > 0) ++positive;
+            if (positive != static_cast<size_t>(m.n_active))
+                throw std::invalid_argument("Gaussian source requires all positive-frequency modes, including stationary modes.");
+            bool any = false;
+            for (int a = 0; a < m.n_active; ++a) {
+                if (!hs.mode_eligible[base + a]) continue;
+                const long double f = m.omega_flat[m.active_flat[a]] / 6.283185307179586476925286766559L;
+                const long double z = (f - hs.frequency_center) / hs.frequency_sigma;
+                hs.mode_weight[base + a] = static_cast<double>(-0.5L * z * z);
+                any = true;
+            }
+            if (!any)
+                throw std::invalid_argument("Gaussian source branch/frequency filters select no positive-frequency modes.");
+        }
     }
     hs.enabled = true;
     hs.selected_cells = selected;
@@ -176,4 +185,10 @@ double HeatSource::integrated_time_factor(double t0, double t1) const {
 
 double HeatSource::base_power_w() const { return std::accumulate(rates.begin(), rates.end(), 0.0) / kEvPerWattPs; }
 
+} // namespace pmc
+
+namespace pmc {
+HeatSource build_heat_source(const SimConfig& c, const Domain& d, const ModeTables& m) {
+    return build_heat_source(c, d, std::vector<const ModeTables*>{&m});
+}
 } // namespace pmc

@@ -30,5 +30,9 @@ struct HeatSource {
 };
 
 HeatSource build_heat_source(const SimConfig& c, const Domain& d, const ModeTables& m);
+// Layered materials: every material's per-mode arrays, concatenated in the order
+// of the global active bank, each material checked as PhononSourceOperator::configure
+// checks it (src/solver/PhononSourceOperator.cpp:41-101).
+HeatSource build_heat_source(const SimConfig& c, const Domain& d, const std::vector<const ModeTables*>& materials);
 
 } // namespace pmc

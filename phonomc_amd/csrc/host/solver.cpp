@@ -102,7 +102,8 @@ void build_layers(Problem& p) {
     // Materials: equal temperature grids, one global bank.
     std::vector<std::shared_ptr<const MaterialEntry>> mats;
     for (const auto& f : c.material_folders)
-        mats.push_back(cached_material(f, c.input_directory, c.temperature_lookup_dt, false));
+        mats.push_back(cached_material(f, c.input_directory, c.temperature_lookup_dt,
+                                       c.heat_source_enabled && c.heat_spectrum == 2));
     for (const auto& e : mats)
         if (e->modes.temps != mats[0]->modes.temps)
             throw std::runtime_error("Layered materials need equal lifetime temperature grids in this build.");
@@ -324,7 +325,6 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
             throw std::runtime_error("Layered materials with scattering.model = 'callaway' are not supported by this build.");
         if (conservative)
             throw std::runtime_error("Layered materials with simulation.conservative_boundaries are not supported by this build.");
-        if (c.heat_source_enabled) throw std::runtime_error("Layered materials with a heat source are not supported by this build.");
         if (!std::isfinite(c.interface_bin_thz) || c.interface_bin_thz <= 0)
             throw std::runtime_error("Interface frequency bin must be finite and positive.");
     }
@@ -443,7 +443,7 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     }
 
     // Prescribed heat source (src/solver/HeatSourceDriving.cpp:8-11).
-    p->heat = build_heat_source(c, d, m);
+    p->heat = layered ? build_heat_source(c, d, p->layer_modes) : build_heat_source(c, d, m);
     if (p->heat.enabled) {
         auto& h = p->heat_view;
         h.n_cells = d.cell_count();

@@ -12,6 +12,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --spectra [--runs]             (weighted and Gaussian source spectra)
     python tests/golden/make_golden.py --oblique [--runs]             (tapered fin: rough walls not axis-aligned)
     python tests/golden/make_golden.py --layered-walls [--runs]       (layered ribbon with rough side walls)
+    python tests/golden/make_golden.py --layered-source               (heated layered film, whole runs)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -41,7 +42,7 @@ WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "
              "finfet_cw": "finfet_callaway.toml", "c1cn": "c1cn_nonlinear.toml", "c2cn": "c2cn_nonlinear.toml",
              "c2r": "c2r_conservative.toml", "c1la": "c1la_amm.toml", "c1lm": "c1lm_mmm.toml",
              "c1sw": "c1sw_weighted.toml", "c1sg": "c1sg_gaussian.toml", "taper": "taper_fin.toml",
-             "c1lr": "c1lr_rough.toml"}
+             "c1lr": "c1lr_rough.toml", "c1ls": "c1ls_source.toml"}
 
 
 def write_input(text, folder, name):
@@ -611,6 +612,11 @@ def layered_rough_vectors(problem, rd, rng, facets, n=3000):
 
 
 def main():
+    if "--layered-source" in sys.argv:
+        np.savez_compressed(os.path.join(OUT, "run_c1ls.npz"),
+                            **whole_run("c1ls", 20000, 400, list(range(1000, 1016)), 10))
+        print("heated layered film runs written to", OUT)
+        return
     if "--layered-walls" in sys.argv:
         rng = np.random.default_rng(20261214)
         text = rt.config_text(WORKLOADS["c1lr"])
