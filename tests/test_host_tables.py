@@ -154,13 +154,9 @@ def test_unsupported_features_fail_loudly():
     # Callaway on a table without an N/U split: the reference's callaway_data message.
     with pytest.raises(RuntimeError, match="Callaway requires gamma_N and gamma_U"):
         rt.Problem(rt.config_text("c1_cross_plane.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))
-    # Layered materials take the relaxation-time model, DMM interfaces and no heat source.
-    with pytest.raises(RuntimeError, match="Layered materials with a heat source"):
-        rt.Problem(rt.config_text("c1l_layered.toml", **{"heat_source.enabled": "true",
-                                                          "heat_source.power_density": "1e18",
-                                                          "heat_source.profile": '"gaussian"',
-                                                          "heat_source.center": "[50, 50, 50]",
-                                                          "heat_source.sigma": "[10, 10, 10]"}))
+    # Layered films take the relaxation-time model; conservative walls stay refused.
+    with pytest.raises(RuntimeError, match="Layered materials with simulation.conservative_boundaries"):
+        rt.Problem(rt.config_text("c1l_layered.toml", **{"simulation.conservative_boundaries": "true"}))
     # Layered films run DMM, AMM and MMM interfaces, but not Callaway.
     with pytest.raises(RuntimeError, match="Layered materials with scattering.model = 'callaway'"):
         rt.Problem(rt.config_text("c1l_layered.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))
