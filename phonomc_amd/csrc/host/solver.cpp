@@ -103,7 +103,7 @@ void build_layers(Problem& p) {
     std::vector<std::shared_ptr<const MaterialEntry>> mats;
     for (const auto& f : c.material_folders)
         mats.push_back(cached_material(f, c.input_directory, c.temperature_lookup_dt,
-                                       c.heat_source_enabled && c.heat_spectrum == 2));
+                                       c.conservative_boundaries || (c.heat_source_enabled && c.heat_spectrum == 2)));
     for (const auto& e : mats)
         if (e->modes.temps != mats[0]->modes.temps)
             throw std::runtime_error("Layered materials need equal lifetime temperature grids in this build.");
@@ -228,7 +228,7 @@ void build_layers(Problem& p) {
 // mode indices shifted by the material's first active mode, per-mode rows
 // widened to every active mode (rows of other materials' modes have no
 // partner or window; no carrier of theirs meets this table).
-void layered_rough_tables(Problem& p, const std::vector<bool>& enabled) {
+void layered_rough_tables(Problem& p, const std::vector<bool>& enabled, double strict_temperature) {
     auto& L = p.layers;
     const Domain& d = p.domain;
     const int nf = static_cast<int>(d.facets.size());
@@ -237,7 +237,7 @@ void layered_rough_tables(Problem& p, const std::vector<bool>& enabled) {
     L.facet_rough.assign(static_cast<size_t>(L.count) * nf, -1);
     for (int k = 0; k < L.count; ++k) {
         std::vector<int> f2r;
-        auto tables = build_rough_tables(d, *p.layer_modes[k], enabled, 0.0, f2r);
+        auto tables = build_rough_tables(d, *p.layer_modes[k], enabled, strict_temperature, f2r);
         const int b = L.mode_begin[k], n = L.mode_begin[k + 1] - b;
         for (int f = 0; f < nf; ++f)
             if (f2r[f] >= 0) L.facet_rough[static_cast<size_t>(k) * nf + f] = static_cast<int32_t>(p.rough.size()) + f2r[f];
@@ -323,8 +323,6 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     if (layered) {
         if (c.collision == CollisionModel::Callaway)
             throw std::runtime_error("Layered materials with scattering.model = 'callaway' are not supported by this build.");
-        if (conservative)
-            throw std::runtime_error("Layered materials with simulation.conservative_boundaries are not supported by this build.");
         if (!std::isfinite(c.interface_bin_thz) || c.interface_bin_thz <= 0)
             throw std::runtime_error("Interface frequency bin must be finite and positive.");
     }
@@ -426,7 +424,7 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         enabled[f] = conservative ? reflecting : fc.condition == 'R';
     }
     p->rough = build_rough_tables(d, m, enabled, conservative ? c.background_temperature : 0.0, p->facet_to_rough);
-    if (layered) layered_rough_tables(*p, enabled);
+    if (layered) layered_rough_tables(*p, enabled, conservative ? c.background_temperature : 0.0);
     if (layered) layered_reservoirs(*p);
     else p->reservoirs = build_reservoir_tables(d, m, p->particle_density, c.time_step, p->facet_to_reservoir);
     if (linearized) {

@@ -789,7 +789,7 @@ int pmc_upload_callaway_nonlinear(pmc_ctx* ctx, const pmc_callaway_rates* r) {
     if (ctx->view.inversion == 1)
         return fail(ctx, PMC_ERR_UNSUPPORTED, "Newton temperature inversion applies to RTA transport");
     if (ctx->view.layers > 1)
-        return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run RTA without conservative walls");
+        return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run relaxation-time scattering");
     View& v = ctx->view;
     const size_t na = static_cast<size_t>(v.na), nt = static_cast<size_t>(r->n_temps);
     for (size_t k = 0; k < nt; ++k)
@@ -1151,8 +1151,8 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     View& v = ctx->view;
     if (!ctx->has_material || !ctx->has_geometry) return fail(ctx, PMC_ERR_ARGUMENT, "layers need material and geometry");
     if (ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "layers must be uploaded before the population is created");
-    if (v.callaway || v.nl_callaway || v.conservative)
-        return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run RTA without conservative walls");
+    if (v.callaway || v.nl_callaway)
+        return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run relaxation-time scattering");
     if (v.inversion != 0)
         return fail(ctx, PMC_ERR_UNSUPPORTED, "Newton temperature inversion covers one material; layers use their tables");
     if (v.geom_kind != 0) return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials need the box geometry");

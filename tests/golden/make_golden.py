@@ -13,6 +13,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --oblique [--runs]             (tapered fin: rough walls not axis-aligned)
     python tests/golden/make_golden.py --layered-walls [--runs]       (layered ribbon with rough side walls)
     python tests/golden/make_golden.py --layered-source               (heated layered film, whole runs)
+    python tests/golden/make_golden.py --layered-strict [--runs]      (layered ribbon, conservative walls)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -42,7 +43,7 @@ WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "
              "finfet_cw": "finfet_callaway.toml", "c1cn": "c1cn_nonlinear.toml", "c2cn": "c2cn_nonlinear.toml",
              "c2r": "c2r_conservative.toml", "c1la": "c1la_amm.toml", "c1lm": "c1lm_mmm.toml",
              "c1sw": "c1sw_weighted.toml", "c1sg": "c1sg_gaussian.toml", "taper": "taper_fin.toml",
-             "c1lr": "c1lr_rough.toml", "c1ls": "c1ls_source.toml"}
+             "c1lr": "c1lr_rough.toml", "c1ls": "c1ls_source.toml", "c1lc": "c1lc_conservative.toml"}
 
 
 def write_input(text, folder, name):
@@ -611,7 +612,56 @@ def layered_rough_vectors(problem, rd, rng, facets, n=3000):
     return out
 
 
+def layered_strict_vectors(problem, rd, rng, facets, n=1500):
+    """Strict-elastic reflections of conservative transport on the side walls of a
+    layered box, per material: the reference's sampler built for each material
+    alone with the background temperature (stationary modes in the bank), signed
+    weights.  Nine in ten inputs are modes incident on the wall, with signed
+    occupations (some negative); the rest are any active mode."""
+    out = {}
+    s, m, *_ = problem.views()
+    t_bg = s.background_temperature
+    lv = problem.layers()
+    mb = rt.arr(lv.mode_begin, lv.count + 1, np.int32)
+    vel = rt.arr(m.velocity, 3 * m.n_active).reshape(m.n_active, 3)
+    neq = rt.arr(m.neq_background, m.n_active)
+    normals = problem.facets()["normals"]
+    for k in range(lv.count):
+        rm = R.RefMaterial(problem.layer_material(k), include_stationary=True)
+        na = lib_count(rm)
+        assert na == mb[k + 1] - mb[k]
+        v, n0 = vel[mb[k]:mb[k + 1]], neq[mb[k]:mb[k + 1]]
+        for facet in facets:
+            incident = np.nonzero(v @ normals[facet] > 0.0)[0]
+            ins = rng.choice(incident, n).astype(np.int32)
+            anyk = rng.random(n) < 0.1
+            ins[anyk] = rng.integers(0, na, int(anyk.sum()))
+            occ = n0[ins] + (n0[ins] + 0.05) * rng.normal(0.0, 0.3, n)
+            occ[rng.random(n) < 0.05] *= -1.0
+            seeds = rng.integers(1, 2 ** 62, n, dtype=np.uint64)
+            oa, oo, u = rd.rough_sample_strict(rm, facet, t_bg, ins, occ, seeds)
+            out[f"m{k}_f{facet}_in"] = ins
+            out[f"m{k}_f{facet}_occ"] = occ
+            out[f"m{k}_f{facet}_u"] = u
+            out[f"m{k}_f{facet}_out"] = oa
+            out[f"m{k}_f{facet}_out_occ"] = oo
+        rm.close()
+    out["t_bg"] = t_bg
+    return out
+
+
 def main():
+    if "--layered-strict" in sys.argv:
+        rng = np.random.default_rng(20261215)
+        text = rt.config_text(WORKLOADS["c1lc"])
+        problem = rt.Problem(text, input_directory=rt.CONFIGS, mode_sampling=1)
+        rd = R.RefDomain(write_input(text, tempfile.mkdtemp(), "c1lc.toml"))
+        np.savez_compressed(os.path.join(OUT, "rough_c1lc.npz"), **layered_strict_vectors(problem, rd, rng, [0, 1]))
+        if "--runs" in sys.argv:
+            np.savez_compressed(os.path.join(OUT, "run_c1lc.npz"),
+                                **whole_run("c1lc", 20000, 400, list(range(1000, 1016)), 10))
+        print("layered strict-wall fixtures written to", OUT)
+        return
     if "--layered-source" in sys.argv:
         np.savez_compressed(os.path.join(OUT, "run_c1ls.npz"),
                             **whole_run("c1ls", 20000, 400, list(range(1000, 1016)), 10))

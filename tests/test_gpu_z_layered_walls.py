@@ -1,5 +1,6 @@
-"""Rough side walls of a layered box on the device (c1lr_rough.toml): each
-material reflects through its own table (pmc_layers.facet_rough).
+"""Rough side walls of a layered box on the device (c1lr_rough.toml, and
+c1lc_conservative.toml with strict-elastic conservative walls): each material
+reflects through its own table (pmc_layers.facet_rough).
 
 - Carrier by carrier against the oracle over 3 steps, both mode-sampling modes:
   identical carrier sets, modes and cells, occupations to 1e-12, rough and
@@ -20,7 +21,7 @@ from test_gpu_statistics import WORKLOADS, gpu_runs, z_score
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 MASK = (1 << 28) - 1
-WORKLOADS.update({"c1lr": "c1lr_rough.toml"})
+WORKLOADS.update({"c1lr": "c1lr_rough.toml", "c1lc": "c1lc_conservative.toml"})
 
 
 def by_uid(c):
@@ -29,10 +30,11 @@ def by_uid(c):
     return {k: v[live][order] for k, v in c.items()}
 
 
-@pytest.mark.parametrize("sampling", [1, 0])
-def test_layered_rough_walls_step_like_the_oracle(sampling):
-    print(f"case c1lr_rough sampling={sampling} n=20000", flush=True)
-    p = rt.Problem(rt.config_text("c1lr_rough.toml", **{"simulation.particle_count": 20000}),
+@pytest.mark.parametrize("cfg,sampling", [("c1lr_rough.toml", 1), ("c1lr_rough.toml", 0),
+                                          ("c1lc_conservative.toml", 1)])
+def test_layered_rough_walls_step_like_the_oracle(cfg, sampling):
+    print(f"case {cfg} sampling={sampling} n=20000", flush=True)
+    p = rt.Problem(rt.config_text(cfg, **{"simulation.particle_count": 20000}),
                    input_directory=rt.CONFIGS, mode_sampling=sampling)
     g = rt.Context(p)
     o = O.Sim(p)
@@ -53,14 +55,15 @@ def test_layered_rough_walls_step_like_the_oracle(sampling):
     assert g.ledger()["rough_events"] > 0 and g.ledger()["interface_events"] > 0
 
 
-def test_layered_rough_walls_whole_runs_match_reference():
-    golden = np.load(os.path.join(HERE, "golden", "run_c1lr.npz"))
-    print(f"case run_c1lr n={int(golden['n'])} steps={int(golden['steps'])}", flush=True)
-    temps, keff, kint, iface = gpu_runs("c1lr", golden)
+@pytest.mark.parametrize("key", ["c1lr", "c1lc"])
+def test_layered_rough_walls_whole_runs_match_reference(key):
+    golden = np.load(os.path.join(HERE, "golden", f"run_{key}.npz"))
+    print(f"case run_{key} n={int(golden['n'])} steps={int(golden['steps'])}", flush=True)
+    temps, keff, kint, iface = gpu_runs(key, golden)
     zt = z_score(temps, golden["temperature"])
     zke, zki = z_score(keff, golden["kappa_eff"]), z_score(kint, golden["kappa_int"])
     zi = z_score(iface.astype(float), golden["interface"].astype(float))
-    print(f"c1lr: cell T max|z| {np.abs(zt).max():.2f}, kappa_eff z {zke:+.2f}, kappa_int z {zki:+.2f}, "
+    print(f"{key}: cell T max|z| {np.abs(zt).max():.2f}, kappa_eff z {zke:+.2f}, kappa_int z {zki:+.2f}, "
           f"interface z {zi}")
     assert np.abs(zt).max() <= 4.5 and np.mean(zt ** 2) <= 2.5
     assert abs(zke) <= 3.0 and abs(zki) <= 3.0

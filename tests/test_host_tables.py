@@ -154,9 +154,8 @@ def test_unsupported_features_fail_loudly():
     # Callaway on a table without an N/U split: the reference's callaway_data message.
     with pytest.raises(RuntimeError, match="Callaway requires gamma_N and gamma_U"):
         rt.Problem(rt.config_text("c1_cross_plane.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))
-    # Layered films take the relaxation-time model; conservative walls stay refused.
-    with pytest.raises(RuntimeError, match="Layered materials with simulation.conservative_boundaries"):
-        rt.Problem(rt.config_text("c1l_layered.toml", **{"simulation.conservative_boundaries": "true"}))
+    # Layered films take relaxation-time scattering, with or without conservative walls.
+    rt.Problem(rt.config_text("c1l_layered.toml", **{"simulation.conservative_boundaries": "true"}))
     # Layered films run DMM, AMM and MMM interfaces, but not Callaway.
     with pytest.raises(RuntimeError, match="Layered materials with scattering.model = 'callaway'"):
         rt.Problem(rt.config_text("c1l_layered.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))

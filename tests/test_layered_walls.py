@@ -17,6 +17,11 @@ carrier's material at src/solver/TransportOperator.cpp:302).
   test_oracle_statistics.
 - Planted mistake: Ge carriers reflected through the Si table leave their
   material.
+- Conservative walls (c1lc_conservative.toml: the same ribbon with
+  simulation.conservative_boundaries): each material's strict-elastic tables
+  (stationary modes in the bank) reproduce the reference's signed reflections
+  bit for bit (rough_c1lc.npz, make_golden.py --layered-strict), and the
+  oracle's whole runs agree with 16 reference runs (run_c1lc.npz).
 """
 import concurrent.futures
 import os
@@ -106,6 +111,48 @@ def test_oracle_runs_match_reference():
     z = z_score(temps, golden["temperature"])
     zi = z_score(iface, golden["interface"].astype(float))
     print(f"c1lr oracle: cell T max|z| {np.abs(z).max():.2f}, mean z^2 {np.mean(z ** 2):.2f}; interface events "
+          f"z {zi[0]:+.2f}, transmitted z {zi[1]:+.2f}")
+    assert np.abs(z).max() <= 4.5
+    assert np.mean(z ** 2) <= 2.5
+    assert np.all(np.abs(zi) <= 3.0)
+
+
+@pytest.fixture(scope="module")
+def strict_ribbon():
+    return rt.Problem(rt.config_text("c1lc_conservative.toml"), input_directory=rt.CONFIGS, mode_sampling=1)
+
+
+def test_strict_reflections_bit_exact(strict_ribbon, golden):
+    g = golden("rough_c1lc.npz")
+    mb, fr, rough = layout(strict_ribbon)
+    assert float(g["t_bg"]) == strict_ribbon.views()[0].background_temperature
+    negative = 0
+    for k in range(len(mb) - 1):
+        for f in (0, 1):
+            t = int(fr[k, f])
+            assert rough[t].strict_elastic == 1
+            ins, occ, u, out, oo = (g[f"m{k}_f{f}_{x}"] for x in ("in", "occ", "u", "out", "out_occ"))
+            for j in range(len(ins)):
+                a, o, _, _ = O.rough_sample_signed(strict_ribbon, t, int(mb[k] + ins[j]), float(occ[j]),
+                                                   [u[j, 0], u[j, 1], 0.5, 0.5], 1)
+                assert a - mb[k] == out[j], (k, f, j)
+                assert o == oo[j], (k, f, j)
+                negative += o < 0
+    assert negative > 10
+
+
+def test_strict_oracle_runs_match_reference():
+    golden = np.load(os.path.join(HERE, "golden", "run_c1lc.npz"))
+    n, steps, interval = int(golden["n"]), int(golden["steps"]), int(golden["interval"])
+    seeds = [8200 + k for k in range(8)]
+    with concurrent.futures.ThreadPoolExecutor(8) as pool:
+        runs = list(pool.map(lambda s: oracle_run("c1lc_conservative.toml", n, steps, interval, s, counters=True),
+                             seeds))
+    temps = np.array([r[0] for r in runs])
+    iface = np.array([r[1] for r in runs], dtype=float)
+    z = z_score(temps, golden["temperature"])
+    zi = z_score(iface, golden["interface"].astype(float))
+    print(f"c1lc oracle: cell T max|z| {np.abs(z).max():.2f}, mean z^2 {np.mean(z ** 2):.2f}; interface events "
           f"z {zi[0]:+.2f}, transmitted z {zi[1]:+.2f}")
     assert np.abs(z).max() <= 4.5
     assert np.mean(z ** 2) <= 2.5
