@@ -241,6 +241,12 @@ typedef struct pmc_layers {
      * indices with mode_begin / mode_count set to the material's range.  NULL:
      * every material uses pmc_facet.rough_table. */
     const int32_t* facet_rough;
+    /* Linearized transport (pmc_upload_callaway before this call): each material's
+     * heat-capacity density C_bg, eV/(K A^3); cell temperatures are
+     * T_bg + (e - E_bg,k) / C_bg,k with background_energy_density holding each
+     * material's exact crystal energy density (BackgroundCache,
+     * include/solver/BackgroundCache.h:13-30).  Diffuse mismatch only; NULL otherwise. */
+    const double* heat_capacity_density;
 } pmc_layers;
 
 /* Linearized Callaway scattering about the fixed reference T_bg; replaces

@@ -1164,6 +1164,7 @@ struct orc_sim {
     int layers = 1, layer_axis = 0;
     double band = 0.0;
     std::vector<double> planes, e_bg_mat, accept, bank_omega, bank_prefix;
+    std::vector<double> cbg_mat; // linearized transport: C_bg of each material
     std::vector<int> mode_begin, cell_mat, lut_begin, bank_begin, bank_mode;
     // acoustic / mixed mismatch (AMM rows in global active indices)
     double amm_fraction = 0.0;
@@ -1311,6 +1312,22 @@ int interface_outcome(const orc_sim& s, int a, const V3& vel, double occ, int se
     out = a;
     nocc = occ;
     nvel = vel;
+    // Linearized transport (:100-105, :124-133): no outgoing mode, or one that does
+    // not leave in the expected direction along the axis, fails (-3); signed
+    // occupations are kept.
+    if (s.linearized) {
+        if (sel < 0) return -3;
+        const double W = s.m.energy_weight;
+        const double n = s.m.neq_background[sel] +
+                         (W * w * (occ - s.m.neq_background[a])) / std::max(1e-300, W * s.m.omega[sel]);
+        const double vs = table_velocity(s, sel)[ax];
+        const bool out_up = transmit ? vel[ax] > 0.0 : !(vel[ax] > 0.0);
+        if (!std::isfinite(n) || !std::isfinite(vs) || (out_up ? !(vs > 0.0) : !(vs < 0.0))) return -3;
+        out = sel;
+        nocc = n;
+        nvel = table_velocity(s, sel);
+        return transmit ? 1 : 0;
+    }
     if (sel >= 0) {
         const double W = s.m.energy_weight;
         const double dev = occ - s.m.neq_background[a];
@@ -1387,7 +1404,7 @@ bool interface_event(orc_sim& s, int64_t i, int dest, Uniforms& U, StepTally& ta
     double nocc = 0.0;
     V3 nvel{};
     const int action = interface_core(s, s.mode[i], s.vel[i], s.occ[i], dest, u, out, nocc, nvel);
-    if (action == -2) return false;
+    if (action == -2 || action == -3) return false;
     tally.iface_events++;
     if (action == 1) tally.iface_transmitted++;
     s.mode[i] = out;
@@ -1549,10 +1566,13 @@ void refresh(orc_sim& s) {
                                  s.lut_begin[k + 1] - s.lut_begin[k], s.cell_e[c]);
         }
     }
-    if (s.linearized) // T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225)
-        for (int c = 0; c < nc; ++c)
-            s.cell_T[c] = s.setup.background_temperature +
-                          (s.cell_e[c] - s.cw.background_energy_density) / s.cw.heat_capacity_density;
+    if (s.linearized) // T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225), by material
+        for (int c = 0; c < nc; ++c) {
+            const bool lay = s.layers > 1;
+            const double ebg = lay ? s.e_bg_mat[s.cell_mat[c]] : s.cw.background_energy_density;
+            const double cbg = lay ? s.cbg_mat[s.cell_mat[c]] : s.cw.heat_capacity_density;
+            s.cell_T[c] = s.setup.background_temperature + (s.cell_e[c] - ebg) / cbg;
+        }
 }
 
 double thermal_energy(const orc_sim& s) {
@@ -1807,6 +1827,12 @@ int orc_set_layers(orc_sim* s, const pmc_layers* L) {
     s->mode_begin.assign(L->mode_begin, L->mode_begin + n + 1);
     s->cell_mat.assign(L->cell_material, L->cell_material + s->g.n_cells);
     s->e_bg_mat.assign(L->background_energy_density, L->background_energy_density + n);
+    if (s->linearized) {
+        if (!L->heat_capacity_density || L->amm_fraction != 0.0) return PMC_ERR_UNSUPPORTED;
+        s->cbg_mat.assign(L->heat_capacity_density, L->heat_capacity_density + n);
+    } else {
+        s->cbg_mat.clear();
+    }
     s->lut_begin.assign(L->lut_begin, L->lut_begin + n + 1);
     s->bank_begin.assign(L->bank_begin, L->bank_begin + 2 * n + 1);
     const int nb = L->bank_begin[2 * n];

@@ -103,7 +103,7 @@ void build_layers(Problem& p) {
     std::vector<std::shared_ptr<const MaterialEntry>> mats;
     for (const auto& f : c.material_folders)
         mats.push_back(cached_material(f, c.input_directory, c.temperature_lookup_dt,
-                                       c.conservative_boundaries || (c.heat_source_enabled && c.heat_spectrum == 2)));
+                                       p.conservative || (c.heat_source_enabled && c.heat_spectrum == 2)));
     for (const auto& e : mats)
         if (e->modes.temps != mats[0]->modes.temps)
             throw std::runtime_error("Layered materials need equal lifetime temperature grids in this build.");
@@ -155,6 +155,46 @@ void build_layers(Problem& p) {
                     t.gamma_total[static_cast<size_t>(r) * t.n_flat + t.active_flat[a]];
     }
     p.background_energy_density = L.background_energy_density[0];
+    // Linearized Callaway (CarrierCollisionOperator::configure per material,
+    // src/solver/CarrierCollisionOperator.cpp:13-36; BackgroundCache per material,
+    // include/solver/BackgroundCache.h:13-30): each material's callaway_data at T_bg,
+    // concatenated over the global bank, its exact crystal energy density as E_bg
+    // and its heat-capacity density C_bg.
+    L.heat_capacity_density.clear();
+    if (p.linearized) {
+        const double tb = c.background_temperature;
+        p.callaway = CallawayTable{};
+        for (int k = 0; k < L.count; ++k) {
+            const auto& t = mats[k]->modes;
+            double hc = 0.0;
+            for (int q = 0; q < t.qpoints; ++q)
+                for (int b = 0; b < t.branches; ++b) hc += t.heat_capacity(tb, q * t.branches + b) / t.energy_normalization;
+            if (!(hc > 0.0) || !std::isfinite(hc))
+                throw std::invalid_argument("Linearized temperature requires positive heat capacity.");
+            L.heat_capacity_density.push_back(hc);
+            L.background_energy_density[k] = t.crystal_energy_density(tb);
+            const CallawayTable ct = callaway_data(t, mats[k]->completed, tb);
+            auto append = [](auto& dst, const auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
+            append(p.callaway.omega, ct.omega);
+            append(p.callaway.capacity, ct.capacity);
+            append(p.callaway.normal_rate, ct.normal_rate);
+            append(p.callaway.resistive_rate, ct.resistive_rate);
+            append(p.callaway.wavevector, ct.wavevector);
+        }
+        p.heat_capacity_density = L.heat_capacity_density[0];
+        p.background_energy_density = L.background_energy_density[0];
+        const size_t total = p.callaway.omega.size();
+        p.callaway_wavevector.resize(3 * total);
+        for (size_t a = 0; a < total; ++a)
+            for (int k = 0; k < 3; ++k) p.callaway_wavevector[3 * a + k] = p.callaway.wavevector[a][k];
+        auto& cv = p.callaway_view;
+        cv.capacity = p.callaway.capacity.data();
+        cv.wavevector = p.callaway_wavevector.data();
+        cv.normal_rate = p.callaway.normal_rate.data();
+        cv.resistive_rate = p.callaway.resistive_rate.data();
+        cv.heat_capacity_density = p.heat_capacity_density;
+        cv.background_energy_density = p.background_energy_density;
+    }
     // DMM banks: per material and direction along the axis, modes sorted by
     // frequency with the cumulative |v_axis| / normalization (InterfaceModeSampler::configure).
     L.bank_begin.assign(1, 0);
@@ -321,8 +361,10 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     if (c.has_gradient_drive) throw std::runtime_error("driving.temperature_gradient is not supported by this build.");
     const bool layered = c.material_folders.size() > 1;
     if (layered) {
-        if (c.collision == CollisionModel::Callaway)
-            throw std::runtime_error("Layered materials with scattering.model = 'callaway' are not supported by this build.");
+        if (nonlinear)
+            throw std::runtime_error("Layered materials with the nonlinear Callaway formulation are not supported by this build.");
+        if (linearized && c.interface_model != "dmm")
+            throw std::runtime_error("Layered materials with scattering.model = 'callaway' take diffuse-mismatch interfaces in this build.");
         if (!std::isfinite(c.interface_bin_thz) || c.interface_bin_thz <= 0)
             throw std::runtime_error("Interface frequency bin must be finite and positive.");
     }
@@ -428,15 +470,25 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     if (layered) layered_reservoirs(*p);
     else p->reservoirs = build_reservoir_tables(d, m, p->particle_density, c.time_step, p->facet_to_reservoir);
     if (linearized) {
-        // Linearized reservoir occupations (src/solver/ReservoirInjector.cpp:104-109).
+        // Linearized reservoir occupations (src/solver/ReservoirInjector.cpp:104-109),
+        // each material's block of the bank from its own table.
+        std::vector<const ModeTables*> banks{&m};
+        std::vector<int> begin{0};
+        if (layered) {
+            banks = p->layer_modes;
+            begin.assign(p->layers.mode_begin.begin(), p->layers.mode_begin.end() - 1);
+        }
         for (auto& r : p->reservoirs)
-            for (int a = 0; a < m.n_active; ++a) {
-                const int f = m.active_flat[a];
-                const double hw = kHbarEvPs * m.omega_flat[f];
-                r.occupation[a] = hw > 0.0 ? m.bose(c.background_temperature, f) +
-                                                 m.heat_capacity(c.background_temperature, f) *
-                                                     (r.temperature - c.background_temperature) / hw
-                                           : 0.0;
+            for (size_t k = 0; k < banks.size(); ++k) {
+                const ModeTables& t = *banks[k];
+                for (int a = 0; a < t.n_active; ++a) {
+                    const int f = t.active_flat[a];
+                    const double hw = kHbarEvPs * t.omega_flat[f];
+                    r.occupation[begin[k] + a] = hw > 0.0 ? t.bose(c.background_temperature, f) +
+                                                                t.heat_capacity(c.background_temperature, f) *
+                                                                    (r.temperature - c.background_temperature) / hw
+                                                          : 0.0;
+                }
             }
     }
 
@@ -523,6 +575,7 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
         lv.amm_edge_mode = L.amm_fraction > 0 && !L.amm_edge_mode.empty() ? L.amm_edge_mode.data() : nullptr;
         lv.amm_edge_prob = L.amm_fraction > 0 && !L.amm_edge_prob.empty() ? L.amm_edge_prob.data() : nullptr;
         lv.facet_rough = L.facet_rough.empty() ? nullptr : L.facet_rough.data();
+        lv.heat_capacity_density = L.heat_capacity_density.empty() ? nullptr : L.heat_capacity_density.data();
     }
     auto& g = p->geometry_view;
     g.kind = 0;

@@ -68,6 +68,8 @@ struct Problem {
         std::vector<AmmAudit> amm_audit;
         // Reflection table of facet f for material k at k * n_facets + f (pmc_layers.facet_rough)
         std::vector<int32_t> facet_rough;
+        // Linearized transport: heat-capacity density C_bg of every material (pmc_layers.heat_capacity_density)
+        std::vector<double> heat_capacity_density;
     } layers;
     pmc_layers layers_view{};
     std::vector<const ModeTables*> layer_modes;            // per material (layered problems)

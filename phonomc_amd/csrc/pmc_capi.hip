@@ -720,6 +720,8 @@ int pmc_upload_callaway(pmc_ctx* ctx, const pmc_callaway_table* t) {
         return fail(ctx, PMC_ERR_ARGUMENT, "Callaway table arrays missing");
     if (!ctx->has_material || !ctx->has_geometry)
         return fail(ctx, PMC_ERR_ARGUMENT, "material and geometry must be uploaded before the Callaway table");
+    if (ctx->view.layers > 1)
+        return fail(ctx, PMC_ERR_ARGUMENT, "upload the Callaway table before the layered-material tables");
     if (ctx->view.inversion == 1)
         return fail(ctx, PMC_ERR_UNSUPPORTED,
                     "Newton temperature inversion applies to RTA transport; linearized Callaway uses "
@@ -1151,8 +1153,17 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     View& v = ctx->view;
     if (!ctx->has_material || !ctx->has_geometry) return fail(ctx, PMC_ERR_ARGUMENT, "layers need material and geometry");
     if (ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "layers must be uploaded before the population is created");
-    if (v.callaway || v.nl_callaway)
-        return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run relaxation-time scattering");
+    if (v.nl_callaway)
+        return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run relaxation-time or linearized Callaway scattering");
+    if (v.linearized) {
+        if (!L->heat_capacity_density)
+            return fail(ctx, PMC_ERR_ARGUMENT, "linearized layers need each material's heat-capacity density");
+        for (int k = 0; k < L->count && k < pmc_dev::kMaxLayers; ++k)
+            if (!(L->heat_capacity_density[k] > 0.0) || !std::isfinite(L->heat_capacity_density[k]))
+                return fail(ctx, PMC_ERR_ARGUMENT, "linearized layers need positive heat-capacity densities");
+        if (L->amm_fraction != 0.0)
+            return fail(ctx, PMC_ERR_UNSUPPORTED, "layered linearized Callaway takes diffuse-mismatch interfaces");
+    }
     if (v.inversion != 0)
         return fail(ctx, PMC_ERR_UNSUPPORTED, "Newton temperature inversion covers one material; layers use their tables");
     if (v.geom_kind != 0) return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials need the box geometry");
@@ -1222,6 +1233,7 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     }
     for (int k = 0; k < n; ++k) {
         v.e_bg_mat[k] = L->background_energy_density[k];
+        v.cbg_mat[k] = v.linearized ? L->heat_capacity_density[k] : 0.0;
         v.layer_accept[k] = vmin / L->spatial_volume[k];
     }
     for (int b = 0; b <= 2 * n; ++b) v.bank_begin[b] = L->bank_begin[b];

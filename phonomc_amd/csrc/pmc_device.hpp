@@ -246,6 +246,7 @@ struct View {
     const int* cell_mat;
     const int* layer_rough; // [layers * nfacets] reflection table by (material, facet); null: facets' own
     double e_bg_mat[kMaxLayers];
+    double cbg_mat[kMaxLayers]; // linearized transport: heat-capacity density of each material
     int lut_begin[kMaxLayers + 1];
     double layer_accept[kMaxLayers]; // initial positions: smallest spatial volume / this material's
     double band;                     // DMM frequency band, rad/ps

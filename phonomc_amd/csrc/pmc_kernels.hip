@@ -763,7 +763,17 @@ __device__ int advance(const View& V, float p[3], int& mw, double& occ, unsigned
                     double nocc = 0.0;
                     // Energy-preserving remap with the (common) energy weight, :111-122.
                     if (sel >= 0) nocc = V.neq[sel] + (V.w * w * (occ - V.neq[m])) / fmax(1e-300, V.w * V.omega[sel]);
-                    if (sel < 0 || !isfinite(nocc) || nocc < 0.0) {
+                    // Linearized transport (:100-105, :124-133): no outgoing mode, or one that
+                    // does not leave in the expected direction along the axis, fails; signed
+                    // occupations are kept.
+                    if (V.linearized) {
+                        if (sel < 0) return -kErrModeMapping;
+                        const double vs = V.veld[3 * static_cast<size_t>(sel) + ax];
+                        const bool out_up = transmit ? inc == 1 : inc == 0;
+                        if (!isfinite(nocc) || !isfinite(vs) || (out_up ? !(vs > 0.0) : !(vs < 0.0)))
+                            return -kErrModeMapping;
+                    }
+                    if (sel < 0 || !isfinite(nocc) || (!V.linearized && nocc < 0.0)) {
                         mw ^= (1 << ax) << kFlipShift; // mirror along the axis, same mode
                         mode_changed = true;
                         p[ax] = va > 0.0f ? nextafterf(at, -INFINITY) : nextafterf(at, INFINITY);
@@ -1408,7 +1418,13 @@ __device__ __forceinline__ double cell_background(const View& V, int c) {
 // Temperature of cell c at energy density e: its material's lookup table, or
 // under linearized transport T_bg + (e - E_bg) / C_bg (src/solver/GridStatistics.cpp:193-225).
 __device__ __forceinline__ double cell_temperature(const View& V, double e, int c) {
-    if (V.linearized) return V.t_bg + (e - V.e_bg) / V.cw_cbg;
+    if (V.linearized) {
+        if (V.layers > 1) {
+            const int k = V.cell_mat[c];
+            return V.t_bg + (e - V.e_bg_mat[k]) / V.cbg_mat[k];
+        }
+        return V.t_bg + (e - V.e_bg) / V.cw_cbg;
+    }
     if (V.layers > 1) {
         const int m = V.cell_mat[c];
         return lut_inverse(V, e, V.lut_begin[m], V.lut_begin[m + 1] - V.lut_begin[m]);

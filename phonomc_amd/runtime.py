@@ -115,7 +115,7 @@ class Layers(C.Structure):
                 ("amm_fraction", C.c_double), ("amm_block", C.POINTER(C.c_int32)),
                 ("amm_reflected", C.POINTER(C.c_int32)), ("amm_edge_begin", C.POINTER(C.c_int32)),
                 ("amm_edge_mode", C.POINTER(C.c_int32)), ("amm_edge_prob", C.POINTER(C.c_double)),
-                ("facet_rough", C.POINTER(C.c_int32))]
+                ("facet_rough", C.POINTER(C.c_int32)), ("heat_capacity_density", C.POINTER(C.c_double))]
 
 
 class LaunchConfig(C.Structure):

@@ -14,6 +14,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --layered-walls [--runs]       (layered ribbon with rough side walls)
     python tests/golden/make_golden.py --layered-source               (heated layered film, whole runs)
     python tests/golden/make_golden.py --layered-strict [--runs]      (layered ribbon, conservative walls)
+    python tests/golden/make_golden.py --layered-callaway             (layered Callaway film, whole runs)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -43,7 +44,8 @@ WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "
              "finfet_cw": "finfet_callaway.toml", "c1cn": "c1cn_nonlinear.toml", "c2cn": "c2cn_nonlinear.toml",
              "c2r": "c2r_conservative.toml", "c1la": "c1la_amm.toml", "c1lm": "c1lm_mmm.toml",
              "c1sw": "c1sw_weighted.toml", "c1sg": "c1sg_gaussian.toml", "taper": "taper_fin.toml",
-             "c1lr": "c1lr_rough.toml", "c1ls": "c1ls_source.toml", "c1lc": "c1lc_conservative.toml"}
+             "c1lr": "c1lr_rough.toml", "c1ls": "c1ls_source.toml", "c1lc": "c1lc_conservative.toml",
+             "c1lw": "c1lw_callaway.toml"}
 
 
 def write_input(text, folder, name):
@@ -651,6 +653,11 @@ def layered_strict_vectors(problem, rd, rng, facets, n=1500):
 
 
 def main():
+    if "--layered-callaway" in sys.argv:
+        np.savez_compressed(os.path.join(OUT, "run_c1lw.npz"),
+                            **whole_run("c1lw", 20000, 400, list(range(1000, 1016)), 10))
+        print("layered Callaway runs written to", OUT)
+        return
     if "--layered-strict" in sys.argv:
         rng = np.random.default_rng(20261215)
         text = rt.config_text(WORKLOADS["c1lc"])

@@ -156,8 +156,8 @@ def test_unsupported_features_fail_loudly():
         rt.Problem(rt.config_text("c1_cross_plane.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))
     # Layered films take relaxation-time scattering, with or without conservative walls.
     rt.Problem(rt.config_text("c1l_layered.toml", **{"simulation.conservative_boundaries": "true"}))
-    # Layered films run DMM, AMM and MMM interfaces, but not Callaway.
-    with pytest.raises(RuntimeError, match="Layered materials with scattering.model = 'callaway'"):
+    # Layered Callaway checks every material's N/U split with the reference's message.
+    with pytest.raises(RuntimeError, match="Callaway requires gamma_N and gamma_U"):
         rt.Problem(rt.config_text("c1l_layered.toml").replace("[io]", "[scattering]\nmodel = \"callaway\"\n[io]"))
 
 

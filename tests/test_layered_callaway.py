@@ -1,0 +1,120 @@
+"""Linearized Callaway scattering in layered materials (c1lw_callaway.toml: the Si/Ge
+film of c1l with the N/U tables of both materials).  The reference configures
+CarrierCollisionOperator with every material's callaway_data at T_bg
+(src/solver/CarrierCollisionOperator.cpp:13-36), a linearized BackgroundCache
+per material (include/solver/BackgroundCache.h:13-30) and the linearized
+interface rule (src/solver/InterfaceInteraction.cpp:100-105, :124-133).
+
+- Host: the layered Callaway table is each material's single-material table
+  concatenated over the global bank; C_bg and E_bg (the exact crystal energy
+  density) are each material's own; the linearized reservoir occupations of
+  each reservoir's material equal the single-material ones.
+- Oracle: the energy balance closes every step, the initial cell temperatures
+  follow the linear profile through each material's C_bg, and 8 oracle runs
+  agree with 16 reference runs (tests/golden/run_c1lw.npz, make_golden.py
+  --layered-callaway) in cell temperatures, kappa and interface totals.
+- Rejections: nonlinear Callaway and AMM interfaces with layered Callaway.
+"""
+import concurrent.futures
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_ctypes as O
+from phonomc_amd import runtime as rt
+
+from test_oracle_statistics import oracle_run, z_score
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def film():
+    return rt.Problem(rt.config_text("c1lw_callaway.toml"), input_directory=rt.CONFIGS, mode_sampling=1)
+
+
+def single(folder):
+    return rt.Problem(rt.config_text("c1c_callaway.toml", **{"io.material_folder": f'"{folder}"'}),
+                      input_directory=rt.CONFIGS, mode_sampling=1)
+
+
+def test_tables_are_the_materials_tables(film):
+    cw = film.callaway()
+    lv = film.layers()
+    s, m, *_ = film.views()
+    na = m.n_active
+    mb = rt.arr(lv.mode_begin, lv.count + 1, np.int32)
+    cbg = rt.arr(lv.heat_capacity_density, lv.count)
+    ebg = rt.arr(lv.background_energy_density, lv.count)
+    cols = {"capacity": 1, "normal_rate": 1, "resistive_rate": 1, "wavevector": 3}
+    for k, folder in enumerate(("synthetic:si16:callaway", "synthetic:ge16:callaway")):
+        one = single(folder)
+        c1, m1 = one.callaway(), one.views()[1]
+        assert m1.n_active == mb[k + 1] - mb[k]
+        for name, width in cols.items():
+            whole = rt.arr(getattr(cw, name), width * na)
+            part = rt.arr(getattr(c1, name), width * m1.n_active)
+            assert np.array_equal(whole[width * mb[k]:width * mb[k + 1]], part), (k, name)
+        assert cbg[k] == c1.heat_capacity_density
+        assert ebg[k] == c1.background_energy_density
+    assert cbg[0] != cbg[1]
+
+
+def test_linearized_reservoir_occupations(film):
+    s, m, g, i, rough, res = film.views()
+    lv = film.layers()
+    mb = rt.arr(lv.mode_begin, lv.count + 1, np.int32)
+    na = m.n_active
+    for r in res:
+        k = 0 if r.temperature > 300 else 1
+        one = single(("synthetic:si16:callaway", "synthetic:ge16:callaway")[k])
+        r1 = [x for x in one.views()[5] if (x.temperature > 300) == (r.temperature > 300)][0]
+        n1 = one.views()[1].n_active
+        occ = rt.arr(r.occupation, na)[mb[k]:mb[k + 1]]
+        assert np.array_equal(occ, rt.arr(r1.occupation, n1))
+
+
+def test_oracle_balance_and_initial_profile(film):
+    p = rt.Problem(rt.config_text("c1lw_callaway.toml", **{"simulation.particle_count": 20000}),
+                   input_directory=rt.CONFIGS)
+    sim = O.Sim(p, threads=8)
+    sim.initialize()
+    t0 = sim.cells()[0]
+    # linear 301 -> 299 K initial profile, within carrier noise, in both materials
+    z = (np.arange(20) + 0.5) / 20
+    assert np.abs(t0 - (301 - 2 * z)).max() < 0.1
+    for _ in range(10):
+        sim.step(1)
+        led = sim.ledger()
+        assert abs(led["step_balance_residual_ev"]) <= 1e-12 * led["thermal_energy_ev"]
+    assert sim.ledger()["interface_events"] > 0
+    sim.close()
+
+
+def test_rejections():
+    with pytest.raises(RuntimeError, match="nonlinear Callaway formulation"):
+        rt.Problem(rt.config_text("c1lw_callaway.toml").replace('model = "callaway"',
+                                                                'model = "callaway"\ncallaway_formulation = "nonlinear"'),
+                   input_directory=rt.CONFIGS)
+    with pytest.raises(RuntimeError, match="diffuse-mismatch interfaces"):
+        rt.Problem(rt.config_text("c1lw_callaway.toml", **{"materials.interface_model": '"amm"',
+                                                           "materials.mass_densities_kg_m3": "[2329, 5323]"}),
+                   input_directory=rt.CONFIGS)
+
+
+def test_oracle_runs_match_reference():
+    golden = np.load(os.path.join(HERE, "golden", "run_c1lw.npz"))
+    n, steps, interval = int(golden["n"]), int(golden["steps"]), int(golden["interval"])
+    seeds = [8300 + k for k in range(8)]
+    with concurrent.futures.ThreadPoolExecutor(8) as pool:
+        runs = list(pool.map(lambda s: oracle_run("c1lw_callaway.toml", n, steps, interval, s, counters=True), seeds))
+    temps = np.array([r[0] for r in runs])
+    iface = np.array([r[1] for r in runs], dtype=float)
+    z = z_score(temps, golden["temperature"])
+    zi = z_score(iface, golden["interface"].astype(float))
+    print(f"c1lw oracle: cell T max|z| {np.abs(z).max():.2f}, mean z^2 {np.mean(z ** 2):.2f}; interface events "
+          f"z {zi[0]:+.2f}, transmitted z {zi[1]:+.2f}")
+    assert np.abs(z).max() <= 4.5
+    assert np.mean(z ** 2) <= 2.5
+    assert np.all(np.abs(zi) <= 3.0)
