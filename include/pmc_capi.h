@@ -245,7 +245,7 @@ typedef struct pmc_layers {
      * heat-capacity density C_bg, eV/(K A^3); cell temperatures are
      * T_bg + (e - E_bg,k) / C_bg,k with background_energy_density holding each
      * material's exact crystal energy density (BackgroundCache,
-     * include/solver/BackgroundCache.h:13-30).  Diffuse mismatch only; NULL otherwise. */
+     * include/solver/BackgroundCache.h:13-30).  NULL otherwise. */
     const double* heat_capacity_density;
 } pmc_layers;
 

@@ -1828,7 +1828,7 @@ int orc_set_layers(orc_sim* s, const pmc_layers* L) {
     s->cell_mat.assign(L->cell_material, L->cell_material + s->g.n_cells);
     s->e_bg_mat.assign(L->background_energy_density, L->background_energy_density + n);
     if (s->linearized) {
-        if (!L->heat_capacity_density || L->amm_fraction != 0.0) return PMC_ERR_UNSUPPORTED;
+        if (!L->heat_capacity_density) return PMC_ERR_UNSUPPORTED;
         s->cbg_mat.assign(L->heat_capacity_density, L->heat_capacity_density + n);
     } else {
         s->cbg_mat.clear();

@@ -363,8 +363,6 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     if (layered) {
         if (nonlinear)
             throw std::runtime_error("Layered materials with the nonlinear Callaway formulation are not supported by this build.");
-        if (linearized && c.interface_model != "dmm")
-            throw std::runtime_error("Layered materials with scattering.model = 'callaway' take diffuse-mismatch interfaces in this build.");
         if (!std::isfinite(c.interface_bin_thz) || c.interface_bin_thz <= 0)
             throw std::runtime_error("Interface frequency bin must be finite and positive.");
     }

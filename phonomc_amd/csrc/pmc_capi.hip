@@ -1161,8 +1161,6 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
         for (int k = 0; k < L->count && k < pmc_dev::kMaxLayers; ++k)
             if (!(L->heat_capacity_density[k] > 0.0) || !std::isfinite(L->heat_capacity_density[k]))
                 return fail(ctx, PMC_ERR_ARGUMENT, "linearized layers need positive heat-capacity densities");
-        if (L->amm_fraction != 0.0)
-            return fail(ctx, PMC_ERR_UNSUPPORTED, "layered linearized Callaway takes diffuse-mismatch interfaces");
     }
     if (v.inversion != 0)
         return fail(ctx, PMC_ERR_UNSUPPORTED, "Newton temperature inversion covers one material; layers use their tables");

@@ -15,6 +15,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --layered-source               (heated layered film, whole runs)
     python tests/golden/make_golden.py --layered-strict [--runs]      (layered ribbon, conservative walls)
     python tests/golden/make_golden.py --layered-callaway             (layered Callaway film, whole runs)
+    python tests/golden/make_golden.py --layered-callaway-amm         (the same with AMM interfaces, 15^3 tables)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -46,6 +47,11 @@ WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "
              "c1sw": "c1sw_weighted.toml", "c1sg": "c1sg_gaussian.toml", "taper": "taper_fin.toml",
              "c1lr": "c1lr_rough.toml", "c1ls": "c1ls_source.toml", "c1lc": "c1lc_conservative.toml",
              "c1lw": "c1lw_callaway.toml"}
+
+
+# acoustic-mismatch interfaces on a layered Si/Ge input
+LAYERED_AMM = {"materials.interface_model": '"amm"', "materials.mass_densities_kg_m3": "[2329, 5323]",
+               "materials.folders": '["synthetic:si15:callaway", "synthetic:ge15:callaway"]'}
 
 
 def write_input(text, folder, name):
@@ -653,9 +659,15 @@ def layered_strict_vectors(problem, rd, rng, facets, n=1500):
 
 
 def main():
+    if "--layered-callaway-amm" in sys.argv:
+        np.savez_compressed(os.path.join(OUT, "run_c1lw_amm.npz"),
+                            **whole_run("c1lw", 20000, 400, list(range(1000, 1016)), 10, **LAYERED_AMM))
+        print("layered Callaway AMM runs written to", OUT)
+        return
     if "--layered-callaway" in sys.argv:
         np.savez_compressed(os.path.join(OUT, "run_c1lw.npz"),
                             **whole_run("c1lw", 20000, 400, list(range(1000, 1016)), 10))
+
         print("layered Callaway runs written to", OUT)
         return
     if "--layered-strict" in sys.argv:

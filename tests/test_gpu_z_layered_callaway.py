@@ -20,6 +20,8 @@ pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 MASK = (1 << 28) - 1
 WORKLOADS.update({"c1lw": "c1lw_callaway.toml"})
+AMM = {"materials.interface_model": '"amm"', "materials.mass_densities_kg_m3": "[2329, 5323]",
+       "materials.folders": '["synthetic:si15:callaway", "synthetic:ge15:callaway"]'}
 
 
 def by_uid(c):
@@ -28,10 +30,10 @@ def by_uid(c):
     return {k: v[live][order] for k, v in c.items()}
 
 
-@pytest.mark.parametrize("sampling", [1, 0])
-def test_layered_callaway_steps_like_the_oracle(sampling):
-    print(f"case c1lw_callaway sampling={sampling} n=20000", flush=True)
-    p = rt.Problem(rt.config_text("c1lw_callaway.toml", **{"simulation.particle_count": 20000}),
+@pytest.mark.parametrize("sampling,over", [(1, {}), (0, {}), (1, AMM)])
+def test_layered_callaway_steps_like_the_oracle(sampling, over):
+    print(f"case c1lw_callaway sampling={sampling} {over} n=20000", flush=True)
+    p = rt.Problem(rt.config_text("c1lw_callaway.toml", **{"simulation.particle_count": 20000, **over}),
                    input_directory=rt.CONFIGS, mode_sampling=sampling)
     g = rt.Context(p)
     o = O.Sim(p)
