@@ -195,6 +195,41 @@ void build_layers(Problem& p) {
         cv.heat_capacity_density = p.heat_capacity_density;
         cv.background_energy_density = p.background_energy_density;
     }
+    // Nonlinear Callaway (NonlinearCallawayCollision::configure per material,
+    // src/solver/NonlinearCallawayCollision.cpp:287-301): every material's N/U split
+    // validated through callaway_data at T_bg, the rate rows and Cartesian q of
+    // each material on the common temperature grid, over the global bank.
+    if (p.nonlinear) {
+        const int total = L.mode_begin.back();
+        auto& r = p.nl_rates;
+        r.temps = temps;
+        r.gamma_normal.assign(static_cast<size_t>(nt) * total, 0.0);
+        r.gamma_umklapp.assign(static_cast<size_t>(nt) * total, 0.0);
+        r.gamma_extra.assign(static_cast<size_t>(nt) * total, 0.0);
+        r.wavevector.assign(3 * static_cast<size_t>(total), 0.0);
+        for (int k = 0; k < L.count; ++k) {
+            const auto& t = mats[k]->modes;
+            const CallawayTable checked = callaway_data(t, mats[k]->completed, c.background_temperature);
+            const size_t nf = static_cast<size_t>(t.n_flat);
+            for (int a = 0; a < t.n_active; ++a) {
+                const size_t g = static_cast<size_t>(L.mode_begin[k] + a);
+                for (int row = 0; row < nt; ++row) {
+                    const size_t f = static_cast<size_t>(row) * nf + static_cast<size_t>(t.active_flat[a]);
+                    r.gamma_normal[static_cast<size_t>(row) * total + g] = t.gamma_normal[f];
+                    r.gamma_umklapp[static_cast<size_t>(row) * total + g] = t.gamma_umklapp[f];
+                    r.gamma_extra[static_cast<size_t>(row) * total + g] = t.gamma_extra[f];
+                }
+                for (int q = 0; q < 3; ++q) r.wavevector[3 * g + q] = checked.wavevector[a][q];
+            }
+        }
+        auto& v = p.nl_view;
+        v.n_temps = nt;
+        v.temps = r.temps.data();
+        v.gamma_normal = r.gamma_normal.data();
+        v.gamma_umklapp = r.gamma_umklapp.data();
+        v.gamma_extra = r.gamma_extra.data();
+        v.wavevector = r.wavevector.data();
+    }
     // DMM banks: per material and direction along the axis, modes sorted by
     // frequency with the cumulative |v_axis| / normalization (InterfaceModeSampler::configure).
     L.bank_begin.assign(1, 0);
@@ -361,8 +396,6 @@ std::unique_ptr<Problem> build_problem(const SimConfig& c, const ProblemOptions&
     if (c.has_gradient_drive) throw std::runtime_error("driving.temperature_gradient is not supported by this build.");
     const bool layered = c.material_folders.size() > 1;
     if (layered) {
-        if (nonlinear)
-            throw std::runtime_error("Layered materials with the nonlinear Callaway formulation are not supported by this build.");
         if (!std::isfinite(c.interface_bin_thz) || c.interface_bin_thz <= 0)
             throw std::runtime_error("Interface frequency bin must be finite and positive.");
     }

@@ -791,7 +791,7 @@ int pmc_upload_callaway_nonlinear(pmc_ctx* ctx, const pmc_callaway_rates* r) {
     if (ctx->view.inversion == 1)
         return fail(ctx, PMC_ERR_UNSUPPORTED, "Newton temperature inversion applies to RTA transport");
     if (ctx->view.layers > 1)
-        return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run relaxation-time scattering");
+        return fail(ctx, PMC_ERR_ARGUMENT, "upload the nonlinear Callaway tables before the layered-material tables");
     View& v = ctx->view;
     const size_t na = static_cast<size_t>(v.na), nt = static_cast<size_t>(r->n_temps);
     for (size_t k = 0; k < nt; ++k)
@@ -1153,8 +1153,6 @@ extern "C" int pmc_upload_layers(pmc_ctx* ctx, const pmc_layers* L) {
     View& v = ctx->view;
     if (!ctx->has_material || !ctx->has_geometry) return fail(ctx, PMC_ERR_ARGUMENT, "layers need material and geometry");
     if (ctx->initialized) return fail(ctx, PMC_ERR_ARGUMENT, "layers must be uploaded before the population is created");
-    if (v.nl_callaway)
-        return fail(ctx, PMC_ERR_UNSUPPORTED, "layered materials run relaxation-time or linearized Callaway scattering");
     if (v.linearized) {
         if (!L->heat_capacity_density)
             return fail(ctx, PMC_ERR_ARGUMENT, "linearized layers need each material's heat-capacity density");

@@ -16,6 +16,7 @@ sources compiled by oracle/ref.mk (oracle/_ref/libphonomc_ref.so).
     python tests/golden/make_golden.py --layered-strict [--runs]      (layered ribbon, conservative walls)
     python tests/golden/make_golden.py --layered-callaway             (layered Callaway film, whole runs)
     python tests/golden/make_golden.py --layered-callaway-amm         (the same with AMM interfaces, 15^3 tables)
+    python tests/golden/make_golden.py --layered-nonlinear            (layered nonlinear Callaway, whole runs)
 
 The reference ships no tests or golden vectors, so these vectors are produced by
 calling its own functions on the synthetic Si-like table and the two film
@@ -46,7 +47,7 @@ WORKLOADS = {"c1": "c1_cross_plane.toml", "c2": "c2_inplane_film.toml", "c1s": "
              "c2r": "c2r_conservative.toml", "c1la": "c1la_amm.toml", "c1lm": "c1lm_mmm.toml",
              "c1sw": "c1sw_weighted.toml", "c1sg": "c1sg_gaussian.toml", "taper": "taper_fin.toml",
              "c1lr": "c1lr_rough.toml", "c1ls": "c1ls_source.toml", "c1lc": "c1lc_conservative.toml",
-             "c1lw": "c1lw_callaway.toml"}
+             "c1lw": "c1lw_callaway.toml", "c1ln": "c1ln_nonlinear.toml"}
 
 
 # acoustic-mismatch interfaces on a layered Si/Ge input
@@ -659,6 +660,11 @@ def layered_strict_vectors(problem, rd, rng, facets, n=1500):
 
 
 def main():
+    if "--layered-nonlinear" in sys.argv:
+        np.savez_compressed(os.path.join(OUT, "run_c1ln.npz"),
+                            **whole_run("c1ln", 20000, 400, list(range(1000, 1016)), 10))
+        print("layered nonlinear Callaway runs written to", OUT)
+        return
     if "--layered-callaway-amm" in sys.argv:
         np.savez_compressed(os.path.join(OUT, "run_c1lw_amm.npz"),
                             **whole_run("c1lw", 20000, 400, list(range(1000, 1016)), 10, **LAYERED_AMM))

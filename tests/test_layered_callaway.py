@@ -15,8 +15,11 @@ interface rule (src/solver/InterfaceInteraction.cpp:100-105, :124-133).
   --layered-callaway) in cell temperatures, kappa and interface totals.
 - Acoustic-mismatch interfaces under linearized Callaway (15^3 tables): 8 oracle
   runs against 16 reference runs (run_c1lw_amm.npz, --layered-callaway-amm).
-- Rejections: the nonlinear formulation; AMM on the completed 16^3 tables, which
-  the reference refuses with the same message.
+- The nonlinear formulation (c1ln_nonlinear.toml): each material's rate rows and
+  wavevectors over the global bank, and 8 oracle runs against 16 reference runs
+  (run_c1ln.npz, --layered-nonlinear).
+- Rejection: AMM on the completed 16^3 tables, which the reference refuses with
+  the same message.
 """
 import concurrent.futures
 import os
@@ -99,11 +102,45 @@ AMM = {"materials.interface_model": '"amm"', "materials.mass_densities_kg_m3": "
        "materials.folders": '["synthetic:si15:callaway", "synthetic:ge15:callaway"]'}
 
 
+def test_nonlinear_rates_are_the_materials_rates():
+    """Layered nonlinear Callaway (c1ln_nonlinear.toml): the rate rows and
+    wavevectors over the global bank are each material's own."""
+    lay = rt.Problem(rt.config_text("c1ln_nonlinear.toml"), input_directory=rt.CONFIGS)
+    nl = lay.callaway_nonlinear()
+    lv = lay.layers()
+    na = lay.views()[1].n_active
+    mb = rt.arr(lv.mode_begin, lv.count + 1, np.int32)
+    nt = nl.n_temps
+    for k, folder in enumerate(("synthetic:si16:callaway", "synthetic:ge16:callaway")):
+        one = rt.Problem(rt.config_text("c1cn_nonlinear.toml", **{"io.material_folder": f'"{folder}"'}),
+                         input_directory=rt.CONFIGS)
+        n1 = one.callaway_nonlinear()
+        m1 = one.views()[1].n_active
+        assert n1.n_temps == nt and m1 == mb[k + 1] - mb[k]
+        for name in ("gamma_normal", "gamma_umklapp", "gamma_extra"):
+            whole = rt.arr(getattr(nl, name), nt * na).reshape(nt, na)[:, mb[k]:mb[k + 1]]
+            assert np.array_equal(whole, rt.arr(getattr(n1, name), nt * m1).reshape(nt, m1)), (k, name)
+        assert np.array_equal(rt.arr(nl.wavevector, 3 * na)[3 * mb[k]:3 * mb[k + 1]], rt.arr(n1.wavevector, 3 * m1))
+
+
+def test_nonlinear_oracle_runs_match_reference():
+    golden = np.load(os.path.join(HERE, "golden", "run_c1ln.npz"))
+    n, steps, interval = int(golden["n"]), int(golden["steps"]), int(golden["interval"])
+    seeds = [8500 + k for k in range(8)]
+    with concurrent.futures.ThreadPoolExecutor(8) as pool:
+        runs = list(pool.map(lambda s: oracle_run("c1ln_nonlinear.toml", n, steps, interval, s, counters=True), seeds))
+    temps = np.array([r[0] for r in runs])
+    iface = np.array([r[1] for r in runs], dtype=float)
+    z = z_score(temps, golden["temperature"])
+    zi = z_score(iface, golden["interface"].astype(float))
+    print(f"c1ln oracle: cell T max|z| {np.abs(z).max():.2f}, mean z^2 {np.mean(z ** 2):.2f}; interface events "
+          f"z {zi[0]:+.2f}, transmitted z {zi[1]:+.2f}")
+    assert np.abs(z).max() <= 4.5
+    assert np.mean(z ** 2) <= 2.5
+    assert np.all(np.abs(zi) <= 3.0)
+
+
 def test_rejections():
-    with pytest.raises(RuntimeError, match="nonlinear Callaway formulation"):
-        rt.Problem(rt.config_text("c1lw_callaway.toml").replace('model = "callaway"',
-                                                                'model = "callaway"\ncallaway_formulation = "nonlinear"'),
-                   input_directory=rt.CONFIGS)
     # AMM on the first-BZ-completed 16^3 tables: the replicas break the mirror symmetry
     # AMM needs; the reference refuses this input with the same message
     # (AcousticInterfaceSampler::configure), checked against oracle/_ref when it was built.
